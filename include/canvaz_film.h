@@ -1,0 +1,312 @@
+/*
+ * canvaz_film.h - C ABI of the MI355X (gfx950) thin-film paint step.
+ *
+ * This is the drop-in boundary that replaces the wgpu dispatch of the
+ * reference's `Film` (src/film/mod.rs:445-1462 over src/gpu/mod.rs) on the
+ * headless `--auto` path.  Every entry point names the reference method it
+ * stands for.  Rules that hold for the whole ABI:
+ *   - status returns are int: 0 = ok, negative = error (CZ_E*); the message of
+ *     the last error on the calling thread is cz_last_error().
+ *   - no exceptions, panics or callbacks cross the ABI.
+ *   - a cz_film owns all of its device memory; host arrays are borrowed for the
+ *     duration of the call only.
+ *   - one cz_film = one HIP stream; calls on one handle are serialised by the
+ *     caller, distinct handles may be driven from different threads.
+ *   - state crossing the ABI is row-major, idx = j*W + i
+ *     (shaders/common.wgsl:140-142), in the reference's per-cell layout.
+ */
+#ifndef CANVAZ_FILM_H
+#define CANVAZ_FILM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CZ_NPIG 12          /* shaders/common.wgsl:47 */
+#define CZ_NBAND 16         /* shaders/common.wgsl:48 */
+#define CZ_PIG_STRIDE 40    /* shaders/common.wgsl:49 */
+#define CZ_STRATA 4         /* shaders/common.wgsl:46 */
+#define CZ_TILE64 64        /* shaders/common.wgsl:50 */
+#define CZ_MAX_DEPOSITS 65536               /* src/film/mod.rs:584 */
+#define CZ_MAX_TILE_ITEMS (65536 * 16)      /* src/film/mod.rs:661,1208 */
+
+#define CZ_OK 0
+#define CZ_EINVAL (-1)      /* bad argument */
+#define CZ_ESIZE (-2)       /* size mismatch (write_state returns false) */
+#define CZ_EHIP (-3)        /* HIP runtime error */
+#define CZ_ENOMEM (-4)      /* device allocation failed / over budget */
+#define CZ_EUNSUP (-5)      /* feature outside the built path */
+
+/* Uniform parameter block, byte-identical to `Params`
+ * (src/film/mod.rs:5-23, shaders/common.wgsl:1-17): 176 bytes. */
+typedef struct cz_params {
+    uint32_t size[2];
+    uint32_t tiles[2];
+    float dx;
+    float dt;
+    float inv_dx;
+    float frame_dt;
+    float g[4];
+    float phys[4];
+    float ein[4];
+    float cure[4];
+    float misc[4];
+    uint32_t flags[4];
+    float front[4];
+    float front2[4];
+    float front3[4];
+} cz_params;
+
+/* One deposit segment, byte-identical to `Deposit`
+ * (src/film/mod.rs:25-39, shaders/common.wgsl:19-34): 96 bytes. */
+typedef struct cz_deposit {
+    float p0[2];
+    float p1[2];
+    float n0[2];
+    float n1[2];
+    float r0;
+    float r1;
+    float h0;
+    float vol;
+    uint32_t kind;
+    uint32_t pad[3];
+    uint32_t pig[8];
+} cz_deposit;
+
+#define CZ_DEPOSIT_RECYCLED 1u   /* src/film/mod.rs:41 */
+#define CZ_DEPOSIT_CAP_START 2u  /* src/film/mod.rs:42 */
+#define CZ_DEPOSIT_CAP_END 4u    /* src/film/mod.rs:43 */
+#define CZ_DEPOSIT_METHOD_SHIFT 8u
+enum { CZ_METHOD_THREAD = 1, CZ_METHOD_DROP = 2, CZ_METHOD_POUR = 3,
+       CZ_METHOD_BRUSH = 4, CZ_METHOD_KNIFE = 5, CZ_METHOD_ROLLER = 6 };
+
+/* `FilmSettings` (src/film/mod.rs:281-313); bools as uint32 0/1.
+ * cz_settings_default() fills the reference defaults (mod.rs:315-350). */
+typedef struct cz_settings {
+    float width_m;
+    float height_m;
+    float rho;
+    float sigma;
+    float n_index;
+    float m_cap;
+    float contact_angle_deg;
+    float hysteresis_deg;
+    float h_star;
+    uint32_t raw_canvas;
+    float weave_um;
+    float cure_rate;
+    float tau_cure;
+    float lnmu_cure;
+    float v_cap;
+    float tilt_deg;
+    float tilt_dir_deg;
+    float gravity;
+    float medium_mu;
+    uint32_t debug_view;
+    float h_min;
+    float m_cap_hydro;
+    uint32_t max_substeps;
+    uint32_t min_substeps;
+    uint32_t implicit_iters;
+    uint32_t implicit_hydro;
+    uint32_t slump;
+    uint32_t capillary;
+    uint32_t capillary_implicit;
+} cz_settings;
+
+/* The environment toggles read by `update_params` every frame
+ * (src/film/mod.rs:870-915).  cz_overrides_from_env() reads PAINT_NO_IMPLICIT,
+ * PAINT_NO_CAP, PAINT_CAPCOND, PAINT_FCAP, PAINT_NO_HOLD, PAINT_KSHAPE,
+ * PAINT_NO_TRANSL, PAINT_NO_SHAPE, PAINT_KLINE, PAINT_HYST. */
+typedef struct cz_overrides {
+    uint32_t no_implicit;
+    uint32_t no_cap;
+    float capcond;      /* default 80 */
+    float fcap;         /* default 2.5 */
+    uint32_t no_hold;
+    float kshape;       /* default 1 */
+    uint32_t no_transl;
+    uint32_t no_shape;
+    float kline;        /* default 0 */
+    uint32_t has_hyst;
+    float hyst_deg;
+} cz_overrides;
+
+/* Frame statistics: the `poll_stats` readback (src/film/mod.rs:935-973) plus
+ * the per-frame solver counts.  h_max / mu_min are the CFL inputs of the next
+ * frame (deterministic lag of one frame). */
+typedef struct cz_stats {
+    float h_max;
+    float mu_min;
+    uint32_t active_tiles;
+    uint32_t substeps_last;
+    uint32_t implicit_steps;
+    uint32_t frame;
+    float cg_report[4];      /* cg_s[1]: (rz0, drift, total, rz_final) */
+    double total_deposited;  /* src/film/mod.rs:1188 */
+    double weave_volume;     /* src/film/mod.rs:689 */
+} cz_stats;
+
+/* The mass report of `compute_stats` (src/main.rs:1679-1819), sums in FP64. */
+typedef struct cz_mass_report {
+    double wet_ml;
+    double cured_ml;
+    double deposited_ml;
+    double err_pct;
+    float hmax;
+    uint32_t pad;
+    uint64_t wet_cells;
+    uint64_t nan_cells;
+} cz_mass_report;
+
+/* Indices of the eight state parts of read_state/write_state, in the order of
+ * `state_buffers` (src/film/mod.rs:1277-1279). */
+enum {
+    CZ_PART_H = 0,          /* f32 [n]            */
+    CZ_PART_M = 1,          /* f32 [n]            */
+    CZ_PART_RHEO = 2,       /* f32 [n][2] (mu, tau_y) */
+    CZ_PART_STRATA_H = 3,   /* f32 [n][4]         */
+    CZ_PART_STRATA_PIG = 4, /* u32 [n][4][8] (16 f16 per stratum) */
+    CZ_PART_STRATA_CURE = 5,/* f32 [n][4]         */
+    CZ_PART_SUB_REFL = 6,   /* u32 [n][8] (16 f16 bands) */
+    CZ_PART_LAY = 7,        /* u32 [n]            */
+    CZ_NPARTS = 8
+};
+
+/* Solver buffers that persist across frames and that the step reads at
+ * cells outside the active tiles; exposed for exact checkpointing and tests. */
+enum {
+    CZ_BUF_DELTA = 16,      /* f32 bits [n]: nose offset, shaders/film.wgsl:570-580 */
+    CZ_BUF_FNORM = 17,      /* f32 [n][2] */
+    CZ_BUF_PRES = 18,       /* f32 [n][2] */
+    CZ_BUF_H_FRAME = 19,    /* f32 [n] */
+    CZ_BUF_CG_M = 20,       /* f32 [n][4] */
+    CZ_BUF_CG_P = 21,       /* f32 [n] */
+    CZ_BUF_TILE_MOVED = 22, /* u32 [tiles16] */
+    CZ_BUF_TILE_FLAGS = 23, /* u32 [tiles16] */
+    CZ_BUF_TMP = 24,        /* f32 [n] */
+    CZ_BUF_FLUX = 25,       /* f32 [n][4] */
+    CZ_BUF_ACC = 26,        /* f32 [n][4] */
+    CZ_BUF_ALPHA = 27,      /* f32 [n] */
+    CZ_BUF_STATS = 28,      /* f32 [tiles64][2] */
+    CZ_BUF_POOL = 29,       /* f32 [2][pw*ph] */
+    CZ_BUF_TILE_LIST = 30,  /* u32 [tiles16] (ascending) */
+    CZ_BUF_ACC_S = 31,      /* f32 [n][4][4] */
+    CZ_BUF_ALPHA_S = 32     /* f32 [n][4] */
+};
+
+typedef struct cz_film cz_film;
+
+/* ---- host-side helpers of the reference (no device needed) ------------- */
+
+/* FilmSettings::default (src/film/mod.rs:315-350). */
+void cz_settings_default(cz_settings* s);
+/* Defaults of the environment toggles, then the values found in the env. */
+void cz_overrides_default(cz_overrides* o);
+void cz_overrides_from_env(cz_overrides* o);
+/* pigment_table (src/color/mod.rs:243-258): refl[12][16], scatter/tint/
+ * viscosity/yield[12] -> out[12*40]. */
+void cz_pigment_table(const float* refl, const float* scatter, const float* tint,
+                      const float* viscosity, const float* yield_stress, float* out);
+/* default_pigments (src/color/mod.rs:190-224) as the five arrays above. */
+void cz_default_pigments(float* refl, float* scatter, float* tint, float* viscosity,
+                         float* yield_stress);
+/* cmf_table (src/color/mod.rs:150-171) -> out[16*3]. */
+void cz_cmf_table(float* out);
+/* weave (src/film/mod.rs:212-233) -> out[w*h]. */
+void cz_weave(uint32_t w, uint32_t h, float dx, float amp, float* out);
+/* substrate_refl (src/film/mod.rs:188-210) without a canvas colour -> out[n*8]. */
+void cz_substrate_refl(uint64_t n, uint32_t raw, uint32_t* out);
+/* Deposit::ribbon / capsule / stroke / laid_by (src/film/mod.rs:62-98). */
+void cz_deposit_ribbon(const float p0[2], const float p1[2], const float n0[2], const float n1[2],
+                       float r0, float r1, uint32_t caps, float volume, const uint32_t pig[8],
+                       float dx, cz_deposit* out);
+void cz_deposit_capsule(const float p0[2], const float p1[2], float r, float volume,
+                        const uint32_t pig[8], float dx, cz_deposit* out);
+void cz_deposit_stroke(const float p0[2], const float p1[2], float r, float volume,
+                       const uint32_t pig[8], float dx, cz_deposit* out);
+void cz_deposit_laid_by(cz_deposit* d, uint32_t method);
+/* Deposit::sampled_volume / height_at / inside_fraction (mod.rs:113-150). */
+float cz_deposit_sampled_volume(const cz_deposit* d, float lattice);
+float cz_deposit_height_at(const cz_deposit* d, float x, float y);
+float cz_deposit_inside_fraction(const cz_deposit* d, float w, float h);
+/* PaintMix::normalized().packed() (src/color/mod.rs:348-371) of frac[16]. */
+void cz_paint_pack(const float* frac16, int normalize, uint32_t out[8]);
+
+/* ---- the film (Film, src/film/mod.rs:445-1462) ------------------------- */
+
+/* Film::new (mod.rs:579-859) on the current HIP device.  pigtab: 12*40 f32,
+ * cmf: 16*3 f32 (NULL -> reference defaults).  Initial state: h = 0,
+ * m = weave, rheo = (1,0), sub_refl = raw canvas, params from
+ * update_params(1/60, 1). */
+int cz_film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigtab,
+                   const float* cmf, cz_film** out);
+void cz_film_destroy(cz_film* f);
+int cz_film_dims(const cz_film* f, uint32_t* w, uint32_t* h, float* dx);
+/* Replace the settings used by the next update_params (FilmSettings is pub). */
+int cz_film_set_settings(cz_film* f, const cz_settings* s);
+int cz_film_get_settings(const cz_film* f, cz_settings* s);
+/* Environment toggles used by update_params (default: read from the env at
+ * create time and at every cz_film_step). */
+int cz_film_set_overrides(cz_film* f, const cz_overrides* o);
+/* gravity_vec (mod.rs:861-867). */
+int cz_film_gravity_vec(const cz_film* f, float out[3]);
+/* update_params (mod.rs:869-917). */
+int cz_film_update_params(cz_film* f, float frame_dt, uint32_t substeps);
+/* choose_substeps (mod.rs:919-933). */
+uint32_t cz_film_choose_substeps(const cz_film* f, float frame_dt);
+int cz_film_get_params(const cz_film* f, cz_params* p);
+int cz_film_set_params(cz_film* f, const cz_params* p);
+/* upload_deposits (mod.rs:1175-1224): culls, accounts total_deposited, bins
+ * into 64-tiles; *sent = deposits taken (the rest are held by the caller). */
+int cz_film_upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* sent);
+/* encode_frame (mod.rs:1322-1462) enqueued on the film's stream with the
+ * current params; no host sync.  The deposits of the last upload are
+ * consumed by this frame. */
+int cz_film_encode_frame(cz_film* f, uint32_t substeps);
+/* encode_sim (src/main.rs:1875-1909): poll the previous frame's stats (lag 1,
+ * deterministic), upload deposits, choose_substeps, update_params,
+ * encode_frame, request stats.  *sent as in upload_deposits (may be NULL). */
+int cz_film_step(cz_film* f, const cz_deposit* deps, uint32_t n, float frame_dt, uint32_t* sent);
+/* Wait for the film's stream and fold the last frame's stats (poll_stats). */
+int cz_film_sync(cz_film* f);
+int cz_film_stats(cz_film* f, cz_stats* st);
+/* pool8 output (mod.rs:956-963): pw*ph surface max then pw*ph relief max. */
+int cz_film_pool(cz_film* f, float* out, uint64_t n_floats);
+/* compute_stats mass report (src/main.rs:1679-1819). */
+int cz_film_mass_report(cz_film* f, cz_mass_report* r);
+/* state_buffers sizes, read_state, write_state (mod.rs:1277-1320). */
+int cz_film_state_sizes(const cz_film* f, uint64_t sizes[CZ_NPARTS]);
+int cz_film_read_state(cz_film* f, void* const parts[CZ_NPARTS]);
+int cz_film_write_state(cz_film* f, const void* const parts[CZ_NPARTS],
+                        const uint64_t sizes[CZ_NPARTS]);
+/* Any state part or CZ_BUF_* solver buffer, host copies (tests/checkpoints). */
+int cz_film_buffer_size(const cz_film* f, int which, uint64_t* bytes);
+int cz_film_read_buffer(cz_film* f, int which, void* host, uint64_t bytes);
+int cz_film_write_buffer(cz_film* f, int which, const void* host, uint64_t bytes);
+/* Device pointer of a buffer (zero-copy interop, e.g. torch tensors). */
+int cz_film_device_ptr(cz_film* f, int which, void** ptr, uint64_t* bytes);
+/* dry_now (mod.rs:1230-1244) and clear (mod.rs:1009-1028). */
+int cz_film_dry_now(cz_film* f);
+int cz_film_clear(cz_film* f);
+/* The HIP stream the film's work is enqueued on (hipStream_t). */
+void* cz_film_stream(cz_film* f);
+/* Capture every frame of cz_film_encode_frame into a hipGraph keyed by the
+ * (substeps, implicit_steps, has_deposits) shape and replay it. 1 = on. */
+int cz_film_use_graphs(cz_film* f, int on);
+/* Average per-kernel device time over the last frames (HIP events on the
+ * film's stream), in microseconds; label = kernel family name. */
+int cz_film_profile(cz_film* f, int on);
+int cz_film_profile_read(cz_film* f, const char* label, double* avg_us, uint64_t* launches);
+
+const char* cz_last_error(void);
+const char* cz_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CANVAZ_FILM_H */

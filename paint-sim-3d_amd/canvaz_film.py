@@ -1,0 +1,348 @@
+"""Host-side mirror of the reference's `Film` (src/film/mod.rs:445-1462) over
+the C ABI of include/canvaz_film.h (libcanvaz_film.so, gfx950).
+
+Method names follow the reference: Film(w, h, settings) = Film::new,
+update_params, choose_substeps, gravity_vec, upload_deposits, encode_frame,
+step (= App::encode_sim, src/main.rs:1875-1909), read_state, write_state,
+dry_now, clear, plus the mass report of compute_stats (src/main.rs:1679).
+
+There is no CPU fallback: if the HIP library cannot be loaded, every entry
+raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from cz_types import (Deposit, MassReport, Overrides, Params, Settings, Stats, PART_NAMES,
+                      default_overrides, default_settings, deposit_array)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcanvaz_film.so")
+
+_lib = None
+
+
+class FilmError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libcanvaz_film.so (raises when it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FilmError(f"{LIB_PATH} is missing: build it with paint-sim-3d_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    P, f32, u32, u64, i32 = C.c_void_p, C.c_float, C.c_uint32, C.c_uint64, C.c_int
+    fp, up = C.POINTER(f32), C.POINTER(u32)
+    sig = {
+        "cz_last_error": (C.c_char_p, []),
+        "cz_version": (C.c_char_p, []),
+        "cz_settings_default": (None, [C.POINTER(Settings)]),
+        "cz_overrides_default": (None, [C.POINTER(Overrides)]),
+        "cz_overrides_from_env": (None, [C.POINTER(Overrides)]),
+        "cz_pigment_table": (None, [fp, fp, fp, fp, fp, fp]),
+        "cz_default_pigments": (None, [fp, fp, fp, fp, fp]),
+        "cz_cmf_table": (None, [fp]),
+        "cz_weave": (None, [u32, u32, f32, f32, fp]),
+        "cz_substrate_refl": (None, [u64, u32, up]),
+        "cz_deposit_ribbon": (None, [fp, fp, fp, fp, f32, f32, u32, f32, up, f32, C.POINTER(Deposit)]),
+        "cz_deposit_capsule": (None, [fp, fp, f32, f32, up, f32, C.POINTER(Deposit)]),
+        "cz_deposit_stroke": (None, [fp, fp, f32, f32, up, f32, C.POINTER(Deposit)]),
+        "cz_deposit_laid_by": (None, [C.POINTER(Deposit), u32]),
+        "cz_deposit_sampled_volume": (f32, [C.POINTER(Deposit), f32]),
+        "cz_deposit_height_at": (f32, [C.POINTER(Deposit), f32, f32]),
+        "cz_deposit_inside_fraction": (f32, [C.POINTER(Deposit), f32, f32]),
+        "cz_paint_pack": (None, [fp, i32, up]),
+        "cz_film_create": (i32, [u32, u32, C.POINTER(Settings), fp, fp, C.POINTER(P)]),
+        "cz_film_destroy": (None, [P]),
+        "cz_film_dims": (i32, [P, up, up, fp]),
+        "cz_film_set_settings": (i32, [P, C.POINTER(Settings)]),
+        "cz_film_get_settings": (i32, [P, C.POINTER(Settings)]),
+        "cz_film_set_overrides": (i32, [P, C.POINTER(Overrides)]),
+        "cz_film_gravity_vec": (i32, [P, fp]),
+        "cz_film_update_params": (i32, [P, f32, u32]),
+        "cz_film_choose_substeps": (u32, [P, f32]),
+        "cz_film_get_params": (i32, [P, C.POINTER(Params)]),
+        "cz_film_set_params": (i32, [P, C.POINTER(Params)]),
+        "cz_film_upload_deposits": (i32, [P, C.POINTER(Deposit), u32, up]),
+        "cz_film_encode_frame": (i32, [P, u32]),
+        "cz_film_step": (i32, [P, C.POINTER(Deposit), u32, f32, up]),
+        "cz_film_sync": (i32, [P]),
+        "cz_film_stats": (i32, [P, C.POINTER(Stats)]),
+        "cz_film_pool": (i32, [P, fp, u64]),
+        "cz_film_mass_report": (i32, [P, C.POINTER(MassReport)]),
+        "cz_film_state_sizes": (i32, [P, C.POINTER(u64)]),
+        "cz_film_read_state": (i32, [P, C.POINTER(C.c_void_p)]),
+        "cz_film_write_state": (i32, [P, C.POINTER(C.c_void_p), C.POINTER(u64)]),
+        "cz_film_buffer_size": (i32, [P, i32, C.POINTER(u64)]),
+        "cz_film_read_buffer": (i32, [P, i32, C.c_void_p, u64]),
+        "cz_film_write_buffer": (i32, [P, i32, C.c_void_p, u64]),
+        "cz_film_device_ptr": (i32, [P, i32, C.POINTER(C.c_void_p), C.POINTER(u64)]),
+        "cz_film_dry_now": (i32, [P]),
+        "cz_film_clear": (i32, [P]),
+        "cz_film_stream": (C.c_void_p, [P]),
+        "cz_film_use_graphs": (i32, [P, i32]),
+        "cz_film_profile": (i32, [P, i32]),
+        "cz_film_profile_read": (i32, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise FilmError(f"canvaz_film error {rc}: {lib().cz_last_error().decode()}")
+    return rc
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _up(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+# ---- tables and deposit geometry (host functions of the reference) --------
+
+def default_pigment_table():
+    """pigment_table(default_pigments()) (src/color/mod.rs:190-258) -> f32[480]."""
+    L = lib()
+    refl = np.zeros(12 * 16, np.float32)
+    sc, ti, vi, yi = (np.zeros(12, np.float32) for _ in range(4))
+    L.cz_default_pigments(_fp(refl), _fp(sc), _fp(ti), _fp(vi), _fp(yi))
+    out = np.zeros(480, np.float32)
+    L.cz_pigment_table(_fp(refl), _fp(sc), _fp(ti), _fp(vi), _fp(yi), _fp(out))
+    return out
+
+
+def pigment_table(refl, scatter, tint, viscosity, yield_stress):
+    out = np.zeros(480, np.float32)
+    a = [np.ascontiguousarray(x, np.float32) for x in (refl, scatter, tint, viscosity, yield_stress)]
+    lib().cz_pigment_table(*[_fp(x) for x in a], _fp(out))
+    return out
+
+
+def cmf_table():
+    out = np.zeros(48, np.float32)
+    lib().cz_cmf_table(_fp(out))
+    return out
+
+
+def paint_pack(frac16, normalize=True):
+    f = np.ascontiguousarray(frac16, np.float32)
+    out = np.zeros(8, np.uint32)
+    lib().cz_paint_pack(_fp(f), 1 if normalize else 0, _up(out))
+    return out
+
+
+def _v2(p):
+    return (C.c_float * 2)(float(p[0]), float(p[1]))
+
+
+def _pig8(pig):
+    return (C.c_uint32 * 8)(*[int(v) for v in pig])
+
+
+def deposit_capsule(p0, p1, r, volume, pig, dx):
+    d = Deposit()
+    lib().cz_deposit_capsule(_v2(p0), _v2(p1), r, volume, _pig8(pig), dx, C.byref(d))
+    return d
+
+
+def deposit_stroke(p0, p1, r, volume, pig, dx):
+    d = Deposit()
+    lib().cz_deposit_stroke(_v2(p0), _v2(p1), r, volume, _pig8(pig), dx, C.byref(d))
+    return d
+
+
+def deposit_ribbon(p0, p1, n0, n1, r0, r1, caps, volume, pig, dx):
+    d = Deposit()
+    lib().cz_deposit_ribbon(_v2(p0), _v2(p1), _v2(n0), _v2(n1), r0, r1, caps, volume, _pig8(pig), dx, C.byref(d))
+    return d
+
+
+def laid_by(d, method):
+    lib().cz_deposit_laid_by(C.byref(d), method)
+    return d
+
+
+# ---- the film --------------------------------------------------------------
+
+BUFFER_IDS = {
+    "h": 0, "m": 1, "rheo": 2, "strata_h": 3, "strata_pig": 4, "strata_cure": 5, "sub_refl": 6, "lay": 7,
+    "delta": 16, "fnorm": 17, "pres": 18, "h_frame": 19, "cg_m": 20, "cg_p": 21, "tile_moved": 22,
+    "tile_flags": 23, "tmp": 24, "flux": 25, "acc": 26, "alpha": 27, "stats": 28, "pool": 29, "tile_list": 30,
+    "acc_s": 31, "alpha_s": 32,
+}
+_U32 = {"strata_pig", "sub_refl", "lay", "delta", "tile_moved", "tile_flags", "tile_list"}
+_CELL_SHAPE = {"h": (), "m": (), "rheo": (2,), "strata_h": (4,), "strata_pig": (4, 8), "strata_cure": (4,),
+               "sub_refl": (8,), "lay": (), "delta": (), "fnorm": (2,), "pres": (2,), "h_frame": (), "cg_m": (4,),
+               "cg_p": (), "tmp": (), "flux": (4,), "acc": (4,), "alpha": (), "acc_s": (4, 4), "alpha_s": (4,)}
+
+
+class Film:
+    """One canvas on the current HIP device (Film::new, src/film/mod.rs:579)."""
+
+    def __init__(self, w, h, settings=None, pigtab=None, cmf=None):
+        L = lib()
+        s = settings if settings is not None else default_settings()
+        pt = None if pigtab is None else np.ascontiguousarray(pigtab, np.float32)
+        cm = None if cmf is None else np.ascontiguousarray(cmf, np.float32)
+        ptr = C.c_void_p()
+        check(L.cz_film_create(w, h, C.byref(s), _fp(pt) if pt is not None else None,
+                               _fp(cm) if cm is not None else None, C.byref(ptr)))
+        self._p = ptr
+        self.w, self.h = w, h
+        self.dx = self.settings.width_m / w
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().cz_film_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def settings(self):
+        s = Settings()
+        check(lib().cz_film_get_settings(self._p, C.byref(s)))
+        return s
+
+    @settings.setter
+    def settings(self, s):
+        check(lib().cz_film_set_settings(self._p, C.byref(s)))
+
+    def set_overrides(self, o):
+        check(lib().cz_film_set_overrides(self._p, C.byref(o)))
+
+    @property
+    def params(self):
+        p = Params()
+        check(lib().cz_film_get_params(self._p, C.byref(p)))
+        return p
+
+    def set_params(self, p):
+        check(lib().cz_film_set_params(self._p, C.byref(p)))
+
+    def gravity_vec(self):
+        g = (C.c_float * 3)()
+        check(lib().cz_film_gravity_vec(self._p, g))
+        return [g[0], g[1], g[2]]
+
+    def update_params(self, frame_dt, substeps):
+        check(lib().cz_film_update_params(self._p, frame_dt, substeps))
+
+    def choose_substeps(self, frame_dt):
+        return lib().cz_film_choose_substeps(self._p, frame_dt)
+
+    def upload_deposits(self, deps):
+        arr, n = deposit_array(deps)
+        sent = C.c_uint32()
+        check(lib().cz_film_upload_deposits(self._p, arr, n, C.byref(sent)))
+        return sent.value
+
+    def encode_frame(self, substeps):
+        check(lib().cz_film_encode_frame(self._p, substeps))
+
+    def step(self, deps=(), frame_dt=1.0 / 60.0):
+        arr, n = deposit_array(deps)
+        sent = C.c_uint32()
+        check(lib().cz_film_step(self._p, arr, n, frame_dt, C.byref(sent)))
+        return sent.value
+
+    def sync(self):
+        check(lib().cz_film_sync(self._p))
+
+    def stats(self):
+        st = Stats()
+        check(lib().cz_film_stats(self._p, C.byref(st)))
+        return st
+
+    def mass_report(self):
+        r = MassReport()
+        check(lib().cz_film_mass_report(self._p, C.byref(r)))
+        return r
+
+    def buffer(self, name):
+        """Host copy of a state part or solver buffer, shaped [H, W, ...] for per-cell ones."""
+        which = BUFFER_IDS[name]
+        nb = C.c_uint64()
+        check(lib().cz_film_buffer_size(self._p, which, C.byref(nb)))
+        dt = np.uint32 if name in _U32 else np.float32
+        a = np.empty(nb.value // 4, dt)
+        check(lib().cz_film_read_buffer(self._p, which, a.ctypes.data_as(C.c_void_p), nb.value))
+        if name in _CELL_SHAPE:
+            return a.reshape((self.h, self.w) + _CELL_SHAPE[name])
+        return a
+
+    def write_buffer(self, name, arr):
+        which = BUFFER_IDS[name]
+        a = np.ascontiguousarray(arr)
+        check(lib().cz_film_write_buffer(self._p, which, a.ctypes.data_as(C.c_void_p), a.nbytes))
+
+    def state_sizes(self):
+        sz = (C.c_uint64 * 8)()
+        check(lib().cz_film_state_sizes(self._p, sz))
+        return list(sz)
+
+    def read_state(self):
+        """read_state (src/film/mod.rs:1281): 8 byte strings in state_buffers order."""
+        sizes = self.state_sizes()
+        bufs = [np.empty(s, np.uint8) for s in sizes]
+        ptrs = (C.c_void_p * 8)(*[b.ctypes.data for b in bufs])
+        check(lib().cz_film_read_state(self._p, ptrs))
+        return [b.tobytes() for b in bufs]
+
+    def write_state(self, parts):
+        """write_state (src/film/mod.rs:1303); returns False on a size mismatch."""
+        if len(parts) != 8:
+            return False
+        arrs = [np.frombuffer(p, np.uint8) if isinstance(p, (bytes, bytearray)) else
+                np.ascontiguousarray(p).view(np.uint8).reshape(-1) for p in parts]
+        ptrs = (C.c_void_p * 8)(*[a.ctypes.data for a in arrs])
+        sizes = (C.c_uint64 * 8)(*[a.nbytes for a in arrs])
+        rc = lib().cz_film_write_state(self._p, ptrs, sizes)
+        if rc == -2:
+            return False
+        check(rc)
+        return True
+
+    def write_state_arrays(self, state):
+        """write_state from a dict of per-part numpy arrays (names of PART_NAMES)."""
+        return self.write_state([np.ascontiguousarray(state[k]) for k in PART_NAMES])
+
+    def dry_now(self):
+        check(lib().cz_film_dry_now(self._p))
+
+    def clear(self):
+        check(lib().cz_film_clear(self._p))
+
+    def device_ptr(self, name):
+        p = C.c_void_p()
+        nb = C.c_uint64()
+        check(lib().cz_film_device_ptr(self._p, BUFFER_IDS[name], C.byref(p), C.byref(nb)))
+        return p.value, nb.value
+
+    def stream(self):
+        return lib().cz_film_stream(self._p)
+
+    def profile(self, on=True):
+        check(lib().cz_film_profile(self._p, 1 if on else 0))
+
+    def profile_read(self, label):
+        avg = C.c_double()
+        cnt = C.c_uint64()
+        check(lib().cz_film_profile_read(self._p, label.encode(), C.byref(avg), C.byref(cnt)))
+        return avg.value, cnt.value

@@ -1,0 +1,744 @@
+// film_dev.h - device view of one canvas and the per-cell device functions
+// of the thin-film step (gfx950).  Formulas follow shaders/film.wgsl,
+// strata.wgsl, deposit.wgsl, cure.wgsl and km.wgsl of the reference, cited
+// per function; the evaluation order of every float expression is the WGSL
+// source order (the library is built with -ffp-contract=off).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/canvaz_film.h"
+#include "czmath.h"
+
+namespace cz {
+
+constexpr float DRAINED_FILM = 1e-9f;
+constexpr uint32_t WAKE_BAND = 8u;
+constexpr int PLUG_RADIUS = 3;
+constexpr float YIELD_MENISCUS = 1e-3f;
+constexpr uint32_t MOVED = 64u;
+constexpr float W_AXIS = 0.6666667f;
+constexpr float W_DIAG_FLUX = 0.23570226f;
+constexpr float W_DIAG_LAP = 0.16666667f;
+constexpr float MOVE_EPS = 1e-6f;
+constexpr float FRAME_EPS = 2e-6f;
+constexpr float COX_LOG = 132.0f;
+constexpr float OIL_SHRINK = 0.04f;
+constexpr int NPIG = 12;
+constexpr int NBAND = 16;
+constexpr int PIG_STRIDE = 40;
+constexpr int TILE64 = 64;
+
+// Everything a kernel needs about one canvas, passed by value as a kernel
+// argument.  Layout in HBM is the reference's row-major SoA/AoS per buffer.
+struct FilmDev {
+    cz_params P;
+    uint32_t W, H, n, tw, th, nt16, t64x, t64y, pw, ph;
+    // state
+    float* h;
+    float* m;
+    float2* rheo;
+    float4* strata_h;
+    uint4* strata_pig;   // [n][4 strata][2] uint4 = 8 u32 per stratum
+    float4* strata_cure;
+    uint32_t* sub_refl;  // [n][8]
+    uint32_t* lay;
+    // persistent solver buffers
+    float2* pres;
+    float4* flux;
+    float4* acc;
+    float* alpha;
+    uint32_t* delta;      // current nose offsets (read)
+    uint32_t* delta_out;  // written by nose
+    float* tmp;
+    float4* acc_s;        // [n][4]
+    float4* alpha_s;
+    float4* strata_h2;
+    uint4* strata_pig2;
+    float4* strata_cure2;
+    float* h_frame;
+    float2* fnorm;
+    float4* cg_m;
+    float *cg_d, *cg_x, *cg_r, *cg_p, *cg_q, *cg_t, *cg_t2;
+    double* cg_part;
+    double* cg_part2;
+    float* cg_s;          // [8]
+    uint32_t* tile_flags;
+    uint32_t* tile_list;
+    uint32_t* tile_moved;
+    uint32_t* active_count;
+    float2* stats_out;
+    float* pool_out;
+    float* tsum_a;        // strata sums, fill pass input
+    float* tsum_b;        // fill pass output
+    const float* pigtab;
+    // deposits
+    const cz_deposit* deposits;
+    const uint32_t* tile_start;
+    const uint32_t* tile_count;
+    const uint32_t* tile_items;
+    float* frame_out;     // [16] per-frame scalars read by the host
+};
+
+__device__ __forceinline__ uint32_t idx(const FilmDev& F, uint32_t i, uint32_t j) { return j * F.W + i; }
+
+// shaders/common.wgsl:152-166
+__device__ __forceinline__ uint32_t refl_i(const FilmDev& F, int i) {
+    int w = (int)F.W;
+    int r = i;
+    if (r < 0) r = -r - 1;
+    if (r >= w) r = 2 * w - r - 1;
+    return (uint32_t)cz_clamp_i(r, 0, w - 1);
+}
+__device__ __forceinline__ uint32_t refl_j(const FilmDev& F, int j) {
+    int hh = (int)F.H;
+    int r = j;
+    if (r < 0) r = -r - 1;
+    if (r >= hh) r = 2 * hh - r - 1;
+    return (uint32_t)cz_clamp_i(r, 0, hh - 1);
+}
+
+// common.wgsl:127-130
+__device__ __forceinline__ bool cell_active(const FilmDev& F, uint32_t i, uint32_t j) {
+    return (F.tile_flags[(i >> 4) + (j >> 4) * F.tw] & 32u) != 0u;
+}
+
+struct Pig16 { float v[16]; };
+
+__device__ __forceinline__ void unpack8(const uint4 a, const uint4 b, float* o) {
+    o[0] = cz_unpack_lo(a.x); o[1] = cz_unpack_hi(a.x); o[2] = cz_unpack_lo(a.y); o[3] = cz_unpack_hi(a.y);
+    o[4] = cz_unpack_lo(a.z); o[5] = cz_unpack_hi(a.z); o[6] = cz_unpack_lo(a.w); o[7] = cz_unpack_hi(a.w);
+    o[8] = cz_unpack_lo(b.x); o[9] = cz_unpack_hi(b.x); o[10] = cz_unpack_lo(b.y); o[11] = cz_unpack_hi(b.y);
+    o[12] = cz_unpack_lo(b.z); o[13] = cz_unpack_hi(b.z); o[14] = cz_unpack_lo(b.w); o[15] = cz_unpack_hi(b.w);
+}
+__device__ __forceinline__ void pack8(const float* o, uint4& a, uint4& b) {
+    a.x = cz_pack2(o[0], o[1]); a.y = cz_pack2(o[2], o[3]); a.z = cz_pack2(o[4], o[5]); a.w = cz_pack2(o[6], o[7]);
+    b.x = cz_pack2(o[8], o[9]); b.y = cz_pack2(o[10], o[11]); b.z = cz_pack2(o[12], o[13]); b.w = cz_pack2(o[14], o[15]);
+}
+
+// common.wgsl:190-195
+__device__ __forceinline__ float disjoining(const FilmDev& F, float hv) {
+    float hs = F.P.cure[3];
+    float a = F.P.ein[2];
+    float x = cz_max(1.0f - hv / hs, 0.0f);
+    return a * x * x;
+}
+
+// film.wgsl:570-589
+__device__ __forceinline__ uint32_t pack_off(int dx, int dy) {
+    uint32_t ux = (uint32_t)cz_clamp_i(dx + 128, 0, 255);
+    uint32_t uy = (uint32_t)cz_clamp_i(dy + 128, 0, 255);
+    return (ux << 8u) | uy | 0x10000u;
+}
+__device__ __forceinline__ void unpack_off(uint32_t bits, int& ox, int& oy) {
+    if ((bits & 0x10000u) == 0u) { ox = 64; oy = 64; return; }
+    ox = (int)((bits >> 8u) & 0xffu) - 128;
+    oy = (int)(bits & 0xffu) - 128;
+}
+__device__ __forceinline__ float nose_dist_b(uint32_t bits) {
+    int ox, oy;
+    unpack_off(bits, ox, oy);
+    return cz_len2((float)ox, (float)oy);
+}
+__device__ __forceinline__ float nose_dist(const FilmDev& F, uint32_t c) { return nose_dist_b(F.delta[c]); }
+
+// film.wgsl:154-156
+__device__ __forceinline__ float hold_level(const FilmDev& F, float d) {
+    return F.P.front[0] * (1.0f - cz_expf(-(cz_max(d, 1.0f) - 0.5f) * F.P.front[1]));
+}
+
+// film.wgsl:147-152
+__device__ __forceinline__ float usurf(const FilmDev& F, int i, int j) {
+    uint32_t c = idx(F, refl_i(F, i), refl_j(F, j));
+    float hc = F.h[c];
+    float hold = 0.0f;
+    if (hc > 0.0f) hold = hold_level(F, nose_dist(F, c));
+    return hc + F.m[c] - hold;
+}
+
+// film.wgsl:190-201
+__device__ inline float line_pressure(const FilmDev& F, uint32_t i, uint32_t j) {
+    if (i < 1u || j < 1u || i + 1u >= F.W || j + 1u >= F.H) return 0.0f;
+    float2 nl = F.fnorm[idx(F, i - 1u, j)];
+    float2 nr = F.fnorm[idx(F, i + 1u, j)];
+    float2 nd = F.fnorm[idx(F, i, j - 1u)];
+    float2 nu = F.fnorm[idx(F, i, j + 1u)];
+    if (nl.x * nl.x + nl.y * nl.y < 0.5f || nr.x * nr.x + nr.y * nr.y < 0.5f ||
+        nd.x * nd.x + nd.y * nd.y < 0.5f || nu.x * nu.x + nu.y * nu.y < 0.5f)
+        return 0.0f;
+    float curv = (nr.x - nl.x + nu.y - nd.y) * 0.5f * F.P.inv_dx;
+    return F.P.phys[1] * curv * F.P.front2[2];
+}
+
+// film.wgsl:44-49
+__device__ __forceinline__ float2 yield_cap(const FilmDev& F, float2 gc, float ty) {
+    float gcl = cz_len2(gc.x, gc.y);
+    float gc_max = F.P.phys[1] / (YIELD_MENISCUS * YIELD_MENISCUS);
+    if (ty <= 0.0f || gcl <= gc_max) return gc;
+    return make_float2(gc.x * (gc_max / gcl), gc.y * (gc_max / gcl));
+}
+
+// film.wgsl:94-107
+__device__ __forceinline__ float hb_flux(float n, float gmag, float hf, float mu, float ty, float& oy, float& oup) {
+    oy = 0.0f;
+    oup = 0.0f;
+    if (gmag < 1e-9f || hf <= 0.0f) return 0.0f;
+    float y = cz_max(hf - ty / gmag, 0.0f);
+    if (y <= 0.0f) return 0.0f;
+    float up;
+    if (cz_abs(n - 1.0f) < 1e-4f) {
+        up = 0.5f * gmag / mu * y * y;
+    } else {
+        up = n / (n + 1.0f) * cz_powf(gmag / mu, 1.0f / n) * cz_powf(y, (n + 1.0f) / n);
+    }
+    float q = up * (hf - n * y / (2.0f * n + 1.0f));
+    oy = y;
+    oup = up;
+    return q;
+}
+
+// film.wgsl:109-126
+__device__ __forceinline__ float band_flux(float n, float up, float y, float za, float zb) {
+    float ta = 0.0f, tb = 0.0f;
+    if (y > 0.0f) {
+        float ra = cz_max(y - za, 0.0f) / y;
+        float rb = cz_max(y - zb, 0.0f) / y;
+        if (cz_abs(n - 1.0f) < 1e-4f) {
+            ta = ra * ra * ra;
+            tb = rb * rb * rb;
+        } else {
+            float e = (2.0f * n + 1.0f) / n;
+            ta = cz_powf(ra, e);
+            tb = cz_powf(rb, e);
+        }
+    }
+    return up * ((zb - za) - n / (2.0f * n + 1.0f) * y * (ta - tb));
+}
+
+// film.wgsl:134-145
+__device__ inline uint32_t feeders(const FilmDev& F, uint32_t c) {
+    int i = (int)(c % F.W);
+    int j = (int)(c / F.W);
+    float hc = F.h[c];
+    float thr = 2.0f * hc + F.P.ein[3];
+    uint32_t n = 0;
+    if (i > 0 && F.h[c - 1u] > thr) n++;
+    if (i + 1 < (int)F.W && F.h[c + 1u] > thr) n++;
+    if (j > 0 && F.h[c - F.W] > thr) n++;
+    if (j + 1 < (int)F.H && F.h[c + F.W] > thr) n++;
+    return n;
+}
+
+// film.wgsl:160-180
+__device__ inline float2 front_normal(const FilmDev& F, uint32_t c) {
+    int i = (int)(c % F.W);
+    int j = (int)(c / F.W);
+    float dx = 0.0f, dy = 0.0f;
+    for (int dj = -2; dj <= 2; dj++) {
+        for (int di = -2; di <= 2; di++) {
+            if (di == 0 && dj == 0) continue;
+            int x = i + di, y = j + dj;
+            if (x < 0 || y < 0 || x >= (int)F.W || y >= (int)F.H) continue;
+            if (di * di + dj * dj > 5) continue;
+            if (F.h[idx(F, (uint32_t)x, (uint32_t)y)] <= F.P.ein[3]) {
+                float ox = (float)di, oy = (float)dj;
+                float dd = ox * ox + oy * oy;
+                dx = dx + ox / dd;
+                dy = dy + oy / dd;
+            }
+        }
+    }
+    float l = cz_len2(dx, dy);
+    if (l < 1e-6f) return make_float2(0.0f, 0.0f);
+    return make_float2(dx / l, dy / l);
+}
+
+// film.wgsl:182-188
+__device__ inline float wet_gain(float2 n) {
+    float s2 = 0.70710678f;
+    float w = W_AXIS * (cz_max(n.x, 0.0f) + cz_max(-n.x, 0.0f) + cz_max(n.y, 0.0f) + cz_max(-n.y, 0.0f));
+    w = w + W_DIAG_FLUX * (cz_max(s2 * (n.x + n.y), 0.0f) + cz_max(-s2 * (n.x + n.y), 0.0f) +
+                           cz_max(s2 * (n.x - n.y), 0.0f) + cz_max(-s2 * (n.x - n.y), 0.0f));
+    float m = cz_max(cz_max(cz_abs(n.x), cz_abs(n.y)), 1e-3f);
+    return 1.0f / (m * cz_max(w, 1e-3f));
+}
+
+// film.wgsl:203-222
+__device__ inline float front_shape(const FilmDev& F, uint32_t c, float2 n) {
+    int i = (int)(c % F.W);
+    int j = (int)(c / F.W);
+    float w = 0.0f, w0 = 0.0f;
+    for (int dj = -3; dj <= 3; dj++) {
+        for (int di = -3; di <= 3; di++) {
+            if (di == 0 && dj == 0) continue;
+            if (di * di + dj * dj > 9) continue;
+            float ox = (float)di, oy = (float)dj;
+            if (ox * n.x + oy * n.y < -0.5f) w0 = w0 + 1.0f;
+            int x = i + di, y = j + dj;
+            if (x < 0 || y < 0 || x >= (int)F.W || y >= (int)F.H) continue;
+            if (F.h[idx(F, (uint32_t)x, (uint32_t)y)] > F.P.ein[3]) w = w + 1.0f;
+        }
+    }
+    if (w0 < 1.0f || (((uint32_t)F.P.front2[3]) & 2u) != 0u) return 1.0f;
+    return cz_powf(cz_clamp(w / w0, 0.0f, 2.5f), F.P.front2[1]);
+}
+
+// film.wgsl:224-229
+__device__ inline float wet_threshold(const FilmDev& F, uint32_t c) {
+    float ty = F.rheo[c].y;
+    float need = cz_max(ty / (F.P.phys[0] * cz_max(F.h[c], 1e-4f)) - cz_len2(F.P.g[0], F.P.g[1]), 0.0f);
+    float ycap = need / cz_max(F.P.g[2], 1e-3f) * F.P.dx;
+    return cz_max(hold_level(F, 1.0f), ycap);
+}
+
+// film.wgsl:231-234
+__device__ __forceinline__ float front_dist(const FilmDev& F, uint32_t c) {
+    if (F.h[c] <= 0.0f) return 0.0f;
+    return nose_dist(F, c);
+}
+
+// film.wgsl:236-244
+__device__ inline float cox_speed(const FilmDev& F, float excess, float mu) {
+    if (excess <= 0.0f) return 0.0f;
+    float theta = cz_atanf(F.P.ein[0]);
+    float span = cz_max(F.P.front[0] - hold_level(F, 1.0f), 1e-6f);
+    float th = theta + (3.14159265f - theta) * cz_clamp(excess / span, 0.0f, 1.0f);
+    float adv = cz_min(theta + F.P.front2[0], 3.1f);
+    if (th <= adv) return 0.0f;
+    return F.P.phys[1] * (th * th * th - adv * adv * adv) / (COX_LOG * cz_max(mu, 1e-4f));
+}
+
+// film.wgsl:246-265
+__device__ inline float cell_excess(const FilmDev& F, uint32_t c) {
+    float e = F.h[c] - hold_level(F, front_dist(F, c));
+    int ox, oy;
+    unpack_off(F.delta[c], ox, oy);
+    float ol = cz_len2((float)ox, (float)oy);
+    if (ol > 0.5f && ol < 60.0f) {
+        float inv = -1.0f / ol;
+        float dirx = (float)ox * inv, diry = (float)oy * inv;
+        float bx = 0.5f * cz_sign((float)(-ox)), by = 0.5f * cz_sign((float)(-oy));
+        int ci = (int)(c % F.W), cj = (int)(c / F.W);
+        for (uint32_t k = 0; k < 3u; k++) {
+            float reach = (float)(4u << k);
+            int sx = (int)(dirx * reach + bx);
+            int sy = (int)(diry * reach + by);
+            int ri = ci + sx, rj = cj + sy;
+            if (ri < 0 || rj < 0 || ri >= (int)F.W || rj >= (int)F.H) break;
+            uint32_t cr = idx(F, (uint32_t)ri, (uint32_t)rj);
+            float hr = F.h[cr];
+            if (hr <= 0.0f) break;
+            e = cz_max(e, hr - hold_level(F, front_dist(F, cr)));
+        }
+    }
+    return e;
+}
+
+// film.wgsl:267-295; returns the translation flux, sign of the normal in tsign
+__device__ inline float rim_translation(const FilmDev& F, uint32_t ca, uint32_t cb, float hf, float2 face_dir,
+                                        float& tsign) {
+    tsign = 0.0f;
+    if ((((uint32_t)F.P.front2[3]) & 1u) != 0u) return 0.0f;
+    float da = front_dist(F, ca);
+    float db = front_dist(F, cb);
+    if (cz_max(da, db) > 3.0f / F.P.front[1]) return 0.0f;
+    float ha = F.h[ca], hb = F.h[cb];
+    float nx = 0.0f, ny = 0.0f;
+    if (da > db && ha > 0.0f) {
+        float2 v = F.fnorm[ca];
+        nx = v.x; ny = v.y;
+    } else if (db > da && hb > 0.0f) {
+        float2 v = F.fnorm[cb];
+        nx = v.x; ny = v.y;
+    } else {
+        if (ha > 0.0f) { float2 v = F.fnorm[ca]; nx = nx + v.x; ny = ny + v.y; }
+        if (hb > 0.0f) { float2 v = F.fnorm[cb]; nx = nx + v.x; ny = ny + v.y; }
+    }
+    float nl = cz_len2(nx, ny);
+    if (nl < 1e-6f) return 0.0f;
+    nx = nx / nl;
+    ny = ny / nl;
+    float s = nx * face_dir.x + ny * face_dir.y;
+    if (cz_abs(s) < 1e-4f) return 0.0f;
+    uint32_t src = ca, dst = cb;
+    if (s < 0.0f) { src = cb; dst = ca; }
+    if (F.h[src] <= 0.0f) return 0.0f;
+    float e = cell_excess(F, src);
+    if (e <= 0.0f) return 0.0f;
+    float e_out = -hold_level(F, 1.0f);
+    if (F.h[dst] > 0.0f) e_out = cell_excess(F, dst);
+    float room = cz_clamp(1.0f - cz_max(e_out, 0.0f) / e, 0.0f, 1.0f);
+    float u = cox_speed(F, e, F.rheo[src].x);
+    tsign = cz_sign(s);
+    return u * hf * room * cz_mix(1.0f, cz_abs(s), F.P.front2[0]);
+}
+
+struct FaceFlux { float q, y, up; };
+
+// film.wgsl:297-372
+__device__ inline FaceFlux face(const FilmDev& F, uint32_t ca, uint32_t cb, float2 gcap, float2 ghyd, float2 gtilt,
+                                bool diag, float2 face_dir) {
+    FaceFlux out = {0.0f, 0.0f, 0.0f};
+    float ha = F.h[ca];
+    float hb = F.h[cb];
+    float hf = 0.5f * (ha + hb);
+    if (hf <= 0.0f) return out;
+    float shape = 1.0f;
+    bool wetting = false;
+    float proj = 1.0f;
+    if (cz_min(ha, hb) <= 0.0f) {
+        uint32_t wet = (ha > 0.0f) ? ca : cb;
+        uint32_t dry = (ha > 0.0f) ? cb : ca;
+        if (F.h[wet] <= wet_threshold(F, wet)) return out;
+        wetting = true;
+        float2 n = F.fnorm[wet];
+        if (n.x * n.x + n.y * n.y < 0.5f) n = front_normal(F, wet);
+        float toward = (ha > 0.0f) ? 1.0f : -1.0f;
+        proj = cz_max(n.x * (face_dir.x * toward) + n.y * (face_dir.y * toward), 0.0f);
+        if (proj <= 0.0f) return out;
+        shape = front_shape(F, dry, n) * wet_gain(n);
+        if (shape <= 0.0f) return out;
+    }
+    float2 ra = F.rheo[ca];
+    float2 rb = F.rheo[cb];
+    float wa = ha / (ha + hb);
+    float wb = 1.0f - wa;
+    float mu = cz_expf(wa * cz_logf(cz_max(ra.x, 1e-4f)) + wb * cz_logf(cz_max(rb.x, 1e-4f)));
+    float ty = wa * ra.y + wb * rb.y;
+    float thin = cz_min(ha, hb) / hf;
+    float fcap = thin * thin * thin;
+    float2 yc = yield_cap(F, make_float2(gcap.x * fcap, gcap.y * fcap), ty);
+    float2 g = make_float2(yc.x + ghyd.x + gtilt.x, yc.y + ghyd.y + gtilt.y);
+    float gmag = cz_len2(g.x, g.y);
+    float ydep = (ty <= 0.0f) ? hf : ((g.x >= 0.0f) ? ha : hb);
+    float qy_y, qy_up;
+    float qy_x = hb_flux(F.P.phys[2], gmag, ydep, mu, ty, qy_y, qy_up);
+    float meff = (qy_x > 0.0f) ? qy_x / cz_max(gmag, 1e-9f) : 0.0f;
+    float mtilt = meff;
+    if (ty > 0.0f) {
+        // plug_mobility (film.wgsl:51-56)
+        float gb = cz_len2(gtilt.x, gtilt.y);
+        if (gb < 1e-6f) {
+            mtilt = 0.0f;
+        } else {
+            uint32_t up = (gtilt.x >= 0.0f) ? ca : cb;
+            mtilt = F.tmp[up] * F.h[up] / gb;
+        }
+    }
+    if (meff <= 0.0f && mtilt <= 0.0f) return out;
+    float fcap2 = cz_powf(thin, F.P.front3[3]);
+    float qn = mtilt * gtilt.x + cz_min(meff, F.P.misc[3]) * ghyd.x + cz_min(meff, F.P.phys[3]) * (gcap.x * fcap2);
+    float trs;
+    float trx = rim_translation(F, ca, cb, hf, face_dir, trs);
+    if (wetting) {
+        float hwet = (ha > 0.0f) ? ha : hb;
+        float want = trx * 2.0f * hwet / cz_max(hf, 1e-9f);
+        qn = trs * cz_max(qn * trs * proj, want) * shape;
+        float vcap = 2.0f;
+        if (cz_abs(qn) > vcap * hf) qn = cz_sign(qn) * vcap * hf;
+    } else {
+        if (trx > 0.0f) qn = trs * cz_max(qn * trs, trx);
+        float vcap = 2.0f;
+        if (cz_abs(qn) > vcap * hf) qn = cz_sign(qn) * vcap * hf;
+        float lo = cz_min(ha, hb);
+        float hi = cz_max(ha, hb);
+        if (lo < 0.5f * hi) {
+            if (diag) {
+                qn = 0.0f;
+            } else {
+                uint32_t tgt = cb;
+                if (hb > ha) tgt = ca;
+                uint32_t nf = feeders(F, tgt);
+                qn = qn / cz_powf((float)(nf > 1u ? nf : 1u), 0.7f);
+            }
+        }
+    }
+    out.q = qn;
+    out.y = qy_y;
+    out.up = qy_up;
+    return out;
+}
+
+__device__ __forceinline__ float2 pressure_at(const FilmDev& F, int i, int j) {
+    return F.pres[idx(F, refl_i(F, i), refl_j(F, j))];
+}
+
+// film.wgsl:9-14
+__device__ __forceinline__ float2 pressure_wet(const FilmDev& F, int i, int j, float2 fallback) {
+    uint32_t c = idx(F, refl_i(F, i), refl_j(F, j));
+    float2 p = F.pres[c];
+    if (F.h[c] > 0.0f) return p;
+    return make_float2(fallback.x, p.y);
+}
+
+struct FaceGeom {
+    bool has;
+    bool diag;
+    uint32_t cb;
+    float2 gcap, ghyd, gtilt, dir;
+};
+
+// Face geometry of the k-th owned face (+x, +y, +x+y, +x-y) of cell (x, y):
+// pressure gradients of film.wgsl:387-439 / 940-978.
+__device__ inline FaceGeom face_geom(const FilmDev& F, uint32_t x, uint32_t y, int k, bool open_check, float2 p00) {
+    FaceGeom fg;
+    fg.has = false;
+    fg.diag = false;
+    int i = (int)x, j = (int)y;
+    float2 gxy = make_float2(F.P.phys[0] * F.P.g[0], F.P.phys[0] * F.P.g[1]);
+    const float s2 = 0.70710678f;
+    const float idx_ = F.P.inv_dx;
+    if (k == 0) {
+        if (!(x + 1u < F.W && cell_active(F, x + 1u, y))) return fg;
+        fg.cb = idx(F, x + 1u, y);
+        float2 p10 = pressure_at(F, i + 1, j);
+        float gmx = -(p10.x - p00.x) * idx_, gmy = -(p10.y - p00.y) * idx_;
+        float2 a = pressure_wet(F, i, j + 1, p00), b = pressure_wet(F, i + 1, j + 1, p10);
+        float2 cc = pressure_wet(F, i, j - 1, p00), d = pressure_wet(F, i + 1, j - 1, p10);
+        float ptx = a.x + b.x - cc.x - d.x, pty = a.y + b.y - cc.y - d.y;
+        fg.gcap = make_float2(gmx, -ptx * 0.25f * idx_);
+        fg.ghyd = make_float2(gmy, -pty * 0.25f * idx_);
+        fg.gtilt = gxy;
+        fg.dir = make_float2(1.0f, 0.0f);
+        fg.has = true;
+    } else if (k == 1) {
+        if (!(y + 1u < F.H && cell_active(F, x, y + 1u))) return fg;
+        fg.cb = idx(F, x, y + 1u);
+        float2 p01 = pressure_at(F, i, j + 1);
+        float gmx = -(p01.x - p00.x) * idx_, gmy = -(p01.y - p00.y) * idx_;
+        float2 a = pressure_wet(F, i + 1, j, p00), b = pressure_wet(F, i + 1, j + 1, p01);
+        float2 cc = pressure_wet(F, i - 1, j, p00), d = pressure_wet(F, i - 1, j + 1, p01);
+        float ptx = a.x + b.x - cc.x - d.x, pty = a.y + b.y - cc.y - d.y;
+        fg.gcap = make_float2(gmx, -ptx * 0.25f * idx_);
+        fg.ghyd = make_float2(gmy, -pty * 0.25f * idx_);
+        fg.gtilt = make_float2(gxy.y, gxy.x);
+        fg.dir = make_float2(0.0f, 1.0f);
+        fg.has = true;
+    } else if (k == 2) {
+        if (!(x + 1u < F.W && y + 1u < F.H && cell_active(F, x + 1u, y + 1u))) return fg;
+        fg.cb = idx(F, x + 1u, y + 1u);
+        if (open_check) {
+            float hs = F.P.ein[3];
+            bool open = cz_min(F.h[idx(F, x, y)], F.h[fg.cb]) > 0.0f || F.h[idx(F, x + 1u, y)] > hs ||
+                        F.h[idx(F, x, y + 1u)] > hs;
+            if (!open) return fg;
+        }
+        float2 p11 = pressure_at(F, i + 1, j + 1);
+        float2 pa = pressure_wet(F, i + 1, j, p00);
+        float2 pb = pressure_wet(F, i, j + 1, p11);
+        fg.gcap = make_float2(-(p11.x - p00.x) * idx_ * s2, -(pa.x - pb.x) * idx_ * s2);
+        fg.ghyd = make_float2(-(p11.y - p00.y) * idx_ * s2, -(pa.y - pb.y) * idx_ * s2);
+        fg.gtilt = make_float2((gxy.x + gxy.y) * s2, (gxy.x - gxy.y) * s2);
+        fg.dir = make_float2(s2, s2);
+        fg.diag = true;
+        fg.has = true;
+    } else {
+        if (!(x + 1u < F.W && y > 0u && cell_active(F, x + 1u, y - 1u))) return fg;
+        fg.cb = idx(F, x + 1u, y - 1u);
+        if (open_check) {
+            float hs = F.P.ein[3];
+            bool open = cz_min(F.h[idx(F, x, y)], F.h[fg.cb]) > 0.0f || F.h[idx(F, x + 1u, y)] > hs ||
+                        F.h[idx(F, x, y - 1u)] > hs;
+            if (!open) return fg;
+        }
+        float2 p1m = pressure_at(F, i + 1, j - 1);
+        float2 pa = pressure_wet(F, i + 1, j, p00);
+        float2 pb = pressure_wet(F, i, j - 1, p1m);
+        fg.gcap = make_float2(-(p1m.x - p00.x) * idx_ * s2, -(pa.x - pb.x) * idx_ * s2);
+        fg.ghyd = make_float2(-(p1m.y - p00.y) * idx_ * s2, -(pa.y - pb.y) * idx_ * s2);
+        fg.gtilt = make_float2((gxy.x - gxy.y) * s2, (gxy.x + gxy.y) * s2);
+        fg.dir = make_float2(s2, -s2);
+        fg.diag = true;
+        fg.has = true;
+    }
+    return fg;
+}
+
+// ---------------------------------------------------------------------------
+// the 4-deep pigment stack (strata.wgsl:1-155)
+
+struct Stack {
+    float th[4];
+    float cu[4];
+    float pg[4][16];
+};
+
+__device__ inline void stack_load(const FilmDev& F, uint32_t c, Stack& s) {
+    float4 t = F.strata_h[c];
+    float4 u = F.strata_cure[c];
+    s.th[0] = t.x; s.th[1] = t.y; s.th[2] = t.z; s.th[3] = t.w;
+    s.cu[0] = u.x; s.cu[1] = u.y; s.cu[2] = u.z; s.cu[3] = u.w;
+#pragma unroll
+    for (int k = 0; k < 4; k++) unpack8(F.strata_pig[(c * 4u + k) * 2u], F.strata_pig[(c * 4u + k) * 2u + 1u], s.pg[k]);
+}
+__device__ inline void stack_store_to(float4* sh, float4* scu, uint4* spg, uint32_t c, const Stack& s) {
+    sh[c] = make_float4(s.th[0], s.th[1], s.th[2], s.th[3]);
+    scu[c] = make_float4(s.cu[0], s.cu[1], s.cu[2], s.cu[3]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint4 a, b;
+        pack8(s.pg[k], a, b);
+        spg[(c * 4u + k) * 2u] = a;
+        spg[(c * 4u + k) * 2u + 1u] = b;
+    }
+}
+
+__device__ __forceinline__ float pig_dist(const float* a, const float* b) {
+    float d = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 13; i++) d = d + cz_abs(a[i] - b[i]);
+    return d;
+}
+
+// strata.wgsl:45-59 (o may alias a)
+__device__ __forceinline__ void pig_mix(const float* a, float wa, const float* b, float wb, float* o) {
+    float a14 = a[14], b14 = b[14];
+    float w = cz_max(wa + wb, 1e-20f);
+#pragma unroll
+    for (int i = 0; i < 16; i++) o[i] = (a[i] * wa + b[i] * wb) / w;
+    float va = (a14 > 0.0f) ? wa : 0.0f;
+    float vb = (b14 > 0.0f) ? wb : 0.0f;
+    if (va + vb > 0.0f) o[14] = (a14 * va + b14 * vb) / (va + vb);
+    else o[14] = 0.0f;
+}
+
+__device__ __forceinline__ uint32_t stack_count(const Stack& s) {
+    uint32_t n = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; k++)
+        if (s.th[k] > 0.0f) n = k + 1u;
+    return n;
+}
+
+__device__ inline void copy16(float* d, const float* s) {
+#pragma unroll
+    for (int i = 0; i < 16; i++) d[i] = s[i];
+}
+
+// strata.wgsl:69-83
+__device__ inline void stack_merge(Stack& s, uint32_t a) {
+    uint32_t b = a + 1u;
+    float ta = s.th[a], tb = s.th[b];
+    pig_mix(s.pg[a], ta, s.pg[b], tb, s.pg[a]);
+    s.cu[a] = (s.cu[a] * ta + s.cu[b] * tb) / cz_max(ta + tb, 1e-20f);
+    s.th[a] = ta + tb;
+    for (uint32_t k = b; k < 3u; k++) {
+        s.th[k] = s.th[k + 1u];
+        s.cu[k] = s.cu[k + 1u];
+        copy16(s.pg[k], s.pg[k + 1u]);
+    }
+    s.th[3] = 0.0f;
+    s.cu[3] = 0.0f;
+}
+
+// strata.wgsl:85-103
+__device__ inline void stack_compact(const FilmDev& F, Stack& s) {
+    for (int sweep = 0; sweep < 4; sweep++) {
+        for (uint32_t k = 0; k < 3u; k++) {
+            if (s.th[k] <= 0.0f && s.th[k + 1u] > 0.0f) {
+                s.th[k] = s.th[k + 1u];
+                s.cu[k] = s.cu[k + 1u];
+                copy16(s.pg[k], s.pg[k + 1u]);
+                s.th[k + 1u] = 0.0f;
+                s.cu[k + 1u] = 0.0f;
+            }
+        }
+    }
+    float tiny = F.P.misc[0];
+    for (uint32_t k = 0; k < 3u; k++) {
+        if (s.th[k] > 0.0f && s.th[k] < tiny && s.th[k + 1u] > 0.0f) stack_merge(s, k);
+    }
+}
+
+// strata.wgsl:105-131
+__device__ inline void stack_push_top(const FilmDev& F, Stack& s, float vol, const float* pig, float cure) {
+    if (vol <= 0.0f) return;
+    uint32_t n = stack_count(s);
+    if (n > 0u) {
+        uint32_t t = n - 1u;
+        bool close = pig_dist(s.pg[t], pig) < F.P.misc[1] || s.th[t] < F.P.misc[0];
+        if (close) {
+            pig_mix(s.pg[t], s.th[t], pig, vol, s.pg[t]);
+            s.cu[t] = (s.cu[t] * s.th[t] + cure * vol) / (s.th[t] + vol);
+            s.th[t] = s.th[t] + vol;
+            return;
+        }
+    }
+    if (n == 4u) {
+        uint32_t best = 0;
+        float bd = 1e30f;
+        for (uint32_t k = 0; k < 3u; k++) {
+            float d = pig_dist(s.pg[k], s.pg[k + 1u]);
+            if (d < bd) { bd = d; best = k; }
+        }
+        stack_merge(s, best);
+    }
+    uint32_t t = stack_count(s);
+    s.th[t] = vol;
+    copy16(s.pg[t], pig);
+    s.cu[t] = cure;
+}
+
+// strata.wgsl:133-155
+__device__ inline void stack_add_band(const FilmDev& F, Stack& s, float zlo, float zhi, float vol, const float* pig,
+                                      float cure) {
+    if (vol <= 0.0f) return;
+    float span = cz_max(zhi - zlo, 1e-12f);
+    float z0 = 0.0f;
+    float above = vol;
+    for (uint32_t k = 0; k < 4u; k++) {
+        float tk = s.th[k];
+        if (tk <= 0.0f) break;
+        float z1 = z0 + tk;
+        float ov = cz_max(cz_min(zhi, z1) - cz_max(zlo, z0), 0.0f);
+        float part = vol * ov / span;
+        if (part > 0.0f) {
+            pig_mix(s.pg[k], tk, pig, part, s.pg[k]);
+            s.cu[k] = (s.cu[k] * tk + cure * part) / (tk + part);
+            s.th[k] = tk + part;
+            above = above - part;
+        }
+        z0 = z1;
+    }
+    if (above > 1e-14f) stack_push_top(F, s, above, pig, cure);
+}
+
+// ---------------------------------------------------------------------------
+// km.wgsl:1-33
+
+__device__ __forceinline__ void km_layer(float k, float s, float t, float& R, float& T) {
+    R = 0.0f;
+    T = 1.0f;
+    if (k + s < 1e-6f || t <= 0.0f) return;
+    float bs = sqrtf(cz_max(k * (k + 2.0f * s), 0.0f));
+    float arg = cz_min(bs * t, 20.0f);
+    float e = cz_expf(arg);
+    float sh = 0.5f * (e - 1.0f / e);
+    float ch = 0.5f * (e + 1.0f / e);
+    float den = (k + s) * sh + bs * ch;
+    if (den < 1e-12f) return;
+    R = s * sh / den;
+    T = bs / den;
+}
+__device__ __forceinline__ float composite(float rtx, float rty, float rb) {
+    return rtx + rty * rty * rb / cz_max(1.0f - rtx * rb, 1e-4f);
+}
+__device__ __forceinline__ void layer_ks(const float* pigtab, const float* pig, uint32_t band, float& ko, float& so) {
+    float k = 0.0f, s = 0.0f, amount = 0.0f, weight = 0.0f;
+    for (uint32_t i = 0; i < (uint32_t)NPIG; i++) {
+        float c = pig[i];
+        if (c > 0.0f) {
+            float w = c * pigtab[i * PIG_STRIDE + 34u];
+            k = k + w * pigtab[i * PIG_STRIDE + band];
+            s = s + w * pigtab[i * PIG_STRIDE + NBAND + band];
+            amount = amount + c;
+            weight = weight + w;
+        }
+    }
+    float sc = amount / cz_max(weight, 1e-9f);
+    ko = k * sc;
+    so = s * sc;
+}
+
+}  // namespace cz

@@ -1,0 +1,848 @@
+// film_host.hip - host side of the film: the C ABI of include/canvaz_film.h
+// over the gfx950 kernels.  Mirrors `Film` (src/film/mod.rs:445-1462):
+// buffers, update_params / choose_substeps, upload_deposits binning,
+// encode_frame order, stats readback, read/write_state, dry_now, clear.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "film_kernels.h"
+#include "host_ref.h"
+
+using namespace cz;
+
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) return set_err(CZ_EHIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+namespace {
+
+constexpr uint32_t kMaxTileBlocks = 2048u;  // resident 256-thread blocks on 256 CUs
+
+struct ProfSlot {
+    std::string label;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct cz_film {
+    uint32_t W = 0, H = 0, n = 0, tw = 0, th = 0, nt16 = 0, t64x = 0, t64y = 0, pw = 0, ph = 0;
+    float dx = 0.0f;
+    cz_settings s;
+    cz_overrides ov;
+    bool ov_from_env = true;
+    cz_params P;
+    float h_max = 0.0f, mu_min = 1.0f, h_max_pending = 0.0f;
+    uint32_t implicit_steps = 1, substeps_last = 1, frame = 0, active_tiles = 0;
+    double total_deposited = 0.0, weave_volume = 0.0;
+    float cg_report[4] = {0, 0, 0, 0};
+    bool stats_pending = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    // device buffers
+    float *h = nullptr, *m = nullptr;
+    float2* rheo = nullptr;
+    float4 *strata_h = nullptr, *strata_cure = nullptr, *strata_h2 = nullptr, *strata_cure2 = nullptr;
+    uint4 *strata_pig = nullptr, *strata_pig2 = nullptr;
+    uint32_t *sub_refl = nullptr, *lay = nullptr;
+    float2 *pres = nullptr, *fnorm = nullptr;
+    float4 *flux = nullptr, *acc = nullptr, *acc_s = nullptr, *alpha_s = nullptr, *cg_m = nullptr;
+    float *alpha = nullptr, *tmp = nullptr, *h_frame = nullptr;
+    uint32_t *delta_a = nullptr, *delta_b = nullptr;
+    bool delta_cur_a = true;
+    float *cg_d = nullptr, *cg_x = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr, *cg_t = nullptr,
+          *cg_t2 = nullptr;
+    double *cg_part = nullptr, *cg_part2 = nullptr;
+    float* cg_s = nullptr;
+    uint32_t *tile_flags = nullptr, *tile_list = nullptr, *tile_moved = nullptr, *active_count = nullptr;
+    float2* stats_out = nullptr;
+    float* pool_out = nullptr;
+    float *tsum_a = nullptr, *tsum_b = nullptr;
+    float* pigtab = nullptr;
+    cz_deposit* deposits = nullptr;
+    uint32_t *tile_start = nullptr, *tile_count = nullptr, *tile_items = nullptr;
+    float* frame_out = nullptr;
+    // host tables and pinned staging
+    float pigtab_h[CZ_NPIG * CZ_PIG_STRIDE];
+    float cmf_h[CZ_NBAND * 3];
+    float* frame_out_h = nullptr;
+    cz_deposit* dep_h = nullptr;
+    uint32_t *tstart_h = nullptr, *tcount_h = nullptr, *titems_h = nullptr;
+    uint32_t deposit_count = 0;
+    // profiling
+    bool prof_on = false;
+    std::vector<ProfSlot> prof_pending;
+    std::vector<hipEvent_t> ev_pool;
+    std::map<std::string, std::pair<double, uint64_t>> prof_acc;
+
+    uint32_t* delta_cur() const { return delta_cur_a ? delta_a : delta_b; }
+    uint32_t* delta_other() const { return delta_cur_a ? delta_b : delta_a; }
+
+    FilmDev dev() const {
+        FilmDev F;
+        memset(&F, 0, sizeof(F));
+        F.P = P;
+        F.W = W; F.H = H; F.n = n; F.tw = tw; F.th = th; F.nt16 = nt16; F.t64x = t64x; F.t64y = t64y;
+        F.pw = pw; F.ph = ph;
+        F.h = h; F.m = m; F.rheo = rheo; F.strata_h = strata_h; F.strata_pig = strata_pig;
+        F.strata_cure = strata_cure; F.sub_refl = sub_refl; F.lay = lay;
+        F.pres = pres; F.flux = flux; F.acc = acc; F.alpha = alpha;
+        F.delta = delta_cur(); F.delta_out = delta_other();
+        F.tmp = tmp; F.acc_s = acc_s; F.alpha_s = alpha_s; F.strata_h2 = strata_h2; F.strata_pig2 = strata_pig2;
+        F.strata_cure2 = strata_cure2; F.h_frame = h_frame; F.fnorm = fnorm; F.cg_m = cg_m;
+        F.cg_d = cg_d; F.cg_x = cg_x; F.cg_r = cg_r; F.cg_p = cg_p; F.cg_q = cg_q; F.cg_t = cg_t; F.cg_t2 = cg_t2;
+        F.cg_part = cg_part; F.cg_part2 = cg_part2; F.cg_s = cg_s;
+        F.tile_flags = tile_flags; F.tile_list = tile_list; F.tile_moved = tile_moved; F.active_count = active_count;
+        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.tsum_b = tsum_b; F.pigtab = pigtab;
+        F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
+        F.frame_out = frame_out;
+        return F;
+    }
+
+    uint32_t tile_grid() const { return std::max(1u, std::min(nt16, kMaxTileBlocks)); }
+
+    hipEvent_t take_event() {
+        if (!ev_pool.empty()) {
+            hipEvent_t e = ev_pool.back();
+            ev_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+};
+
+namespace {
+
+template <typename T>
+int dalloc(cz_film* f, T** p, size_t count) {
+    void* v = nullptr;
+    size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    hipError_t e = hipMalloc(&v, bytes);
+    if (e != hipSuccess) return set_err(CZ_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    f->allocs.push_back(v);
+    *p = (T*)v;
+    e = hipMemsetAsync(v, 0, bytes, f->stream);
+    if (e != hipSuccess) return set_err(CZ_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    return CZ_OK;
+}
+
+// RAII-free launch helper: optional event pair around a launch, by label.
+struct Launch {
+    cz_film* f;
+    const char* label;
+    hipEvent_t a = nullptr, b = nullptr;
+    Launch(cz_film* ff, const char* l) : f(ff), label(l) {
+        if (f->prof_on) {
+            a = f->take_event();
+            b = f->take_event();
+            if (a) hipEventRecord(a, f->stream);
+        }
+    }
+    ~Launch() {
+        if (f->prof_on && a && b) {
+            hipEventRecord(b, f->stream);
+            f->prof_pending.push_back({label, a, b});
+        }
+    }
+};
+
+void fold_profile(cz_film* f) {
+    for (auto& s : f->prof_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) {
+            auto& acc = f->prof_acc[s.label];
+            acc.first += (double)ms * 1000.0;
+            acc.second += 1;
+        }
+        f->ev_pool.push_back(s.a);
+        f->ev_pool.push_back(s.b);
+    }
+    f->prof_pending.clear();
+}
+
+// update_params (src/film/mod.rs:869-917)
+void update_params(cz_film* f, float frame_dt, uint32_t substeps) {
+    if (f->ov.no_implicit) f->s.implicit_hydro = 0;
+    const cz_settings& s = f->s;
+    float g[3];
+    cz_film_gravity_vec(f, g);
+    float theta = czh::to_radians(s.contact_angle_deg);
+    float pi_a = 3.0f * s.sigma * (1.0f - cosf(theta)) / s.h_star;
+    float dt = frame_dt / (float)substeps;
+    float dx = f->dx;
+    float m_cap_exp = 0.5f * dx * dx * dx * dx / (32.0f * s.sigma * dt);
+    float m_cap = fmaxf(fminf(s.m_cap, m_cap_exp), 1e-14f);
+    float m_exp = 0.1f * dx * dx / (dt * s.rho * fmaxf(g[2], 0.1f));
+    cz_params& P = f->P;
+    P.dt = dt;
+    P.frame_dt = frame_dt;
+    P.g[0] = g[0]; P.g[1] = g[1]; P.g[2] = g[2]; P.g[3] = s.gravity;
+    P.phys[0] = s.rho; P.phys[1] = s.sigma; P.phys[2] = s.n_index;
+    P.phys[3] = (s.capillary && !f->ov.no_cap) ? m_cap : 0.0f;
+    float lc = sqrtf(s.sigma / (s.rho * 9.81f));
+    P.ein[0] = fmaxf(tanf(theta), 0.05f); P.ein[1] = lc * sinf(0.5f * theta); P.ein[2] = pi_a; P.ein[3] = s.h_min;
+    P.cure[0] = s.cure_rate; P.cure[1] = s.tau_cure; P.cure[2] = s.lnmu_cure; P.cure[3] = s.h_star;
+    P.misc[0] = 2.0e-6f; P.misc[1] = 0.12f; P.misc[2] = s.medium_mu; P.misc[3] = fminf(m_exp, s.m_cap_hydro);
+    P.flags[0] = s.debug_view; P.flags[1] = s.slump; P.flags[2] = 0; P.flags[3] = 0;
+    float lc_z = sqrtf(s.sigma / (s.rho * fmaxf(g[2], 0.1f)));
+    float cap_explicit = fminf(m_exp, s.m_cap_hydro);
+    float h_big = fmaxf(fmaxf(f->h_max, f->h_max_pending), 1e-4f);
+    float m_phys = h_big * h_big * h_big / (3.0f * fmaxf(f->mu_min, 1e-3f)) - cap_explicit;
+    float k_max = 40.0f;
+    float kappa_frame = frame_dt * s.rho * fmaxf(g[2], 0.1f) / (dx * dx);
+    uint32_t n_i = 0;
+    if (m_phys > 0.0f) n_i = std::min((uint32_t)fmaxf(ceilf(kappa_frame * m_phys / k_max), 1.0f), 4u);
+    float dt_i = frame_dt / (float)std::max(n_i, 1u);
+    float m_imp_max = k_max * dx * dx / (dt_i * s.rho * fmaxf(g[2], 0.1f));
+    f->implicit_steps = n_i;
+    float m_phys_cap = h_big * h_big * h_big / (3.0f * fmaxf(f->mu_min, 1e-3f));
+    float m_cap_impl = (f->ov.capcond / 28.0f) * czh::powi4(dx) / (dt_i * s.sigma);
+    float m_c = s.capillary_implicit ? fminf(m_phys_cap, m_cap_impl) : 0.0f;
+    float cc = dt_i * s.sigma * m_c / czh::powi4(dx);
+    float kc = m_c * s.sigma / (dx * dx * dx);
+    P.front3[0] = cc; P.front3[1] = kc; P.front3[2] = m_c; P.front3[3] = f->ov.fcap;
+    P.front[0] = f->ov.no_hold ? 0.0f : 2.0f * lc_z * sinf(0.5f * theta);
+    P.front[1] = dx / lc_z; P.front[2] = m_imp_max; P.front[3] = dt_i;
+    uint32_t flags = (f->ov.no_transl ? 1u : 0u) | ((f->ov.no_shape ? 1u : 0u) << 1);
+    float hyst = f->ov.has_hyst ? f->ov.hyst_deg : s.hysteresis_deg;
+    P.front2[0] = czh::to_radians(hyst); P.front2[1] = f->ov.kshape; P.front2[2] = f->ov.kline;
+    P.front2[3] = (float)flags;
+}
+
+// poll_stats (src/film/mod.rs:935-973) on the frame scalars of the last frame.
+int poll_stats(cz_film* f) {
+    if (!f->stats_pending) return CZ_OK;
+    HIP_TRY(hipStreamSynchronize(f->stream));
+    fold_profile(f);
+    const float* o = f->frame_out_h;
+    f->h_max = o[0];
+    f->mu_min = (o[1] > 1e8f) ? 1.0f : o[1];
+    f->active_tiles = cz_f2bits(o[2]);
+    for (int k = 0; k < 4; k++) f->cg_report[k] = o[4 + k];
+    f->h_max_pending = 0.0f;
+    f->stats_pending = false;
+    return CZ_OK;
+}
+
+int wait_idle(cz_film* f) {
+    HIP_TRY(hipStreamSynchronize(f->stream));
+    fold_profile(f);
+    return CZ_OK;
+}
+
+// upload_deposits (src/film/mod.rs:1175-1224)
+int upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* sent) {
+    int rc = wait_idle(f);  // the pinned staging may still feed the last frame's copies
+    if (rc) return rc;
+    uint32_t count = std::min<uint32_t>(n, CZ_MAX_DEPOSITS);
+    f->deposit_count = count;
+    uint32_t nt = f->t64x * f->t64y;
+    std::vector<uint32_t> cnt(nt, 0u);
+    std::vector<int32_t> rect((size_t)count * 4, -1);
+    float tile_m = 64.0f * f->dx;
+    for (uint32_t di = 0; di < count; di++) {
+        const cz_deposit& d = deps[di];
+        f->h_max_pending = fmaxf(f->h_max_pending, czh::max_height(d) + f->h_max);
+        float rb = fmaxf(d.r0, d.r1);
+        bool inside = fmaxf(d.p0[0], d.p1[0]) + rb > 0.0f && fmaxf(d.p0[1], d.p1[1]) + rb > 0.0f &&
+                      fminf(d.p0[0], d.p1[0]) - rb < f->s.width_m && fminf(d.p0[1], d.p1[1]) - rb < f->s.height_m;
+        if (!inside) continue;
+        if ((d.kind & CZ_DEPOSIT_RECYCLED) == 0u)
+            f->total_deposited += (double)d.vol * (double)czh::inside_fraction(d, f->s.width_m, f->s.height_m);
+        float minx = fminf(d.p0[0], d.p1[0]) - rb, maxx = fmaxf(d.p0[0], d.p1[0]) + rb;
+        float miny = fminf(d.p0[1], d.p1[1]) - rb, maxy = fmaxf(d.p0[1], d.p1[1]) + rb;
+        int64_t mx = (int64_t)f->t64x - 1, my = (int64_t)f->t64y - 1;
+        int64_t tx0 = std::min((int64_t)fmaxf(floorf(minx / tile_m), 0.0f), mx);
+        int64_t tx1 = std::min((int64_t)fmaxf(floorf(maxx / tile_m), 0.0f), mx);
+        int64_t ty0 = std::min((int64_t)fmaxf(floorf(miny / tile_m), 0.0f), my);
+        int64_t ty1 = std::min((int64_t)fmaxf(floorf(maxy / tile_m), 0.0f), my);
+        if (maxx < 0.0f || maxy < 0.0f) continue;
+        rect[4 * di] = (int32_t)tx0; rect[4 * di + 1] = (int32_t)tx1;
+        rect[4 * di + 2] = (int32_t)ty0; rect[4 * di + 3] = (int32_t)ty1;
+        for (int64_t ty = ty0; ty <= ty1; ty++)
+            for (int64_t tx = tx0; tx <= tx1; tx++) cnt[(size_t)(ty * f->t64x + tx)]++;
+    }
+    uint32_t cap = CZ_MAX_TILE_ITEMS, items = 0;
+    for (uint32_t t = 0; t < nt; t++) {
+        uint32_t take = std::min(cnt[t], cap - items);
+        f->tstart_h[t] = items;
+        f->tcount_h[t] = take;
+        items += take;
+    }
+    std::vector<uint32_t> fill(nt, 0u);
+    for (uint32_t di = 0; di < count; di++) {
+        if (rect[4 * di] < 0) continue;
+        for (int32_t ty = rect[4 * di + 2]; ty <= rect[4 * di + 3]; ty++)
+            for (int32_t tx = rect[4 * di]; tx <= rect[4 * di + 1]; tx++) {
+                uint32_t t = (uint32_t)ty * f->t64x + (uint32_t)tx;
+                if (fill[t] < f->tcount_h[t]) f->titems_h[f->tstart_h[t] + fill[t]++] = di;
+            }
+    }
+    if (count > 0) memcpy(f->dep_h, deps, (size_t)count * sizeof(cz_deposit));
+    if (count > 0) HIP_TRY(hipMemcpyAsync(f->deposits, f->dep_h, (size_t)count * sizeof(cz_deposit), hipMemcpyHostToDevice, f->stream));
+    HIP_TRY(hipMemcpyAsync(f->tile_start, f->tstart_h, nt * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+    HIP_TRY(hipMemcpyAsync(f->tile_count, f->tcount_h, nt * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+    if (items > 0) HIP_TRY(hipMemcpyAsync(f->tile_items, f->titems_h, items * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+    if (sent) *sent = count;
+    return CZ_OK;
+}
+
+#define LAUNCH(label, kern, grid, block, ...)                              \
+    do {                                                                   \
+        Launch _l(f, label);                                               \
+        hipLaunchKernelGGL(kern, grid, block, 0, f->stream, __VA_ARGS__);  \
+    } while (0)
+
+// encode_frame (src/film/mod.rs:1322-1462)
+int encode_frame(cz_film* f, uint32_t substeps) {
+    const dim3 b256(256);
+    const dim3 gcell((f->W + 15u) / 16u, (f->H + 15u) / 16u);
+    const dim3 gt(f->tile_grid());
+    FilmDev F = f->dev();
+    if (f->deposit_count > 0) LAUNCH("deposit", k_deposit, gcell, b256, F);
+    LAUNCH("mark", k_mark_tiles, dim3(f->nt16), b256, F);
+    LAUNCH("compact", k_compact_tiles, dim3(1), dim3(1024), F);
+    LAUNCH("cure", k_cure_clear, gcell, b256, F, 1);
+    if (f->s.implicit_hydro) {
+        for (uint32_t it = 0; it < f->implicit_steps; it++) {
+            F = f->dev();
+            LAUNCH("lap", k_lap_plug, gt, b256, F, 0);
+            LAUNCH("cg_setup", k_cg_setup, gt, b256, F);
+            LAUNCH("cg_lap", k_cg_lap, gt, b256, F, 1u);
+            LAUNCH("cg_init", k_cg_init, gt, b256, F);
+            LAUNCH("cg_reduce", k_cg_reduce, dim3(1), b256, F, 0);
+            for (uint32_t k = 0; k < f->s.implicit_iters; k++) {
+                LAUNCH("cg_lap", k_cg_lap, gt, b256, F, 0u);
+                LAUNCH("cg_iter_a", k_cg_iter_a, gt, b256, F);
+                LAUNCH("cg_reduce", k_cg_reduce, dim3(1), b256, F, 1);
+                LAUNCH("cg_iter_b", k_cg_iter_b, gt, b256, F);
+                LAUNCH("cg_reduce", k_cg_reduce, dim3(1), b256, F, 2);
+                LAUNCH("cg_iter_c", k_cg_iter_c, gt, b256, F);
+            }
+            LAUNCH("cg_lap", k_cg_lap, gt, b256, F, 4u);
+            LAUNCH("cg_finish", k_cg_finish, gt, b256, F);
+            LAUNCH("cg_reduce", k_cg_reduce, dim3(1), b256, F, 3);
+            LAUNCH("cg_rescale", k_cg_rescale, gt, b256, F);
+        }
+    }
+    for (uint32_t s = 0; s < substeps; s++) {
+        F = f->dev();
+        LAUNCH("lap_plug", k_lap_plug, gt, b256, F, 1);
+        LAUNCH("flux", k_fluxes, gt, b256, F);
+        LAUNCH("limit", k_limiter, gt, b256, F);
+        LAUNCH("apply", k_apply, gt, b256, F);
+        LAUNCH("nose", k_nose, gt, b256, F);
+        f->delta_cur_a = !f->delta_cur_a;
+    }
+    F = f->dev();
+    LAUNCH("delta_sync", k_delta_sync, gt, b256, F, f->delta_other());
+    LAUNCH("lap", k_lap_plug, gt, b256, F, 0);
+    LAUNCH("split", k_strata_split, gt, b256, F);
+    LAUNCH("reduce_pool", k_reduce_pool, dim3(f->t64x, f->t64y), b256, F);
+    LAUNCH("strata_limit", k_strata_limit, gt, b256, F);
+    LAUNCH("strata_update", k_strata_update, gt, b256, F);
+    LAUNCH("copy_back", k_strata_copy_back, gt, b256, F);
+    float* tin = f->tsum_a;
+    float* tout = f->tsum_b;
+    for (int k = 0; k < 6; k++) {
+        LAUNCH("strata_fill", k_strata_fill, gt, b256, F, (const float*)tin, tout);
+        std::swap(tin, tout);
+    }
+    LAUNCH("frame_stats", k_frame_stats, dim3(1), b256, F);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(f->frame_out_h, f->frame_out, 16 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+    f->stats_pending = true;
+    f->substeps_last = substeps;
+    f->frame++;
+    f->deposit_count = 0;
+    return CZ_OK;
+}
+
+struct BufDesc {
+    void* ptr;
+    uint64_t bytes;
+};
+
+BufDesc buffer_of(const cz_film* f, int which) {
+    uint64_t n = f->n;
+    switch (which) {
+    case CZ_PART_H: return {f->h, 4 * n};
+    case CZ_PART_M: return {f->m, 4 * n};
+    case CZ_PART_RHEO: return {f->rheo, 8 * n};
+    case CZ_PART_STRATA_H: return {f->strata_h, 16 * n};
+    case CZ_PART_STRATA_PIG: return {f->strata_pig, 128 * n};
+    case CZ_PART_STRATA_CURE: return {f->strata_cure, 16 * n};
+    case CZ_PART_SUB_REFL: return {f->sub_refl, 32 * n};
+    case CZ_PART_LAY: return {f->lay, 4 * n};
+    case CZ_BUF_DELTA: return {f->delta_cur(), 4 * n};
+    case CZ_BUF_FNORM: return {f->fnorm, 8 * n};
+    case CZ_BUF_PRES: return {f->pres, 8 * n};
+    case CZ_BUF_H_FRAME: return {f->h_frame, 4 * n};
+    case CZ_BUF_CG_M: return {f->cg_m, 16 * n};
+    case CZ_BUF_CG_P: return {f->cg_p, 4 * n};
+    case CZ_BUF_TILE_MOVED: return {f->tile_moved, 4 * (uint64_t)f->nt16};
+    case CZ_BUF_TILE_FLAGS: return {f->tile_flags, 4 * (uint64_t)f->nt16};
+    case CZ_BUF_TMP: return {f->tmp, 4 * n};
+    case CZ_BUF_FLUX: return {f->flux, 16 * n};
+    case CZ_BUF_ACC: return {f->acc, 16 * n};
+    case CZ_BUF_ALPHA: return {f->alpha, 4 * n};
+    case CZ_BUF_STATS: return {f->stats_out, 8 * (uint64_t)f->t64x * f->t64y};
+    case CZ_BUF_POOL: return {f->pool_out, 8 * (uint64_t)f->pw * f->ph};
+    case CZ_BUF_TILE_LIST: return {f->tile_list, 4 * (uint64_t)f->nt16};
+    case CZ_BUF_ACC_S: return {f->acc_s, 64 * n};
+    case CZ_BUF_ALPHA_S: return {f->alpha_s, 16 * n};
+    default: return {nullptr, 0};
+    }
+}
+
+// The initial contents of Film::new / clear (src/film/mod.rs:686-693, 1009-1028).
+int init_state(cz_film* f) {
+    std::vector<float> wv((size_t)f->n);
+    cz_weave(f->W, f->H, f->dx, f->s.weave_um * 1e-6f, wv.data());
+    double wsum = 0.0;
+    for (float v : wv) wsum += (double)v;
+    f->weave_volume = wsum * (double)f->dx * (double)f->dx;
+    HIP_TRY(hipMemcpyAsync(f->m, wv.data(), wv.size() * 4, hipMemcpyHostToDevice, f->stream));
+    std::vector<uint32_t> white((size_t)f->n * 8);
+    cz_substrate_refl(f->n, f->s.raw_canvas, white.data());
+    HIP_TRY(hipMemcpyAsync(f->sub_refl, white.data(), white.size() * 4, hipMemcpyHostToDevice, f->stream));
+    std::vector<float2> ones((size_t)f->n, make_float2(1.0f, 0.0f));
+    HIP_TRY(hipMemcpyAsync(f->rheo, ones.data(), ones.size() * 8, hipMemcpyHostToDevice, f->stream));
+    HIP_TRY(hipStreamSynchronize(f->stream));
+    return CZ_OK;
+}
+
+void destroy(cz_film* f) {
+    if (!f) return;
+    if (f->stream) hipStreamSynchronize(f->stream);
+    for (void* p : f->allocs) hipFree(p);
+    if (f->frame_out_h) hipHostFree(f->frame_out_h);
+    if (f->dep_h) hipHostFree(f->dep_h);
+    if (f->tstart_h) hipHostFree(f->tstart_h);
+    if (f->tcount_h) hipHostFree(f->tcount_h);
+    if (f->titems_h) hipHostFree(f->titems_h);
+    for (auto& s : f->prof_pending) {
+        hipEventDestroy(s.a);
+        hipEventDestroy(s.b);
+    }
+    for (auto e : f->ev_pool) hipEventDestroy(e);
+    if (f->stream) hipStreamDestroy(f->stream);
+    delete f;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cz_last_error(void) { return g_err; }
+const char* cz_version(void) { return "canvaz-film-mi355x 0.1 (gfx950)"; }
+
+int cz_film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigtab, const float* cmf, cz_film** out) {
+    if (!out || w == 0 || h == 0) return set_err(CZ_EINVAL, "cz_film_create: bad size %ux%u", w, h);
+    if ((uint64_t)w * h > (1ull << 31)) return set_err(CZ_EINVAL, "cz_film_create: %ux%u cells exceed 2^31", w, h);
+    *out = nullptr;
+    cz_film* f = new cz_film();
+    if (s) f->s = *s; else cz_settings_default(&f->s);
+    if (f->s.slump) {
+        destroy(f);
+        return set_err(CZ_EUNSUP, "cz_film_create: the slump pass (settings.slump) is not built");
+    }
+    cz_overrides_from_env(&f->ov);
+    f->W = w; f->H = h; f->n = w * h;
+    f->tw = (w + 15u) / 16u; f->th = (h + 15u) / 16u; f->nt16 = f->tw * f->th;
+    f->t64x = (w + 63u) / 64u; f->t64y = (h + 63u) / 64u;
+    f->pw = (w + 3u) / 4u; f->ph = (h + 3u) / 4u;
+    f->dx = f->s.width_m / (float)w;
+    if (pigtab) memcpy(f->pigtab_h, pigtab, sizeof(f->pigtab_h));
+    else {
+        float refl[12 * 16], sc[12], ti[12], vi[12], yi[12];
+        cz_default_pigments(refl, sc, ti, vi, yi);
+        cz_pigment_table(refl, sc, ti, vi, yi, f->pigtab_h);
+    }
+    if (cmf) memcpy(f->cmf_h, cmf, sizeof(f->cmf_h));
+    else cz_cmf_table(f->cmf_h);
+    int rc = CZ_OK;
+    hipError_t e = hipGetDevice(&f->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        destroy(f);
+        return set_err(CZ_EHIP, "cz_film_create: %s", hipGetErrorString(e));
+    }
+    size_t n = f->n, nt16 = f->nt16, nt64 = (size_t)f->t64x * f->t64y, np = (size_t)f->pw * f->ph;
+#define A(p, cnt) if (!rc) rc = dalloc(f, &f->p, (cnt))
+    A(h, n); A(m, n); A(rheo, n); A(strata_h, n); A(strata_pig, 8 * n); A(strata_cure, n); A(sub_refl, 8 * n);
+    A(lay, n); A(pres, n); A(flux, n); A(acc, n); A(alpha, n); A(delta_a, n); A(delta_b, n); A(tmp, n);
+    A(acc_s, 4 * n); A(alpha_s, n); A(strata_h2, n); A(strata_pig2, 8 * n); A(strata_cure2, n); A(h_frame, n);
+    A(fnorm, n); A(cg_m, n); A(cg_d, n); A(cg_x, n); A(cg_r, n); A(cg_p, n); A(cg_q, n); A(cg_t, n); A(cg_t2, n);
+    A(cg_part, nt16); A(cg_part2, nt16); A(cg_s, 8); A(tile_flags, nt16); A(tile_list, nt16); A(tile_moved, nt16);
+    A(active_count, 4); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n); A(tsum_b, n);
+    A(pigtab, CZ_NPIG * CZ_PIG_STRIDE); A(deposits, CZ_MAX_DEPOSITS); A(tile_start, nt64); A(tile_count, nt64);
+    A(tile_items, CZ_MAX_TILE_ITEMS); A(frame_out, 16);
+#undef A
+    if (!rc) {
+        e = hipHostMalloc((void**)&f->frame_out_h, 16 * sizeof(float), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&f->dep_h, CZ_MAX_DEPOSITS * sizeof(cz_deposit), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&f->tstart_h, nt64 * 4, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&f->tcount_h, nt64 * 4, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&f->titems_h, CZ_MAX_TILE_ITEMS * 4, hipHostMallocDefault);
+        if (e != hipSuccess) rc = set_err(CZ_ENOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
+    }
+    if (!rc) {
+        e = hipMemcpyAsync(f->pigtab, f->pigtab_h, sizeof(f->pigtab_h), hipMemcpyHostToDevice, f->stream);
+        if (e != hipSuccess) rc = set_err(CZ_EHIP, "pigtab upload: %s", hipGetErrorString(e));
+    }
+    if (!rc) rc = init_state(f);
+    if (rc) {
+        destroy(f);
+        return rc;
+    }
+    f->P.size[0] = w; f->P.size[1] = h;
+    f->P.tiles[0] = f->t64x; f->P.tiles[1] = f->t64y;
+    f->P.dx = f->dx; f->P.inv_dx = 1.0f / f->dx;
+    update_params(f, 1.0f / 60.0f, 1);
+    *out = f;
+    return CZ_OK;
+}
+
+void cz_film_destroy(cz_film* f) { destroy(f); }
+
+int cz_film_dims(const cz_film* f, uint32_t* w, uint32_t* h, float* dx) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    if (w) *w = f->W;
+    if (h) *h = f->H;
+    if (dx) *dx = f->dx;
+    return CZ_OK;
+}
+
+int cz_film_set_settings(cz_film* f, const cz_settings* s) {
+    if (!f || !s) return set_err(CZ_EINVAL, "null argument");
+    if (s->slump) return set_err(CZ_EUNSUP, "the slump pass (settings.slump) is not built");
+    if (s->width_m != f->s.width_m) return set_err(CZ_EINVAL, "width_m is fixed at creation");
+    f->s = *s;
+    return CZ_OK;
+}
+
+int cz_film_get_settings(const cz_film* f, cz_settings* s) {
+    if (!f || !s) return set_err(CZ_EINVAL, "null argument");
+    *s = f->s;
+    return CZ_OK;
+}
+
+int cz_film_set_overrides(cz_film* f, const cz_overrides* o) {
+    if (!f || !o) return set_err(CZ_EINVAL, "null argument");
+    f->ov = *o;
+    f->ov_from_env = false;
+    return CZ_OK;
+}
+
+// gravity_vec (src/film/mod.rs:861-867)
+int cz_film_gravity_vec(const cz_film* f, float out[3]) {
+    if (!f || !out) return set_err(CZ_EINVAL, "null argument");
+    const cz_settings& s = f->s;
+    float tilt = czh::to_radians(s.tilt_deg);
+    float dir = czh::to_radians(s.tilt_dir_deg);
+    float gxy = s.gravity * sinf(tilt);
+    out[0] = gxy * cosf(dir);
+    out[1] = gxy * sinf(dir);
+    out[2] = fmaxf(s.gravity * cosf(tilt), 0.0f);
+    return CZ_OK;
+}
+
+int cz_film_update_params(cz_film* f, float frame_dt, uint32_t substeps) {
+    if (!f || substeps == 0 || !(frame_dt > 0.0f)) return set_err(CZ_EINVAL, "update_params: bad arguments");
+    update_params(f, frame_dt, substeps);
+    return CZ_OK;
+}
+
+// choose_substeps (src/film/mod.rs:919-933)
+uint32_t cz_film_choose_substeps(const cz_film* f, float frame_dt) {
+    const cz_settings& s = f->s;
+    float g[3];
+    cz_film_gravity_vec(f, g);
+    float gxy = sqrtf(g[0] * g[0] + g[1] * g[1]);
+    float h_max = fmaxf(fmaxf(f->h_max, f->h_max_pending), 1e-5f);
+    float mu = fmaxf(f->mu_min, 1e-3f);
+    float speed = fminf(s.rho * gxy * h_max * h_max / mu, s.v_cap);
+    float dt_cfl = (speed > 1e-6f) ? 0.5f * f->dx / speed : frame_dt;
+    float mob = h_max * h_max * h_max / (3.0f * mu);
+    uint32_t n_hyd = (uint32_t)fmaxf(ceilf(frame_dt * s.rho * g[2] * fminf(mob, s.m_cap_hydro) / (0.2f * f->dx * f->dx)), 1.0f);
+    float dx4 = czh::powi4(f->dx);
+    uint32_t n_cap = (uint32_t)fmaxf(ceilf(frame_dt * 32.0f * s.sigma * fminf(mob, s.m_cap) / dx4), 1.0f);
+    uint32_t n_cfl = (uint32_t)fmaxf(ceilf(frame_dt / dt_cfl), 1.0f);
+    uint32_t nn = std::max(std::max(n_cfl, n_hyd), n_cap);
+    return std::min(std::max(nn, s.min_substeps), s.max_substeps);
+}
+
+int cz_film_get_params(const cz_film* f, cz_params* p) {
+    if (!f || !p) return set_err(CZ_EINVAL, "null argument");
+    *p = f->P;
+    return CZ_OK;
+}
+
+int cz_film_set_params(cz_film* f, const cz_params* p) {
+    if (!f || !p) return set_err(CZ_EINVAL, "null argument");
+    if (p->size[0] != f->W || p->size[1] != f->H) return set_err(CZ_EINVAL, "set_params: size mismatch");
+    f->P = *p;
+    return CZ_OK;
+}
+
+int cz_film_upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* sent) {
+    if (!f || (n > 0 && !deps)) return set_err(CZ_EINVAL, "upload_deposits: bad arguments");
+    return upload_deposits(f, deps, n, sent);
+}
+
+int cz_film_encode_frame(cz_film* f, uint32_t substeps) {
+    if (!f || substeps == 0) return set_err(CZ_EINVAL, "encode_frame: bad arguments");
+    return encode_frame(f, substeps);
+}
+
+// encode_sim (src/main.rs:1875-1909) with stats of the previous frame.
+int cz_film_step(cz_film* f, const cz_deposit* deps, uint32_t n, float frame_dt, uint32_t* sent) {
+    if (!f || (n > 0 && !deps) || !(frame_dt > 0.0f)) return set_err(CZ_EINVAL, "step: bad arguments");
+    int rc = poll_stats(f);
+    if (rc) return rc;
+    if (f->ov_from_env) cz_overrides_from_env(&f->ov);
+    rc = upload_deposits(f, deps, n, sent);
+    if (rc) return rc;
+    uint32_t substeps = cz_film_choose_substeps(f, frame_dt);
+    update_params(f, frame_dt, substeps);
+    return encode_frame(f, substeps);
+}
+
+int cz_film_sync(cz_film* f) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    return poll_stats(f) ? CZ_EHIP : wait_idle(f);
+}
+
+int cz_film_stats(cz_film* f, cz_stats* st) {
+    if (!f || !st) return set_err(CZ_EINVAL, "null argument");
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    const float* o = f->frame_out_h;
+    st->h_max = f->stats_pending ? o[0] : f->h_max;
+    st->mu_min = f->stats_pending ? ((o[1] > 1e8f) ? 1.0f : o[1]) : f->mu_min;
+    st->active_tiles = f->stats_pending ? cz_f2bits(o[2]) : f->active_tiles;
+    st->substeps_last = f->substeps_last;
+    st->implicit_steps = f->implicit_steps;
+    st->frame = f->frame;
+    for (int k = 0; k < 4; k++) st->cg_report[k] = f->stats_pending ? o[4 + k] : f->cg_report[k];
+    st->total_deposited = f->total_deposited;
+    st->weave_volume = f->weave_volume;
+    return CZ_OK;
+}
+
+int cz_film_pool(cz_film* f, float* out, uint64_t n_floats) {
+    if (!f || !out) return set_err(CZ_EINVAL, "null argument");
+    uint64_t need = 2ull * f->pw * f->ph;
+    if (n_floats < need) return set_err(CZ_ESIZE, "pool: need %llu floats", (unsigned long long)need);
+    HIP_TRY(hipMemcpyAsync(out, f->pool_out, need * 4, hipMemcpyDeviceToHost, f->stream));
+    return wait_idle(f);
+}
+
+// compute_stats (src/main.rs:1679-1819): host fold in FP64 of h and m.
+int cz_film_mass_report(cz_film* f, cz_mass_report* r) {
+    if (!f || !r) return set_err(CZ_EINVAL, "null argument");
+    std::vector<float> hv(f->n), mv(f->n);
+    HIP_TRY(hipMemcpyAsync(hv.data(), f->h, (size_t)f->n * 4, hipMemcpyDeviceToHost, f->stream));
+    HIP_TRY(hipMemcpyAsync(mv.data(), f->m, (size_t)f->n * 4, hipMemcpyDeviceToHost, f->stream));
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    double dx = (double)f->dx, sum = 0.0, summ = 0.0;
+    float hmax = 0.0f;
+    uint64_t wet = 0, nan = 0;
+    for (uint32_t c = 0; c < f->n; c++) {
+        float a = hv[c], b = mv[c];
+        if (!std::isfinite(a)) { nan++; continue; }
+        sum += (double)a;
+        summ += (double)b;
+        if (a > 0.0f) wet++;
+        hmax = fmaxf(hmax, a);
+    }
+    double vol = sum * dx * dx, volm = summ * dx * dx - f->weave_volume;
+    r->wet_ml = vol * 1e6;
+    r->cured_ml = volm * 1e6;
+    r->deposited_ml = f->total_deposited * 1e6;
+    r->err_pct = f->total_deposited > 0.0 ? (vol + volm - f->total_deposited) / f->total_deposited * 100.0 : 0.0;
+    r->hmax = hmax;
+    r->pad = 0;
+    r->wet_cells = wet;
+    r->nan_cells = nan;
+    return CZ_OK;
+}
+
+int cz_film_state_sizes(const cz_film* f, uint64_t sizes[CZ_NPARTS]) {
+    if (!f || !sizes) return set_err(CZ_EINVAL, "null argument");
+    for (int k = 0; k < CZ_NPARTS; k++) sizes[k] = buffer_of(f, k).bytes;
+    return CZ_OK;
+}
+
+int cz_film_read_state(cz_film* f, void* const parts[CZ_NPARTS]) {
+    if (!f || !parts) return set_err(CZ_EINVAL, "null argument");
+    for (int k = 0; k < CZ_NPARTS; k++) {
+        BufDesc b = buffer_of(f, k);
+        if (!parts[k]) return set_err(CZ_EINVAL, "read_state: part %d is null", k);
+        HIP_TRY(hipMemcpyAsync(parts[k], b.ptr, b.bytes, hipMemcpyDeviceToHost, f->stream));
+    }
+    return wait_idle(f);
+}
+
+// write_state (src/film/mod.rs:1303-1320)
+int cz_film_write_state(cz_film* f, const void* const parts[CZ_NPARTS], const uint64_t sizes[CZ_NPARTS]) {
+    if (!f || !parts || !sizes) return set_err(CZ_EINVAL, "null argument");
+    for (int k = 0; k < CZ_NPARTS; k++) {
+        if (!parts[k] || sizes[k] != buffer_of(f, k).bytes)
+            return set_err(CZ_ESIZE, "write_state: part %d has %llu bytes, expected %llu", k,
+                           (unsigned long long)sizes[k], (unsigned long long)buffer_of(f, k).bytes);
+    }
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    for (int k = 0; k < CZ_NPARTS; k++) {
+        BufDesc b = buffer_of(f, k);
+        HIP_TRY(hipMemcpyAsync(b.ptr, parts[k], b.bytes, hipMemcpyHostToDevice, f->stream));
+    }
+    uint64_t n = f->n;
+    HIP_TRY(hipMemsetAsync(f->pres, 0, 8 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->flux, 0, 16 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->acc, 0, 16 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->alpha, 0, 4 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->delta_a, 0, 4 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->delta_b, 0, 4 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->tmp, 0, 4 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->acc_s, 0, 64 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->strata_h2, 0, 16 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->strata_pig2, 0, 128 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->alpha_s, 0, 16 * n, f->stream));
+    std::vector<uint32_t> ones(f->nt16, 1u);
+    HIP_TRY(hipMemcpyAsync(f->tile_moved, ones.data(), ones.size() * 4, hipMemcpyHostToDevice, f->stream));
+    return wait_idle(f);
+}
+
+int cz_film_buffer_size(const cz_film* f, int which, uint64_t* bytes) {
+    if (!f || !bytes) return set_err(CZ_EINVAL, "null argument");
+    BufDesc b = buffer_of(f, which);
+    if (!b.ptr) return set_err(CZ_EINVAL, "unknown buffer %d", which);
+    *bytes = b.bytes;
+    return CZ_OK;
+}
+
+int cz_film_read_buffer(cz_film* f, int which, void* host, uint64_t bytes) {
+    if (!f || !host) return set_err(CZ_EINVAL, "null argument");
+    BufDesc b = buffer_of(f, which);
+    if (!b.ptr) return set_err(CZ_EINVAL, "unknown buffer %d", which);
+    if (bytes != b.bytes) return set_err(CZ_ESIZE, "buffer %d has %llu bytes", which, (unsigned long long)b.bytes);
+    HIP_TRY(hipMemcpyAsync(host, b.ptr, bytes, hipMemcpyDeviceToHost, f->stream));
+    return wait_idle(f);
+}
+
+int cz_film_write_buffer(cz_film* f, int which, const void* host, uint64_t bytes) {
+    if (!f || !host) return set_err(CZ_EINVAL, "null argument");
+    BufDesc b = buffer_of(f, which);
+    if (!b.ptr) return set_err(CZ_EINVAL, "unknown buffer %d", which);
+    if (bytes != b.bytes) return set_err(CZ_ESIZE, "buffer %d has %llu bytes", which, (unsigned long long)b.bytes);
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(b.ptr, host, bytes, hipMemcpyHostToDevice, f->stream));
+    if (which == CZ_BUF_DELTA) HIP_TRY(hipMemcpyAsync(f->delta_other(), host, bytes, hipMemcpyHostToDevice, f->stream));
+    return wait_idle(f);
+}
+
+int cz_film_device_ptr(cz_film* f, int which, void** ptr, uint64_t* bytes) {
+    if (!f || !ptr) return set_err(CZ_EINVAL, "null argument");
+    BufDesc b = buffer_of(f, which);
+    if (!b.ptr) return set_err(CZ_EINVAL, "unknown buffer %d", which);
+    *ptr = b.ptr;
+    if (bytes) *bytes = b.bytes;
+    return CZ_OK;
+}
+
+// dry_now (src/film/mod.rs:1230-1244): one cure pass at 4 / frame_dt.
+int cz_film_dry_now(cz_film* f) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    cz_params saved = f->P;
+    f->P.cure[0] = 4.0f / fmaxf(f->P.frame_dt, 1e-4f);
+    FilmDev F = f->dev();
+    const dim3 gcell((f->W + 15u) / 16u, (f->H + 15u) / 16u);
+    hipLaunchKernelGGL(k_cure_clear, gcell, dim3(256), 0, f->stream, F, 0);
+    HIP_TRY(hipGetLastError());
+    f->P = saved;
+    return wait_idle(f);
+}
+
+// clear (src/film/mod.rs:1009-1028)
+int cz_film_clear(cz_film* f) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    uint64_t n = f->n;
+    struct Z { void* p; uint64_t b; } zs[] = {
+        {f->h, 4 * n}, {f->m, 4 * n}, {f->pres, 8 * n}, {f->flux, 16 * n}, {f->acc, 16 * n}, {f->alpha, 4 * n},
+        {f->delta_a, 4 * n}, {f->delta_b, 4 * n}, {f->tmp, 4 * n}, {f->acc_s, 64 * n}, {f->strata_h, 16 * n},
+        {f->strata_pig, 128 * n}, {f->strata_cure, 16 * n}, {f->strata_h2, 16 * n}, {f->strata_pig2, 128 * n},
+        {f->alpha_s, 16 * n}, {f->tile_flags, 4 * (uint64_t)f->nt16}, {f->lay, 4 * n}};
+    for (auto& z : zs) HIP_TRY(hipMemsetAsync(z.p, 0, z.b, f->stream));
+    rc = init_state(f);
+    if (rc) return rc;
+    f->total_deposited = 0.0;
+    f->h_max = 0.0f;
+    f->h_max_pending = 0.0f;
+    return CZ_OK;
+}
+
+void* cz_film_stream(cz_film* f) { return f ? (void*)f->stream : nullptr; }
+
+int cz_film_use_graphs(cz_film* f, int on) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    if (on) return set_err(CZ_EUNSUP, "graph replay is not built in this version");
+    return CZ_OK;
+}
+
+int cz_film_profile(cz_film* f, int on) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    f->prof_on = on != 0;
+    f->prof_acc.clear();
+    return CZ_OK;
+}
+
+int cz_film_profile_read(cz_film* f, const char* label, double* avg_us, uint64_t* launches) {
+    if (!f || !label) return set_err(CZ_EINVAL, "null argument");
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    auto it = f->prof_acc.find(label);
+    if (it == f->prof_acc.end() || it->second.second == 0) {
+        if (avg_us) *avg_us = 0.0;
+        if (launches) *launches = 0;
+        return CZ_OK;
+    }
+    if (avg_us) *avg_us = it->second.first / (double)it->second.second;
+    if (launches) *launches = it->second.second;
+    return CZ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1226 @@
+// film_kernels.hip - gfx950 kernels of the thin-film paint step.
+//
+// One kernel per reference dispatch (or a fused group), cited per kernel.
+// Kernels over the active tiles walk the ordered tile list with a grid-stride
+// loop: a block of 256 threads (4 wave64) takes one 16x16 tile per
+// iteration, so the launch size never depends on the device-side count and
+// no host round trip is needed inside a frame.
+//
+// Where the reference reads, inside one dispatch, a neighbour value that the
+// same dispatch writes, these kernels read the value from before the
+// dispatch: nose writes a second offset buffer, the strata fill passes carry
+// the stack sums in ping-pong buffers, and cg_finish leaves the new height in
+// a scratch buffer until the rescale pass.
+#include "film_kernels.h"
+
+namespace cz {
+
+#define CZ_FOR_TILES(F)                                                              \
+    const uint32_t _nact = *(F).active_count;                                        \
+    const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;                  \
+    for (uint32_t _a = blockIdx.x; _a < _nact; _a += gridDim.x)
+
+#define CZ_TILE_CELL(F)                                                              \
+    const uint32_t _t = (F).tile_list[_a];                                           \
+    const uint32_t x = (_t % (F).tw) * 16u + _lx, y = (_t / (F).tw) * 16u + _ly;     \
+    const bool in_grid = x < (F).W && y < (F).H;                                     \
+    const uint32_t c = in_grid ? idx((F), x, y) : 0u;
+
+// ---------------------------------------------------------------------------
+// deposit.wgsl:6-29
+__device__ inline float ribbon_height(float px, float py, const cz_deposit& d) {
+    float abx = d.p1[0] - d.p0[0], aby = d.p1[1] - d.p0[1];
+    float l2 = abx * abx + aby * aby;
+    if (l2 <= 0.0f) {
+        float xx = cz_len2(px - d.p0[0], py - d.p0[1]) / d.r0;
+        return (xx < 1.0f) ? d.h0 * (1.0f - xx * xx) : 0.0f;
+    }
+    float l = sqrtf(l2);
+    float dirx = abx / l, diry = aby / l;
+    float q0x = px - d.p0[0], q0y = py - d.p0[1];
+    float q1x = px - d.p1[0], q1y = py - d.p1[1];
+    bool cap_start = (d.kind & CZ_DEPOSIT_CAP_START) != 0u;
+    bool cap_end = (d.kind & CZ_DEPOSIT_CAP_END) != 0u;
+    if (!cap_start && q0x * d.n0[0] + q0y * d.n0[1] < 0.0f) return 0.0f;
+    if (!cap_end && q1x * d.n1[0] + q1y * d.n1[1] >= 0.0f) return 0.0f;
+    float u = q0x * dirx + q0y * diry;
+    if (u < -d.r0 || u > l + d.r1) return 0.0f;
+    float dist = cz_abs(q0x * diry - q0y * dirx);
+    if (cap_start && u < 0.0f) dist = cz_len2(q0x, q0y);
+    if (cap_end && u > l) dist = cz_len2(q1x, q1y);
+    float rt = cz_mix(d.r0, d.r1, cz_clamp(u / l, 0.0f, 1.0f));
+    float xx = dist / rt;
+    return (xx < 1.0f) ? d.h0 * (rt / d.r0) * (1.0f - xx * xx) : 0.0f;
+}
+
+// deposit.wgsl:31-75 (full grid, 16x16 blocks)
+__global__ __launch_bounds__(256) void k_deposit(FilmDev F) {
+    uint32_t i = blockIdx.x * 16u + (threadIdx.x & 15u);
+    uint32_t j = blockIdx.y * 16u + (threadIdx.x >> 4);
+    if (i >= F.W || j >= F.H) return;
+    uint32_t tile = (i / TILE64) + (j / TILE64) * F.P.tiles[0];
+    uint32_t n = F.tile_count[tile];
+    if (n == 0u) return;
+    uint32_t start = F.tile_start[tile];
+    uint32_t c = idx(F, i, j);
+    float px = ((float)i + 0.5f) * F.P.dx, py = ((float)j + 0.5f) * F.P.dx;
+    Stack s;
+    stack_load(F, c, s);
+    float added = 0.0f, top_add = 0.0f;
+    uint32_t top_lay = 0;
+    for (uint32_t q = 0; q < n; q++) {
+        const cz_deposit d = F.deposits[F.tile_items[start + q]];
+        float abx = d.p1[0] - d.p0[0], aby = d.p1[1] - d.p0[1];
+        float along = cz_clamp(((px - d.p0[0]) * abx + (py - d.p0[1]) * aby) / cz_max(abx * abx + aby * aby, 1e-30f), 0.0f, 1.0f);
+        float reach = 1.415f * cz_max(d.r0, d.r1) + 0.6f * F.P.dx;
+        float offx = px - d.p0[0] - along * abx, offy = py - d.p0[1] - along * aby;
+        if (offx * offx + offy * offy > reach * reach) continue;
+        float add = 0.0f;
+        for (uint32_t sy = 0; sy < 4u; sy++) {
+            for (uint32_t sx = 0; sx < 4u; sx++) {
+                float psx = px + ((float)sx - 1.5f) * (0.25f * F.P.dx);
+                float psy = py + ((float)sy - 1.5f) * (0.25f * F.P.dx);
+                add = add + 0.0625f * ribbon_height(psx, psy, d);
+            }
+        }
+        if (add > 0.0f) {
+            float pig[16];
+            unpack8(make_uint4(d.pig[0], d.pig[1], d.pig[2], d.pig[3]), make_uint4(d.pig[4], d.pig[5], d.pig[6], d.pig[7]), pig);
+            stack_push_top(F, s, add, pig, 0.0f);
+            added = added + add;
+            if (add > top_add) {
+                top_add = add;
+                float vx = d.p1[0] - d.p0[0] + 1e-9f, vy = d.p1[1] - d.p0[1] + 0.0f;
+                float vl = cz_len2(vx, vy);
+                float turn = cz_atan2f(vy / vl, vx / vl) / 6.2831853f + 0.5f;
+                uint32_t method = d.kind >> CZ_DEPOSIT_METHOD_SHIFT;
+                top_lay = (method & 0xffu) | ((uint32_t)(cz_clamp(turn, 0.0f, 1.0f) * 65535.0f) << 8u);
+            }
+        }
+    }
+    if (added > 0.0f) {
+        F.h[c] = F.h[c] + added;
+        F.lay[c] = top_lay;
+        F.tile_moved[(i >> 4) + (j >> 4) * F.tw] = 1u;
+        stack_compact(F, s);
+        stack_store_to(F.strata_h, F.strata_cure, F.strata_pig, c, s);
+    }
+}
+
+// film.wgsl:1365-1391 (one block per 16x16 tile)
+__global__ __launch_bounds__(256) void k_mark_tiles(FilmDev F) {
+    __shared__ uint32_t any;
+    if (threadIdx.x == 0) any = 0u;
+    __syncthreads();
+    uint32_t t = blockIdx.x;
+    uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+    uint32_t i = (t % F.tw) * 16u + lx, j = (t / F.tw) * 16u + ly;
+    if (i < F.W && j < F.H) {
+        uint32_t c = idx(F, i, j);
+        float hc = F.h[c];
+        if (hc > 0.0f) {
+            uint32_t bits = 1u;
+            if (lx < WAKE_BAND) bits |= 2u;
+            if (lx >= 16u - WAKE_BAND) bits |= 4u;
+            if (ly < WAKE_BAND) bits |= 8u;
+            if (ly >= 16u - WAKE_BAND) bits |= 16u;
+            if (cz_abs(hc - F.h_frame[c]) > FRAME_EPS) bits |= MOVED;
+            atomicOr(&any, bits);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t flags = any;
+        if (F.tile_moved[t] != 0u) flags |= MOVED;
+        F.tile_moved[t] = 0u;
+        F.tile_flags[t] = flags;
+    }
+}
+
+// film.wgsl:1393-1425 as an ordered compaction (one block of 1024 threads):
+// the list is ascending in tile index instead of atomic append order.
+__global__ __launch_bounds__(1024) void k_compact_tiles(FilmDev F) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t base;
+    if (threadIdx.x == 0) base = 0u;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t chunk = 0; chunk < F.nt16; chunk += 1024u) {
+        uint32_t t = chunk + threadIdx.x;
+        uint32_t active = 0u, fl_t = 0u;
+        if (t < F.nt16) {
+            int tx = (int)(t % F.tw), ty = (int)(t / F.tw);
+            fl_t = F.tile_flags[t];
+            uint32_t anyv = fl_t & MOVED;
+            for (int dy = -1; dy <= 1; dy++) {
+                for (int dxi = -1; dxi <= 1; dxi++) {
+                    if (dxi == 0 && dy == 0) continue;
+                    int xx = tx + dxi, yy = ty + dy;
+                    if (xx >= 0 && yy >= 0 && xx < (int)F.tw && yy < (int)F.th) {
+                        uint32_t fl = F.tile_flags[(uint32_t)xx + (uint32_t)yy * F.tw];
+                        bool need = true;
+                        if (dxi == 1 && (fl & 2u) == 0u) need = false;
+                        if (dxi == -1 && (fl & 4u) == 0u) need = false;
+                        if (dy == 1 && (fl & 8u) == 0u) need = false;
+                        if (dy == -1 && (fl & 16u) == 0u) need = false;
+                        if (need) anyv |= fl & MOVED;
+                    }
+                }
+            }
+            active = anyv != 0u ? 1u : 0u;
+        }
+        // block-wide exclusive scan of `active`
+        unsigned long long bal = __ballot(active);
+        uint32_t below = (uint32_t)__popcll(bal & ((lane == 0u) ? 0ull : (~0ull >> (64u - lane))));
+        if (lane == 0u) wsum[wave] = (uint32_t)__popcll(bal);
+        __syncthreads();
+        uint32_t woff = 0u, total = 0u;
+        for (uint32_t w = 0; w < 16u; w++) {
+            if (w < wave) woff += wsum[w];
+            total += wsum[w];
+        }
+        if (t < F.nt16) {
+            if (active) {
+                F.tile_list[base + woff + below] = t;
+                F.tile_flags[t] = fl_t | 32u;
+            } else {
+                F.tile_flags[t] = fl_t & 31u;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) base += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *F.active_count = base;
+}
+
+// ---------------------------------------------------------------------------
+// cure.wgsl:3-48,62-123
+
+__device__ inline void composition_rheo(const FilmDev& F, const Stack& s, float& mu_o, float& ty_o) {
+    float total = 0.0f, lnmu = 0.0f, tau = 0.0f, thin = 0.0f, wsum = 0.0f, cure = 0.0f;
+    for (uint32_t k = 0; k < 4u; k++) {
+        float t = s.th[k];
+        if (t <= 0.0f) continue;
+        total = total + t;
+        cure = cure + t * s.cu[k];
+        float csum = 0.0f, lm = 0.0f, ty = 0.0f;
+        for (uint32_t i = 0; i < (uint32_t)NPIG; i++) {
+            float cc = s.pg[k][i];
+            if (cc > 0.0f) {
+                csum = csum + cc;
+                lm = lm + cc * cz_logf(cz_max(F.pigtab[i * PIG_STRIDE + 32u], 1e-3f));
+                ty = ty + cc * s.pg[k][13];
+            }
+        }
+        if (s.pg[k][14] > 0.0f) lm = csum * (s.pg[k][14] - 10.0f);
+        if (csum > 0.0f) {
+            lnmu = lnmu + t * lm / csum;
+            tau = tau + t * ty / csum;
+            wsum = wsum + t;
+        }
+        thin = thin + t * s.pg[k][12];
+    }
+    if (total <= 0.0f) { mu_o = 1.0f; ty_o = 0.0f; return; }
+    float mu = cz_logf(F.P.misc[2]);
+    float ty = 0.0f;
+    if (wsum > 0.0f) { mu = lnmu / wsum; ty = tau / wsum; }
+    float tf = cz_clamp(thin / total, 0.0f, 0.95f);
+    float cb = cure / total;
+    mu = mu + (3.0f * cz_logf(1.0f - tf) + F.P.cure[2] * cb * cb);
+    ty = ty * cz_powf(1.0f - tf, 4.0f) + F.P.cure[1] * cb * cb;
+    mu_o = cz_expf(mu);
+    ty_o = ty;
+}
+
+__device__ inline float2 packed_rheo(const FilmDev& F, uint32_t c, const float* t, const float* u) {
+    float total = 0.0f, lnmu = 0.0f, tau = 0.0f, thin = 0.0f, wsum = 0.0f, cure = 0.0f;
+    for (uint32_t k = 0; k < 4u; k++) {
+        float tk = t[k];
+        if (tk <= 0.0f) continue;
+        total = total + tk;
+        cure = cure + tk * u[k];
+        uint4 a = F.strata_pig[(c * 4u + k) * 2u];
+        uint4 b = F.strata_pig[(c * 4u + k) * 2u + 1u];
+        float pc[12] = {cz_unpack_lo(a.x), cz_unpack_hi(a.x), cz_unpack_lo(a.y), cz_unpack_hi(a.y),
+                        cz_unpack_lo(a.z), cz_unpack_hi(a.z), cz_unpack_lo(a.w), cz_unpack_hi(a.w),
+                        cz_unpack_lo(b.x), cz_unpack_hi(b.x), cz_unpack_lo(b.y), cz_unpack_hi(b.y)};
+        float exx = cz_unpack_lo(b.z), exy = cz_unpack_hi(b.z);
+        float mu14 = cz_unpack_lo(b.w);
+        float csum = 0.0f, ty = 0.0f;
+        for (uint32_t i = 0; i < (uint32_t)NPIG; i++) {
+            float cc = pc[i];
+            if (cc > 0.0f) {
+                csum = csum + cc;
+                ty = ty + cc * exy;
+            }
+        }
+        if (csum > 0.0f) {
+            if (mu14 > 0.0f) {
+                lnmu = lnmu + tk * (csum * (mu14 - 10.0f)) / csum;
+            } else {
+                // layer_lnmu (cure.wgsl:125-135)
+                float lm = 0.0f;
+                for (uint32_t i = 0; i < (uint32_t)NPIG; i++) {
+                    float cc = pc[i];
+                    if (cc > 0.0f) lm = lm + cc * cz_logf(cz_max(F.pigtab[i * PIG_STRIDE + 32u], 1e-3f));
+                }
+                lnmu = lnmu + tk * lm / csum;
+            }
+            tau = tau + tk * ty / csum;
+            wsum = wsum + tk;
+        }
+        thin = thin + tk * exx;
+    }
+    if (total <= 0.0f) return make_float2(1.0f, 0.0f);
+    float mu = cz_logf(F.P.misc[2]);
+    float ty = 0.0f;
+    if (wsum > 0.0f) { mu = lnmu / wsum; ty = tau / wsum; }
+    float tf = cz_clamp(thin / total, 0.0f, 0.95f);
+    float cb = cure / total;
+    mu = mu + (3.0f * cz_logf(1.0f - tf) + F.P.cure[2] * cb * cb);
+    ty = ty * cz_powf(1.0f - tf, 4.0f) + F.P.cure[1] * cb * cb;
+    return make_float2(cz_expf(mu), ty);
+}
+
+// cure_step (cure.wgsl:119-159) over the full grid, fused with clear_acc
+// (film.wgsl:1427-1434) for the cells of active tiles when do_clear.
+__global__ __launch_bounds__(256) void k_cure_clear(FilmDev F, int do_clear) {
+    uint32_t i = blockIdx.x * 16u + (threadIdx.x & 15u);
+    uint32_t j = blockIdx.y * 16u + (threadIdx.x >> 4);
+    if (i >= F.W || j >= F.H) return;
+    uint32_t c = idx(F, i, j);
+    float hc = F.h[c];
+    if (hc <= 0.0f) {
+        F.rheo[c] = make_float2(1.0f, 0.0f);
+    } else {
+        float rate = F.P.cure[0] * F.P.frame_dt;
+        float4 t4 = F.strata_h[c];
+        float4 cu4 = F.strata_cure[c];
+        float t[4] = {t4.x, t4.y, t4.z, t4.w};
+        float cu[4] = {cu4.x, cu4.y, cu4.z, cu4.w};
+        float u[4];
+        for (int k = 0; k < 4; k++) u[k] = (t[k] > 0.0f) ? cz_min(cu[k] + rate, 1.0f) : cu[k];
+        if (!(t[0] > 0.0f && u[0] >= 1.0f)) {
+            F.strata_cure[c] = make_float4(u[0], u[1], u[2], u[3]);
+            F.rheo[c] = packed_rheo(F, c, t, u);
+        } else {
+            Stack s;
+            stack_load(F, c, s);
+            for (int k = 0; k < 4; k++) s.cu[k] = u[k];
+            float committed = 0.0f;
+            uint32_t guard = 0;
+            float mc = F.m[c];
+            while (s.th[0] > 0.0f && s.cu[0] >= 1.0f && guard < 4u) {
+                guard++;
+                float r[16];
+                for (int b = 0; b < 8; b++) {
+                    uint32_t v = F.sub_refl[c * 8u + b];
+                    r[2 * b] = cz_unpack_lo(v);
+                    r[2 * b + 1] = cz_unpack_hi(v);
+                }
+                for (uint32_t b = 0; b < (uint32_t)NBAND; b++) {
+                    float kk, ss, R, T;
+                    layer_ks(F.pigtab, s.pg[0], b, kk, ss);
+                    km_layer(kk, ss, s.th[0], R, T);
+                    r[b] = composite(R, T, r[b]);
+                }
+                for (int b = 0; b < 8; b++) F.sub_refl[c * 8u + b] = cz_pack2(r[2 * b], r[2 * b + 1]);
+                committed = committed + s.th[0];
+                mc = mc + s.th[0] * (1.0f - OIL_SHRINK) * (1.0f - cz_clamp(s.pg[0][12], 0.0f, 0.95f));
+                s.th[0] = 0.0f;
+                s.cu[0] = 0.0f;
+                stack_compact(F, s);
+            }
+            F.m[c] = mc;
+            if (committed > 0.0f) {
+                hc = cz_max(hc - committed, 0.0f);
+                F.h[c] = hc;
+            }
+            float mu, ty;
+            composition_rheo(F, s, mu, ty);
+            F.rheo[c] = make_float2(mu, ty);
+            stack_store_to(F.strata_h, F.strata_cure, F.strata_pig, c, s);
+        }
+    }
+    if (do_clear && cell_active(F, i, j)) {
+        F.acc[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        F.h_frame[c] = hc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// laplacian (film.wgsl:16-42) and, when do_plug, plug (film.wgsl:58-87)
+__global__ __launch_bounds__(256) void k_lap_plug(FilmDev F, int do_plug) {
+    const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        int i = (int)x, j = (int)y;
+        float hv = F.h[c];
+        float s0 = hv + F.m[c];
+        float hold = 0.0f, pl = 0.0f;
+        if (hv > 0.0f) hold = hold_level(F, nose_dist(F, c));
+        float u0 = s0 - hold;
+        float ax = usurf(F, i - 1, j) + usurf(F, i + 1, j) + usurf(F, i, j - 1) + usurf(F, i, j + 1);
+        float dg = usurf(F, i - 1, j - 1) + usurf(F, i + 1, j - 1) + usurf(F, i - 1, j + 1) + usurf(F, i + 1, j + 1);
+        float l = (W_AXIS * ax + W_DIAG_LAP * dg - (4.0f * W_AXIS + 4.0f * W_DIAG_LAP) * u0) * F.P.inv_dx * F.P.inv_dx;
+        if (hv > 0.0f) {
+            float d = nose_dist(F, c);
+            if (d <= 3.0f) {
+                float2 nn = F.fnorm[c];
+                if (nn.x * nn.x + nn.y * nn.y > 0.5f) pl = line_pressure(F, x, y) * cz_expf(-(d - 1.0f) * 0.7f);
+            }
+        }
+        F.pres[c] = make_float2(-F.P.phys[1] * l + disjoining(F, hv), F.P.phys[0] * F.P.g[2] * (s0 - hold) + pl);
+        if (!do_plug) continue;
+        float tv = 0.0f;
+        if (hv > 0.0f && !(gb < 1e-6f)) {
+            float sum = 0.0f, mass = 0.0f;
+            bool yielding = false;
+            for (int dy = -PLUG_RADIUS; dy <= PLUG_RADIUS; dy++) {
+                int yy = j + dy;
+                if (yy < 0 || yy >= (int)F.H) continue;
+                for (int dxi = -PLUG_RADIUS; dxi <= PLUG_RADIUS; dxi++) {
+                    int xx = i + dxi;
+                    if (xx < 0 || xx >= (int)F.W) continue;
+                    uint32_t nn = idx(F, (uint32_t)xx, (uint32_t)yy);
+                    float hn = F.h[nn];
+                    if (hn <= 0.0f) continue;
+                    float2 r = F.rheo[nn];
+                    if (r.y > 0.0f) yielding = true;
+                    float oy, oup;
+                    float q = hb_flux(F.P.phys[2], gb, hn, cz_max(r.x, 1e-4f), r.y, oy, oup);
+                    sum = sum + q;
+                    mass = mass + hn;
+                }
+            }
+            if (yielding && mass > 0.0f) tv = sum / mass;
+        }
+        F.tmp[c] = tv;
+    }
+}
+
+// fluxes (film.wgsl:375-455)
+__global__ __launch_bounds__(256) void k_fluxes(FilmDev F) {
+    const float wts[4] = {W_AXIS, W_AXIS, W_DIAG_FLUX, W_DIAG_FLUX};
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        float2 p00 = pressure_at(F, (int)x, (int)y);
+        float q[4];
+        for (int k = 0; k < 4; k++) {
+            q[k] = 0.0f;
+            FaceGeom fg = face_geom(F, x, y, k, true, p00);
+            if (!fg.has) continue;
+            FaceFlux ff = face(F, c, fg.cb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+            q[k] = ff.q * wts[k];
+        }
+        F.flux[c] = make_float4(q[0], q[1], q[2], q[3]);
+    }
+}
+
+__device__ __forceinline__ bool nb_ok(const FilmDev& F, int i, int j) {
+    if (i < 0 || j < 0 || i >= (int)F.W || j >= (int)F.H) return false;
+    return cell_active(F, (uint32_t)i, (uint32_t)j);
+}
+
+// limiter (film.wgsl:521-537)
+__global__ __launch_bounds__(256) void k_limiter(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        int i = (int)x, j = (int)y;
+        float4 f = F.flux[c];
+        float out = cz_max(f.x, 0.0f) + cz_max(f.y, 0.0f) + cz_max(f.z, 0.0f) + cz_max(f.w, 0.0f);
+        if (nb_ok(F, i - 1, j)) out = out + cz_max(-F.flux[c - 1u].x, 0.0f);
+        if (nb_ok(F, i, j - 1)) out = out + cz_max(-F.flux[c - F.W].y, 0.0f);
+        if (nb_ok(F, i - 1, j - 1)) out = out + cz_max(-F.flux[c - F.W - 1u].z, 0.0f);
+        if (nb_ok(F, i - 1, j + 1)) out = out + cz_max(-F.flux[c + F.W - 1u].w, 0.0f);
+        float a = 1.0f;
+        if (out > 0.0f) a = cz_min(1.0f, 0.5f * F.h[c] * F.P.dx / (F.P.dt * out));
+        F.alpha[c] = a;
+    }
+}
+
+struct Inflow { float best; uint32_t src; float div; float accv; };
+
+__device__ __forceinline__ void own_face(const FilmDev& F, float q, uint32_t cb, bool has, float ac, Inflow& st) {
+    float ab = ac;
+    if (q < 0.0f && has) {
+        ab = F.alpha[cb];
+        if (-q * ab > st.best) { st.best = -q * ab; st.src = cb; }
+    }
+    st.div = st.div + q * ab;
+    st.accv = q * ab;
+}
+__device__ __forceinline__ void nb_face(const FilmDev& F, float q, uint32_t cn, float ac, Inflow& st) {
+    float an = F.alpha[cn];
+    if (q < 0.0f) an = ac;
+    if (q * an > st.best) { st.best = q * an; st.src = cn; }
+    st.div = st.div - q * an;
+}
+
+// Copy of the top stratum (and rheology) of src into the newly wetted cell c
+// (film.wgsl:847-860, strata.wgsl:312-322).
+__device__ inline void inherit_from(const FilmDev& F, uint32_t c, uint32_t src, float hn, bool copy_rheo) {
+    if (copy_rheo) F.rheo[c] = F.rheo[src];
+    float4 ts = F.strata_h[src];
+    uint32_t ks = 0;
+    if (ts.x > 0.0f) ks = 0;
+    if (ts.y > 0.0f) ks = 1;
+    if (ts.z > 0.0f) ks = 2;
+    if (ts.w > 0.0f) ks = 3;
+    float4 cs4 = F.strata_cure[src];
+    float cs = ks == 0 ? cs4.x : (ks == 1 ? cs4.y : (ks == 2 ? cs4.z : cs4.w));
+    F.strata_h[c] = make_float4(hn, 0.0f, 0.0f, 0.0f);
+    uint4 a = F.strata_pig[(src * 4u + ks) * 2u];
+    uint4 b = F.strata_pig[(src * 4u + ks) * 2u + 1u];
+    F.strata_pig[c * 8u] = a;
+    F.strata_pig[c * 8u + 1u] = b;
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t k = 2u; k < 8u; k++) F.strata_pig[c * 8u + k] = z;
+    F.strata_cure[c] = make_float4(cs, 0.0f, 0.0f, 0.0f);
+}
+
+// apply (film.wgsl:803-863)
+__global__ __launch_bounds__(256) void k_apply(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        bool moved = false;
+        if (in_grid) {
+            int i = (int)x, j = (int)y;
+            float ac = F.alpha[c];
+            Inflow st = {0.0f, c, 0.0f, 0.0f};
+            float4 f = F.flux[c];
+            bool hx = x + 1u < F.W, hy = y + 1u < F.H, hm = y > 0u;
+            uint32_t xp = (x + 1u < F.W - 1u) ? x + 1u : F.W - 1u;
+            uint32_t yp = (y + 1u < F.H - 1u) ? y + 1u : F.H - 1u;
+            float accv[4];
+            own_face(F, f.x, hx ? idx(F, xp, y) : c, hx, ac, st);
+            accv[0] = st.accv;
+            own_face(F, f.y, hy ? idx(F, x, yp) : c, hy, ac, st);
+            accv[1] = st.accv;
+            own_face(F, f.z, (hx && hy) ? idx(F, xp, yp) : c, hx && hy, ac, st);
+            accv[2] = st.accv;
+            own_face(F, f.w, (hx && hm) ? idx(F, xp, hm ? y - 1u : 0u) : c, hx && hm, ac, st);
+            accv[3] = st.accv;
+            if (nb_ok(F, i - 1, j)) { uint32_t cn = c - 1u; nb_face(F, F.flux[cn].x, cn, ac, st); }
+            if (nb_ok(F, i, j - 1)) { uint32_t cn = c - F.W; nb_face(F, F.flux[cn].y, cn, ac, st); }
+            if (nb_ok(F, i - 1, j - 1)) { uint32_t cn = c - F.W - 1u; nb_face(F, F.flux[cn].z, cn, ac, st); }
+            if (nb_ok(F, i - 1, j + 1)) { uint32_t cn = c + F.W - 1u; nb_face(F, F.flux[cn].w, cn, ac, st); }
+            uint32_t src = st.src;
+            float h0 = F.h[c];
+            float hn = cz_max(h0 - F.P.dt * st.div * F.P.inv_dx, 0.0f);
+            if (hn < h0 && hn < DRAINED_FILM) hn = 0.0f;
+            F.h[c] = hn;
+            moved = cz_abs(hn - h0) > MOVE_EPS;
+            if (h0 <= 0.0f && src != c) inherit_from(F, c, src, hn, true);
+            float4 a = F.acc[c];
+            F.acc[c] = make_float4(a.x + accv[0] * F.P.dt, a.y + accv[1] * F.P.dt, a.z + accv[2] * F.P.dt,
+                                   a.w + accv[3] * F.P.dt);
+        }
+        if (__syncthreads_or(moved ? 1 : 0) && threadIdx.x == 0) F.tile_moved[_t] = 1u;
+    }
+}
+
+// nose (film.wgsl:680-723): reads F.delta, writes F.delta_out
+__global__ __launch_bounds__(256) void k_nose(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        if (F.h[c] <= 0.0f) {
+            F.delta_out[c] = 0u;
+            F.fnorm[c] = make_float2(0.0f, 0.0f);
+            continue;
+        }
+        int i = (int)x, j = (int)y;
+        int bx = 64, by = 64;
+        float bd = 1e9f;
+        for (int dj = -1; dj <= 1; dj++) {
+            for (int di = -1; di <= 1; di++) {
+                int xx = i + di, yy = j + dj;
+                if (xx < 0 || yy < 0 || xx >= (int)F.W || yy >= (int)F.H) continue;
+                uint32_t nn = idx(F, (uint32_t)xx, (uint32_t)yy);
+                int cx = di, cy = dj;
+                if (F.h[nn] > 0.0f) {
+                    uint32_t dv = F.delta[nn];
+                    if ((dv & 0x10000u) == 0u) continue;
+                    int ox, oy;
+                    unpack_off(dv, ox, oy);
+                    cx = ox + di;
+                    cy = oy + dj;
+                    int tx = i + cx, ty = j + cy;
+                    if (tx < 0 || ty < 0 || tx >= (int)F.W || ty >= (int)F.H) continue;
+                    if (F.h[idx(F, (uint32_t)tx, (uint32_t)ty)] > 0.0f) continue;
+                } else if (di == 0 && dj == 0) {
+                    continue;
+                }
+                float d = cz_len2((float)cx, (float)cy);
+                if (d < bd) { bd = d; bx = cx; by = cy; }
+            }
+        }
+        if (bd > 1e8f) {
+            F.delta_out[c] = 0u;
+            F.fnorm[c] = make_float2(0.0f, 0.0f);
+            continue;
+        }
+        F.delta_out[c] = pack_off(bx, by);
+        F.fnorm[c] = (bd <= 3.0f) ? front_normal(F, c) : make_float2(0.0f, 0.0f);
+    }
+}
+
+// Copy the offsets of active cells from the buffer written last to the other
+// one, so that both hold the same values everywhere between frames.
+__global__ __launch_bounds__(256) void k_delta_sync(FilmDev F, uint32_t* dst) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        dst[c] = F.delta[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// strata_split (film.wgsl:457-514)
+
+__device__ inline float4 band_split(const FilmDev& F, uint32_t cu, float hf, float y, float up, float total) {
+    float4 sh = F.strata_h[cu];
+    float hu = sh.x + sh.y + sh.z + sh.w;
+    if (hu <= 0.0f) return make_float4(total, 0.0f, 0.0f, 0.0f);
+    float fr[4] = {sh.x / hu, sh.y / hu, sh.z / hu, sh.w / hu};
+    if (y > 0.0f && up > 0.0f) {
+        float n = F.P.phys[2];
+        float k = hf / hu;
+        float z1 = sh.x * k;
+        float z2 = z1 + sh.y * k;
+        float z3 = z2 + sh.z * k;
+        float z4 = z3 + sh.w * k;
+        float b[4] = {band_flux(n, up, y, 0.0f, z1), band_flux(n, up, y, z1, z2), band_flux(n, up, y, z2, z3),
+                      band_flux(n, up, y, z3, z4)};
+        float bs = b[0] + b[1] + b[2] + b[3];
+        if (bs > 0.0f)
+            for (int q = 0; q < 4; q++) fr[q] = b[q] / bs;
+    }
+    return make_float4(fr[0] * total, fr[1] * total, fr[2] * total, fr[3] * total);
+}
+
+__global__ __launch_bounds__(256) void k_strata_split(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        float2 p00 = pressure_at(F, (int)x, (int)y);
+        float4 a = F.acc[c];
+        float av[4] = {a.x, a.y, a.z, a.w};
+        float4 o[4];
+        for (int k = 0; k < 4; k++) {
+            o[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            bool exists = (k == 0) ? (x + 1u < F.W)
+                        : (k == 1) ? (y + 1u < F.H)
+                        : (k == 2) ? (x + 1u < F.W && y + 1u < F.H)
+                                   : (x + 1u < F.W && y > 0u);
+            if (av[k] != 0.0f && exists) {
+                FaceGeom fg = face_geom(F, x, y, k, true, p00);
+                FaceFlux ff = {0.0f, 0.0f, 0.0f};
+                if (fg.has) ff = face(F, c, fg.cb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+                uint32_t cb = (k == 0) ? c + 1u : (k == 1) ? c + F.W : (k == 2) ? c + F.W + 1u : c - F.W + 1u;
+                uint32_t cu = (av[k] < 0.0f) ? cb : c;
+                o[k] = band_split(F, cu, 0.5f * (F.h[c] + F.h[cb]), ff.y, ff.up, av[k]);
+            }
+        }
+        for (int k = 0; k < 4; k++) F.acc_s[4u * c + k] = o[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reduce_tiles (film.wgsl:1330-1361) fused with pool8 (film.wgsl:1304-1324):
+// thread (lx, ly) of the block for 64-tile (tx, ty) owns the same 4x4 cells
+// in both, i.e. pool cell (16 tx + lx, 16 ty + ly).
+__global__ __launch_bounds__(256) void k_reduce_pool(FilmDev F) {
+    __shared__ float s_h[256];
+    __shared__ float s_mu[256];
+    uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+    float hm = 0.0f, mm = 1e9f, mx = 0.0f, ms = 0.0f;
+    for (uint32_t dy = 0; dy < 4u; dy++) {
+        for (uint32_t dxi = 0; dxi < 4u; dxi++) {
+            uint32_t i = blockIdx.x * TILE64 + lx * 4u + dxi;
+            uint32_t j = blockIdx.y * TILE64 + ly * 4u + dy;
+            if (i < F.W && j < F.H) {
+                uint32_t c = idx(F, i, j);
+                float v = F.h[c];
+                float mv = F.m[c];
+                if (v > 4.0f * F.P.ein[3]) {
+                    hm = cz_max(hm, v);
+                    mm = cz_min(mm, F.rheo[c].x);
+                }
+                mx = cz_max(mx, v + mv);
+                ms = cz_max(ms, mv);
+            }
+        }
+    }
+    uint32_t pi = blockIdx.x * 16u + lx, pj = blockIdx.y * 16u + ly;
+    if (pi < F.pw && pj < F.ph) {
+        F.pool_out[pj * F.pw + pi] = mx;
+        F.pool_out[F.pw * F.ph + pj * F.pw + pi] = ms;
+    }
+    s_h[threadIdx.x] = hm;
+    s_mu[threadIdx.x] = mm;
+    __syncthreads();
+    for (uint32_t s = 128u; s > 0u; s >>= 1) {
+        if (threadIdx.x < s) {
+            s_h[threadIdx.x] = cz_max(s_h[threadIdx.x], s_h[threadIdx.x + s]);
+            s_mu[threadIdx.x] = cz_min(s_mu[threadIdx.x], s_mu[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) F.stats_out[blockIdx.x + blockIdx.y * F.P.tiles[0]] = make_float2(s_h[0], s_mu[0]);
+}
+
+// Fold of the per-64-tile stats (poll_stats, src/film/mod.rs:944-951) and the
+// frame scalars the host reads back: [0] h_max, [1] mu_min, [2] active count
+// (bits), [3] unused, [4..7] cg_s[1].
+__global__ __launch_bounds__(256) void k_frame_stats(FilmDev F) {
+    __shared__ float s_h[256];
+    __shared__ float s_mu[256];
+    uint32_t nt = F.t64x * F.t64y;
+    float hm = 0.0f, mm = 1e9f;
+    for (uint32_t t = threadIdx.x; t < nt; t += 256u) {
+        float2 v = F.stats_out[t];
+        hm = cz_max(hm, v.x);
+        mm = cz_min(mm, v.y);
+    }
+    s_h[threadIdx.x] = hm;
+    s_mu[threadIdx.x] = mm;
+    __syncthreads();
+    for (uint32_t s = 128u; s > 0u; s >>= 1) {
+        if (threadIdx.x < s) {
+            s_h[threadIdx.x] = cz_max(s_h[threadIdx.x], s_h[threadIdx.x + s]);
+            s_mu[threadIdx.x] = cz_min(s_mu[threadIdx.x], s_mu[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        F.frame_out[0] = s_h[0];
+        F.frame_out[1] = s_mu[0];
+        F.frame_out[2] = cz_bits2f(*F.active_count);
+        F.frame_out[3] = 0.0f;
+        for (int k = 0; k < 4; k++) F.frame_out[4 + k] = F.cg_s[4 + k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// strata advection (strata.wgsl:157-323)
+
+__global__ __launch_bounds__(256) void k_strata_limit(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        float4 s0 = F.acc_s[4u * c], s1 = F.acc_s[4u * c + 1u], s2 = F.acc_s[4u * c + 2u], s3 = F.acc_s[4u * c + 3u];
+        float out[4] = {cz_max(s0.x, 0.0f) + cz_max(s1.x, 0.0f) + cz_max(s2.x, 0.0f) + cz_max(s3.x, 0.0f),
+                        cz_max(s0.y, 0.0f) + cz_max(s1.y, 0.0f) + cz_max(s2.y, 0.0f) + cz_max(s3.y, 0.0f),
+                        cz_max(s0.z, 0.0f) + cz_max(s1.z, 0.0f) + cz_max(s2.z, 0.0f) + cz_max(s3.z, 0.0f),
+                        cz_max(s0.w, 0.0f) + cz_max(s1.w, 0.0f) + cz_max(s2.w, 0.0f) + cz_max(s3.w, 0.0f)};
+        float4 q[4];
+        bool has[4] = {x > 0u && cell_active(F, x - 1u, y), y > 0u && cell_active(F, x, y - 1u),
+                       x > 0u && y > 0u && cell_active(F, x - 1u, y - 1u),
+                       x > 0u && y + 1u < F.H && cell_active(F, x - 1u, y + 1u)};
+        if (has[0]) q[0] = F.acc_s[4u * (c - 1u)];
+        if (has[1]) q[1] = F.acc_s[4u * (c - F.W) + 1u];
+        if (has[2]) q[2] = F.acc_s[4u * (c - F.W - 1u) + 2u];
+        if (has[3]) q[3] = F.acc_s[4u * (c + F.W - 1u) + 3u];
+        for (int k = 0; k < 4; k++) {
+            if (!has[k]) continue;
+            out[0] = out[0] + cz_max(-q[k].x, 0.0f);
+            out[1] = out[1] + cz_max(-q[k].y, 0.0f);
+            out[2] = out[2] + cz_max(-q[k].z, 0.0f);
+            out[3] = out[3] + cz_max(-q[k].w, 0.0f);
+        }
+        float4 th4 = F.strata_h[c];
+        float th[4] = {th4.x, th4.y, th4.z, th4.w};
+        float a[4];
+        for (int k = 0; k < 4; k++) {
+            a[k] = 1.0f;
+            if (out[k] > 0.0f) a[k] = cz_min(1.0f, th[k] * F.P.dx / out[k]);
+        }
+        F.alpha_s[c] = make_float4(a[0], a[1], a[2], a[3]);
+    }
+}
+
+__device__ inline void inflow_from(const FilmDev& F, Stack& s, uint32_t cn, const float* f) {
+    float4 an4 = F.alpha_s[cn];
+    float4 thn4 = F.strata_h[cn];
+    float an[4] = {an4.x, an4.y, an4.z, an4.w};
+    float thn[4] = {thn4.x, thn4.y, thn4.z, thn4.w};
+    float4 cu4 = F.strata_cure[cn];
+    float cun[4] = {cu4.x, cu4.y, cu4.z, cu4.w};
+    float z0 = 0.0f;
+    for (uint32_t k = 0; k < 4u; k++) {
+        float z1 = z0 + thn[k];
+        float v = f[k] * an[k] * F.P.inv_dx;
+        if (v > 0.0f && thn[k] > 0.0f) {
+            float pig[16];
+            unpack8(F.strata_pig[(cn * 4u + k) * 2u], F.strata_pig[(cn * 4u + k) * 2u + 1u], pig);
+            stack_add_band(F, s, z0, z1, v, pig, cun[k]);
+        }
+        z0 = z1;
+    }
+}
+
+__device__ __forceinline__ bool zero4(const float4 v) { return v.x == 0.0f && v.y == 0.0f && v.z == 0.0f && v.w == 0.0f; }
+__device__ __forceinline__ void to4(const float4 v, float* o) { o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w; }
+
+// strata_update (strata.wgsl:190-274); also records the new stack sum of each
+// active cell for the fill passes.
+__global__ __launch_bounds__(256) void k_strata_update(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        float4 fr = F.acc_s[4u * c], fu = F.acc_s[4u * c + 1u], fdg = F.acc_s[4u * c + 2u], feg = F.acc_s[4u * c + 3u];
+        const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float4 fl = z4, fd = z4, fll = z4, flu = z4;
+        uint32_t cl = c, cd = c, cll = c, clu = c;
+        if (x > 0u && cell_active(F, x - 1u, y)) { cl = c - 1u; fl = F.acc_s[4u * cl]; }
+        if (y > 0u && cell_active(F, x, y - 1u)) { cd = c - F.W; fd = F.acc_s[4u * cd + 1u]; }
+        if (x > 0u && y > 0u && cell_active(F, x - 1u, y - 1u)) { cll = c - F.W - 1u; fll = F.acc_s[4u * cll + 2u]; }
+        if (x > 0u && y + 1u < F.H && cell_active(F, x - 1u, y + 1u)) { clu = c + F.W - 1u; flu = F.acc_s[4u * clu + 3u]; }
+        if (zero4(fr) && zero4(fu) && zero4(fdg) && zero4(feg) && zero4(fl) && zero4(fd) && zero4(fll) && zero4(flu)) {
+            float hc = F.h[c];
+            float4 t4 = F.strata_h[c];
+            float th[4] = {cz_max(t4.x, 0.0f), cz_max(t4.y, 0.0f), cz_max(t4.z, 0.0f), cz_max(t4.w, 0.0f)};
+            float nt[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            float4 cu = F.strata_cure[c];
+            if (hc <= 0.0f) {
+                cu = z4;
+            } else {
+                float sum = th[0] + th[1] + th[2] + th[3];
+                for (int k = 0; k < 4; k++) nt[k] = th[k];
+                if (sum > 0.0f)
+                    for (int k = 0; k < 4; k++) nt[k] = th[k] * (hc / sum);
+            }
+            bool plain = true;
+            for (uint32_t k = 0; k < 3u; k++) {
+                if (nt[k] <= 0.0f && nt[k + 1u] > 0.0f) plain = false;
+                if (nt[k] > 0.0f && nt[k] < F.P.misc[0] && nt[k + 1u] > 0.0f) plain = false;
+            }
+            if (plain) {
+                F.strata_h2[c] = make_float4(nt[0], nt[1], nt[2], nt[3]);
+                F.strata_cure2[c] = cu;
+                for (uint32_t k = 0; k < 8u; k++) F.strata_pig2[c * 8u + k] = F.strata_pig[c * 8u + k];
+                F.tsum_a[c] = nt[0] + nt[1] + nt[2] + nt[3];
+                continue;
+            }
+        }
+        Stack s;
+        stack_load(F, c, s);
+        float4 a4 = F.alpha_s[c];
+        float a[4] = {a4.x, a4.y, a4.z, a4.w};
+        float vr[4], vu[4], vdg[4], veg[4], vl[4], vd[4], vll[4], vlu[4];
+        to4(fr, vr); to4(fu, vu); to4(fdg, vdg); to4(feg, veg); to4(fl, vl); to4(fd, vd); to4(fll, vll); to4(flu, vlu);
+        for (int k = 0; k < 4; k++) {
+            float out = cz_max(vr[k], 0.0f) + cz_max(vu[k], 0.0f) + cz_max(vdg[k], 0.0f) + cz_max(veg[k], 0.0f) +
+                        cz_max(-vl[k], 0.0f) + cz_max(-vd[k], 0.0f) + cz_max(-vll[k], 0.0f) + cz_max(-vlu[k], 0.0f);
+            s.th[k] = cz_max(s.th[k] - out * a[k] * F.P.inv_dx, 0.0f);
+        }
+        float nf[4];
+        if (x + 1u < F.W) { for (int k = 0; k < 4; k++) nf[k] = -vr[k]; inflow_from(F, s, c + 1u, nf); }
+        if (y + 1u < F.H) { for (int k = 0; k < 4; k++) nf[k] = -vu[k]; inflow_from(F, s, c + F.W, nf); }
+        if (x + 1u < F.W && y + 1u < F.H) { for (int k = 0; k < 4; k++) nf[k] = -vdg[k]; inflow_from(F, s, c + F.W + 1u, nf); }
+        if (x + 1u < F.W && y > 0u) { for (int k = 0; k < 4; k++) nf[k] = -veg[k]; inflow_from(F, s, c - F.W + 1u, nf); }
+        if (cl != c) inflow_from(F, s, cl, vl);
+        if (cd != c) inflow_from(F, s, cd, vd);
+        if (cll != c) inflow_from(F, s, cll, vll);
+        if (clu != c) inflow_from(F, s, clu, vlu);
+        float hc = F.h[c];
+        float sum = s.th[0] + s.th[1] + s.th[2] + s.th[3];
+        if (hc <= 0.0f) {
+            for (int k = 0; k < 4; k++) { s.th[k] = 0.0f; s.cu[k] = 0.0f; }
+        } else if (sum > 0.0f) {
+            float k = hc / sum;
+            for (int q = 0; q < 4; q++) s.th[q] = s.th[q] * k;
+        }
+        stack_compact(F, s);
+        stack_store_to(F.strata_h2, F.strata_cure2, F.strata_pig2, c, s);
+        F.tsum_a[c] = s.th[0] + s.th[1] + s.th[2] + s.th[3];
+    }
+}
+
+// strata_copy_back (strata.wgsl:276-286)
+__global__ __launch_bounds__(256) void k_strata_copy_back(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        F.strata_h[c] = F.strata_h2[c];
+        F.strata_cure[c] = F.strata_cure2[c];
+        for (uint32_t k = 0; k < 8u; k++) F.strata_pig[c * 8u + k] = F.strata_pig2[c * 8u + k];
+    }
+}
+
+// strata_fill (strata.wgsl:288-323), one pass: the stack sums of active cells
+// come from tin (values before the pass), those of inactive cells from
+// strata_h (not written by the pass); tout receives the sums after the pass.
+__device__ __forceinline__ float tsum_of(const FilmDev& F, const float* tin, uint32_t n, uint32_t xi, uint32_t yj) {
+    if (cell_active(F, xi, yj)) return tin[n];
+    float4 t = F.strata_h[n];
+    return t.x + t.y + t.z + t.w;
+}
+
+__global__ __launch_bounds__(256) void k_strata_fill(FilmDev F, const float* tin, float* tout) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        float own = tin[c];
+        tout[c] = own;
+        float hc = F.h[c];
+        if (hc <= 0.0f) continue;
+        if (own > 0.0f) continue;
+        uint32_t src = c;
+        float best = 0.0f;
+        if (x > 0u) { float v = tsum_of(F, tin, c - 1u, x - 1u, y); if (v > best) { best = v; src = c - 1u; } }
+        if (y > 0u) { float v = tsum_of(F, tin, c - F.W, x, y - 1u); if (v > best) { best = v; src = c - F.W; } }
+        if (x + 1u < F.W) { float v = tsum_of(F, tin, c + 1u, x + 1u, y); if (v > best) { best = v; src = c + 1u; } }
+        if (y + 1u < F.H) { float v = tsum_of(F, tin, c + F.W, x, y + 1u); if (v > best) { best = v; src = c + F.W; } }
+        if (src == c) continue;
+        inherit_from(F, c, src, hc, false);
+        tout[c] = hc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// implicit hydrostatic CG (film.wgsl:895-1302)
+
+// film.wgsl:895-920
+__device__ inline float face_mob(const FilmDev& F, uint32_t ca, uint32_t cb, float2 gcap, float2 ghyd, float2 gtilt,
+                                 bool diag) {
+    float ha = F.h[ca], hb = F.h[cb];
+    if (cz_min(ha, hb) <= 0.0f) return 0.0f;
+    float hf = 0.5f * (ha + hb);
+    float2 ra = F.rheo[ca], rb = F.rheo[cb];
+    float wa = ha / (ha + hb);
+    float wb = 1.0f - wa;
+    float mu = cz_expf(wa * cz_logf(cz_max(ra.x, 1e-4f)) + wb * cz_logf(cz_max(rb.x, 1e-4f)));
+    float ty = wa * ra.y + wb * rb.y;
+    float thin = cz_min(ha, hb) / hf;
+    float fcap = thin * thin * thin;
+    float2 yc = yield_cap(F, make_float2(gcap.x * fcap, gcap.y * fcap), ty);
+    float2 g = make_float2(yc.x + ghyd.x + gtilt.x, yc.y + ghyd.y + gtilt.y);
+    float gmag = cz_len2(g.x, g.y);
+    float ydep = (ty <= 0.0f) ? hf : ((g.x >= 0.0f) ? ha : hb);
+    float qy, qu;
+    float q = hb_flux(F.P.phys[2], gmag, ydep, mu, ty, qy, qu);
+    if (q <= 0.0f) return 0.0f;
+    float mob = cz_min(cz_max(q / gmag - F.P.misc[3], 0.0f), F.P.front[2]);
+    if (cz_min(ha, hb) < 0.5f * cz_max(ha, hb)) {
+        if (diag) return 0.0f;
+        uint32_t tgt = cb;
+        if (hb > ha) tgt = ca;
+        uint32_t nf = feeders(F, tgt);
+        mob = mob / cz_powf((float)(nf > 1u ? nf : 1u), 0.7f);
+    }
+    return mob;
+}
+
+__device__ __forceinline__ float kappa(const FilmDev& F) {
+    return F.P.front[3] * F.P.phys[0] * F.P.g[2] * F.P.inv_dx * F.P.inv_dx;
+}
+__device__ __forceinline__ float hold_u(const FilmDev& F, uint32_t c) {
+    float hold = 0.0f;
+    if (F.h[c] > 0.0f) hold = hold_level(F, nose_dist(F, c));
+    return F.m[c] - hold;
+}
+
+__global__ __launch_bounds__(256) void k_cg_setup(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        float2 p00 = pressure_at(F, (int)x, (int)y);
+        float mob[4];
+        for (int k = 0; k < 4; k++) {
+            mob[k] = 0.0f;
+            FaceGeom fg = face_geom(F, x, y, k, false, p00);
+            if (fg.has) mob[k] = face_mob(F, c, fg.cb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag);
+        }
+        F.cg_m[c] = make_float4(mob[0], mob[1], mob[2], mob[3]);
+    }
+}
+
+// film.wgsl:987-1003
+__device__ __forceinline__ void cg_nb(const FilmDev& F, uint32_t k, uint32_t i, uint32_t j, uint32_t c, uint32_t& oc,
+                                      float& om) {
+    oc = 0u;
+    om = 0.0f;
+    switch (k) {
+    case 0: if (i + 1u < F.W) { oc = c + 1u; om = W_AXIS * F.cg_m[c].x; } break;
+    case 1: if (j + 1u < F.H) { oc = c + F.W; om = W_AXIS * F.cg_m[c].y; } break;
+    case 2: if (i + 1u < F.W && j + 1u < F.H) { oc = c + F.W + 1u; om = W_DIAG_LAP * F.cg_m[c].z; } break;
+    case 3: if (i + 1u < F.W && j > 0u) { oc = c - F.W + 1u; om = W_DIAG_LAP * F.cg_m[c].w; } break;
+    case 4: if (i > 0u) { oc = c - 1u; om = W_AXIS * F.cg_m[oc].x; } break;
+    case 5: if (j > 0u) { oc = c - F.W; om = W_AXIS * F.cg_m[oc].y; } break;
+    case 6: if (i > 0u && j > 0u) { oc = c - F.W - 1u; om = W_DIAG_LAP * F.cg_m[oc].z; } break;
+    default: if (i > 0u && j + 1u < F.H) { oc = c + F.W - 1u; om = W_DIAG_LAP * F.cg_m[oc].w; } break;
+    }
+}
+
+__device__ __forceinline__ float lap_src(const FilmDev& F, uint32_t c, uint32_t which) {
+    switch (which) {
+    case 0: return F.cg_p[c];
+    case 1: return F.h[c];
+    case 2: return F.cg_t2[c];
+    case 3: return F.cg_t[c];
+    case 4: return F.cg_x[c] + hold_u(F, c);
+    default: return hold_u(F, c);
+    }
+}
+
+// film.wgsl:1016-1036
+__device__ inline float lap_w(const FilmDev& F, uint32_t i, uint32_t j, uint32_t c, uint32_t which) {
+    int ii = (int)i, jj = (int)j;
+    float vc = lap_src(F, c, which);
+    float acc_v = 0.0f;
+    for (int dj = -1; dj <= 1; dj++) {
+        for (int di = -1; di <= 1; di++) {
+            if (di == 0 && dj == 0) continue;
+            int xx = ii + di, yy = jj + dj;
+            if (xx < 0 || yy < 0 || xx >= (int)F.W || yy >= (int)F.H) continue;
+            if (!cell_active(F, (uint32_t)xx, (uint32_t)yy)) continue;
+            uint32_t n = idx(F, (uint32_t)xx, (uint32_t)yy);
+            if (F.h[n] <= 0.0f) continue;
+            float w = W_AXIS;
+            if (di != 0 && dj != 0) w = W_DIAG_LAP;
+            acc_v = acc_v + w * (lap_src(F, n, which) - vc);
+        }
+    }
+    return acc_v;
+}
+
+// film.wgsl:1038-1058
+__device__ inline float lap_diag(const FilmDev& F, uint32_t i, uint32_t j) {
+    int ii = (int)i, jj = (int)j;
+    float s1 = 0.0f, s2 = 0.0f;
+    for (int dj = -1; dj <= 1; dj++) {
+        for (int di = -1; di <= 1; di++) {
+            if (di == 0 && dj == 0) continue;
+            int xx = ii + di, yy = jj + dj;
+            if (xx < 0 || yy < 0 || xx >= (int)F.W || yy >= (int)F.H) continue;
+            if (!cell_active(F, (uint32_t)xx, (uint32_t)yy)) continue;
+            if (F.h[idx(F, (uint32_t)xx, (uint32_t)yy)] <= 0.0f) continue;
+            float w = W_AXIS;
+            if (di != 0 && dj != 0) w = W_DIAG_LAP;
+            s1 = s1 + w;
+            s2 = s2 + w * w;
+        }
+    }
+    return s1 * s1 + s2;
+}
+
+// Block sum of one double per thread (wave64 shuffles, then LDS).
+__device__ inline double block_sum_d(double v) {
+    __shared__ double part[4];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0u) part[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = (part[0] + part[1]) + (part[2] + part[3]);
+    return r;
+}
+
+// which: 1 = cg_lap_h (t <- L h, t2 <- L hold_u), 0 = cg_lap_p (t <- L p),
+// 4 = cg_lap_u (t <- L (x + hold_u))   (film.wgsl:1060-1083)
+__global__ __launch_bounds__(256) void k_cg_lap(FilmDev F, uint32_t which) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        if (which == 1u) {
+            float a = lap_w(F, x, y, c, 1u);
+            float b = lap_w(F, x, y, c, 5u);
+            F.cg_t[c] = a;
+            F.cg_t2[c] = b;
+        } else {
+            F.cg_t[c] = lap_w(F, x, y, c, which);
+        }
+    }
+}
+
+// cg_init (film.wgsl:1110-1144)
+__global__ __launch_bounds__(256) void k_cg_init(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        double v = 0.0;
+        if (in_grid) {
+            float k = kappa(F);
+            float hc = F.h[c];
+            float uc = hold_u(F, c);
+            float diag = 1.0f, b = hc, ax = hc;
+            for (uint32_t f = 0; f < 8u; f++) {
+                uint32_t nc;
+                float nm;
+                cg_nb(F, f, x, y, c, nc, nm);
+                if (nm > 0.0f) {
+                    diag = diag + k * nm;
+                    b = b + k * nm * (hold_u(F, nc) - uc);
+                    ax = ax + k * nm * (hc - F.h[nc]);
+                }
+            }
+            if (hc > 0.0f) {
+                float cc = F.P.front3[0];
+                diag = diag + cc * lap_diag(F, x, y);
+                b = b - cc * lap_w(F, x, y, c, 2u);
+                ax = ax + cc * lap_w(F, x, y, c, 3u);
+            }
+            float r = b - ax;
+            F.cg_d[c] = diag;
+            F.cg_x[c] = hc;
+            F.cg_r[c] = r;
+            F.cg_p[c] = r / diag;
+            v = (double)(r * r / diag);
+        }
+        double s = block_sum_d(v);
+        if (threadIdx.x == 0) F.cg_part[_a] = s;
+    }
+}
+
+// cg_iter_a (film.wgsl:1146-1157): q = A p, partial p.q
+__global__ __launch_bounds__(256) void k_cg_iter_a(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        double v = 0.0;
+        if (in_grid) {
+            float k = kappa(F);
+            float pc = F.cg_p[c];
+            float accv = pc;
+            for (uint32_t f = 0; f < 8u; f++) {
+                uint32_t nc;
+                float nm;
+                cg_nb(F, f, x, y, c, nc, nm);
+                if (nm > 0.0f) accv = accv + k * nm * (pc - F.cg_p[nc]);
+            }
+            if (F.h[c] > 0.0f) accv = accv + F.P.front3[0] * lap_w(F, x, y, c, 3u);
+            F.cg_q[c] = accv;
+            v = (double)(pc * accv);
+        }
+        double s = block_sum_d(v);
+        if (threadIdx.x == 0) F.cg_part[_a] = s;
+    }
+}
+
+// cg_iter_b (film.wgsl:1159-1172)
+__global__ __launch_bounds__(256) void k_cg_iter_b(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        double v = 0.0;
+        if (in_grid) {
+            float alpha = F.cg_s[2];
+            float r = F.cg_r[c] - alpha * F.cg_q[c];
+            F.cg_x[c] = F.cg_x[c] + alpha * F.cg_p[c];
+            F.cg_r[c] = r;
+            v = (double)(r * r / F.cg_d[c]);
+        }
+        double s = block_sum_d(v);
+        if (threadIdx.x == 0) F.cg_part[_a] = s;
+    }
+}
+
+// cg_iter_c (film.wgsl:1174-1181)
+__global__ __launch_bounds__(256) void k_cg_iter_c(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        float beta = F.cg_s[3];
+        F.cg_p[c] = F.cg_r[c] / F.cg_d[c] + beta * F.cg_p[c];
+    }
+}
+
+// Single-block FP64 fold of the per-tile partials (film.wgsl:1183-1255):
+// mode 0 = cg_reduce_init, 1 = cg_reduce_a, 2 = cg_reduce_b, 3 = cg_reduce_mass.
+__global__ __launch_bounds__(256) void k_cg_reduce(FilmDev F, int mode) {
+    uint32_t n = *F.active_count;
+    double a = 0.0, b = 0.0;
+    for (uint32_t t = threadIdx.x; t < n; t += 256u) {
+        a += F.cg_part[t];
+        if (mode == 3) b += F.cg_part2[t];
+    }
+    double sa = block_sum_d(a);
+    double sb = (mode == 3) ? block_sum_d(b) : 0.0;
+    if (threadIdx.x != 0) return;
+    float* s = F.cg_s;
+    if (mode == 0) {
+        float rz = (float)sa;
+        s[0] = rz; s[1] = 0.0f; s[2] = 0.0f; s[3] = 0.0f;
+        s[4] = rz; s[5] = 0.0f; s[6] = 0.0f; s[7] = 0.0f;
+    } else if (mode == 1) {
+        float pq = (float)sa;
+        float alpha = 0.0f;
+        if (pq > 0.0f) alpha = s[0] / pq;
+        s[1] = pq; s[2] = alpha; s[3] = 0.0f;
+    } else if (mode == 2) {
+        float rz = (float)sa;
+        float beta = 0.0f;
+        if (s[0] > 0.0f) beta = rz / s[0];
+        s[0] = rz; s[3] = beta;
+    } else {
+        float d = (float)sa, tot = (float)sb;
+        s[5] = d; s[6] = tot; s[7] = s[0];
+    }
+}
+
+// cg_finish (film.wgsl:1261-1302); the new height goes to cg_q so that
+// neighbours read the height from before the pass; k_cg_rescale commits it.
+__global__ __launch_bounds__(256) void k_cg_finish(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        bool moved = false;
+        double dv = 0.0, tv = 0.0;
+        if (in_grid) {
+            float h0 = F.h[c];
+            float hn = cz_max(F.cg_x[c], 0.0f);
+            float rg = F.P.phys[0] * F.P.g[2] * F.P.inv_dx;
+            float pc = F.cg_x[c] + hold_u(F, c);
+            float f[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            float4 mob = F.cg_m[c];
+            if (mob.x > 0.0f) { uint32_t n = c + 1u; f[0] = W_AXIS * mob.x * rg * (pc - (F.cg_x[n] + hold_u(F, n))); }
+            if (mob.y > 0.0f) { uint32_t n = c + F.W; f[1] = W_AXIS * mob.y * rg * (pc - (F.cg_x[n] + hold_u(F, n))); }
+            if (mob.z > 0.0f) { uint32_t n = c + F.W + 1u; f[2] = W_DIAG_FLUX * mob.z * rg * (pc - (F.cg_x[n] + hold_u(F, n))) * 0.70710678f; }
+            if (mob.w > 0.0f) { uint32_t n = c - F.W + 1u; f[3] = W_DIAG_FLUX * mob.w * rg * (pc - (F.cg_x[n] + hold_u(F, n))) * 0.70710678f; }
+            if (h0 > 0.0f) {
+                float kc = F.P.front3[1];
+                float tc = F.cg_t[c];
+                if (x + 1u < F.W) { uint32_t n = c + 1u; if (F.h[n] > 0.0f && cell_active(F, x + 1u, y)) f[0] = f[0] + W_AXIS * kc * (F.cg_t[n] - tc); }
+                if (y + 1u < F.H) { uint32_t n = c + F.W; if (F.h[n] > 0.0f && cell_active(F, x, y + 1u)) f[1] = f[1] + W_AXIS * kc * (F.cg_t[n] - tc); }
+                if (x + 1u < F.W && y + 1u < F.H) { uint32_t n = c + F.W + 1u; if (F.h[n] > 0.0f && cell_active(F, x + 1u, y + 1u)) f[2] = f[2] + W_DIAG_FLUX * kc * (F.cg_t[n] - tc) * 0.70710678f; }
+                if (x + 1u < F.W && y > 0u) { uint32_t n = c - F.W + 1u; if (F.h[n] > 0.0f && cell_active(F, x + 1u, y - 1u)) f[3] = f[3] + W_DIAG_FLUX * kc * (F.cg_t[n] - tc) * 0.70710678f; }
+            }
+            float4 a = F.acc[c];
+            float fw = F.P.front[3];
+            F.acc[c] = make_float4(a.x + f[0] * fw, a.y + f[1] * fw, a.z + f[2] * fw, a.w + f[3] * fw);
+            F.cg_q[c] = hn;
+            moved = cz_abs(hn - h0) > MOVE_EPS;
+            dv = (double)(hn - h0);
+            tv = (double)hn;
+        }
+        double sd = block_sum_d(dv);
+        double st = block_sum_d(tv);
+        if (threadIdx.x == 0) {
+            F.cg_part[_a] = sd;
+            F.cg_part2[_a] = st;
+        }
+        if (__syncthreads_or(moved ? 1 : 0) && threadIdx.x == 0) F.tile_moved[_t] = 1u;
+    }
+}
+
+// cg_rescale (film.wgsl:1224-1233) applied to the height left by k_cg_finish.
+__global__ __launch_bounds__(256) void k_cg_rescale(FilmDev F) {
+    CZ_FOR_TILES(F) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        float hn = F.cg_q[c];
+        float s1y = F.cg_s[5], s1z = F.cg_s[6];
+        if (s1z <= 0.0f) {
+            F.h[c] = hn;
+            continue;
+        }
+        float fct = 1.0f - s1y / s1z;
+        F.h[c] = cz_max(hn * fct, 0.0f);
+    }
+}
+
+}  // namespace cz

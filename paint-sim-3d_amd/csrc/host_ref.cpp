@@ -1,0 +1,371 @@
+// host_ref.cpp - host-side functions of the reference's film and colour
+// modules that sit on the step's boundary: FilmSettings defaults, the
+// environment toggles, pigment/CMF tables, weave and substrate init, Deposit
+// geometry and PaintMix packing.  Pure host C++ (FP32 as in the Rust source).
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/canvaz_film.h"
+#include "czmath.h"
+#include "host_ref.h"
+
+namespace czh {
+
+static const float PI_F = 3.14159265358979323846f;
+
+float to_radians(float d) { return d * (PI_F / 180.0f); }
+float powi4(float x) {
+    float a = x * x;
+    return a * a;
+}
+
+// Deposit::profile (src/film/mod.rs:152-185)
+float height_at(const cz_deposit& d, float px, float py) {
+    float dx = d.p1[0] - d.p0[0], dy = d.p1[1] - d.p0[1];
+    float l = sqrtf(dx * dx + dy * dy);
+    float dir0 = 1.0f, dir1 = 0.0f;
+    if (l > 0.0f) {
+        dir0 = dx / l;
+        dir1 = dy / l;
+    } else {
+        l = 0.0f;
+    }
+    bool cap_start = (d.kind & CZ_DEPOSIT_CAP_START) != 0;
+    bool cap_end = (d.kind & CZ_DEPOSIT_CAP_END) != 0;
+    float q0x = px - d.p0[0], q0y = py - d.p0[1];
+    float q1x = px - d.p1[0], q1y = py - d.p1[1];
+    if (l <= 0.0f) {
+        float x = sqrtf(q0x * q0x + q0y * q0y) / d.r0;
+        return (x < 1.0f) ? d.h0 * (1.0f - x * x) : 0.0f;
+    }
+    if (!cap_start && q0x * d.n0[0] + q0y * d.n0[1] < 0.0f) return 0.0f;
+    if (!cap_end && q1x * d.n1[0] + q1y * d.n1[1] >= 0.0f) return 0.0f;
+    float u = q0x * dir0 + q0y * dir1;
+    if (u < -d.r0 || u > l + d.r1) return 0.0f;
+    float dist = fabsf(q0x * dir1 - q0y * dir0);
+    if (cap_start && u < 0.0f) dist = sqrtf(q0x * q0x + q0y * q0y);
+    if (cap_end && u > l) dist = sqrtf(q1x * q1x + q1y * q1y);
+    float t = std::min(std::max(u / l, 0.0f), 1.0f);
+    float rt = d.r0 + (d.r1 - d.r0) * t;
+    float x = dist / rt;
+    return (x < 1.0f) ? d.h0 * (rt / d.r0) * (1.0f - x * x) : 0.0f;
+}
+
+// Deposit::sampled_volume_in (src/film/mod.rs:117-131)
+float sampled_volume_in(const cz_deposit& d, float lattice, float lo0, float lo1, float hi0, float hi1) {
+    float rm = std::max(d.r0, d.r1) + lattice;
+    int x0 = (int)floorf(std::max(std::min(d.p0[0], d.p1[0]) - rm, lo0) / lattice);
+    int x1 = (int)ceilf(std::min(std::max(d.p0[0], d.p1[0]) + rm, hi0) / lattice);
+    int y0 = (int)floorf(std::max(std::min(d.p0[1], d.p1[1]) - rm, lo1) / lattice);
+    int y1 = (int)ceilf(std::min(std::max(d.p0[1], d.p1[1]) + rm, hi1) / lattice);
+    float sum = 0.0f;
+    for (int i = x0; i < x1; i++)
+        for (int j = y0; j < y1; j++) sum += height_at(d, ((float)i + 0.5f) * lattice, ((float)j + 0.5f) * lattice);
+    return sum * lattice * lattice;
+}
+
+float sampled_volume(const cz_deposit& d, float lattice) {
+    return sampled_volume_in(d, lattice, -INFINITY, -INFINITY, INFINITY, INFINITY);
+}
+
+// Deposit::inside_fraction (src/film/mod.rs:133-145)
+float inside_fraction(const cz_deposit& d, float w, float h) {
+    float rb = std::max(d.r0, d.r1);
+    bool full = std::min(d.p0[0], d.p1[0]) - rb >= 0.0f && std::min(d.p0[1], d.p1[1]) - rb >= 0.0f &&
+                std::max(d.p0[0], d.p1[0]) + rb <= w && std::max(d.p0[1], d.p1[1]) + rb <= h;
+    if (full) return 1.0f;
+    float lattice = 0.25f * std::max(std::min(d.r0, d.r1), 1e-4f);
+    float all = sampled_volume(d, lattice);
+    if (all <= 0.0f) return 0.0f;
+    float v = sampled_volume_in(d, lattice, 0.0f, 0.0f, w, h) / all;
+    return std::min(std::max(v, 0.0f), 1.0f);
+}
+
+float max_height(const cz_deposit& d) { return d.h0 * std::max(d.r0, d.r1) / d.r0; }
+
+}  // namespace czh
+
+using namespace czh;
+
+extern "C" {
+
+void cz_settings_default(cz_settings* s) {
+    memset(s, 0, sizeof(*s));
+    s->width_m = 1.2f;
+    s->height_m = 0.8f;
+    s->rho = 1000.0f;
+    s->sigma = 0.035f;
+    s->n_index = 1.0f;
+    s->m_cap = 1.0e-8f;
+    s->contact_angle_deg = 45.0f;
+    s->hysteresis_deg = 15.0f;
+    s->h_star = 2.0e-5f;
+    s->raw_canvas = 1;
+    s->weave_um = 25.0f;
+    s->cure_rate = 1.0f / 600.0f;
+    s->tau_cure = 300.0f;
+    s->lnmu_cure = 5.0f;
+    s->v_cap = 1.0f;
+    s->tilt_deg = 0.0f;
+    s->tilt_dir_deg = -90.0f;
+    s->gravity = 9.81f;
+    s->medium_mu = 0.08f;
+    s->debug_view = 0;
+    s->h_min = 1.5e-5f;
+    s->m_cap_hydro = 1.0e-7f;
+    s->max_substeps = 64;
+    s->min_substeps = 4;
+    s->implicit_iters = 40;
+    s->implicit_hydro = 1;
+    s->slump = 0;
+    s->capillary = 1;
+    s->capillary_implicit = 0;
+}
+
+void cz_overrides_default(cz_overrides* o) {
+    memset(o, 0, sizeof(*o));
+    o->capcond = 80.0f;
+    o->fcap = 2.5f;
+    o->kshape = 1.0f;
+    o->kline = 0.0f;
+}
+
+static bool env_set(const char* k) { return getenv(k) != nullptr; }
+static bool env_float(const char* k, float* out) {
+    const char* v = getenv(k);
+    if (!v) return false;
+    char* end = nullptr;
+    float f = strtof(v, &end);
+    if (end == v) return false;
+    *out = f;
+    return true;
+}
+
+// The env reads of update_params (src/film/mod.rs:870-915).
+void cz_overrides_from_env(cz_overrides* o) {
+    cz_overrides_default(o);
+    o->no_implicit = env_set("PAINT_NO_IMPLICIT");
+    o->no_cap = env_set("PAINT_NO_CAP");
+    env_float("PAINT_CAPCOND", &o->capcond);
+    env_float("PAINT_FCAP", &o->fcap);
+    o->no_hold = env_set("PAINT_NO_HOLD");
+    env_float("PAINT_KSHAPE", &o->kshape);
+    o->no_transl = env_set("PAINT_NO_TRANSL");
+    o->no_shape = env_set("PAINT_NO_SHAPE");
+    env_float("PAINT_KLINE", &o->kline);
+    o->has_hyst = env_float("PAINT_HYST", &o->hyst_deg);
+}
+
+// pigment_table (src/color/mod.rs:243-258)
+void cz_pigment_table(const float* refl, const float* scatter, const float* tint, const float* viscosity,
+                      const float* yield_stress, float* t) {
+    memset(t, 0, CZ_NPIG * CZ_PIG_STRIDE * sizeof(float));
+    for (int i = 0; i < CZ_NPIG; i++) {
+        for (int b = 0; b < CZ_NBAND; b++) {
+            float r = std::min(std::max(refl[i * CZ_NBAND + b], 0.01f), 0.98f);
+            float ks = (1.0f - r) * (1.0f - r) / (2.0f * r);
+            float s = scatter[i];
+            t[i * CZ_PIG_STRIDE + b] = ks * s;
+            t[i * CZ_PIG_STRIDE + CZ_NBAND + b] = s;
+        }
+        t[i * CZ_PIG_STRIDE + 32] = viscosity[i];
+        t[i * CZ_PIG_STRIDE + 33] = yield_stress[i];
+        t[i * CZ_PIG_STRIDE + 34] = tint[i];
+    }
+}
+
+// default_pigments (src/color/mod.rs:173-224): piecewise-linear reflectance
+// samples evaluated at the band centres 410..710 nm, clamped to [0.01, 0.98].
+void cz_default_pigments(float* refl, float* scatter, float* tint, float* viscosity, float* yield_stress) {
+    struct Pt { float l, r; };
+    struct Pg { const char* name; float s, mu, ty; int n; Pt pts[8]; };
+    static const Pg pigs[12] = {
+        {"Titanium White", 3.0e4f, 40.0f, 40.0f, 4, {{400, 0.86f}, {450, 0.92f}, {550, 0.94f}, {700, 0.93f}}},
+        {"Ivory Black", 6.0e3f, 30.0f, 30.0f, 3, {{400, 0.045f}, {550, 0.045f}, {700, 0.055f}}},
+        {"Cadmium Yellow", 2.2e4f, 35.0f, 35.0f, 7, {{400, 0.05f}, {460, 0.06f}, {490, 0.15f}, {510, 0.45f}, {530, 0.75f}, {560, 0.86f}, {700, 0.88f}}},
+        {"Cadmium Red", 2.0e4f, 35.0f, 35.0f, 6, {{400, 0.05f}, {560, 0.06f}, {590, 0.20f}, {610, 0.55f}, {630, 0.78f}, {700, 0.82f}}},
+        {"Alizarin Crimson", 9.0e2f, 25.0f, 15.0f, 8, {{400, 0.12f}, {430, 0.14f}, {470, 0.07f}, {540, 0.04f}, {580, 0.06f}, {610, 0.22f}, {640, 0.48f}, {700, 0.60f}}},
+        {"Ultramarine Blue", 4.0e3f, 30.0f, 20.0f, 6, {{400, 0.25f}, {440, 0.36f}, {470, 0.32f}, {520, 0.12f}, {580, 0.05f}, {700, 0.06f}}},
+        {"Phthalo Blue", 1.0e3f, 28.0f, 15.0f, 6, {{400, 0.12f}, {440, 0.22f}, {470, 0.26f}, {510, 0.14f}, {560, 0.04f}, {700, 0.03f}}},
+        {"Phthalo Green", 1.0e3f, 28.0f, 15.0f, 7, {{400, 0.04f}, {470, 0.09f}, {500, 0.27f}, {520, 0.30f}, {550, 0.17f}, {600, 0.04f}, {700, 0.03f}}},
+        {"Yellow Ochre", 1.5e4f, 38.0f, 40.0f, 6, {{400, 0.07f}, {470, 0.10f}, {520, 0.22f}, {570, 0.42f}, {620, 0.52f}, {700, 0.56f}}},
+        {"Burnt Umber", 4.0e3f, 38.0f, 40.0f, 4, {{400, 0.04f}, {500, 0.05f}, {600, 0.10f}, {700, 0.17f}}},
+        {"Viridian", 2.0e3f, 30.0f, 20.0f, 6, {{400, 0.06f}, {460, 0.12f}, {500, 0.26f}, {530, 0.22f}, {580, 0.07f}, {700, 0.05f}}},
+        {"Cadmium Orange", 2.0e4f, 35.0f, 35.0f, 6, {{400, 0.05f}, {520, 0.06f}, {550, 0.15f}, {580, 0.50f}, {610, 0.80f}, {700, 0.85f}}},
+    };
+    for (int i = 0; i < 12; i++) {
+        const Pg& p = pigs[i];
+        for (int b = 0; b < CZ_NBAND; b++) {
+            float lam = 410.0f + 20.0f * (float)b;
+            float v = p.pts[p.n - 1].r;
+            if (lam <= p.pts[0].l) {
+                v = p.pts[0].r;
+            } else {
+                for (int k = 0; k + 1 < p.n; k++) {
+                    if (lam >= p.pts[k].l && lam <= p.pts[k + 1].l) {
+                        float t = (lam - p.pts[k].l) / (p.pts[k + 1].l - p.pts[k].l);
+                        v = p.pts[k].r + t * (p.pts[k + 1].r - p.pts[k].r);
+                        break;
+                    }
+                }
+            }
+            refl[i * CZ_NBAND + b] = std::min(std::max(v, 0.01f), 0.98f);
+        }
+        scatter[i] = p.s;
+        viscosity[i] = p.mu;
+        yield_stress[i] = p.ty;
+        std::string nm(p.name);
+        float tn = 1.0f;  // default_tint (src/color/mod.rs:217-224)
+        if (nm == "Phthalo Blue" || nm == "Phthalo Green") tn = 25.0f;
+        else if (nm == "Alizarin Crimson") tn = 4.0f;
+        else if (nm == "Ultramarine Blue" || nm == "Viridian" || nm == "Ivory Black") tn = 1.5f;
+        tint[i] = tn;
+    }
+}
+
+// cmf_table (src/color/mod.rs:110-171): CIE 1931 x D65, band pairs, sum(y) = 1.
+void cz_cmf_table(float* t) {
+    static const float CMF[32][3] = {
+        {0.0143f, 0.0004f, 0.0679f}, {0.0435f, 0.0012f, 0.2074f}, {0.1344f, 0.0040f, 0.6456f}, {0.2839f, 0.0116f, 1.3856f},
+        {0.3483f, 0.0230f, 1.7471f}, {0.3362f, 0.0380f, 1.7721f}, {0.2908f, 0.0600f, 1.6692f}, {0.1954f, 0.0910f, 1.2876f},
+        {0.0956f, 0.1390f, 0.8130f}, {0.0320f, 0.2080f, 0.4652f}, {0.0049f, 0.3230f, 0.2720f}, {0.0093f, 0.5030f, 0.1582f},
+        {0.0633f, 0.7100f, 0.0782f}, {0.1655f, 0.8620f, 0.0422f}, {0.2904f, 0.9540f, 0.0203f}, {0.4334f, 0.9950f, 0.0087f},
+        {0.5945f, 0.9950f, 0.0039f}, {0.7621f, 0.9520f, 0.0021f}, {0.9163f, 0.8700f, 0.0017f}, {1.0263f, 0.7570f, 0.0011f},
+        {1.0622f, 0.6310f, 0.0008f}, {1.0026f, 0.5030f, 0.0003f}, {0.8544f, 0.3810f, 0.0002f}, {0.6424f, 0.2650f, 0.0000f},
+        {0.4479f, 0.1750f, 0.0f},    {0.2835f, 0.1070f, 0.0f},    {0.1649f, 0.0610f, 0.0f},    {0.0874f, 0.0320f, 0.0f},
+        {0.0468f, 0.0170f, 0.0f},    {0.0227f, 0.0082f, 0.0f},    {0.0114f, 0.0041f, 0.0f},    {0.0058f, 0.0021f, 0.0f}};
+    static const float D65[32] = {82.8f, 91.5f, 93.4f, 86.7f, 104.9f, 117.0f, 117.8f, 114.9f, 115.9f, 108.8f, 109.4f,
+                                  107.8f, 104.8f, 107.7f, 104.4f, 104.0f, 100.0f, 96.3f, 95.8f, 88.7f, 90.0f, 89.6f,
+                                  87.7f, 83.3f, 83.7f, 80.0f, 80.2f, 82.3f, 78.3f, 69.7f, 71.6f, 74.3f};
+    float ysum = 0.0f;
+    for (int b = 0; b < CZ_NBAND; b++) {
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        for (int s = 0; s < 2; s++) {
+            int k = 2 * b + s;
+            float w = D65[k];
+            acc[0] += CMF[k][0] * w;
+            acc[1] += CMF[k][1] * w;
+            acc[2] += CMF[k][2] * w;
+        }
+        t[b * 3] = acc[0];
+        t[b * 3 + 1] = acc[1];
+        t[b * 3 + 2] = acc[2];
+        ysum += acc[1];
+    }
+    for (int i = 0; i < CZ_NBAND * 3; i++) t[i] /= ysum;
+}
+
+// weave (src/film/mod.rs:212-233)
+void cz_weave(uint32_t w, uint32_t h, float dx, float amp, float* v) {
+    memset(v, 0, (size_t)w * h * sizeof(float));
+    if (amp <= 0.0f) return;
+    float period = 1.4e-3f;
+    float k = (2.0f * PI_F) / period;
+    for (uint32_t j = 0; j < h; j++) {
+        float y = ((float)j + 0.5f) * dx;
+        float sy = sinf(k * y);
+        for (uint32_t i = 0; i < w; i++) {
+            float x = ((float)i + 0.5f) * dx;
+            uint32_t hsh = (i * 73856093u) ^ (j * 19349663u);
+            hsh ^= hsh >> 13;
+            hsh = hsh * 0x5bd1e995u;
+            hsh ^= hsh >> 15;
+            float r = (float)(hsh & 0xffffu) / 65535.0f * 2.0f - 1.0f;
+            float s = sinf(k * x) * sy;
+            v[(size_t)j * w + i] = amp * (1.0f + 0.6f * s + 0.4f * r);
+        }
+    }
+}
+
+// substrate_refl without a canvas colour (src/film/mod.rs:188-210)
+void cz_substrate_refl(uint64_t n, uint32_t raw, uint32_t* v) {
+    float bands[16];
+    for (int b = 0; b < 16; b++) {
+        float t = (float)b / 15.0f;
+        bands[b] = raw ? 0.50f + 0.16f * t : 0.84f + 0.04f * t;
+    }
+    uint32_t packed[8];
+    for (int k = 0; k < 8; k++) packed[k] = cz_f2h(bands[2 * k]) | (cz_f2h(bands[2 * k + 1]) << 16);
+    for (uint64_t c = 0; c < n; c++) memcpy(&v[c * 8], packed, sizeof(packed));
+}
+
+// Deposit::ribbon (src/film/mod.rs:63-80)
+void cz_deposit_ribbon(const float p0[2], const float p1[2], const float n0[2], const float n1[2], float r0, float r1,
+                       uint32_t caps, float volume, const uint32_t pig[8], float dx, cz_deposit* out) {
+    float min_r = 0.8f * dx;
+    float a = std::max(r0, min_r), b = std::max(r1, min_r);
+    cz_deposit d;
+    memset(&d, 0, sizeof(d));
+    d.p0[0] = p0[0]; d.p0[1] = p0[1]; d.p1[0] = p1[0]; d.p1[1] = p1[1];
+    d.n0[0] = n0[0]; d.n0[1] = n0[1]; d.n1[0] = n1[0]; d.n1[1] = n1[1];
+    d.r0 = a; d.r1 = b; d.h0 = a; d.vol = volume; d.kind = caps;
+    memcpy(d.pig, pig, 8 * sizeof(uint32_t));
+    float lattice = 0.25f * dx;
+    float sampled = sampled_volume(d, lattice);
+    float rm = std::max(a, b);
+    float disc = PI_F * rm * rm * rm / 2.0f;
+    if (sampled < 0.25f * disc) {
+        float mid0 = 0.5f * (p0[0] + p1[0]), mid1 = 0.5f * (p0[1] + p1[1]);
+        d.p0[0] = mid0; d.p0[1] = mid1; d.p1[0] = mid0; d.p1[1] = mid1;
+        d.r0 = rm; d.r1 = rm; d.h0 = rm;
+        d.h0 = volume / sampled_volume(d, lattice) * rm;
+        *out = d;
+        return;
+    }
+    d.h0 = volume / sampled * a;
+    *out = d;
+}
+
+static void segment_dir(const float* p0, const float* p1, float* d, float* l) {
+    float dx = p1[0] - p0[0], dy = p1[1] - p0[1];
+    float len = sqrtf(dx * dx + dy * dy);
+    if (len > 0.0f) {
+        d[0] = dx / len; d[1] = dy / len; *l = len;
+    } else {
+        d[0] = 1.0f; d[1] = 0.0f; *l = 0.0f;
+    }
+}
+
+// Deposit::capsule / stroke (src/film/mod.rs:82-93)
+void cz_deposit_capsule(const float p0[2], const float p1[2], float r, float volume, const uint32_t pig[8], float dx,
+                        cz_deposit* out) {
+    float d[2], l;
+    segment_dir(p0, p1, d, &l);
+    cz_deposit_ribbon(p0, p1, d, d, r, r, CZ_DEPOSIT_CAP_START | CZ_DEPOSIT_CAP_END, volume, pig, dx, out);
+}
+
+void cz_deposit_stroke(const float p0[2], const float p1[2], float r, float volume, const uint32_t pig[8], float dx,
+                       cz_deposit* out) {
+    float d[2], l;
+    segment_dir(p0, p1, d, &l);
+    if (l < std::max(r, 0.8f * dx)) {
+        cz_deposit_capsule(p0, p1, r, volume, pig, dx, out);
+        return;
+    }
+    cz_deposit_ribbon(p0, p1, d, d, r, r, 0, volume, pig, dx, out);
+}
+
+void cz_deposit_laid_by(cz_deposit* d, uint32_t method) { d->kind |= method << CZ_DEPOSIT_METHOD_SHIFT; }
+
+float cz_deposit_sampled_volume(const cz_deposit* d, float lattice) { return sampled_volume(*d, lattice); }
+float cz_deposit_height_at(const cz_deposit* d, float x, float y) { return height_at(*d, x, y); }
+float cz_deposit_inside_fraction(const cz_deposit* d, float w, float h) { return inside_fraction(*d, w, h); }
+
+// PaintMix::normalized (src/color/mod.rs:358-371) and packed (348-356)
+void cz_paint_pack(const float* frac16, int normalize, uint32_t out[8]) {
+    float f[16];
+    memcpy(f, frac16, sizeof(f));
+    if (normalize) {
+        float s = 0.0f;
+        for (int i = 0; i < 13; i++) s += f[i];
+        if (s > 0.0f)
+            for (int i = 0; i < 13; i++) f[i] /= s;
+    }
+    for (int k = 0; k < 8; k++) out[k] = cz_f2h(f[2 * k]) | (cz_f2h(f[2 * k + 1]) << 16);
+}
+
+}  // extern "C"
