@@ -148,6 +148,12 @@ typedef struct cz_stats {
     float cg_report[4];      /* cg_s[1]: (rz0, drift, total, rz_final) */
     double total_deposited;  /* src/film/mod.rs:1188 */
     double weave_volume;     /* src/film/mod.rs:689 */
+    /* running sums over the frames folded so far (since create or
+     * cz_film_reset_counters): sum of active cells, and of active cells
+     * times that frame's substeps */
+    uint64_t act_cells;
+    uint64_t act_cell_substeps;
+    uint64_t frames_counted;
 } cz_stats;
 
 /* The mass report of `compute_stats` (src/main.rs:1679-1819), sums in FP64. */
@@ -301,6 +307,11 @@ int cz_film_use_graphs(cz_film* f, int on);
  * film's stream), in microseconds; label = kernel family name. */
 int cz_film_profile(cz_film* f, int on);
 int cz_film_profile_read(cz_film* f, const char* label, double* avg_us, uint64_t* launches);
+/* Restrict the event timing of cz_film_profile to one kernel family
+ * (NULL or "" = all). */
+int cz_film_profile_only(cz_film* f, const char* label);
+/* Zero the running sums of cz_stats. */
+int cz_film_reset_counters(cz_film* f);
 
 const char* cz_last_error(void);
 const char* cz_version(void);

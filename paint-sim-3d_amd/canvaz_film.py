@@ -87,6 +87,8 @@ def lib():
         "cz_film_use_graphs": (i32, [P, i32]),
         "cz_film_profile": (i32, [P, i32]),
         "cz_film_profile_read": (i32, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]),
+        "cz_film_profile_only": (i32, [P, C.c_char_p]),
+        "cz_film_reset_counters": (i32, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -340,6 +342,12 @@ class Film:
 
     def profile(self, on=True):
         check(lib().cz_film_profile(self._p, 1 if on else 0))
+
+    def profile_only(self, label=None):
+        check(lib().cz_film_profile_only(self._p, label.encode() if label else None))
+
+    def reset_counters(self):
+        check(lib().cz_film_reset_counters(self._p))
 
     def profile_read(self, label):
         avg = C.c_double()

@@ -34,6 +34,10 @@ static int set_err(int code, const char* fmt, ...) {
         if (_e != hipSuccess) return set_err(CZ_EHIP, "%s: %s", #expr, hipGetErrorString(_e)); \
     } while (0)
 
+struct GraphKeyStore {
+    unsigned char bytes[7 * 4 + sizeof(cz_params)];
+};
+
 namespace {
 
 constexpr uint32_t kMaxTileBlocks = 2048u;  // resident 256-thread blocks on 256 CUs
@@ -54,9 +58,11 @@ struct cz_film {
     cz_params P;
     float h_max = 0.0f, mu_min = 1.0f, h_max_pending = 0.0f;
     uint32_t implicit_steps = 1, substeps_last = 1, frame = 0, active_tiles = 0;
+    uint32_t active_hint = 0xffffffffu;  // grid sizing: last folded active count
     double total_deposited = 0.0, weave_volume = 0.0;
     float cg_report[4] = {0, 0, 0, 0};
     bool stats_pending = false;
+    uint64_t cnt_act_cells = 0, cnt_act_cell_substeps = 0, cnt_frames = 0;
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
@@ -91,7 +97,10 @@ struct cz_film {
     uint32_t *tstart_h = nullptr, *tcount_h = nullptr, *titems_h = nullptr;
     uint32_t deposit_count = 0;
     // profiling
+    bool use_graphs = true;
+    std::vector<std::pair<struct GraphKeyStore, hipGraphExec_t>> graphs;
     bool prof_on = false;
+    std::string prof_only;
     std::vector<ProfSlot> prof_pending;
     std::vector<hipEvent_t> ev_pool;
     std::map<std::string, std::pair<double, uint64_t>> prof_acc;
@@ -120,7 +129,18 @@ struct cz_film {
         return F;
     }
 
-    uint32_t tile_grid() const { return std::max(1u, std::min(nt16, kMaxTileBlocks)); }
+    // Grid of the tile kernels: the kernels walk the device-side list with a
+    // grid-stride loop, so any size is correct; size it from the last known
+    // active count (4 work items per tile for the face-parallel flux).
+    uint32_t tile_grid(uint32_t per_tile = 1) const {
+        uint64_t est = (uint64_t)std::max(std::min(active_hint, nt16), 1u) * per_tile;
+        est = est + est / 4u + 16u;
+        uint64_t p2 = 64;  // power of two: few distinct grids, few distinct graphs
+        while (p2 < est) p2 <<= 1;
+        est = p2;
+        uint64_t cap = std::min<uint64_t>((uint64_t)nt16 * per_tile, (uint64_t)kMaxTileBlocks * per_tile);
+        return (uint32_t)std::max<uint64_t>(1u, std::min(est, cap));
+    }
 
     hipEvent_t take_event() {
         if (!ev_pool.empty()) {
@@ -155,14 +175,14 @@ struct Launch {
     const char* label;
     hipEvent_t a = nullptr, b = nullptr;
     Launch(cz_film* ff, const char* l) : f(ff), label(l) {
-        if (f->prof_on) {
+        if (f->prof_on && (f->prof_only.empty() || f->prof_only == l)) {
             a = f->take_event();
             b = f->take_event();
             if (a) hipEventRecord(a, f->stream);
         }
     }
     ~Launch() {
-        if (f->prof_on && a && b) {
+        if (a && b) {
             hipEventRecord(b, f->stream);
             f->prof_pending.push_back({label, a, b});
         }
@@ -241,9 +261,14 @@ int poll_stats(cz_film* f) {
     f->h_max = o[0];
     f->mu_min = (o[1] > 1e8f) ? 1.0f : o[1];
     f->active_tiles = cz_f2bits(o[2]);
+    f->active_hint = f->active_tiles;
     for (int k = 0; k < 4; k++) f->cg_report[k] = o[4 + k];
     f->h_max_pending = 0.0f;
     f->stats_pending = false;
+    uint64_t cells = cz_f2bits(o[3]);
+    f->cnt_act_cells += cells;
+    f->cnt_act_cell_substeps += cells * (uint64_t)f->substeps_last;
+    f->cnt_frames += 1;
     return CZ_OK;
 }
 
@@ -316,8 +341,8 @@ int upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* se
         hipLaunchKernelGGL(kern, grid, block, 0, f->stream, __VA_ARGS__);  \
     } while (0)
 
-// encode_frame (src/film/mod.rs:1322-1462)
-int encode_frame(cz_film* f, uint32_t substeps) {
+// encode_frame (src/film/mod.rs:1322-1462): the launches of one frame.
+int record_frame(cz_film* f, uint32_t substeps) {
     const dim3 b256(256);
     const dim3 gcell((f->W + 15u) / 16u, (f->H + 15u) / 16u);
     const dim3 gt(f->tile_grid());
@@ -351,7 +376,7 @@ int encode_frame(cz_film* f, uint32_t substeps) {
     for (uint32_t s = 0; s < substeps; s++) {
         F = f->dev();
         LAUNCH("lap_plug", k_lap_plug, gt, b256, F, 1);
-        LAUNCH("flux", k_fluxes, gt, b256, F);
+        LAUNCH("flux", k_fluxes, dim3(f->tile_grid(4)), b256, F);
         LAUNCH("limit", k_limiter, gt, b256, F);
         LAUNCH("apply", k_apply, gt, b256, F);
         LAUNCH("nose", k_nose, gt, b256, F);
@@ -374,10 +399,81 @@ int encode_frame(cz_film* f, uint32_t substeps) {
     LAUNCH("frame_stats", k_frame_stats, dim3(1), b256, F);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(f->frame_out_h, f->frame_out, 16 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
+    return CZ_OK;
+}
+
+// Host-side effects of one frame: the offset buffer written last, the stats
+// readback in flight.
+void frame_done(cz_film* f, uint32_t substeps, bool parity_done) {
+    if (!parity_done && (substeps & 1u)) f->delta_cur_a = !f->delta_cur_a;
     f->stats_pending = true;
     f->substeps_last = substeps;
     f->frame++;
     f->deposit_count = 0;
+}
+
+// The frame as a hipGraph, captured once per frame shape and replayed: the
+// shape is everything the launches depend on (substeps, implicit steps,
+// deposit pass, offset-buffer parity, params, grid sizes).
+struct GraphKey {
+    uint32_t substeps, implicit_steps, implicit_on, has_dep, parity, grid1, grid4;
+    cz_params P;
+    bool operator==(const GraphKey& o) const { return memcmp(this, &o, sizeof(GraphKey)) == 0; }
+};
+
+int encode_frame(cz_film* f, uint32_t substeps) {
+    if (!f->use_graphs || f->prof_on) {
+        bool start_a = f->delta_cur_a;
+        int rc = record_frame(f, substeps);
+        if (rc) return rc;
+        (void)start_a;
+        frame_done(f, substeps, true);
+        return CZ_OK;
+    }
+    GraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.substeps = substeps;
+    key.implicit_steps = f->implicit_steps;
+    key.implicit_on = f->s.implicit_hydro;
+    key.has_dep = f->deposit_count > 0;
+    key.parity = f->delta_cur_a ? 1u : 0u;
+    key.grid1 = f->tile_grid(1);
+    key.grid4 = f->tile_grid(4);
+    key.P = f->P;
+    hipGraphExec_t exec = nullptr;
+    for (size_t k = 0; k < f->graphs.size(); k++) {
+        if (memcmp(&f->graphs[k].first, &key, sizeof(key)) == 0) {
+            exec = f->graphs[k].second;
+            // move to the back (most recently used)
+            auto e = f->graphs[k];
+            f->graphs.erase(f->graphs.begin() + (long)k);
+            f->graphs.push_back(e);
+            break;
+        }
+    }
+    bool start_a = f->delta_cur_a;
+    if (!exec) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
+        int rc = record_frame(f, substeps);
+        hipError_t e = hipStreamEndCapture(f->stream, &graph);
+        if (rc) return rc;
+        if (e != hipSuccess) return set_err(CZ_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) return set_err(CZ_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+        if (f->graphs.size() >= 16) {
+            hipGraphExecDestroy(f->graphs.front().second);
+            f->graphs.erase(f->graphs.begin());
+        }
+        GraphKeyStore kk;
+        static_assert(sizeof(GraphKeyStore) == sizeof(GraphKey), "graph key size");
+        memcpy(&kk, &key, sizeof(key));
+        f->graphs.push_back({kk, exec});
+        f->delta_cur_a = start_a;  // record_frame flipped it while capturing
+    }
+    HIP_TRY(hipGraphLaunch(exec, f->stream));
+    frame_done(f, substeps, false);
     return CZ_OK;
 }
 
@@ -449,6 +545,7 @@ void destroy(cz_film* f) {
         hipEventDestroy(s.b);
     }
     for (auto e : f->ev_pool) hipEventDestroy(e);
+    for (auto& g : f->graphs) hipGraphExecDestroy(g.second);
     if (f->stream) hipStreamDestroy(f->stream);
     delete f;
 }
@@ -651,6 +748,9 @@ int cz_film_stats(cz_film* f, cz_stats* st) {
     for (int k = 0; k < 4; k++) st->cg_report[k] = f->stats_pending ? o[4 + k] : f->cg_report[k];
     st->total_deposited = f->total_deposited;
     st->weave_volume = f->weave_volume;
+    st->act_cells = f->cnt_act_cells;
+    st->act_cell_substeps = f->cnt_act_cell_substeps;
+    st->frames_counted = f->cnt_frames;
     return CZ_OK;
 }
 
@@ -737,6 +837,7 @@ int cz_film_write_state(cz_film* f, const void* const parts[CZ_NPARTS], const ui
     HIP_TRY(hipMemsetAsync(f->alpha_s, 0, 16 * n, f->stream));
     std::vector<uint32_t> ones(f->nt16, 1u);
     HIP_TRY(hipMemcpyAsync(f->tile_moved, ones.data(), ones.size() * 4, hipMemcpyHostToDevice, f->stream));
+    f->active_hint = f->nt16;
     return wait_idle(f);
 }
 
@@ -817,7 +918,9 @@ void* cz_film_stream(cz_film* f) { return f ? (void*)f->stream : nullptr; }
 
 int cz_film_use_graphs(cz_film* f, int on) {
     if (!f) return set_err(CZ_EINVAL, "null film");
-    if (on) return set_err(CZ_EUNSUP, "graph replay is not built in this version");
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    f->use_graphs = on != 0;
     return CZ_OK;
 }
 
@@ -827,6 +930,20 @@ int cz_film_profile(cz_film* f, int on) {
     if (rc) return rc;
     f->prof_on = on != 0;
     f->prof_acc.clear();
+    return CZ_OK;
+}
+
+int cz_film_profile_only(cz_film* f, const char* label) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    f->prof_only = label ? label : "";
+    return CZ_OK;
+}
+
+int cz_film_reset_counters(cz_film* f) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    int rc = poll_stats(f);
+    if (rc) return rc;
+    f->cnt_act_cells = f->cnt_act_cell_substeps = f->cnt_frames = 0;
     return CZ_OK;
 }
 

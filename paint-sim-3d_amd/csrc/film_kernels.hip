@@ -350,11 +350,46 @@ __global__ __launch_bounds__(256) void k_cure_clear(FilmDev F, int do_clear) {
 }
 
 // ---------------------------------------------------------------------------
-// laplacian (film.wgsl:16-42) and, when do_plug, plug (film.wgsl:58-87)
+// laplacian (film.wgsl:16-42) and, when do_plug, plug (film.wgsl:58-87).
+// plug's 7x7 window mean of hb_flux(g_tilt, h_n, mu_n, tau_n) only depends on
+// the neighbour's own values, so each block evaluates hb_flux once per cell
+// of its tile plus a 3-cell halo into LDS and then sums the window in the
+// reference's order (rows outer, columns inner).
+constexpr int PLUG_SPAN = 16 + 2 * PLUG_RADIUS;
+
 __global__ __launch_bounds__(256) void k_lap_plug(FilmDev F, int do_plug) {
+    __shared__ float s_q[PLUG_SPAN * PLUG_SPAN];
+    __shared__ float s_h[PLUG_SPAN * PLUG_SPAN];
+    __shared__ uint8_t s_y[PLUG_SPAN * PLUG_SPAN];
     const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
+    const bool plug_on = do_plug && !(gb < 1e-6f);
     CZ_FOR_TILES(F) {
         CZ_TILE_CELL(F)
+        if (plug_on) {
+            const int ox = (int)((_t % F.tw) * 16u) - PLUG_RADIUS, oy = (int)((_t / F.tw) * 16u) - PLUG_RADIUS;
+            __syncthreads();
+            for (uint32_t q = threadIdx.x; q < (uint32_t)(PLUG_SPAN * PLUG_SPAN); q += 256u) {
+                int gx = ox + (int)(q % PLUG_SPAN), gy = oy + (int)(q / PLUG_SPAN);
+                float hn = 0.0f, qv = 0.0f;
+                uint8_t yl = 0;
+                if (gx >= 0 && gy >= 0 && gx < (int)F.W && gy < (int)F.H) {
+                    uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+                    hn = F.h[nn];
+                    if (hn > 0.0f) {
+                        float2 r = F.rheo[nn];
+                        yl = r.y > 0.0f ? 1 : 0;
+                        float oy_, oup_;
+                        qv = hb_flux(F.P.phys[2], gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
+                    } else {
+                        hn = 0.0f;
+                    }
+                }
+                s_q[q] = qv;
+                s_h[q] = hn;
+                s_y[q] = yl;
+            }
+            __syncthreads();
+        }
         if (!in_grid) continue;
         int i = (int)x, j = (int)y;
         float hv = F.h[c];
@@ -375,23 +410,16 @@ __global__ __launch_bounds__(256) void k_lap_plug(FilmDev F, int do_plug) {
         F.pres[c] = make_float2(-F.P.phys[1] * l + disjoining(F, hv), F.P.phys[0] * F.P.g[2] * (s0 - hold) + pl);
         if (!do_plug) continue;
         float tv = 0.0f;
-        if (hv > 0.0f && !(gb < 1e-6f)) {
+        if (hv > 0.0f && plug_on) {
             float sum = 0.0f, mass = 0.0f;
             bool yielding = false;
-            for (int dy = -PLUG_RADIUS; dy <= PLUG_RADIUS; dy++) {
-                int yy = j + dy;
-                if (yy < 0 || yy >= (int)F.H) continue;
-                for (int dxi = -PLUG_RADIUS; dxi <= PLUG_RADIUS; dxi++) {
-                    int xx = i + dxi;
-                    if (xx < 0 || xx >= (int)F.W) continue;
-                    uint32_t nn = idx(F, (uint32_t)xx, (uint32_t)yy);
-                    float hn = F.h[nn];
+            for (int dy = 0; dy < 2 * PLUG_RADIUS + 1; dy++) {
+                for (int dxi = 0; dxi < 2 * PLUG_RADIUS + 1; dxi++) {
+                    int q = (int)(_ly + dy) * PLUG_SPAN + (int)(_lx + dxi);
+                    float hn = s_h[q];
                     if (hn <= 0.0f) continue;
-                    float2 r = F.rheo[nn];
-                    if (r.y > 0.0f) yielding = true;
-                    float oy, oup;
-                    float q = hb_flux(F.P.phys[2], gb, hn, cz_max(r.x, 1e-4f), r.y, oy, oup);
-                    sum = sum + q;
+                    if (s_y[q]) yielding = true;
+                    sum = sum + s_q[q];
                     mass = mass + hn;
                 }
             }
@@ -401,22 +429,28 @@ __global__ __launch_bounds__(256) void k_lap_plug(FilmDev F, int do_plug) {
     }
 }
 
-// fluxes (film.wgsl:375-455)
+// fluxes (film.wgsl:375-455).  The four owned faces of a cell are
+// independent, so each block takes one (tile, face) pair: 4x the waves of a
+// cell-per-thread launch to hide the dependent loads of the contact-line
+// terms.
 __global__ __launch_bounds__(256) void k_fluxes(FilmDev F) {
-    const float wts[4] = {W_AXIS, W_AXIS, W_DIAG_FLUX, W_DIAG_FLUX};
-    CZ_FOR_TILES(F) {
-        CZ_TILE_CELL(F)
-        if (!in_grid) continue;
+    const uint32_t nact = *F.active_count;
+    const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+    float* flux = (float*)F.flux;
+    for (uint32_t wk = blockIdx.x; wk < 4u * nact; wk += gridDim.x) {
+        const uint32_t t = F.tile_list[wk >> 2];
+        const int k = (int)(wk & 3u);
+        const uint32_t x = (t % F.tw) * 16u + lx, y = (t / F.tw) * 16u + ly;
+        if (x >= F.W || y >= F.H) continue;
+        const uint32_t c = idx(F, x, y);
         float2 p00 = pressure_at(F, (int)x, (int)y);
-        float q[4];
-        for (int k = 0; k < 4; k++) {
-            q[k] = 0.0f;
-            FaceGeom fg = face_geom(F, x, y, k, true, p00);
-            if (!fg.has) continue;
+        float q = 0.0f;
+        FaceGeom fg = face_geom(F, x, y, k, true, p00);
+        if (fg.has) {
             FaceFlux ff = face(F, c, fg.cb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
-            q[k] = ff.q * wts[k];
+            q = ff.q * ((k < 2) ? W_AXIS : W_DIAG_FLUX);
         }
-        F.flux[c] = make_float4(q[0], q[1], q[2], q[3]);
+        flux[4u * c + (uint32_t)k] = q;
     }
 }
 
@@ -677,10 +711,12 @@ __global__ __launch_bounds__(256) void k_reduce_pool(FilmDev F) {
 
 // Fold of the per-64-tile stats (poll_stats, src/film/mod.rs:944-951) and the
 // frame scalars the host reads back: [0] h_max, [1] mu_min, [2] active count
-// (bits), [3] unused, [4..7] cg_s[1].
+// (bits), [3] cells of the active tiles (bits), [4..7] cg_s[1].
 __global__ __launch_bounds__(256) void k_frame_stats(FilmDev F) {
     __shared__ float s_h[256];
     __shared__ float s_mu[256];
+    __shared__ uint32_t s_cells;
+    if (threadIdx.x == 0) s_cells = 0u;
     uint32_t nt = F.t64x * F.t64y;
     float hm = 0.0f, mm = 1e9f;
     for (uint32_t t = threadIdx.x; t < nt; t += 256u) {
@@ -688,6 +724,16 @@ __global__ __launch_bounds__(256) void k_frame_stats(FilmDev F) {
         hm = cz_max(hm, v.x);
         mm = cz_min(mm, v.y);
     }
+    uint32_t cells = 0u, nact = *F.active_count;
+    for (uint32_t a = threadIdx.x; a < nact; a += 256u) {
+        uint32_t t = F.tile_list[a];
+        uint32_t tx = t % F.tw, ty = t / F.tw;
+        uint32_t cw = F.W - tx * 16u < 16u ? F.W - tx * 16u : 16u;
+        uint32_t ch = F.H - ty * 16u < 16u ? F.H - ty * 16u : 16u;
+        cells += cw * ch;
+    }
+    __syncthreads();
+    atomicAdd(&s_cells, cells);
     s_h[threadIdx.x] = hm;
     s_mu[threadIdx.x] = mm;
     __syncthreads();
@@ -702,7 +748,7 @@ __global__ __launch_bounds__(256) void k_frame_stats(FilmDev F) {
         F.frame_out[0] = s_h[0];
         F.frame_out[1] = s_mu[0];
         F.frame_out[2] = cz_bits2f(*F.active_count);
-        F.frame_out[3] = 0.0f;
+        F.frame_out[3] = cz_bits2f(s_cells);
         for (int k = 0; k < 4; k++) F.frame_out[4 + k] = F.cg_s[4 + k];
     }
 }
@@ -843,14 +889,25 @@ __global__ __launch_bounds__(256) void k_strata_update(FilmDev F) {
     }
 }
 
-// strata_copy_back (strata.wgsl:276-286)
+// strata_copy_back (strata.wgsl:276-286): the pigment words of a tile row are
+// one 2 KiB run, copied by consecutive threads.
 __global__ __launch_bounds__(256) void k_strata_copy_back(FilmDev F) {
     CZ_FOR_TILES(F) {
         CZ_TILE_CELL(F)
-        if (!in_grid) continue;
-        F.strata_h[c] = F.strata_h2[c];
-        F.strata_cure[c] = F.strata_cure2[c];
-        for (uint32_t k = 0; k < 8u; k++) F.strata_pig[c * 8u + k] = F.strata_pig2[c * 8u + k];
+        if (in_grid) {
+            F.strata_h[c] = F.strata_h2[c];
+            F.strata_cure[c] = F.strata_cure2[c];
+        }
+        const uint32_t x0 = (_t % F.tw) * 16u, y0 = (_t / F.tw) * 16u;
+        for (uint32_t k = 0; k < 8u; k++) {
+            uint32_t q = k * 256u + threadIdx.x;
+            uint32_t row = q >> 7, off = q & 127u;
+            uint32_t xx = x0 + (off >> 3), yy = y0 + row;
+            if (xx < F.W && yy < F.H) {
+                uint32_t base = idx(F, x0, yy) * 8u + off;
+                F.strata_pig[base] = F.strata_pig2[base];
+            }
+        }
     }
 }
 
