@@ -29,11 +29,9 @@ HBM_PEAK_GBS = 8000.0
 
 # Compulsory HBM bytes per active cell of one launch of each kernel family.
 ALGO_BYTES = {
-    "lap_plug": 4 + 4 + 4 + 8 + 8 + 8 + 4,  # R h m delta fnorm rheo, W pres tmp
-    "flux": 4 + 8 + 8 + 4 + 4 + 8 + 16,     # R h rheo pres tmp delta fnorm, W flux
-    "limit": 16 + 4 + 4,                     # R flux h, W alpha
-    "apply": 4 + 16 + 4 + 16 + 4 + 16,       # R alpha flux h acc, W h acc
-    "nose": 4 + 4 + 4 + 8,                   # R h delta, W delta fnorm
+    "lap_plug": 4 + 4 + 4 + 8 + 8 + 4 + 8 + 8 + 4,  # R h m delta fnorm rheo, W delta fnorm pres tmp
+    "flux": 4 + 8 + 8 + 4 + 4 + 8 + 16,             # R h rheo pres tmp delta fnorm, W flux
+    "apply": 16 + 4 + 16 + 4 + 16,                  # R flux h acc, W h acc
 }
 SUBSTEP_FAMILIES = tuple(ALGO_BYTES)
 

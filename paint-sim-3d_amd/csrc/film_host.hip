@@ -354,7 +354,7 @@ int record_frame(cz_film* f, uint32_t substeps) {
     if (f->s.implicit_hydro) {
         for (uint32_t it = 0; it < f->implicit_steps; it++) {
             F = f->dev();
-            LAUNCH("lap", k_lap_plug, gt, b256, F, 0);
+            LAUNCH("lap", k_lap_plug, gt, b256, F, 0, 0);
             LAUNCH("cg_setup", k_cg_setup, gt, b256, F);
             LAUNCH("cg_lap", k_cg_lap, gt, b256, F, 1u);
             LAUNCH("cg_init", k_cg_init, gt, b256, F);
@@ -373,18 +373,23 @@ int record_frame(cz_film* f, uint32_t substeps) {
             LAUNCH("cg_rescale", k_cg_rescale, gt, b256, F);
         }
     }
+    // substeps: [nose of the previous substep +] lap + plug, flux, limiter + apply
     for (uint32_t s = 0; s < substeps; s++) {
         F = f->dev();
-        LAUNCH("lap_plug", k_lap_plug, gt, b256, F, 1);
+        LAUNCH("lap_plug", k_lap_plug, gt, b256, F, s > 0 ? 1 : 0, 1);
+        if (s > 0) {
+            f->delta_cur_a = !f->delta_cur_a;
+            F = f->dev();
+        }
         LAUNCH("flux", k_fluxes, dim3(f->tile_grid(4)), b256, F);
-        LAUNCH("limit", k_limiter, gt, b256, F);
-        LAUNCH("apply", k_apply, gt, b256, F);
-        LAUNCH("nose", k_nose, gt, b256, F);
-        f->delta_cur_a = !f->delta_cur_a;
+        LAUNCH("apply", k_limit_apply, gt, b256, F);
     }
+    // nose of the last substep + the laplacian strata_split reads
+    F = f->dev();
+    LAUNCH("lap", k_lap_plug, gt, b256, F, 1, 0);
+    f->delta_cur_a = !f->delta_cur_a;
     F = f->dev();
     LAUNCH("delta_sync", k_delta_sync, gt, b256, F, f->delta_other());
-    LAUNCH("lap", k_lap_plug, gt, b256, F, 0);
     LAUNCH("split", k_strata_split, gt, b256, F);
     LAUNCH("reduce_pool", k_reduce_pool, dim3(f->t64x, f->t64y), b256, F);
     LAUNCH("strata_limit", k_strata_limit, gt, b256, F);

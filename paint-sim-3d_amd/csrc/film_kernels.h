@@ -7,11 +7,9 @@ __global__ void k_deposit(FilmDev F);
 __global__ void k_mark_tiles(FilmDev F);
 __global__ void k_compact_tiles(FilmDev F);
 __global__ void k_cure_clear(FilmDev F, int do_clear);
-__global__ void k_lap_plug(FilmDev F, int do_plug);
+__global__ void k_lap_plug(FilmDev F, int do_nose, int do_plug);
 __global__ void k_fluxes(FilmDev F);
-__global__ void k_limiter(FilmDev F);
-__global__ void k_apply(FilmDev F);
-__global__ void k_nose(FilmDev F);
+__global__ void k_limit_apply(FilmDev F);
 __global__ void k_delta_sync(FilmDev F, uint32_t* dst);
 __global__ void k_strata_split(FilmDev F);
 __global__ void k_reduce_pool(FilmDev F);

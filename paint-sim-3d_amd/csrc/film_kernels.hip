@@ -350,24 +350,87 @@ __global__ __launch_bounds__(256) void k_cure_clear(FilmDev F, int do_clear) {
 }
 
 // ---------------------------------------------------------------------------
-// laplacian (film.wgsl:16-42) and, when do_plug, plug (film.wgsl:58-87).
-// plug's 7x7 window mean of hb_flux(g_tilt, h_n, mu_n, tau_n) only depends on
-// the neighbour's own values, so each block evaluates hb_flux once per cell
-// of its tile plus a 3-cell halo into LDS and then sums the window in the
-// reference's order (rows outer, columns inner).
-constexpr int PLUG_SPAN = 16 + 2 * PLUG_RADIUS;
+// nose (film.wgsl:680-723) of cell (x, y): offset to the nearest dry cell
+// from the offsets of the previous substep (F.delta) and the heights after
+// apply, and the front normal within 3 cells.
+__device__ inline void nose_cell(const FilmDev& F, uint32_t x, uint32_t y, uint32_t& d_out, float2& fn_out) {
+    uint32_t c = idx(F, x, y);
+    d_out = 0u;
+    fn_out = make_float2(0.0f, 0.0f);
+    if (F.h[c] <= 0.0f) return;
+    int i = (int)x, j = (int)y;
+    int bx = 64, by = 64;
+    float bd = 1e9f;
+    for (int dj = -1; dj <= 1; dj++) {
+        for (int di = -1; di <= 1; di++) {
+            int xx = i + di, yy = j + dj;
+            if (xx < 0 || yy < 0 || xx >= (int)F.W || yy >= (int)F.H) continue;
+            uint32_t nn = idx(F, (uint32_t)xx, (uint32_t)yy);
+            int cx = di, cy = dj;
+            if (F.h[nn] > 0.0f) {
+                uint32_t dv = F.delta[nn];
+                if ((dv & 0x10000u) == 0u) continue;
+                int ox, oy;
+                unpack_off(dv, ox, oy);
+                cx = ox + di;
+                cy = oy + dj;
+                int tx = i + cx, ty = j + cy;
+                if (tx < 0 || ty < 0 || tx >= (int)F.W || ty >= (int)F.H) continue;
+                if (F.h[idx(F, (uint32_t)tx, (uint32_t)ty)] > 0.0f) continue;
+            } else if (di == 0 && dj == 0) {
+                continue;
+            }
+            float d = cz_len2((float)cx, (float)cy);
+            if (d < bd) { bd = d; bx = cx; by = cy; }
+        }
+    }
+    if (bd > 1e8f) return;
+    d_out = pack_off(bx, by);
+    if (bd <= 3.0f) fn_out = front_normal(F, c);
+}
 
-__global__ __launch_bounds__(256) void k_lap_plug(FilmDev F, int do_plug) {
+// One substep's first phase on a tile: [nose of the previous substep]
+// (film.wgsl:680-723) + laplacian (film.wgsl:16-42) + plug (film.wgsl:58-87).
+// nose is evaluated for the tile and a 1-cell ring (active cells; the ring is
+// recomputed identically by the neighbouring tile's block) into LDS, since
+// the laplacian of a cell reads the new offsets and front normals of its 8
+// neighbours; the tile's own results go to F.delta_out / F.fnorm.  plug's
+// 7x7 window mean of hb_flux(g_tilt, h_n, mu_n, tau_n) depends on the
+// neighbour's own values only: hb_flux is evaluated once per cell of the tile
+// and a 3-cell ring into LDS and the window summed in the reference's order.
+constexpr int PLUG_SPAN = 16 + 2 * PLUG_RADIUS;
+constexpr int RING_SPAN = 18;
+
+__global__ __launch_bounds__(256) void k_lap_plug(FilmDev F, int do_nose, int do_plug) {
     __shared__ float s_q[PLUG_SPAN * PLUG_SPAN];
     __shared__ float s_h[PLUG_SPAN * PLUG_SPAN];
     __shared__ uint8_t s_y[PLUG_SPAN * PLUG_SPAN];
+    __shared__ uint32_t s_d[RING_SPAN * RING_SPAN];
+    __shared__ float2 s_fn[RING_SPAN * RING_SPAN];
     const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
     const bool plug_on = do_plug && !(gb < 1e-6f);
     CZ_FOR_TILES(F) {
         CZ_TILE_CELL(F)
+        const int tx0 = (int)((_t % F.tw) * 16u), ty0 = (int)((_t / F.tw) * 16u);
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
+            int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
+            uint32_t dv = 0u;
+            float2 fn = make_float2(0.0f, 0.0f);
+            if (gx >= 0 && gy >= 0 && gx < (int)F.W && gy < (int)F.H) {
+                uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+                if (do_nose && cell_active(F, (uint32_t)gx, (uint32_t)gy)) {
+                    nose_cell(F, (uint32_t)gx, (uint32_t)gy, dv, fn);
+                } else {
+                    dv = F.delta[nn];
+                    fn = F.fnorm[nn];
+                }
+            }
+            s_d[q] = dv;
+            s_fn[q] = fn;
+        }
         if (plug_on) {
-            const int ox = (int)((_t % F.tw) * 16u) - PLUG_RADIUS, oy = (int)((_t / F.tw) * 16u) - PLUG_RADIUS;
-            __syncthreads();
+            const int ox = tx0 - PLUG_RADIUS, oy = ty0 - PLUG_RADIUS;
             for (uint32_t q = threadIdx.x; q < (uint32_t)(PLUG_SPAN * PLUG_SPAN); q += 256u) {
                 int gx = ox + (int)(q % PLUG_SPAN), gy = oy + (int)(q / PLUG_SPAN);
                 float hn = 0.0f, qv = 0.0f;
@@ -388,23 +451,51 @@ __global__ __launch_bounds__(256) void k_lap_plug(FilmDev F, int do_plug) {
                 s_h[q] = hn;
                 s_y[q] = yl;
             }
-            __syncthreads();
         }
+        __syncthreads();
         if (!in_grid) continue;
+        const int own = (int)(_ly + 1u) * RING_SPAN + (int)(_lx + 1u);
+        if (do_nose) {
+            F.delta_out[c] = s_d[own];
+            F.fnorm[c] = s_fn[own];
+        }
         int i = (int)x, j = (int)y;
+        // usurf (film.wgsl:147-152) with the offsets from LDS
+        auto usurf_l = [&](int ii, int jj) -> float {
+            uint32_t ri = refl_i(F, ii), rj = refl_j(F, jj);
+            uint32_t cc = idx(F, ri, rj);
+            float hc = F.h[cc];
+            float hold = 0.0f;
+            if (hc > 0.0f) hold = hold_level(F, nose_dist_b(s_d[((int)rj - ty0 + 1) * RING_SPAN + ((int)ri - tx0 + 1)]));
+            return hc + F.m[cc] - hold;
+        };
         float hv = F.h[c];
         float s0 = hv + F.m[c];
         float hold = 0.0f, pl = 0.0f;
-        if (hv > 0.0f) hold = hold_level(F, nose_dist(F, c));
+        float dself = nose_dist_b(s_d[own]);
+        if (hv > 0.0f) hold = hold_level(F, dself);
         float u0 = s0 - hold;
-        float ax = usurf(F, i - 1, j) + usurf(F, i + 1, j) + usurf(F, i, j - 1) + usurf(F, i, j + 1);
-        float dg = usurf(F, i - 1, j - 1) + usurf(F, i + 1, j - 1) + usurf(F, i - 1, j + 1) + usurf(F, i + 1, j + 1);
+        float ax = usurf_l(i - 1, j) + usurf_l(i + 1, j) + usurf_l(i, j - 1) + usurf_l(i, j + 1);
+        float dg = usurf_l(i - 1, j - 1) + usurf_l(i + 1, j - 1) + usurf_l(i - 1, j + 1) + usurf_l(i + 1, j + 1);
         float l = (W_AXIS * ax + W_DIAG_LAP * dg - (4.0f * W_AXIS + 4.0f * W_DIAG_LAP) * u0) * F.P.inv_dx * F.P.inv_dx;
         if (hv > 0.0f) {
-            float d = nose_dist(F, c);
+            float d = dself;
             if (d <= 3.0f) {
-                float2 nn = F.fnorm[c];
-                if (nn.x * nn.x + nn.y * nn.y > 0.5f) pl = line_pressure(F, x, y) * cz_expf(-(d - 1.0f) * 0.7f);
+                float2 nn = s_fn[own];
+                if (nn.x * nn.x + nn.y * nn.y > 0.5f) {
+                    // line_pressure (film.wgsl:190-201) with the normals from LDS
+                    float lp = 0.0f;
+                    if (!(x < 1u || y < 1u || x + 1u >= F.W || y + 1u >= F.H)) {
+                        float2 nl = s_fn[own - 1], nr = s_fn[own + 1];
+                        float2 nd = s_fn[own - RING_SPAN], nu = s_fn[own + RING_SPAN];
+                        if (!(nl.x * nl.x + nl.y * nl.y < 0.5f || nr.x * nr.x + nr.y * nr.y < 0.5f ||
+                              nd.x * nd.x + nd.y * nd.y < 0.5f || nu.x * nu.x + nu.y * nu.y < 0.5f)) {
+                            float curv = (nr.x - nl.x + nu.y - nd.y) * 0.5f * F.P.inv_dx;
+                            lp = F.P.phys[1] * curv * F.P.front2[2];
+                        }
+                    }
+                    pl = lp * cz_expf(-(d - 1.0f) * 0.7f);
+                }
             }
         }
         F.pres[c] = make_float2(-F.P.phys[1] * l + disjoining(F, hv), F.P.phys[0] * F.P.g[2] * (s0 - hold) + pl);
@@ -459,41 +550,22 @@ __device__ __forceinline__ bool nb_ok(const FilmDev& F, int i, int j) {
     return cell_active(F, (uint32_t)i, (uint32_t)j);
 }
 
-// limiter (film.wgsl:521-537)
-__global__ __launch_bounds__(256) void k_limiter(FilmDev F) {
-    CZ_FOR_TILES(F) {
-        CZ_TILE_CELL(F)
-        if (!in_grid) continue;
-        int i = (int)x, j = (int)y;
-        float4 f = F.flux[c];
-        float out = cz_max(f.x, 0.0f) + cz_max(f.y, 0.0f) + cz_max(f.z, 0.0f) + cz_max(f.w, 0.0f);
-        if (nb_ok(F, i - 1, j)) out = out + cz_max(-F.flux[c - 1u].x, 0.0f);
-        if (nb_ok(F, i, j - 1)) out = out + cz_max(-F.flux[c - F.W].y, 0.0f);
-        if (nb_ok(F, i - 1, j - 1)) out = out + cz_max(-F.flux[c - F.W - 1u].z, 0.0f);
-        if (nb_ok(F, i - 1, j + 1)) out = out + cz_max(-F.flux[c + F.W - 1u].w, 0.0f);
-        float a = 1.0f;
-        if (out > 0.0f) a = cz_min(1.0f, 0.5f * F.h[c] * F.P.dx / (F.P.dt * out));
-        F.alpha[c] = a;
-    }
+// limiter (film.wgsl:521-537) of cell (x, y)
+__device__ inline float limiter_cell(const FilmDev& F, uint32_t x, uint32_t y) {
+    uint32_t c = idx(F, x, y);
+    int i = (int)x, j = (int)y;
+    float4 f = F.flux[c];
+    float out = cz_max(f.x, 0.0f) + cz_max(f.y, 0.0f) + cz_max(f.z, 0.0f) + cz_max(f.w, 0.0f);
+    if (nb_ok(F, i - 1, j)) out = out + cz_max(-F.flux[c - 1u].x, 0.0f);
+    if (nb_ok(F, i, j - 1)) out = out + cz_max(-F.flux[c - F.W].y, 0.0f);
+    if (nb_ok(F, i - 1, j - 1)) out = out + cz_max(-F.flux[c - F.W - 1u].z, 0.0f);
+    if (nb_ok(F, i - 1, j + 1)) out = out + cz_max(-F.flux[c + F.W - 1u].w, 0.0f);
+    float a = 1.0f;
+    if (out > 0.0f) a = cz_min(1.0f, 0.5f * F.h[c] * F.P.dx / (F.P.dt * out));
+    return a;
 }
 
 struct Inflow { float best; uint32_t src; float div; float accv; };
-
-__device__ __forceinline__ void own_face(const FilmDev& F, float q, uint32_t cb, bool has, float ac, Inflow& st) {
-    float ab = ac;
-    if (q < 0.0f && has) {
-        ab = F.alpha[cb];
-        if (-q * ab > st.best) { st.best = -q * ab; st.src = cb; }
-    }
-    st.div = st.div + q * ab;
-    st.accv = q * ab;
-}
-__device__ __forceinline__ void nb_face(const FilmDev& F, float q, uint32_t cn, float ac, Inflow& st) {
-    float an = F.alpha[cn];
-    if (q < 0.0f) an = ac;
-    if (q * an > st.best) { st.best = q * an; st.src = cn; }
-    st.div = st.div - q * an;
-}
 
 // Copy of the top stratum (and rheology) of src into the newly wetted cell c
 // (film.wgsl:847-860, strata.wgsl:312-322).
@@ -517,32 +589,66 @@ __device__ inline void inherit_from(const FilmDev& F, uint32_t c, uint32_t src, 
     F.strata_cure[c] = make_float4(cs, 0.0f, 0.0f, 0.0f);
 }
 
-// apply (film.wgsl:803-863)
-__global__ __launch_bounds__(256) void k_apply(FilmDev F) {
+// limiter + apply (film.wgsl:521-537, 803-863) on a tile.  apply reads the
+// limiter factor of the cell and of its 8 neighbours, so the block evaluates
+// the limiter for the tile and a 1-cell ring (active cells) into LDS; the
+// factors never leave the chip.
+__global__ __launch_bounds__(256) void k_limit_apply(FilmDev F) {
+    __shared__ float s_a[RING_SPAN * RING_SPAN];
     CZ_FOR_TILES(F) {
         CZ_TILE_CELL(F)
+        const int tx0 = (int)((_t % F.tw) * 16u), ty0 = (int)((_t / F.tw) * 16u);
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
+            int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
+            float a = 0.0f;
+            if (gx >= 0 && gy >= 0 && gx < (int)F.W && gy < (int)F.H && cell_active(F, (uint32_t)gx, (uint32_t)gy))
+                a = limiter_cell(F, (uint32_t)gx, (uint32_t)gy);
+            s_a[q] = a;
+        }
+        __syncthreads();
         bool moved = false;
         if (in_grid) {
+            auto alpha_at = [&](uint32_t cc) -> float {
+                int xx = (int)(cc % F.W), yy = (int)(cc / F.W);
+                return s_a[(yy - ty0 + 1) * RING_SPAN + (xx - tx0 + 1)];
+            };
             int i = (int)x, j = (int)y;
-            float ac = F.alpha[c];
+            float ac = s_a[(int)(_ly + 1u) * RING_SPAN + (int)(_lx + 1u)];
             Inflow st = {0.0f, c, 0.0f, 0.0f};
             float4 f = F.flux[c];
             bool hx = x + 1u < F.W, hy = y + 1u < F.H, hm = y > 0u;
             uint32_t xp = (x + 1u < F.W - 1u) ? x + 1u : F.W - 1u;
             uint32_t yp = (y + 1u < F.H - 1u) ? y + 1u : F.H - 1u;
             float accv[4];
-            own_face(F, f.x, hx ? idx(F, xp, y) : c, hx, ac, st);
-            accv[0] = st.accv;
-            own_face(F, f.y, hy ? idx(F, x, yp) : c, hy, ac, st);
-            accv[1] = st.accv;
-            own_face(F, f.z, (hx && hy) ? idx(F, xp, yp) : c, hx && hy, ac, st);
-            accv[2] = st.accv;
-            own_face(F, f.w, (hx && hm) ? idx(F, xp, hm ? y - 1u : 0u) : c, hx && hm, ac, st);
-            accv[3] = st.accv;
-            if (nb_ok(F, i - 1, j)) { uint32_t cn = c - 1u; nb_face(F, F.flux[cn].x, cn, ac, st); }
-            if (nb_ok(F, i, j - 1)) { uint32_t cn = c - F.W; nb_face(F, F.flux[cn].y, cn, ac, st); }
-            if (nb_ok(F, i - 1, j - 1)) { uint32_t cn = c - F.W - 1u; nb_face(F, F.flux[cn].z, cn, ac, st); }
-            if (nb_ok(F, i - 1, j + 1)) { uint32_t cn = c + F.W - 1u; nb_face(F, F.flux[cn].w, cn, ac, st); }
+            const float fq[4] = {f.x, f.y, f.z, f.w};
+            const bool has[4] = {hx, hy, hx && hy, hx && hm};
+            const uint32_t cbs[4] = {hx ? idx(F, xp, y) : c, hy ? idx(F, x, yp) : c, (hx && hy) ? idx(F, xp, yp) : c,
+                                     (hx && hm) ? idx(F, xp, hm ? y - 1u : 0u) : c};
+            // own_face (film.wgsl:546-554)
+            for (int k = 0; k < 4; k++) {
+                float q = fq[k];
+                float ab = ac;
+                if (q < 0.0f && has[k]) {
+                    ab = alpha_at(cbs[k]);
+                    if (-q * ab > st.best) { st.best = -q * ab; st.src = cbs[k]; }
+                }
+                st.div = st.div + q * ab;
+                accv[k] = q * ab;
+            }
+            // nb_face (film.wgsl:556-561)
+            const bool nbk[4] = {nb_ok(F, i - 1, j), nb_ok(F, i, j - 1), nb_ok(F, i - 1, j - 1), nb_ok(F, i - 1, j + 1)};
+            const uint32_t cns[4] = {c - 1u, c - F.W, c - F.W - 1u, c + F.W - 1u};
+            for (int k = 0; k < 4; k++) {
+                if (!nbk[k]) continue;
+                uint32_t cn = cns[k];
+                float4 fn = F.flux[cn];
+                float q = (k == 0) ? fn.x : (k == 1) ? fn.y : (k == 2) ? fn.z : fn.w;
+                float an = alpha_at(cn);
+                if (q < 0.0f) an = ac;
+                if (q * an > st.best) { st.best = q * an; st.src = cn; }
+                st.div = st.div - q * an;
+            }
             uint32_t src = st.src;
             float h0 = F.h[c];
             float hn = cz_max(h0 - F.P.dt * st.div * F.P.inv_dx, 0.0f);
@@ -555,52 +661,6 @@ __global__ __launch_bounds__(256) void k_apply(FilmDev F) {
                                    a.w + accv[3] * F.P.dt);
         }
         if (__syncthreads_or(moved ? 1 : 0) && threadIdx.x == 0) F.tile_moved[_t] = 1u;
-    }
-}
-
-// nose (film.wgsl:680-723): reads F.delta, writes F.delta_out
-__global__ __launch_bounds__(256) void k_nose(FilmDev F) {
-    CZ_FOR_TILES(F) {
-        CZ_TILE_CELL(F)
-        if (!in_grid) continue;
-        if (F.h[c] <= 0.0f) {
-            F.delta_out[c] = 0u;
-            F.fnorm[c] = make_float2(0.0f, 0.0f);
-            continue;
-        }
-        int i = (int)x, j = (int)y;
-        int bx = 64, by = 64;
-        float bd = 1e9f;
-        for (int dj = -1; dj <= 1; dj++) {
-            for (int di = -1; di <= 1; di++) {
-                int xx = i + di, yy = j + dj;
-                if (xx < 0 || yy < 0 || xx >= (int)F.W || yy >= (int)F.H) continue;
-                uint32_t nn = idx(F, (uint32_t)xx, (uint32_t)yy);
-                int cx = di, cy = dj;
-                if (F.h[nn] > 0.0f) {
-                    uint32_t dv = F.delta[nn];
-                    if ((dv & 0x10000u) == 0u) continue;
-                    int ox, oy;
-                    unpack_off(dv, ox, oy);
-                    cx = ox + di;
-                    cy = oy + dj;
-                    int tx = i + cx, ty = j + cy;
-                    if (tx < 0 || ty < 0 || tx >= (int)F.W || ty >= (int)F.H) continue;
-                    if (F.h[idx(F, (uint32_t)tx, (uint32_t)ty)] > 0.0f) continue;
-                } else if (di == 0 && dj == 0) {
-                    continue;
-                }
-                float d = cz_len2((float)cx, (float)cy);
-                if (d < bd) { bd = d; bx = cx; by = cy; }
-            }
-        }
-        if (bd > 1e8f) {
-            F.delta_out[c] = 0u;
-            F.fnorm[c] = make_float2(0.0f, 0.0f);
-            continue;
-        }
-        F.delta_out[c] = pack_off(bx, by);
-        F.fnorm[c] = (bd <= 3.0f) ? front_normal(F, c) : make_float2(0.0f, 0.0f);
     }
 }
 
