@@ -241,6 +241,20 @@ float cz_deposit_height_at(const cz_deposit* d, float x, float y);
 float cz_deposit_inside_fraction(const cz_deposit* d, float w, float h);
 /* PaintMix::normalized().packed() (src/color/mod.rs:348-371) of frac[16]. */
 void cz_paint_pack(const float* frac16, int normalize, uint32_t out[8]);
+/* The lagged per-frame stats the host side of the step keeps
+ * (Film.h_max / mu_min / h_max_pending, src/film/mod.rs:855). */
+typedef struct cz_host_state {
+    float h_max;
+    float mu_min;
+    float h_max_pending;
+    uint32_t pad;
+} cz_host_state;
+/* gravity_vec, update_params, choose_substeps (src/film/mod.rs:861-933) as
+ * pure host functions of (settings, toggles, grid, stats). */
+void cz_gravity_vec_host(const cz_settings* s, float out[3]);
+void cz_update_params_host(const cz_settings* s, const cz_overrides* o, uint32_t w, uint32_t h, float frame_dt,
+                           uint32_t substeps, const cz_host_state* hs, cz_params* out, uint32_t* implicit_steps);
+uint32_t cz_choose_substeps_host(const cz_settings* s, uint32_t w, float frame_dt, const cz_host_state* hs);
 
 /* ---- the film (Film, src/film/mod.rs:445-1462) ------------------------- */
 

@@ -2377,6 +2377,12 @@ void of_poll_stats(ofilm* F) {
     for (int k = 0; k < 4; k++) F->cg_report[k] = F->cg_s[4 + k];
 }
 
+void of_set_host_state(ofilm* F, float h_max, float mu_min, float pending) {
+    F->h_max = h_max;
+    F->mu_min = mu_min;
+    F->h_max_pending = pending;
+}
+
 void of_host_state(const ofilm* F, float* hmax_mumin_pending3, double* tot_weave2) {
     hmax_mumin_pending3[0] = F->h_max;
     hmax_mumin_pending3[1] = F->mu_min;

@@ -70,6 +70,7 @@ def lib():
         L.of_step.argtypes = [P, C.POINTER(Deposit), u32, f32]
         L.of_poll_stats.argtypes = [P]
         L.of_host_state.argtypes = [P, C.POINTER(f32), C.POINTER(C.c_double)]
+        L.of_set_host_state.argtypes = [P, f32, f32, f32]
         L.of_mass_report.argtypes = [P, C.POINTER(MassReport)]
         for k in ("of_k_mark_compact", "of_k_laplacian", "of_k_plug", "of_k_fluxes", "of_k_cure"):
             getattr(L, k).argtypes = [P]
@@ -195,6 +196,9 @@ class OracleFilm:
         lib().of_host_state(self._p, a, b)
         return {"h_max": a[0], "mu_min": a[1], "h_max_pending": a[2],
                 "total_deposited": b[0], "weave_volume": b[1]}
+
+    def set_host_state(self, h_max, mu_min, pending):
+        lib().of_set_host_state(self._p, h_max, mu_min, pending)
 
     def update_params(self, frame_dt, substeps):
         lib().of_update_params(self._p, frame_dt, substeps)

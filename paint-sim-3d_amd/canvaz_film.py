@@ -14,7 +14,7 @@ import os
 
 import numpy as np
 
-from cz_types import (Deposit, MassReport, Overrides, Params, Settings, Stats, PART_NAMES,
+from cz_types import (Deposit, HostState, MassReport, Overrides, Params, Settings, Stats, PART_NAMES,
                       default_overrides, default_settings, deposit_array)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -56,6 +56,10 @@ def lib():
         "cz_deposit_height_at": (f32, [C.POINTER(Deposit), f32, f32]),
         "cz_deposit_inside_fraction": (f32, [C.POINTER(Deposit), f32, f32]),
         "cz_paint_pack": (None, [fp, i32, up]),
+        "cz_gravity_vec_host": (None, [C.POINTER(Settings), fp]),
+        "cz_update_params_host": (None, [C.POINTER(Settings), C.POINTER(Overrides), u32, u32, f32, u32,
+                                         C.POINTER(HostState), C.POINTER(Params), up]),
+        "cz_choose_substeps_host": (u32, [C.POINTER(Settings), u32, f32, C.POINTER(HostState)]),
         "cz_film_create": (i32, [u32, u32, C.POINTER(Settings), fp, fp, C.POINTER(P)]),
         "cz_film_destroy": (None, [P]),
         "cz_film_dims": (i32, [P, up, up, fp]),
@@ -143,6 +147,21 @@ def paint_pack(frac16, normalize=True):
     out = np.zeros(8, np.uint32)
     lib().cz_paint_pack(_fp(f), 1 if normalize else 0, _up(out))
     return out
+
+
+def update_params_host(settings, overrides, w, h, frame_dt, substeps, h_max, mu_min, pending):
+    """update_params (src/film/mod.rs:869-917) without a device: (Params, implicit_steps)."""
+    hs = HostState(h_max, mu_min, pending, 0)
+    p = Params()
+    ni = C.c_uint32()
+    lib().cz_update_params_host(C.byref(settings), C.byref(overrides), w, h, frame_dt, substeps, C.byref(hs),
+                                C.byref(p), C.byref(ni))
+    return p, ni.value
+
+
+def choose_substeps_host(settings, w, frame_dt, h_max, mu_min, pending):
+    hs = HostState(h_max, mu_min, pending, 0)
+    return lib().cz_choose_substeps_host(C.byref(settings), w, frame_dt, C.byref(hs))
 
 
 def _v2(p):

@@ -355,6 +355,87 @@ float cz_deposit_sampled_volume(const cz_deposit* d, float lattice) { return sam
 float cz_deposit_height_at(const cz_deposit* d, float x, float y) { return height_at(*d, x, y); }
 float cz_deposit_inside_fraction(const cz_deposit* d, float w, float h) { return inside_fraction(*d, w, h); }
 
+// gravity_vec (src/film/mod.rs:861-867)
+void cz_gravity_vec_host(const cz_settings* s, float out[3]) {
+    float tilt = to_radians(s->tilt_deg);
+    float dir = to_radians(s->tilt_dir_deg);
+    float gxy = s->gravity * sinf(tilt);
+    out[0] = gxy * cosf(dir);
+    out[1] = gxy * sinf(dir);
+    out[2] = fmaxf(s->gravity * cosf(tilt), 0.0f);
+}
+
+// update_params (src/film/mod.rs:869-917): the uniform block of one frame.
+void cz_update_params_host(const cz_settings* sp, const cz_overrides* ov, uint32_t w, uint32_t h, float frame_dt,
+                           uint32_t substeps, const cz_host_state* hs, cz_params* P, uint32_t* implicit_steps) {
+    const cz_settings& s = *sp;
+    float g[3];
+    cz_gravity_vec_host(sp, g);
+    float dx = s.width_m / (float)w;
+    float theta = to_radians(s.contact_angle_deg);
+    float pi_a = 3.0f * s.sigma * (1.0f - cosf(theta)) / s.h_star;
+    float dt = frame_dt / (float)substeps;
+    float m_cap_exp = 0.5f * dx * dx * dx * dx / (32.0f * s.sigma * dt);
+    float m_cap = fmaxf(fminf(s.m_cap, m_cap_exp), 1e-14f);
+    float m_exp = 0.1f * dx * dx / (dt * s.rho * fmaxf(g[2], 0.1f));
+    P->size[0] = w; P->size[1] = h;
+    P->tiles[0] = (w + 63u) / 64u; P->tiles[1] = (h + 63u) / 64u;
+    P->dx = dx; P->inv_dx = 1.0f / dx;
+    P->dt = dt;
+    P->frame_dt = frame_dt;
+    P->g[0] = g[0]; P->g[1] = g[1]; P->g[2] = g[2]; P->g[3] = s.gravity;
+    P->phys[0] = s.rho; P->phys[1] = s.sigma; P->phys[2] = s.n_index;
+    P->phys[3] = (s.capillary && !ov->no_cap) ? m_cap : 0.0f;
+    float lc = sqrtf(s.sigma / (s.rho * 9.81f));
+    P->ein[0] = fmaxf(tanf(theta), 0.05f); P->ein[1] = lc * sinf(0.5f * theta); P->ein[2] = pi_a; P->ein[3] = s.h_min;
+    P->cure[0] = s.cure_rate; P->cure[1] = s.tau_cure; P->cure[2] = s.lnmu_cure; P->cure[3] = s.h_star;
+    P->misc[0] = 2.0e-6f; P->misc[1] = 0.12f; P->misc[2] = s.medium_mu; P->misc[3] = fminf(m_exp, s.m_cap_hydro);
+    P->flags[0] = s.debug_view; P->flags[1] = s.slump; P->flags[2] = 0; P->flags[3] = 0;
+    float lc_z = sqrtf(s.sigma / (s.rho * fmaxf(g[2], 0.1f)));
+    float cap_explicit = fminf(m_exp, s.m_cap_hydro);
+    float h_big = fmaxf(fmaxf(hs->h_max, hs->h_max_pending), 1e-4f);
+    float m_phys = h_big * h_big * h_big / (3.0f * fmaxf(hs->mu_min, 1e-3f)) - cap_explicit;
+    float k_max = 40.0f;
+    float kappa_frame = frame_dt * s.rho * fmaxf(g[2], 0.1f) / (dx * dx);
+    uint32_t n_i = 0;
+    if (m_phys > 0.0f) n_i = std::min((uint32_t)fmaxf(ceilf(kappa_frame * m_phys / k_max), 1.0f), 4u);
+    float dt_i = frame_dt / (float)std::max(n_i, 1u);
+    float m_imp_max = k_max * dx * dx / (dt_i * s.rho * fmaxf(g[2], 0.1f));
+    *implicit_steps = n_i;
+    float m_phys_cap = h_big * h_big * h_big / (3.0f * fmaxf(hs->mu_min, 1e-3f));
+    float m_cap_impl = (ov->capcond / 28.0f) * powi4(dx) / (dt_i * s.sigma);
+    float m_c = s.capillary_implicit ? fminf(m_phys_cap, m_cap_impl) : 0.0f;
+    float cc = dt_i * s.sigma * m_c / powi4(dx);
+    float kc = m_c * s.sigma / (dx * dx * dx);
+    P->front3[0] = cc; P->front3[1] = kc; P->front3[2] = m_c; P->front3[3] = ov->fcap;
+    P->front[0] = ov->no_hold ? 0.0f : 2.0f * lc_z * sinf(0.5f * theta);
+    P->front[1] = dx / lc_z; P->front[2] = m_imp_max; P->front[3] = dt_i;
+    uint32_t flags = (ov->no_transl ? 1u : 0u) | ((ov->no_shape ? 1u : 0u) << 1);
+    float hyst = ov->has_hyst ? ov->hyst_deg : s.hysteresis_deg;
+    P->front2[0] = to_radians(hyst); P->front2[1] = ov->kshape; P->front2[2] = ov->kline;
+    P->front2[3] = (float)flags;
+}
+
+// choose_substeps (src/film/mod.rs:919-933)
+uint32_t cz_choose_substeps_host(const cz_settings* sp, uint32_t w, float frame_dt, const cz_host_state* hs) {
+    const cz_settings& s = *sp;
+    float g[3];
+    cz_gravity_vec_host(sp, g);
+    float dx = s.width_m / (float)w;
+    float gxy = sqrtf(g[0] * g[0] + g[1] * g[1]);
+    float h_max = fmaxf(fmaxf(hs->h_max, hs->h_max_pending), 1e-5f);
+    float mu = fmaxf(hs->mu_min, 1e-3f);
+    float speed = fminf(s.rho * gxy * h_max * h_max / mu, s.v_cap);
+    float dt_cfl = (speed > 1e-6f) ? 0.5f * dx / speed : frame_dt;
+    float mob = h_max * h_max * h_max / (3.0f * mu);
+    uint32_t n_hyd = (uint32_t)fmaxf(ceilf(frame_dt * s.rho * g[2] * fminf(mob, s.m_cap_hydro) / (0.2f * dx * dx)), 1.0f);
+    float dx4 = powi4(dx);
+    uint32_t n_cap = (uint32_t)fmaxf(ceilf(frame_dt * 32.0f * s.sigma * fminf(mob, s.m_cap) / dx4), 1.0f);
+    uint32_t n_cfl = (uint32_t)fmaxf(ceilf(frame_dt / dt_cfl), 1.0f);
+    uint32_t nn = std::max(std::max(n_cfl, n_hyd), n_cap);
+    return std::min(std::max(nn, s.min_substeps), s.max_substeps);
+}
+
 // PaintMix::normalized (src/color/mod.rs:358-371) and packed (348-356)
 void cz_paint_pack(const float* frac16, int normalize, uint32_t out[8]) {
     float f[16];

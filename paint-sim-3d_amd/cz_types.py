@@ -44,6 +44,10 @@ class Stats(C.Structure):
                 ("frames_counted", C.c_uint64)]
 
 
+class HostState(C.Structure):
+    _fields_ = [("h_max", f32), ("mu_min", f32), ("h_max_pending", f32), ("pad", u32)]
+
+
 class MassReport(C.Structure):
     _fields_ = [("wet_ml", C.c_double), ("cured_ml", C.c_double), ("deposited_ml", C.c_double),
                 ("err_pct", C.c_double), ("hmax", f32), ("pad", u32), ("wet_cells", C.c_uint64),

@@ -1,0 +1,56 @@
+"""The C-ABI library loads and exports every entry point that
+include/canvaz_film.h declares; host-only entry points answer without a
+device; device entry points fail with an error (not a crash) when no HIP
+device is present."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+import canvaz_film as czf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include"))) if f.endswith(".h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(cz_\w+)\s*\(", txt, re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    names = declared_functions()
+    assert len(names) > 40
+    lib = C.CDLL(czf.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", czf.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (cz_\w+)", out))
+    assert set(names) <= exported
+
+
+def test_host_entry_points_without_device():
+    s = czf.default_settings()
+    L = czf.lib()
+    d = czf.default_settings()
+    L.cz_settings_default(C.byref(d))
+    assert bytes(d) == bytes(s)
+    assert L.cz_version().decode().startswith("canvaz-film")
+
+
+def test_create_without_device_is_an_error():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is present")
+    ptr = C.c_void_p()
+    rc = czf.lib().cz_film_create(64, 64, None, None, None, C.byref(ptr))
+    assert rc < 0
+    assert czf.lib().cz_last_error()
+    assert not ptr.value
