@@ -58,6 +58,59 @@ def cpu_baseline(state_fn, settings_fn):
                       f"({o.active_count()} active tiles, {dt:.1f} s)"}
 
 
+def bench_km(args):
+    """Spectral Kubelka-Munk mix: 4096x4096 cells x 12 pigments x 36 bands,
+    BF16 concentrations ~ Dirichlet(0.3), FP32 K and S tables log-uniform in
+    [0.01, 10]; value = cells / s; the kernel reads 24 B and writes 144 B per
+    cell and does 2 x 12 x 72 x 2 MFMA flops per cell (hi and lo parts)."""
+    import numpy as np
+    import torch
+    import km
+    cells, bands = 4096 * 4096, 36
+    rng = np.random.default_rng(36)
+    log("bench km: generating inputs")
+    conc = torch.from_numpy(rng.dirichlet(np.full(12, 0.3), cells).astype(np.float32)).to(torch.bfloat16).cuda()
+    kt = np.exp(rng.uniform(np.log(0.01), np.log(10), (12, bands))).astype(np.float32)
+    st = np.exp(rng.uniform(np.log(0.01), np.log(10), (12, bands))).astype(np.float32)
+    ops = torch.from_numpy(km.prepare(kt, st).view(np.int16)).cuda()
+    out = torch.empty((cells, bands), dtype=torch.float32, device="cuda")
+    for _ in range(args.warmup):
+        km.mix(conc, ops, bands, out)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(args.steps):
+        km.mix(conc, ops, bands, out)
+    ev1.record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    avg_us = ev0.elapsed_time(ev1) * 1e3 / args.steps
+    bytes_per = (24 + 4 * bands) * cells
+    achieved = bytes_per / (avg_us * 1e-6) / 1e9
+    flops = 2 * 12 * 2 * (2 * bands) * cells
+    # check a sample against the FP64 restatement
+    import ctypes as C
+    import oracle as orc
+    idx = rng.integers(0, cells, 4096)
+    cs = conc[idx].cpu().view(torch.int16).numpy().view(np.uint16).copy()
+    ref = np.zeros((len(idx), bands), np.float64)
+    orc.lib().of_km_mix_f64(cs.ctypes.data_as(C.POINTER(C.c_uint16)), kt.ctypes.data_as(C.POINTER(C.c_float)),
+                            st.ctypes.data_as(C.POINTER(C.c_float)), len(idx), bands,
+                            ref.ctypes.data_as(C.POINTER(C.c_double)))
+    err = float(np.abs(out[idx].cpu().numpy() - ref).max())
+    line = {"metric": "km mix cells/s", "value": cells * args.steps / elapsed, "unit": "cells/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic",
+            "config": {"workload": "KM mix 4096x4096 cells x 12 pigments x 36 bands", "max_abs_err_vs_fp64": err},
+            "roofline": {"bound": "hbm", "kernel": "k_km_mix", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "avg_launch_us": avg_us, "mfma_tflops": flops / (avg_us * 1e-6) / 1e12},
+            "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +119,7 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--workload", default="film", choices=("film", "km"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (canvases are independent "
@@ -78,6 +132,8 @@ def main():
     torch.cuda.set_device(0)
     import canvaz_film as czf
     import scenes
+    if args.workload == "km":
+        return bench_km(args)
 
     n = args.size
     log(f"bench: building the {n}x{n} blob canvas")
