@@ -1920,6 +1920,12 @@ void of_encode_frame(ofilm* F, uint32_t substeps, uint32_t implicit_steps, uint3
 /* host side: settings, params, substeps (src/film/mod.rs:315-350,861-933)  */
 
 static const float PI_F = 3.14159265358979323846f;
+/* Host-side sin/cos/tan: evaluated in double and rounded once to float, so
+ * that the result does not depend on which libm code path the CPU's feature
+ * set selects (the reference's f32 calls are within one ULP of these). */
+static inline float hsinf(float x) { return (float)sin((double)x); }
+static inline float hcosf(float x) { return (float)cos((double)x); }
+static inline float htanf(float x) { return (float)tan((double)x); }
 static inline float to_radians(float d) { return d * (PI_F / 180.0f); }
 static inline float powi4(float x) { float a = x * x; return a * a; }
 
@@ -1943,10 +1949,10 @@ void of_gravity_vec(const ofilm* F, float* g) {
     const cz_settings* s = &F->s;
     float tilt = to_radians(s->tilt_deg);
     float dir = to_radians(s->tilt_dir_deg);
-    float gxy = s->gravity * sinf(tilt);
-    g[0] = gxy * cosf(dir);
-    g[1] = gxy * sinf(dir);
-    g[2] = cz_max(s->gravity * cosf(tilt), 0.0f);
+    float gxy = s->gravity * hsinf(tilt);
+    g[0] = gxy * hcosf(dir);
+    g[1] = gxy * hsinf(dir);
+    g[2] = cz_max(s->gravity * hcosf(tilt), 0.0f);
 }
 
 void of_update_params(ofilm* F, float frame_dt, uint32_t substeps) {
@@ -1955,7 +1961,7 @@ void of_update_params(ofilm* F, float frame_dt, uint32_t substeps) {
     float g[3];
     of_gravity_vec(F, g);
     float theta = to_radians(s->contact_angle_deg);
-    float pi_a = 3.0f * s->sigma * (1.0f - cosf(theta)) / s->h_star;
+    float pi_a = 3.0f * s->sigma * (1.0f - hcosf(theta)) / s->h_star;
     float dt = frame_dt / (float)substeps;
     float dx = F->dx;
     float m_cap_exp = 0.5f * dx * dx * dx * dx / (32.0f * s->sigma * dt);
@@ -1968,7 +1974,7 @@ void of_update_params(ofilm* F, float frame_dt, uint32_t substeps) {
     P->phys[0] = s->rho; P->phys[1] = s->sigma; P->phys[2] = s->n_index;
     P->phys[3] = (s->capillary && !F->ov.no_cap) ? m_cap : 0.0f;
     float lc = sqrtf(s->sigma / (s->rho * 9.81f));
-    P->ein[0] = fmaxf(tanf(theta), 0.05f); P->ein[1] = lc * sinf(0.5f * theta); P->ein[2] = pi_a; P->ein[3] = s->h_min;
+    P->ein[0] = fmaxf(htanf(theta), 0.05f); P->ein[1] = lc * hsinf(0.5f * theta); P->ein[2] = pi_a; P->ein[3] = s->h_min;
     P->cure[0] = s->cure_rate; P->cure[1] = s->tau_cure; P->cure[2] = s->lnmu_cure; P->cure[3] = s->h_star;
     P->misc[0] = 2.0e-6f; P->misc[1] = 0.12f; P->misc[2] = s->medium_mu; P->misc[3] = fminf(m_exp, s->m_cap_hydro);
     P->flags[0] = s->debug_view; P->flags[1] = s->slump; P->flags[2] = 0; P->flags[3] = 0;
@@ -1995,7 +2001,7 @@ void of_update_params(ofilm* F, float frame_dt, uint32_t substeps) {
     float cc = dt_i * s->sigma * m_c / powi4(dx);
     float kc = m_c * s->sigma / (dx * dx * dx);
     P->front3[0] = cc; P->front3[1] = kc; P->front3[2] = m_c; P->front3[3] = F->ov.fcap;
-    P->front[0] = F->ov.no_hold ? 0.0f : 2.0f * lc_z * sinf(0.5f * theta);
+    P->front[0] = F->ov.no_hold ? 0.0f : 2.0f * lc_z * hsinf(0.5f * theta);
     P->front[1] = dx / lc_z; P->front[2] = m_imp_max; P->front[3] = dt_i;
     uint32_t flags = (F->ov.no_transl ? 1u : 0u) | ((F->ov.no_shape ? 1u : 0u) << 1);
     float hyst = F->ov.has_hyst ? F->ov.hyst_deg : s->hysteresis_deg;
@@ -2200,7 +2206,7 @@ void of_weave(uint32_t w, uint32_t h, float dx, float amp, float* v) {
             hsh = hsh * 0x5bd1e995u;
             hsh ^= hsh >> 15;
             float r = (float)(hsh & 0xffffu) / 65535.0f * 2.0f - 1.0f;
-            float s = sinf(k * x) * sinf(k * y);
+            float s = hsinf(k * x) * hsinf(k * y);
             v[(size_t)j * w + i] = amp * (1.0f + 0.6f * s + 0.4f * r);
         }
     }

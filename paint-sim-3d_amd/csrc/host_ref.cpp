@@ -14,6 +14,12 @@
 #include "host_ref.h"
 
 namespace czh {
+/* Host-side sin/cos/tan: evaluated in double and rounded once to float, so
+ * that the result does not depend on which libm code path the CPU's feature
+ * set selects (the reference's f32 calls are within one ULP of these). */
+static inline float hsinf(float x) { return (float)sin((double)x); }
+static inline float hcosf(float x) { return (float)cos((double)x); }
+static inline float htanf(float x) { return (float)tan((double)x); }
 
 static const float PI_F = 3.14159265358979323846f;
 
@@ -267,7 +273,7 @@ void cz_weave(uint32_t w, uint32_t h, float dx, float amp, float* v) {
     float k = (2.0f * PI_F) / period;
     for (uint32_t j = 0; j < h; j++) {
         float y = ((float)j + 0.5f) * dx;
-        float sy = sinf(k * y);
+        float sy = hsinf(k * y);
         for (uint32_t i = 0; i < w; i++) {
             float x = ((float)i + 0.5f) * dx;
             uint32_t hsh = (i * 73856093u) ^ (j * 19349663u);
@@ -275,7 +281,7 @@ void cz_weave(uint32_t w, uint32_t h, float dx, float amp, float* v) {
             hsh = hsh * 0x5bd1e995u;
             hsh ^= hsh >> 15;
             float r = (float)(hsh & 0xffffu) / 65535.0f * 2.0f - 1.0f;
-            float s = sinf(k * x) * sy;
+            float s = hsinf(k * x) * sy;
             v[(size_t)j * w + i] = amp * (1.0f + 0.6f * s + 0.4f * r);
         }
     }
@@ -359,10 +365,10 @@ float cz_deposit_inside_fraction(const cz_deposit* d, float w, float h) { return
 void cz_gravity_vec_host(const cz_settings* s, float out[3]) {
     float tilt = to_radians(s->tilt_deg);
     float dir = to_radians(s->tilt_dir_deg);
-    float gxy = s->gravity * sinf(tilt);
-    out[0] = gxy * cosf(dir);
-    out[1] = gxy * sinf(dir);
-    out[2] = fmaxf(s->gravity * cosf(tilt), 0.0f);
+    float gxy = s->gravity * hsinf(tilt);
+    out[0] = gxy * hcosf(dir);
+    out[1] = gxy * hsinf(dir);
+    out[2] = fmaxf(s->gravity * hcosf(tilt), 0.0f);
 }
 
 // update_params (src/film/mod.rs:869-917): the uniform block of one frame.
@@ -373,7 +379,7 @@ void cz_update_params_host(const cz_settings* sp, const cz_overrides* ov, uint32
     cz_gravity_vec_host(sp, g);
     float dx = s.width_m / (float)w;
     float theta = to_radians(s.contact_angle_deg);
-    float pi_a = 3.0f * s.sigma * (1.0f - cosf(theta)) / s.h_star;
+    float pi_a = 3.0f * s.sigma * (1.0f - hcosf(theta)) / s.h_star;
     float dt = frame_dt / (float)substeps;
     float m_cap_exp = 0.5f * dx * dx * dx * dx / (32.0f * s.sigma * dt);
     float m_cap = fmaxf(fminf(s.m_cap, m_cap_exp), 1e-14f);
@@ -387,7 +393,7 @@ void cz_update_params_host(const cz_settings* sp, const cz_overrides* ov, uint32
     P->phys[0] = s.rho; P->phys[1] = s.sigma; P->phys[2] = s.n_index;
     P->phys[3] = (s.capillary && !ov->no_cap) ? m_cap : 0.0f;
     float lc = sqrtf(s.sigma / (s.rho * 9.81f));
-    P->ein[0] = fmaxf(tanf(theta), 0.05f); P->ein[1] = lc * sinf(0.5f * theta); P->ein[2] = pi_a; P->ein[3] = s.h_min;
+    P->ein[0] = fmaxf(htanf(theta), 0.05f); P->ein[1] = lc * hsinf(0.5f * theta); P->ein[2] = pi_a; P->ein[3] = s.h_min;
     P->cure[0] = s.cure_rate; P->cure[1] = s.tau_cure; P->cure[2] = s.lnmu_cure; P->cure[3] = s.h_star;
     P->misc[0] = 2.0e-6f; P->misc[1] = 0.12f; P->misc[2] = s.medium_mu; P->misc[3] = fminf(m_exp, s.m_cap_hydro);
     P->flags[0] = s.debug_view; P->flags[1] = s.slump; P->flags[2] = 0; P->flags[3] = 0;
@@ -408,7 +414,7 @@ void cz_update_params_host(const cz_settings* sp, const cz_overrides* ov, uint32
     float cc = dt_i * s.sigma * m_c / powi4(dx);
     float kc = m_c * s.sigma / (dx * dx * dx);
     P->front3[0] = cc; P->front3[1] = kc; P->front3[2] = m_c; P->front3[3] = ov->fcap;
-    P->front[0] = ov->no_hold ? 0.0f : 2.0f * lc_z * sinf(0.5f * theta);
+    P->front[0] = ov->no_hold ? 0.0f : 2.0f * lc_z * hsinf(0.5f * theta);
     P->front[1] = dx / lc_z; P->front[2] = m_imp_max; P->front[3] = dt_i;
     uint32_t flags = (ov->no_transl ? 1u : 0u) | ((ov->no_shape ? 1u : 0u) << 1);
     float hyst = ov->has_hyst ? ov->hyst_deg : s.hysteresis_deg;
