@@ -222,7 +222,9 @@ class Film:
                                _fp(cm) if cm is not None else None, C.byref(ptr)))
         self._p = ptr
         self.w, self.h = w, h
-        self.dx = self.settings.width_m / w
+        d = C.c_float()
+        check(L.cz_film_dims(self._p, None, None, C.byref(d)))
+        self.dx = d.value  # the FP32 cell size the step uses (width_m / w in f32)
 
     def close(self):
         if getattr(self, "_p", None):
