@@ -193,7 +193,9 @@ def main():
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * n * args.steps / elapsed
     log(f"bench: {ms_per_step:.3f} ms/frame, substeps {st.substeps_last}, active cells/frame "
-        f"{st.act_cells / max(1, frames):.0f}, {dom}: {avg_us:.2f} us x {launches}")
+        f"{st.act_cells / max(1, frames):.0f}, {dom}: {avg_us:.2f} us x {launches}; timed pass "
+        f"{st.act_cells}/{st.act_cell_substeps}/{st.frames_counted}, profiled pass "
+        f"{st2.act_cells}/{st2.act_cell_substeps}/{st2.frames_counted}")
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU restatement baseline")

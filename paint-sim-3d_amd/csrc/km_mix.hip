@@ -40,6 +40,33 @@ __device__ __forceinline__ float r_inf(float k, float s) {
     return 1.0f / (1.0f + a + sqrtf(a * (a + 2.0f)));
 }
 
+// A operand of one lane for the 16-cell group at `base`: 8 concentrations of
+// the lane's k-range (groups 0/2: pigments 0-7, groups 1/3: pigments 8-11).
+__device__ __forceinline__ u16x8 load_a(const uint16_t* __restrict__ conc, uint64_t cell, uint64_t cells,
+                                        uint32_t grp) {
+    u16x8 av = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (cell < cells) {
+        const uint16_t* cp = conc + cell * KM_NPIG;
+        if ((grp & 1u) == 0u) {
+            const uint2 v0 = __builtin_nontemporal_load((const uint2*)(cp));
+            const uint2 v1 = __builtin_nontemporal_load((const uint2*)(cp + 4));
+            av[0] = (unsigned short)(v0.x & 0xffffu); av[1] = (unsigned short)(v0.x >> 16);
+            av[2] = (unsigned short)(v0.y & 0xffffu); av[3] = (unsigned short)(v0.y >> 16);
+            av[4] = (unsigned short)(v1.x & 0xffffu); av[5] = (unsigned short)(v1.x >> 16);
+            av[6] = (unsigned short)(v1.y & 0xffffu); av[7] = (unsigned short)(v1.y >> 16);
+        } else {
+            const uint2 v2 = __builtin_nontemporal_load((const uint2*)(cp + 8));
+            av[0] = (unsigned short)(v2.x & 0xffffu); av[1] = (unsigned short)(v2.x >> 16);
+            av[2] = (unsigned short)(v2.y & 0xffffu); av[3] = (unsigned short)(v2.y >> 16);
+        }
+    }
+    return av;
+}
+
+// Each wave takes KM_BATCH consecutive 16-cell groups per iteration and issues
+// all of their loads before computing, to keep enough bytes in flight.
+constexpr int KM_BATCH = 4;
+
 template <int NT>
 __global__ __launch_bounds__(256) void k_km_mix(const uint16_t* __restrict__ conc, const u16x8* __restrict__ bops,
                                                 uint64_t cells, uint32_t bands, float* __restrict__ out) {
@@ -54,62 +81,55 @@ __global__ __launch_bounds__(256) void k_km_mix(const uint16_t* __restrict__ con
         b[t] = __builtin_bit_cast(bf16x8, v);
     }
     const uint64_t groups = (cells + 15u) / 16u;
+    const uint64_t batches = (groups + KM_BATCH - 1) / KM_BATCH;
     const uint64_t wave_id = (uint64_t)blockIdx.x * 4u + wave;
     const uint64_t nwaves = (uint64_t)gridDim.x * 4u;
     float* st = stage[wave];
-    for (uint64_t gi = wave_id; gi < groups; gi += nwaves) {
-        const uint64_t base = gi * 16u;
-        const uint64_t cell = base + row;
-        // A operand: 8 concentrations of this lane's k-range
-        u16x8 av = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (cell < cells) {
-            const uint16_t* cp = conc + cell * KM_NPIG;
-            if ((grp & 1u) == 0u) {
-                const uint2 v0 = *(const uint2*)(cp);
-                const uint2 v1 = *(const uint2*)(cp + 4);
-                av[0] = (unsigned short)(v0.x & 0xffffu); av[1] = (unsigned short)(v0.x >> 16);
-                av[2] = (unsigned short)(v0.y & 0xffffu); av[3] = (unsigned short)(v0.y >> 16);
-                av[4] = (unsigned short)(v1.x & 0xffffu); av[5] = (unsigned short)(v1.x >> 16);
-                av[6] = (unsigned short)(v1.y & 0xffffu); av[7] = (unsigned short)(v1.y >> 16);
+    for (uint64_t bi = wave_id; bi < batches; bi += nwaves) {
+        u16x8 av[KM_BATCH];
+#pragma unroll
+        for (int g = 0; g < KM_BATCH; g++) {
+            const uint64_t base = (bi * KM_BATCH + g) * 16u;
+            av[g] = load_a(conc, base + row, cells, grp);
+        }
+#pragma unroll
+        for (int g = 0; g < KM_BATCH; g++) {
+            const uint64_t base = (bi * KM_BATCH + g) * 16u;
+            if (base >= cells) break;
+            const bf16x8 a = __builtin_bit_cast(bf16x8, av[g]);
+            f32x4 acc[2 * NT];
+#pragma unroll
+            for (int t = 0; t < 2 * NT; t++) {
+                f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], z, 0, 0, 0);
+            }
+            // accumulator: lane holds rows 4*grp + j (cells), column `row` (band in tile)
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const uint32_t band = 16u * t + row;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t cl = 4u * grp + j;
+                    if (band < bands) st[cl * bands + band] = r_inf(acc[t][j], acc[NT + t][j]);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            // contiguous write-out of the group's 16 x bands floats
+            const uint64_t ncell = (cells - base) < 16u ? (cells - base) : 16u;
+            const uint32_t nflt = (uint32_t)ncell * bands;
+            float* dst = out + base * bands;
+            if ((bands & 3u) == 0u) {
+                for (uint32_t q = lane; q < nflt / 4u; q += 64u) {
+                    f32x4 v = *(const f32x4*)(st + 4u * q);
+                    __builtin_nontemporal_store(v, (f32x4*)(dst + 4u * q));
+                }
             } else {
-                const uint2 v2 = *(const uint2*)(cp + 8);
-                av[0] = (unsigned short)(v2.x & 0xffffu); av[1] = (unsigned short)(v2.x >> 16);
-                av[2] = (unsigned short)(v2.y & 0xffffu); av[3] = (unsigned short)(v2.y >> 16);
+                for (uint32_t q = lane; q < nflt; q += 64u) __builtin_nontemporal_store(st[q], dst + q);
             }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         }
-        const bf16x8 a = __builtin_bit_cast(bf16x8, av);
-        f32x4 acc[2 * NT];
-#pragma unroll
-        for (int t = 0; t < 2 * NT; t++) {
-            f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], z, 0, 0, 0);
-        }
-        // accumulator: lane holds rows 4*grp + j (cells), column `row` (band in tile)
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-            const uint32_t band = 16u * t + row;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const uint32_t cl = 4u * grp + j;
-                if (band < bands) st[cl * bands + band] = r_inf(acc[t][j], acc[NT + t][j]);
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        // contiguous write-out of the group's 16 x bands floats
-        const uint64_t ncell = (cells - base) < 16u ? (cells - base) : 16u;
-        const uint32_t nflt = (uint32_t)ncell * bands;
-        float* dst = out + base * bands;
-        if ((bands & 3u) == 0u) {
-            for (uint32_t q = lane; q < nflt / 4u; q += 64u) {
-                f32x4 v = *(const f32x4*)(st + 4u * q);
-                __builtin_nontemporal_store(v, (f32x4*)(dst + 4u * q));
-            }
-        } else {
-            for (uint32_t q = lane; q < nflt; q += 64u) __builtin_nontemporal_store(st[q], dst + q);
-        }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     }
 }
 
@@ -183,8 +203,8 @@ int cz_km_mix(const uint16_t* conc_bf16, const uint16_t* ops, uint64_t cells, ui
     if (cells == 0) return CZ_OK;
     const uint32_t NT = (bands + 15u) / 16u;
     const uint64_t groups = (cells + 15u) / 16u;
-    const uint64_t want = (groups + 3u) / 4u;
-    const uint32_t grid = (uint32_t)(want < 8192u ? want : 8192u);
+    const uint64_t want = (groups + 4u * KM_BATCH - 1u) / (4u * KM_BATCH);
+    const uint32_t grid = (uint32_t)(want < 4096u ? want : 4096u);
     hipStream_t s = (hipStream_t)stream;
     const u16x8* b = (const u16x8*)ops;
     if (NT == 1) hipLaunchKernelGGL(k_km_mix<1>, dim3(grid), dim3(256), 0, s, conc_bf16, b, cells, bands, out_refl);
