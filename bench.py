@@ -111,22 +111,112 @@ def bench_km(args):
     print(json.dumps(line), flush=True)
 
 
+def batch_canvases(count=64, seed=4):
+    """The batched workload: `count` square canvases, side log-uniform in
+    [256, 2048], seeded blob states as the plain canvas, tilt U[0, 60] deg."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = np.round(np.exp(rng.uniform(np.log(256.0), np.log(2048.0), count))).astype(int)
+    tilts = rng.uniform(0.0, 60.0, count)
+    return [(int(w), float(t), 5000 + k) for k, (w, t) in enumerate(zip(sides, tilts))]
+
+
+def bench_batch(args, plain_value):
+    """All canvases stepped together through one cz_batch for `frames`
+    frames from their initial states; then each canvas stepped alone through
+    the plain path for the same frames, timed, and compared with its batched
+    twin (largest relative height difference, floor 1e-9 m)."""
+    import numpy as np
+    import torch
+    import canvaz_film as czf
+    import scenes
+    specs = batch_canvases(args.batch_canvases)
+    frames = args.batch_frames
+    log(f"bench batch: generating {len(specs)} canvas states")
+    states = [scenes.blobs_state(w, w, seed=sd) for (w, _, sd) in specs]
+    cells = sum(w * w for (w, _, _) in specs)
+    films = []
+    for (w, tilt, _), st in zip(specs, states):
+        f = czf.Film(w, w, scenes.blobs_settings(w, tilt_deg=tilt))
+        assert f.write_state_arrays(st)
+        films.append(f)
+    b = czf.Batch(max(len(films), 1))
+    if args.graphs:
+        b.use_graphs(True)
+    for f in films:
+        b.add(f)
+    log(f"bench batch: {len(films)} canvases, {cells / 1e6:.1f} M cells, timing {frames} frames")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        b.step()
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    batched_value = cells * frames / t_batch
+    subs = [f.stats().act_cell_substeps / max(1, f.stats().act_cells) for f in films]
+    act = sum(f.stats().act_cells for f in films) / max(1, frames)
+    log(f"bench batch: {t_batch * 1e3 / frames:.2f} ms/frame, {batched_value:.3e} cell-steps/s; "
+        f"plain comparison")
+    t_plain, worst = 0.0, 0.0
+    for k, ((w, tilt, _), st) in enumerate(zip(specs, states)):
+        g = czf.Film(w, w, scenes.blobs_settings(w, tilt_deg=tilt))
+        assert g.write_state_arrays(st)
+        g.sync()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            g.step()
+        g.sync()
+        t_plain += time.perf_counter() - t0
+        hb, hp = films[k].buffer("h").astype(np.float64), g.buffer("h").astype(np.float64)
+        worst = max(worst, float(np.max(np.abs(hb - hp) / np.maximum(np.maximum(np.abs(hb), np.abs(hp)), 1e-9))))
+        g.close()
+    b.close()
+    for f in films:
+        f.close()
+    plain_same = cells * frames / t_plain
+    return {"value": batched_value, "unit": "cell-steps/s", "ms_per_frame": t_batch * 1e3 / frames,
+            "canvases": len(specs), "cells": cells, "frames": frames,
+            "vs_plain_2048": batched_value / plain_value if plain_value else None,
+            "plain_same_canvases": plain_same, "vs_plain_same_canvases": batched_value / plain_same,
+            "max_rel_err_vs_plain": worst, "active_cells_per_frame": act,
+            "mean_substeps": float(np.mean(subs)), "graphs": bool(args.graphs),
+            "workload": f"{len(specs)} square canvases, side log-uniform [256, 2048], blob states seeded as the "
+                        f"plain canvas, tilt U[0, 60] deg, {frames} frames from the initial states"}
+
+
+FRAME_FAMILIES = SUBSTEP_FAMILIES + ("lap", "mark", "compact", "cure", "deposit", "split", "reduce_pool",
+                                     "strata_limit", "strata_update", "copy_back", "strata_fill", "delta_sync",
+                                     "frame_stats", "cg_setup", "cg_lap", "cg_init", "cg_iter_a", "cg_iter_b",
+                                     "cg_iter_c", "cg_reduce", "cg_finish", "cg_rescale")
+
+
+def frame_algo_bytes(n_grid, st):
+    """Compulsory bytes of the frames folded in `st`:
+    N_grid*20 + N_act*(632 + 76*S + I*(180 + 68*K)) per frame, K = 40 CG iterations."""
+    return (n_grid * 20 * st.frames_counted + 632 * st.act_cells + 76 * st.act_cell_substeps
+            + (180 + 68 * 40) * st.act_cell_implicit)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-graphs", action="store_true")
-    ap.add_argument("--workload", default="film", choices=("film", "km"))
+    ap.add_argument("--graphs", action="store_true", help="replay each frame shape as a captured hipGraph")
+    ap.add_argument("--no-events", action="store_true",
+                    help="time the frames without HIP events around the dominant kernel (roofline from a replay)")
+    ap.add_argument("--workload", default="film", choices=("film", "km", "batch"))
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload in the film run")
+    ap.add_argument("--batch-canvases", type=int, default=64)
+    ap.add_argument("--batch-frames", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (canvases are independent "
               f"and would be placed one GPU each; nothing here runs on more than one GPU)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
     assert torch.cuda.is_available(), "bench needs a HIP device"
     torch.cuda.set_device(0)
@@ -134,85 +224,104 @@ def main():
     import scenes
     if args.workload == "km":
         return bench_km(args)
+    if args.workload == "batch":
+        print(json.dumps(bench_batch(args, None)), flush=True)
+        return
 
     n = args.size
     log(f"bench: building the {n}x{n} blob canvas")
-    state_fn = lambda: scenes.blobs_state(n, n, seed=2048)  # noqa: E731
+    state = scenes.blobs_state(n, n, seed=2048)
     settings_fn = lambda w: scenes.blobs_settings(w, tilt_deg=15.0)  # noqa: E731
-    film = czf.Film(n, n, settings_fn(n))
-    czf.check(czf.lib().cz_film_use_graphs(film._p, 0 if args.no_graphs else 1))
-    assert film.write_state_arrays(state_fn())
-    log(f"bench: warmup {args.warmup} frames")
-    for _ in range(args.warmup):
+
+    def fresh():
+        f = czf.Film(n, n, settings_fn(n))
+        czf.check(czf.lib().cz_film_use_graphs(f._p, 1 if args.graphs else 0))
+        assert f.write_state_arrays(state)
+        return f
+
+    # warmup film: the first frames of the same run, every family timed, to pick the dominant one
+    film = fresh()
+    film.profile(True)
+    for _ in range(max(1, args.warmup)):
         film.step()
     film.sync()
-    # pick the dominant family on one profiled frame
-    film.profile(True)
-    film.step()
-    film.sync()
-    fam_time = {}
-    breakdown = {}
-    for fam in SUBSTEP_FAMILIES + ("lap", "split", "reduce_pool", "strata_limit", "strata_update", "copy_back",
-                                   "strata_fill", "cure", "mark", "compact", "deposit", "delta_sync",
-                                   "frame_stats", "cg_setup", "cg_lap", "cg_init", "cg_iter_a", "cg_iter_b",
-                                   "cg_iter_c", "cg_reduce", "cg_finish", "cg_rescale"):
+    fam_time, breakdown = {}, {}
+    for fam in FRAME_FAMILIES:
         avg, cnt = film.profile_read(fam)
         if cnt:
             breakdown[fam] = (avg, cnt)
-        if fam in SUBSTEP_FAMILIES:
             fam_time[fam] = avg * cnt
-    log("bench: one-frame breakdown (us avg x launches): " +
+    log(f"bench: warmup {args.warmup} frames, per-family device time (us avg x launches): " +
         ", ".join(f"{k} {v[0]:.1f}x{v[1]}" for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][0] * kv[1][1])))
-    dom = max(fam_time, key=fam_time.get)
-    film.profile(False)
+    dom = max((f for f in fam_time if f in SUBSTEP_FAMILIES), key=fam_time.get)
+    film.close()
+    del film
+
+    # timed pass: a fresh film from the initial state, K frames
+    film = fresh()
+    if not args.no_events:
+        film.profile(True)
+        film.profile_only(dom)
     film.reset_counters()
-    log(f"bench: timing {args.steps} frames (dominant kernel {dom})")
+    log(f"bench: timing {args.steps} frames (events around {dom}: {not args.no_events})")
     torch.cuda.synchronize()
     film.sync()
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for _ in range(args.steps):
         film.step()
     film.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     st = film.stats()
-    frames = st.frames_counted
-    # event timing of the dominant kernel over a second pass of the same length
-    film.profile(True)
-    film.profile_only(dom)
-    film.reset_counters()
-    for k in range(args.steps):
-        film.step()
-    film.sync()
-    st2 = film.stats()
+    if args.no_events:
+        # the same frames again with events around the dominant family
+        film.close()
+        del film
+        film = fresh()
+        film.profile(True)
+        film.profile_only(dom)
+        film.reset_counters()
+        for _ in range(args.steps):
+            film.step()
+        film.sync()
+        st = film.stats()
     avg_us, launches = film.profile_read(dom)
     film.profile(False)
-    cells_per_launch = st2.act_cell_substeps / max(1, launches)
+    film.close()
+    del film
+
+    cells_per_launch = st.act_cell_substeps / max(1, launches)
     bytes_per_launch = ALGO_BYTES[dom] * cells_per_launch
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * n * args.steps / elapsed
-    log(f"bench: {ms_per_step:.3f} ms/frame, substeps {st.substeps_last}, active cells/frame "
-        f"{st.act_cells / max(1, frames):.0f}, {dom}: {avg_us:.2f} us x {launches}; timed pass "
-        f"{st.act_cells}/{st.act_cell_substeps}/{st.frames_counted}, profiled pass "
-        f"{st2.act_cells}/{st2.act_cell_substeps}/{st2.frames_counted}")
+    frames = max(1, st.frames_counted)
+    frame_gbs = frame_algo_bytes(n * n, st) / elapsed / 1e9
+    log(f"bench: {ms_per_step:.3f} ms/frame, active cells/frame {st.act_cells / frames:.0f}, "
+        f"substeps/frame {st.act_cell_substeps / max(1, st.act_cells):.1f}, {dom}: {avg_us:.2f} us x {launches}, "
+        f"{cells_per_launch:.0f} cells/launch")
+    batched = None if args.no_batch else bench_batch(args, value)
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU restatement baseline")
-        cpu = cpu_baseline(state_fn, settings_fn)
+        cpu = cpu_baseline(lambda: state, settings_fn)
     line = {
         "metric": "film cell-steps/s", "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"plain film {n}x{n} FP32, 64 seeded Gaussian blobs, tilt 15 deg, frame_dt 1/60",
-                   "cells": n * n, "substeps_last": st.substeps_last,
-                   "active_cells_per_frame": st.act_cells / max(1, frames),
-                   "implicit_steps": st.implicit_steps},
+        "config": {"workload": f"plain film {n}x{n} FP32, 64 seeded Gaussian blobs, tilt 15 deg, frame_dt 1/60, "
+                               f"{args.steps} frames from the initial state",
+                   "cells": n * n, "graphs": bool(args.graphs),
+                   "active_cells_per_frame": st.act_cells / frames,
+                   "substeps_per_frame": st.act_cell_substeps / max(1, st.act_cells),
+                   "implicit_cell_steps": st.act_cell_implicit,
+                   "frame_algo_GBps": frame_gbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algo_bytes_per_cell": ALGO_BYTES[dom], "avg_launch_us": avg_us,
+                     "algo_bytes_per_cell": ALGO_BYTES[dom], "avg_launch_us": avg_us, "launches": launches,
                      "cells_per_launch": cells_per_launch},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 

@@ -149,11 +149,12 @@ typedef struct cz_stats {
     double total_deposited;  /* src/film/mod.rs:1188 */
     double weave_volume;     /* src/film/mod.rs:689 */
     /* running sums over the frames folded so far (since create or
-     * cz_film_reset_counters): sum of active cells, and of active cells
-     * times that frame's substeps */
+     * cz_film_reset_counters): sum of active cells, of active cells times
+     * that frame's substeps, and of active cells times its implicit steps */
     uint64_t act_cells;
     uint64_t act_cell_substeps;
     uint64_t frames_counted;
+    uint64_t act_cell_implicit;
 } cz_stats;
 
 /* The mass report of `compute_stats` (src/main.rs:1679-1819), sums in FP64. */
@@ -315,7 +316,8 @@ int cz_film_clear(cz_film* f);
 /* The HIP stream the film's work is enqueued on (hipStream_t). */
 void* cz_film_stream(cz_film* f);
 /* Capture every frame of cz_film_encode_frame into a hipGraph keyed by the
- * (substeps, implicit_steps, has_deposits) shape and replay it. 1 = on. */
+ * (substeps, implicit_steps, has_deposits) shape and replay it. 1 = on;
+ * off by default (direct launches measured at least as fast). */
 int cz_film_use_graphs(cz_film* f, int on);
 /* Average per-kernel device time over the last frames (HIP events on the
  * film's stream), in microseconds; label = kernel family name. */
@@ -326,6 +328,109 @@ int cz_film_profile_read(cz_film* f, const char* label, double* avg_us, uint64_t
 int cz_film_profile_only(cz_film* f, const char* label);
 /* Zero the running sums of cz_stats. */
 int cz_film_reset_counters(cz_film* f);
+
+/* ---- Seeded canvases generated on the device -----------------------------
+ * The plain workload's canvas (no counterpart in the reference, which
+ * starts from an empty film): a sum of nblobs Gaussian blobs (centre
+ * uniform, sigma uniform in cells, peak log-uniform, truncated at 3 sigma,
+ * zero below 1e-9 m); every wet cell one stratum of Dirichlet(alpha)
+ * pigments with log-uniform yield stress and viscosity in slots 13/14;
+ * relief the canvas weave (mod.rs:212-233), substrate the raw canvas
+ * (mod.rs:188-210).  Counter-based randomness: the result depends only on
+ * the scene and the canvas size. */
+typedef struct cz_scene {
+    uint64_t seed;
+    uint32_t nblobs;          /* <= 64 */
+    float sigma_lo, sigma_hi; /* cells */
+    float peak_lo, peak_hi;   /* m */
+    float tau_lo, tau_hi;     /* Pa */
+    float mu_lo, mu_hi;       /* Pa s */
+    float alpha;
+    float weave_um;
+} cz_scene;
+void cz_scene_default(cz_scene* s);
+/* Replace the film's state with the scene (as write_state does). */
+int cz_film_seed_blobs(cz_film* f, const cz_scene* s);
+/* Re-dimension a film within the sizes its buffers were created for, as a
+ * new Film::new of w x h with settings s (fresh host state); the state
+ * parts are then written or seeded by the caller.  CZ_ESIZE if it does not
+ * fit.  Lets a pool of films serve canvases of varying size. */
+int cz_film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s);
+/* The same cells computed on the host, rows [j0, j1): h, m, rheo (2 per
+ * cell) and stratum 0's packed pigments (8 per cell). */
+int cz_scene_cells_host(uint32_t w, uint32_t h, float dx, const cz_scene* s, uint32_t j0, uint32_t j1,
+                        float* h_out, float* m_out, float* rheo_out, uint32_t* pig0_out);
+
+/* ---- Many canvases stepped together ------------------------------------
+ * The reference steps one Film per App (encode_sim, src/main.rs:1875-1909);
+ * a cz_batch steps many films, each exactly as cz_film_step would step it
+ * alone without deposits (its own params, substeps, implicit steps and
+ * lag-1 stats), with one launch per kernel phase over the union of the
+ * members' tiles.  While a film is a member, the calls that enqueue work on
+ * it (step, encode_frame, upload_deposits, write_state, write_buffer,
+ * dry_now, clear) return CZ_EINVAL; its reads are valid after
+ * cz_batch_sync.  Destroying a member removes it from its batch. */
+typedef struct cz_batch cz_batch;
+int cz_batch_create(uint32_t max_members, cz_batch** out);   /* max_members <= 1024 */
+void cz_batch_destroy(cz_batch* b);
+/* Join / leave between frames; a joining film's own pending work is
+ * finished first.  Members must share settings.implicit_iters. */
+int cz_batch_add(cz_batch* b, cz_film* f);
+int cz_batch_remove(cz_batch* b, cz_film* f);
+uint32_t cz_batch_size(const cz_batch* b);
+/* One frame of every member, enqueued on the batch's stream. */
+int cz_batch_step(cz_batch* b, float frame_dt);
+/* Wait for the last frame and fold its stats into every member. */
+int cz_batch_sync(cz_batch* b);
+/* Replay each frame shape from a captured hipGraph (1 = on, default off). */
+int cz_batch_use_graphs(cz_batch* b, int on);
+void* cz_batch_stream(cz_batch* b);
+
+/* ---- Serving ---------------------------------------------------------------
+ * A loop that admits step requests (a seeded canvas and a frame count) as
+ * they arrive, first come first served, under a device-memory budget and a
+ * batch-size limit, steps all admitted canvases together (cz_batch; each
+ * canvas exactly as alone) and retires each when its frames are done;
+ * canvases stay resident from admission to completion.  No counterpart in
+ * the reference (one canvas per App); its per-canvas step is encode_sim
+ * (src/main.rs:1875-1909). */
+typedef struct cz_request {
+    uint64_t id;
+    uint32_t w, h;
+    uint32_t steps;          /* frames */
+    uint32_t pad;
+    double arrival_s;        /* after the start of cz_server_run; non-decreasing */
+    cz_settings settings;    /* width_m = w * cell size */
+    cz_scene scene;          /* initial state (cz_film_seed_blobs) */
+} cz_request;
+typedef struct cz_result {
+    uint64_t id;
+    double arrival_s, admit_s, done_s;  /* done: its last frame finished on the device */
+    uint64_t cell_steps;
+    int32_t status;          /* 0 done, else the CZ_E* code it failed with */
+    uint32_t pad;
+} cz_result;
+typedef struct cz_server_config {
+    uint64_t hbm_budget;     /* bytes of films held at once (active and pooled) */
+    uint32_t max_members;    /* canvases stepped together, <= 1024 */
+    float frame_dt;
+    int32_t keep_films;      /* 1: keep finished canvases for cz_server_film */
+    uint32_t pad;
+} cz_server_config;
+typedef struct cz_server_stats {
+    double wall_s;
+    uint64_t frames, admitted, cell_steps, peak_members, peak_bytes;
+    uint64_t films_created, films_reused;
+} cz_server_stats;
+typedef struct cz_server cz_server;
+void cz_server_config_default(cz_server_config* c);
+int cz_server_create(const cz_server_config* c, cz_server** out);
+void cz_server_destroy(cz_server* s);
+/* Serve the requests in real time (arrival_s measured from the call) until
+ * every one is done; res[k] belongs to reqs[k]. */
+int cz_server_run(cz_server* s, const cz_request* reqs, uint32_t n, cz_result* res, cz_server_stats* st);
+/* A finished canvas kept with keep_films (owned by the server), or NULL. */
+cz_film* cz_server_film(cz_server* s, uint64_t id);
 
 const char* cz_last_error(void);
 const char* cz_version(void);

@@ -14,7 +14,8 @@ import os
 
 import numpy as np
 
-from cz_types import (Deposit, HostState, MassReport, Overrides, Params, Settings, Stats, PART_NAMES,
+from cz_types import (Deposit, HostState, MassReport, Overrides, Params, Request, Result, Scene, ServerConfig,
+                      ServerStats, Settings, Stats, PART_NAMES,
                       default_overrides, default_settings, deposit_array)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -93,6 +94,24 @@ def lib():
         "cz_film_profile_read": (i32, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]),
         "cz_film_profile_only": (i32, [P, C.c_char_p]),
         "cz_film_reset_counters": (i32, [P]),
+        "cz_scene_default": (None, [C.POINTER(Scene)]),
+        "cz_film_seed_blobs": (i32, [P, C.POINTER(Scene)]),
+        "cz_film_reshape": (i32, [P, u32, u32, C.POINTER(Settings)]),
+        "cz_scene_cells_host": (i32, [u32, u32, f32, C.POINTER(Scene), u32, u32, fp, fp, fp, up]),
+        "cz_server_config_default": (None, [C.POINTER(ServerConfig)]),
+        "cz_server_create": (i32, [C.POINTER(ServerConfig), C.POINTER(P)]),
+        "cz_server_destroy": (None, [P]),
+        "cz_server_run": (i32, [P, C.POINTER(Request), u32, C.POINTER(Result), C.POINTER(ServerStats)]),
+        "cz_server_film": (P, [P, C.c_uint64]),
+        "cz_batch_create": (i32, [u32, C.POINTER(P)]),
+        "cz_batch_destroy": (None, [P]),
+        "cz_batch_add": (i32, [P, P]),
+        "cz_batch_remove": (i32, [P, P]),
+        "cz_batch_size": (u32, [P]),
+        "cz_batch_step": (i32, [P, f32]),
+        "cz_batch_sync": (i32, [P]),
+        "cz_batch_use_graphs": (i32, [P, i32]),
+        "cz_batch_stream": (C.c_void_p, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -346,6 +365,18 @@ class Film:
         """write_state from a dict of per-part numpy arrays (names of PART_NAMES)."""
         return self.write_state([np.ascontiguousarray(state[k]) for k in PART_NAMES])
 
+    def reshape(self, w, h, settings):
+        """Re-dimension within the created capacity (a fresh film of w x h)."""
+        check(lib().cz_film_reshape(self._p, w, h, C.byref(settings)))
+        self.w, self.h = w, h
+        d = C.c_float()
+        check(lib().cz_film_dims(self._p, None, None, C.byref(d)))
+        self.dx = d.value
+
+    def seed_blobs(self, scene):
+        """Replace the state with a seeded blob canvas generated on the device."""
+        check(lib().cz_film_seed_blobs(self._p, C.byref(scene)))
+
     def dry_now(self):
         check(lib().cz_film_dry_now(self._p))
 
@@ -375,3 +406,130 @@ class Film:
         cnt = C.c_uint64()
         check(lib().cz_film_profile_read(self._p, label.encode(), C.byref(avg), C.byref(cnt)))
         return avg.value, cnt.value
+
+
+class Batch:
+    """Films stepped together, each as `Film.step` would step it alone
+    without deposits, with one launch per kernel phase over all members."""
+
+    def __init__(self, max_members=256):
+        ptr = C.c_void_p()
+        check(lib().cz_batch_create(max_members, C.byref(ptr)))
+        self._p = ptr
+        self.members = []
+
+    def add(self, film):
+        check(lib().cz_batch_add(self._p, film._p))
+        self.members.append(film)
+
+    def remove(self, film):
+        check(lib().cz_batch_remove(self._p, film._p))
+        self.members.remove(film)
+
+    def __len__(self):
+        return lib().cz_batch_size(self._p)
+
+    def step(self, frame_dt=1.0 / 60.0):
+        check(lib().cz_batch_step(self._p, frame_dt))
+
+    def sync(self):
+        check(lib().cz_batch_sync(self._p))
+
+    def use_graphs(self, on=True):
+        check(lib().cz_batch_use_graphs(self._p, 1 if on else 0))
+
+    def stream(self):
+        return lib().cz_batch_stream(self._p)
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().cz_batch_destroy(self._p)
+            self._p = None
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def scene(seed=1, **kw):
+    """cz_scene with the plain workload's defaults, fields overridden by kw."""
+    sc = Scene()
+    lib().cz_scene_default(C.byref(sc))
+    sc.seed = seed
+    for k, v in kw.items():
+        setattr(sc, k, v)
+    return sc
+
+
+def scene_cells_host(w, h, dx, sc, j0=0, j1=None):
+    """Host twin of the device generator for rows [j0, j1): (h, m, rheo, pig0)."""
+    j1 = h if j1 is None else j1
+    rows = j1 - j0
+    hv = np.zeros((rows, w), np.float32)
+    mv = np.zeros((rows, w), np.float32)
+    rv = np.zeros((rows, w, 2), np.float32)
+    pv = np.zeros((rows, w, 8), np.uint32)
+    check(lib().cz_scene_cells_host(w, h, dx, C.byref(sc), j0, j1, _fp(hv), _fp(mv), _fp(rv), _up(pv)))
+    return hv, mv, rv, pv
+
+
+class _Borrowed(Film):
+    """A film owned by the library (a server's kept canvas): readable, not destroyed here."""
+
+    def __init__(self, ptr, w, h):
+        self._p = C.c_void_p(ptr)
+        self.w, self.h = w, h
+        d = C.c_float()
+        check(lib().cz_film_dims(self._p, None, None, C.byref(d)))
+        self.dx = d.value
+
+    def close(self):
+        self._p = None
+
+
+class Server:
+    """The serving loop: requests of seeded canvases, admitted by arrival
+    time under a device-memory budget, stepped together, retired when done."""
+
+    def __init__(self, hbm_budget=200 << 30, max_members=256, frame_dt=1.0 / 60.0, keep_films=False):
+        cfg = ServerConfig()
+        lib().cz_server_config_default(C.byref(cfg))
+        cfg.hbm_budget, cfg.max_members, cfg.frame_dt, cfg.keep_films = hbm_budget, max_members, frame_dt, int(keep_films)
+        ptr = C.c_void_p()
+        check(lib().cz_server_create(C.byref(cfg), C.byref(ptr)))
+        self._p = ptr
+
+    def run(self, requests):
+        """requests: sequence of Request in arrival order -> (results array, ServerStats)."""
+        n = len(requests)
+        reqs = (Request * max(n, 1))(*requests)
+        res = (Result * max(n, 1))()
+        st = ServerStats()
+        check(lib().cz_server_run(self._p, reqs, n, res, C.byref(st)))
+        return list(res)[:n], st
+
+    def film(self, rid, w, h):
+        ptr = lib().cz_server_film(self._p, rid)
+        return _Borrowed(ptr, w, h) if ptr else None
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().cz_server_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def request(rid, w, h, steps, arrival_s, settings, sc):
+    r = Request()
+    r.id, r.w, r.h, r.steps, r.arrival_s = rid, w, h, steps, arrival_s
+    r.settings = settings
+    r.scene = sc
+    return r

@@ -18,6 +18,7 @@ constexpr uint32_t WAKE_BAND = 8u;
 constexpr int PLUG_RADIUS = 3;
 constexpr float YIELD_MENISCUS = 1e-3f;
 constexpr uint32_t MOVED = 64u;
+constexpr uint32_t NEEDS_CURE = 128u;  // k_mark -> compaction only, never stored
 constexpr float W_AXIS = 0.6666667f;
 constexpr float W_DIAG_FLUX = 0.23570226f;
 constexpr float W_DIAG_LAP = 0.16666667f;
@@ -35,6 +36,7 @@ constexpr int TILE64 = 64;
 struct FilmDev {
     cz_params P;
     uint32_t W, H, n, tw, th, nt16, t64x, t64y, pw, ph;
+    uint32_t imp_steps;   // implicit steps of the frame being encoded
     // state
     float* h;
     float* m;
@@ -68,6 +70,8 @@ struct FilmDev {
     uint32_t* tile_list;
     uint32_t* tile_moved;
     uint32_t* active_count;
+    uint32_t* cure_list;      // tiles with cells for the wet cure branch
+    uint32_t* cure_count;
     float2* stats_out;
     float* pool_out;
     float* tsum_a;        // strata sums, fill pass input
@@ -603,6 +607,10 @@ __device__ __forceinline__ void pig_mix(const float* a, float wa, const float* b
     else o[14] = 0.0f;
 }
 
+// The stack operations below index the stack arrays with compile-time
+// indices only (loops fully unrolled, runtime positions turned into selects),
+// so that a Stack lives in registers: a runtime index would place the whole
+// struct in scratch memory.
 __device__ __forceinline__ uint32_t stack_count(const Stack& s) {
     uint32_t n = 0;
 #pragma unroll
@@ -617,13 +625,15 @@ __device__ inline void copy16(float* d, const float* s) {
 }
 
 // strata.wgsl:69-83
-__device__ inline void stack_merge(Stack& s, uint32_t a) {
-    uint32_t b = a + 1u;
-    float ta = s.th[a], tb = s.th[b];
-    pig_mix(s.pg[a], ta, s.pg[b], tb, s.pg[a]);
-    s.cu[a] = (s.cu[a] * ta + s.cu[b] * tb) / cz_max(ta + tb, 1e-20f);
-    s.th[a] = ta + tb;
-    for (uint32_t k = b; k < 3u; k++) {
+template <uint32_t A>
+__device__ __forceinline__ void stack_merge_at(Stack& s) {
+    constexpr uint32_t B = A + 1u;
+    float ta = s.th[A], tb = s.th[B];
+    pig_mix(s.pg[A], ta, s.pg[B], tb, s.pg[A]);
+    s.cu[A] = (s.cu[A] * ta + s.cu[B] * tb) / cz_max(ta + tb, 1e-20f);
+    s.th[A] = ta + tb;
+#pragma unroll
+    for (uint32_t k = B; k < 3u; k++) {
         s.th[k] = s.th[k + 1u];
         s.cu[k] = s.cu[k + 1u];
         copy16(s.pg[k], s.pg[k + 1u]);
@@ -631,10 +641,17 @@ __device__ inline void stack_merge(Stack& s, uint32_t a) {
     s.th[3] = 0.0f;
     s.cu[3] = 0.0f;
 }
+__device__ __forceinline__ void stack_merge(Stack& s, uint32_t a) {
+    if (a == 0u) stack_merge_at<0>(s);
+    else if (a == 1u) stack_merge_at<1>(s);
+    else stack_merge_at<2>(s);
+}
 
 // strata.wgsl:85-103
 __device__ inline void stack_compact(const FilmDev& F, Stack& s) {
+#pragma unroll
     for (int sweep = 0; sweep < 4; sweep++) {
+#pragma unroll
         for (uint32_t k = 0; k < 3u; k++) {
             if (s.th[k] <= 0.0f && s.th[k + 1u] > 0.0f) {
                 s.th[k] = s.th[k + 1u];
@@ -646,6 +663,7 @@ __device__ inline void stack_compact(const FilmDev& F, Stack& s) {
         }
     }
     float tiny = F.P.misc[0];
+#pragma unroll
     for (uint32_t k = 0; k < 3u; k++) {
         if (s.th[k] > 0.0f && s.th[k] < tiny && s.th[k + 1u] > 0.0f) stack_merge(s, k);
     }
@@ -655,29 +673,37 @@ __device__ inline void stack_compact(const FilmDev& F, Stack& s) {
 __device__ inline void stack_push_top(const FilmDev& F, Stack& s, float vol, const float* pig, float cure) {
     if (vol <= 0.0f) return;
     uint32_t n = stack_count(s);
-    if (n > 0u) {
-        uint32_t t = n - 1u;
+    bool done = false;
+#pragma unroll
+    for (uint32_t t = 0; t < 4u; t++) {  // the top stratum t = n - 1
+        if (t + 1u != n) continue;
         bool close = pig_dist(s.pg[t], pig) < F.P.misc[1] || s.th[t] < F.P.misc[0];
         if (close) {
             pig_mix(s.pg[t], s.th[t], pig, vol, s.pg[t]);
             s.cu[t] = (s.cu[t] * s.th[t] + cure * vol) / (s.th[t] + vol);
             s.th[t] = s.th[t] + vol;
-            return;
+            done = true;
         }
     }
+    if (done) return;
     if (n == 4u) {
         uint32_t best = 0;
         float bd = 1e30f;
+#pragma unroll
         for (uint32_t k = 0; k < 3u; k++) {
             float d = pig_dist(s.pg[k], s.pg[k + 1u]);
             if (d < bd) { bd = d; best = k; }
         }
         stack_merge(s, best);
     }
-    uint32_t t = stack_count(s);
-    s.th[t] = vol;
-    copy16(s.pg[t], pig);
-    s.cu[t] = cure;
+    const uint32_t top = stack_count(s);
+#pragma unroll
+    for (uint32_t t = 0; t < 4u; t++) {
+        if (t != top) continue;
+        s.th[t] = vol;
+        copy16(s.pg[t], pig);
+        s.cu[t] = cure;
+    }
 }
 
 // strata.wgsl:133-155
@@ -687,9 +713,14 @@ __device__ inline void stack_add_band(const FilmDev& F, Stack& s, float zlo, flo
     float span = cz_max(zhi - zlo, 1e-12f);
     float z0 = 0.0f;
     float above = vol;
+    bool ended = false;
+#pragma unroll
     for (uint32_t k = 0; k < 4u; k++) {
         float tk = s.th[k];
-        if (tk <= 0.0f) break;
+        if (ended || tk <= 0.0f) {
+            ended = true;
+            continue;
+        }
         float z1 = z0 + tk;
         float ov = cz_max(cz_min(zhi, z1) - cz_max(zlo, z0), 0.0f);
         float part = vol * ov / span;
@@ -726,6 +757,7 @@ __device__ __forceinline__ float composite(float rtx, float rty, float rb) {
 }
 __device__ __forceinline__ void layer_ks(const float* pigtab, const float* pig, uint32_t band, float& ko, float& so) {
     float k = 0.0f, s = 0.0f, amount = 0.0f, weight = 0.0f;
+#pragma unroll
     for (uint32_t i = 0; i < (uint32_t)NPIG; i++) {
         float c = pig[i];
         if (c > 0.0f) {

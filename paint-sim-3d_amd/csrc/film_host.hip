@@ -13,146 +13,20 @@
 #include <string>
 #include <vector>
 
-#include "film_kernels.h"
+#include "film_internal.h"
 #include "host_ref.h"
 
 using namespace cz;
 
 static thread_local char g_err[512] = "";
 
-static int set_err(int code, const char* fmt, ...) {
+int cz_set_err(int code, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
     return code;
 }
-
-#define HIP_TRY(expr)                                                                     \
-    do {                                                                                  \
-        hipError_t _e = (expr);                                                           \
-        if (_e != hipSuccess) return set_err(CZ_EHIP, "%s: %s", #expr, hipGetErrorString(_e)); \
-    } while (0)
-
-struct GraphKeyStore {
-    unsigned char bytes[7 * 4 + sizeof(cz_params)];
-};
-
-namespace {
-
-constexpr uint32_t kMaxTileBlocks = 2048u;  // resident 256-thread blocks on 256 CUs
-
-struct ProfSlot {
-    std::string label;
-    hipEvent_t a, b;
-};
-
-}  // namespace
-
-struct cz_film {
-    uint32_t W = 0, H = 0, n = 0, tw = 0, th = 0, nt16 = 0, t64x = 0, t64y = 0, pw = 0, ph = 0;
-    float dx = 0.0f;
-    cz_settings s;
-    cz_overrides ov;
-    bool ov_from_env = true;
-    cz_params P;
-    float h_max = 0.0f, mu_min = 1.0f, h_max_pending = 0.0f;
-    uint32_t implicit_steps = 1, substeps_last = 1, frame = 0, active_tiles = 0;
-    uint32_t active_hint = 0xffffffffu;  // grid sizing: last folded active count
-    double total_deposited = 0.0, weave_volume = 0.0;
-    float cg_report[4] = {0, 0, 0, 0};
-    bool stats_pending = false;
-    uint64_t cnt_act_cells = 0, cnt_act_cell_substeps = 0, cnt_frames = 0;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::vector<void*> allocs;
-    // device buffers
-    float *h = nullptr, *m = nullptr;
-    float2* rheo = nullptr;
-    float4 *strata_h = nullptr, *strata_cure = nullptr, *strata_h2 = nullptr, *strata_cure2 = nullptr;
-    uint4 *strata_pig = nullptr, *strata_pig2 = nullptr;
-    uint32_t *sub_refl = nullptr, *lay = nullptr;
-    float2 *pres = nullptr, *fnorm = nullptr;
-    float4 *flux = nullptr, *acc = nullptr, *acc_s = nullptr, *alpha_s = nullptr, *cg_m = nullptr;
-    float *alpha = nullptr, *tmp = nullptr, *h_frame = nullptr;
-    uint32_t *delta_a = nullptr, *delta_b = nullptr;
-    bool delta_cur_a = true;
-    float *cg_d = nullptr, *cg_x = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr, *cg_t = nullptr,
-          *cg_t2 = nullptr;
-    double *cg_part = nullptr, *cg_part2 = nullptr;
-    float* cg_s = nullptr;
-    uint32_t *tile_flags = nullptr, *tile_list = nullptr, *tile_moved = nullptr, *active_count = nullptr;
-    float2* stats_out = nullptr;
-    float* pool_out = nullptr;
-    float *tsum_a = nullptr, *tsum_b = nullptr;
-    float* pigtab = nullptr;
-    cz_deposit* deposits = nullptr;
-    uint32_t *tile_start = nullptr, *tile_count = nullptr, *tile_items = nullptr;
-    float* frame_out = nullptr;
-    // host tables and pinned staging
-    float pigtab_h[CZ_NPIG * CZ_PIG_STRIDE];
-    float cmf_h[CZ_NBAND * 3];
-    float* frame_out_h = nullptr;
-    cz_deposit* dep_h = nullptr;
-    uint32_t *tstart_h = nullptr, *tcount_h = nullptr, *titems_h = nullptr;
-    uint32_t deposit_count = 0;
-    // profiling
-    bool use_graphs = true;
-    std::vector<std::pair<struct GraphKeyStore, hipGraphExec_t>> graphs;
-    bool prof_on = false;
-    std::string prof_only;
-    std::vector<ProfSlot> prof_pending;
-    std::vector<hipEvent_t> ev_pool;
-    std::map<std::string, std::pair<double, uint64_t>> prof_acc;
-
-    uint32_t* delta_cur() const { return delta_cur_a ? delta_a : delta_b; }
-    uint32_t* delta_other() const { return delta_cur_a ? delta_b : delta_a; }
-
-    FilmDev dev() const {
-        FilmDev F;
-        memset(&F, 0, sizeof(F));
-        F.P = P;
-        F.W = W; F.H = H; F.n = n; F.tw = tw; F.th = th; F.nt16 = nt16; F.t64x = t64x; F.t64y = t64y;
-        F.pw = pw; F.ph = ph;
-        F.h = h; F.m = m; F.rheo = rheo; F.strata_h = strata_h; F.strata_pig = strata_pig;
-        F.strata_cure = strata_cure; F.sub_refl = sub_refl; F.lay = lay;
-        F.pres = pres; F.flux = flux; F.acc = acc; F.alpha = alpha;
-        F.delta = delta_cur(); F.delta_out = delta_other();
-        F.tmp = tmp; F.acc_s = acc_s; F.alpha_s = alpha_s; F.strata_h2 = strata_h2; F.strata_pig2 = strata_pig2;
-        F.strata_cure2 = strata_cure2; F.h_frame = h_frame; F.fnorm = fnorm; F.cg_m = cg_m;
-        F.cg_d = cg_d; F.cg_x = cg_x; F.cg_r = cg_r; F.cg_p = cg_p; F.cg_q = cg_q; F.cg_t = cg_t; F.cg_t2 = cg_t2;
-        F.cg_part = cg_part; F.cg_part2 = cg_part2; F.cg_s = cg_s;
-        F.tile_flags = tile_flags; F.tile_list = tile_list; F.tile_moved = tile_moved; F.active_count = active_count;
-        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.tsum_b = tsum_b; F.pigtab = pigtab;
-        F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
-        F.frame_out = frame_out;
-        return F;
-    }
-
-    // Grid of the tile kernels: the kernels walk the device-side list with a
-    // grid-stride loop, so any size is correct; size it from the last known
-    // active count (4 work items per tile for the face-parallel flux).
-    uint32_t tile_grid(uint32_t per_tile = 1) const {
-        uint64_t est = (uint64_t)std::max(std::min(active_hint, nt16), 1u) * per_tile;
-        est = est + est / 4u + 16u;
-        uint64_t p2 = 64;  // power of two: few distinct grids, few distinct graphs
-        while (p2 < est) p2 <<= 1;
-        est = p2;
-        uint64_t cap = std::min<uint64_t>((uint64_t)nt16 * per_tile, (uint64_t)kMaxTileBlocks * per_tile);
-        return (uint32_t)std::max<uint64_t>(1u, std::min(est, cap));
-    }
-
-    hipEvent_t take_event() {
-        if (!ev_pool.empty()) {
-            hipEvent_t e = ev_pool.back();
-            ev_pool.pop_back();
-            return e;
-        }
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        return e;
-    }
-};
 
 namespace {
 
@@ -163,6 +37,7 @@ int dalloc(cz_film* f, T** p, size_t count) {
     hipError_t e = hipMalloc(&v, bytes);
     if (e != hipSuccess) return set_err(CZ_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
     f->allocs.push_back(v);
+    f->bytes += bytes;
     *p = (T*)v;
     e = hipMemsetAsync(v, 0, bytes, f->stream);
     if (e != hipSuccess) return set_err(CZ_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
@@ -203,19 +78,17 @@ void fold_profile(cz_film* f) {
     f->prof_pending.clear();
 }
 
-// update_params (src/film/mod.rs:869-917) on the film's host state.
+}  // namespace
+
+namespace czi {
+
 void update_params(cz_film* f, float frame_dt, uint32_t substeps) {
     if (f->ov.no_implicit) f->s.implicit_hydro = 0;
     cz_host_state hs = {f->h_max, f->mu_min, f->h_max_pending, 0u};
     cz_update_params_host(&f->s, &f->ov, f->W, f->H, frame_dt, substeps, &hs, &f->P, &f->implicit_steps);
 }
 
-// poll_stats (src/film/mod.rs:935-973) on the frame scalars of the last frame.
-int poll_stats(cz_film* f) {
-    if (!f->stats_pending) return CZ_OK;
-    HIP_TRY(hipStreamSynchronize(f->stream));
-    fold_profile(f);
-    const float* o = f->frame_out_h;
+void fold_frame_out(cz_film* f, const float* o) {
     f->h_max = o[0];
     f->mu_min = (o[1] > 1e8f) ? 1.0f : o[1];
     f->active_tiles = cz_f2bits(o[2]);
@@ -226,13 +99,55 @@ int poll_stats(cz_film* f) {
     uint64_t cells = cz_f2bits(o[3]);
     f->cnt_act_cells += cells;
     f->cnt_act_cell_substeps += cells * (uint64_t)f->substeps_last;
+    f->cnt_act_cell_implicit += cells * (uint64_t)f->implicit_last;
     f->cnt_frames += 1;
-    return CZ_OK;
 }
 
 int wait_idle(cz_film* f) {
     HIP_TRY(hipStreamSynchronize(f->stream));
     fold_profile(f);
+    return CZ_OK;
+}
+
+int reset_solver_state(cz_film* f) {
+    uint64_t n = f->n;
+    HIP_TRY(hipMemsetAsync(f->pres, 0, 8 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->flux, 0, 16 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->acc, 0, 16 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->alpha, 0, 4 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->delta_a, 0, 4 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->delta_b, 0, 4 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->tmp, 0, 4 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->acc_s, 0, 64 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->strata_h2, 0, 16 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->strata_pig2, 0, 128 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->alpha_s, 0, 16 * n, f->stream));
+    HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)f->tile_moved, 1, f->nt16, f->stream));
+    f->active_hint = f->nt16;
+    return wait_idle(f);
+}
+
+void frame_done(cz_film* f, uint32_t substeps, bool parity_done) {
+    if (!parity_done && (substeps & 1u)) f->delta_cur_a = !f->delta_cur_a;
+    f->stats_pending = true;
+    f->substeps_last = substeps;
+    f->implicit_last = f->s.implicit_hydro ? f->implicit_steps : 0u;
+    f->frame++;
+    f->deposit_count = 0;
+}
+
+}  // namespace czi
+
+using namespace czi;
+
+namespace {
+
+// poll_stats (src/film/mod.rs:935-973) on the frame scalars of the last frame.
+int poll_stats(cz_film* f) {
+    if (!f->stats_pending) return CZ_OK;
+    HIP_TRY(hipStreamSynchronize(f->stream));
+    fold_profile(f);
+    fold_frame_out(f, f->frame_out_h);
     return CZ_OK;
 }
 
@@ -284,11 +199,13 @@ int upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* se
                 if (fill[t] < f->tcount_h[t]) f->titems_h[f->tstart_h[t] + fill[t]++] = di;
             }
     }
-    if (count > 0) memcpy(f->dep_h, deps, (size_t)count * sizeof(cz_deposit));
-    if (count > 0) HIP_TRY(hipMemcpyAsync(f->deposits, f->dep_h, (size_t)count * sizeof(cz_deposit), hipMemcpyHostToDevice, f->stream));
-    HIP_TRY(hipMemcpyAsync(f->tile_start, f->tstart_h, nt * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
-    HIP_TRY(hipMemcpyAsync(f->tile_count, f->tcount_h, nt * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
-    if (items > 0) HIP_TRY(hipMemcpyAsync(f->tile_items, f->titems_h, items * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+    if (count > 0) {  // the deposit pass runs only when there are deposits
+        memcpy(f->dep_h, deps, (size_t)count * sizeof(cz_deposit));
+        HIP_TRY(hipMemcpyAsync(f->deposits, f->dep_h, (size_t)count * sizeof(cz_deposit), hipMemcpyHostToDevice, f->stream));
+        HIP_TRY(hipMemcpyAsync(f->tile_start, f->tstart_h, nt * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+        HIP_TRY(hipMemcpyAsync(f->tile_count, f->tcount_h, nt * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+        if (items > 0) HIP_TRY(hipMemcpyAsync(f->tile_items, f->titems_h, items * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+    }
     if (sent) *sent = count;
     return CZ_OK;
 }
@@ -302,77 +219,64 @@ int upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* se
 // encode_frame (src/film/mod.rs:1322-1462): the launches of one frame.
 int record_frame(cz_film* f, uint32_t substeps) {
     const dim3 b256(256);
-    const dim3 gcell((f->W + 15u) / 16u, (f->H + 15u) / 16u);
     const dim3 gt(f->tile_grid());
+    const uint32_t n64 = f->t64x * f->t64y;
     FilmDev F = f->dev();
-    if (f->deposit_count > 0) LAUNCH("deposit", k_deposit, gcell, b256, F);
-    LAUNCH("mark", k_mark_tiles, dim3(f->nt16), b256, F);
-    LAUNCH("compact", k_compact_tiles, dim3(1), dim3(1024), F);
-    LAUNCH("cure", k_cure_clear, gcell, b256, F, 1);
-    if (f->s.implicit_hydro) {
+    if (f->deposit_count > 0) {
+        const dim3 gcell((f->W + 15u) / 16u, (f->H + 15u) / 16u);
+        LAUNCH("deposit", k_deposit, gcell, b256, F);
+    }
+    LAUNCH("mark", k_mark<OneFilmAll>, dim3((f->nt16 + 3u) / 4u), b256, OneFilmAll{F, f->nt16});
+    LAUNCH("compact", k_compact_tiles<OneCanvas>, dim3(1), dim3(1024), OneCanvas{F});
+    LAUNCH("cure", k_cure<OneFilmCure>, dim3(std::min(f->nt16, czi::kMaxTileBlocks)), b256, OneFilmCure{F});
+    const bool implicit = f->s.implicit_hydro && f->implicit_steps > 0;
+    if (implicit) {
         for (uint32_t it = 0; it < f->implicit_steps; it++) {
-            F = f->dev();
-            LAUNCH("lap", k_lap_plug, gt, b256, F, 0, 0);
-            LAUNCH("cg_setup", k_cg_setup, gt, b256, F);
-            LAUNCH("cg_lap", k_cg_lap, gt, b256, F, 1u);
-            LAUNCH("cg_init", k_cg_init, gt, b256, F);
-            LAUNCH("cg_reduce", k_cg_reduce, dim3(1), b256, F, 0);
+            const OneFilm w{f->dev()};
+            const OneCanvas c1{w.F};
+            LAUNCH("lap", k_lap_plug<OneFilm>, gt, b256, w, 0, 0, it == 0 ? 1 : 0);
+            LAUNCH("cg_setup", k_cg_setup<OneFilm>, gt, b256, w);
+            LAUNCH("cg_lap", k_cg_lap<OneFilm>, gt, b256, w, 1u);
+            LAUNCH("cg_init", k_cg_init<OneFilm>, gt, b256, w);
+            LAUNCH("cg_reduce", k_cg_reduce<OneCanvas>, dim3(1), b256, c1, 0);
             for (uint32_t k = 0; k < f->s.implicit_iters; k++) {
-                LAUNCH("cg_lap", k_cg_lap, gt, b256, F, 0u);
-                LAUNCH("cg_iter_a", k_cg_iter_a, gt, b256, F);
-                LAUNCH("cg_reduce", k_cg_reduce, dim3(1), b256, F, 1);
-                LAUNCH("cg_iter_b", k_cg_iter_b, gt, b256, F);
-                LAUNCH("cg_reduce", k_cg_reduce, dim3(1), b256, F, 2);
-                LAUNCH("cg_iter_c", k_cg_iter_c, gt, b256, F);
+                LAUNCH("cg_lap", k_cg_lap<OneFilm>, gt, b256, w, 0u);
+                LAUNCH("cg_iter_a", k_cg_iter_a<OneFilm>, gt, b256, w);
+                LAUNCH("cg_reduce", k_cg_reduce<OneCanvas>, dim3(1), b256, c1, 1);
+                LAUNCH("cg_iter_b", k_cg_iter_b<OneFilm>, gt, b256, w);
+                LAUNCH("cg_reduce", k_cg_reduce<OneCanvas>, dim3(1), b256, c1, 2);
+                LAUNCH("cg_iter_c", k_cg_iter_c<OneFilm>, gt, b256, w);
             }
-            LAUNCH("cg_lap", k_cg_lap, gt, b256, F, 4u);
-            LAUNCH("cg_finish", k_cg_finish, gt, b256, F);
-            LAUNCH("cg_reduce", k_cg_reduce, dim3(1), b256, F, 3);
-            LAUNCH("cg_rescale", k_cg_rescale, gt, b256, F);
+            LAUNCH("cg_lap", k_cg_lap<OneFilm>, gt, b256, w, 4u);
+            LAUNCH("cg_finish", k_cg_finish<OneFilm>, gt, b256, w);
+            LAUNCH("cg_reduce", k_cg_reduce<OneCanvas>, dim3(1), b256, c1, 3);
+            LAUNCH("cg_rescale", k_cg_rescale<OneFilm>, gt, b256, w);
         }
     }
     // substeps: [nose of the previous substep +] lap + plug, flux, limiter + apply
     for (uint32_t s = 0; s < substeps; s++) {
-        F = f->dev();
-        LAUNCH("lap_plug", k_lap_plug, gt, b256, F, s > 0 ? 1 : 0, 1);
-        if (s > 0) {
-            f->delta_cur_a = !f->delta_cur_a;
-            F = f->dev();
-        }
-        LAUNCH("flux", k_fluxes, dim3(f->tile_grid(4)), b256, F);
-        LAUNCH("apply", k_limit_apply, gt, b256, F);
+        LAUNCH("lap_plug", k_lap_plug<OneFilm>, gt, b256, OneFilm{f->dev()}, s > 0 ? 1 : 0, 1,
+               (s == 0 && !implicit) ? 1 : 0);
+        if (s > 0) f->delta_cur_a = !f->delta_cur_a;
+        const OneFilm w{f->dev()};
+        LAUNCH("flux", k_fluxes<OneFilm>, dim3(f->tile_grid(4)), b256, w);
+        LAUNCH("apply", k_limit_apply<OneFilm>, gt, b256, w);
     }
     // nose of the last substep + the laplacian strata_split reads
-    F = f->dev();
-    LAUNCH("lap", k_lap_plug, gt, b256, F, 1, 0);
+    LAUNCH("lap", k_lap_plug<OneFilm>, gt, b256, OneFilm{f->dev()}, 1, 0, 0);
     f->delta_cur_a = !f->delta_cur_a;
-    F = f->dev();
-    LAUNCH("delta_sync", k_delta_sync, gt, b256, F, f->delta_other());
-    LAUNCH("split", k_strata_split, gt, b256, F);
-    LAUNCH("reduce_pool", k_reduce_pool, dim3(f->t64x, f->t64y), b256, F);
-    LAUNCH("strata_limit", k_strata_limit, gt, b256, F);
-    LAUNCH("strata_update", k_strata_update, gt, b256, F);
-    LAUNCH("copy_back", k_strata_copy_back, gt, b256, F);
-    float* tin = f->tsum_a;
-    float* tout = f->tsum_b;
-    for (int k = 0; k < 6; k++) {
-        LAUNCH("strata_fill", k_strata_fill, gt, b256, F, (const float*)tin, tout);
-        std::swap(tin, tout);
-    }
-    LAUNCH("frame_stats", k_frame_stats, dim3(1), b256, F);
+    const OneFilm w{f->dev()};
+    LAUNCH("delta_sync", k_delta_sync<OneFilm>, gt, b256, w);
+    LAUNCH("split", k_strata_split<OneFilm>, gt, b256, w);
+    LAUNCH("reduce_pool", k_reduce_pool<OneFilmAll>, dim3(n64), b256, OneFilmAll{w.F, n64});
+    LAUNCH("strata_limit", k_strata_limit<OneFilm>, gt, b256, w);
+    LAUNCH("strata_update", k_strata_update<OneFilm>, gt, b256, w);
+    LAUNCH("copy_back", k_strata_copy_back<OneFilm>, gt, b256, w);
+    for (int k = 0; k < 6; k++) LAUNCH("strata_fill", k_strata_fill<OneFilm>, gt, b256, w, k);
+    LAUNCH("frame_stats", k_frame_stats<OneCanvas>, dim3(1), b256, OneCanvas{w.F});
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(f->frame_out_h, f->frame_out, 16 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
     return CZ_OK;
-}
-
-// Host-side effects of one frame: the offset buffer written last, the stats
-// readback in flight.
-void frame_done(cz_film* f, uint32_t substeps, bool parity_done) {
-    if (!parity_done && (substeps & 1u)) f->delta_cur_a = !f->delta_cur_a;
-    f->stats_pending = true;
-    f->substeps_last = substeps;
-    f->frame++;
-    f->deposit_count = 0;
 }
 
 // The frame as a hipGraph, captured once per frame shape and replayed: the
@@ -496,6 +400,7 @@ int init_state(cz_film* f) {
 
 void destroy(cz_film* f) {
     if (!f) return;
+    batch_detach(f);
     if (f->stream) hipStreamSynchronize(f->stream);
     for (void* p : f->allocs) hipFree(p);
     if (f->frame_out_h) hipHostFree(f->frame_out_h);
@@ -520,9 +425,48 @@ extern "C" {
 const char* cz_last_error(void) { return g_err; }
 const char* cz_version(void) { return "canvaz-film-mi355x 0.1 (gfx950)"; }
 
-int cz_film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigtab, const float* cmf, cz_film** out) {
+}  // extern "C"
+
+namespace {
+
+bool set_dims(cz_film* f, uint32_t w, uint32_t h) {
+    f->W = w; f->H = h; f->n = w * h;
+    f->tw = (w + 15u) / 16u; f->th = (h + 15u) / 16u; f->nt16 = f->tw * f->th;
+    f->t64x = (w + 63u) / 64u; f->t64y = (h + 63u) / 64u;
+    f->pw = (w + 3u) / 4u; f->ph = (h + 3u) / 4u;
+    f->dx = f->s.width_m / (float)w;
+    return f->n <= f->cap_n && f->nt16 <= f->cap_nt16 && f->t64x * f->t64y <= f->cap_nt64 &&
+           f->pw * f->ph <= f->cap_np;
+}
+
+// Host state of a film that starts afresh (Film::new): lag-1 stats,
+// counters, parities, params of update_params(1/60, 1).
+void reset_host_state(cz_film* f) {
+    f->h_max = 0.0f; f->mu_min = 1.0f; f->h_max_pending = 0.0f;
+    f->implicit_steps = 1; f->substeps_last = 1; f->implicit_last = 0; f->frame = 0; f->active_tiles = 0;
+    f->active_hint = f->nt16;
+    f->total_deposited = 0.0;
+    for (int k = 0; k < 4; k++) f->cg_report[k] = 0.0f;
+    f->stats_pending = false;
+    f->cnt_act_cells = f->cnt_act_cell_substeps = f->cnt_frames = f->cnt_act_cell_implicit = 0;
+    f->delta_cur_a = true;
+    f->deposit_count = 0;
+    memset(&f->P, 0, sizeof(f->P));
+    f->P.size[0] = f->W; f->P.size[1] = f->H;
+    f->P.tiles[0] = f->t64x; f->P.tiles[1] = f->t64y;
+    f->P.dx = f->dx; f->P.inv_dx = 1.0f / f->dx;
+    update_params(f, 1.0f / 60.0f, 1);
+}
+
+}  // namespace
+
+namespace czi {
+
+int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigtab, const float* cmf, bool init,
+                cz_film** out) {
     if (!out || w == 0 || h == 0) return set_err(CZ_EINVAL, "cz_film_create: bad size %ux%u", w, h);
     if ((uint64_t)w * h > (1ull << 31)) return set_err(CZ_EINVAL, "cz_film_create: %ux%u cells exceed 2^31", w, h);
+    if ((w + 15u) / 16u > 21845u) return set_err(CZ_EINVAL, "cz_film_create: width %u exceeds 349520 cells", w);
     *out = nullptr;
     cz_film* f = new cz_film();
     if (s) f->s = *s; else cz_settings_default(&f->s);
@@ -531,11 +475,11 @@ int cz_film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pi
         return set_err(CZ_EUNSUP, "cz_film_create: the slump pass (settings.slump) is not built");
     }
     cz_overrides_from_env(&f->ov);
-    f->W = w; f->H = h; f->n = w * h;
-    f->tw = (w + 15u) / 16u; f->th = (h + 15u) / 16u; f->nt16 = f->tw * f->th;
-    f->t64x = (w + 63u) / 64u; f->t64y = (h + 63u) / 64u;
-    f->pw = (w + 3u) / 4u; f->ph = (h + 3u) / 4u;
-    f->dx = f->s.width_m / (float)w;
+    f->cap_n = w * h;
+    f->cap_nt16 = ((w + 15u) / 16u) * ((h + 15u) / 16u);
+    f->cap_nt64 = ((w + 63u) / 64u) * ((h + 63u) / 64u);
+    f->cap_np = ((w + 3u) / 4u) * ((h + 3u) / 4u);
+    set_dims(f, w, h);
     if (pigtab) memcpy(f->pigtab_h, pigtab, sizeof(f->pigtab_h));
     else {
         float refl[12 * 16], sc[12], ti[12], vi[12], yi[12];
@@ -558,7 +502,7 @@ int cz_film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pi
     A(acc_s, 4 * n); A(alpha_s, n); A(strata_h2, n); A(strata_pig2, 8 * n); A(strata_cure2, n); A(h_frame, n);
     A(fnorm, n); A(cg_m, n); A(cg_d, n); A(cg_x, n); A(cg_r, n); A(cg_p, n); A(cg_q, n); A(cg_t, n); A(cg_t2, n);
     A(cg_part, nt16); A(cg_part2, nt16); A(cg_s, 8); A(tile_flags, nt16); A(tile_list, nt16); A(tile_moved, nt16);
-    A(active_count, 4); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n); A(tsum_b, n);
+    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n); A(tsum_b, n);
     A(pigtab, CZ_NPIG * CZ_PIG_STRIDE); A(deposits, CZ_MAX_DEPOSITS); A(tile_start, nt64); A(tile_count, nt64);
     A(tile_items, CZ_MAX_TILE_ITEMS); A(frame_out, 16);
 #undef A
@@ -574,17 +518,58 @@ int cz_film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pi
         e = hipMemcpyAsync(f->pigtab, f->pigtab_h, sizeof(f->pigtab_h), hipMemcpyHostToDevice, f->stream);
         if (e != hipSuccess) rc = set_err(CZ_EHIP, "pigtab upload: %s", hipGetErrorString(e));
     }
-    if (!rc) rc = init_state(f);
+    if (!rc && init) rc = init_state(f);
+    if (!rc && !init) {
+        e = hipStreamSynchronize(f->stream);
+        if (e != hipSuccess) rc = set_err(CZ_EHIP, "cz_film_create: %s", hipGetErrorString(e));
+    }
     if (rc) {
         destroy(f);
         return rc;
     }
-    f->P.size[0] = w; f->P.size[1] = h;
-    f->P.tiles[0] = f->t64x; f->P.tiles[1] = f->t64y;
-    f->P.dx = f->dx; f->P.inv_dx = 1.0f / f->dx;
-    update_params(f, 1.0f / 60.0f, 1);
+    reset_host_state(f);
     *out = f;
     return CZ_OK;
+}
+
+int film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s) {
+    if (f->batch) return set_err(CZ_EINVAL, "reshape: the film is a batch member");
+    if (s->slump) return set_err(CZ_EUNSUP, "the slump pass (settings.slump) is not built");
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    const cz_settings keep = f->s;
+    f->s = *s;
+    if (w == 0 || h == 0 || !set_dims(f, w, h)) {
+        f->s = keep;
+        set_dims(f, f->W, f->H);
+        return set_err(CZ_ESIZE, "reshape: %ux%u exceeds the film's capacity", w, h);
+    }
+    for (auto& g : f->graphs) hipGraphExecDestroy(g.second);
+    f->graphs.clear();
+    f->weave_volume = 0.0;
+    reset_host_state(f);
+    // buffers a fresh film starts with at zero and may read at cells outside
+    // the active tiles before writing them (the others are written first or
+    // cleared with the state parts)
+    const uint64_t n = f->n;
+    HIP_TRY(hipMemsetAsync(f->fnorm, 0, 8 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->cg_m, 0, 16 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->cg_p, 0, 4 * n, f->stream));
+    HIP_TRY(hipMemsetAsync(f->h_frame, 0, 4 * n, f->stream));
+    return wait_idle(f);
+}
+
+}  // namespace czi
+
+extern "C" {
+
+int cz_film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigtab, const float* cmf, cz_film** out) {
+    return film_create(w, h, s, pigtab, cmf, true, out);
+}
+
+int cz_film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s) {
+    if (!f || !s) return set_err(CZ_EINVAL, "null argument");
+    return film_reshape(f, w, h, s);
 }
 
 void cz_film_destroy(cz_film* f) { destroy(f); }
@@ -651,17 +636,20 @@ int cz_film_set_params(cz_film* f, const cz_params* p) {
 }
 
 int cz_film_upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* sent) {
+    if (f && f->batch) return set_err(CZ_EINVAL, "upload_deposits: the film is stepped by a batch (cz_batch_remove it first)");
     if (!f || (n > 0 && !deps)) return set_err(CZ_EINVAL, "upload_deposits: bad arguments");
     return upload_deposits(f, deps, n, sent);
 }
 
 int cz_film_encode_frame(cz_film* f, uint32_t substeps) {
+    if (f && f->batch) return set_err(CZ_EINVAL, "encode_frame: the film is stepped by a batch (cz_batch_remove it first)");
     if (!f || substeps == 0) return set_err(CZ_EINVAL, "encode_frame: bad arguments");
     return encode_frame(f, substeps);
 }
 
 // encode_sim (src/main.rs:1875-1909) with stats of the previous frame.
 int cz_film_step(cz_film* f, const cz_deposit* deps, uint32_t n, float frame_dt, uint32_t* sent) {
+    if (f && f->batch) return set_err(CZ_EINVAL, "step: the film is stepped by a batch (cz_batch_remove it first)");
     if (!f || (n > 0 && !deps) || !(frame_dt > 0.0f)) return set_err(CZ_EINVAL, "step: bad arguments");
     int rc = poll_stats(f);
     if (rc) return rc;
@@ -695,6 +683,7 @@ int cz_film_stats(cz_film* f, cz_stats* st) {
     st->act_cells = f->cnt_act_cells;
     st->act_cell_substeps = f->cnt_act_cell_substeps;
     st->frames_counted = f->cnt_frames;
+    st->act_cell_implicit = f->cnt_act_cell_implicit;
     return CZ_OK;
 }
 
@@ -755,6 +744,7 @@ int cz_film_read_state(cz_film* f, void* const parts[CZ_NPARTS]) {
 
 // write_state (src/film/mod.rs:1303-1320)
 int cz_film_write_state(cz_film* f, const void* const parts[CZ_NPARTS], const uint64_t sizes[CZ_NPARTS]) {
+    if (f && f->batch) return set_err(CZ_EINVAL, "write_state: the film is stepped by a batch (cz_batch_remove it first)");
     if (!f || !parts || !sizes) return set_err(CZ_EINVAL, "null argument");
     for (int k = 0; k < CZ_NPARTS; k++) {
         if (!parts[k] || sizes[k] != buffer_of(f, k).bytes)
@@ -767,22 +757,7 @@ int cz_film_write_state(cz_film* f, const void* const parts[CZ_NPARTS], const ui
         BufDesc b = buffer_of(f, k);
         HIP_TRY(hipMemcpyAsync(b.ptr, parts[k], b.bytes, hipMemcpyHostToDevice, f->stream));
     }
-    uint64_t n = f->n;
-    HIP_TRY(hipMemsetAsync(f->pres, 0, 8 * n, f->stream));
-    HIP_TRY(hipMemsetAsync(f->flux, 0, 16 * n, f->stream));
-    HIP_TRY(hipMemsetAsync(f->acc, 0, 16 * n, f->stream));
-    HIP_TRY(hipMemsetAsync(f->alpha, 0, 4 * n, f->stream));
-    HIP_TRY(hipMemsetAsync(f->delta_a, 0, 4 * n, f->stream));
-    HIP_TRY(hipMemsetAsync(f->delta_b, 0, 4 * n, f->stream));
-    HIP_TRY(hipMemsetAsync(f->tmp, 0, 4 * n, f->stream));
-    HIP_TRY(hipMemsetAsync(f->acc_s, 0, 64 * n, f->stream));
-    HIP_TRY(hipMemsetAsync(f->strata_h2, 0, 16 * n, f->stream));
-    HIP_TRY(hipMemsetAsync(f->strata_pig2, 0, 128 * n, f->stream));
-    HIP_TRY(hipMemsetAsync(f->alpha_s, 0, 16 * n, f->stream));
-    std::vector<uint32_t> ones(f->nt16, 1u);
-    HIP_TRY(hipMemcpyAsync(f->tile_moved, ones.data(), ones.size() * 4, hipMemcpyHostToDevice, f->stream));
-    f->active_hint = f->nt16;
-    return wait_idle(f);
+    return reset_solver_state(f);
 }
 
 int cz_film_buffer_size(const cz_film* f, int which, uint64_t* bytes) {
@@ -803,6 +778,7 @@ int cz_film_read_buffer(cz_film* f, int which, void* host, uint64_t bytes) {
 }
 
 int cz_film_write_buffer(cz_film* f, int which, const void* host, uint64_t bytes) {
+    if (f && f->batch) return set_err(CZ_EINVAL, "write_buffer: the film is stepped by a batch (cz_batch_remove it first)");
     if (!f || !host) return set_err(CZ_EINVAL, "null argument");
     BufDesc b = buffer_of(f, which);
     if (!b.ptr) return set_err(CZ_EINVAL, "unknown buffer %d", which);
@@ -825,14 +801,13 @@ int cz_film_device_ptr(cz_film* f, int which, void** ptr, uint64_t* bytes) {
 
 // dry_now (src/film/mod.rs:1230-1244): one cure pass at 4 / frame_dt.
 int cz_film_dry_now(cz_film* f) {
+    if (f && f->batch) return set_err(CZ_EINVAL, "dry_now: the film is stepped by a batch (cz_batch_remove it first)");
     if (!f) return set_err(CZ_EINVAL, "null film");
     int rc = wait_idle(f);
     if (rc) return rc;
     cz_params saved = f->P;
     f->P.cure[0] = 4.0f / fmaxf(f->P.frame_dt, 1e-4f);
-    FilmDev F = f->dev();
-    const dim3 gcell((f->W + 15u) / 16u, (f->H + 15u) / 16u);
-    hipLaunchKernelGGL(k_cure_clear, gcell, dim3(256), 0, f->stream, F, 0);
+    hipLaunchKernelGGL(k_cure_clear<OneFilmAll>, dim3(f->nt16), dim3(256), 0, f->stream, OneFilmAll{f->dev(), f->nt16}, 0);
     HIP_TRY(hipGetLastError());
     f->P = saved;
     return wait_idle(f);
@@ -840,6 +815,7 @@ int cz_film_dry_now(cz_film* f) {
 
 // clear (src/film/mod.rs:1009-1028)
 int cz_film_clear(cz_film* f) {
+    if (f && f->batch) return set_err(CZ_EINVAL, "clear: the film is stepped by a batch (cz_batch_remove it first)");
     if (!f) return set_err(CZ_EINVAL, "null film");
     int rc = wait_idle(f);
     if (rc) return rc;
@@ -887,7 +863,7 @@ int cz_film_reset_counters(cz_film* f) {
     if (!f) return set_err(CZ_EINVAL, "null film");
     int rc = poll_stats(f);
     if (rc) return rc;
-    f->cnt_act_cells = f->cnt_act_cell_substeps = f->cnt_frames = 0;
+    f->cnt_act_cells = f->cnt_act_cell_substeps = f->cnt_frames = f->cnt_act_cell_implicit = 0;
     return CZ_OK;
 }
 

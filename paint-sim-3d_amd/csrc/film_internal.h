@@ -1,0 +1,179 @@
+// film_internal.h - the film handle and the host helpers shared by the
+// single-canvas entry points (film_host.hip) and the batched ones
+// (batch_host.hip).  Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "film_kernels.h"
+
+using namespace cz;
+
+// Message of the last error on this thread (cz_last_error); returns code.
+int cz_set_err(int code, const char* fmt, ...);
+#define set_err cz_set_err
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) return set_err(CZ_EHIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+struct GraphKeyStore {
+    unsigned char bytes[7 * 4 + sizeof(cz_params)];
+};
+
+namespace czi {
+
+constexpr uint32_t kMaxTileBlocks = 2048u;  // resident 256-thread blocks on 256 CUs
+
+struct ProfSlot {
+    std::string label;
+    hipEvent_t a, b;
+};
+
+}  // namespace czi
+
+struct cz_film {
+    uint32_t W = 0, H = 0, n = 0, tw = 0, th = 0, nt16 = 0, t64x = 0, t64y = 0, pw = 0, ph = 0;
+    float dx = 0.0f;
+    cz_settings s;
+    cz_overrides ov;
+    bool ov_from_env = true;
+    cz_params P;
+    float h_max = 0.0f, mu_min = 1.0f, h_max_pending = 0.0f;
+    uint32_t implicit_steps = 1, substeps_last = 1, implicit_last = 0, frame = 0, active_tiles = 0;
+    uint32_t active_hint = 0xffffffffu;  // grid sizing: last folded active count
+    double total_deposited = 0.0, weave_volume = 0.0;
+    float cg_report[4] = {0, 0, 0, 0};
+    bool stats_pending = false;
+    uint64_t cnt_act_cells = 0, cnt_act_cell_substeps = 0, cnt_frames = 0, cnt_act_cell_implicit = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    void* batch = nullptr;  // the cz_batch this film is a member of
+    uint64_t bytes = 0;     // device memory held
+    uint32_t cap_n = 0, cap_nt16 = 0, cap_nt64 = 0, cap_np = 0;  // sizes the buffers were made for
+
+    std::vector<void*> allocs;
+    // device buffers
+    float *h = nullptr, *m = nullptr;
+    float2* rheo = nullptr;
+    float4 *strata_h = nullptr, *strata_cure = nullptr, *strata_h2 = nullptr, *strata_cure2 = nullptr;
+    uint4 *strata_pig = nullptr, *strata_pig2 = nullptr;
+    uint32_t *sub_refl = nullptr, *lay = nullptr;
+    float2 *pres = nullptr, *fnorm = nullptr;
+    float4 *flux = nullptr, *acc = nullptr, *acc_s = nullptr, *alpha_s = nullptr, *cg_m = nullptr;
+    float *alpha = nullptr, *tmp = nullptr, *h_frame = nullptr;
+    uint32_t *delta_a = nullptr, *delta_b = nullptr;
+    bool delta_cur_a = true;
+    float *cg_d = nullptr, *cg_x = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr, *cg_t = nullptr,
+          *cg_t2 = nullptr;
+    double *cg_part = nullptr, *cg_part2 = nullptr;
+    float* cg_s = nullptr;
+    uint32_t *tile_flags = nullptr, *tile_list = nullptr, *tile_moved = nullptr, *active_count = nullptr;
+    uint32_t *cure_list = nullptr, *cure_count = nullptr;
+    float2* stats_out = nullptr;
+    float* pool_out = nullptr;
+    float *tsum_a = nullptr, *tsum_b = nullptr;
+    float* pigtab = nullptr;
+    cz_deposit* deposits = nullptr;
+    uint32_t *tile_start = nullptr, *tile_count = nullptr, *tile_items = nullptr;
+    float* frame_out = nullptr;
+    // host tables and pinned staging
+    float pigtab_h[CZ_NPIG * CZ_PIG_STRIDE];
+    float cmf_h[CZ_NBAND * 3];
+    float* frame_out_h = nullptr;
+    cz_deposit* dep_h = nullptr;
+    uint32_t *tstart_h = nullptr, *tcount_h = nullptr, *titems_h = nullptr;
+    uint32_t deposit_count = 0;
+    // profiling
+    bool use_graphs = false;
+    std::vector<std::pair<struct GraphKeyStore, hipGraphExec_t>> graphs;
+    bool prof_on = false;
+    std::string prof_only;
+    std::vector<czi::ProfSlot> prof_pending;
+    std::vector<hipEvent_t> ev_pool;
+    std::map<std::string, std::pair<double, uint64_t>> prof_acc;
+
+    uint32_t* delta_cur() const { return delta_cur_a ? delta_a : delta_b; }
+    uint32_t* delta_other() const { return delta_cur_a ? delta_b : delta_a; }
+
+    FilmDev dev() const {
+        FilmDev F;
+        memset(&F, 0, sizeof(F));
+        F.P = P;
+        F.W = W; F.H = H; F.n = n; F.tw = tw; F.th = th; F.nt16 = nt16; F.t64x = t64x; F.t64y = t64y;
+        F.pw = pw; F.ph = ph;
+        F.imp_steps = s.implicit_hydro ? implicit_steps : 0u;
+        F.h = h; F.m = m; F.rheo = rheo; F.strata_h = strata_h; F.strata_pig = strata_pig;
+        F.strata_cure = strata_cure; F.sub_refl = sub_refl; F.lay = lay;
+        F.pres = pres; F.flux = flux; F.acc = acc; F.alpha = alpha;
+        F.delta = delta_cur(); F.delta_out = delta_other();
+        F.tmp = tmp; F.acc_s = acc_s; F.alpha_s = alpha_s; F.strata_h2 = strata_h2; F.strata_pig2 = strata_pig2;
+        F.strata_cure2 = strata_cure2; F.h_frame = h_frame; F.fnorm = fnorm; F.cg_m = cg_m;
+        F.cg_d = cg_d; F.cg_x = cg_x; F.cg_r = cg_r; F.cg_p = cg_p; F.cg_q = cg_q; F.cg_t = cg_t; F.cg_t2 = cg_t2;
+        F.cg_part = cg_part; F.cg_part2 = cg_part2; F.cg_s = cg_s;
+        F.tile_flags = tile_flags; F.tile_list = tile_list; F.tile_moved = tile_moved; F.active_count = active_count;
+        F.cure_list = cure_list; F.cure_count = cure_count;
+        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.tsum_b = tsum_b; F.pigtab = pigtab;
+        F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
+        F.frame_out = frame_out;
+        return F;
+    }
+
+    // Grid of the tile kernels: the kernels walk the device-side list with a
+    // grid-stride loop, so any size is correct; size it from the last known
+    // active count (4 work items per tile for the face-parallel flux).
+    uint32_t tile_grid(uint32_t per_tile = 1) const {
+        uint64_t est = (uint64_t)std::max(std::min(active_hint, nt16), 1u) * per_tile;
+        est = est + est / 4u + 16u;
+        uint64_t p2 = 64;  // power of two: few distinct grids, few distinct graphs
+        while (p2 < est) p2 <<= 1;
+        est = p2;
+        uint64_t cap = std::min<uint64_t>((uint64_t)nt16 * per_tile, (uint64_t)czi::kMaxTileBlocks * per_tile);
+        return (uint32_t)std::max<uint64_t>(1u, std::min(est, cap));
+    }
+
+    hipEvent_t take_event() {
+        if (!ev_pool.empty()) {
+            hipEvent_t e = ev_pool.back();
+            ev_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+};
+
+
+namespace czi {
+// update_params (src/film/mod.rs:869-917) on the film's host state.
+void update_params(cz_film* f, float frame_dt, uint32_t substeps);
+// Fold one frame's scalars (k_frame_stats layout) into the film's host state
+// (poll_stats, src/film/mod.rs:935-973).
+void fold_frame_out(cz_film* f, const float* o);
+// Wait for the film's stream and fold its event timings.
+int wait_idle(cz_film* f);
+// The zeroing of write_state (src/film/mod.rs:1303-1320) after new state
+// parts: solver buffers cleared, every tile marked moved; waits for it.
+int reset_solver_state(cz_film* f);
+// Host-side effects of an encoded frame of `substeps` substeps.
+void frame_done(cz_film* f, uint32_t substeps, bool parity_done);
+// Film::new on the current device; init = false leaves the state parts zero
+// (for a caller that writes or seeds them next).
+int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigtab, const float* cmf, bool init,
+                cz_film** out);
+// Re-dimension a film within the capacity it was created with, as a new
+// Film with settings s (host state fresh; the caller writes the state).
+int film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s);
+// Take a film out of its batch (before it is destroyed).
+void batch_detach(cz_film* f);
+}  // namespace czi

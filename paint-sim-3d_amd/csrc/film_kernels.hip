@@ -15,13 +15,17 @@
 
 namespace cz {
 
-#define CZ_FOR_TILES(F)                                                              \
-    const uint32_t _nact = *(F).active_count;                                        \
+// Binds, per item of the launch: _a (item slot), F (the item's canvas) and
+// _t (its tile).
+#define CZ_FOR_TILES(WS)                                                             \
+    const uint32_t _nitems = (WS).count();                                           \
     const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;                  \
-    for (uint32_t _a = blockIdx.x; _a < _nact; _a += gridDim.x)
+    for (uint32_t _a = blockIdx.x; _a < _nitems; _a += gridDim.x)                    \
+        if (const uint32_t _item = (WS).item(_a); true)                              \
+            if (const FilmDev& F = (WS).film(_item); true)                           \
+                if (const uint32_t _t = (WS).tile(F, _item); true)
 
 #define CZ_TILE_CELL(F)                                                              \
-    const uint32_t _t = (F).tile_list[_a];                                           \
     const uint32_t x = (_t % (F).tw) * 16u + _lx, y = (_t / (F).tw) * 16u + _ly;     \
     const bool in_grid = x < (F).W && y < (F).H;                                     \
     const uint32_t c = in_grid ? idx((F), x, y) : 0u;
@@ -107,91 +111,108 @@ __global__ __launch_bounds__(256) void k_deposit(FilmDev F) {
     }
 }
 
-// film.wgsl:1365-1391 (one block per 16x16 tile)
-__global__ __launch_bounds__(256) void k_mark_tiles(FilmDev F) {
-    __shared__ uint32_t any;
-    if (threadIdx.x == 0) any = 0u;
-    __syncthreads();
-    uint32_t t = blockIdx.x;
-    uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
-    uint32_t i = (t % F.tw) * 16u + lx, j = (t / F.tw) * 16u + ly;
-    if (i < F.W && j < F.H) {
-        uint32_t c = idx(F, i, j);
-        float hc = F.h[c];
-        if (hc > 0.0f) {
-            uint32_t bits = 1u;
-            if (lx < WAKE_BAND) bits |= 2u;
-            if (lx >= 16u - WAKE_BAND) bits |= 4u;
-            if (ly < WAKE_BAND) bits |= 8u;
-            if (ly >= 16u - WAKE_BAND) bits |= 16u;
-            if (cz_abs(hc - F.h_frame[c]) > FRAME_EPS) bits |= MOVED;
-            atomicOr(&any, bits);
+// film.wgsl:1393-1425 as an ordered compaction (one block of 1024 threads
+// per canvas): the list is ascending in tile index instead of atomic append
+// order.  The same pass lists the tiles with cells for the cure pass (the
+// NEEDS_CURE bit of k_mark, which it strips from the flags).  The flag bytes of a band of tile rows and its one-row halo are
+// staged in LDS first, so every wake test reads the flags from before the
+// pass and the neighbour reads stay on chip.
+constexpr uint32_t COMPACT_LDS = 65536u;
+
+template <class CS>
+__global__ __launch_bounds__(1024) void k_compact_tiles(CS cs) {
+    const FilmDev& F = cs.film(blockIdx.x);
+    __shared__ uint8_t sf[COMPACT_LDS];
+    __shared__ uint32_t wsum[16], wsum2[16];
+    __shared__ uint32_t base, base2;
+    const uint32_t tw = F.tw, th = F.th;
+    const uint32_t band = COMPACT_LDS / tw - 2u;  // tile rows per band (tw <= 21845 at creation)
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) base = base2 = 0u;
+    for (uint32_t ty0 = 0; ty0 < th; ty0 += band) {
+        const uint32_t ty1 = ty0 + band < th ? ty0 + band : th;
+        const uint32_t ry0 = ty0 > 0u ? ty0 - 1u : 0u, ry1 = ty1 + 1u < th ? ty1 + 1u : th;
+        const uint32_t nst = (ry1 - ry0) * tw;
+        const uint32_t* src = F.tile_flags + ry0 * tw;
+        __syncthreads();
+        for (uint32_t q0 = 0; q0 < nst; q0 += 8u * 1024u) {
+            uint32_t v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint32_t q = q0 + (uint32_t)k * 1024u + threadIdx.x;
+                v[k] = q < nst ? src[q] : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint32_t q = q0 + (uint32_t)k * 1024u + threadIdx.x;
+                if (q < nst) sf[q] = (uint8_t)v[k];
+            }
+        }
+        __syncthreads();
+        for (uint32_t chunk = ty0 * tw; chunk < ty1 * tw; chunk += 1024u) {
+            const uint32_t t = chunk + threadIdx.x;
+            uint32_t active = 0u, fl_t = 0u;
+            if (t < ty1 * tw) {
+                const int tx = (int)(t % tw), ty = (int)(t / tw);
+                fl_t = sf[t - ry0 * tw];
+                uint32_t anyv = fl_t & MOVED;
+                for (int dy = -1; dy <= 1; dy++) {
+                    for (int dxi = -1; dxi <= 1; dxi++) {
+                        if (dxi == 0 && dy == 0) continue;
+                        const int xx = tx + dxi, yy = ty + dy;
+                        if (xx >= 0 && yy >= 0 && xx < (int)tw && yy < (int)th) {
+                            const uint32_t fl = sf[((uint32_t)yy - ry0) * tw + (uint32_t)xx];
+                            bool need = true;
+                            if (dxi == 1 && (fl & 2u) == 0u) need = false;
+                            if (dxi == -1 && (fl & 4u) == 0u) need = false;
+                            if (dy == 1 && (fl & 8u) == 0u) need = false;
+                            if (dy == -1 && (fl & 16u) == 0u) need = false;
+                            if (need) anyv |= fl & MOVED;
+                        }
+                    }
+                }
+                active = anyv != 0u ? 1u : 0u;
+            }
+            // block-wide exclusive scans of `active` and of the cure bit
+            const uint32_t cure = (fl_t & NEEDS_CURE) != 0u ? 1u : 0u;
+            const unsigned long long lmask = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+            const unsigned long long bal = __ballot(active), bal2 = __ballot(cure);
+            const uint32_t below = (uint32_t)__popcll(bal & lmask), below2 = (uint32_t)__popcll(bal2 & lmask);
+            if (lane == 0u) {
+                wsum[wave] = (uint32_t)__popcll(bal);
+                wsum2[wave] = (uint32_t)__popcll(bal2);
+            }
+            __syncthreads();
+            uint32_t woff = 0u, total = 0u, woff2 = 0u, total2 = 0u;
+            for (uint32_t w = 0; w < 16u; w++) {
+                if (w < wave) {
+                    woff += wsum[w];
+                    woff2 += wsum2[w];
+                }
+                total += wsum[w];
+                total2 += wsum2[w];
+            }
+            if (t < ty1 * tw) {
+                if (active) {
+                    F.tile_list[base + woff + below] = t;
+                    F.tile_flags[t] = (fl_t & 95u) | 32u;
+                } else {
+                    F.tile_flags[t] = fl_t & 31u;
+                }
+                if (cure) F.cure_list[base2 + woff2 + below2] = t;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                base += total;
+                base2 += total2;
+            }
         }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t flags = any;
-        if (F.tile_moved[t] != 0u) flags |= MOVED;
-        F.tile_moved[t] = 0u;
-        F.tile_flags[t] = flags;
+        *F.active_count = base;
+        *F.cure_count = base2;
     }
-}
-
-// film.wgsl:1393-1425 as an ordered compaction (one block of 1024 threads):
-// the list is ascending in tile index instead of atomic append order.
-__global__ __launch_bounds__(1024) void k_compact_tiles(FilmDev F) {
-    __shared__ uint32_t wsum[16];
-    __shared__ uint32_t base;
-    if (threadIdx.x == 0) base = 0u;
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    for (uint32_t chunk = 0; chunk < F.nt16; chunk += 1024u) {
-        uint32_t t = chunk + threadIdx.x;
-        uint32_t active = 0u, fl_t = 0u;
-        if (t < F.nt16) {
-            int tx = (int)(t % F.tw), ty = (int)(t / F.tw);
-            fl_t = F.tile_flags[t];
-            uint32_t anyv = fl_t & MOVED;
-            for (int dy = -1; dy <= 1; dy++) {
-                for (int dxi = -1; dxi <= 1; dxi++) {
-                    if (dxi == 0 && dy == 0) continue;
-                    int xx = tx + dxi, yy = ty + dy;
-                    if (xx >= 0 && yy >= 0 && xx < (int)F.tw && yy < (int)F.th) {
-                        uint32_t fl = F.tile_flags[(uint32_t)xx + (uint32_t)yy * F.tw];
-                        bool need = true;
-                        if (dxi == 1 && (fl & 2u) == 0u) need = false;
-                        if (dxi == -1 && (fl & 4u) == 0u) need = false;
-                        if (dy == 1 && (fl & 8u) == 0u) need = false;
-                        if (dy == -1 && (fl & 16u) == 0u) need = false;
-                        if (need) anyv |= fl & MOVED;
-                    }
-                }
-            }
-            active = anyv != 0u ? 1u : 0u;
-        }
-        // block-wide exclusive scan of `active`
-        unsigned long long bal = __ballot(active);
-        uint32_t below = (uint32_t)__popcll(bal & ((lane == 0u) ? 0ull : (~0ull >> (64u - lane))));
-        if (lane == 0u) wsum[wave] = (uint32_t)__popcll(bal);
-        __syncthreads();
-        uint32_t woff = 0u, total = 0u;
-        for (uint32_t w = 0; w < 16u; w++) {
-            if (w < wave) woff += wsum[w];
-            total += wsum[w];
-        }
-        if (t < F.nt16) {
-            if (active) {
-                F.tile_list[base + woff + below] = t;
-                F.tile_flags[t] = fl_t | 32u;
-            } else {
-                F.tile_flags[t] = fl_t & 31u;
-            }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) base += total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *F.active_count = base;
 }
 
 // ---------------------------------------------------------------------------
@@ -199,12 +220,14 @@ __global__ __launch_bounds__(1024) void k_compact_tiles(FilmDev F) {
 
 __device__ inline void composition_rheo(const FilmDev& F, const Stack& s, float& mu_o, float& ty_o) {
     float total = 0.0f, lnmu = 0.0f, tau = 0.0f, thin = 0.0f, wsum = 0.0f, cure = 0.0f;
+#pragma unroll
     for (uint32_t k = 0; k < 4u; k++) {
         float t = s.th[k];
         if (t <= 0.0f) continue;
         total = total + t;
         cure = cure + t * s.cu[k];
         float csum = 0.0f, lm = 0.0f, ty = 0.0f;
+#pragma unroll
         for (uint32_t i = 0; i < (uint32_t)NPIG; i++) {
             float cc = s.pg[k][i];
             if (cc > 0.0f) {
@@ -283,69 +306,165 @@ __device__ inline float2 packed_rheo(const FilmDev& F, uint32_t c, const float* 
     return make_float2(cz_expf(mu), ty);
 }
 
-// cure_step (cure.wgsl:119-159) over the full grid, fused with clear_acc
-// (film.wgsl:1427-1434) for the cells of active tiles when do_clear.
-__global__ __launch_bounds__(256) void k_cure_clear(FilmDev F, int do_clear) {
-    uint32_t i = blockIdx.x * 16u + (threadIdx.x & 15u);
-    uint32_t j = blockIdx.y * 16u + (threadIdx.x >> 4);
-    if (i >= F.W || j >= F.H) return;
-    uint32_t c = idx(F, i, j);
-    float hc = F.h[c];
-    if (hc <= 0.0f) {
-        F.rheo[c] = make_float2(1.0f, 0.0f);
+// cure_step (cure.wgsl:119-159) of one wet cell with height hc: cure
+// advance, rheology, and the commit of fully cured bottom strata into the
+// relief and the substrate reflectance.  Returns the height after commits.
+__device__ inline float cure_wet_cell(const FilmDev& F, uint32_t c, float hc) {
+    float rate = F.P.cure[0] * F.P.frame_dt;
+    float4 t4 = F.strata_h[c];
+    float4 cu4 = F.strata_cure[c];
+    float t[4] = {t4.x, t4.y, t4.z, t4.w};
+    float cu[4] = {cu4.x, cu4.y, cu4.z, cu4.w};
+    float u[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) u[k] = (t[k] > 0.0f) ? cz_min(cu[k] + rate, 1.0f) : cu[k];
+    if (!(t[0] > 0.0f && u[0] >= 1.0f)) {
+        F.strata_cure[c] = make_float4(u[0], u[1], u[2], u[3]);
+        F.rheo[c] = packed_rheo(F, c, t, u);
     } else {
-        float rate = F.P.cure[0] * F.P.frame_dt;
-        float4 t4 = F.strata_h[c];
-        float4 cu4 = F.strata_cure[c];
-        float t[4] = {t4.x, t4.y, t4.z, t4.w};
-        float cu[4] = {cu4.x, cu4.y, cu4.z, cu4.w};
-        float u[4];
-        for (int k = 0; k < 4; k++) u[k] = (t[k] > 0.0f) ? cz_min(cu[k] + rate, 1.0f) : cu[k];
-        if (!(t[0] > 0.0f && u[0] >= 1.0f)) {
-            F.strata_cure[c] = make_float4(u[0], u[1], u[2], u[3]);
-            F.rheo[c] = packed_rheo(F, c, t, u);
-        } else {
-            Stack s;
-            stack_load(F, c, s);
-            for (int k = 0; k < 4; k++) s.cu[k] = u[k];
-            float committed = 0.0f;
-            uint32_t guard = 0;
-            float mc = F.m[c];
-            while (s.th[0] > 0.0f && s.cu[0] >= 1.0f && guard < 4u) {
-                guard++;
-                float r[16];
-                for (int b = 0; b < 8; b++) {
-                    uint32_t v = F.sub_refl[c * 8u + b];
-                    r[2 * b] = cz_unpack_lo(v);
-                    r[2 * b + 1] = cz_unpack_hi(v);
-                }
-                for (uint32_t b = 0; b < (uint32_t)NBAND; b++) {
+        Stack s;
+        stack_load(F, c, s);
+#pragma unroll
+        for (int k = 0; k < 4; k++) s.cu[k] = u[k];
+        float committed = 0.0f;
+        uint32_t guard = 0;
+        float mc = F.m[c];
+        while (s.th[0] > 0.0f && s.cu[0] >= 1.0f && guard < 4u) {
+            guard++;
+            // KM composite of the bottom stratum over the substrate, one
+            // packed word (two bands) at a time
+#pragma unroll 1
+            for (uint32_t w = 0; w < 8u; w++) {
+                const uint32_t v = F.sub_refl[c * 8u + w];
+                float r2[2] = {cz_unpack_lo(v), cz_unpack_hi(v)};
+#pragma unroll
+                for (uint32_t h2 = 0; h2 < 2u; h2++) {
                     float kk, ss, R, T;
-                    layer_ks(F.pigtab, s.pg[0], b, kk, ss);
+                    layer_ks(F.pigtab, s.pg[0], 2u * w + h2, kk, ss);
                     km_layer(kk, ss, s.th[0], R, T);
-                    r[b] = composite(R, T, r[b]);
+                    r2[h2] = composite(R, T, r2[h2]);
                 }
-                for (int b = 0; b < 8; b++) F.sub_refl[c * 8u + b] = cz_pack2(r[2 * b], r[2 * b + 1]);
-                committed = committed + s.th[0];
-                mc = mc + s.th[0] * (1.0f - OIL_SHRINK) * (1.0f - cz_clamp(s.pg[0][12], 0.0f, 0.95f));
-                s.th[0] = 0.0f;
-                s.cu[0] = 0.0f;
-                stack_compact(F, s);
+                F.sub_refl[c * 8u + w] = cz_pack2(r2[0], r2[1]);
             }
-            F.m[c] = mc;
-            if (committed > 0.0f) {
-                hc = cz_max(hc - committed, 0.0f);
-                F.h[c] = hc;
-            }
-            float mu, ty;
-            composition_rheo(F, s, mu, ty);
-            F.rheo[c] = make_float2(mu, ty);
-            stack_store_to(F.strata_h, F.strata_cure, F.strata_pig, c, s);
+            committed = committed + s.th[0];
+            mc = mc + s.th[0] * (1.0f - OIL_SHRINK) * (1.0f - cz_clamp(s.pg[0][12], 0.0f, 0.95f));
+            s.th[0] = 0.0f;
+            s.cu[0] = 0.0f;
+            stack_compact(F, s);
+        }
+        F.m[c] = mc;
+        if (committed > 0.0f) {
+            hc = cz_max(hc - committed, 0.0f);
+            F.h[c] = hc;
+        }
+        float mu, ty;
+        composition_rheo(F, s, mu, ty);
+        F.rheo[c] = make_float2(mu, ty);
+        stack_store_to(F.strata_h, F.strata_cure, F.strata_pig, c, s);
+    }
+    return hc;
+}
+
+// cure_step over the full grid fused with clear_acc (film.wgsl:1427-1434)
+// for the cells of active tiles when do_clear (the dry_now pass).
+template <class WS>
+__global__ __launch_bounds__(256) void k_cure_clear(WS ws, int do_clear) {
+    CZ_FOR_TILES(ws) {
+        uint32_t i = (_t % F.tw) * 16u + _lx;
+        uint32_t j = (_t / F.tw) * 16u + _ly;
+        if (i >= F.W || j >= F.H) continue;
+        uint32_t c = idx(F, i, j);
+        float hc = F.h[c];
+        if (hc <= 0.0f) F.rheo[c] = make_float2(1.0f, 0.0f);
+        else hc = cure_wet_cell(F, c, hc);
+        if (do_clear && cell_active(F, i, j)) {
+            F.acc[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            F.h_frame[c] = hc;
         }
     }
-    if (do_clear && cell_active(F, i, j)) {
-        F.acc[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        F.h_frame[c] = hc;
+}
+
+// mark_tiles (film.wgsl:1365-1391), one wave per 16x16 tile: lane (r, q)
+// owns cells 4q..4q+3 of tile row r, read as one 16-byte load when rows are
+// 4-aligned.  The same pass writes the rheology of dry cells (the dry branch
+// of cure_step, cure.wgsl:122-125, which reads nothing but the height) and
+// marks with NEEDS_CURE the tiles holding cells for the wet branch; those
+// run in k_cure over the list the compaction builds.  Each cell's mark bits
+// use its height before its cure, as with the passes run in the reference's
+// order (both touch only the cell itself).
+template <class WS>
+__global__ __launch_bounds__(256) void k_mark(WS ws) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t r = lane >> 2, q = lane & 3u;
+    const uint32_t n = ws.count();
+    for (uint32_t a = blockIdx.x * 4u + (threadIdx.x >> 6); a < n; a += gridDim.x * 4u) {
+        const uint32_t it = ws.item(a);
+        const FilmDev& F = ws.film(it);
+        const uint32_t t = ws.tile(F, it);
+        const uint32_t x0 = (t % F.tw) * 16u + 4u * q, y = (t / F.tw) * 16u + r;
+        uint32_t bits = 0u;
+        if (y < F.H && x0 < F.W) {
+            const uint32_t c0 = idx(F, x0, y);
+            const bool vec = (F.W & 3u) == 0u;  // then x0 + 3 < W and c0 is 4-aligned
+            float hv[4];
+            if (vec) {
+                const float4 v = *(const float4*)(F.h + c0);
+                hv[0] = v.x; hv[1] = v.y; hv[2] = v.z; hv[3] = v.w;
+            } else {
+#pragma unroll
+                for (uint32_t k = 0; k < 4u; k++) hv[k] = (x0 + k < F.W) ? F.h[c0 + k] : 0.0f;
+            }
+            bool all_dry = true;
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; k++) {
+                if (x0 + k >= F.W) continue;
+                const float hc = hv[k];
+                if (hc > 0.0f) {
+                    const uint32_t lx = 4u * q + k;
+                    bits |= 1u;
+                    if (lx < WAKE_BAND) bits |= 2u;
+                    if (lx >= 16u - WAKE_BAND) bits |= 4u;
+                    if (r < WAKE_BAND) bits |= 8u;
+                    if (r >= 16u - WAKE_BAND) bits |= 16u;
+                    if (cz_abs(hc - F.h_frame[c0 + k]) > FRAME_EPS) bits |= MOVED;
+                }
+                if (!(hc <= 0.0f)) {
+                    all_dry = false;
+                    bits |= NEEDS_CURE;
+                }
+            }
+            if (vec && all_dry) {
+                float4* rp = (float4*)(F.rheo + c0);
+                rp[0] = make_float4(1.0f, 0.0f, 1.0f, 0.0f);
+                rp[1] = make_float4(1.0f, 0.0f, 1.0f, 0.0f);
+            } else {
+#pragma unroll
+                for (uint32_t k = 0; k < 4u; k++)
+                    if (x0 + k < F.W && hv[k] <= 0.0f) F.rheo[c0 + k] = make_float2(1.0f, 0.0f);
+            }
+        }
+        uint32_t flags = 0u;
+        const uint32_t bl[7] = {1u, 2u, 4u, 8u, 16u, MOVED, NEEDS_CURE};
+#pragma unroll
+        for (int k = 0; k < 7; k++)
+            if (__ballot((bits & bl[k]) != 0u)) flags |= bl[k];
+        if (lane == 0u) {
+            if (F.tile_moved[t] != 0u) flags |= MOVED;
+            F.tile_moved[t] = 0u;
+            F.tile_flags[t] = flags;
+        }
+    }
+}
+
+// The wet branch of cure_step (cure.wgsl:126-159) on the tiles k_mark
+// listed, one cell per thread.
+template <class WS>
+__global__ __launch_bounds__(256) void k_cure(WS ws) {
+    CZ_FOR_TILES(ws) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        const float hc = F.h[c];
+        if (!(hc <= 0.0f)) cure_wet_cell(F, c, hc);
     }
 }
 
@@ -401,16 +520,17 @@ __device__ inline void nose_cell(const FilmDev& F, uint32_t x, uint32_t y, uint3
 constexpr int PLUG_SPAN = 16 + 2 * PLUG_RADIUS;
 constexpr int RING_SPAN = 18;
 
-__global__ __launch_bounds__(256) void k_lap_plug(FilmDev F, int do_nose, int do_plug) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_lap_plug(WS ws, int do_nose, int do_plug, int do_clear) {
     __shared__ float s_q[PLUG_SPAN * PLUG_SPAN];
     __shared__ float s_h[PLUG_SPAN * PLUG_SPAN];
     __shared__ uint8_t s_y[PLUG_SPAN * PLUG_SPAN];
     __shared__ uint32_t s_d[RING_SPAN * RING_SPAN];
     __shared__ float2 s_fn[RING_SPAN * RING_SPAN];
-    const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
-    const bool plug_on = do_plug && !(gb < 1e-6f);
-    CZ_FOR_TILES(F) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
+        const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
+        const bool plug_on = do_plug && !(gb < 1e-6f);
         const int tx0 = (int)((_t % F.tw) * 16u), ty0 = (int)((_t / F.tw) * 16u);
         __syncthreads();
         for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
@@ -454,6 +574,13 @@ __global__ __launch_bounds__(256) void k_lap_plug(FilmDev F, int do_nose, int do
         }
         __syncthreads();
         if (!in_grid) continue;
+        // clear_acc (film.wgsl:1427-1434) after the cure pass, with the first
+        // active-tile kernel of the frame: do_clear 1 = clear, 2 = clear when
+        // the canvas has no implicit step this frame (its implicit lap cleared)
+        if (do_clear == 1 || (do_clear == 2 && F.imp_steps == 0u)) {
+            F.acc[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            F.h_frame[c] = F.h[c];
+        }
         const int own = (int)(_ly + 1u) * RING_SPAN + (int)(_lx + 1u);
         if (do_nose) {
             F.delta_out[c] = s_d[own];
@@ -524,12 +651,15 @@ __global__ __launch_bounds__(256) void k_lap_plug(FilmDev F, int do_nose, int do
 // independent, so each block takes one (tile, face) pair: 4x the waves of a
 // cell-per-thread launch to hide the dependent loads of the contact-line
 // terms.
-__global__ __launch_bounds__(256) void k_fluxes(FilmDev F) {
-    const uint32_t nact = *F.active_count;
+template <class WS>
+__global__ __launch_bounds__(256) void k_fluxes(WS ws) {
+    const uint32_t nact = ws.count();
     const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
-    float* flux = (float*)F.flux;
     for (uint32_t wk = blockIdx.x; wk < 4u * nact; wk += gridDim.x) {
-        const uint32_t t = F.tile_list[wk >> 2];
+        const uint32_t it = ws.item(wk >> 2);
+        const FilmDev& F = ws.film(it);
+        float* flux = (float*)F.flux;
+        const uint32_t t = ws.tile(F, it);
         const int k = (int)(wk & 3u);
         const uint32_t x = (t % F.tw) * 16u + lx, y = (t / F.tw) * 16u + ly;
         if (x >= F.W || y >= F.H) continue;
@@ -593,9 +723,10 @@ __device__ inline void inherit_from(const FilmDev& F, uint32_t c, uint32_t src, 
 // limiter factor of the cell and of its 8 neighbours, so the block evaluates
 // the limiter for the tile and a 1-cell ring (active cells) into LDS; the
 // factors never leave the chip.
-__global__ __launch_bounds__(256) void k_limit_apply(FilmDev F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_limit_apply(WS ws) {
     __shared__ float s_a[RING_SPAN * RING_SPAN];
-    CZ_FOR_TILES(F) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         const int tx0 = (int)((_t % F.tw) * 16u), ty0 = (int)((_t / F.tw) * 16u);
         __syncthreads();
@@ -666,11 +797,12 @@ __global__ __launch_bounds__(256) void k_limit_apply(FilmDev F) {
 
 // Copy the offsets of active cells from the buffer written last to the other
 // one, so that both hold the same values everywhere between frames.
-__global__ __launch_bounds__(256) void k_delta_sync(FilmDev F, uint32_t* dst) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_delta_sync(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
-        dst[c] = F.delta[c];
+        F.delta_out[c] = F.delta[c];
     }
 }
 
@@ -698,8 +830,9 @@ __device__ inline float4 band_split(const FilmDev& F, uint32_t cu, float hf, flo
     return make_float4(fr[0] * total, fr[1] * total, fr[2] * total, fr[3] * total);
 }
 
-__global__ __launch_bounds__(256) void k_strata_split(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_strata_split(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
         float2 p00 = pressure_at(F, (int)x, (int)y);
@@ -728,51 +861,80 @@ __global__ __launch_bounds__(256) void k_strata_split(FilmDev F) {
 // ---------------------------------------------------------------------------
 // reduce_tiles (film.wgsl:1330-1361) fused with pool8 (film.wgsl:1304-1324):
 // thread (lx, ly) of the block for 64-tile (tx, ty) owns the same 4x4 cells
-// in both, i.e. pool cell (16 tx + lx, 16 ty + ly).
-__global__ __launch_bounds__(256) void k_reduce_pool(FilmDev F) {
-    __shared__ float s_h[256];
-    __shared__ float s_mu[256];
-    uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
-    float hm = 0.0f, mm = 1e9f, mx = 0.0f, ms = 0.0f;
-    for (uint32_t dy = 0; dy < 4u; dy++) {
-        for (uint32_t dxi = 0; dxi < 4u; dxi++) {
-            uint32_t i = blockIdx.x * TILE64 + lx * 4u + dxi;
-            uint32_t j = blockIdx.y * TILE64 + ly * 4u + dy;
-            if (i < F.W && j < F.H) {
-                uint32_t c = idx(F, i, j);
-                float v = F.h[c];
-                float mv = F.m[c];
-                if (v > 4.0f * F.P.ein[3]) {
-                    hm = cz_max(hm, v);
-                    mm = cz_min(mm, F.rheo[c].x);
+// in both, i.e. pool cell (16 tx + lx, 16 ty + ly); each of its four rows is
+// one 16-byte load of h and of m when rows are 4-aligned.
+__device__ __forceinline__ float wave_max(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = cz_max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = cz_min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+template <class WS>
+__global__ __launch_bounds__(256) void k_reduce_pool(WS ws) {
+    __shared__ float s_h[4];
+    __shared__ float s_mu[4];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    CZ_FOR_TILES(ws) {
+        const uint32_t lx = _lx, ly = _ly;
+        const uint32_t bx = _t % F.P.tiles[0], by = _t / F.P.tiles[0];
+        const uint32_t i0 = bx * TILE64 + lx * 4u;
+        const bool vec = (F.W & 3u) == 0u;
+        const float hmin4 = 4.0f * F.P.ein[3];
+        float hm = 0.0f, mm = 1e9f, mx = 0.0f, ms = 0.0f;
+        for (uint32_t dy = 0; dy < 4u; dy++) {
+            const uint32_t j = by * TILE64 + ly * 4u + dy;
+            if (j >= F.H || i0 >= F.W) continue;
+            const uint32_t c0 = idx(F, i0, j);
+            float hv[4], mv[4];
+            if (vec) {
+                const float4 a4 = *(const float4*)(F.h + c0);
+                const float4 m4 = *(const float4*)(F.m + c0);
+                hv[0] = a4.x; hv[1] = a4.y; hv[2] = a4.z; hv[3] = a4.w;
+                mv[0] = m4.x; mv[1] = m4.y; mv[2] = m4.z; mv[3] = m4.w;
+            } else {
+                for (uint32_t k = 0; k < 4u; k++) {
+                    hv[k] = (i0 + k < F.W) ? F.h[c0 + k] : 0.0f;
+                    mv[k] = (i0 + k < F.W) ? F.m[c0 + k] : 0.0f;
                 }
-                mx = cz_max(mx, v + mv);
-                ms = cz_max(ms, mv);
+            }
+            for (uint32_t k = 0; k < 4u; k++) {
+                if (i0 + k >= F.W) continue;
+                if (hv[k] > hmin4) {
+                    hm = cz_max(hm, hv[k]);
+                    mm = cz_min(mm, F.rheo[c0 + k].x);
+                }
+                mx = cz_max(mx, hv[k] + mv[k]);
+                ms = cz_max(ms, mv[k]);
             }
         }
-    }
-    uint32_t pi = blockIdx.x * 16u + lx, pj = blockIdx.y * 16u + ly;
-    if (pi < F.pw && pj < F.ph) {
-        F.pool_out[pj * F.pw + pi] = mx;
-        F.pool_out[F.pw * F.ph + pj * F.pw + pi] = ms;
-    }
-    s_h[threadIdx.x] = hm;
-    s_mu[threadIdx.x] = mm;
-    __syncthreads();
-    for (uint32_t s = 128u; s > 0u; s >>= 1) {
-        if (threadIdx.x < s) {
-            s_h[threadIdx.x] = cz_max(s_h[threadIdx.x], s_h[threadIdx.x + s]);
-            s_mu[threadIdx.x] = cz_min(s_mu[threadIdx.x], s_mu[threadIdx.x + s]);
+        const uint32_t pi = bx * 16u + lx, pj = by * 16u + ly;
+        if (pi < F.pw && pj < F.ph) {
+            F.pool_out[pj * F.pw + pi] = mx;
+            F.pool_out[F.pw * F.ph + pj * F.pw + pi] = ms;
+        }
+        hm = wave_max(hm);
+        mm = wave_min(mm);
+        __syncthreads();
+        if (lane == 0u) {
+            s_h[wave] = hm;
+            s_mu[wave] = mm;
         }
         __syncthreads();
+        if (threadIdx.x == 0)
+            F.stats_out[_t] = make_float2(cz_max(cz_max(s_h[0], s_h[1]), cz_max(s_h[2], s_h[3])),
+                                          cz_min(cz_min(s_mu[0], s_mu[1]), cz_min(s_mu[2], s_mu[3])));
     }
-    if (threadIdx.x == 0) F.stats_out[blockIdx.x + blockIdx.y * F.P.tiles[0]] = make_float2(s_h[0], s_mu[0]);
 }
 
 // Fold of the per-64-tile stats (poll_stats, src/film/mod.rs:944-951) and the
 // frame scalars the host reads back: [0] h_max, [1] mu_min, [2] active count
 // (bits), [3] cells of the active tiles (bits), [4..7] cg_s[1].
-__global__ __launch_bounds__(256) void k_frame_stats(FilmDev F) {
+template <class CS>
+__global__ __launch_bounds__(256) void k_frame_stats(CS cs) {
+    const FilmDev& F = cs.film(blockIdx.x);
     __shared__ float s_h[256];
     __shared__ float s_mu[256];
     __shared__ uint32_t s_cells;
@@ -816,8 +978,9 @@ __global__ __launch_bounds__(256) void k_frame_stats(FilmDev F) {
 // ---------------------------------------------------------------------------
 // strata advection (strata.wgsl:157-323)
 
-__global__ __launch_bounds__(256) void k_strata_limit(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_strata_limit(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
         float4 s0 = F.acc_s[4u * c], s1 = F.acc_s[4u * c + 1u], s2 = F.acc_s[4u * c + 2u], s3 = F.acc_s[4u * c + 3u];
@@ -851,21 +1014,28 @@ __global__ __launch_bounds__(256) void k_strata_limit(FilmDev F) {
     }
 }
 
-__device__ inline void inflow_from(const FilmDev& F, Stack& s, uint32_t cn, const float* f) {
-    float4 an4 = F.alpha_s[cn];
-    float4 thn4 = F.strata_h[cn];
-    float an[4] = {an4.x, an4.y, an4.z, an4.w};
-    float thn[4] = {thn4.x, thn4.y, thn4.z, thn4.w};
-    float4 cu4 = F.strata_cure[cn];
-    float cun[4] = {cu4.x, cu4.y, cu4.z, cu4.w};
+__device__ __forceinline__ float sel4(float4 v, uint32_t k) {
+    return k == 0u ? v.x : (k == 1u ? v.y : (k == 2u ? v.z : v.w));
+}
+
+// inflow_from (strata.wgsl:167-188): the strata of neighbour cn moving into
+// the cell with face fluxes f (one per stratum).  The stratum loop stays a
+// loop (one copy of stack_add_band) and reads its per-stratum values by
+// select, so nothing is indexed at run time.
+__device__ inline void inflow_from(const FilmDev& F, Stack& s, uint32_t cn, float4 f) {
+    const float4 an4 = F.alpha_s[cn];
+    const float4 thn4 = F.strata_h[cn];
+    const float4 cu4 = F.strata_cure[cn];
     float z0 = 0.0f;
+#pragma unroll 1
     for (uint32_t k = 0; k < 4u; k++) {
-        float z1 = z0 + thn[k];
-        float v = f[k] * an[k] * F.P.inv_dx;
-        if (v > 0.0f && thn[k] > 0.0f) {
+        const float thk = sel4(thn4, k);
+        float z1 = z0 + thk;
+        float v = sel4(f, k) * sel4(an4, k) * F.P.inv_dx;
+        if (v > 0.0f && thk > 0.0f) {
             float pig[16];
             unpack8(F.strata_pig[(cn * 4u + k) * 2u], F.strata_pig[(cn * 4u + k) * 2u + 1u], pig);
-            stack_add_band(F, s, z0, z1, v, pig, cun[k]);
+            stack_add_band(F, s, z0, z1, v, pig, sel4(cu4, k));
         }
         z0 = z1;
     }
@@ -876,8 +1046,9 @@ __device__ __forceinline__ void to4(const float4 v, float* o) { o[0] = v.x; o[1]
 
 // strata_update (strata.wgsl:190-274); also records the new stack sum of each
 // active cell for the fill passes.
-__global__ __launch_bounds__(256) void k_strata_update(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_strata_update(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
         float4 fr = F.acc_s[4u * c], fu = F.acc_s[4u * c + 1u], fdg = F.acc_s[4u * c + 2u], feg = F.acc_s[4u * c + 3u];
@@ -898,8 +1069,10 @@ __global__ __launch_bounds__(256) void k_strata_update(FilmDev F) {
                 cu = z4;
             } else {
                 float sum = th[0] + th[1] + th[2] + th[3];
+                #pragma unroll
                 for (int k = 0; k < 4; k++) nt[k] = th[k];
                 if (sum > 0.0f)
+                    #pragma unroll
                     for (int k = 0; k < 4; k++) nt[k] = th[k] * (hc / sum);
             }
             bool plain = true;
@@ -921,26 +1094,40 @@ __global__ __launch_bounds__(256) void k_strata_update(FilmDev F) {
         float a[4] = {a4.x, a4.y, a4.z, a4.w};
         float vr[4], vu[4], vdg[4], veg[4], vl[4], vd[4], vll[4], vlu[4];
         to4(fr, vr); to4(fu, vu); to4(fdg, vdg); to4(feg, veg); to4(fl, vl); to4(fd, vd); to4(fll, vll); to4(flu, vlu);
+        #pragma unroll
         for (int k = 0; k < 4; k++) {
             float out = cz_max(vr[k], 0.0f) + cz_max(vu[k], 0.0f) + cz_max(vdg[k], 0.0f) + cz_max(veg[k], 0.0f) +
                         cz_max(-vl[k], 0.0f) + cz_max(-vd[k], 0.0f) + cz_max(-vll[k], 0.0f) + cz_max(-vlu[k], 0.0f);
             s.th[k] = cz_max(s.th[k] - out * a[k] * F.P.inv_dx, 0.0f);
         }
-        float nf[4];
-        if (x + 1u < F.W) { for (int k = 0; k < 4; k++) nf[k] = -vr[k]; inflow_from(F, s, c + 1u, nf); }
-        if (y + 1u < F.H) { for (int k = 0; k < 4; k++) nf[k] = -vu[k]; inflow_from(F, s, c + F.W, nf); }
-        if (x + 1u < F.W && y + 1u < F.H) { for (int k = 0; k < 4; k++) nf[k] = -vdg[k]; inflow_from(F, s, c + F.W + 1u, nf); }
-        if (x + 1u < F.W && y > 0u) { for (int k = 0; k < 4; k++) nf[k] = -veg[k]; inflow_from(F, s, c - F.W + 1u, nf); }
-        if (cl != c) inflow_from(F, s, cl, vl);
-        if (cd != c) inflow_from(F, s, cd, vd);
-        if (cll != c) inflow_from(F, s, cll, vll);
-        if (clu != c) inflow_from(F, s, clu, vlu);
+        // the eight inflows in the reference's order, one call site
+#pragma unroll 1
+        for (uint32_t q = 0; q < 8u; q++) {
+            bool ok = false;
+            uint32_t cn = c;
+            float4 nf = z4;
+            switch (q) {
+            case 0: ok = x + 1u < F.W; cn = c + 1u; nf = make_float4(-fr.x, -fr.y, -fr.z, -fr.w); break;
+            case 1: ok = y + 1u < F.H; cn = c + F.W; nf = make_float4(-fu.x, -fu.y, -fu.z, -fu.w); break;
+            case 2: ok = x + 1u < F.W && y + 1u < F.H; cn = c + F.W + 1u;
+                    nf = make_float4(-fdg.x, -fdg.y, -fdg.z, -fdg.w); break;
+            case 3: ok = x + 1u < F.W && y > 0u; cn = c - F.W + 1u;
+                    nf = make_float4(-feg.x, -feg.y, -feg.z, -feg.w); break;
+            case 4: ok = cl != c; cn = cl; nf = fl; break;
+            case 5: ok = cd != c; cn = cd; nf = fd; break;
+            case 6: ok = cll != c; cn = cll; nf = fll; break;
+            default: ok = clu != c; cn = clu; nf = flu; break;
+            }
+            if (ok) inflow_from(F, s, cn, nf);
+        }
         float hc = F.h[c];
         float sum = s.th[0] + s.th[1] + s.th[2] + s.th[3];
         if (hc <= 0.0f) {
+            #pragma unroll
             for (int k = 0; k < 4; k++) { s.th[k] = 0.0f; s.cu[k] = 0.0f; }
         } else if (sum > 0.0f) {
             float k = hc / sum;
+            #pragma unroll
             for (int q = 0; q < 4; q++) s.th[q] = s.th[q] * k;
         }
         stack_compact(F, s);
@@ -951,8 +1138,9 @@ __global__ __launch_bounds__(256) void k_strata_update(FilmDev F) {
 
 // strata_copy_back (strata.wgsl:276-286): the pigment words of a tile row are
 // one 2 KiB run, copied by consecutive threads.
-__global__ __launch_bounds__(256) void k_strata_copy_back(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_strata_copy_back(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (in_grid) {
             F.strata_h[c] = F.strata_h2[c];
@@ -980,9 +1168,12 @@ __device__ __forceinline__ float tsum_of(const FilmDev& F, const float* tin, uin
     return t.x + t.y + t.z + t.w;
 }
 
-__global__ __launch_bounds__(256) void k_strata_fill(FilmDev F, const float* tin, float* tout) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_strata_fill(WS ws, int pass) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
+        const float* tin = (pass & 1) ? F.tsum_b : F.tsum_a;
+        float* tout = (pass & 1) ? F.tsum_a : F.tsum_b;
         if (!in_grid) continue;
         float own = tin[c];
         tout[c] = own;
@@ -1044,8 +1235,9 @@ __device__ __forceinline__ float hold_u(const FilmDev& F, uint32_t c) {
     return F.m[c] - hold;
 }
 
-__global__ __launch_bounds__(256) void k_cg_setup(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_cg_setup(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
         float2 p00 = pressure_at(F, (int)x, (int)y);
@@ -1143,8 +1335,9 @@ __device__ inline double block_sum_d(double v) {
 
 // which: 1 = cg_lap_h (t <- L h, t2 <- L hold_u), 0 = cg_lap_p (t <- L p),
 // 4 = cg_lap_u (t <- L (x + hold_u))   (film.wgsl:1060-1083)
-__global__ __launch_bounds__(256) void k_cg_lap(FilmDev F, uint32_t which) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_cg_lap(WS ws, uint32_t which) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
         if (which == 1u) {
@@ -1159,8 +1352,9 @@ __global__ __launch_bounds__(256) void k_cg_lap(FilmDev F, uint32_t which) {
 }
 
 // cg_init (film.wgsl:1110-1144)
-__global__ __launch_bounds__(256) void k_cg_init(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_cg_init(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         double v = 0.0;
         if (in_grid) {
@@ -1197,8 +1391,9 @@ __global__ __launch_bounds__(256) void k_cg_init(FilmDev F) {
 }
 
 // cg_iter_a (film.wgsl:1146-1157): q = A p, partial p.q
-__global__ __launch_bounds__(256) void k_cg_iter_a(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_cg_iter_a(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         double v = 0.0;
         if (in_grid) {
@@ -1221,8 +1416,9 @@ __global__ __launch_bounds__(256) void k_cg_iter_a(FilmDev F) {
 }
 
 // cg_iter_b (film.wgsl:1159-1172)
-__global__ __launch_bounds__(256) void k_cg_iter_b(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_cg_iter_b(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         double v = 0.0;
         if (in_grid) {
@@ -1238,8 +1434,9 @@ __global__ __launch_bounds__(256) void k_cg_iter_b(FilmDev F) {
 }
 
 // cg_iter_c (film.wgsl:1174-1181)
-__global__ __launch_bounds__(256) void k_cg_iter_c(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_cg_iter_c(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
         float beta = F.cg_s[3];
@@ -1249,12 +1446,16 @@ __global__ __launch_bounds__(256) void k_cg_iter_c(FilmDev F) {
 
 // Single-block FP64 fold of the per-tile partials (film.wgsl:1183-1255):
 // mode 0 = cg_reduce_init, 1 = cg_reduce_a, 2 = cg_reduce_b, 3 = cg_reduce_mass.
-__global__ __launch_bounds__(256) void k_cg_reduce(FilmDev F, int mode) {
+template <class CS>
+__global__ __launch_bounds__(256) void k_cg_reduce(CS cs, int mode) {
+    const FilmDev& F = cs.film(blockIdx.x);
+    const double* part = F.cg_part + cs.base(blockIdx.x);
+    const double* part2 = F.cg_part2 + cs.base(blockIdx.x);
     uint32_t n = *F.active_count;
     double a = 0.0, b = 0.0;
     for (uint32_t t = threadIdx.x; t < n; t += 256u) {
-        a += F.cg_part[t];
-        if (mode == 3) b += F.cg_part2[t];
+        a += part[t];
+        if (mode == 3) b += part2[t];
     }
     double sa = block_sum_d(a);
     double sb = (mode == 3) ? block_sum_d(b) : 0.0;
@@ -1282,8 +1483,9 @@ __global__ __launch_bounds__(256) void k_cg_reduce(FilmDev F, int mode) {
 
 // cg_finish (film.wgsl:1261-1302); the new height goes to cg_q so that
 // neighbours read the height from before the pass; k_cg_rescale commits it.
-__global__ __launch_bounds__(256) void k_cg_finish(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_cg_finish(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         bool moved = false;
         double dv = 0.0, tv = 0.0;
@@ -1325,8 +1527,9 @@ __global__ __launch_bounds__(256) void k_cg_finish(FilmDev F) {
 }
 
 // cg_rescale (film.wgsl:1224-1233) applied to the height left by k_cg_finish.
-__global__ __launch_bounds__(256) void k_cg_rescale(FilmDev F) {
-    CZ_FOR_TILES(F) {
+template <class WS>
+__global__ __launch_bounds__(256) void k_cg_rescale(WS ws) {
+    CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
         float hn = F.cg_q[c];
@@ -1339,5 +1542,114 @@ __global__ __launch_bounds__(256) void k_cg_rescale(FilmDev F) {
         F.h[c] = cz_max(hn * fct, 0.0f);
     }
 }
+
+// ---------------------------------------------------------------------------
+// The item lists of one batched frame (one block per position k of the
+// substep order): member order[k] appends its active tiles at the sum of the
+// active counts of the members before it, so the items of the members still
+// stepping at substep s are a prefix of the list; the same for the implicit
+// order.  Block 0 writes the prefix lengths.
+constexpr uint32_t CZ_PLAN_MAX = 1024u;
+
+__global__ __launch_bounds__(256) void k_batch_plan(BatchPlan B) {
+    __shared__ uint32_t cnt[CZ_PLAN_MAX];
+    __shared__ uint32_t icnt[CZ_PLAN_MAX];
+    __shared__ uint32_t ccnt[CZ_PLAN_MAX];
+    __shared__ uint32_t red[3][256];
+    const uint32_t k = blockIdx.x;
+    for (uint32_t j = threadIdx.x; j < B.members; j += 256u) {
+        const FilmDev& G = desc_at(B.films, B.order[j] * CZ_VARIANTS);
+        cnt[j] = *G.active_count;
+        ccnt[j] = *G.cure_count;
+        icnt[j] = (B.imp[j] > 0u) ? *desc_at(B.films, B.iorder[j] * CZ_VARIANTS).active_count : 0u;
+    }
+    __syncthreads();
+    uint32_t so = 0u, io = 0u, co = 0u;
+    for (uint32_t j = threadIdx.x; j < k; j += 256u) {
+        so += cnt[j];
+        io += icnt[j];
+        co += ccnt[j];
+    }
+    red[0][threadIdx.x] = so;
+    red[1][threadIdx.x] = io;
+    red[2][threadIdx.x] = co;
+    __syncthreads();
+    for (uint32_t w = 128u; w > 0u; w >>= 1) {
+        if (threadIdx.x < w) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + w];
+            red[1][threadIdx.x] += red[1][threadIdx.x + w];
+            red[2][threadIdx.x] += red[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    const uint32_t off_s = red[0][0], off_i = red[1][0], off_c = red[2][0];
+    {
+        const uint32_t m = B.order[k];
+        const FilmDev& F = desc_at(B.films, m * CZ_VARIANTS);
+        for (uint32_t a = threadIdx.x; a < cnt[k]; a += 256u)
+            B.items_sub[off_s + a] = (m << CZ_ITEM_TILE_BITS) | F.tile_list[a];
+        for (uint32_t a = threadIdx.x; a < ccnt[k]; a += 256u)
+            B.items_cure[off_c + a] = (m << CZ_ITEM_TILE_BITS) | F.cure_list[a];
+        if (k + 1u == B.members && threadIdx.x == 0) *B.n_cure = off_c + ccnt[k];
+    }
+    if (B.imp[k] > 0u) {
+        const uint32_t m = B.iorder[k];
+        const FilmDev& F = desc_at(B.films, m * CZ_VARIANTS);
+        for (uint32_t a = threadIdx.x; a < icnt[k]; a += 256u)
+            B.items_imp[off_i + a] = (m << CZ_ITEM_TILE_BITS) | F.tile_list[a];
+        if (threadIdx.x == 0) {
+            B.imp_members[k] = m;
+            B.imp_bases[k] = off_i;
+        }
+    }
+    if (k != 0u) return;
+    for (uint32_t s = threadIdx.x; s <= B.max_sub; s += 256u) {
+        uint32_t n = 0u;
+        for (uint32_t j = 0; j < B.members; j++)
+            if (s == B.max_sub || B.sub[j] > s) n += cnt[j];
+        B.lim_sub[s] = n;
+    }
+    for (uint32_t s = threadIdx.x; s < B.max_imp; s += 256u) {
+        uint32_t n = 0u;
+        for (uint32_t j = 0; j < B.members; j++)
+            if (B.imp[j] > s) n += icnt[j];
+        B.lim_imp[s] = n;
+    }
+}
+
+// Instantiations: full-grid kernels over one canvas's tiles or a batch's
+// static all-tile lists, tile kernels over one canvas's active list or a
+// batch's union lists.
+#define CZ_INST1(name, WSX, ...) template __global__ void name<WSX>(WSX, ##__VA_ARGS__);
+CZ_INST1(k_cure_clear, OneFilmAll, int)
+CZ_INST1(k_cure_clear, ManyFilms, int)
+CZ_INST1(k_reduce_pool, OneFilmAll)
+CZ_INST1(k_reduce_pool, ManyFilms)
+#define CZ_INST2(name, ...) CZ_INST1(name, OneFilm, ##__VA_ARGS__) CZ_INST1(name, ManyFilms, ##__VA_ARGS__)
+CZ_INST2(k_lap_plug, int, int, int)
+CZ_INST1(k_mark, OneFilmAll)
+CZ_INST1(k_mark, ManyFilms)
+CZ_INST1(k_cure, OneFilmCure)
+CZ_INST1(k_cure, ManyFilms)
+CZ_INST2(k_fluxes)
+CZ_INST2(k_limit_apply)
+CZ_INST2(k_delta_sync)
+CZ_INST2(k_strata_split)
+CZ_INST2(k_strata_limit)
+CZ_INST2(k_strata_update)
+CZ_INST2(k_strata_copy_back)
+CZ_INST2(k_strata_fill, int)
+CZ_INST2(k_cg_setup)
+CZ_INST2(k_cg_lap, uint32_t)
+CZ_INST2(k_cg_init)
+CZ_INST2(k_cg_iter_a)
+CZ_INST2(k_cg_iter_b)
+CZ_INST2(k_cg_iter_c)
+CZ_INST2(k_cg_finish)
+CZ_INST2(k_cg_rescale)
+#define CZ_INST3(name, ...) CZ_INST1(name, OneCanvas, ##__VA_ARGS__) CZ_INST1(name, ManyCanvases, ##__VA_ARGS__)
+CZ_INST3(k_compact_tiles)
+CZ_INST3(k_frame_stats)
+CZ_INST3(k_cg_reduce, int)
 
 }  // namespace cz

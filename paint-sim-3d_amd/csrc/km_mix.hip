@@ -28,6 +28,7 @@
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 namespace {
 
@@ -48,14 +49,14 @@ __device__ __forceinline__ u16x8 load_a(const uint16_t* __restrict__ conc, uint6
     if (cell < cells) {
         const uint16_t* cp = conc + cell * KM_NPIG;
         if ((grp & 1u) == 0u) {
-            const uint2 v0 = __builtin_nontemporal_load((const uint2*)(cp));
-            const uint2 v1 = __builtin_nontemporal_load((const uint2*)(cp + 4));
+            const u32x2 v0 = __builtin_nontemporal_load((const u32x2*)(cp));
+            const u32x2 v1 = __builtin_nontemporal_load((const u32x2*)(cp + 4));
             av[0] = (unsigned short)(v0.x & 0xffffu); av[1] = (unsigned short)(v0.x >> 16);
             av[2] = (unsigned short)(v0.y & 0xffffu); av[3] = (unsigned short)(v0.y >> 16);
             av[4] = (unsigned short)(v1.x & 0xffffu); av[5] = (unsigned short)(v1.x >> 16);
             av[6] = (unsigned short)(v1.y & 0xffffu); av[7] = (unsigned short)(v1.y >> 16);
         } else {
-            const uint2 v2 = __builtin_nontemporal_load((const uint2*)(cp + 8));
+            const u32x2 v2 = __builtin_nontemporal_load((const u32x2*)(cp + 8));
             av[0] = (unsigned short)(v2.x & 0xffffu); av[1] = (unsigned short)(v2.x >> 16);
             av[2] = (unsigned short)(v2.y & 0xffffu); av[3] = (unsigned short)(v2.y >> 16);
         }

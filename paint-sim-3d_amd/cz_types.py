@@ -41,7 +41,34 @@ class Stats(C.Structure):
     _fields_ = [("h_max", f32), ("mu_min", f32), ("active_tiles", u32), ("substeps_last", u32),
                 ("implicit_steps", u32), ("frame", u32), ("cg_report", f32 * 4), ("total_deposited", C.c_double),
                 ("weave_volume", C.c_double), ("act_cells", C.c_uint64), ("act_cell_substeps", C.c_uint64),
-                ("frames_counted", C.c_uint64)]
+                ("frames_counted", C.c_uint64), ("act_cell_implicit", C.c_uint64)]
+
+
+class Scene(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("nblobs", u32), ("sigma_lo", f32), ("sigma_hi", f32), ("peak_lo", f32),
+                ("peak_hi", f32), ("tau_lo", f32), ("tau_hi", f32), ("mu_lo", f32), ("mu_hi", f32),
+                ("alpha", f32), ("weave_um", f32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("w", u32), ("h", u32), ("steps", u32), ("pad", u32), ("arrival_s", C.c_double),
+                ("settings", Settings), ("scene", Scene)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("arrival_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double),
+                ("cell_steps", C.c_uint64), ("status", C.c_int32), ("pad", u32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_members", u32), ("frame_dt", f32), ("keep_films", C.c_int32),
+                ("pad", u32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("frames", C.c_uint64), ("admitted", C.c_uint64),
+                ("cell_steps", C.c_uint64), ("peak_members", C.c_uint64), ("peak_bytes", C.c_uint64),
+                ("films_created", C.c_uint64), ("films_reused", C.c_uint64)]
 
 
 class HostState(C.Structure):

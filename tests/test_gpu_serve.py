@@ -1,0 +1,96 @@
+"""Serving loop: canvases admitted as they arrive, stepped together with the
+canvases already running, retired when their frames are done.  Every
+request's final state equals the same canvas stepped alone (bit for bit;
+the stated bar is 1e-5 relative), admission respects the memory budget and
+the batch limit, and recycled films behave as fresh ones."""
+import numpy as np
+import pytest
+
+import canvaz_film as czf
+import scenes
+
+pytestmark = pytest.mark.gpu
+
+PARTS = ("h", "strata_h", "strata_pig", "strata_cure", "rheo", "m", "sub_refl", "lay", "delta", "tile_flags")
+
+
+def trace(n=10, seed=3):
+    rng = np.random.default_rng(seed)
+    reqs, specs = [], []
+    t = 0.0
+    for k in range(n):
+        w = int(rng.integers(40, 150))
+        h = int(rng.integers(40, 150))
+        steps = int(rng.integers(2, 9))
+        tilt = float(rng.uniform(0.0, 50.0))
+        s = scenes.blobs_settings(w, tilt_deg=tilt)
+        s.height_m = h * s.width_m / w
+        sc = czf.scene(seed=100 + k, nblobs=12, sigma_lo=2.0, sigma_hi=10.0)
+        reqs.append(czf.request(k, w, h, steps, t, s, sc))
+        specs.append((w, h, steps, s, sc))
+        t += float(rng.exponential(0.004))
+    return reqs, specs
+
+
+def alone(w, h, steps, s, sc):
+    f = czf.Film(w, h, s)
+    f.seed_blobs(sc)
+    for _ in range(steps):
+        f.step()
+    f.sync()
+    return f
+
+
+def assert_same(a, b, tag):
+    for name in PARTS:
+        x, y = a.buffer(name), b.buffer(name)
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), f"{tag}: {name} differs"
+
+
+def test_each_result_equals_alone(gpu):
+    reqs, specs = trace()
+    srv = czf.Server(hbm_budget=8 << 30, max_members=4, keep_films=True)
+    res, st = srv.run(reqs)
+    assert st.admitted == len(reqs)
+    assert 1 < st.peak_members <= 4
+    for r, (w, h, steps, s, sc) in zip(res, specs):
+        assert r.status == 0
+        assert r.arrival_s <= r.admit_s <= r.done_s
+        assert r.cell_steps == w * h * steps
+        g = srv.film(r.id, w, h)
+        assert g is not None and g.stats().frame == steps
+        assert_same(g, alone(w, h, steps, s, sc), f"request {r.id}")
+    assert st.cell_steps == sum(w * h * steps for (w, h, steps, _, _) in specs)
+    srv.close()
+
+
+def test_budget_and_recycling(gpu):
+    reqs, specs = trace(12, seed=4)
+    one = max(w * h for (w, h, _, _, _) in specs)
+    budget = int(2.5 * (one * 600 + 12 * 2 ** 20))   # about two canvases at a time
+    srv = czf.Server(hbm_budget=budget, max_members=8)
+    res, st = srv.run(reqs)
+    assert all(r.status == 0 for r in res)
+    assert st.peak_bytes <= budget
+    assert st.films_reused > 0
+    assert st.films_created + st.films_reused == len(reqs)
+    srv.close()
+
+
+def test_reshaped_film_equals_fresh(gpu):
+    big = czf.Film(160, 144, scenes.blobs_settings(160))
+    w, h = 96, 72
+    s = scenes.blobs_settings(w, tilt_deg=30.0)
+    s.height_m = h * s.width_m / w
+    sc = czf.scene(seed=9, nblobs=10, sigma_lo=2.0, sigma_hi=8.0)
+    big.seed_blobs(czf.scene(seed=1))
+    big.step()
+    big.sync()
+    big.reshape(w, h, s)
+    big.seed_blobs(sc)
+    for _ in range(5):
+        big.step()
+    big.sync()
+    assert_same(big, alone(w, h, 5, s, sc), "reshaped")
+    with pytest.raises(czf.FilmError):
+        big.reshape(200, 200, scenes.blobs_settings(200))
