@@ -122,8 +122,8 @@ def batch_canvases(count=64, seed=4):
 
 
 def bench_batch(args, plain_value):
-    """All canvases stepped together through one cz_batch for `frames`
-    frames from their initial states; then each canvas stepped alone through
+    """All canvases, seeded on the device, stepped together through one
+    cz_batch for `frames` frames from their initial states; then each canvas stepped alone through
     the plain path for the same frames, timed, and compared with its batched
     twin (largest relative height difference, floor 1e-9 m)."""
     import numpy as np
@@ -132,13 +132,11 @@ def bench_batch(args, plain_value):
     import scenes
     specs = batch_canvases(args.batch_canvases)
     frames = args.batch_frames
-    log(f"bench batch: generating {len(specs)} canvas states")
-    states = [scenes.blobs_state(w, w, seed=sd) for (w, _, sd) in specs]
     cells = sum(w * w for (w, _, _) in specs)
     films = []
-    for (w, tilt, _), st in zip(specs, states):
+    for (w, tilt, sd) in specs:
         f = czf.Film(w, w, scenes.blobs_settings(w, tilt_deg=tilt))
-        assert f.write_state_arrays(st)
+        f.seed_blobs(czf.scene(seed=sd))
         films.append(f)
     b = czf.Batch(max(len(films), 1))
     if args.graphs:
@@ -158,9 +156,9 @@ def bench_batch(args, plain_value):
     log(f"bench batch: {t_batch * 1e3 / frames:.2f} ms/frame, {batched_value:.3e} cell-steps/s; "
         f"plain comparison")
     t_plain, worst = 0.0, 0.0
-    for k, ((w, tilt, _), st) in enumerate(zip(specs, states)):
+    for k, (w, tilt, sd) in enumerate(specs):
         g = czf.Film(w, w, scenes.blobs_settings(w, tilt_deg=tilt))
-        assert g.write_state_arrays(st)
+        g.seed_blobs(czf.scene(seed=sd))
         g.sync()
         t0 = time.perf_counter()
         for _ in range(frames):
@@ -180,8 +178,58 @@ def bench_batch(args, plain_value):
             "plain_same_canvases": plain_same, "vs_plain_same_canvases": batched_value / plain_same,
             "max_rel_err_vs_plain": worst, "active_cells_per_frame": act,
             "mean_substeps": float(np.mean(subs)), "graphs": bool(args.graphs),
-            "workload": f"{len(specs)} square canvases, side log-uniform [256, 2048], blob states seeded as the "
-                        f"plain canvas, tilt U[0, 60] deg, {frames} frames from the initial states"}
+            "workload": f"{len(specs)} square canvases, side log-uniform [256, 2048], seeded blob canvases "
+                        f"(64 blobs, device generator), tilt U[0, 60] deg, {frames} frames from the initial states"}
+
+
+def serve_trace(seconds, rate=200.0, seed=5):
+    """Poisson arrivals at `rate` per second for `seconds`; canvas side
+    log-uniform in [512, 4096]; steps Pareto(1.5, min 10) truncated at 2000;
+    each canvas a seeded blob scene with the plain workload's cell size and
+    a 15 deg tilt."""
+    import numpy as np
+    import canvaz_film as czf
+    import scenes
+    rng = np.random.default_rng(seed)
+    t, k, reqs = 0.0, 0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= seconds:
+            break
+        w = int(round(float(np.exp(rng.uniform(np.log(512.0), np.log(4096.0))))))
+        steps = int(min(2000, np.floor(10.0 / rng.uniform(0.0, 1.0) ** (1.0 / 1.5))))
+        reqs.append(czf.request(k, w, w, steps, t, scenes.blobs_settings(w, tilt_deg=15.0), czf.scene(seed=7000 + k)))
+        k += 1
+    return reqs
+
+
+def bench_serve(args):
+    """The serving loop on a Poisson trace: latency = completion - arrival."""
+    import numpy as np
+    import canvaz_film as czf
+    reqs = serve_trace(args.serve_seconds)
+    offered = sum(r.w * r.h * r.steps for r in reqs) / args.serve_seconds
+    log(f"bench serve: {len(reqs)} requests over {args.serve_seconds} s, offered {offered:.3e} cell-steps/s")
+    srv = czf.Server(hbm_budget=args.serve_budget_gb << 30, max_members=args.serve_members)
+    res, st = srv.run(reqs)
+    srv.close()
+    ok = [r for r in res if r.status == 0]
+    lat = np.array([r.done_s - r.arrival_s for r in ok])
+    wait = np.array([r.admit_s - r.arrival_s for r in ok])
+    cs = sum(r.cell_steps for r in ok)
+    out = {"requests": len(reqs), "completed": len(ok), "failed": len(reqs) - len(ok),
+           "p50_latency_s": float(np.percentile(lat, 50)) if len(ok) else None,
+           "p99_latency_s": float(np.percentile(lat, 99)) if len(ok) else None,
+           "mean_queue_wait_s": float(wait.mean()) if len(ok) else None,
+           "throughput_cell_steps_per_s": cs / st.wall_s, "wall_s": st.wall_s,
+           "offered_cell_steps_per_s": offered, "frames": st.frames, "peak_members": st.peak_members,
+           "peak_gb": st.peak_bytes / 2 ** 30, "films_created": st.films_created, "films_reused": st.films_reused,
+           "workload": f"Poisson 200 req/s for {args.serve_seconds} s, side log-uniform [512, 4096], steps "
+                       f"Pareto(1.5, min 10, max 2000), seeded blob canvases, budget {args.serve_budget_gb} GB, "
+                       f"at most {args.serve_members} canvases per batch"}
+    log(f"bench serve: p50 {out['p50_latency_s']:.3f} s, p99 {out['p99_latency_s']:.3f} s, "
+        f"{out['throughput_cell_steps_per_s']:.3e} cell-steps/s over {st.wall_s:.1f} s")
+    return out
 
 
 FRAME_FAMILIES = SUBSTEP_FAMILIES + ("lap", "mark", "compact", "cure", "deposit", "split", "reduce_pool",
@@ -207,7 +255,11 @@ def main():
     ap.add_argument("--graphs", action="store_true", help="replay each frame shape as a captured hipGraph")
     ap.add_argument("--no-events", action="store_true",
                     help="time the frames without HIP events around the dominant kernel (roofline from a replay)")
-    ap.add_argument("--workload", default="film", choices=("film", "km", "batch"))
+    ap.add_argument("--workload", default="film", choices=("film", "km", "batch", "serve"))
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload in the film run")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-budget-gb", type=int, default=200)
+    ap.add_argument("--serve-members", type=int, default=256)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload in the film run")
     ap.add_argument("--batch-canvases", type=int, default=64)
     ap.add_argument("--batch-frames", type=int, default=200)
@@ -226,6 +278,9 @@ def main():
         return bench_km(args)
     if args.workload == "batch":
         print(json.dumps(bench_batch(args, None)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
         return
 
     n = args.size
@@ -301,6 +356,7 @@ def main():
         f"substeps/frame {st.act_cell_substeps / max(1, st.act_cells):.1f}, {dom}: {avg_us:.2f} us x {launches}, "
         f"{cells_per_launch:.0f} cells/launch")
     batched = None if args.no_batch else bench_batch(args, value)
+    served = None if args.no_serve else bench_serve(args)
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU restatement baseline")
@@ -322,6 +378,7 @@ def main():
                      "cells_per_launch": cells_per_launch},
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
 
