@@ -319,6 +319,12 @@ void* cz_film_stream(cz_film* f);
  * (substeps, implicit_steps, has_deposits) shape and replay it. 1 = on;
  * off by default (direct launches measured at least as fast). */
 int cz_film_use_graphs(cz_film* f, int on);
+/* Shape of the flux pass: 0 = one block per tile over an LDS window of the
+ * tile and a 4-cell ring, 1 = one block per (tile, face) reading through the
+ * caches (for very small active sets), -1 = chosen online (default): per
+ * active-set size the two shapes are timed with HIP events on live frames
+ * (they give bit-identical results) and the faster one is used. */
+int cz_film_flux_shape(cz_film* f, int shape);
 /* Average per-kernel device time over the last frames (HIP events on the
  * film's stream), in microseconds; label = kernel family name. */
 int cz_film_profile(cz_film* f, int on);
@@ -384,6 +390,8 @@ int cz_batch_step(cz_batch* b, float frame_dt);
 int cz_batch_sync(cz_batch* b);
 /* Replay each frame shape from a captured hipGraph (1 = on, default off). */
 int cz_batch_use_graphs(cz_batch* b, int on);
+/* As cz_film_flux_shape, for the batched flux launches. */
+int cz_batch_flux_shape(cz_batch* b, int shape);
 void* cz_batch_stream(cz_batch* b);
 
 /* ---- Serving ---------------------------------------------------------------

@@ -90,6 +90,7 @@ def lib():
         "cz_film_clear": (i32, [P]),
         "cz_film_stream": (C.c_void_p, [P]),
         "cz_film_use_graphs": (i32, [P, i32]),
+        "cz_film_flux_shape": (i32, [P, i32]),
         "cz_film_profile": (i32, [P, i32]),
         "cz_film_profile_read": (i32, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]),
         "cz_film_profile_only": (i32, [P, C.c_char_p]),
@@ -111,6 +112,7 @@ def lib():
         "cz_batch_step": (i32, [P, f32]),
         "cz_batch_sync": (i32, [P]),
         "cz_batch_use_graphs": (i32, [P, i32]),
+        "cz_batch_flux_shape": (i32, [P, i32]),
         "cz_batch_stream": (C.c_void_p, [P]),
     }
     for name, (res, args) in sig.items():
@@ -373,6 +375,10 @@ class Film:
         check(lib().cz_film_dims(self._p, None, None, C.byref(d)))
         self.dx = d.value
 
+    def flux_shape(self, shape):
+        """-1 = tuned online, 0 = LDS window per tile, 1 = block per (tile, face)."""
+        check(lib().cz_film_flux_shape(self._p, shape))
+
     def seed_blobs(self, scene):
         """Replace the state with a seeded blob canvas generated on the device."""
         check(lib().cz_film_seed_blobs(self._p, C.byref(scene)))
@@ -437,6 +443,9 @@ class Batch:
 
     def use_graphs(self, on=True):
         check(lib().cz_batch_use_graphs(self._p, 1 if on else 0))
+
+    def flux_shape(self, shape):
+        check(lib().cz_batch_flux_shape(self._p, shape))
 
     def stream(self):
         return lib().cz_batch_stream(self._p)

@@ -55,6 +55,11 @@ struct cz_batch {
     uint64_t frames = 0;
     bool use_graphs = false;
     std::vector<std::pair<std::vector<uint32_t>, hipGraphExec_t>> graphs;
+    // flux pass shape per launch (0 = LDS window per tile, 1 = block per
+    // (tile, face)), tuned online
+    czi::ShapeTuner flux_tuner;
+    std::vector<czi::TunerSlot> tuner_pending;
+    std::vector<hipEvent_t> ev_pool;
 };
 
 namespace {
@@ -142,8 +147,20 @@ int poll(cz_batch* b) {
     if (!b->stats_pending) return CZ_OK;
     HIP_TRY(hipStreamSynchronize(b->stream));
     for (uint32_t m = 0; m < (uint32_t)b->members.size(); m++) fold_frame_out(b->members[m], b->frame_out_h + 16u * m);
+    fold_tuners(b->tuner_pending, b->ev_pool);
     b->stats_pending = false;
     return CZ_OK;
+}
+
+hipEvent_t take_event(cz_batch* b) {
+    if (!b->ev_pool.empty()) {
+        hipEvent_t e = b->ev_pool.back();
+        b->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
 }
 
 uint32_t grid_for(uint64_t est, uint64_t total, uint32_t per) {
@@ -160,7 +177,9 @@ uint32_t grid_for(uint64_t est, uint64_t total, uint32_t per) {
 struct FrameShape {
     uint32_t members = 0, max_sub = 0, max_imp = 0, iters = 0;
     std::vector<uint32_t> m_imp;   // members with more than i implicit steps
-    std::vector<uint32_t> g_sub, g_flux, g_imp;
+    std::vector<uint32_t> g_sub, g_flux, g_imp;   // g_flux: the flux shape of each substep
+    std::vector<int> f_bucket;
+    std::vector<uint8_t> f_timed;
     uint32_t g_all = 0, g16 = 0, g64 = 0, g_cure = 0;
     std::vector<uint32_t> key() const {
         std::vector<uint32_t> k = {members, max_sub, max_imp, iters, g_all, g16, g64, g_cure};
@@ -231,7 +250,21 @@ int record(cz_batch* b, const FrameShape& sh) {
         const dim3 g(sh.g_sub[s]);
         hipLaunchKernelGGL(k_lap_plug<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v_lap), s > 0 ? 1 : 0, 1,
                            s == 0 ? 2 : 0);
-        hipLaunchKernelGGL(k_fluxes<ManyFilms>, dim3(sh.g_flux[s]), b256, 0, st, many(b->items_sub, lim_sub + s, v));
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (sh.f_timed[s]) {
+            ea = take_event(b);
+            eb = take_event(b);
+            if (ea && eb) hipEventRecord(ea, st);
+        }
+        if (sh.g_flux[s] == 1u)
+            hipLaunchKernelGGL(k_fluxes_faces<ManyFilms>, dim3(std::min<uint32_t>(4u * sh.g_sub[s], 4u * kGridCap)), b256, 0,
+                               st, many(b->items_sub, lim_sub + s, v));
+        else
+            hipLaunchKernelGGL(k_fluxes<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v));
+        if (ea && eb) {
+            hipEventRecord(eb, st);
+            b->tuner_pending.push_back({&b->flux_tuner, sh.f_bucket[s], (int)sh.g_flux[s], ea, eb});
+        }
         hipLaunchKernelGGL(k_limit_apply<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v));
     }
     const uint32_t* all = lim_sub + sh.max_sub;
@@ -264,6 +297,11 @@ void destroy_batch(cz_batch* b) {
     for (void* p : {(void*)b->desc_h, (void*)b->plan_h, (void*)b->lims_h, (void*)b->frame_out_h})
         if (p) hipHostFree(p);
     for (auto& g : b->graphs) hipGraphExecDestroy(g.second);
+    for (auto& t : b->tuner_pending) {
+        hipEventDestroy(t.a);
+        hipEventDestroy(t.b);
+    }
+    for (auto e : b->ev_pool) hipEventDestroy(e);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
 }
@@ -355,6 +393,12 @@ int cz_batch_use_graphs(cz_batch* b, int on) {
     return CZ_OK;
 }
 
+int cz_batch_flux_shape(cz_batch* b, int shape) {
+    if (!b || shape < -1 || shape > 1) return set_err(CZ_EINVAL, "flux_shape: -1, 0 or 1");
+    b->flux_tuner.fixed = shape;
+    return CZ_OK;
+}
+
 void* cz_batch_stream(cz_batch* b) { return b ? (void*)b->stream : nullptr; }
 
 int cz_batch_step(cz_batch* b, float frame_dt) {
@@ -404,7 +448,12 @@ int cz_batch_step(cz_batch* b, float frame_dt) {
                 tot += b->members[m]->nt16;
             }
         sh.g_sub.push_back(grid_for(est, tot, 1));
-        sh.g_flux.push_back(grid_for(est, tot, 4));
+        int bk = 0;
+        bool timed = false;
+        const int shape = b->flux_tuner.choose(est, !b->use_graphs, bk, timed);
+        sh.g_flux.push_back((uint32_t)shape);
+        sh.f_bucket.push_back(bk);
+        sh.f_timed.push_back(timed ? 1 : 0);
     }
     for (uint32_t i = 0; i < sh.max_imp; i++) {
         uint64_t est = 0, tot = 0;

@@ -51,6 +51,7 @@ struct FilmDev {
     float4* flux;
     float4* acc;
     float* alpha;
+    float* h_sub;         // heights at the start of the substep's apply (written by the lap pass)
     uint32_t* delta;      // current nose offsets (read)
     uint32_t* delta_out;  // written by nose
     float* tmp;
@@ -147,32 +148,22 @@ __device__ __forceinline__ float nose_dist_b(uint32_t bits) {
 }
 __device__ __forceinline__ float nose_dist(const FilmDev& F, uint32_t c) { return nose_dist_b(F.delta[c]); }
 
+// a / (a^2 + b^2) for a, b in -2..2 (0 at the centre): the weights of the
+// front normal's sum (film.wgsl:160-180), folded at compile time with the
+// same IEEE division the shader performs.
+struct FnQuot {
+    float v[5][5];
+    constexpr FnQuot() : v() {
+        for (int a = -2; a <= 2; a++)
+            for (int b = -2; b <= 2; b++)
+                v[a + 2][b + 2] = (a == 0 && b == 0) ? 0.0f : (float)a / (float)(a * a + b * b);
+    }
+};
+__device__ constexpr FnQuot FN_QUOT{};
+
 // film.wgsl:154-156
 __device__ __forceinline__ float hold_level(const FilmDev& F, float d) {
     return F.P.front[0] * (1.0f - cz_expf(-(cz_max(d, 1.0f) - 0.5f) * F.P.front[1]));
-}
-
-// film.wgsl:147-152
-__device__ __forceinline__ float usurf(const FilmDev& F, int i, int j) {
-    uint32_t c = idx(F, refl_i(F, i), refl_j(F, j));
-    float hc = F.h[c];
-    float hold = 0.0f;
-    if (hc > 0.0f) hold = hold_level(F, nose_dist(F, c));
-    return hc + F.m[c] - hold;
-}
-
-// film.wgsl:190-201
-__device__ inline float line_pressure(const FilmDev& F, uint32_t i, uint32_t j) {
-    if (i < 1u || j < 1u || i + 1u >= F.W || j + 1u >= F.H) return 0.0f;
-    float2 nl = F.fnorm[idx(F, i - 1u, j)];
-    float2 nr = F.fnorm[idx(F, i + 1u, j)];
-    float2 nd = F.fnorm[idx(F, i, j - 1u)];
-    float2 nu = F.fnorm[idx(F, i, j + 1u)];
-    if (nl.x * nl.x + nl.y * nl.y < 0.5f || nr.x * nr.x + nr.y * nr.y < 0.5f ||
-        nd.x * nd.x + nd.y * nd.y < 0.5f || nu.x * nu.x + nu.y * nu.y < 0.5f)
-        return 0.0f;
-    float curv = (nr.x - nl.x + nu.y - nd.y) * 0.5f * F.P.inv_dx;
-    return F.P.phys[1] * curv * F.P.front2[2];
 }
 
 // film.wgsl:44-49
@@ -220,24 +211,50 @@ __device__ __forceinline__ float band_flux(float n, float up, float y, float za,
     return up * ((zb - za) - n / (2.0f * n + 1.0f) * y * (ta - tb));
 }
 
+// The face terms below (film.wgsl:134-372, 387-439, 895-920) read the
+// per-cell fields through an accessor, by cell coordinates inside the canvas:
+// GlobalRead goes to HBM; the flux kernel's window reader serves its tile and
+// halo from LDS and only the far steps of cell_excess (h_far, delta_far) from
+// HBM.  The arithmetic is the same whichever accessor is used.
+struct GlobalRead {
+    const FilmDev& F;
+    __device__ __forceinline__ uint32_t at(int i, int j) const { return (uint32_t)j * F.W + (uint32_t)i; }
+    __device__ __forceinline__ float h(int i, int j) const { return F.h[at(i, j)]; }
+    __device__ __forceinline__ float2 rheo(int i, int j) const { return F.rheo[at(i, j)]; }
+    __device__ __forceinline__ uint32_t delta(int i, int j) const { return F.delta[at(i, j)]; }
+    __device__ __forceinline__ float2 fnorm(int i, int j) const { return F.fnorm[at(i, j)]; }
+    __device__ __forceinline__ float2 pres(int i, int j) const { return F.pres[at(i, j)]; }
+    __device__ __forceinline__ float tmp(int i, int j) const { return F.tmp[at(i, j)]; }
+    __device__ __forceinline__ bool active(int i, int j) const { return cell_active(F, (uint32_t)i, (uint32_t)j); }
+    __device__ __forceinline__ float h_far(int i, int j) const { return h(i, j); }
+    __device__ __forceinline__ uint32_t delta_far(int i, int j) const { return delta(i, j); }
+    // ln of the clamped viscosity, front distance, hold level at d = 1 and
+    // the static contact angle: evaluated on demand here, staged by the
+    // flux kernel's window reader
+    __device__ __forceinline__ float lmu(int i, int j) const { return cz_logf(cz_max(rheo(i, j).x, 1e-4f)); }
+    __device__ __forceinline__ float fdist(int i, int j) const {
+        return h(i, j) <= 0.0f ? 0.0f : nose_dist_b(delta(i, j));
+    }
+    __device__ __forceinline__ float hold1() const { return hold_level(F, 1.0f); }
+    __device__ __forceinline__ float theta() const { return cz_atanf(F.P.ein[0]); }
+};
+
 // film.wgsl:134-145
-__device__ inline uint32_t feeders(const FilmDev& F, uint32_t c) {
-    int i = (int)(c % F.W);
-    int j = (int)(c / F.W);
-    float hc = F.h[c];
+template <class A>
+__device__ __forceinline__ uint32_t feeders(const FilmDev& F, const A& a, int i, int j) {
+    float hc = a.h(i, j);
     float thr = 2.0f * hc + F.P.ein[3];
     uint32_t n = 0;
-    if (i > 0 && F.h[c - 1u] > thr) n++;
-    if (i + 1 < (int)F.W && F.h[c + 1u] > thr) n++;
-    if (j > 0 && F.h[c - F.W] > thr) n++;
-    if (j + 1 < (int)F.H && F.h[c + F.W] > thr) n++;
+    if (i > 0 && a.h(i - 1, j) > thr) n++;
+    if (i + 1 < (int)F.W && a.h(i + 1, j) > thr) n++;
+    if (j > 0 && a.h(i, j - 1) > thr) n++;
+    if (j + 1 < (int)F.H && a.h(i, j + 1) > thr) n++;
     return n;
 }
 
 // film.wgsl:160-180
-__device__ inline float2 front_normal(const FilmDev& F, uint32_t c) {
-    int i = (int)(c % F.W);
-    int j = (int)(c / F.W);
+template <class A>
+__device__ __forceinline__ float2 front_normal(const FilmDev& F, const A& a, int i, int j) {
     float dx = 0.0f, dy = 0.0f;
     for (int dj = -2; dj <= 2; dj++) {
         for (int di = -2; di <= 2; di++) {
@@ -245,11 +262,9 @@ __device__ inline float2 front_normal(const FilmDev& F, uint32_t c) {
             int x = i + di, y = j + dj;
             if (x < 0 || y < 0 || x >= (int)F.W || y >= (int)F.H) continue;
             if (di * di + dj * dj > 5) continue;
-            if (F.h[idx(F, (uint32_t)x, (uint32_t)y)] <= F.P.ein[3]) {
-                float ox = (float)di, oy = (float)dj;
-                float dd = ox * ox + oy * oy;
-                dx = dx + ox / dd;
-                dy = dy + oy / dd;
+            if (a.h(x, y) <= F.P.ein[3]) {
+                dx = dx + FN_QUOT.v[di + 2][dj + 2];
+                dy = dy + FN_QUOT.v[dj + 2][di + 2];
             }
         }
     }
@@ -259,7 +274,7 @@ __device__ inline float2 front_normal(const FilmDev& F, uint32_t c) {
 }
 
 // film.wgsl:182-188
-__device__ inline float wet_gain(float2 n) {
+__device__ __forceinline__ float wet_gain(float2 n) {
     float s2 = 0.70710678f;
     float w = W_AXIS * (cz_max(n.x, 0.0f) + cz_max(-n.x, 0.0f) + cz_max(n.y, 0.0f) + cz_max(-n.y, 0.0f));
     w = w + W_DIAG_FLUX * (cz_max(s2 * (n.x + n.y), 0.0f) + cz_max(-s2 * (n.x + n.y), 0.0f) +
@@ -269,9 +284,8 @@ __device__ inline float wet_gain(float2 n) {
 }
 
 // film.wgsl:203-222
-__device__ inline float front_shape(const FilmDev& F, uint32_t c, float2 n) {
-    int i = (int)(c % F.W);
-    int j = (int)(c / F.W);
+template <class A>
+__device__ __forceinline__ float front_shape(const FilmDev& F, const A& a, int i, int j, float2 n) {
     float w = 0.0f, w0 = 0.0f;
     for (int dj = -3; dj <= 3; dj++) {
         for (int di = -3; di <= 3; di++) {
@@ -281,7 +295,7 @@ __device__ inline float front_shape(const FilmDev& F, uint32_t c, float2 n) {
             if (ox * n.x + oy * n.y < -0.5f) w0 = w0 + 1.0f;
             int x = i + di, y = j + dj;
             if (x < 0 || y < 0 || x >= (int)F.W || y >= (int)F.H) continue;
-            if (F.h[idx(F, (uint32_t)x, (uint32_t)y)] > F.P.ein[3]) w = w + 1.0f;
+            if (a.h(x, y) > F.P.ein[3]) w = w + 1.0f;
         }
     }
     if (w0 < 1.0f || (((uint32_t)F.P.front2[3]) & 2u) != 0u) return 1.0f;
@@ -289,24 +303,26 @@ __device__ inline float front_shape(const FilmDev& F, uint32_t c, float2 n) {
 }
 
 // film.wgsl:224-229
-__device__ inline float wet_threshold(const FilmDev& F, uint32_t c) {
-    float ty = F.rheo[c].y;
-    float need = cz_max(ty / (F.P.phys[0] * cz_max(F.h[c], 1e-4f)) - cz_len2(F.P.g[0], F.P.g[1]), 0.0f);
+template <class A>
+__device__ __forceinline__ float wet_threshold(const FilmDev& F, const A& a, int i, int j) {
+    float ty = a.rheo(i, j).y;
+    float need = cz_max(ty / (F.P.phys[0] * cz_max(a.h(i, j), 1e-4f)) - cz_len2(F.P.g[0], F.P.g[1]), 0.0f);
     float ycap = need / cz_max(F.P.g[2], 1e-3f) * F.P.dx;
-    return cz_max(hold_level(F, 1.0f), ycap);
+    return cz_max(a.hold1(), ycap);
 }
 
 // film.wgsl:231-234
-__device__ __forceinline__ float front_dist(const FilmDev& F, uint32_t c) {
-    if (F.h[c] <= 0.0f) return 0.0f;
-    return nose_dist(F, c);
+template <class A>
+__device__ __forceinline__ float front_dist(const A& a, int i, int j) {
+    return a.fdist(i, j);
 }
 
 // film.wgsl:236-244
-__device__ inline float cox_speed(const FilmDev& F, float excess, float mu) {
+template <class A>
+__device__ __forceinline__ float cox_speed(const FilmDev& F, const A& a, float excess, float mu) {
     if (excess <= 0.0f) return 0.0f;
-    float theta = cz_atanf(F.P.ein[0]);
-    float span = cz_max(F.P.front[0] - hold_level(F, 1.0f), 1e-6f);
+    float theta = a.theta();
+    float span = cz_max(F.P.front[0] - a.hold1(), 1e-6f);
     float th = theta + (3.14159265f - theta) * cz_clamp(excess / span, 0.0f, 1.0f);
     float adv = cz_min(theta + F.P.front2[0], 3.1f);
     if (th <= adv) return 0.0f;
@@ -314,50 +330,50 @@ __device__ inline float cox_speed(const FilmDev& F, float excess, float mu) {
 }
 
 // film.wgsl:246-265
-__device__ inline float cell_excess(const FilmDev& F, uint32_t c) {
-    float e = F.h[c] - hold_level(F, front_dist(F, c));
+template <class A>
+__device__ __forceinline__ float cell_excess(const FilmDev& F, const A& a, int ci, int cj) {
+    float e = a.h(ci, cj) - hold_level(F, front_dist(a, ci, cj));
     int ox, oy;
-    unpack_off(F.delta[c], ox, oy);
+    unpack_off(a.delta(ci, cj), ox, oy);
     float ol = cz_len2((float)ox, (float)oy);
     if (ol > 0.5f && ol < 60.0f) {
         float inv = -1.0f / ol;
         float dirx = (float)ox * inv, diry = (float)oy * inv;
         float bx = 0.5f * cz_sign((float)(-ox)), by = 0.5f * cz_sign((float)(-oy));
-        int ci = (int)(c % F.W), cj = (int)(c / F.W);
         for (uint32_t k = 0; k < 3u; k++) {
             float reach = (float)(4u << k);
             int sx = (int)(dirx * reach + bx);
             int sy = (int)(diry * reach + by);
             int ri = ci + sx, rj = cj + sy;
             if (ri < 0 || rj < 0 || ri >= (int)F.W || rj >= (int)F.H) break;
-            uint32_t cr = idx(F, (uint32_t)ri, (uint32_t)rj);
-            float hr = F.h[cr];
+            float hr = a.h_far(ri, rj);
             if (hr <= 0.0f) break;
-            e = cz_max(e, hr - hold_level(F, front_dist(F, cr)));
+            e = cz_max(e, hr - hold_level(F, nose_dist_b(a.delta_far(ri, rj))));
         }
     }
     return e;
 }
 
 // film.wgsl:267-295; returns the translation flux, sign of the normal in tsign
-__device__ inline float rim_translation(const FilmDev& F, uint32_t ca, uint32_t cb, float hf, float2 face_dir,
-                                        float& tsign) {
+template <class A>
+__device__ __forceinline__ float rim_translation(const FilmDev& F, const A& a, int ia, int ja, int ib, int jb, float hf,
+                                        float2 face_dir, float& tsign) {
     tsign = 0.0f;
     if ((((uint32_t)F.P.front2[3]) & 1u) != 0u) return 0.0f;
-    float da = front_dist(F, ca);
-    float db = front_dist(F, cb);
+    float da = front_dist(a, ia, ja);
+    float db = front_dist(a, ib, jb);
     if (cz_max(da, db) > 3.0f / F.P.front[1]) return 0.0f;
-    float ha = F.h[ca], hb = F.h[cb];
+    float ha = a.h(ia, ja), hb = a.h(ib, jb);
     float nx = 0.0f, ny = 0.0f;
     if (da > db && ha > 0.0f) {
-        float2 v = F.fnorm[ca];
+        float2 v = a.fnorm(ia, ja);
         nx = v.x; ny = v.y;
     } else if (db > da && hb > 0.0f) {
-        float2 v = F.fnorm[cb];
+        float2 v = a.fnorm(ib, jb);
         nx = v.x; ny = v.y;
     } else {
-        if (ha > 0.0f) { float2 v = F.fnorm[ca]; nx = nx + v.x; ny = ny + v.y; }
-        if (hb > 0.0f) { float2 v = F.fnorm[cb]; nx = nx + v.x; ny = ny + v.y; }
+        if (ha > 0.0f) { float2 v = a.fnorm(ia, ja); nx = nx + v.x; ny = ny + v.y; }
+        if (hb > 0.0f) { float2 v = a.fnorm(ib, jb); nx = nx + v.x; ny = ny + v.y; }
     }
     float nl = cz_len2(nx, ny);
     if (nl < 1e-6f) return 0.0f;
@@ -365,50 +381,51 @@ __device__ inline float rim_translation(const FilmDev& F, uint32_t ca, uint32_t 
     ny = ny / nl;
     float s = nx * face_dir.x + ny * face_dir.y;
     if (cz_abs(s) < 1e-4f) return 0.0f;
-    uint32_t src = ca, dst = cb;
-    if (s < 0.0f) { src = cb; dst = ca; }
-    if (F.h[src] <= 0.0f) return 0.0f;
-    float e = cell_excess(F, src);
+    int si = ia, sj = ja, di = ib, dj = jb;
+    if (s < 0.0f) { si = ib; sj = jb; di = ia; dj = ja; }
+    if (a.h(si, sj) <= 0.0f) return 0.0f;
+    float e = cell_excess(F, a, si, sj);
     if (e <= 0.0f) return 0.0f;
-    float e_out = -hold_level(F, 1.0f);
-    if (F.h[dst] > 0.0f) e_out = cell_excess(F, dst);
+    float e_out = -a.hold1();
+    if (a.h(di, dj) > 0.0f) e_out = cell_excess(F, a, di, dj);
     float room = cz_clamp(1.0f - cz_max(e_out, 0.0f) / e, 0.0f, 1.0f);
-    float u = cox_speed(F, e, F.rheo[src].x);
+    float u = cox_speed(F, a, e, a.rheo(si, sj).x);
     tsign = cz_sign(s);
     return u * hf * room * cz_mix(1.0f, cz_abs(s), F.P.front2[0]);
 }
 
 struct FaceFlux { float q, y, up; };
 
-// film.wgsl:297-372
-__device__ inline FaceFlux face(const FilmDev& F, uint32_t ca, uint32_t cb, float2 gcap, float2 ghyd, float2 gtilt,
-                                bool diag, float2 face_dir) {
+// film.wgsl:297-372: face between cell a = (ia, ja) and cell b = (ib, jb)
+template <class A>
+__device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia, int ja, int ib, int jb, float2 gcap,
+                                float2 ghyd, float2 gtilt, bool diag, float2 face_dir) {
     FaceFlux out = {0.0f, 0.0f, 0.0f};
-    float ha = F.h[ca];
-    float hb = F.h[cb];
+    float ha = acc.h(ia, ja);
+    float hb = acc.h(ib, jb);
     float hf = 0.5f * (ha + hb);
     if (hf <= 0.0f) return out;
     float shape = 1.0f;
     bool wetting = false;
     float proj = 1.0f;
     if (cz_min(ha, hb) <= 0.0f) {
-        uint32_t wet = (ha > 0.0f) ? ca : cb;
-        uint32_t dry = (ha > 0.0f) ? cb : ca;
-        if (F.h[wet] <= wet_threshold(F, wet)) return out;
+        const int wi = (ha > 0.0f) ? ia : ib, wj = (ha > 0.0f) ? ja : jb;
+        const int di = (ha > 0.0f) ? ib : ia, dj = (ha > 0.0f) ? jb : ja;
+        if (acc.h(wi, wj) <= wet_threshold(F, acc, wi, wj)) return out;
         wetting = true;
-        float2 n = F.fnorm[wet];
-        if (n.x * n.x + n.y * n.y < 0.5f) n = front_normal(F, wet);
+        float2 n = acc.fnorm(wi, wj);
+        if (n.x * n.x + n.y * n.y < 0.5f) n = front_normal(F, acc, wi, wj);
         float toward = (ha > 0.0f) ? 1.0f : -1.0f;
         proj = cz_max(n.x * (face_dir.x * toward) + n.y * (face_dir.y * toward), 0.0f);
         if (proj <= 0.0f) return out;
-        shape = front_shape(F, dry, n) * wet_gain(n);
+        shape = front_shape(F, acc, di, dj, n) * wet_gain(n);
         if (shape <= 0.0f) return out;
     }
-    float2 ra = F.rheo[ca];
-    float2 rb = F.rheo[cb];
+    float2 ra = acc.rheo(ia, ja);
+    float2 rb = acc.rheo(ib, jb);
     float wa = ha / (ha + hb);
     float wb = 1.0f - wa;
-    float mu = cz_expf(wa * cz_logf(cz_max(ra.x, 1e-4f)) + wb * cz_logf(cz_max(rb.x, 1e-4f)));
+    float mu = cz_expf(wa * acc.lmu(ia, ja) + wb * acc.lmu(ib, jb));
     float ty = wa * ra.y + wb * rb.y;
     float thin = cz_min(ha, hb) / hf;
     float fcap = thin * thin * thin;
@@ -426,15 +443,15 @@ __device__ inline FaceFlux face(const FilmDev& F, uint32_t ca, uint32_t cb, floa
         if (gb < 1e-6f) {
             mtilt = 0.0f;
         } else {
-            uint32_t up = (gtilt.x >= 0.0f) ? ca : cb;
-            mtilt = F.tmp[up] * F.h[up] / gb;
+            const int ui = (gtilt.x >= 0.0f) ? ia : ib, uj = (gtilt.x >= 0.0f) ? ja : jb;
+            mtilt = acc.tmp(ui, uj) * acc.h(ui, uj) / gb;
         }
     }
     if (meff <= 0.0f && mtilt <= 0.0f) return out;
     float fcap2 = cz_powf(thin, F.P.front3[3]);
     float qn = mtilt * gtilt.x + cz_min(meff, F.P.misc[3]) * ghyd.x + cz_min(meff, F.P.phys[3]) * (gcap.x * fcap2);
     float trs;
-    float trx = rim_translation(F, ca, cb, hf, face_dir, trs);
+    float trx = rim_translation(F, acc, ia, ja, ib, jb, hf, face_dir, trs);
     if (wetting) {
         float hwet = (ha > 0.0f) ? ha : hb;
         float want = trx * 2.0f * hwet / cz_max(hf, 1e-9f);
@@ -451,9 +468,8 @@ __device__ inline FaceFlux face(const FilmDev& F, uint32_t ca, uint32_t cb, floa
             if (diag) {
                 qn = 0.0f;
             } else {
-                uint32_t tgt = cb;
-                if (hb > ha) tgt = ca;
-                uint32_t nf = feeders(F, tgt);
+                const int ti = (hb > ha) ? ia : ib, tj = (hb > ha) ? ja : jb;
+                uint32_t nf = feeders(F, acc, ti, tj);
                 qn = qn / cz_powf((float)(nf > 1u ? nf : 1u), 0.7f);
             }
         }
@@ -464,73 +480,78 @@ __device__ inline FaceFlux face(const FilmDev& F, uint32_t ca, uint32_t cb, floa
     return out;
 }
 
-__device__ __forceinline__ float2 pressure_at(const FilmDev& F, int i, int j) {
-    return F.pres[idx(F, refl_i(F, i), refl_j(F, j))];
+template <class A>
+__device__ __forceinline__ float2 pressure_at(const FilmDev& F, const A& a, int i, int j) {
+    return a.pres((int)refl_i(F, i), (int)refl_j(F, j));
 }
 
 // film.wgsl:9-14
-__device__ __forceinline__ float2 pressure_wet(const FilmDev& F, int i, int j, float2 fallback) {
-    uint32_t c = idx(F, refl_i(F, i), refl_j(F, j));
-    float2 p = F.pres[c];
-    if (F.h[c] > 0.0f) return p;
+template <class A>
+__device__ __forceinline__ float2 pressure_wet(const FilmDev& F, const A& a, int i, int j, float2 fallback) {
+    const int ri = (int)refl_i(F, i), rj = (int)refl_j(F, j);
+    float2 p = a.pres(ri, rj);
+    if (a.h(ri, rj) > 0.0f) return p;
     return make_float2(fallback.x, p.y);
 }
 
 struct FaceGeom {
     bool has;
     bool diag;
-    uint32_t cb;
+    int ib, jb;   // the other cell of the face
     float2 gcap, ghyd, gtilt, dir;
 };
 
 // Face geometry of the k-th owned face (+x, +y, +x+y, +x-y) of cell (x, y):
 // pressure gradients of film.wgsl:387-439 / 940-978.
-__device__ inline FaceGeom face_geom(const FilmDev& F, uint32_t x, uint32_t y, int k, bool open_check, float2 p00) {
+template <class A>
+__device__ __forceinline__ FaceGeom face_geom(const FilmDev& F, const A& a, int i, int j, int k, bool open_check, float2 p00) {
     FaceGeom fg;
     fg.has = false;
     fg.diag = false;
-    int i = (int)x, j = (int)y;
+    fg.ib = i;
+    fg.jb = j;
+    const int W_ = (int)F.W, H_ = (int)F.H;
     float2 gxy = make_float2(F.P.phys[0] * F.P.g[0], F.P.phys[0] * F.P.g[1]);
     const float s2 = 0.70710678f;
     const float idx_ = F.P.inv_dx;
     if (k == 0) {
-        if (!(x + 1u < F.W && cell_active(F, x + 1u, y))) return fg;
-        fg.cb = idx(F, x + 1u, y);
-        float2 p10 = pressure_at(F, i + 1, j);
+        if (!(i + 1 < W_ && a.active(i + 1, j))) return fg;
+        fg.ib = i + 1;
+        float2 p10 = pressure_at(F, a, i + 1, j);
         float gmx = -(p10.x - p00.x) * idx_, gmy = -(p10.y - p00.y) * idx_;
-        float2 a = pressure_wet(F, i, j + 1, p00), b = pressure_wet(F, i + 1, j + 1, p10);
-        float2 cc = pressure_wet(F, i, j - 1, p00), d = pressure_wet(F, i + 1, j - 1, p10);
-        float ptx = a.x + b.x - cc.x - d.x, pty = a.y + b.y - cc.y - d.y;
+        float2 pa = pressure_wet(F, a, i, j + 1, p00), pb = pressure_wet(F, a, i + 1, j + 1, p10);
+        float2 pc = pressure_wet(F, a, i, j - 1, p00), pd = pressure_wet(F, a, i + 1, j - 1, p10);
+        float ptx = pa.x + pb.x - pc.x - pd.x, pty = pa.y + pb.y - pc.y - pd.y;
         fg.gcap = make_float2(gmx, -ptx * 0.25f * idx_);
         fg.ghyd = make_float2(gmy, -pty * 0.25f * idx_);
         fg.gtilt = gxy;
         fg.dir = make_float2(1.0f, 0.0f);
         fg.has = true;
     } else if (k == 1) {
-        if (!(y + 1u < F.H && cell_active(F, x, y + 1u))) return fg;
-        fg.cb = idx(F, x, y + 1u);
-        float2 p01 = pressure_at(F, i, j + 1);
+        if (!(j + 1 < H_ && a.active(i, j + 1))) return fg;
+        fg.jb = j + 1;
+        float2 p01 = pressure_at(F, a, i, j + 1);
         float gmx = -(p01.x - p00.x) * idx_, gmy = -(p01.y - p00.y) * idx_;
-        float2 a = pressure_wet(F, i + 1, j, p00), b = pressure_wet(F, i + 1, j + 1, p01);
-        float2 cc = pressure_wet(F, i - 1, j, p00), d = pressure_wet(F, i - 1, j + 1, p01);
-        float ptx = a.x + b.x - cc.x - d.x, pty = a.y + b.y - cc.y - d.y;
+        float2 pa = pressure_wet(F, a, i + 1, j, p00), pb = pressure_wet(F, a, i + 1, j + 1, p01);
+        float2 pc = pressure_wet(F, a, i - 1, j, p00), pd = pressure_wet(F, a, i - 1, j + 1, p01);
+        float ptx = pa.x + pb.x - pc.x - pd.x, pty = pa.y + pb.y - pc.y - pd.y;
         fg.gcap = make_float2(gmx, -ptx * 0.25f * idx_);
         fg.ghyd = make_float2(gmy, -pty * 0.25f * idx_);
         fg.gtilt = make_float2(gxy.y, gxy.x);
         fg.dir = make_float2(0.0f, 1.0f);
         fg.has = true;
     } else if (k == 2) {
-        if (!(x + 1u < F.W && y + 1u < F.H && cell_active(F, x + 1u, y + 1u))) return fg;
-        fg.cb = idx(F, x + 1u, y + 1u);
+        if (!(i + 1 < W_ && j + 1 < H_ && a.active(i + 1, j + 1))) return fg;
+        fg.ib = i + 1;
+        fg.jb = j + 1;
         if (open_check) {
             float hs = F.P.ein[3];
-            bool open = cz_min(F.h[idx(F, x, y)], F.h[fg.cb]) > 0.0f || F.h[idx(F, x + 1u, y)] > hs ||
-                        F.h[idx(F, x, y + 1u)] > hs;
+            bool open = cz_min(a.h(i, j), a.h(i + 1, j + 1)) > 0.0f || a.h(i + 1, j) > hs || a.h(i, j + 1) > hs;
             if (!open) return fg;
         }
-        float2 p11 = pressure_at(F, i + 1, j + 1);
-        float2 pa = pressure_wet(F, i + 1, j, p00);
-        float2 pb = pressure_wet(F, i, j + 1, p11);
+        float2 p11 = pressure_at(F, a, i + 1, j + 1);
+        float2 pa = pressure_wet(F, a, i + 1, j, p00);
+        float2 pb = pressure_wet(F, a, i, j + 1, p11);
         fg.gcap = make_float2(-(p11.x - p00.x) * idx_ * s2, -(pa.x - pb.x) * idx_ * s2);
         fg.ghyd = make_float2(-(p11.y - p00.y) * idx_ * s2, -(pa.y - pb.y) * idx_ * s2);
         fg.gtilt = make_float2((gxy.x + gxy.y) * s2, (gxy.x - gxy.y) * s2);
@@ -538,17 +559,17 @@ __device__ inline FaceGeom face_geom(const FilmDev& F, uint32_t x, uint32_t y, i
         fg.diag = true;
         fg.has = true;
     } else {
-        if (!(x + 1u < F.W && y > 0u && cell_active(F, x + 1u, y - 1u))) return fg;
-        fg.cb = idx(F, x + 1u, y - 1u);
+        if (!(i + 1 < W_ && j > 0 && a.active(i + 1, j - 1))) return fg;
+        fg.ib = i + 1;
+        fg.jb = j - 1;
         if (open_check) {
             float hs = F.P.ein[3];
-            bool open = cz_min(F.h[idx(F, x, y)], F.h[fg.cb]) > 0.0f || F.h[idx(F, x + 1u, y)] > hs ||
-                        F.h[idx(F, x, y - 1u)] > hs;
+            bool open = cz_min(a.h(i, j), a.h(i + 1, j - 1)) > 0.0f || a.h(i + 1, j) > hs || a.h(i, j - 1) > hs;
             if (!open) return fg;
         }
-        float2 p1m = pressure_at(F, i + 1, j - 1);
-        float2 pa = pressure_wet(F, i + 1, j, p00);
-        float2 pb = pressure_wet(F, i, j - 1, p1m);
+        float2 p1m = pressure_at(F, a, i + 1, j - 1);
+        float2 pa = pressure_wet(F, a, i + 1, j, p00);
+        float2 pb = pressure_wet(F, a, i, j - 1, p1m);
         fg.gcap = make_float2(-(p1m.x - p00.x) * idx_ * s2, -(pa.x - pb.x) * idx_ * s2);
         fg.ghyd = make_float2(-(p1m.y - p00.y) * idx_ * s2, -(pa.y - pb.y) * idx_ * s2);
         fg.gtilt = make_float2((gxy.x - gxy.y) * s2, (gxy.x + gxy.y) * s2);

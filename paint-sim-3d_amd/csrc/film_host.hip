@@ -76,11 +76,22 @@ void fold_profile(cz_film* f) {
         f->ev_pool.push_back(s.b);
     }
     f->prof_pending.clear();
+    fold_tuners(f->tuner_pending, f->ev_pool);
 }
 
 }  // namespace
 
 namespace czi {
+
+void fold_tuners(std::vector<TunerSlot>& pending, std::vector<hipEvent_t>& pool) {
+    for (auto& s : pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) s.t->record(s.bucket, s.shape, (double)ms * 1000.0);
+        pool.push_back(s.a);
+        pool.push_back(s.b);
+    }
+    pending.clear();
+}
 
 void update_params(cz_film* f, float frame_dt, uint32_t substeps) {
     if (f->ov.no_implicit) f->s.implicit_hydro = 0;
@@ -259,7 +270,24 @@ int record_frame(cz_film* f, uint32_t substeps) {
                (s == 0 && !implicit) ? 1 : 0);
         if (s > 0) f->delta_cur_a = !f->delta_cur_a;
         const OneFilm w{f->dev()};
-        LAUNCH("flux", k_fluxes<OneFilm>, dim3(f->tile_grid(4)), b256, w);
+        int bk = 0;
+        bool timed = false;
+        const int shape = f->flux_tuner.choose(std::min(f->active_hint, f->nt16), !f->capturing && !f->prof_on, bk,
+                                               timed);
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (timed) {
+            ea = f->take_event();
+            eb = f->take_event();
+            if (ea && eb) hipEventRecord(ea, f->stream);
+        }
+        if (shape == 1)
+            LAUNCH("flux", k_fluxes_faces<OneFilm>, dim3(f->tile_grid(4)), b256, w);
+        else
+            LAUNCH("flux", k_fluxes<OneFilm>, gt, b256, w);
+        if (ea && eb) {
+            hipEventRecord(eb, f->stream);
+            f->tuner_pending.push_back({&f->flux_tuner, bk, shape, ea, eb});
+        }
         LAUNCH("apply", k_limit_apply<OneFilm>, gt, b256, w);
     }
     // nose of the last substep + the laplacian strata_split reads
@@ -322,7 +350,9 @@ int encode_frame(cz_film* f, uint32_t substeps) {
     if (!exec) {
         hipGraph_t graph = nullptr;
         HIP_TRY(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
+        f->capturing = true;
         int rc = record_frame(f, substeps);
+        f->capturing = false;
         hipError_t e = hipStreamEndCapture(f->stream, &graph);
         if (rc) return rc;
         if (e != hipSuccess) return set_err(CZ_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
@@ -412,6 +442,10 @@ void destroy(cz_film* f) {
         hipEventDestroy(s.a);
         hipEventDestroy(s.b);
     }
+    for (auto& t : f->tuner_pending) {
+        hipEventDestroy(t.a);
+        hipEventDestroy(t.b);
+    }
     for (auto e : f->ev_pool) hipEventDestroy(e);
     for (auto& g : f->graphs) hipGraphExecDestroy(g.second);
     if (f->stream) hipStreamDestroy(f->stream);
@@ -498,7 +532,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     size_t n = f->n, nt16 = f->nt16, nt64 = (size_t)f->t64x * f->t64y, np = (size_t)f->pw * f->ph;
 #define A(p, cnt) if (!rc) rc = dalloc(f, &f->p, (cnt))
     A(h, n); A(m, n); A(rheo, n); A(strata_h, n); A(strata_pig, 8 * n); A(strata_cure, n); A(sub_refl, 8 * n);
-    A(lay, n); A(pres, n); A(flux, n); A(acc, n); A(alpha, n); A(delta_a, n); A(delta_b, n); A(tmp, n);
+    A(lay, n); A(pres, n); A(flux, n); A(acc, n); A(alpha, n); A(h_sub, n); A(delta_a, n); A(delta_b, n); A(tmp, n);
     A(acc_s, 4 * n); A(alpha_s, n); A(strata_h2, n); A(strata_pig2, 8 * n); A(strata_cure2, n); A(h_frame, n);
     A(fnorm, n); A(cg_m, n); A(cg_d, n); A(cg_x, n); A(cg_r, n); A(cg_p, n); A(cg_q, n); A(cg_t, n); A(cg_t2, n);
     A(cg_part, nt16); A(cg_part2, nt16); A(cg_s, 8); A(tile_flags, nt16); A(tile_list, nt16); A(tile_moved, nt16);
@@ -835,6 +869,12 @@ int cz_film_clear(cz_film* f) {
 }
 
 void* cz_film_stream(cz_film* f) { return f ? (void*)f->stream : nullptr; }
+
+int cz_film_flux_shape(cz_film* f, int shape) {
+    if (!f || shape < -1 || shape > 1) return set_err(CZ_EINVAL, "flux_shape: -1, 0 or 1");
+    f->flux_tuner.fixed = shape;
+    return CZ_OK;
+}
 
 int cz_film_use_graphs(cz_film* f, int on) {
     if (!f) return set_err(CZ_EINVAL, "null film");

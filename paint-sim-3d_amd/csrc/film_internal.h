@@ -39,6 +39,56 @@ struct ProfSlot {
     hipEvent_t a, b;
 };
 
+// Online choice between launch shapes of one kernel family whose results are
+// bit-identical (so exploring costs nothing but time): per size bucket
+// (log2 of the work items) each shape is timed with HIP events until it has
+// WARM samples, then the faster one is used, re-timed once every RECHECK
+// uses.  fixed >= 0 pins a shape.
+struct ShapeTuner {
+    static constexpr int NB = 24, NS = 2, WARM = 3, RECHECK = 64;
+    double sum[NB][NS] = {};
+    uint32_t cnt[NB][NS] = {};
+    uint32_t uses[NB] = {};
+    int fixed = -1;
+    static int bucket(uint64_t items) {
+        int b = 0;
+        while (b + 1 < NB && (1ull << (b + 1)) <= items) b++;
+        return b;
+    }
+    // shape to use; timed = record events around this launch (only when
+    // can_time: not inside a graph capture)
+    int choose(uint64_t items, bool can_time, int& bk, bool& timed) {
+        bk = bucket(items);
+        timed = false;
+        if (fixed >= 0) return fixed;
+        if (!can_time) {
+            if (cnt[bk][0] == 0 || cnt[bk][1] == 0) return cnt[bk][1] > 0 ? 1 : 0;
+            return (sum[bk][0] / cnt[bk][0] <= sum[bk][1] / cnt[bk][1]) ? 0 : 1;
+        }
+        for (int k = 0; k < NS; k++)
+            if (cnt[bk][k] < (uint32_t)WARM) {
+                timed = true;
+                return k;
+            }
+        const int best = (sum[bk][0] / cnt[bk][0] <= sum[bk][1] / cnt[bk][1]) ? 0 : 1;
+        if (++uses[bk] % RECHECK == 0) {
+            timed = true;
+            return 1 - best;
+        }
+        return best;
+    }
+    void record(int bk, int shape, double us) {
+        sum[bk][shape] += us;
+        cnt[bk][shape] += 1;
+    }
+};
+
+struct TunerSlot {
+    ShapeTuner* t;
+    int bucket, shape;
+    hipEvent_t a, b;
+};
+
 }  // namespace czi
 
 struct cz_film {
@@ -51,6 +101,11 @@ struct cz_film {
     float h_max = 0.0f, mu_min = 1.0f, h_max_pending = 0.0f;
     uint32_t implicit_steps = 1, substeps_last = 1, implicit_last = 0, frame = 0, active_tiles = 0;
     uint32_t active_hint = 0xffffffffu;  // grid sizing: last folded active count
+    // flux pass shape: 0 = one block per tile over an LDS window, 1 = one
+    // block per (tile, face) reading through the caches
+    czi::ShapeTuner flux_tuner;
+    std::vector<czi::TunerSlot> tuner_pending;
+    bool capturing = false;
     double total_deposited = 0.0, weave_volume = 0.0;
     float cg_report[4] = {0, 0, 0, 0};
     bool stats_pending = false;
@@ -70,7 +125,7 @@ struct cz_film {
     uint32_t *sub_refl = nullptr, *lay = nullptr;
     float2 *pres = nullptr, *fnorm = nullptr;
     float4 *flux = nullptr, *acc = nullptr, *acc_s = nullptr, *alpha_s = nullptr, *cg_m = nullptr;
-    float *alpha = nullptr, *tmp = nullptr, *h_frame = nullptr;
+    float *alpha = nullptr, *tmp = nullptr, *h_frame = nullptr, *h_sub = nullptr;
     uint32_t *delta_a = nullptr, *delta_b = nullptr;
     bool delta_cur_a = true;
     float *cg_d = nullptr, *cg_x = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr, *cg_t = nullptr,
@@ -114,7 +169,7 @@ struct cz_film {
         F.imp_steps = s.implicit_hydro ? implicit_steps : 0u;
         F.h = h; F.m = m; F.rheo = rheo; F.strata_h = strata_h; F.strata_pig = strata_pig;
         F.strata_cure = strata_cure; F.sub_refl = sub_refl; F.lay = lay;
-        F.pres = pres; F.flux = flux; F.acc = acc; F.alpha = alpha;
+        F.pres = pres; F.flux = flux; F.acc = acc; F.alpha = alpha; F.h_sub = h_sub;
         F.delta = delta_cur(); F.delta_out = delta_other();
         F.tmp = tmp; F.acc_s = acc_s; F.alpha_s = alpha_s; F.strata_h2 = strata_h2; F.strata_pig2 = strata_pig2;
         F.strata_cure2 = strata_cure2; F.h_frame = h_frame; F.fnorm = fnorm; F.cg_m = cg_m;
@@ -174,6 +229,8 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
 // Re-dimension a film within the capacity it was created with, as a new
 // Film with settings s (host state fresh; the caller writes the state).
 int film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s);
+// Fold the tuner timings of finished frames (events already complete).
+void fold_tuners(std::vector<TunerSlot>& pending, std::vector<hipEvent_t>& pool);
 // Take a film out of its batch (before it is destroyed).
 void batch_detach(cz_film* f);
 }  // namespace czi

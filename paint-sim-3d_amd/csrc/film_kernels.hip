@@ -469,103 +469,174 @@ __global__ __launch_bounds__(256) void k_cure(WS ws) {
 }
 
 // ---------------------------------------------------------------------------
-// nose (film.wgsl:680-723) of cell (x, y): offset to the nearest dry cell
-// from the offsets of the previous substep (F.delta) and the heights after
-// apply, and the front normal within 3 cells.
-__device__ inline void nose_cell(const FilmDev& F, uint32_t x, uint32_t y, uint32_t& d_out, float2& fn_out) {
-    uint32_t c = idx(F, x, y);
-    d_out = 0u;
-    fn_out = make_float2(0.0f, 0.0f);
-    if (F.h[c] <= 0.0f) return;
-    int i = (int)x, j = (int)y;
-    int bx = 64, by = 64;
-    float bd = 1e9f;
-    for (int dj = -1; dj <= 1; dj++) {
-        for (int di = -1; di <= 1; di++) {
-            int xx = i + di, yy = j + dj;
-            if (xx < 0 || yy < 0 || xx >= (int)F.W || yy >= (int)F.H) continue;
-            uint32_t nn = idx(F, (uint32_t)xx, (uint32_t)yy);
-            int cx = di, cy = dj;
-            if (F.h[nn] > 0.0f) {
-                uint32_t dv = F.delta[nn];
-                if ((dv & 0x10000u) == 0u) continue;
-                int ox, oy;
-                unpack_off(dv, ox, oy);
-                cx = ox + di;
-                cy = oy + dj;
-                int tx = i + cx, ty = j + cy;
-                if (tx < 0 || ty < 0 || tx >= (int)F.W || ty >= (int)F.H) continue;
-                if (F.h[idx(F, (uint32_t)tx, (uint32_t)ty)] > 0.0f) continue;
-            } else if (di == 0 && dj == 0) {
-                continue;
-            }
-            float d = cz_len2((float)cx, (float)cy);
-            if (d < bd) { bd = d; bx = cx; by = cy; }
-        }
-    }
-    if (bd > 1e8f) return;
-    d_out = pack_off(bx, by);
-    if (bd <= 3.0f) fn_out = front_normal(F, c);
-}
-
 // One substep's first phase on a tile: [nose of the previous substep]
 // (film.wgsl:680-723) + laplacian (film.wgsl:16-42) + plug (film.wgsl:58-87).
-// nose is evaluated for the tile and a 1-cell ring (active cells; the ring is
-// recomputed identically by the neighbouring tile's block) into LDS, since
-// the laplacian of a cell reads the new offsets and front normals of its 8
-// neighbours; the tile's own results go to F.delta_out / F.fnorm.  plug's
-// 7x7 window mean of hb_flux(g_tilt, h_n, mu_n, tau_n) depends on the
-// neighbour's own values only: hb_flux is evaluated once per cell of the tile
-// and a 3-cell ring into LDS and the window summed in the reference's order.
-constexpr int PLUG_SPAN = 16 + 2 * PLUG_RADIUS;
+//
+// The block first stages h, m, the previous offsets, rheology and front
+// normals of the tile and a 3-cell ring (a 22x22 window) in LDS with row-wise
+// loads, plus the flags of the 3x3 tiles around it.  Then, from LDS:
+//  - nose for the tile and a 1-cell ring (active cells; the ring is
+//    recomputed identically by the neighbouring tile's block), since the
+//    laplacian of a cell reads the new offsets and front normals of its 8
+//    neighbours.  Only a nose target beyond the window reads global memory;
+//  - plug's hb_flux(g_tilt, h_n, mu_n, tau_n) once per window cell, summed
+//    over each 7x7 window in the reference's order;
+//  - the laplacian and pressures of the tile's cells.
+// The tile's results go to F.delta_out / F.fnorm / F.pres / F.tmp.
+constexpr int PLUG_SPAN = 16 + 2 * PLUG_RADIUS;   // = the staged window
 constexpr int RING_SPAN = 18;
+constexpr int WIN = PLUG_SPAN, WIN_R = PLUG_RADIUS;
+
+struct LapWin {
+    float h[WIN * WIN];
+    float m[WIN * WIN];
+    uint32_t d[WIN * WIN];
+    float2 rheo[WIN * WIN];
+    float2 fn[WIN * WIN];
+    uint32_t tf[9];
+};
 
 template <class WS>
 __global__ __launch_bounds__(256) void k_lap_plug(WS ws, int do_nose, int do_plug, int do_clear) {
-    __shared__ float s_q[PLUG_SPAN * PLUG_SPAN];
-    __shared__ float s_h[PLUG_SPAN * PLUG_SPAN];
-    __shared__ uint8_t s_y[PLUG_SPAN * PLUG_SPAN];
+    __shared__ LapWin sw;
+    __shared__ float s_q[WIN * WIN];
+    __shared__ float s_h[WIN * WIN];
+    __shared__ uint8_t s_y[WIN * WIN];
     __shared__ uint32_t s_d[RING_SPAN * RING_SPAN];
     __shared__ float2 s_fn[RING_SPAN * RING_SPAN];
+    __shared__ float s_us[RING_SPAN * RING_SPAN];
     CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
         const bool plug_on = do_plug && !(gb < 1e-6f);
-        const int tx0 = (int)((_t % F.tw) * 16u), ty0 = (int)((_t / F.tw) * 16u);
+        const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+        const int tx0 = ttx * 16, ty0 = tty * 16;
+        const int wx0 = tx0 - WIN_R, wy0 = ty0 - WIN_R;
+        const int W_ = (int)F.W, H_ = (int)F.H;
         __syncthreads();
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += 256u) {
+            const int gx = wx0 + (int)(q % WIN), gy = wy0 + (int)(q / WIN);
+            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+                const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+                sw.h[q] = F.h[nn];
+                sw.m[q] = F.m[nn];
+                sw.d[q] = F.delta[nn];
+                sw.rheo[q] = F.rheo[nn];
+                sw.fn[q] = F.fnorm[nn];
+            } else {
+                sw.h[q] = 0.0f;
+                sw.m[q] = 0.0f;
+                sw.d[q] = 0u;
+                sw.rheo[q] = make_float2(0.0f, 0.0f);
+                sw.fn[q] = make_float2(0.0f, 0.0f);
+            }
+        }
+        if (threadIdx.x < 9u) {
+            const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
+            sw.tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
+                                     ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
+        }
+        __syncthreads();
+        // window accessors (callers keep (gx, gy) inside the canvas)
+        auto wq = [&](int gx, int gy) -> int { return (gy - wy0) * WIN + (gx - wx0); };
+        auto in_win = [&](int gx, int gy) -> bool {
+            return gx >= wx0 && gy >= wy0 && gx < wx0 + WIN && gy < wy0 + WIN;
+        };
+        auto act_w = [&](int gx, int gy) -> bool {
+            return (sw.tf[((gy >> 4) - tty + 1) * 3 + ((gx >> 4) - ttx + 1)] & 32u) != 0u;
+        };
+        // nose (film.wgsl:680-723) of ring cell (i, j): offset to the nearest
+        // dry cell from the previous offsets and the heights after apply, and
+        // the front normal (film.wgsl:160-180) within 3 cells
+        auto nose_w = [&](int i, int j, uint32_t& d_out, float2& fn_out) {
+            d_out = 0u;
+            fn_out = make_float2(0.0f, 0.0f);
+            if (sw.h[wq(i, j)] <= 0.0f) return;
+            int bx = 64, by = 64;
+            float bd = 1e9f;
+            for (int dj = -1; dj <= 1; dj++) {
+                for (int di = -1; di <= 1; di++) {
+                    const int xx = i + di, yy = j + dj;
+                    if (xx < 0 || yy < 0 || xx >= W_ || yy >= H_) continue;
+                    int cx = di, cy = dj;
+                    if (sw.h[wq(xx, yy)] > 0.0f) {
+                        const uint32_t dv = sw.d[wq(xx, yy)];
+                        if ((dv & 0x10000u) == 0u) continue;
+                        int ox, oy;
+                        unpack_off(dv, ox, oy);
+                        cx = ox + di;
+                        cy = oy + dj;
+                        const int tx = i + cx, ty = j + cy;
+                        if (tx < 0 || ty < 0 || tx >= W_ || ty >= H_) continue;
+                        // both reads issued, then a select (no pointer select between spaces)
+                        const bool inside = in_win(tx, ty);
+                        const float hl = sw.h[inside ? wq(tx, ty) : 0];
+                        const float hg = F.h[idx(F, (uint32_t)tx, (uint32_t)ty)];
+                        const float ht = inside ? hl : hg;
+                        if (ht > 0.0f) continue;
+                    } else if (di == 0 && dj == 0) {
+                        continue;
+                    }
+                    const float d = cz_len2((float)cx, (float)cy);
+                    if (d < bd) { bd = d; bx = cx; by = cy; }
+                }
+            }
+            if (bd > 1e8f) return;
+            d_out = pack_off(bx, by);
+            if (bd <= 3.0f) {
+                float dx = 0.0f, dy = 0.0f;
+                for (int dj = -2; dj <= 2; dj++) {
+                    for (int di = -2; di <= 2; di++) {
+                        if (di == 0 && dj == 0) continue;
+                        const int xx = i + di, yy = j + dj;
+                        if (xx < 0 || yy < 0 || xx >= W_ || yy >= H_) continue;
+                        if (di * di + dj * dj > 5) continue;
+                        if (sw.h[wq(xx, yy)] <= F.P.ein[3]) {
+                            dx = dx + FN_QUOT.v[di + 2][dj + 2];
+                            dy = dy + FN_QUOT.v[dj + 2][di + 2];
+                        }
+                    }
+                }
+                const float l = cz_len2(dx, dy);
+                fn_out = (l < 1e-6f) ? make_float2(0.0f, 0.0f) : make_float2(dx / l, dy / l);
+            }
+        };
         for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
-            int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
+            const int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
             uint32_t dv = 0u;
             float2 fn = make_float2(0.0f, 0.0f);
-            if (gx >= 0 && gy >= 0 && gx < (int)F.W && gy < (int)F.H) {
-                uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
-                if (do_nose && cell_active(F, (uint32_t)gx, (uint32_t)gy)) {
-                    nose_cell(F, (uint32_t)gx, (uint32_t)gy, dv, fn);
+            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+                if (do_nose && act_w(gx, gy)) {
+                    nose_w(gx, gy, dv, fn);
                 } else {
-                    dv = F.delta[nn];
-                    fn = F.fnorm[nn];
+                    dv = sw.d[wq(gx, gy)];
+                    fn = sw.fn[wq(gx, gy)];
                 }
             }
             s_d[q] = dv;
             s_fn[q] = fn;
+            // usurf (film.wgsl:147-152) of the ring cell, once for the nine
+            // laplacians that read it
+            float us = 0.0f;
+            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+                const float hc = sw.h[wq(gx, gy)];
+                float hold = 0.0f;
+                if (hc > 0.0f) hold = hold_level(F, nose_dist_b(dv));
+                us = hc + sw.m[wq(gx, gy)] - hold;
+            }
+            s_us[q] = us;
         }
         if (plug_on) {
-            const int ox = tx0 - PLUG_RADIUS, oy = ty0 - PLUG_RADIUS;
-            for (uint32_t q = threadIdx.x; q < (uint32_t)(PLUG_SPAN * PLUG_SPAN); q += 256u) {
-                int gx = ox + (int)(q % PLUG_SPAN), gy = oy + (int)(q / PLUG_SPAN);
-                float hn = 0.0f, qv = 0.0f;
+            for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += 256u) {
+                float hn = sw.h[q], qv = 0.0f;
                 uint8_t yl = 0;
-                if (gx >= 0 && gy >= 0 && gx < (int)F.W && gy < (int)F.H) {
-                    uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
-                    hn = F.h[nn];
-                    if (hn > 0.0f) {
-                        float2 r = F.rheo[nn];
-                        yl = r.y > 0.0f ? 1 : 0;
-                        float oy_, oup_;
-                        qv = hb_flux(F.P.phys[2], gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
-                    } else {
-                        hn = 0.0f;
-                    }
+                if (hn > 0.0f) {
+                    const float2 r = sw.rheo[q];
+                    yl = r.y > 0.0f ? 1 : 0;
+                    float oy_, oup_;
+                    qv = hb_flux(F.P.phys[2], gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
+                } else {
+                    hn = 0.0f;
                 }
                 s_q[q] = qv;
                 s_h[q] = hn;
@@ -577,47 +648,46 @@ __global__ __launch_bounds__(256) void k_lap_plug(WS ws, int do_nose, int do_plu
         // clear_acc (film.wgsl:1427-1434) after the cure pass, with the first
         // active-tile kernel of the frame: do_clear 1 = clear, 2 = clear when
         // the canvas has no implicit step this frame (its implicit lap cleared)
+        const int i = (int)x, j = (int)y;
+        const float hv = sw.h[wq(i, j)];
         if (do_clear == 1 || (do_clear == 2 && F.imp_steps == 0u)) {
             F.acc[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            F.h_frame[c] = F.h[c];
+            F.h_frame[c] = hv;
         }
+        // the heights the next apply's limiter reads for this cell: apply
+        // updates h while neighbouring tiles' blocks evaluate their rings
+        if (do_plug) F.h_sub[c] = hv;
         const int own = (int)(_ly + 1u) * RING_SPAN + (int)(_lx + 1u);
         if (do_nose) {
             F.delta_out[c] = s_d[own];
             F.fnorm[c] = s_fn[own];
         }
-        int i = (int)x, j = (int)y;
-        // usurf (film.wgsl:147-152) with the offsets from LDS
+        // usurf at mirrored coordinates, from the ring values
         auto usurf_l = [&](int ii, int jj) -> float {
-            uint32_t ri = refl_i(F, ii), rj = refl_j(F, jj);
-            uint32_t cc = idx(F, ri, rj);
-            float hc = F.h[cc];
-            float hold = 0.0f;
-            if (hc > 0.0f) hold = hold_level(F, nose_dist_b(s_d[((int)rj - ty0 + 1) * RING_SPAN + ((int)ri - tx0 + 1)]));
-            return hc + F.m[cc] - hold;
+            const int ri = (int)refl_i(F, ii), rj = (int)refl_j(F, jj);
+            return s_us[(rj - ty0 + 1) * RING_SPAN + (ri - tx0 + 1)];
         };
-        float hv = F.h[c];
-        float s0 = hv + F.m[c];
+        const float s0 = hv + sw.m[wq(i, j)];
         float hold = 0.0f, pl = 0.0f;
-        float dself = nose_dist_b(s_d[own]);
+        const float dself = nose_dist_b(s_d[own]);
         if (hv > 0.0f) hold = hold_level(F, dself);
-        float u0 = s0 - hold;
-        float ax = usurf_l(i - 1, j) + usurf_l(i + 1, j) + usurf_l(i, j - 1) + usurf_l(i, j + 1);
-        float dg = usurf_l(i - 1, j - 1) + usurf_l(i + 1, j - 1) + usurf_l(i - 1, j + 1) + usurf_l(i + 1, j + 1);
-        float l = (W_AXIS * ax + W_DIAG_LAP * dg - (4.0f * W_AXIS + 4.0f * W_DIAG_LAP) * u0) * F.P.inv_dx * F.P.inv_dx;
+        const float u0 = s0 - hold;
+        const float ax = usurf_l(i - 1, j) + usurf_l(i + 1, j) + usurf_l(i, j - 1) + usurf_l(i, j + 1);
+        const float dg = usurf_l(i - 1, j - 1) + usurf_l(i + 1, j - 1) + usurf_l(i - 1, j + 1) + usurf_l(i + 1, j + 1);
+        const float l = (W_AXIS * ax + W_DIAG_LAP * dg - (4.0f * W_AXIS + 4.0f * W_DIAG_LAP) * u0) * F.P.inv_dx * F.P.inv_dx;
         if (hv > 0.0f) {
-            float d = dself;
+            const float d = dself;
             if (d <= 3.0f) {
-                float2 nn = s_fn[own];
+                const float2 nn = s_fn[own];
                 if (nn.x * nn.x + nn.y * nn.y > 0.5f) {
                     // line_pressure (film.wgsl:190-201) with the normals from LDS
                     float lp = 0.0f;
                     if (!(x < 1u || y < 1u || x + 1u >= F.W || y + 1u >= F.H)) {
-                        float2 nl = s_fn[own - 1], nr = s_fn[own + 1];
-                        float2 nd = s_fn[own - RING_SPAN], nu = s_fn[own + RING_SPAN];
+                        const float2 nl = s_fn[own - 1], nr = s_fn[own + 1];
+                        const float2 nd = s_fn[own - RING_SPAN], nu = s_fn[own + RING_SPAN];
                         if (!(nl.x * nl.x + nl.y * nl.y < 0.5f || nr.x * nr.x + nr.y * nr.y < 0.5f ||
                               nd.x * nd.x + nd.y * nd.y < 0.5f || nu.x * nu.x + nu.y * nu.y < 0.5f)) {
-                            float curv = (nr.x - nl.x + nu.y - nd.y) * 0.5f * F.P.inv_dx;
+                            const float curv = (nr.x - nl.x + nu.y - nd.y) * 0.5f * F.P.inv_dx;
                             lp = F.P.phys[1] * curv * F.P.front2[2];
                         }
                     }
@@ -633,8 +703,8 @@ __global__ __launch_bounds__(256) void k_lap_plug(WS ws, int do_nose, int do_plu
             bool yielding = false;
             for (int dy = 0; dy < 2 * PLUG_RADIUS + 1; dy++) {
                 for (int dxi = 0; dxi < 2 * PLUG_RADIUS + 1; dxi++) {
-                    int q = (int)(_ly + dy) * PLUG_SPAN + (int)(_lx + dxi);
-                    float hn = s_h[q];
+                    const int q = (int)(_ly + dy) * PLUG_SPAN + (int)(_lx + dxi);
+                    const float hn = s_h[q];
                     if (hn <= 0.0f) continue;
                     if (s_y[q]) yielding = true;
                     sum = sum + s_q[q];
@@ -647,12 +717,12 @@ __global__ __launch_bounds__(256) void k_lap_plug(WS ws, int do_nose, int do_plu
     }
 }
 
-// fluxes (film.wgsl:375-455).  The four owned faces of a cell are
-// independent, so each block takes one (tile, face) pair: 4x the waves of a
-// cell-per-thread launch to hide the dependent loads of the contact-line
-// terms.
+// fluxes (film.wgsl:375-455), face-parallel: each block takes one
+// (tile, face) pair and reads its fields from HBM through the caches; 4x the
+// waves of a cell-per-thread launch to hide the dependent loads of the
+// contact-line terms (the shape for very small active sets).
 template <class WS>
-__global__ __launch_bounds__(256) void k_fluxes(WS ws) {
+__global__ __launch_bounds__(256) void k_fluxes_faces(WS ws) {
     const uint32_t nact = ws.count();
     const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
     for (uint32_t wk = blockIdx.x; wk < 4u * nact; wk += gridDim.x) {
@@ -664,35 +734,125 @@ __global__ __launch_bounds__(256) void k_fluxes(WS ws) {
         const uint32_t x = (t % F.tw) * 16u + lx, y = (t / F.tw) * 16u + ly;
         if (x >= F.W || y >= F.H) continue;
         const uint32_t c = idx(F, x, y);
-        float2 p00 = pressure_at(F, (int)x, (int)y);
+        const GlobalRead g{F};
+        float2 p00 = pressure_at(F, g, (int)x, (int)y);
         float q = 0.0f;
-        FaceGeom fg = face_geom(F, x, y, k, true, p00);
+        FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, true, p00);
         if (fg.has) {
-            FaceFlux ff = face(F, c, fg.cb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+            FaceFlux ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
             q = ff.q * ((k < 2) ? W_AXIS : W_DIAG_FLUX);
         }
         flux[4u * c + (uint32_t)k] = q;
     }
 }
 
-__device__ __forceinline__ bool nb_ok(const FilmDev& F, int i, int j) {
-    if (i < 0 || j < 0 || i >= (int)F.W || j >= (int)F.H) return false;
-    return cell_active(F, (uint32_t)i, (uint32_t)j);
-}
+// The fields of the face terms staged for a tile and a FLUX_R-cell ring
+// (front_shape of the dry neighbour reaches 4 cells from the tile); reads
+// outside the window (the far steps of cell_excess) go to HBM.
+constexpr int FLUX_R = 4;
+constexpr int FLUX_WIN = 16 + 2 * FLUX_R;
 
-// limiter (film.wgsl:521-537) of cell (x, y)
-__device__ inline float limiter_cell(const FilmDev& F, uint32_t x, uint32_t y) {
-    uint32_t c = idx(F, x, y);
-    int i = (int)x, j = (int)y;
-    float4 f = F.flux[c];
-    float out = cz_max(f.x, 0.0f) + cz_max(f.y, 0.0f) + cz_max(f.z, 0.0f) + cz_max(f.w, 0.0f);
-    if (nb_ok(F, i - 1, j)) out = out + cz_max(-F.flux[c - 1u].x, 0.0f);
-    if (nb_ok(F, i, j - 1)) out = out + cz_max(-F.flux[c - F.W].y, 0.0f);
-    if (nb_ok(F, i - 1, j - 1)) out = out + cz_max(-F.flux[c - F.W - 1u].z, 0.0f);
-    if (nb_ok(F, i - 1, j + 1)) out = out + cz_max(-F.flux[c + F.W - 1u].w, 0.0f);
-    float a = 1.0f;
-    if (out > 0.0f) a = cz_min(1.0f, 0.5f * F.h[c] * F.P.dx / (F.P.dt * out));
-    return a;
+struct FluxWin {
+    float h[FLUX_WIN * FLUX_WIN];
+    uint32_t d[FLUX_WIN * FLUX_WIN];
+    float2 rheo[FLUX_WIN * FLUX_WIN];
+    float2 fn[FLUX_WIN * FLUX_WIN];
+    float2 pres[FLUX_WIN * FLUX_WIN];
+    float tmp[FLUX_WIN * FLUX_WIN];
+    float lmu[FLUX_WIN * FLUX_WIN];   // ln max(mu, 1e-4)
+    float fd[FLUX_WIN * FLUX_WIN];    // front distance (0 when dry)
+    uint32_t tf[9];
+};
+
+struct WinRead {
+    const FilmDev& F;
+    const FluxWin& w;
+    int wx0, wy0, ttx, tty;
+    __device__ __forceinline__ int q(int i, int j) const { return (j - wy0) * FLUX_WIN + (i - wx0); }
+    // cells within FLUX_R of the tile: always inside the window
+    __device__ __forceinline__ float h(int i, int j) const { return w.h[q(i, j)]; }
+    __device__ __forceinline__ uint32_t delta(int i, int j) const { return w.d[q(i, j)]; }
+    __device__ __forceinline__ float2 rheo(int i, int j) const { return w.rheo[q(i, j)]; }
+    __device__ __forceinline__ float2 fnorm(int i, int j) const { return w.fn[q(i, j)]; }
+    __device__ __forceinline__ float2 pres(int i, int j) const { return w.pres[q(i, j)]; }
+    __device__ __forceinline__ float tmp(int i, int j) const { return w.tmp[q(i, j)]; }
+    __device__ __forceinline__ float lmu(int i, int j) const { return w.lmu[q(i, j)]; }
+    __device__ __forceinline__ float fdist(int i, int j) const { return w.fd[q(i, j)]; }
+    float hold1_, theta_;
+    __device__ __forceinline__ float hold1() const { return hold1_; }
+    __device__ __forceinline__ float theta() const { return theta_; }
+    // the far steps of cell_excess: the window when inside, else HBM (both
+    // reads are issued, so no pointer selects between the two spaces)
+    __device__ __forceinline__ bool in(int i, int j) const {
+        return (unsigned)(i - wx0) < (unsigned)FLUX_WIN && (unsigned)(j - wy0) < (unsigned)FLUX_WIN;
+    }
+    __device__ __forceinline__ float h_far(int i, int j) const {
+        const bool inside = in(i, j);
+        const float vl = w.h[inside ? q(i, j) : 0];
+        const float vg = F.h[(uint32_t)j * F.W + (uint32_t)i];
+        return inside ? vl : vg;
+    }
+    __device__ __forceinline__ uint32_t delta_far(int i, int j) const {
+        const bool inside = in(i, j);
+        const uint32_t vl = w.d[inside ? q(i, j) : 0];
+        const uint32_t vg = F.delta[(uint32_t)j * F.W + (uint32_t)i];
+        return inside ? vl : vg;
+    }
+    // cells within one of the tile: one of the 3x3 tiles around it
+    __device__ __forceinline__ bool active(int i, int j) const {
+        return (w.tf[((j >> 4) - tty + 1) * 3 + ((i >> 4) - ttx + 1)] & 32u) != 0u;
+    }
+};
+
+// fluxes (film.wgsl:375-455) of a tile, one cell's four faces per thread,
+// with the fields staged in LDS (the shape for large active sets).
+template <class WS>
+__global__ __launch_bounds__(256) void k_fluxes(WS ws) {
+    __shared__ FluxWin sw;
+    CZ_FOR_TILES(ws) {
+        CZ_TILE_CELL(F)
+        const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+        const int wx0 = ttx * 16 - FLUX_R, wy0 = tty * 16 - FLUX_R;
+        const int W_ = (int)F.W, H_ = (int)F.H;
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(FLUX_WIN * FLUX_WIN); q += 256u) {
+            const int gx = wx0 + (int)(q % FLUX_WIN), gy = wy0 + (int)(q / FLUX_WIN);
+            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+                const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+                sw.h[q] = F.h[nn];
+                sw.d[q] = F.delta[nn];
+                sw.rheo[q] = F.rheo[nn];
+                sw.fn[q] = F.fnorm[nn];
+                sw.pres[q] = F.pres[nn];
+                sw.tmp[q] = F.tmp[nn];
+                const float hv = sw.h[q];
+                sw.lmu[q] = cz_logf(cz_max(sw.rheo[q].x, 1e-4f));
+                sw.fd[q] = hv <= 0.0f ? 0.0f : nose_dist_b(sw.d[q]);
+            }
+        }
+        if (threadIdx.x < 9u) {
+            const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
+            sw.tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
+                                     ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
+        }
+        __syncthreads();
+        if (!in_grid) continue;
+        const WinRead g{F, sw, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
+        const float2 p00 = pressure_at(F, g, (int)x, (int)y);
+        float4 qv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 1
+        for (int k = 0; k < 4; k++) {
+            FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, true, p00);
+            if (!fg.has) continue;
+            FaceFlux ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+            const float q = ff.q * ((k < 2) ? W_AXIS : W_DIAG_FLUX);
+            if (k == 0) qv.x = q;
+            else if (k == 1) qv.y = q;
+            else if (k == 2) qv.z = q;
+            else qv.w = q;
+        }
+        F.flux[c] = qv;
+    }
 }
 
 struct Inflow { float best; uint32_t src; float div; float accv; };
@@ -722,72 +882,110 @@ __device__ inline void inherit_from(const FilmDev& F, uint32_t c, uint32_t src, 
 // limiter + apply (film.wgsl:521-537, 803-863) on a tile.  apply reads the
 // limiter factor of the cell and of its 8 neighbours, so the block evaluates
 // the limiter for the tile and a 1-cell ring (active cells) into LDS; the
-// factors never leave the chip.
+// factors never leave the chip.  Fluxes and heights of the tile and a 2-cell
+// ring (what the ring's limiter reads) are staged in LDS first; the heights
+// come from the copy the lap pass wrote (h_sub), since the blocks of the
+// neighbouring tiles update h during this pass.
+constexpr int AP_R = 2;
+constexpr int AP_WIN = 16 + 2 * AP_R;
+
 template <class WS>
 __global__ __launch_bounds__(256) void k_limit_apply(WS ws) {
+    __shared__ float4 s_f[AP_WIN * AP_WIN];
+    __shared__ float s_hh[AP_WIN * AP_WIN];
     __shared__ float s_a[RING_SPAN * RING_SPAN];
+    __shared__ uint32_t s_tf[9];
     CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
-        const int tx0 = (int)((_t % F.tw) * 16u), ty0 = (int)((_t / F.tw) * 16u);
+        const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+        const int tx0 = ttx * 16, ty0 = tty * 16;
+        const int wx0 = tx0 - AP_R, wy0 = ty0 - AP_R;
+        const int W_ = (int)F.W, H_ = (int)F.H;
         __syncthreads();
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += 256u) {
+            const int gx = wx0 + (int)(q % AP_WIN), gy = wy0 + (int)(q / AP_WIN);
+            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+                const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+                s_f[q] = F.flux[nn];
+                s_hh[q] = F.h_sub[nn];
+            }
+        }
+        if (threadIdx.x < 9u) {
+            const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
+            s_tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
+                                    ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
+        }
+        __syncthreads();
+        auto wq = [&](int gx, int gy) -> int { return (gy - wy0) * AP_WIN + (gx - wx0); };
+        // in the canvas and in an active tile (cells within 2 of the tile)
+        auto nb_ok = [&](int gi, int gj) -> bool {
+            if (gi < 0 || gj < 0 || gi >= W_ || gj >= H_) return false;
+            return (s_tf[((gj >> 4) - tty + 1) * 3 + ((gi >> 4) - ttx + 1)] & 32u) != 0u;
+        };
+        // limiter (film.wgsl:521-537) of cell (gi, gj)
+        auto limiter = [&](int gi, int gj) -> float {
+            const float4 f = s_f[wq(gi, gj)];
+            float out = cz_max(f.x, 0.0f) + cz_max(f.y, 0.0f) + cz_max(f.z, 0.0f) + cz_max(f.w, 0.0f);
+            if (nb_ok(gi - 1, gj)) out = out + cz_max(-s_f[wq(gi - 1, gj)].x, 0.0f);
+            if (nb_ok(gi, gj - 1)) out = out + cz_max(-s_f[wq(gi, gj - 1)].y, 0.0f);
+            if (nb_ok(gi - 1, gj - 1)) out = out + cz_max(-s_f[wq(gi - 1, gj - 1)].z, 0.0f);
+            if (nb_ok(gi - 1, gj + 1)) out = out + cz_max(-s_f[wq(gi - 1, gj + 1)].w, 0.0f);
+            float a = 1.0f;
+            if (out > 0.0f) a = cz_min(1.0f, 0.5f * s_hh[wq(gi, gj)] * F.P.dx / (F.P.dt * out));
+            return a;
+        };
         for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
-            int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
-            float a = 0.0f;
-            if (gx >= 0 && gy >= 0 && gx < (int)F.W && gy < (int)F.H && cell_active(F, (uint32_t)gx, (uint32_t)gy))
-                a = limiter_cell(F, (uint32_t)gx, (uint32_t)gy);
-            s_a[q] = a;
+            const int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
+            s_a[q] = nb_ok(gx, gy) ? limiter(gx, gy) : 0.0f;
         }
         __syncthreads();
         bool moved = false;
         if (in_grid) {
-            auto alpha_at = [&](uint32_t cc) -> float {
-                int xx = (int)(cc % F.W), yy = (int)(cc / F.W);
-                return s_a[(yy - ty0 + 1) * RING_SPAN + (xx - tx0 + 1)];
-            };
-            int i = (int)x, j = (int)y;
-            float ac = s_a[(int)(_ly + 1u) * RING_SPAN + (int)(_lx + 1u)];
+            const int i = (int)x, j = (int)y;
+            auto alpha_at = [&](int gi, int gj) -> float { return s_a[(gj - ty0 + 1) * RING_SPAN + (gi - tx0 + 1)]; };
+            const float ac = alpha_at(i, j);
             Inflow st = {0.0f, c, 0.0f, 0.0f};
-            float4 f = F.flux[c];
-            bool hx = x + 1u < F.W, hy = y + 1u < F.H, hm = y > 0u;
-            uint32_t xp = (x + 1u < F.W - 1u) ? x + 1u : F.W - 1u;
-            uint32_t yp = (y + 1u < F.H - 1u) ? y + 1u : F.H - 1u;
+            const float4 f = s_f[wq(i, j)];
+            const bool hx = x + 1u < F.W, hy = y + 1u < F.H, hm = y > 0u;
+            const int xp = (x + 1u < F.W - 1u) ? i + 1 : W_ - 1;
+            const int yp = (y + 1u < F.H - 1u) ? j + 1 : H_ - 1;
+            const int ym = hm ? j - 1 : 0;
             float accv[4];
             const float fq[4] = {f.x, f.y, f.z, f.w};
             const bool has[4] = {hx, hy, hx && hy, hx && hm};
-            const uint32_t cbs[4] = {hx ? idx(F, xp, y) : c, hy ? idx(F, x, yp) : c, (hx && hy) ? idx(F, xp, yp) : c,
-                                     (hx && hm) ? idx(F, xp, hm ? y - 1u : 0u) : c};
+            const int bi[4] = {hx ? xp : i, hx ? i : i, (hx && hy) ? xp : i, (hx && hm) ? xp : i};
+            const int bj[4] = {j, hy ? yp : j, (hx && hy) ? yp : j, (hx && hm) ? ym : j};
             // own_face (film.wgsl:546-554)
             for (int k = 0; k < 4; k++) {
-                float q = fq[k];
+                const float q = fq[k];
                 float ab = ac;
                 if (q < 0.0f && has[k]) {
-                    ab = alpha_at(cbs[k]);
-                    if (-q * ab > st.best) { st.best = -q * ab; st.src = cbs[k]; }
+                    ab = alpha_at(bi[k], bj[k]);
+                    if (-q * ab > st.best) { st.best = -q * ab; st.src = idx(F, (uint32_t)bi[k], (uint32_t)bj[k]); }
                 }
                 st.div = st.div + q * ab;
                 accv[k] = q * ab;
             }
             // nb_face (film.wgsl:556-561)
-            const bool nbk[4] = {nb_ok(F, i - 1, j), nb_ok(F, i, j - 1), nb_ok(F, i - 1, j - 1), nb_ok(F, i - 1, j + 1)};
-            const uint32_t cns[4] = {c - 1u, c - F.W, c - F.W - 1u, c + F.W - 1u};
+            const int ni[4] = {i - 1, i, i - 1, i - 1};
+            const int nj[4] = {j, j - 1, j - 1, j + 1};
             for (int k = 0; k < 4; k++) {
-                if (!nbk[k]) continue;
-                uint32_t cn = cns[k];
-                float4 fn = F.flux[cn];
-                float q = (k == 0) ? fn.x : (k == 1) ? fn.y : (k == 2) ? fn.z : fn.w;
-                float an = alpha_at(cn);
+                if (!nb_ok(ni[k], nj[k])) continue;
+                const float4 fn = s_f[wq(ni[k], nj[k])];
+                const float q = (k == 0) ? fn.x : (k == 1) ? fn.y : (k == 2) ? fn.z : fn.w;
+                float an = alpha_at(ni[k], nj[k]);
                 if (q < 0.0f) an = ac;
-                if (q * an > st.best) { st.best = q * an; st.src = cn; }
+                if (q * an > st.best) { st.best = q * an; st.src = idx(F, (uint32_t)ni[k], (uint32_t)nj[k]); }
                 st.div = st.div - q * an;
             }
-            uint32_t src = st.src;
-            float h0 = F.h[c];
+            const uint32_t src = st.src;
+            const float h0 = s_hh[wq(i, j)];
             float hn = cz_max(h0 - F.P.dt * st.div * F.P.inv_dx, 0.0f);
             if (hn < h0 && hn < DRAINED_FILM) hn = 0.0f;
             F.h[c] = hn;
             moved = cz_abs(hn - h0) > MOVE_EPS;
             if (h0 <= 0.0f && src != c) inherit_from(F, c, src, hn, true);
-            float4 a = F.acc[c];
+            const float4 a = F.acc[c];
             F.acc[c] = make_float4(a.x + accv[0] * F.P.dt, a.y + accv[1] * F.P.dt, a.z + accv[2] * F.P.dt,
                                    a.w + accv[3] * F.P.dt);
         }
@@ -835,7 +1033,8 @@ __global__ __launch_bounds__(256) void k_strata_split(WS ws) {
     CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
-        float2 p00 = pressure_at(F, (int)x, (int)y);
+        const GlobalRead g{F};
+        float2 p00 = pressure_at(F, g, (int)x, (int)y);
         float4 a = F.acc[c];
         float av[4] = {a.x, a.y, a.z, a.w};
         float4 o[4];
@@ -846,9 +1045,9 @@ __global__ __launch_bounds__(256) void k_strata_split(WS ws) {
                         : (k == 2) ? (x + 1u < F.W && y + 1u < F.H)
                                    : (x + 1u < F.W && y > 0u);
             if (av[k] != 0.0f && exists) {
-                FaceGeom fg = face_geom(F, x, y, k, true, p00);
+                FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, true, p00);
                 FaceFlux ff = {0.0f, 0.0f, 0.0f};
-                if (fg.has) ff = face(F, c, fg.cb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+                if (fg.has) ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
                 uint32_t cb = (k == 0) ? c + 1u : (k == 1) ? c + F.W : (k == 2) ? c + F.W + 1u : c - F.W + 1u;
                 uint32_t cu = (av[k] < 0.0f) ? cb : c;
                 o[k] = band_split(F, cu, 0.5f * (F.h[c] + F.h[cb]), ff.y, ff.up, av[k]);
@@ -1220,7 +1419,7 @@ __device__ inline float face_mob(const FilmDev& F, uint32_t ca, uint32_t cb, flo
         if (diag) return 0.0f;
         uint32_t tgt = cb;
         if (hb > ha) tgt = ca;
-        uint32_t nf = feeders(F, tgt);
+        uint32_t nf = feeders(F, GlobalRead{F}, (int)(tgt % F.W), (int)(tgt / F.W));
         mob = mob / cz_powf((float)(nf > 1u ? nf : 1u), 0.7f);
     }
     return mob;
@@ -1240,12 +1439,13 @@ __global__ __launch_bounds__(256) void k_cg_setup(WS ws) {
     CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
-        float2 p00 = pressure_at(F, (int)x, (int)y);
+        const GlobalRead g{F};
+        float2 p00 = pressure_at(F, g, (int)x, (int)y);
         float mob[4];
         for (int k = 0; k < 4; k++) {
             mob[k] = 0.0f;
-            FaceGeom fg = face_geom(F, x, y, k, false, p00);
-            if (fg.has) mob[k] = face_mob(F, c, fg.cb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag);
+            FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, false, p00);
+            if (fg.has) mob[k] = face_mob(F, c, idx(F, (uint32_t)fg.ib, (uint32_t)fg.jb), fg.gcap, fg.ghyd, fg.gtilt, fg.diag);
         }
         F.cg_m[c] = make_float4(mob[0], mob[1], mob[2], mob[3]);
     }
@@ -1632,6 +1832,7 @@ CZ_INST1(k_mark, ManyFilms)
 CZ_INST1(k_cure, OneFilmCure)
 CZ_INST1(k_cure, ManyFilms)
 CZ_INST2(k_fluxes)
+CZ_INST2(k_fluxes_faces)
 CZ_INST2(k_limit_apply)
 CZ_INST2(k_delta_sync)
 CZ_INST2(k_strata_split)

@@ -43,3 +43,39 @@ def test_graph_replay_equals_direct(gpu):
     direct.sync()
     graph.sync()
     assert_same(direct, graph, "end")
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_flux_shapes_agree(gpu, batched):
+    """The two launch shapes of the flux pass (and the online choice between
+    them) give bit-identical frames."""
+    w = h = 112
+    s = scenes.blobs_settings(w, tilt_deg=35.0)
+    st = scenes.blobs_state(w, h, seed=8, nblobs=10, sigma_cells=(2.0, 10.0))
+    films = []
+    for shape in (0, 1, -1):
+        f = czf.Film(w, h, s)
+        assert f.write_state_arrays(st)
+        films.append(f)
+        if not batched:
+            f.flux_shape(shape)
+    batches = []
+    if batched:
+        for f, shape in zip(films, (0, 1, -1)):
+            b = czf.Batch(2)
+            b.flux_shape(shape)
+            b.add(f)
+            batches.append(b)
+    for _ in range(12):
+        if batched:
+            for b in batches:
+                b.step()
+        else:
+            for f in films:
+                f.step()
+    for b in batches:
+        b.sync()
+    for f in films:
+        f.sync()
+    assert_same(films[0], films[1], "window vs faces")
+    assert_same(films[0], films[2], "window vs tuned")
