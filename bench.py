@@ -34,6 +34,11 @@ ALGO_BYTES = {
     "apply": 16 + 4 + 16 + 4 + 16,                  # R flux h acc, W h acc
 }
 SUBSTEP_FAMILIES = tuple(ALGO_BYTES)
+SUBSTEP_BYTES = sum(ALGO_BYTES.values())            # per active cell and substep
+ALGO_BYTES["lap"] = 4 + 4 + 4 + 8 + 4 + 8 + 8       # last nose: R h m delta fnorm, W delta fnorm pres
+# families the roofline is reported for: the per-pass substep kernels and the
+# one-launch substeps kernel (k_substeps) that replaces them on small active sets
+ROOFLINE_FAMILIES = SUBSTEP_FAMILIES + ("substeps",)
 
 
 def log(msg):
@@ -139,8 +144,7 @@ def bench_batch(args, plain_value):
         f.seed_blobs(czf.scene(seed=sd))
         films.append(f)
     b = czf.Batch(max(len(films), 1))
-    if args.graphs:
-        b.use_graphs(True)
+    b.use_graphs(not args.no_graphs)
     for f in films:
         b.add(f)
     log(f"bench batch: {len(films)} canvases, {cells / 1e6:.1f} M cells, timing {frames} frames")
@@ -177,7 +181,7 @@ def bench_batch(args, plain_value):
             "vs_plain_2048": batched_value / plain_value if plain_value else None,
             "plain_same_canvases": plain_same, "vs_plain_same_canvases": batched_value / plain_same,
             "max_rel_err_vs_plain": worst, "active_cells_per_frame": act,
-            "mean_substeps": float(np.mean(subs)), "graphs": bool(args.graphs),
+            "mean_substeps": float(np.mean(subs)), "graphs": not args.no_graphs,
             "workload": f"{len(specs)} square canvases, side log-uniform [256, 2048], seeded blob canvases "
                         f"(64 blobs, device generator), tilt U[0, 60] deg, {frames} frames from the initial states"}
 
@@ -232,7 +236,7 @@ def bench_serve(args):
     return out
 
 
-FRAME_FAMILIES = SUBSTEP_FAMILIES + ("lap", "mark", "compact", "cure", "deposit", "split", "reduce_pool",
+FRAME_FAMILIES = SUBSTEP_FAMILIES + ("substeps", "lap", "mark", "compact", "cure", "deposit", "split", "reduce_pool",
                                      "strata_limit", "strata_update", "copy_back", "strata_fill", "delta_sync",
                                      "frame_stats", "cg_setup", "cg_lap", "cg_init", "cg_iter_a", "cg_iter_b",
                                      "cg_iter_c", "cg_reduce", "cg_finish", "cg_rescale")
@@ -252,9 +256,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graphs", action="store_true", help="replay each frame shape as a captured hipGraph")
-    ap.add_argument("--no-events", action="store_true",
-                    help="time the frames without HIP events around the dominant kernel (roofline from a replay)")
+    ap.add_argument("--no-graphs", action="store_true", help="direct launches instead of captured hipGraph replays")
+    ap.add_argument("--substep-launch", default="auto", choices=("auto", "per-pass", "one"),
+                    help="substeps as one launch per frame on small active sets (auto), always, or never")
+    ap.add_argument("--events-in-timed", action="store_true",
+                    help="record the roofline's HIP events inside the timed frames (default: a separate replay)")
     ap.add_argument("--workload", default="film", choices=("film", "km", "batch", "serve"))
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload in the film run")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
@@ -290,7 +296,8 @@ def main():
 
     def fresh():
         f = czf.Film(n, n, settings_fn(n))
-        czf.check(czf.lib().cz_film_use_graphs(f._p, 1 if args.graphs else 0))
+        czf.check(czf.lib().cz_film_use_graphs(f._p, 0 if args.no_graphs else 1))
+        f.substep_launch({"auto": -1, "per-pass": 0, "one": 1}[args.substep_launch])
         assert f.write_state_arrays(state)
         return f
 
@@ -308,17 +315,15 @@ def main():
             fam_time[fam] = avg * cnt
     log(f"bench: warmup {args.warmup} frames, per-family device time (us avg x launches): " +
         ", ".join(f"{k} {v[0]:.1f}x{v[1]}" for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][0] * kv[1][1])))
-    dom = max((f for f in fam_time if f in SUBSTEP_FAMILIES), key=fam_time.get)
     film.close()
     del film
 
     # timed pass: a fresh film from the initial state, K frames
     film = fresh()
-    if not args.no_events:
+    if args.events_in_timed:
         film.profile(True)
-        film.profile_only(dom)
     film.reset_counters()
-    log(f"bench: timing {args.steps} frames (events around {dom}: {not args.no_events})")
+    log(f"bench: timing {args.steps} frames (events in the timed frames: {args.events_in_timed})")
     torch.cuda.synchronize()
     film.sync()
     t0 = time.perf_counter()
@@ -328,30 +333,40 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     st = film.stats()
-    if args.no_events:
-        # the same frames again with events around the dominant family
+    if not args.events_in_timed:
+        # the same frames again with events around every launch
         film.close()
         del film
         film = fresh()
         film.profile(True)
-        film.profile_only(dom)
         film.reset_counters()
         for _ in range(args.steps):
             film.step()
         film.sync()
         st = film.stats()
-    avg_us, launches = film.profile_read(dom)
+    prof = {fam: film.profile_read(fam) for fam in ROOFLINE_FAMILIES}
     film.profile(False)
     film.close()
     del film
-
-    cells_per_launch = st.act_cell_substeps / max(1, launches)
-    bytes_per_launch = ALGO_BYTES[dom] * cells_per_launch
+    dom = max(ROOFLINE_FAMILIES, key=lambda fam: prof[fam][0] * prof[fam][1])
+    avg_us, launches = prof[dom]
+    if dom == "substeps":
+        # one launch per frame: every substep's three passes and the last nose
+        cells_per_launch = st.one_launch_cell_substeps / max(1, launches)
+        bytes_per_launch = (SUBSTEP_BYTES * st.one_launch_cell_substeps
+                            + ALGO_BYTES["lap"] * st.one_launch_cells) / max(1, launches)
+        algo_per_cell = SUBSTEP_BYTES
+    else:
+        cells_per_launch = (st.act_cell_substeps - st.one_launch_cell_substeps) / max(1, launches)
+        bytes_per_launch = ALGO_BYTES[dom] * cells_per_launch
+        algo_per_cell = ALGO_BYTES[dom]
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * n * args.steps / elapsed
     frames = max(1, st.frames_counted)
     frame_gbs = frame_algo_bytes(n * n, st) / elapsed / 1e9
+    log(f"bench: replay, per-family device time (us avg x launches): " +
+        ", ".join(f"{k} {v[0]:.2f}x{v[1]}" for k, v in prof.items() if v[1]))
     log(f"bench: {ms_per_step:.3f} ms/frame, active cells/frame {st.act_cells / frames:.0f}, "
         f"substeps/frame {st.act_cell_substeps / max(1, st.act_cells):.1f}, {dom}: {avg_us:.2f} us x {launches}, "
         f"{cells_per_launch:.0f} cells/launch")
@@ -367,14 +382,15 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"plain film {n}x{n} FP32, 64 seeded Gaussian blobs, tilt 15 deg, frame_dt 1/60, "
                                f"{args.steps} frames from the initial state",
-                   "cells": n * n, "graphs": bool(args.graphs),
+                   "cells": n * n, "graphs": not args.no_graphs, "substep_launch": args.substep_launch,
+                   "one_launch_frames": st.one_launch_frames / max(1, st.frames_counted),
                    "active_cells_per_frame": st.act_cells / frames,
                    "substeps_per_frame": st.act_cell_substeps / max(1, st.act_cells),
                    "implicit_cell_steps": st.act_cell_implicit,
                    "frame_algo_GBps": frame_gbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algo_bytes_per_cell": ALGO_BYTES[dom], "avg_launch_us": avg_us, "launches": launches,
+                     "algo_bytes_per_cell": algo_per_cell, "avg_launch_us": avg_us, "launches": launches,
                      "cells_per_launch": cells_per_launch},
         "cpu_baseline": cpu,
         "batched": batched,

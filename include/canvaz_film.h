@@ -155,6 +155,10 @@ typedef struct cz_stats {
     uint64_t act_cell_substeps;
     uint64_t frames_counted;
     uint64_t act_cell_implicit;
+    /* the same sums over the frames whose substeps ran as one launch */
+    uint64_t one_launch_frames;
+    uint64_t one_launch_cells;
+    uint64_t one_launch_cell_substeps;
 } cz_stats;
 
 /* The mass report of `compute_stats` (src/main.rs:1679-1819), sums in FP64. */
@@ -315,10 +319,21 @@ int cz_film_dry_now(cz_film* f);
 int cz_film_clear(cz_film* f);
 /* The HIP stream the film's work is enqueued on (hipStream_t). */
 void* cz_film_stream(cz_film* f);
-/* Capture every frame of cz_film_encode_frame into a hipGraph keyed by the
- * (substeps, implicit_steps, has_deposits) shape and replay it. 1 = on;
- * off by default (direct launches measured at least as fast). */
+/* Replay each frame of cz_film_encode_frame from a hipGraph captured once
+ * per frame shape (substeps, implicit steps, deposit pass, offset-buffer
+ * parity, params, grid sizes, flux shape). 1 = on (default), 0 = direct
+ * launches.  Frames whose flux size class is still being timed by the shape
+ * tuner, and profiled frames, run direct. */
 int cz_film_use_graphs(cz_film* f, int on);
+/* How the substeps of a frame are launched: 0 = three launches per substep
+ * (lap/plug, flux, limiter + apply), 1 = one launch for all of them with
+ * grid-wide barriers between the passes (k_substeps), -1 = chosen per frame
+ * (default; currently always the three launches, measured faster on gfx950).
+ * Results are identical; no counterpart in the reference, whose encoder
+ * records the passes of every substep. */
+int cz_film_substep_launch(cz_film* f, int mode);
+/* Frame graphs captured and frames replayed from them so far. */
+int cz_film_graph_counts(const cz_film* f, uint64_t* captures, uint64_t* replays);
 /* Shape of the flux pass: 0 = one block per tile over an LDS window of the
  * tile and a 4-cell ring, 1 = one block per (tile, face) reading through the
  * caches (for very small active sets), -1 = chosen online (default): per
@@ -388,8 +403,11 @@ uint32_t cz_batch_size(const cz_batch* b);
 int cz_batch_step(cz_batch* b, float frame_dt);
 /* Wait for the last frame and fold its stats into every member. */
 int cz_batch_sync(cz_batch* b);
-/* Replay each frame shape from a captured hipGraph (1 = on, default off). */
+/* Replay each frame shape from a captured hipGraph (1 = on, the default;
+ * frames still timing flux shapes run direct). */
 int cz_batch_use_graphs(cz_batch* b, int on);
+/* As cz_film_graph_counts, for the batch's frames. */
+int cz_batch_graph_counts(const cz_batch* b, uint64_t* captures, uint64_t* replays);
 /* As cz_film_flux_shape, for the batched flux launches. */
 int cz_batch_flux_shape(cz_batch* b, int shape);
 void* cz_batch_stream(cz_batch* b);
@@ -439,6 +457,13 @@ void cz_server_destroy(cz_server* s);
 int cz_server_run(cz_server* s, const cz_request* reqs, uint32_t n, cz_result* res, cz_server_stats* st);
 /* A finished canvas kept with keep_films (owned by the server), or NULL. */
 cz_film* cz_server_film(cz_server* s, uint64_t id);
+
+/* Device self-check with no counterpart in the reference: the kernels pack
+ * and unpack f16 slots (pigments, substrate bands) with the hardware
+ * conversions; this counts, over every non-NaN f32 and f16 bit pattern, the
+ * inputs where they differ from the software round-to-nearest-even forms
+ * the host restatement uses (both counts are 0 on a conforming device). */
+int cz_f16_selftest(uint64_t* bad_f2h, uint64_t* bad_h2f);
 
 const char* cz_last_error(void);
 const char* cz_version(void);

@@ -90,6 +90,9 @@ def lib():
         "cz_film_clear": (i32, [P]),
         "cz_film_stream": (C.c_void_p, [P]),
         "cz_film_use_graphs": (i32, [P, i32]),
+        "cz_film_graph_counts": (i32, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "cz_f16_selftest": (i32, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "cz_film_substep_launch": (i32, [P, i32]),
         "cz_film_flux_shape": (i32, [P, i32]),
         "cz_film_profile": (i32, [P, i32]),
         "cz_film_profile_read": (i32, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]),
@@ -112,6 +115,7 @@ def lib():
         "cz_batch_step": (i32, [P, f32]),
         "cz_batch_sync": (i32, [P]),
         "cz_batch_use_graphs": (i32, [P, i32]),
+        "cz_batch_graph_counts": (i32, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "cz_batch_flux_shape": (i32, [P, i32]),
         "cz_batch_stream": (C.c_void_p, [P]),
     }
@@ -379,6 +383,20 @@ class Film:
         """-1 = tuned online, 0 = LDS window per tile, 1 = block per (tile, face)."""
         check(lib().cz_film_flux_shape(self._p, shape))
 
+    def substep_launch(self, mode):
+        """-1 = by active-set size, 0 = three launches per substep, 1 = one launch per frame."""
+        check(lib().cz_film_substep_launch(self._p, mode))
+
+    def use_graphs(self, on=True):
+        """Replay frames from captured hipGraphs (the default) or launch directly."""
+        check(lib().cz_film_use_graphs(self._p, 1 if on else 0))
+
+    def graph_counts(self):
+        """(graphs captured, frames replayed from them)."""
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        check(lib().cz_film_graph_counts(self._p, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def seed_blobs(self, scene):
         """Replace the state with a seeded blob canvas generated on the device."""
         check(lib().cz_film_seed_blobs(self._p, C.byref(scene)))
@@ -443,6 +461,12 @@ class Batch:
 
     def use_graphs(self, on=True):
         check(lib().cz_batch_use_graphs(self._p, 1 if on else 0))
+
+    def graph_counts(self):
+        """(graphs captured, frames replayed from them)."""
+        a, b = C.c_uint64(0), C.c_uint64(0)
+        check(lib().cz_batch_graph_counts(self._p, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def flux_shape(self, shape):
         check(lib().cz_batch_flux_shape(self._p, shape))

@@ -53,8 +53,10 @@ struct cz_batch {
     bool lists_dirty = true;
     bool stats_pending = false;
     uint64_t frames = 0;
-    bool use_graphs = false;
+    bool use_graphs = true;
     std::vector<std::pair<std::vector<uint32_t>, hipGraphExec_t>> graphs;
+    std::vector<std::vector<uint32_t>> graph_seen;  // shapes met once, not yet captured
+    uint64_t graph_captures = 0, graph_replays = 0;
     // flux pass shape per launch (0 = LDS window per tile, 1 = block per
     // (tile, face)), tuned online
     czi::ShapeTuner flux_tuner;
@@ -276,8 +278,7 @@ int record(cz_batch* b, const FrameShape& sh) {
     hipLaunchKernelGGL(k_reduce_pool<ManyFilms>, dim3(sh.g64), b256, 0, st, many(b->all64, n_all64, 3));
     hipLaunchKernelGGL(k_strata_limit<ManyFilms>, ga, b256, 0, st, w);
     hipLaunchKernelGGL(k_strata_update<ManyFilms>, ga, b256, 0, st, w);
-    hipLaunchKernelGGL(k_strata_copy_back<ManyFilms>, ga, b256, 0, st, w);
-    for (int k = 0; k < 6; k++) hipLaunchKernelGGL(k_strata_fill<ManyFilms>, ga, b256, 0, st, w, k);
+    hipLaunchKernelGGL(k_strata_fill<ManyFilms>, ga, b256, 0, st, w);
     hipLaunchKernelGGL(k_frame_stats<ManyCanvases>, dim3(sh.members), b256, 0, st,
                        ManyCanvases{b->desc, nullptr, nullptr, 3u});
     HIP_TRY(hipGetLastError());
@@ -387,6 +388,13 @@ int cz_batch_remove(cz_batch* b, cz_film* f) {
 
 uint32_t cz_batch_size(const cz_batch* b) { return b ? (uint32_t)b->members.size() : 0u; }
 
+int cz_batch_graph_counts(const cz_batch* b, uint64_t* captures, uint64_t* replays) {
+    if (!b || !captures || !replays) return set_err(CZ_EINVAL, "graph_counts: null argument");
+    *captures = b->graph_captures;
+    *replays = b->graph_replays;
+    return CZ_OK;
+}
+
 int cz_batch_use_graphs(cz_batch* b, int on) {
     if (!b) return set_err(CZ_EINVAL, "null batch");
     b->use_graphs = on != 0;
@@ -440,6 +448,10 @@ int cz_batch_step(cz_batch* b, float frame_dt) {
         sh.max_imp = std::max(sh.max_imp, imp[k]);
         hint_all += std::min(b->members[k]->active_hint, b->members[k]->nt16);
     }
+    // per substep: active-set estimate and grid; a frame whose flux size
+    // classes still need timing samples runs uncaptured
+    std::vector<uint64_t> est_sub(sh.max_sub);
+    bool learn = false;
     for (uint32_t s = 0; s < sh.max_sub; s++) {
         uint64_t est = 0, tot = 0;
         for (uint32_t m = 0; m < nm; m++)
@@ -448,9 +460,14 @@ int cz_batch_step(cz_batch* b, float frame_dt) {
                 tot += b->members[m]->nt16;
             }
         sh.g_sub.push_back(grid_for(est, tot, 1));
+        est_sub[s] = est;
+        learn = learn || b->flux_tuner.learning(est);
+    }
+    const bool graph = b->use_graphs && !learn;
+    for (uint32_t s = 0; s < sh.max_sub; s++) {
         int bk = 0;
         bool timed = false;
-        const int shape = b->flux_tuner.choose(est, !b->use_graphs, bk, timed);
+        const int shape = b->flux_tuner.choose(est_sub[s], !graph, bk, timed);
         sh.g_flux.push_back((uint32_t)shape);
         sh.f_bucket.push_back(bk);
         sh.f_timed.push_back(timed ? 1 : 0);
@@ -489,12 +506,21 @@ int cz_batch_step(cz_batch* b, float frame_dt) {
     HIP_TRY(hipMemcpyAsync(b->desc, b->desc_h, (size_t)nm * CZ_VARIANTS * sizeof(FilmDev), hipMemcpyHostToDevice,
                            b->stream));
     HIP_TRY(hipMemcpyAsync(b->plan, b->plan_h, (size_t)b->max_members * 4 * 4, hipMemcpyHostToDevice, b->stream));
-    if (b->use_graphs) {
+    if (graph) {
         std::vector<uint32_t> key = sh.key();
         hipGraphExec_t exec = nullptr;
         for (auto& g : b->graphs)
             if (g.first == key) exec = g.second;
-        if (!exec) {
+        bool seen = exec != nullptr;
+        for (const auto& o : b->graph_seen) seen = seen || o == key;
+        if (!seen) {
+            // a shape is captured the second time it comes: one-off shapes
+            // (membership or substep changes) run direct
+            if (b->graph_seen.size() >= 64) b->graph_seen.erase(b->graph_seen.begin());
+            b->graph_seen.push_back(key);
+            rc = record(b, sh);
+            if (rc) return rc;
+        } else if (!exec) {
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             rc = record(b, sh);
@@ -509,13 +535,20 @@ int cz_batch_step(cz_batch* b, float frame_dt) {
                 b->graphs.erase(b->graphs.begin());
             }
             b->graphs.push_back({key, exec});
+            b->graph_captures++;
         }
-        HIP_TRY(hipGraphLaunch(exec, b->stream));
+        if (exec) {
+            HIP_TRY(hipGraphLaunch(exec, b->stream));
+            b->graph_replays++;
+        }
     } else {
         rc = record(b, sh);
         if (rc) return rc;
     }
-    for (uint32_t m = 0; m < nm; m++) frame_done(b->members[m], sub[m], false);
+    for (uint32_t m = 0; m < nm; m++) {
+        b->members[m]->one_launch_now = false;
+        frame_done(b->members[m], sub[m], false);
+    }
     b->stats_pending = true;
     b->frames++;
     return CZ_OK;

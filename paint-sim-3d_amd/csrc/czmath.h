@@ -16,7 +16,8 @@
  *   atan  reduction to |t| <= tan(pi/8) by 1/x and (x-1)/(x+1), odd Taylor
  *         series to t^19.                                    (<= 2 ULP)
  *   f16   IEEE round-to-nearest-even with subnormals (pack2x16float), and
- *         the exact inverse (unpack2x16float).
+ *         the exact inverse (unpack2x16float); the device build uses the
+ *         hardware conversions, which round the same way.
  * Every operation is an IEEE-754 single operation or an explicit fmaf;
  * translation units that include this header are built without FP
  * contraction, so host and device round identically.
@@ -155,7 +156,7 @@ CZ_INLINE float cz_atan2f(float y, float x) {
 }
 
 /* f32 -> binary16 bits, round to nearest even, with subnormals. */
-CZ_INLINE uint32_t cz_f2h(float f) {
+CZ_INLINE uint32_t cz_f2h_sw(float f) {
     uint32_t x = cz_f2bits(f);
     uint32_t sign = (x >> 16) & 0x8000u;
     uint32_t ax = x & 0x7fffffffu;
@@ -181,7 +182,7 @@ CZ_INLINE uint32_t cz_f2h(float f) {
 }
 
 /* binary16 bits -> f32 (exact). */
-CZ_INLINE float cz_h2f(uint32_t h) {
+CZ_INLINE float cz_h2f_sw(uint32_t h) {
     uint32_t sign = (h & 0x8000u) << 16;
     uint32_t e = (h >> 10) & 0x1fu;
     uint32_t m = h & 0x3ffu;
@@ -192,6 +193,20 @@ CZ_INLINE float cz_h2f(uint32_t h) {
     if (e == 31u) return cz_bits2f(sign | 0x7f800000u | (m << 13));
     return cz_bits2f(sign | ((e + 112u) << 23) | (m << 13));
 }
+
+#if defined(__HIP_DEVICE_COMPILE__)
+/* On the device the same two conversions are single instructions:
+ * v_cvt_f16_f32 rounds to nearest even and keeps f16 subnormals (the default
+ * MODE of HIP kernels), v_cvt_f32_f16 is exact.  They give the software
+ * forms' bits for every non-NaN input (cz_f16_selftest checks all 2^32 f32
+ * and 2^16 f16 patterns on the device); only NaN payloads may differ, and
+ * the film state holds no NaN. */
+CZ_INLINE uint32_t cz_f2h(float f) { return (uint32_t)__builtin_bit_cast(unsigned short, (_Float16)f); }
+CZ_INLINE float cz_h2f(uint32_t h) { return (float)__builtin_bit_cast(_Float16, (unsigned short)h); }
+#else
+CZ_INLINE uint32_t cz_f2h(float f) { return cz_f2h_sw(f); }
+CZ_INLINE float cz_h2f(uint32_t h) { return cz_h2f_sw(h); }
+#endif
 
 /* WGSL unpack2x16float / pack2x16float. */
 CZ_INLINE float cz_unpack_lo(uint32_t v) { return cz_h2f(v & 0xffffu); }

@@ -76,7 +76,6 @@ struct FilmDev {
     float2* stats_out;
     float* pool_out;
     float* tsum_a;        // strata sums, fill pass input
-    float* tsum_b;        // fill pass output
     const float* pigtab;
     // deposits
     const cz_deposit* deposits;
@@ -84,6 +83,7 @@ struct FilmDev {
     const uint32_t* tile_count;
     const uint32_t* tile_items;
     float* frame_out;     // [16] per-frame scalars read by the host
+    uint32_t* grid_sync;  // [4] barrier of the one-launch substeps: count, generation, timeout
 };
 
 __device__ __forceinline__ uint32_t idx(const FilmDev& F, uint32_t i, uint32_t j) { return j * F.W + i; }

@@ -111,7 +111,13 @@ void fold_frame_out(cz_film* f, const float* o) {
     f->cnt_act_cells += cells;
     f->cnt_act_cell_substeps += cells * (uint64_t)f->substeps_last;
     f->cnt_act_cell_implicit += cells * (uint64_t)f->implicit_last;
+    if (f->one_launch_last) {
+        f->cnt_one_frames += 1;
+        f->cnt_one_cells += cells;
+        f->cnt_one_cell_substeps += cells * (uint64_t)f->substeps_last;
+    }
     f->cnt_frames += 1;
+    if (cz_f2bits(o[15]) != 0u) f->sync_fault = true;
 }
 
 int wait_idle(cz_film* f) {
@@ -143,6 +149,7 @@ void frame_done(cz_film* f, uint32_t substeps, bool parity_done) {
     f->stats_pending = true;
     f->substeps_last = substeps;
     f->implicit_last = f->s.implicit_hydro ? f->implicit_steps : 0u;
+    f->one_launch_last = f->one_launch_now;
     f->frame++;
     f->deposit_count = 0;
 }
@@ -159,6 +166,7 @@ int poll_stats(cz_film* f) {
     HIP_TRY(hipStreamSynchronize(f->stream));
     fold_profile(f);
     fold_frame_out(f, f->frame_out_h);
+    if (f->sync_fault) return set_err(CZ_EHIP, "a grid barrier of the one-launch substeps timed out: the frame is invalid");
     return CZ_OK;
 }
 
@@ -264,8 +272,21 @@ int record_frame(cz_film* f, uint32_t substeps) {
             LAUNCH("cg_rescale", k_cg_rescale<OneFilm>, gt, b256, w);
         }
     }
+    const uint32_t pg = f->persist_grid();
+    f->one_launch_now = pg > 0u;
+    if (pg > 0u) {
+        // every substep and the last nose in one launch
+        const FilmDev Fa = f->dev();
+        f->delta_cur_a = !f->delta_cur_a;
+        const FilmDev Fb = f->dev();
+        f->delta_cur_a = !f->delta_cur_a;
+        LAUNCH("substeps", k_substeps, dim3(pg), b256, Fa, Fb, substeps, implicit ? 0 : 1);
+        // the host's offset-buffer parity as the per-pass launches leave it:
+        // one flip per nose (substeps 1.. and the last one)
+        if (substeps & 1u) f->delta_cur_a = !f->delta_cur_a;
+    }
     // substeps: [nose of the previous substep +] lap + plug, flux, limiter + apply
-    for (uint32_t s = 0; s < substeps; s++) {
+    for (uint32_t s = 0; s < (pg > 0u ? 0u : substeps); s++) {
         LAUNCH("lap_plug", k_lap_plug<OneFilm>, gt, b256, OneFilm{f->dev()}, s > 0 ? 1 : 0, 1,
                (s == 0 && !implicit) ? 1 : 0);
         if (s > 0) f->delta_cur_a = !f->delta_cur_a;
@@ -291,16 +312,17 @@ int record_frame(cz_film* f, uint32_t substeps) {
         LAUNCH("apply", k_limit_apply<OneFilm>, gt, b256, w);
     }
     // nose of the last substep + the laplacian strata_split reads
-    LAUNCH("lap", k_lap_plug<OneFilm>, gt, b256, OneFilm{f->dev()}, 1, 0, 0);
-    f->delta_cur_a = !f->delta_cur_a;
+    if (pg == 0u) {
+        LAUNCH("lap", k_lap_plug<OneFilm>, gt, b256, OneFilm{f->dev()}, 1, 0, 0);
+        f->delta_cur_a = !f->delta_cur_a;
+    }
     const OneFilm w{f->dev()};
     LAUNCH("delta_sync", k_delta_sync<OneFilm>, gt, b256, w);
     LAUNCH("split", k_strata_split<OneFilm>, gt, b256, w);
     LAUNCH("reduce_pool", k_reduce_pool<OneFilmAll>, dim3(n64), b256, OneFilmAll{w.F, n64});
     LAUNCH("strata_limit", k_strata_limit<OneFilm>, gt, b256, w);
     LAUNCH("strata_update", k_strata_update<OneFilm>, gt, b256, w);
-    LAUNCH("copy_back", k_strata_copy_back<OneFilm>, gt, b256, w);
-    for (int k = 0; k < 6; k++) LAUNCH("strata_fill", k_strata_fill<OneFilm>, gt, b256, w, k);
+    LAUNCH("strata_fill", k_strata_fill<OneFilm>, gt, b256, w);
     LAUNCH("frame_stats", k_frame_stats<OneCanvas>, dim3(1), b256, OneCanvas{w.F});
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(f->frame_out_h, f->frame_out, 16 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
@@ -311,20 +333,24 @@ int record_frame(cz_film* f, uint32_t substeps) {
 // shape is everything the launches depend on (substeps, implicit steps,
 // deposit pass, offset-buffer parity, params, grid sizes).
 struct GraphKey {
-    uint32_t substeps, implicit_steps, implicit_on, has_dep, parity, grid1, grid4;
+    uint32_t substeps, implicit_steps, implicit_on, has_dep, parity, grid1, grid4, flux_shape, persist;
     cz_params P;
     bool operator==(const GraphKey& o) const { return memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
 
+int encode_direct(cz_film* f, uint32_t substeps) {
+    int rc = record_frame(f, substeps);
+    if (rc) return rc;
+    frame_done(f, substeps, true);
+    return CZ_OK;
+}
+
 int encode_frame(cz_film* f, uint32_t substeps) {
-    if (!f->use_graphs || f->prof_on) {
-        bool start_a = f->delta_cur_a;
-        int rc = record_frame(f, substeps);
-        if (rc) return rc;
-        (void)start_a;
-        frame_done(f, substeps, true);
-        return CZ_OK;
-    }
+    if (f->sync_fault) return set_err(CZ_EHIP, "the film's last one-launch substeps failed; it takes no more frames");
+    const uint64_t flux_items = std::min(f->active_hint, f->nt16);
+    const uint32_t pg = f->persist_grid();
+    if (!f->use_graphs || f->prof_on || (pg == 0u && f->flux_tuner.learning(flux_items)))
+        return encode_direct(f, substeps);
     GraphKey key;
     memset(&key, 0, sizeof(key));
     key.substeps = substeps;
@@ -334,6 +360,10 @@ int encode_frame(cz_film* f, uint32_t substeps) {
     key.parity = f->delta_cur_a ? 1u : 0u;
     key.grid1 = f->tile_grid(1);
     key.grid4 = f->tile_grid(4);
+    int bk = 0;
+    bool timed = false;
+    key.flux_shape = pg > 0u ? 0u : (uint32_t)f->flux_tuner.choose(flux_items, false, bk, timed);
+    key.persist = pg;
     key.P = f->P;
     hipGraphExec_t exec = nullptr;
     for (size_t k = 0; k < f->graphs.size(); k++) {
@@ -348,6 +378,17 @@ int encode_frame(cz_film* f, uint32_t substeps) {
     }
     bool start_a = f->delta_cur_a;
     if (!exec) {
+        // a shape is captured the second time it comes: one-off shapes (a
+        // changing active set, a deposit frame) run direct
+        GraphKeyStore ks;
+        memcpy(&ks, &key, sizeof(key));
+        bool seen = false;
+        for (const GraphKeyStore& o : f->graph_seen) seen = seen || memcmp(&o, &ks, sizeof(ks)) == 0;
+        if (!seen) {
+            if (f->graph_seen.size() >= 64) f->graph_seen.erase(f->graph_seen.begin());
+            f->graph_seen.push_back(ks);
+            return encode_direct(f, substeps);
+        }
         hipGraph_t graph = nullptr;
         HIP_TRY(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
         f->capturing = true;
@@ -363,6 +404,7 @@ int encode_frame(cz_film* f, uint32_t substeps) {
             hipGraphExecDestroy(f->graphs.front().second);
             f->graphs.erase(f->graphs.begin());
         }
+        f->graph_captures++;
         GraphKeyStore kk;
         static_assert(sizeof(GraphKeyStore) == sizeof(GraphKey), "graph key size");
         memcpy(&kk, &key, sizeof(key));
@@ -370,6 +412,7 @@ int encode_frame(cz_film* f, uint32_t substeps) {
         f->delta_cur_a = start_a;  // record_frame flipped it while capturing
     }
     HIP_TRY(hipGraphLaunch(exec, f->stream));
+    f->graph_replays++;
     frame_done(f, substeps, false);
     return CZ_OK;
 }
@@ -483,6 +526,7 @@ void reset_host_state(cz_film* f) {
     for (int k = 0; k < 4; k++) f->cg_report[k] = 0.0f;
     f->stats_pending = false;
     f->cnt_act_cells = f->cnt_act_cell_substeps = f->cnt_frames = f->cnt_act_cell_implicit = 0;
+    f->cnt_one_frames = f->cnt_one_cells = f->cnt_one_cell_substeps = 0;
     f->delta_cur_a = true;
     f->deposit_count = 0;
     memset(&f->P, 0, sizeof(f->P));
@@ -536,7 +580,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     A(acc_s, 4 * n); A(alpha_s, n); A(strata_h2, n); A(strata_pig2, 8 * n); A(strata_cure2, n); A(h_frame, n);
     A(fnorm, n); A(cg_m, n); A(cg_d, n); A(cg_x, n); A(cg_r, n); A(cg_p, n); A(cg_q, n); A(cg_t, n); A(cg_t2, n);
     A(cg_part, nt16); A(cg_part2, nt16); A(cg_s, 8); A(tile_flags, nt16); A(tile_list, nt16); A(tile_moved, nt16);
-    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n); A(tsum_b, n);
+    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
     A(pigtab, CZ_NPIG * CZ_PIG_STRIDE); A(deposits, CZ_MAX_DEPOSITS); A(tile_start, nt64); A(tile_count, nt64);
     A(tile_items, CZ_MAX_TILE_ITEMS); A(frame_out, 16);
 #undef A
@@ -580,6 +624,7 @@ int film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s) {
     }
     for (auto& g : f->graphs) hipGraphExecDestroy(g.second);
     f->graphs.clear();
+    f->graph_seen.clear();
     f->weave_volume = 0.0;
     reset_host_state(f);
     // buffers a fresh film starts with at zero and may read at cells outside
@@ -718,6 +763,9 @@ int cz_film_stats(cz_film* f, cz_stats* st) {
     st->act_cell_substeps = f->cnt_act_cell_substeps;
     st->frames_counted = f->cnt_frames;
     st->act_cell_implicit = f->cnt_act_cell_implicit;
+    st->one_launch_frames = f->cnt_one_frames;
+    st->one_launch_cells = f->cnt_one_cells;
+    st->one_launch_cell_substeps = f->cnt_one_cell_substeps;
     return CZ_OK;
 }
 
@@ -870,9 +918,22 @@ int cz_film_clear(cz_film* f) {
 
 void* cz_film_stream(cz_film* f) { return f ? (void*)f->stream : nullptr; }
 
+int cz_film_substep_launch(cz_film* f, int mode) {
+    if (!f || mode < -1 || mode > 1) return set_err(CZ_EINVAL, "substep_launch: -1, 0 or 1");
+    f->substep_mode = mode;
+    return CZ_OK;
+}
+
 int cz_film_flux_shape(cz_film* f, int shape) {
     if (!f || shape < -1 || shape > 1) return set_err(CZ_EINVAL, "flux_shape: -1, 0 or 1");
     f->flux_tuner.fixed = shape;
+    return CZ_OK;
+}
+
+int cz_film_graph_counts(const cz_film* f, uint64_t* captures, uint64_t* replays) {
+    if (!f || !captures || !replays) return set_err(CZ_EINVAL, "graph_counts: null argument");
+    *captures = f->graph_captures;
+    *replays = f->graph_replays;
     return CZ_OK;
 }
 
@@ -904,6 +965,7 @@ int cz_film_reset_counters(cz_film* f) {
     int rc = poll_stats(f);
     if (rc) return rc;
     f->cnt_act_cells = f->cnt_act_cell_substeps = f->cnt_frames = f->cnt_act_cell_implicit = 0;
+    f->cnt_one_frames = f->cnt_one_cells = f->cnt_one_cell_substeps = 0;
     return CZ_OK;
 }
 

@@ -27,12 +27,19 @@ int cz_set_err(int code, const char* fmt, ...);
     } while (0)
 
 struct GraphKeyStore {
-    unsigned char bytes[7 * 4 + sizeof(cz_params)];
+    unsigned char bytes[9 * 4 + sizeof(cz_params)];
 };
 
 namespace czi {
 
 constexpr uint32_t kMaxTileBlocks = 2048u;  // resident 256-thread blocks on 256 CUs
+// one-launch substeps (k_substeps): one block per expected active tile (a
+// power of two) up to kPersistBlocks.  The automatic choice takes it for
+// active sets of at most kPersistTiles tiles; 0 = never: on gfx950 a grid
+// barrier (agent-scope release, atomic arrival, acquire) costs about 9 us
+// against 3.3 us for a dependent launch inside a hipGraph, so the three
+// launches per substep are faster at every active-set size measured.
+constexpr uint32_t kPersistTiles = 0u, kPersistBlocks = 256u;
 
 struct ProfSlot {
     std::string label;
@@ -77,6 +84,13 @@ struct ShapeTuner {
         }
         return best;
     }
+    // true while a shape of this size class still lacks its warm-up samples
+    // (a graph replay cannot take them: such frames run uncaptured)
+    bool learning(uint64_t items) const {
+        if (fixed >= 0) return false;
+        const int b = bucket(items);
+        return cnt[b][0] < (uint32_t)WARM || cnt[b][1] < (uint32_t)WARM;
+    }
     void record(int bk, int shape, double us) {
         sum[bk][shape] += us;
         cnt[bk][shape] += 1;
@@ -106,10 +120,25 @@ struct cz_film {
     czi::ShapeTuner flux_tuner;
     std::vector<czi::TunerSlot> tuner_pending;
     bool capturing = false;
+    // substep launches: -1 chosen by the active-set size, 0 three launches per
+    // substep, 1 one launch for all substeps
+    int substep_mode = -1;
+    bool one_launch_now = false;  // set by record_frame for the frame it records
+    bool sync_fault = false;  // a grid barrier of k_substeps timed out
+    uint32_t persist_grid() const {
+        const uint32_t act = std::min(active_hint, nt16);
+        if (substep_mode == 0 || (substep_mode < 0 && (czi::kPersistTiles == 0u || act > czi::kPersistTiles)))
+            return 0u;
+        uint32_t p2 = 16u;  // power of two: few distinct grids, few distinct graphs
+        while (p2 < act) p2 <<= 1;
+        return std::max(1u, std::min(std::min(p2, czi::kPersistBlocks), nt16));
+    }
     double total_deposited = 0.0, weave_volume = 0.0;
     float cg_report[4] = {0, 0, 0, 0};
     bool stats_pending = false;
     uint64_t cnt_act_cells = 0, cnt_act_cell_substeps = 0, cnt_frames = 0, cnt_act_cell_implicit = 0;
+    uint64_t cnt_one_frames = 0, cnt_one_cells = 0, cnt_one_cell_substeps = 0;
+    bool one_launch_last = false;  // the frame being folded ran its substeps as one launch
     int device = 0;
     hipStream_t stream = nullptr;
     void* batch = nullptr;  // the cz_batch this film is a member of
@@ -134,9 +163,10 @@ struct cz_film {
     float* cg_s = nullptr;
     uint32_t *tile_flags = nullptr, *tile_list = nullptr, *tile_moved = nullptr, *active_count = nullptr;
     uint32_t *cure_list = nullptr, *cure_count = nullptr;
+    uint32_t* grid_sync = nullptr;
     float2* stats_out = nullptr;
     float* pool_out = nullptr;
-    float *tsum_a = nullptr, *tsum_b = nullptr;
+    float* tsum_a = nullptr;
     float* pigtab = nullptr;
     cz_deposit* deposits = nullptr;
     uint32_t *tile_start = nullptr, *tile_count = nullptr, *tile_items = nullptr;
@@ -149,8 +179,10 @@ struct cz_film {
     uint32_t *tstart_h = nullptr, *tcount_h = nullptr, *titems_h = nullptr;
     uint32_t deposit_count = 0;
     // profiling
-    bool use_graphs = false;
+    bool use_graphs = true;
     std::vector<std::pair<struct GraphKeyStore, hipGraphExec_t>> graphs;
+    std::vector<struct GraphKeyStore> graph_seen;  // shapes met once, not yet captured
+    uint64_t graph_captures = 0, graph_replays = 0;
     bool prof_on = false;
     std::string prof_only;
     std::vector<czi::ProfSlot> prof_pending;
@@ -177,7 +209,7 @@ struct cz_film {
         F.cg_part = cg_part; F.cg_part2 = cg_part2; F.cg_s = cg_s;
         F.tile_flags = tile_flags; F.tile_list = tile_list; F.tile_moved = tile_moved; F.active_count = active_count;
         F.cure_list = cure_list; F.cure_count = cure_count;
-        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.tsum_b = tsum_b; F.pigtab = pigtab;
+        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.pigtab = pigtab;
         F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
         F.frame_out = frame_out;
         return F;

@@ -114,6 +114,7 @@ struct BatchPlan {
 };
 
 __global__ void k_deposit(FilmDev F);
+__global__ void k_substeps(FilmDev Fa, FilmDev Fb, uint32_t substeps, int clear_first);
 __global__ void k_batch_plan(BatchPlan B);
 
 #define CZ_TILE_KERNELS(X)                                          \
@@ -129,8 +130,7 @@ __global__ void k_batch_plan(BatchPlan B);
     X(k_reduce_pool, (WS ws), (ws))                                 \
     X(k_strata_limit, (WS ws), (ws))                                \
     X(k_strata_update, (WS ws), (ws))                               \
-    X(k_strata_copy_back, (WS ws), (ws))                            \
-    X(k_strata_fill, (WS ws, int pass), (ws, pass))                 \
+    X(k_strata_fill, (WS ws), (ws))                                 \
     X(k_cg_setup, (WS ws), (ws))                                    \
     X(k_cg_lap, (WS ws, uint32_t which), (ws, which))               \
     X(k_cg_init, (WS ws), (ws))                                     \

@@ -496,225 +496,241 @@ struct LapWin {
     uint32_t tf[9];
 };
 
-template <class WS>
-__global__ __launch_bounds__(256) void k_lap_plug(WS ws, int do_nose, int do_plug, int do_clear) {
-    __shared__ LapWin sw;
-    __shared__ float s_q[WIN * WIN];
-    __shared__ float s_h[WIN * WIN];
-    __shared__ uint8_t s_y[WIN * WIN];
-    __shared__ uint32_t s_d[RING_SPAN * RING_SPAN];
-    __shared__ float2 s_fn[RING_SPAN * RING_SPAN];
-    __shared__ float s_us[RING_SPAN * RING_SPAN];
-    CZ_FOR_TILES(ws) {
-        CZ_TILE_CELL(F)
-        const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
-        const bool plug_on = do_plug && !(gb < 1e-6f);
-        const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
-        const int tx0 = ttx * 16, ty0 = tty * 16;
-        const int wx0 = tx0 - WIN_R, wy0 = ty0 - WIN_R;
-        const int W_ = (int)F.W, H_ = (int)F.H;
-        __syncthreads();
-        for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += 256u) {
-            const int gx = wx0 + (int)(q % WIN), gy = wy0 + (int)(q / WIN);
-            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
-                const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
-                sw.h[q] = F.h[nn];
-                sw.m[q] = F.m[nn];
-                sw.d[q] = F.delta[nn];
-                sw.rheo[q] = F.rheo[nn];
-                sw.fn[q] = F.fnorm[nn];
-            } else {
-                sw.h[q] = 0.0f;
-                sw.m[q] = 0.0f;
-                sw.d[q] = 0u;
-                sw.rheo[q] = make_float2(0.0f, 0.0f);
-                sw.fn[q] = make_float2(0.0f, 0.0f);
+struct LapShared {
+    LapWin sw;
+    float s_q[WIN * WIN];
+    float s_h[WIN * WIN];
+    uint8_t s_y[WIN * WIN];
+    uint32_t s_d[RING_SPAN * RING_SPAN];
+    float2 s_fn[RING_SPAN * RING_SPAN];
+    float s_us[RING_SPAN * RING_SPAN];
+};
+
+// One tile of the lap/plug pass (every thread of the block calls it).
+__device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, LapShared& S, int do_nose, int do_plug,
+                                              int do_clear) {
+    LapWin& sw = S.sw;
+    float* s_q = S.s_q;
+    float* s_h = S.s_h;
+    uint8_t* s_y = S.s_y;
+    uint32_t* s_d = S.s_d;
+    float2* s_fn = S.s_fn;
+    float* s_us = S.s_us;
+    const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;
+    CZ_TILE_CELL(F)
+    const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
+    const bool plug_on = do_plug && !(gb < 1e-6f);
+    const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+    const int tx0 = ttx * 16, ty0 = tty * 16;
+    const int wx0 = tx0 - WIN_R, wy0 = ty0 - WIN_R;
+    const int W_ = (int)F.W, H_ = (int)F.H;
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += 256u) {
+        const int gx = wx0 + (int)(q % WIN), gy = wy0 + (int)(q / WIN);
+        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+            const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+            sw.h[q] = F.h[nn];
+            sw.m[q] = F.m[nn];
+            sw.d[q] = F.delta[nn];
+            sw.rheo[q] = F.rheo[nn];
+            sw.fn[q] = F.fnorm[nn];
+        } else {
+            sw.h[q] = 0.0f;
+            sw.m[q] = 0.0f;
+            sw.d[q] = 0u;
+            sw.rheo[q] = make_float2(0.0f, 0.0f);
+            sw.fn[q] = make_float2(0.0f, 0.0f);
+        }
+    }
+    if (threadIdx.x < 9u) {
+        const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
+        sw.tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
+                                 ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
+    }
+    __syncthreads();
+    // window accessors (callers keep (gx, gy) inside the canvas)
+    auto wq = [&](int gx, int gy) -> int { return (gy - wy0) * WIN + (gx - wx0); };
+    auto in_win = [&](int gx, int gy) -> bool {
+        return gx >= wx0 && gy >= wy0 && gx < wx0 + WIN && gy < wy0 + WIN;
+    };
+    auto act_w = [&](int gx, int gy) -> bool {
+        return (sw.tf[((gy >> 4) - tty + 1) * 3 + ((gx >> 4) - ttx + 1)] & 32u) != 0u;
+    };
+    // nose (film.wgsl:680-723) of ring cell (i, j): offset to the nearest
+    // dry cell from the previous offsets and the heights after apply, and
+    // the front normal (film.wgsl:160-180) within 3 cells
+    auto nose_w = [&](int i, int j, uint32_t& d_out, float2& fn_out) {
+        d_out = 0u;
+        fn_out = make_float2(0.0f, 0.0f);
+        if (sw.h[wq(i, j)] <= 0.0f) return;
+        int bx = 64, by = 64;
+        float bd = 1e9f;
+        for (int dj = -1; dj <= 1; dj++) {
+            for (int di = -1; di <= 1; di++) {
+                const int xx = i + di, yy = j + dj;
+                if (xx < 0 || yy < 0 || xx >= W_ || yy >= H_) continue;
+                int cx = di, cy = dj;
+                if (sw.h[wq(xx, yy)] > 0.0f) {
+                    const uint32_t dv = sw.d[wq(xx, yy)];
+                    if ((dv & 0x10000u) == 0u) continue;
+                    int ox, oy;
+                    unpack_off(dv, ox, oy);
+                    cx = ox + di;
+                    cy = oy + dj;
+                    const int tx = i + cx, ty = j + cy;
+                    if (tx < 0 || ty < 0 || tx >= W_ || ty >= H_) continue;
+                    // both reads issued, then a select (no pointer select between spaces)
+                    const bool inside = in_win(tx, ty);
+                    const float hl = sw.h[inside ? wq(tx, ty) : 0];
+                    const float hg = F.h[idx(F, (uint32_t)tx, (uint32_t)ty)];
+                    const float ht = inside ? hl : hg;
+                    if (ht > 0.0f) continue;
+                } else if (di == 0 && dj == 0) {
+                    continue;
+                }
+                const float d = cz_len2((float)cx, (float)cy);
+                if (d < bd) { bd = d; bx = cx; by = cy; }
             }
         }
-        if (threadIdx.x < 9u) {
-            const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
-            sw.tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
-                                     ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
-        }
-        __syncthreads();
-        // window accessors (callers keep (gx, gy) inside the canvas)
-        auto wq = [&](int gx, int gy) -> int { return (gy - wy0) * WIN + (gx - wx0); };
-        auto in_win = [&](int gx, int gy) -> bool {
-            return gx >= wx0 && gy >= wy0 && gx < wx0 + WIN && gy < wy0 + WIN;
-        };
-        auto act_w = [&](int gx, int gy) -> bool {
-            return (sw.tf[((gy >> 4) - tty + 1) * 3 + ((gx >> 4) - ttx + 1)] & 32u) != 0u;
-        };
-        // nose (film.wgsl:680-723) of ring cell (i, j): offset to the nearest
-        // dry cell from the previous offsets and the heights after apply, and
-        // the front normal (film.wgsl:160-180) within 3 cells
-        auto nose_w = [&](int i, int j, uint32_t& d_out, float2& fn_out) {
-            d_out = 0u;
-            fn_out = make_float2(0.0f, 0.0f);
-            if (sw.h[wq(i, j)] <= 0.0f) return;
-            int bx = 64, by = 64;
-            float bd = 1e9f;
-            for (int dj = -1; dj <= 1; dj++) {
-                for (int di = -1; di <= 1; di++) {
+        if (bd > 1e8f) return;
+        d_out = pack_off(bx, by);
+        if (bd <= 3.0f) {
+            float dx = 0.0f, dy = 0.0f;
+            for (int dj = -2; dj <= 2; dj++) {
+                for (int di = -2; di <= 2; di++) {
+                    if (di == 0 && dj == 0) continue;
                     const int xx = i + di, yy = j + dj;
                     if (xx < 0 || yy < 0 || xx >= W_ || yy >= H_) continue;
-                    int cx = di, cy = dj;
-                    if (sw.h[wq(xx, yy)] > 0.0f) {
-                        const uint32_t dv = sw.d[wq(xx, yy)];
-                        if ((dv & 0x10000u) == 0u) continue;
-                        int ox, oy;
-                        unpack_off(dv, ox, oy);
-                        cx = ox + di;
-                        cy = oy + dj;
-                        const int tx = i + cx, ty = j + cy;
-                        if (tx < 0 || ty < 0 || tx >= W_ || ty >= H_) continue;
-                        // both reads issued, then a select (no pointer select between spaces)
-                        const bool inside = in_win(tx, ty);
-                        const float hl = sw.h[inside ? wq(tx, ty) : 0];
-                        const float hg = F.h[idx(F, (uint32_t)tx, (uint32_t)ty)];
-                        const float ht = inside ? hl : hg;
-                        if (ht > 0.0f) continue;
-                    } else if (di == 0 && dj == 0) {
-                        continue;
-                    }
-                    const float d = cz_len2((float)cx, (float)cy);
-                    if (d < bd) { bd = d; bx = cx; by = cy; }
-                }
-            }
-            if (bd > 1e8f) return;
-            d_out = pack_off(bx, by);
-            if (bd <= 3.0f) {
-                float dx = 0.0f, dy = 0.0f;
-                for (int dj = -2; dj <= 2; dj++) {
-                    for (int di = -2; di <= 2; di++) {
-                        if (di == 0 && dj == 0) continue;
-                        const int xx = i + di, yy = j + dj;
-                        if (xx < 0 || yy < 0 || xx >= W_ || yy >= H_) continue;
-                        if (di * di + dj * dj > 5) continue;
-                        if (sw.h[wq(xx, yy)] <= F.P.ein[3]) {
-                            dx = dx + FN_QUOT.v[di + 2][dj + 2];
-                            dy = dy + FN_QUOT.v[dj + 2][di + 2];
-                        }
+                    if (di * di + dj * dj > 5) continue;
+                    if (sw.h[wq(xx, yy)] <= F.P.ein[3]) {
+                        dx = dx + FN_QUOT.v[di + 2][dj + 2];
+                        dy = dy + FN_QUOT.v[dj + 2][di + 2];
                     }
                 }
-                const float l = cz_len2(dx, dy);
-                fn_out = (l < 1e-6f) ? make_float2(0.0f, 0.0f) : make_float2(dx / l, dy / l);
             }
-        };
-        for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
-            const int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
-            uint32_t dv = 0u;
-            float2 fn = make_float2(0.0f, 0.0f);
-            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
-                if (do_nose && act_w(gx, gy)) {
-                    nose_w(gx, gy, dv, fn);
-                } else {
-                    dv = sw.d[wq(gx, gy)];
-                    fn = sw.fn[wq(gx, gy)];
-                }
-            }
-            s_d[q] = dv;
-            s_fn[q] = fn;
-            // usurf (film.wgsl:147-152) of the ring cell, once for the nine
-            // laplacians that read it
-            float us = 0.0f;
-            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
-                const float hc = sw.h[wq(gx, gy)];
-                float hold = 0.0f;
-                if (hc > 0.0f) hold = hold_level(F, nose_dist_b(dv));
-                us = hc + sw.m[wq(gx, gy)] - hold;
-            }
-            s_us[q] = us;
+            const float l = cz_len2(dx, dy);
+            fn_out = (l < 1e-6f) ? make_float2(0.0f, 0.0f) : make_float2(dx / l, dy / l);
         }
-        if (plug_on) {
-            for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += 256u) {
-                float hn = sw.h[q], qv = 0.0f;
-                uint8_t yl = 0;
-                if (hn > 0.0f) {
-                    const float2 r = sw.rheo[q];
-                    yl = r.y > 0.0f ? 1 : 0;
-                    float oy_, oup_;
-                    qv = hb_flux(F.P.phys[2], gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
-                } else {
-                    hn = 0.0f;
-                }
-                s_q[q] = qv;
-                s_h[q] = hn;
-                s_y[q] = yl;
+    };
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
+        const int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
+        uint32_t dv = 0u;
+        float2 fn = make_float2(0.0f, 0.0f);
+        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+            if (do_nose && act_w(gx, gy)) {
+                nose_w(gx, gy, dv, fn);
+            } else {
+                dv = sw.d[wq(gx, gy)];
+                fn = sw.fn[wq(gx, gy)];
             }
         }
-        __syncthreads();
-        if (!in_grid) continue;
-        // clear_acc (film.wgsl:1427-1434) after the cure pass, with the first
-        // active-tile kernel of the frame: do_clear 1 = clear, 2 = clear when
-        // the canvas has no implicit step this frame (its implicit lap cleared)
-        const int i = (int)x, j = (int)y;
-        const float hv = sw.h[wq(i, j)];
-        if (do_clear == 1 || (do_clear == 2 && F.imp_steps == 0u)) {
-            F.acc[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            F.h_frame[c] = hv;
+        s_d[q] = dv;
+        s_fn[q] = fn;
+        // usurf (film.wgsl:147-152) of the ring cell, once for the nine
+        // laplacians that read it
+        float us = 0.0f;
+        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+            const float hc = sw.h[wq(gx, gy)];
+            float hold = 0.0f;
+            if (hc > 0.0f) hold = hold_level(F, nose_dist_b(dv));
+            us = hc + sw.m[wq(gx, gy)] - hold;
         }
-        // the heights the next apply's limiter reads for this cell: apply
-        // updates h while neighbouring tiles' blocks evaluate their rings
-        if (do_plug) F.h_sub[c] = hv;
-        const int own = (int)(_ly + 1u) * RING_SPAN + (int)(_lx + 1u);
-        if (do_nose) {
-            F.delta_out[c] = s_d[own];
-            F.fnorm[c] = s_fn[own];
-        }
-        // usurf at mirrored coordinates, from the ring values
-        auto usurf_l = [&](int ii, int jj) -> float {
-            const int ri = (int)refl_i(F, ii), rj = (int)refl_j(F, jj);
-            return s_us[(rj - ty0 + 1) * RING_SPAN + (ri - tx0 + 1)];
-        };
-        const float s0 = hv + sw.m[wq(i, j)];
-        float hold = 0.0f, pl = 0.0f;
-        const float dself = nose_dist_b(s_d[own]);
-        if (hv > 0.0f) hold = hold_level(F, dself);
-        const float u0 = s0 - hold;
-        const float ax = usurf_l(i - 1, j) + usurf_l(i + 1, j) + usurf_l(i, j - 1) + usurf_l(i, j + 1);
-        const float dg = usurf_l(i - 1, j - 1) + usurf_l(i + 1, j - 1) + usurf_l(i - 1, j + 1) + usurf_l(i + 1, j + 1);
-        const float l = (W_AXIS * ax + W_DIAG_LAP * dg - (4.0f * W_AXIS + 4.0f * W_DIAG_LAP) * u0) * F.P.inv_dx * F.P.inv_dx;
-        if (hv > 0.0f) {
-            const float d = dself;
-            if (d <= 3.0f) {
-                const float2 nn = s_fn[own];
-                if (nn.x * nn.x + nn.y * nn.y > 0.5f) {
-                    // line_pressure (film.wgsl:190-201) with the normals from LDS
-                    float lp = 0.0f;
-                    if (!(x < 1u || y < 1u || x + 1u >= F.W || y + 1u >= F.H)) {
-                        const float2 nl = s_fn[own - 1], nr = s_fn[own + 1];
-                        const float2 nd = s_fn[own - RING_SPAN], nu = s_fn[own + RING_SPAN];
-                        if (!(nl.x * nl.x + nl.y * nl.y < 0.5f || nr.x * nr.x + nr.y * nr.y < 0.5f ||
-                              nd.x * nd.x + nd.y * nd.y < 0.5f || nu.x * nu.x + nu.y * nu.y < 0.5f)) {
-                            const float curv = (nr.x - nl.x + nu.y - nd.y) * 0.5f * F.P.inv_dx;
-                            lp = F.P.phys[1] * curv * F.P.front2[2];
-                        }
-                    }
-                    pl = lp * cz_expf(-(d - 1.0f) * 0.7f);
-                }
-            }
-        }
-        F.pres[c] = make_float2(-F.P.phys[1] * l + disjoining(F, hv), F.P.phys[0] * F.P.g[2] * (s0 - hold) + pl);
-        if (!do_plug) continue;
-        float tv = 0.0f;
-        if (hv > 0.0f && plug_on) {
-            float sum = 0.0f, mass = 0.0f;
-            bool yielding = false;
-            for (int dy = 0; dy < 2 * PLUG_RADIUS + 1; dy++) {
-                for (int dxi = 0; dxi < 2 * PLUG_RADIUS + 1; dxi++) {
-                    const int q = (int)(_ly + dy) * PLUG_SPAN + (int)(_lx + dxi);
-                    const float hn = s_h[q];
-                    if (hn <= 0.0f) continue;
-                    if (s_y[q]) yielding = true;
-                    sum = sum + s_q[q];
-                    mass = mass + hn;
-                }
-            }
-            if (yielding && mass > 0.0f) tv = sum / mass;
-        }
-        F.tmp[c] = tv;
+        s_us[q] = us;
     }
+    if (plug_on) {
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += 256u) {
+            float hn = sw.h[q], qv = 0.0f;
+            uint8_t yl = 0;
+            if (hn > 0.0f) {
+                const float2 r = sw.rheo[q];
+                yl = r.y > 0.0f ? 1 : 0;
+                float oy_, oup_;
+                qv = hb_flux(F.P.phys[2], gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
+            } else {
+                hn = 0.0f;
+            }
+            s_q[q] = qv;
+            s_h[q] = hn;
+            s_y[q] = yl;
+        }
+    }
+    __syncthreads();
+    if (!in_grid) return;
+    // clear_acc (film.wgsl:1427-1434) after the cure pass, with the first
+    // active-tile kernel of the frame: do_clear 1 = clear, 2 = clear when
+    // the canvas has no implicit step this frame (its implicit lap cleared)
+    const int i = (int)x, j = (int)y;
+    const float hv = sw.h[wq(i, j)];
+    if (do_clear == 1 || (do_clear == 2 && F.imp_steps == 0u)) {
+        F.acc[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        F.h_frame[c] = hv;
+    }
+    // the heights the next apply's limiter reads for this cell: apply
+    // updates h while neighbouring tiles' blocks evaluate their rings
+    if (do_plug) F.h_sub[c] = hv;
+    const int own = (int)(_ly + 1u) * RING_SPAN + (int)(_lx + 1u);
+    if (do_nose) {
+        F.delta_out[c] = s_d[own];
+        F.fnorm[c] = s_fn[own];
+    }
+    // usurf at mirrored coordinates, from the ring values
+    auto usurf_l = [&](int ii, int jj) -> float {
+        const int ri = (int)refl_i(F, ii), rj = (int)refl_j(F, jj);
+        return s_us[(rj - ty0 + 1) * RING_SPAN + (ri - tx0 + 1)];
+    };
+    const float s0 = hv + sw.m[wq(i, j)];
+    float hold = 0.0f, pl = 0.0f;
+    const float dself = nose_dist_b(s_d[own]);
+    if (hv > 0.0f) hold = hold_level(F, dself);
+    const float u0 = s0 - hold;
+    const float ax = usurf_l(i - 1, j) + usurf_l(i + 1, j) + usurf_l(i, j - 1) + usurf_l(i, j + 1);
+    const float dg = usurf_l(i - 1, j - 1) + usurf_l(i + 1, j - 1) + usurf_l(i - 1, j + 1) + usurf_l(i + 1, j + 1);
+    const float l = (W_AXIS * ax + W_DIAG_LAP * dg - (4.0f * W_AXIS + 4.0f * W_DIAG_LAP) * u0) * F.P.inv_dx * F.P.inv_dx;
+    if (hv > 0.0f) {
+        const float d = dself;
+        if (d <= 3.0f) {
+            const float2 nn = s_fn[own];
+            if (nn.x * nn.x + nn.y * nn.y > 0.5f) {
+                // line_pressure (film.wgsl:190-201) with the normals from LDS
+                float lp = 0.0f;
+                if (!(x < 1u || y < 1u || x + 1u >= F.W || y + 1u >= F.H)) {
+                    const float2 nl = s_fn[own - 1], nr = s_fn[own + 1];
+                    const float2 nd = s_fn[own - RING_SPAN], nu = s_fn[own + RING_SPAN];
+                    if (!(nl.x * nl.x + nl.y * nl.y < 0.5f || nr.x * nr.x + nr.y * nr.y < 0.5f ||
+                          nd.x * nd.x + nd.y * nd.y < 0.5f || nu.x * nu.x + nu.y * nu.y < 0.5f)) {
+                        const float curv = (nr.x - nl.x + nu.y - nd.y) * 0.5f * F.P.inv_dx;
+                        lp = F.P.phys[1] * curv * F.P.front2[2];
+                    }
+                }
+                pl = lp * cz_expf(-(d - 1.0f) * 0.7f);
+            }
+        }
+    }
+    F.pres[c] = make_float2(-F.P.phys[1] * l + disjoining(F, hv), F.P.phys[0] * F.P.g[2] * (s0 - hold) + pl);
+    if (!do_plug) return;
+    float tv = 0.0f;
+    if (hv > 0.0f && plug_on) {
+        float sum = 0.0f, mass = 0.0f;
+        bool yielding = false;
+        for (int dy = 0; dy < 2 * PLUG_RADIUS + 1; dy++) {
+            for (int dxi = 0; dxi < 2 * PLUG_RADIUS + 1; dxi++) {
+                const int q = (int)(_ly + dy) * PLUG_SPAN + (int)(_lx + dxi);
+                const float hn = s_h[q];
+                if (hn <= 0.0f) continue;
+                if (s_y[q]) yielding = true;
+                sum = sum + s_q[q];
+                mass = mass + hn;
+            }
+        }
+        if (yielding && mass > 0.0f) tv = sum / mass;
+    }
+    F.tmp[c] = tv;
+}
+
+template <class WS>
+__global__ __launch_bounds__(256) void k_lap_plug(WS ws, int do_nose, int do_plug, int do_clear) {
+    __shared__ LapShared S;
+    CZ_FOR_TILES(ws) lap_plug_tile(F, _t, S, do_nose, do_plug, do_clear);
 }
 
 // fluxes (film.wgsl:375-455), face-parallel: each block takes one
@@ -806,53 +822,57 @@ struct WinRead {
 
 // fluxes (film.wgsl:375-455) of a tile, one cell's four faces per thread,
 // with the fields staged in LDS (the shape for large active sets).
+// One tile of the staged flux pass (every thread of the block calls it).
+__device__ __forceinline__ void flux_tile(const FilmDev& F, uint32_t _t, FluxWin& sw) {
+    const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;
+    CZ_TILE_CELL(F)
+    const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+    const int wx0 = ttx * 16 - FLUX_R, wy0 = tty * 16 - FLUX_R;
+    const int W_ = (int)F.W, H_ = (int)F.H;
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(FLUX_WIN * FLUX_WIN); q += 256u) {
+        const int gx = wx0 + (int)(q % FLUX_WIN), gy = wy0 + (int)(q / FLUX_WIN);
+        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+            const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+            sw.h[q] = F.h[nn];
+            sw.d[q] = F.delta[nn];
+            sw.rheo[q] = F.rheo[nn];
+            sw.fn[q] = F.fnorm[nn];
+            sw.pres[q] = F.pres[nn];
+            sw.tmp[q] = F.tmp[nn];
+            const float hv = sw.h[q];
+            sw.lmu[q] = cz_logf(cz_max(sw.rheo[q].x, 1e-4f));
+            sw.fd[q] = hv <= 0.0f ? 0.0f : nose_dist_b(sw.d[q]);
+        }
+    }
+    if (threadIdx.x < 9u) {
+        const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
+        sw.tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
+                                 ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
+    }
+    __syncthreads();
+    if (!in_grid) return;
+    const WinRead g{F, sw, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
+    const float2 p00 = pressure_at(F, g, (int)x, (int)y);
+    float4 qv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll 1
+    for (int k = 0; k < 4; k++) {
+        FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, true, p00);
+        if (!fg.has) continue;
+        FaceFlux ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+        const float q = ff.q * ((k < 2) ? W_AXIS : W_DIAG_FLUX);
+        if (k == 0) qv.x = q;
+        else if (k == 1) qv.y = q;
+        else if (k == 2) qv.z = q;
+        else qv.w = q;
+    }
+    F.flux[c] = qv;
+}
+
 template <class WS>
 __global__ __launch_bounds__(256) void k_fluxes(WS ws) {
     __shared__ FluxWin sw;
-    CZ_FOR_TILES(ws) {
-        CZ_TILE_CELL(F)
-        const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
-        const int wx0 = ttx * 16 - FLUX_R, wy0 = tty * 16 - FLUX_R;
-        const int W_ = (int)F.W, H_ = (int)F.H;
-        __syncthreads();
-        for (uint32_t q = threadIdx.x; q < (uint32_t)(FLUX_WIN * FLUX_WIN); q += 256u) {
-            const int gx = wx0 + (int)(q % FLUX_WIN), gy = wy0 + (int)(q / FLUX_WIN);
-            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
-                const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
-                sw.h[q] = F.h[nn];
-                sw.d[q] = F.delta[nn];
-                sw.rheo[q] = F.rheo[nn];
-                sw.fn[q] = F.fnorm[nn];
-                sw.pres[q] = F.pres[nn];
-                sw.tmp[q] = F.tmp[nn];
-                const float hv = sw.h[q];
-                sw.lmu[q] = cz_logf(cz_max(sw.rheo[q].x, 1e-4f));
-                sw.fd[q] = hv <= 0.0f ? 0.0f : nose_dist_b(sw.d[q]);
-            }
-        }
-        if (threadIdx.x < 9u) {
-            const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
-            sw.tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
-                                     ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
-        }
-        __syncthreads();
-        if (!in_grid) continue;
-        const WinRead g{F, sw, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
-        const float2 p00 = pressure_at(F, g, (int)x, (int)y);
-        float4 qv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll 1
-        for (int k = 0; k < 4; k++) {
-            FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, true, p00);
-            if (!fg.has) continue;
-            FaceFlux ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
-            const float q = ff.q * ((k < 2) ? W_AXIS : W_DIAG_FLUX);
-            if (k == 0) qv.x = q;
-            else if (k == 1) qv.y = q;
-            else if (k == 2) qv.z = q;
-            else qv.w = q;
-        }
-        F.flux[c] = qv;
-    }
+    CZ_FOR_TILES(ws) flux_tile(F, _t, sw);
 }
 
 struct Inflow { float best; uint32_t src; float div; float accv; };
@@ -889,108 +909,201 @@ __device__ inline void inherit_from(const FilmDev& F, uint32_t c, uint32_t src, 
 constexpr int AP_R = 2;
 constexpr int AP_WIN = 16 + 2 * AP_R;
 
+struct ApplyShared {
+    float4 s_f[AP_WIN * AP_WIN];
+    float s_hh[AP_WIN * AP_WIN];
+    float s_a[RING_SPAN * RING_SPAN];
+    uint32_t s_tf[9];
+};
+
+// One tile of the limiter + apply pass (every thread of the block calls it).
+__device__ __forceinline__ void apply_tile(const FilmDev& F, uint32_t _t, ApplyShared& S) {
+    float4* s_f = S.s_f;
+    float* s_hh = S.s_hh;
+    float* s_a = S.s_a;
+    uint32_t* s_tf = S.s_tf;
+    const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;
+    CZ_TILE_CELL(F)
+    const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+    const int tx0 = ttx * 16, ty0 = tty * 16;
+    const int wx0 = tx0 - AP_R, wy0 = ty0 - AP_R;
+    const int W_ = (int)F.W, H_ = (int)F.H;
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += 256u) {
+        const int gx = wx0 + (int)(q % AP_WIN), gy = wy0 + (int)(q / AP_WIN);
+        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+            const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+            s_f[q] = F.flux[nn];
+            s_hh[q] = F.h_sub[nn];
+        }
+    }
+    if (threadIdx.x < 9u) {
+        const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
+        s_tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
+                                ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
+    }
+    __syncthreads();
+    auto wq = [&](int gx, int gy) -> int { return (gy - wy0) * AP_WIN + (gx - wx0); };
+    // in the canvas and in an active tile (cells within 2 of the tile)
+    auto nb_ok = [&](int gi, int gj) -> bool {
+        if (gi < 0 || gj < 0 || gi >= W_ || gj >= H_) return false;
+        return (s_tf[((gj >> 4) - tty + 1) * 3 + ((gi >> 4) - ttx + 1)] & 32u) != 0u;
+    };
+    // limiter (film.wgsl:521-537) of cell (gi, gj)
+    auto limiter = [&](int gi, int gj) -> float {
+        const float4 f = s_f[wq(gi, gj)];
+        float out = cz_max(f.x, 0.0f) + cz_max(f.y, 0.0f) + cz_max(f.z, 0.0f) + cz_max(f.w, 0.0f);
+        if (nb_ok(gi - 1, gj)) out = out + cz_max(-s_f[wq(gi - 1, gj)].x, 0.0f);
+        if (nb_ok(gi, gj - 1)) out = out + cz_max(-s_f[wq(gi, gj - 1)].y, 0.0f);
+        if (nb_ok(gi - 1, gj - 1)) out = out + cz_max(-s_f[wq(gi - 1, gj - 1)].z, 0.0f);
+        if (nb_ok(gi - 1, gj + 1)) out = out + cz_max(-s_f[wq(gi - 1, gj + 1)].w, 0.0f);
+        float a = 1.0f;
+        if (out > 0.0f) a = cz_min(1.0f, 0.5f * s_hh[wq(gi, gj)] * F.P.dx / (F.P.dt * out));
+        return a;
+    };
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
+        const int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
+        s_a[q] = nb_ok(gx, gy) ? limiter(gx, gy) : 0.0f;
+    }
+    __syncthreads();
+    bool moved = false;
+    if (in_grid) {
+        const int i = (int)x, j = (int)y;
+        auto alpha_at = [&](int gi, int gj) -> float { return s_a[(gj - ty0 + 1) * RING_SPAN + (gi - tx0 + 1)]; };
+        const float ac = alpha_at(i, j);
+        Inflow st = {0.0f, c, 0.0f, 0.0f};
+        const float4 f = s_f[wq(i, j)];
+        const bool hx = x + 1u < F.W, hy = y + 1u < F.H, hm = y > 0u;
+        const int xp = (x + 1u < F.W - 1u) ? i + 1 : W_ - 1;
+        const int yp = (y + 1u < F.H - 1u) ? j + 1 : H_ - 1;
+        const int ym = hm ? j - 1 : 0;
+        float accv[4];
+        const float fq[4] = {f.x, f.y, f.z, f.w};
+        const bool has[4] = {hx, hy, hx && hy, hx && hm};
+        const int bi[4] = {hx ? xp : i, hx ? i : i, (hx && hy) ? xp : i, (hx && hm) ? xp : i};
+        const int bj[4] = {j, hy ? yp : j, (hx && hy) ? yp : j, (hx && hm) ? ym : j};
+        // own_face (film.wgsl:546-554)
+        for (int k = 0; k < 4; k++) {
+            const float q = fq[k];
+            float ab = ac;
+            if (q < 0.0f && has[k]) {
+                ab = alpha_at(bi[k], bj[k]);
+                if (-q * ab > st.best) { st.best = -q * ab; st.src = idx(F, (uint32_t)bi[k], (uint32_t)bj[k]); }
+            }
+            st.div = st.div + q * ab;
+            accv[k] = q * ab;
+        }
+        // nb_face (film.wgsl:556-561)
+        const int ni[4] = {i - 1, i, i - 1, i - 1};
+        const int nj[4] = {j, j - 1, j - 1, j + 1};
+        for (int k = 0; k < 4; k++) {
+            if (!nb_ok(ni[k], nj[k])) continue;
+            const float4 fn = s_f[wq(ni[k], nj[k])];
+            const float q = (k == 0) ? fn.x : (k == 1) ? fn.y : (k == 2) ? fn.z : fn.w;
+            float an = alpha_at(ni[k], nj[k]);
+            if (q < 0.0f) an = ac;
+            if (q * an > st.best) { st.best = q * an; st.src = idx(F, (uint32_t)ni[k], (uint32_t)nj[k]); }
+            st.div = st.div - q * an;
+        }
+        const uint32_t src = st.src;
+        const float h0 = s_hh[wq(i, j)];
+        float hn = cz_max(h0 - F.P.dt * st.div * F.P.inv_dx, 0.0f);
+        if (hn < h0 && hn < DRAINED_FILM) hn = 0.0f;
+        F.h[c] = hn;
+        moved = cz_abs(hn - h0) > MOVE_EPS;
+        if (h0 <= 0.0f && src != c) inherit_from(F, c, src, hn, true);
+        const float4 a = F.acc[c];
+        F.acc[c] = make_float4(a.x + accv[0] * F.P.dt, a.y + accv[1] * F.P.dt, a.z + accv[2] * F.P.dt,
+                               a.w + accv[3] * F.P.dt);
+    }
+    if (__syncthreads_or(moved ? 1 : 0) && threadIdx.x == 0) F.tile_moved[_t] = 1u;
+}
+
 template <class WS>
 __global__ __launch_bounds__(256) void k_limit_apply(WS ws) {
-    __shared__ float4 s_f[AP_WIN * AP_WIN];
-    __shared__ float s_hh[AP_WIN * AP_WIN];
-    __shared__ float s_a[RING_SPAN * RING_SPAN];
-    __shared__ uint32_t s_tf[9];
-    CZ_FOR_TILES(ws) {
-        CZ_TILE_CELL(F)
-        const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
-        const int tx0 = ttx * 16, ty0 = tty * 16;
-        const int wx0 = tx0 - AP_R, wy0 = ty0 - AP_R;
-        const int W_ = (int)F.W, H_ = (int)F.H;
-        __syncthreads();
-        for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += 256u) {
-            const int gx = wx0 + (int)(q % AP_WIN), gy = wy0 + (int)(q / AP_WIN);
-            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
-                const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
-                s_f[q] = F.flux[nn];
-                s_hh[q] = F.h_sub[nn];
-            }
-        }
-        if (threadIdx.x < 9u) {
-            const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
-            s_tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
-                                    ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
-        }
-        __syncthreads();
-        auto wq = [&](int gx, int gy) -> int { return (gy - wy0) * AP_WIN + (gx - wx0); };
-        // in the canvas and in an active tile (cells within 2 of the tile)
-        auto nb_ok = [&](int gi, int gj) -> bool {
-            if (gi < 0 || gj < 0 || gi >= W_ || gj >= H_) return false;
-            return (s_tf[((gj >> 4) - tty + 1) * 3 + ((gi >> 4) - ttx + 1)] & 32u) != 0u;
-        };
-        // limiter (film.wgsl:521-537) of cell (gi, gj)
-        auto limiter = [&](int gi, int gj) -> float {
-            const float4 f = s_f[wq(gi, gj)];
-            float out = cz_max(f.x, 0.0f) + cz_max(f.y, 0.0f) + cz_max(f.z, 0.0f) + cz_max(f.w, 0.0f);
-            if (nb_ok(gi - 1, gj)) out = out + cz_max(-s_f[wq(gi - 1, gj)].x, 0.0f);
-            if (nb_ok(gi, gj - 1)) out = out + cz_max(-s_f[wq(gi, gj - 1)].y, 0.0f);
-            if (nb_ok(gi - 1, gj - 1)) out = out + cz_max(-s_f[wq(gi - 1, gj - 1)].z, 0.0f);
-            if (nb_ok(gi - 1, gj + 1)) out = out + cz_max(-s_f[wq(gi - 1, gj + 1)].w, 0.0f);
-            float a = 1.0f;
-            if (out > 0.0f) a = cz_min(1.0f, 0.5f * s_hh[wq(gi, gj)] * F.P.dx / (F.P.dt * out));
-            return a;
-        };
-        for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
-            const int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
-            s_a[q] = nb_ok(gx, gy) ? limiter(gx, gy) : 0.0f;
-        }
-        __syncthreads();
-        bool moved = false;
-        if (in_grid) {
-            const int i = (int)x, j = (int)y;
-            auto alpha_at = [&](int gi, int gj) -> float { return s_a[(gj - ty0 + 1) * RING_SPAN + (gi - tx0 + 1)]; };
-            const float ac = alpha_at(i, j);
-            Inflow st = {0.0f, c, 0.0f, 0.0f};
-            const float4 f = s_f[wq(i, j)];
-            const bool hx = x + 1u < F.W, hy = y + 1u < F.H, hm = y > 0u;
-            const int xp = (x + 1u < F.W - 1u) ? i + 1 : W_ - 1;
-            const int yp = (y + 1u < F.H - 1u) ? j + 1 : H_ - 1;
-            const int ym = hm ? j - 1 : 0;
-            float accv[4];
-            const float fq[4] = {f.x, f.y, f.z, f.w};
-            const bool has[4] = {hx, hy, hx && hy, hx && hm};
-            const int bi[4] = {hx ? xp : i, hx ? i : i, (hx && hy) ? xp : i, (hx && hm) ? xp : i};
-            const int bj[4] = {j, hy ? yp : j, (hx && hy) ? yp : j, (hx && hm) ? ym : j};
-            // own_face (film.wgsl:546-554)
-            for (int k = 0; k < 4; k++) {
-                const float q = fq[k];
-                float ab = ac;
-                if (q < 0.0f && has[k]) {
-                    ab = alpha_at(bi[k], bj[k]);
-                    if (-q * ab > st.best) { st.best = -q * ab; st.src = idx(F, (uint32_t)bi[k], (uint32_t)bj[k]); }
+    __shared__ ApplyShared S;
+    CZ_FOR_TILES(ws) apply_tile(F, _t, S);
+}
+
+// ---------------------------------------------------------------------------
+// All substeps of a frame in one launch, for small active sets, where the
+// three launches per substep would cost more in dispatch gaps than in work:
+// the lap/plug, flux and limiter+apply passes of every substep run over the
+// active list in turn with a grid-wide barrier between passes (the three
+// global synchronisations of a substep), then the nose of the last substep.
+// Each pass calls the tile function of its own kernel, so the results are
+// those of the separate launches.  Fa and Fb are the film with the two
+// offset buffers in either order: the nose of substeps 1.. writes the other
+// buffer, which the rest of the substep reads.
+//
+// The grid is small (at most a few hundred blocks of 256 threads, resident
+// together on the 256 CUs).  A barrier's wait is bounded: past the spin
+// limit the block sets the timeout word and stops waiting, every later wait
+// returns at once, k_frame_stats reports the word to the host, and the frame
+// is reported as failed.
+struct SubstepShared {
+    union {
+        LapShared lap;
+        FluxWin flux;
+        ApplyShared ap;
+    };
+};
+
+constexpr uint32_t kBarrierSpins = 1u << 20;
+
+// Block arrival is counted with relaxed atomics after a release fence; the
+// last block to arrive resets the count and publishes the next generation;
+// the others poll the generation with relaxed loads (an acquire per poll
+// would invalidate the L2 on every iteration) and take one acquire fence
+// when it changes.
+__device__ __forceinline__ void grid_barrier(uint32_t* bar, uint32_t nblocks) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived + 1u == nblocks) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0u;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (__hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+                if (++spins > kBarrierSpins) {
+                    __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
                 }
-                st.div = st.div + q * ab;
-                accv[k] = q * ab;
+                __builtin_amdgcn_s_sleep(1);
             }
-            // nb_face (film.wgsl:556-561)
-            const int ni[4] = {i - 1, i, i - 1, i - 1};
-            const int nj[4] = {j, j - 1, j - 1, j + 1};
-            for (int k = 0; k < 4; k++) {
-                if (!nb_ok(ni[k], nj[k])) continue;
-                const float4 fn = s_f[wq(ni[k], nj[k])];
-                const float q = (k == 0) ? fn.x : (k == 1) ? fn.y : (k == 2) ? fn.z : fn.w;
-                float an = alpha_at(ni[k], nj[k]);
-                if (q < 0.0f) an = ac;
-                if (q * an > st.best) { st.best = q * an; st.src = idx(F, (uint32_t)ni[k], (uint32_t)nj[k]); }
-                st.div = st.div - q * an;
-            }
-            const uint32_t src = st.src;
-            const float h0 = s_hh[wq(i, j)];
-            float hn = cz_max(h0 - F.P.dt * st.div * F.P.inv_dx, 0.0f);
-            if (hn < h0 && hn < DRAINED_FILM) hn = 0.0f;
-            F.h[c] = hn;
-            moved = cz_abs(hn - h0) > MOVE_EPS;
-            if (h0 <= 0.0f && src != c) inherit_from(F, c, src, hn, true);
-            const float4 a = F.acc[c];
-            F.acc[c] = make_float4(a.x + accv[0] * F.P.dt, a.y + accv[1] * F.P.dt, a.z + accv[2] * F.P.dt,
-                                   a.w + accv[3] * F.P.dt);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        if (__syncthreads_or(moved ? 1 : 0) && threadIdx.x == 0) F.tile_moved[_t] = 1u;
     }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_substeps(FilmDev Fa, FilmDev Fb, uint32_t substeps, int clear_first) {
+    __shared__ SubstepShared S;
+    const uint32_t n = *Fa.active_count;
+    const uint32_t G = gridDim.x;
+    uint32_t* bar = Fa.grid_sync;
+    bool flip = false;
+    for (uint32_t s = 0; s < substeps; s++) {
+        {
+            const FilmDev& F = flip ? Fb : Fa;
+            for (uint32_t a = blockIdx.x; a < n; a += G)
+                lap_plug_tile(F, F.tile_list[a], S.lap, s > 0 ? 1 : 0, 1, (s == 0 && clear_first) ? 1 : 0);
+        }
+        grid_barrier(bar, G);
+        if (s > 0) flip = !flip;
+        const FilmDev& F = flip ? Fb : Fa;
+        for (uint32_t a = blockIdx.x; a < n; a += G) flux_tile(F, F.tile_list[a], S.flux);
+        grid_barrier(bar, G);
+        for (uint32_t a = blockIdx.x; a < n; a += G) apply_tile(F, F.tile_list[a], S.ap);
+        grid_barrier(bar, G);
+    }
+    const FilmDev& F = flip ? Fb : Fa;
+    for (uint32_t a = blockIdx.x; a < n; a += G) lap_plug_tile(F, F.tile_list[a], S.lap, 1, 0, 0);
 }
 
 // Copy the offsets of active cells from the buffer written last to the other
@@ -1171,6 +1284,7 @@ __global__ __launch_bounds__(256) void k_frame_stats(CS cs) {
         F.frame_out[2] = cz_bits2f(*F.active_count);
         F.frame_out[3] = cz_bits2f(s_cells);
         for (int k = 0; k < 4; k++) F.frame_out[4 + k] = F.cg_s[4 + k];
+        F.frame_out[15] = cz_bits2f(F.grid_sync ? F.grid_sync[2] : 0u);
     }
 }
 
@@ -1337,57 +1451,126 @@ __global__ __launch_bounds__(256) void k_strata_update(WS ws) {
 
 // strata_copy_back (strata.wgsl:276-286): the pigment words of a tile row are
 // one 2 KiB run, copied by consecutive threads.
+// strata_copy_back + the six strata_fill passes (strata.wgsl:276-323) in one
+// launch per active tile.
+//
+// A fill pass gives a wet cell of an active tile whose stack is empty the top
+// stratum of the neighbour (x-1, y-1, x+1, y+1 order, strictly largest stack
+// sum) with a non-empty stack, as one stratum of the cell's height; the
+// passes are Jacobi steps (each reads the sums before it).  A cell filled in
+// pass p holds exactly what its source held in its top stratum, so a cell
+// filled from a filled cell ends with the stratum of the first cell of the
+// chain (its root), which no pass writes.  The block runs the six passes on
+// the sums of the tile and a 6-cell ring in LDS (a pass moves information one
+// cell, so the tile's results are exact), tracking each cell's root, and then
+// writes the tile: filled cells from their root (the post-update buffers if
+// the root's tile is active: what copy_back would have put there), every
+// other cell by the copy_back of the post-update stack.
+constexpr int FL_R = 6;
+constexpr int FL_WIN = 16 + 2 * FL_R;
+constexpr int FL_CELLS = FL_WIN * FL_WIN;
+
 template <class WS>
-__global__ __launch_bounds__(256) void k_strata_copy_back(WS ws) {
+__global__ __launch_bounds__(256) void k_strata_fill(WS ws) {
+    __shared__ float s_t[2][FL_CELLS];       // stack sums, ping-pong over the passes
+    __shared__ uint16_t s_root[2][FL_CELLS]; // window index of the cell's root
+    __shared__ float s_h[FL_CELLS];
+    __shared__ uint8_t s_kind[FL_CELLS];     // 1 in canvas, 2 active tile
     CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
-        if (in_grid) {
-            F.strata_h[c] = F.strata_h2[c];
-            F.strata_cure[c] = F.strata_cure2[c];
+        const int wx0 = (int)((_t % F.tw) * 16u) - FL_R, wy0 = (int)((_t / F.tw) * 16u) - FL_R;
+        __syncthreads();  // the previous tile's window is no longer read
+        for (int q = (int)threadIdx.x; q < FL_CELLS; q += 256) {
+            const int gx = wx0 + q % FL_WIN, gy = wy0 + q / FL_WIN;
+            float t = 0.0f, hv = 0.0f;
+            uint8_t kind = 0u;
+            if (gx >= 0 && gy >= 0 && gx < (int)F.W && gy < (int)F.H) {
+                const uint32_t cg = idx(F, (uint32_t)gx, (uint32_t)gy);
+                kind = 1u;
+                if (cell_active(F, (uint32_t)gx, (uint32_t)gy)) {
+                    kind |= 2u;
+                    t = F.tsum_a[cg];
+                } else {
+                    const float4 v = F.strata_h[cg];
+                    t = v.x + v.y + v.z + v.w;
+                }
+                hv = F.h[cg];
+            }
+            s_t[0][q] = t;
+            s_root[0][q] = (uint16_t)q;
+            s_h[q] = hv;
+            s_kind[q] = kind;
         }
+        __syncthreads();
+#pragma unroll 1
+        for (int pass = 0; pass < 6; pass++) {
+            const int in = pass & 1, out = in ^ 1;
+            for (int q = (int)threadIdx.x; q < FL_CELLS; q += 256) {
+                float own = s_t[in][q];
+                uint16_t root = s_root[in][q];
+                const float hv = s_h[q];
+                if (s_kind[q] == 3u && !(hv <= 0.0f) && !(own > 0.0f)) {
+                    const int qx = q % FL_WIN, gx = wx0 + qx, gy = wy0 + q / FL_WIN;
+                    int src = -1;
+                    float best = 0.0f;
+                    // neighbours outside the window only reach cells within
+                    // 6 of its edge, whose results the tile never uses
+                    if (gx > 0 && qx > 0) { const float v = s_t[in][q - 1]; if (v > best) { best = v; src = q - 1; } }
+                    if (gy > 0 && q >= FL_WIN) { const float v = s_t[in][q - FL_WIN]; if (v > best) { best = v; src = q - FL_WIN; } }
+                    if (gx + 1 < (int)F.W && qx + 1 < FL_WIN) { const float v = s_t[in][q + 1]; if (v > best) { best = v; src = q + 1; } }
+                    if (gy + 1 < (int)F.H && q + FL_WIN < FL_CELLS) { const float v = s_t[in][q + FL_WIN]; if (v > best) { best = v; src = q + FL_WIN; } }
+                    if (src >= 0) {
+                        own = hv;
+                        root = s_root[in][src];
+                    }
+                }
+                s_t[out][q] = own;
+                s_root[out][q] = root;
+            }
+            __syncthreads();
+        }
+        // six passes end in buffer 0
+        const int qc = (_ly + FL_R) * FL_WIN + (_lx + FL_R);
+        const int rq = s_root[0][qc];
+        if (in_grid) {
+            if (rq != qc) {
+                const uint32_t r = idx(F, (uint32_t)(wx0 + rq % FL_WIN), (uint32_t)(wy0 + rq / FL_WIN));
+                const bool ra = (s_kind[rq] & 2u) != 0u;
+                const float4 ts = ra ? F.strata_h2[r] : F.strata_h[r];
+                uint32_t ks = 0;
+                if (ts.x > 0.0f) ks = 0;
+                if (ts.y > 0.0f) ks = 1;
+                if (ts.z > 0.0f) ks = 2;
+                if (ts.w > 0.0f) ks = 3;
+                const float4 cs4 = ra ? F.strata_cure2[r] : F.strata_cure[r];
+                const float cs = ks == 0 ? cs4.x : (ks == 1 ? cs4.y : (ks == 2 ? cs4.z : cs4.w));
+                const uint4* pg = ra ? F.strata_pig2 : F.strata_pig;
+                const uint4 a = pg[(r * 4u + ks) * 2u];
+                const uint4 b = pg[(r * 4u + ks) * 2u + 1u];
+                F.strata_h[c] = make_float4(s_h[qc], 0.0f, 0.0f, 0.0f);
+                F.strata_cure[c] = make_float4(cs, 0.0f, 0.0f, 0.0f);
+                F.strata_pig[c * 8u] = a;
+                F.strata_pig[c * 8u + 1u] = b;
+                const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+                for (uint32_t k = 2u; k < 8u; k++) F.strata_pig[c * 8u + k] = z;
+            } else {
+                F.strata_h[c] = F.strata_h2[c];
+                F.strata_cure[c] = F.strata_cure2[c];
+            }
+        }
+        // copy_back of the pigment words of the cells not filled: a tile row
+        // is one 2 KiB run, copied by consecutive threads
         const uint32_t x0 = (_t % F.tw) * 16u, y0 = (_t / F.tw) * 16u;
         for (uint32_t k = 0; k < 8u; k++) {
-            uint32_t q = k * 256u + threadIdx.x;
-            uint32_t row = q >> 7, off = q & 127u;
-            uint32_t xx = x0 + (off >> 3), yy = y0 + row;
-            if (xx < F.W && yy < F.H) {
-                uint32_t base = idx(F, x0, yy) * 8u + off;
+            const uint32_t q = k * 256u + threadIdx.x;
+            const uint32_t row = q >> 7, off = q & 127u;
+            const uint32_t xx = x0 + (off >> 3), yy = y0 + row;
+            const int wq = ((int)row + FL_R) * FL_WIN + (int)(off >> 3) + FL_R;
+            if (xx < F.W && yy < F.H && s_root[0][wq] == (uint16_t)wq) {
+                const uint32_t base = idx(F, x0, yy) * 8u + off;
                 F.strata_pig[base] = F.strata_pig2[base];
             }
         }
-    }
-}
-
-// strata_fill (strata.wgsl:288-323), one pass: the stack sums of active cells
-// come from tin (values before the pass), those of inactive cells from
-// strata_h (not written by the pass); tout receives the sums after the pass.
-__device__ __forceinline__ float tsum_of(const FilmDev& F, const float* tin, uint32_t n, uint32_t xi, uint32_t yj) {
-    if (cell_active(F, xi, yj)) return tin[n];
-    float4 t = F.strata_h[n];
-    return t.x + t.y + t.z + t.w;
-}
-
-template <class WS>
-__global__ __launch_bounds__(256) void k_strata_fill(WS ws, int pass) {
-    CZ_FOR_TILES(ws) {
-        CZ_TILE_CELL(F)
-        const float* tin = (pass & 1) ? F.tsum_b : F.tsum_a;
-        float* tout = (pass & 1) ? F.tsum_a : F.tsum_b;
-        if (!in_grid) continue;
-        float own = tin[c];
-        tout[c] = own;
-        float hc = F.h[c];
-        if (hc <= 0.0f) continue;
-        if (own > 0.0f) continue;
-        uint32_t src = c;
-        float best = 0.0f;
-        if (x > 0u) { float v = tsum_of(F, tin, c - 1u, x - 1u, y); if (v > best) { best = v; src = c - 1u; } }
-        if (y > 0u) { float v = tsum_of(F, tin, c - F.W, x, y - 1u); if (v > best) { best = v; src = c - F.W; } }
-        if (x + 1u < F.W) { float v = tsum_of(F, tin, c + 1u, x + 1u, y); if (v > best) { best = v; src = c + 1u; } }
-        if (y + 1u < F.H) { float v = tsum_of(F, tin, c + F.W, x, y + 1u); if (v > best) { best = v; src = c + F.W; } }
-        if (src == c) continue;
-        inherit_from(F, c, src, hc, false);
-        tout[c] = hc;
     }
 }
 
@@ -1838,8 +2021,7 @@ CZ_INST2(k_delta_sync)
 CZ_INST2(k_strata_split)
 CZ_INST2(k_strata_limit)
 CZ_INST2(k_strata_update)
-CZ_INST2(k_strata_copy_back)
-CZ_INST2(k_strata_fill, int)
+CZ_INST2(k_strata_fill)
 CZ_INST2(k_cg_setup)
 CZ_INST2(k_cg_lap, uint32_t)
 CZ_INST2(k_cg_init)

@@ -41,7 +41,9 @@ class Stats(C.Structure):
     _fields_ = [("h_max", f32), ("mu_min", f32), ("active_tiles", u32), ("substeps_last", u32),
                 ("implicit_steps", u32), ("frame", u32), ("cg_report", f32 * 4), ("total_deposited", C.c_double),
                 ("weave_volume", C.c_double), ("act_cells", C.c_uint64), ("act_cell_substeps", C.c_uint64),
-                ("frames_counted", C.c_uint64), ("act_cell_implicit", C.c_uint64)]
+                ("frames_counted", C.c_uint64), ("act_cell_implicit", C.c_uint64),
+                ("one_launch_frames", C.c_uint64), ("one_launch_cells", C.c_uint64),
+                ("one_launch_cell_substeps", C.c_uint64)]
 
 
 class Scene(C.Structure):

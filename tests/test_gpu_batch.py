@@ -90,12 +90,15 @@ def test_batch_graphs_equal_direct(gpu):
     direct = [make(k) for k in (0, 1, 3)]
     graph = [make(k) for k in (0, 1, 3)]
     bd, bg = czf.Batch(4), czf.Batch(4)
+    bd.use_graphs(False)
     bg.use_graphs(True)
+    for b in (bd, bg):
+        b.flux_shape(0)  # no tuner frames: every repeated shape is replayed
     for f in direct:
         bd.add(f)
     for f in graph:
         bg.add(f)
-    for frame in range(10):
+    for frame in range(24):
         bd.step()
         bg.step()
         if frame % 3 == 2:
@@ -107,6 +110,9 @@ def test_batch_graphs_equal_direct(gpu):
     bg.sync()
     for a, b in zip(direct, graph):
         assert_same(a, b, "graph end")
+    caps, reps = bg.graph_counts()
+    assert caps > 0 and reps > 0, (caps, reps)
+    assert bd.graph_counts() == (0, 0)
 
 
 def test_member_calls_are_refused(gpu):

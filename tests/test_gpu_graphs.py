@@ -24,10 +24,13 @@ def test_graph_replay_equals_direct(gpu):
     st = scenes.blobs_state(w, h, seed=5, nblobs=8, sigma_cells=(2.0, 8.0))
     direct = czf.Film(w, h, s)
     graph = czf.Film(w, h, s)
-    czf.check(czf.lib().cz_film_use_graphs(graph._p, 1))
+    direct.use_graphs(False)
+    graph.use_graphs(True)
+    for f in (direct, graph):
+        f.flux_shape(0)  # no tuner frames: every repeated shape is replayed
     assert direct.write_state_arrays(st) and graph.write_state_arrays(st)
     pig = scenes.jet_paint()
-    for f in range(24):
+    for f in range(48):
         deps = []
         if f % 5 == 2:
             x = (0.2 + 0.02 * f) * w * direct.dx
@@ -43,6 +46,9 @@ def test_graph_replay_equals_direct(gpu):
     direct.sync()
     graph.sync()
     assert_same(direct, graph, "end")
+    caps, reps = graph.graph_counts()
+    assert caps > 0 and reps > caps, (caps, reps)
+    assert direct.graph_counts() == (0, 0)
 
 
 @pytest.mark.parametrize("batched", [False, True])
