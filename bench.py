@@ -236,8 +236,8 @@ def bench_serve(args):
     return out
 
 
-FRAME_FAMILIES = SUBSTEP_FAMILIES + ("substeps", "lap", "mark", "compact", "cure", "deposit", "split", "reduce_pool",
-                                     "strata_limit", "strata_update", "copy_back", "strata_fill", "delta_sync",
+FRAME_FAMILIES = SUBSTEP_FAMILIES + ("substeps", "lap", "mark", "compact", "compact_list", "cure", "deposit", "split", "reduce_pool",
+                                     "strata_limit", "strata_update", "strata_fill",
                                      "frame_stats", "cg_setup", "cg_lap", "cg_init", "cg_iter_a", "cg_iter_b",
                                      "cg_iter_c", "cg_reduce", "cg_finish", "cg_rescale")
 

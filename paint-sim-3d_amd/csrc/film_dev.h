@@ -84,6 +84,8 @@ struct FilmDev {
     const uint32_t* tile_items;
     float* frame_out;     // [16] per-frame scalars read by the host
     uint32_t* grid_sync;  // [4] barrier of the one-launch substeps: count, generation, timeout
+    uint8_t* tile_bits;   // [nt16] compaction: 1 active, 2 cure
+    uint32_t* compact_cnt; // [2 per 256 tiles] compaction: active and cure counts per block
 };
 
 __device__ __forceinline__ uint32_t idx(const FilmDev& F, uint32_t i, uint32_t j) { return j * F.W + i; }

@@ -246,7 +246,8 @@ int record_frame(cz_film* f, uint32_t substeps) {
         LAUNCH("deposit", k_deposit, gcell, b256, F);
     }
     LAUNCH("mark", k_mark<OneFilmAll>, dim3((f->nt16 + 3u) / 4u), b256, OneFilmAll{F, f->nt16});
-    LAUNCH("compact", k_compact_tiles<OneCanvas>, dim3(1), dim3(1024), OneCanvas{F});
+    LAUNCH("compact", k_compact_bits, dim3((f->nt16 + 255u) / 256u), b256, F);
+    LAUNCH("compact_list", k_compact_list, dim3((f->nt16 + 255u) / 256u), b256, F);
     LAUNCH("cure", k_cure<OneFilmCure>, dim3(std::min(f->nt16, czi::kMaxTileBlocks)), b256, OneFilmCure{F});
     const bool implicit = f->s.implicit_hydro && f->implicit_steps > 0;
     if (implicit) {
@@ -317,7 +318,6 @@ int record_frame(cz_film* f, uint32_t substeps) {
         f->delta_cur_a = !f->delta_cur_a;
     }
     const OneFilm w{f->dev()};
-    LAUNCH("delta_sync", k_delta_sync<OneFilm>, gt, b256, w);
     LAUNCH("split", k_strata_split<OneFilm>, gt, b256, w);
     LAUNCH("reduce_pool", k_reduce_pool<OneFilmAll>, dim3(n64), b256, OneFilmAll{w.F, n64});
     LAUNCH("strata_limit", k_strata_limit<OneFilm>, gt, b256, w);
@@ -580,7 +580,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     A(acc_s, 4 * n); A(alpha_s, n); A(strata_h2, n); A(strata_pig2, 8 * n); A(strata_cure2, n); A(h_frame, n);
     A(fnorm, n); A(cg_m, n); A(cg_d, n); A(cg_x, n); A(cg_r, n); A(cg_p, n); A(cg_q, n); A(cg_t, n); A(cg_t2, n);
     A(cg_part, nt16); A(cg_part2, nt16); A(cg_s, 8); A(tile_flags, nt16); A(tile_list, nt16); A(tile_moved, nt16);
-    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
+    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(tile_bits, nt16); A(compact_cnt, 2 * ((nt16 + 255) / 256) + 2); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
     A(pigtab, CZ_NPIG * CZ_PIG_STRIDE); A(deposits, CZ_MAX_DEPOSITS); A(tile_start, nt64); A(tile_count, nt64);
     A(tile_items, CZ_MAX_TILE_ITEMS); A(frame_out, 16);
 #undef A

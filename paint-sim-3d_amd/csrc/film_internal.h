@@ -164,6 +164,8 @@ struct cz_film {
     uint32_t *tile_flags = nullptr, *tile_list = nullptr, *tile_moved = nullptr, *active_count = nullptr;
     uint32_t *cure_list = nullptr, *cure_count = nullptr;
     uint32_t* grid_sync = nullptr;
+    uint8_t* tile_bits = nullptr;
+    uint32_t* compact_cnt = nullptr;
     float2* stats_out = nullptr;
     float* pool_out = nullptr;
     float* tsum_a = nullptr;
@@ -209,7 +211,7 @@ struct cz_film {
         F.cg_part = cg_part; F.cg_part2 = cg_part2; F.cg_s = cg_s;
         F.tile_flags = tile_flags; F.tile_list = tile_list; F.tile_moved = tile_moved; F.active_count = active_count;
         F.cure_list = cure_list; F.cure_count = cure_count;
-        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.pigtab = pigtab;
+        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.tile_bits = tile_bits; F.compact_cnt = compact_cnt; F.pigtab = pigtab;
         F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
         F.frame_out = frame_out;
         return F;

@@ -115,6 +115,8 @@ struct BatchPlan {
 
 __global__ void k_deposit(FilmDev F);
 __global__ void k_substeps(FilmDev Fa, FilmDev Fb, uint32_t substeps, int clear_first);
+__global__ void k_compact_bits(FilmDev F);
+__global__ void k_compact_list(FilmDev F);
 __global__ void k_batch_plan(BatchPlan B);
 
 #define CZ_TILE_KERNELS(X)                                          \
@@ -125,7 +127,6 @@ __global__ void k_batch_plan(BatchPlan B);
     X(k_fluxes, (WS ws), (ws))                                      \
     X(k_fluxes_faces, (WS ws), (ws))                                \
     X(k_limit_apply, (WS ws), (ws))                                 \
-    X(k_delta_sync, (WS ws), (ws))                                  \
     X(k_strata_split, (WS ws), (ws))                                \
     X(k_reduce_pool, (WS ws), (ws))                                 \
     X(k_strata_limit, (WS ws), (ws))                                \

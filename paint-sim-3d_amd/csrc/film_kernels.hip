@@ -215,6 +215,91 @@ __global__ __launch_bounds__(1024) void k_compact_tiles(CS cs) {
     }
 }
 
+// The same compaction for one canvas in two short launches of one thread per
+// tile, for canvases whose tile count would keep one block busy for tens of
+// microseconds.  k_compact_bits applies the wake rule to the flags from
+// k_mark, writes the tile's flags, one byte (1 active, 2 cure) and the
+// block's two counts.  The flag bits other tiles' wake tests read (1..16 and
+// MOVED) are the same before and after the write, so tiles need no ordering
+// among themselves.  k_compact_list: each block sums the counts of the
+// blocks before it, then places its active and cure tiles by a block scan,
+// so both lists come out ascending.
+__global__ __launch_bounds__(256) void k_compact_bits(FilmDev F) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    bool active = false, cure = false;
+    if (t < F.nt16) {
+        const uint32_t tw = F.tw, th = F.th;
+        const int tx = (int)(t % tw), ty = (int)(t / tw);
+        const uint32_t fl_t = F.tile_flags[t];
+        uint32_t anyv = fl_t & MOVED;
+        for (int dy = -1; dy <= 1; dy++) {
+            for (int dxi = -1; dxi <= 1; dxi++) {
+                if (dxi == 0 && dy == 0) continue;
+                const int xx = tx + dxi, yy = ty + dy;
+                if (xx >= 0 && yy >= 0 && xx < (int)tw && yy < (int)th) {
+                    const uint32_t fl = F.tile_flags[(uint32_t)yy * tw + (uint32_t)xx];
+                    bool need = true;
+                    if (dxi == 1 && (fl & 2u) == 0u) need = false;
+                    if (dxi == -1 && (fl & 4u) == 0u) need = false;
+                    if (dy == 1 && (fl & 8u) == 0u) need = false;
+                    if (dy == -1 && (fl & 16u) == 0u) need = false;
+                    if (need) anyv |= fl & MOVED;
+                }
+            }
+        }
+        active = anyv != 0u;
+        cure = (fl_t & NEEDS_CURE) != 0u;
+        F.tile_flags[t] = active ? ((fl_t & 95u) | 32u) : (fl_t & 31u);
+        F.tile_bits[t] = (uint8_t)((active ? 1u : 0u) | (cure ? 2u : 0u));
+    }
+    const uint32_t na = (uint32_t)__syncthreads_count(active ? 1 : 0);
+    const uint32_t nc = (uint32_t)__syncthreads_count(cure ? 1 : 0);
+    if (threadIdx.x == 0) {
+        F.compact_cnt[2u * blockIdx.x] = na;
+        F.compact_cnt[2u * blockIdx.x + 1u] = nc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_compact_list(FilmDev F) {
+    __shared__ uint32_t s_base[2], s_wa[4], s_wc[4];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (wave == 0u) {
+        uint32_t a = 0u, c = 0u;
+        for (uint32_t q = lane; q < blockIdx.x; q += 64u) {
+            a += F.compact_cnt[2u * q];
+            c += F.compact_cnt[2u * q + 1u];
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, 64);
+            c += __shfl_xor(c, off, 64);
+        }
+        if (lane == 0u) {
+            s_base[0] = a;
+            s_base[1] = c;
+        }
+    }
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t b = t < F.nt16 ? F.tile_bits[t] : 0u;
+    const unsigned long long lmask = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+    const unsigned long long ba = __ballot((b & 1u) != 0u), bc = __ballot((b & 2u) != 0u);
+    if (lane == 0u) {
+        s_wa[wave] = (uint32_t)__popcll(ba);
+        s_wc[wave] = (uint32_t)__popcll(bc);
+    }
+    __syncthreads();
+    uint32_t oa = s_base[0], oc = s_base[1];
+    for (uint32_t w = 0; w < wave; w++) {
+        oa += s_wa[w];
+        oc += s_wc[w];
+    }
+    if (b & 1u) F.tile_list[oa + (uint32_t)__popcll(ba & lmask)] = t;
+    if (b & 2u) F.cure_list[oc + (uint32_t)__popcll(bc & lmask)] = t;
+    if (blockIdx.x + 1u == gridDim.x && threadIdx.x == 0u) {
+        *F.active_count = s_base[0] + s_wa[0] + s_wa[1] + s_wa[2] + s_wa[3];
+        *F.cure_count = s_base[1] + s_wc[0] + s_wc[1] + s_wc[2] + s_wc[3];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // cure.wgsl:3-48,62-123
 
@@ -1106,17 +1191,6 @@ __global__ __launch_bounds__(256) void k_substeps(FilmDev Fa, FilmDev Fb, uint32
     for (uint32_t a = blockIdx.x; a < n; a += G) lap_plug_tile(F, F.tile_list[a], S.lap, 1, 0, 0);
 }
 
-// Copy the offsets of active cells from the buffer written last to the other
-// one, so that both hold the same values everywhere between frames.
-template <class WS>
-__global__ __launch_bounds__(256) void k_delta_sync(WS ws) {
-    CZ_FOR_TILES(ws) {
-        CZ_TILE_CELL(F)
-        if (!in_grid) continue;
-        F.delta_out[c] = F.delta[c];
-    }
-}
-
 // ---------------------------------------------------------------------------
 // strata_split (film.wgsl:457-514)
 
@@ -1141,11 +1215,16 @@ __device__ inline float4 band_split(const FilmDev& F, uint32_t cu, float hf, flo
     return make_float4(fr[0] * total, fr[1] * total, fr[2] * total, fr[3] * total);
 }
 
+// The same launch copies the offsets of active cells from the buffer the last
+// nose wrote (F.delta, which split reads) to the other one (F.delta_out,
+// which nothing in the launch reads), so that both hold the same values
+// everywhere between frames.
 template <class WS>
 __global__ __launch_bounds__(256) void k_strata_split(WS ws) {
     CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
         if (!in_grid) continue;
+        F.delta_out[c] = F.delta[c];
         const GlobalRead g{F};
         float2 p00 = pressure_at(F, g, (int)x, (int)y);
         float4 a = F.acc[c];
@@ -2017,7 +2096,6 @@ CZ_INST1(k_cure, ManyFilms)
 CZ_INST2(k_fluxes)
 CZ_INST2(k_fluxes_faces)
 CZ_INST2(k_limit_apply)
-CZ_INST2(k_delta_sync)
 CZ_INST2(k_strata_split)
 CZ_INST2(k_strata_limit)
 CZ_INST2(k_strata_update)
