@@ -36,9 +36,12 @@ ALGO_BYTES = {
 SUBSTEP_FAMILIES = tuple(ALGO_BYTES)
 SUBSTEP_BYTES = sum(ALGO_BYTES.values())            # per active cell and substep
 ALGO_BYTES["lap"] = 4 + 4 + 4 + 8 + 4 + 8 + 8       # last nose: R h m delta fnorm, W delta fnorm pres
+# flux + apply in one launch: the flux pass's reads and the apply's, the
+# tile's fluxes written once and never read back
+ALGO_BYTES["flux_apply"] = (4 + 8 + 8 + 4 + 4 + 8) + (4 + 16) + (16 + 4 + 16)
 # families the roofline is reported for: the per-pass substep kernels and the
 # one-launch substeps kernel (k_substeps) that replaces them on small active sets
-ROOFLINE_FAMILIES = SUBSTEP_FAMILIES + ("substeps",)
+ROOFLINE_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "substeps")
 
 
 def log(msg):
@@ -236,7 +239,7 @@ def bench_serve(args):
     return out
 
 
-FRAME_FAMILIES = SUBSTEP_FAMILIES + ("substeps", "lap", "mark", "compact", "compact_list", "cure", "deposit", "split", "reduce_pool",
+FRAME_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "substeps", "lap", "mark", "compact", "compact_list", "cure", "deposit", "split", "reduce_pool",
                                      "strata_limit", "strata_update", "strata_fill",
                                      "frame_stats", "cg_setup", "cg_lap", "cg_init", "cg_iter_a", "cg_iter_b",
                                      "cg_iter_c", "cg_reduce", "cg_finish", "cg_rescale")
@@ -257,6 +260,8 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="direct launches instead of captured hipGraph replays")
+    ap.add_argument("--flux-shape", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="flux pass shape: -1 tuned online, 0 window, 1 face-parallel, 2 flux + apply fused")
     ap.add_argument("--substep-launch", default="auto", choices=("auto", "per-pass", "one"),
                     help="substeps as one launch per frame on small active sets (auto), always, or never")
     ap.add_argument("--events-in-timed", action="store_true",
@@ -298,6 +303,7 @@ def main():
         f = czf.Film(n, n, settings_fn(n))
         czf.check(czf.lib().cz_film_use_graphs(f._p, 0 if args.no_graphs else 1))
         f.substep_launch({"auto": -1, "per-pass": 0, "one": 1}[args.substep_launch])
+        f.flux_shape(args.flux_shape)
         assert f.write_state_arrays(state)
         return f
 
@@ -333,6 +339,7 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     st = film.stats()
+    timed_st = st
     if not args.events_in_timed:
         # the same frames again with events around every launch
         film.close()
@@ -357,7 +364,12 @@ def main():
                             + ALGO_BYTES["lap"] * st.one_launch_cells) / max(1, launches)
         algo_per_cell = SUBSTEP_BYTES
     else:
-        cells_per_launch = (st.act_cell_substeps - st.one_launch_cell_substeps) / max(1, launches)
+        # cells x substeps the family's launches covered: every per-pass
+        # substep runs lap_plug; flux, apply and flux_apply by flux shape
+        shp = list(st.flux_shape_cell_substeps)
+        covered = {"lap_plug": st.act_cell_substeps - st.one_launch_cell_substeps,
+                   "flux": shp[0] + shp[1], "apply": shp[0] + shp[1], "flux_apply": shp[2]}[dom]
+        cells_per_launch = covered / max(1, launches)
         bytes_per_launch = ALGO_BYTES[dom] * cells_per_launch
         algo_per_cell = ALGO_BYTES[dom]
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
@@ -384,6 +396,8 @@ def main():
                                f"{args.steps} frames from the initial state",
                    "cells": n * n, "graphs": not args.no_graphs, "substep_launch": args.substep_launch,
                    "one_launch_frames": st.one_launch_frames / max(1, st.frames_counted),
+                   "flux_shape": args.flux_shape,
+                   "flux_shape_cell_substeps": list(timed_st.flux_shape_cell_substeps),
                    "active_cells_per_frame": st.act_cells / frames,
                    "substeps_per_frame": st.act_cell_substeps / max(1, st.act_cells),
                    "implicit_cell_steps": st.act_cell_implicit,

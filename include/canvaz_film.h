@@ -159,6 +159,9 @@ typedef struct cz_stats {
     uint64_t one_launch_frames;
     uint64_t one_launch_cells;
     uint64_t one_launch_cell_substeps;
+    /* active cells times substeps run with each flux-pass shape (0 window,
+     * 1 face-parallel, 2 flux + apply in one launch) */
+    uint64_t flux_shape_cell_substeps[3];
 } cz_stats;
 
 /* The mass report of `compute_stats` (src/main.rs:1679-1819), sums in FP64. */
@@ -334,11 +337,13 @@ int cz_film_use_graphs(cz_film* f, int on);
 int cz_film_substep_launch(cz_film* f, int mode);
 /* Frame graphs captured and frames replayed from them so far. */
 int cz_film_graph_counts(const cz_film* f, uint64_t* captures, uint64_t* replays);
-/* Shape of the flux pass: 0 = one block per tile over an LDS window of the
- * tile and a 4-cell ring, 1 = one block per (tile, face) reading through the
- * caches (for very small active sets), -1 = chosen online (default): per
- * active-set size the two shapes are timed with HIP events on live frames
- * (they give bit-identical results) and the faster one is used. */
+/* Shape of the flux and limiter + apply passes: 0 = flux with one block per
+ * tile over an LDS window of the tile and a 4-cell ring, 1 = flux with one
+ * block per (tile, face) reading through the caches, 2 = flux and apply in
+ * one launch (each block computes the fluxes of its tile and a 2-cell ring
+ * itself; for small active sets), -1 = chosen online (default): per
+ * active-set size the shapes are timed with HIP events on live frames (they
+ * give bit-identical results) and the fastest is used. */
 int cz_film_flux_shape(cz_film* f, int shape);
 /* Average per-kernel device time over the last frames (HIP events on the
  * film's stream), in microseconds; label = kernel family name. */

@@ -258,16 +258,22 @@ int record(cz_batch* b, const FrameShape& sh) {
             eb = take_event(b);
             if (ea && eb) hipEventRecord(ea, st);
         }
-        if (sh.g_flux[s] == 1u)
-            hipLaunchKernelGGL(k_fluxes_faces<ManyFilms>, dim3(std::min<uint32_t>(4u * sh.g_sub[s], 4u * kGridCap)), b256, 0,
-                               st, many(b->items_sub, lim_sub + s, v));
-        else
-            hipLaunchKernelGGL(k_fluxes<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v));
+        // the tuner times the flux and apply passes together (shape 2 is both
+        // in one launch)
+        if (sh.g_flux[s] == 2u) {
+            hipLaunchKernelGGL(k_flux_apply<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v));
+        } else {
+            if (sh.g_flux[s] == 1u)
+                hipLaunchKernelGGL(k_fluxes_faces<ManyFilms>, dim3(std::min<uint32_t>(4u * sh.g_sub[s], 4u * kGridCap)),
+                                   b256, 0, st, many(b->items_sub, lim_sub + s, v));
+            else
+                hipLaunchKernelGGL(k_fluxes<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v));
+            hipLaunchKernelGGL(k_limit_apply<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v));
+        }
         if (ea && eb) {
             hipEventRecord(eb, st);
             b->tuner_pending.push_back({&b->flux_tuner, sh.f_bucket[s], (int)sh.g_flux[s], ea, eb});
         }
-        hipLaunchKernelGGL(k_limit_apply<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v));
     }
     const uint32_t* all = lim_sub + sh.max_sub;
     const dim3 ga(sh.g_all);
@@ -323,6 +329,7 @@ int cz_batch_create(uint32_t max_members, cz_batch** out) {
     *out = nullptr;
     cz_batch* b = new cz_batch();
     b->max_members = max_members;
+    b->flux_tuner.ns = 3;  // window, face-parallel, flux + apply in one launch
     int rc = CZ_OK;
     hipError_t e = hipGetDevice(&b->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -401,7 +408,7 @@ int cz_batch_use_graphs(cz_batch* b, int on) {
 }
 
 int cz_batch_flux_shape(cz_batch* b, int shape) {
-    if (!b || shape < -1 || shape > 1) return set_err(CZ_EINVAL, "flux_shape: -1, 0 or 1");
+    if (!b || shape < -1 || shape > 2) return set_err(CZ_EINVAL, "flux_shape: -1, 0, 1 or 2");
     b->flux_tuner.fixed = shape;
     return CZ_OK;
 }
@@ -546,6 +553,7 @@ int cz_batch_step(cz_batch* b, float frame_dt) {
     }
     for (uint32_t m = 0; m < nm; m++) {
         b->members[m]->one_launch_now = false;
+        for (uint32_t& v : b->members[m]->shape_subs_now) v = 0u;
         frame_done(b->members[m], sub[m], false);
     }
     b->stats_pending = true;

@@ -111,6 +111,7 @@ void fold_frame_out(cz_film* f, const float* o) {
     f->cnt_act_cells += cells;
     f->cnt_act_cell_substeps += cells * (uint64_t)f->substeps_last;
     f->cnt_act_cell_implicit += cells * (uint64_t)f->implicit_last;
+    for (int k = 0; k < 3; k++) f->cnt_shape_cell_substeps[k] += cells * (uint64_t)f->shape_subs_last[k];
     if (f->one_launch_last) {
         f->cnt_one_frames += 1;
         f->cnt_one_cells += cells;
@@ -150,6 +151,7 @@ void frame_done(cz_film* f, uint32_t substeps, bool parity_done) {
     f->substeps_last = substeps;
     f->implicit_last = f->s.implicit_hydro ? f->implicit_steps : 0u;
     f->one_launch_last = f->one_launch_now;
+    for (int k = 0; k < 3; k++) f->shape_subs_last[k] = f->shape_subs_now[k];
     f->frame++;
     f->deposit_count = 0;
 }
@@ -275,6 +277,7 @@ int record_frame(cz_film* f, uint32_t substeps) {
     }
     const uint32_t pg = f->persist_grid();
     f->one_launch_now = pg > 0u;
+    for (uint32_t& v : f->shape_subs_now) v = 0u;
     if (pg > 0u) {
         // every substep and the last nose in one launch
         const FilmDev Fa = f->dev();
@@ -294,23 +297,29 @@ int record_frame(cz_film* f, uint32_t substeps) {
         const OneFilm w{f->dev()};
         int bk = 0;
         bool timed = false;
-        const int shape = f->flux_tuner.choose(std::min(f->active_hint, f->nt16), !f->capturing && !f->prof_on, bk,
-                                               timed);
+        const int shape = f->flux_tuner.choose(std::min(f->active_hint, f->nt16), !f->capturing, bk, timed);
         hipEvent_t ea = nullptr, eb = nullptr;
         if (timed) {
             ea = f->take_event();
             eb = f->take_event();
             if (ea && eb) hipEventRecord(ea, f->stream);
         }
-        if (shape == 1)
-            LAUNCH("flux", k_fluxes_faces<OneFilm>, dim3(f->tile_grid(4)), b256, w);
-        else
-            LAUNCH("flux", k_fluxes<OneFilm>, gt, b256, w);
+        // the tuner times the flux and apply passes together (shape 2 is both
+        // in one launch)
+        if (shape == 2) {
+            LAUNCH("flux_apply", k_flux_apply<OneFilm>, gt, b256, w);
+        } else {
+            if (shape == 1)
+                LAUNCH("flux", k_fluxes_faces<OneFilm>, dim3(f->tile_grid(4)), b256, w);
+            else
+                LAUNCH("flux", k_fluxes<OneFilm>, gt, b256, w);
+            LAUNCH("apply", k_limit_apply<OneFilm>, gt, b256, w);
+        }
         if (ea && eb) {
             hipEventRecord(eb, f->stream);
             f->tuner_pending.push_back({&f->flux_tuner, bk, shape, ea, eb});
         }
-        LAUNCH("apply", k_limit_apply<OneFilm>, gt, b256, w);
+        f->shape_subs_now[shape]++;
     }
     // nose of the last substep + the laplacian strata_split reads
     if (pg == 0u) {
@@ -527,6 +536,7 @@ void reset_host_state(cz_film* f) {
     f->stats_pending = false;
     f->cnt_act_cells = f->cnt_act_cell_substeps = f->cnt_frames = f->cnt_act_cell_implicit = 0;
     f->cnt_one_frames = f->cnt_one_cells = f->cnt_one_cell_substeps = 0;
+    for (uint64_t& v : f->cnt_shape_cell_substeps) v = 0;
     f->delta_cur_a = true;
     f->deposit_count = 0;
     memset(&f->P, 0, sizeof(f->P));
@@ -547,6 +557,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     if ((w + 15u) / 16u > 21845u) return set_err(CZ_EINVAL, "cz_film_create: width %u exceeds 349520 cells", w);
     *out = nullptr;
     cz_film* f = new cz_film();
+    f->flux_tuner.ns = 3;  // window, face-parallel, flux + apply in one launch
     if (s) f->s = *s; else cz_settings_default(&f->s);
     if (f->s.slump) {
         destroy(f);
@@ -766,6 +777,7 @@ int cz_film_stats(cz_film* f, cz_stats* st) {
     st->one_launch_frames = f->cnt_one_frames;
     st->one_launch_cells = f->cnt_one_cells;
     st->one_launch_cell_substeps = f->cnt_one_cell_substeps;
+    for (int k = 0; k < 3; k++) st->flux_shape_cell_substeps[k] = f->cnt_shape_cell_substeps[k];
     return CZ_OK;
 }
 
@@ -925,7 +937,7 @@ int cz_film_substep_launch(cz_film* f, int mode) {
 }
 
 int cz_film_flux_shape(cz_film* f, int shape) {
-    if (!f || shape < -1 || shape > 1) return set_err(CZ_EINVAL, "flux_shape: -1, 0 or 1");
+    if (!f || shape < -1 || shape > 2) return set_err(CZ_EINVAL, "flux_shape: -1, 0, 1 or 2");
     f->flux_tuner.fixed = shape;
     return CZ_OK;
 }
@@ -966,6 +978,7 @@ int cz_film_reset_counters(cz_film* f) {
     if (rc) return rc;
     f->cnt_act_cells = f->cnt_act_cell_substeps = f->cnt_frames = f->cnt_act_cell_implicit = 0;
     f->cnt_one_frames = f->cnt_one_cells = f->cnt_one_cell_substeps = 0;
+    for (uint64_t& v : f->cnt_shape_cell_substeps) v = 0;
     return CZ_OK;
 }
 

@@ -52,7 +52,8 @@ struct ProfSlot {
 // WARM samples, then the faster one is used, re-timed once every RECHECK
 // uses.  fixed >= 0 pins a shape.
 struct ShapeTuner {
-    static constexpr int NB = 24, NS = 2, WARM = 3, RECHECK = 64;
+    static constexpr int NB = 24, NS = 3, WARM = 3, RECHECK = 64;
+    int ns = 2;  // shapes in use (at most NS)
     double sum[NB][NS] = {};
     uint32_t cnt[NB][NS] = {};
     uint32_t uses[NB] = {};
@@ -68,28 +69,42 @@ struct ShapeTuner {
         bk = bucket(items);
         timed = false;
         if (fixed >= 0) return fixed;
-        if (!can_time) {
-            if (cnt[bk][0] == 0 || cnt[bk][1] == 0) return cnt[bk][1] > 0 ? 1 : 0;
-            return (sum[bk][0] / cnt[bk][0] <= sum[bk][1] / cnt[bk][1]) ? 0 : 1;
-        }
-        for (int k = 0; k < NS; k++)
+        if (!can_time) return best_of(bk);
+        for (int k = 0; k < ns; k++)
             if (cnt[bk][k] < (uint32_t)WARM) {
                 timed = true;
                 return k;
             }
-        const int best = (sum[bk][0] / cnt[bk][0] <= sum[bk][1] / cnt[bk][1]) ? 0 : 1;
+        const int best = best_of(bk);
         if (++uses[bk] % RECHECK == 0) {
+            // re-time one of the others, in turn
             timed = true;
-            return 1 - best;
+            return (best + 1 + (int)((uses[bk] / RECHECK) % (uint32_t)(ns - 1))) % ns;
         }
         return best;
+    }
+    // the fastest shape with samples (shape 0 if none has any)
+    int best_of(int bk) const {
+        int best = -1;
+        double bt = 0.0;
+        for (int k = 0; k < ns; k++) {
+            if (cnt[bk][k] == 0) continue;
+            const double t = sum[bk][k] / cnt[bk][k];
+            if (best < 0 || t < bt) {
+                best = k;
+                bt = t;
+            }
+        }
+        return best < 0 ? 0 : best;
     }
     // true while a shape of this size class still lacks its warm-up samples
     // (a graph replay cannot take them: such frames run uncaptured)
     bool learning(uint64_t items) const {
         if (fixed >= 0) return false;
         const int b = bucket(items);
-        return cnt[b][0] < (uint32_t)WARM || cnt[b][1] < (uint32_t)WARM;
+        for (int k = 0; k < ns; k++)
+            if (cnt[b][k] < (uint32_t)WARM) return true;
+        return false;
     }
     void record(int bk, int shape, double us) {
         sum[bk][shape] += us;
@@ -139,6 +154,8 @@ struct cz_film {
     uint64_t cnt_act_cells = 0, cnt_act_cell_substeps = 0, cnt_frames = 0, cnt_act_cell_implicit = 0;
     uint64_t cnt_one_frames = 0, cnt_one_cells = 0, cnt_one_cell_substeps = 0;
     bool one_launch_last = false;  // the frame being folded ran its substeps as one launch
+    uint64_t cnt_shape_cell_substeps[3] = {0, 0, 0};
+    uint32_t shape_subs_now[3] = {0, 0, 0}, shape_subs_last[3] = {0, 0, 0};  // substeps per flux shape
     int device = 0;
     hipStream_t stream = nullptr;
     void* batch = nullptr;  // the cz_batch this film is a member of

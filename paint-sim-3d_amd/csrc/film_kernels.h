@@ -127,6 +127,7 @@ __global__ void k_batch_plan(BatchPlan B);
     X(k_fluxes, (WS ws), (ws))                                      \
     X(k_fluxes_faces, (WS ws), (ws))                                \
     X(k_limit_apply, (WS ws), (ws))                                 \
+    X(k_flux_apply, (WS ws), (ws))                                  \
     X(k_strata_split, (WS ws), (ws))                                \
     X(k_reduce_pool, (WS ws), (ws))                                 \
     X(k_strata_limit, (WS ws), (ws))                                \

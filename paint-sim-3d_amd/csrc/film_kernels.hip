@@ -853,24 +853,36 @@ __global__ __launch_bounds__(256) void k_fluxes_faces(WS ws) {
 constexpr int FLUX_R = 4;
 constexpr int FLUX_WIN = 16 + 2 * FLUX_R;
 
-struct FluxWin {
-    float h[FLUX_WIN * FLUX_WIN];
-    uint32_t d[FLUX_WIN * FLUX_WIN];
-    float2 rheo[FLUX_WIN * FLUX_WIN];
-    float2 fn[FLUX_WIN * FLUX_WIN];
-    float2 pres[FLUX_WIN * FLUX_WIN];
-    float tmp[FLUX_WIN * FLUX_WIN];
-    float lmu[FLUX_WIN * FLUX_WIN];   // ln max(mu, 1e-4)
-    float fd[FLUX_WIN * FLUX_WIN];    // front distance (0 when dry)
+// The fields the flux of a cell reads, staged for the tile and an R-cell ring
+// (the flux of a cell reads cells up to 4 away, plus the far steps of
+// cell_excess).
+template <int R>
+struct FluxWinT {
+    static constexpr int N = 16 + 2 * R;
+    float h[N * N];
+    uint32_t d[N * N];
+    float2 rheo[N * N];
+    float2 fn[N * N];
+    float2 pres[N * N];
+    float tmp[N * N];
+    float lmu[N * N];   // ln max(mu, 1e-4)
+    float fd[N * N];    // front distance (0 when dry)
     uint32_t tf[9];
 };
+using FluxWin = FluxWinT<FLUX_R>;
 
-struct WinRead {
+// Accessor of a staged window.  SNAP: the heights are the ones before this
+// substep's apply (h_sub for cells of active tiles, h elsewhere, where no
+// apply writes), as the fused flux + apply launch needs while other blocks
+// apply; otherwise the live heights, which equal them during the flux pass.
+template <int R, bool SNAP>
+struct WinReadT {
+    static constexpr int N = FluxWinT<R>::N;
     const FilmDev& F;
-    const FluxWin& w;
+    const FluxWinT<R>& w;
     int wx0, wy0, ttx, tty;
-    __device__ __forceinline__ int q(int i, int j) const { return (j - wy0) * FLUX_WIN + (i - wx0); }
-    // cells within FLUX_R of the tile: always inside the window
+    __device__ __forceinline__ int q(int i, int j) const { return (j - wy0) * N + (i - wx0); }
+    // cells within R of the tile: always inside the window
     __device__ __forceinline__ float h(int i, int j) const { return w.h[q(i, j)]; }
     __device__ __forceinline__ uint32_t delta(int i, int j) const { return w.d[q(i, j)]; }
     __device__ __forceinline__ float2 rheo(int i, int j) const { return w.rheo[q(i, j)]; }
@@ -885,12 +897,17 @@ struct WinRead {
     // the far steps of cell_excess: the window when inside, else HBM (both
     // reads are issued, so no pointer selects between the two spaces)
     __device__ __forceinline__ bool in(int i, int j) const {
-        return (unsigned)(i - wx0) < (unsigned)FLUX_WIN && (unsigned)(j - wy0) < (unsigned)FLUX_WIN;
+        return (unsigned)(i - wx0) < (unsigned)N && (unsigned)(j - wy0) < (unsigned)N;
     }
     __device__ __forceinline__ float h_far(int i, int j) const {
         const bool inside = in(i, j);
+        const uint32_t c = (uint32_t)j * F.W + (uint32_t)i;
         const float vl = w.h[inside ? q(i, j) : 0];
-        const float vg = F.h[(uint32_t)j * F.W + (uint32_t)i];
+        float vg = F.h[c];
+        if (SNAP) {
+            const float vs = F.h_sub[c];
+            vg = cell_active(F, (uint32_t)i, (uint32_t)j) ? vs : vg;
+        }
         return inside ? vl : vg;
     }
     __device__ __forceinline__ uint32_t delta_far(int i, int j) const {
@@ -899,59 +916,79 @@ struct WinRead {
         const uint32_t vg = F.delta[(uint32_t)j * F.W + (uint32_t)i];
         return inside ? vl : vg;
     }
-    // cells within one of the tile: one of the 3x3 tiles around it
+    // cells within 16 of the tile: one of the 3x3 tiles around it
     __device__ __forceinline__ bool active(int i, int j) const {
         return (w.tf[((j >> 4) - tty + 1) * 3 + ((i >> 4) - ttx + 1)] & 32u) != 0u;
     }
 };
+using WinRead = WinReadT<FLUX_R, false>;
 
-// fluxes (film.wgsl:375-455) of a tile, one cell's four faces per thread,
-// with the fields staged in LDS (the shape for large active sets).
-// One tile of the staged flux pass (every thread of the block calls it).
-__device__ __forceinline__ void flux_tile(const FilmDev& F, uint32_t _t, FluxWin& sw) {
-    const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;
-    CZ_TILE_CELL(F)
-    const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
-    const int wx0 = ttx * 16 - FLUX_R, wy0 = tty * 16 - FLUX_R;
+// Stage the window of tile (ttx, tty) with origin (wx0, wy0).  The 3x3 tile
+// flags are loaded first (they pick the heights' source when SNAP).
+template <int R, bool SNAP>
+__device__ __forceinline__ void flux_stage(const FilmDev& F, FluxWinT<R>& sw, int ttx, int tty, int wx0, int wy0) {
+    constexpr int N = FluxWinT<R>::N;
     const int W_ = (int)F.W, H_ = (int)F.H;
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < (uint32_t)(FLUX_WIN * FLUX_WIN); q += 256u) {
-        const int gx = wx0 + (int)(q % FLUX_WIN), gy = wy0 + (int)(q / FLUX_WIN);
-        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
-            const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
-            sw.h[q] = F.h[nn];
-            sw.d[q] = F.delta[nn];
-            sw.rheo[q] = F.rheo[nn];
-            sw.fn[q] = F.fnorm[nn];
-            sw.pres[q] = F.pres[nn];
-            sw.tmp[q] = F.tmp[nn];
-            const float hv = sw.h[q];
-            sw.lmu[q] = cz_logf(cz_max(sw.rheo[q].x, 1e-4f));
-            sw.fd[q] = hv <= 0.0f ? 0.0f : nose_dist_b(sw.d[q]);
-        }
-    }
     if (threadIdx.x < 9u) {
         const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
         sw.tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
                                  ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
     }
+    if (SNAP) __syncthreads();
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(N * N); q += 256u) {
+        const int gx = wx0 + (int)(q % N), gy = wy0 + (int)(q / N);
+        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+            const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+            float hv = F.h[nn];
+            if (SNAP) {
+                const float hs = F.h_sub[nn];
+                const bool act = (sw.tf[((gy >> 4) - tty + 1) * 3 + ((gx >> 4) - ttx + 1)] & 32u) != 0u;
+                hv = act ? hs : hv;
+            }
+            sw.h[q] = hv;
+            sw.d[q] = F.delta[nn];
+            sw.rheo[q] = F.rheo[nn];
+            sw.fn[q] = F.fnorm[nn];
+            sw.pres[q] = F.pres[nn];
+            sw.tmp[q] = F.tmp[nn];
+            sw.lmu[q] = cz_logf(cz_max(sw.rheo[q].x, 1e-4f));
+            sw.fd[q] = hv <= 0.0f ? 0.0f : nose_dist_b(sw.d[q]);
+        }
+    }
     __syncthreads();
-    if (!in_grid) return;
-    const WinRead g{F, sw, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
-    const float2 p00 = pressure_at(F, g, (int)x, (int)y);
+}
+
+// fluxes (film.wgsl:375-455) of cell (x, y): its four owned faces
+template <class A>
+__device__ __forceinline__ float4 cell_fluxes(const FilmDev& F, const A& g, int x, int y) {
+    const float2 p00 = pressure_at(F, g, x, y);
     float4 qv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll 1
     for (int k = 0; k < 4; k++) {
-        FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, true, p00);
+        FaceGeom fg = face_geom(F, g, x, y, k, true, p00);
         if (!fg.has) continue;
-        FaceFlux ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+        FaceFlux ff = face(F, g, x, y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
         const float q = ff.q * ((k < 2) ? W_AXIS : W_DIAG_FLUX);
         if (k == 0) qv.x = q;
         else if (k == 1) qv.y = q;
         else if (k == 2) qv.z = q;
         else qv.w = q;
     }
-    F.flux[c] = qv;
+    return qv;
+}
+
+// One tile of the staged flux pass (every thread of the block calls it): the
+// tile's cells, one cell's four faces per thread, the fields in LDS.
+__device__ __forceinline__ void flux_tile(const FilmDev& F, uint32_t _t, FluxWin& sw) {
+    const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;
+    CZ_TILE_CELL(F)
+    const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+    const int wx0 = ttx * 16 - FLUX_R, wy0 = tty * 16 - FLUX_R;
+    flux_stage<FLUX_R, false>(F, sw, ttx, tty, wx0, wy0);
+    if (!in_grid) return;
+    const WinRead g{F, sw, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
+    F.flux[c] = cell_fluxes(F, g, (int)x, (int)y);
 }
 
 template <class WS>
@@ -1001,8 +1038,9 @@ struct ApplyShared {
     uint32_t s_tf[9];
 };
 
-// One tile of the limiter + apply pass (every thread of the block calls it).
-__device__ __forceinline__ void apply_tile(const FilmDev& F, uint32_t _t, ApplyShared& S) {
+// The limiter + apply of a tile from the staged fluxes and heights of the tile
+// and a 2-cell ring (S.s_f, S.s_hh) and the 3x3 tile flags (S.s_tf).
+__device__ __forceinline__ void apply_core(const FilmDev& F, uint32_t _t, ApplyShared& S) {
     float4* s_f = S.s_f;
     float* s_hh = S.s_hh;
     float* s_a = S.s_a;
@@ -1013,21 +1051,6 @@ __device__ __forceinline__ void apply_tile(const FilmDev& F, uint32_t _t, ApplyS
     const int tx0 = ttx * 16, ty0 = tty * 16;
     const int wx0 = tx0 - AP_R, wy0 = ty0 - AP_R;
     const int W_ = (int)F.W, H_ = (int)F.H;
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += 256u) {
-        const int gx = wx0 + (int)(q % AP_WIN), gy = wy0 + (int)(q / AP_WIN);
-        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
-            const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
-            s_f[q] = F.flux[nn];
-            s_hh[q] = F.h_sub[nn];
-        }
-    }
-    if (threadIdx.x < 9u) {
-        const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
-        s_tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
-                                ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
-    }
-    __syncthreads();
     auto wq = [&](int gx, int gy) -> int { return (gy - wy0) * AP_WIN + (gx - wx0); };
     // in the canvas and in an active tile (cells within 2 of the tile)
     auto nb_ok = [&](int gi, int gj) -> bool {
@@ -1102,6 +1125,76 @@ __device__ __forceinline__ void apply_tile(const FilmDev& F, uint32_t _t, ApplyS
                                a.w + accv[3] * F.P.dt);
     }
     if (__syncthreads_or(moved ? 1 : 0) && threadIdx.x == 0) F.tile_moved[_t] = 1u;
+}
+
+// One tile of the limiter + apply pass (every thread of the block calls it).
+__device__ __forceinline__ void apply_tile(const FilmDev& F, uint32_t _t, ApplyShared& S) {
+    const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+    const int wx0 = ttx * 16 - AP_R, wy0 = tty * 16 - AP_R;
+    const int W_ = (int)F.W, H_ = (int)F.H;
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += 256u) {
+        const int gx = wx0 + (int)(q % AP_WIN), gy = wy0 + (int)(q / AP_WIN);
+        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+            const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+            S.s_f[q] = F.flux[nn];
+            S.s_hh[q] = F.h_sub[nn];
+        }
+    }
+    if (threadIdx.x < 9u) {
+        const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
+        S.s_tf[threadIdx.x] = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
+                                  ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
+    }
+    __syncthreads();
+    apply_core(F, _t, S);
+}
+
+// fluxes + limiter + apply of a tile in one launch, for small active sets
+// where the launch between the two passes costs more than the work: the
+// block computes the fluxes of the tile and a 2-cell ring itself (the cells
+// the tile's limiter ring reads) from a window of the tile and a 6-cell ring,
+// instead of reading them from the flux pass.  Other blocks apply while this
+// one computes, so the window's heights are the ones before this substep's
+// apply (h_sub on active tiles; elsewhere h, which no apply writes).  The
+// rheology apply may write (a newly wetted cell inheriting) belongs to a cell
+// that was dry before the apply, whose rheology enters the fluxes only with
+// weight 0 (film.wgsl:324-327), so its live value gives the same fluxes.
+constexpr int FA_R = AP_R + FLUX_R;
+
+struct FluxApplyShared {
+    FluxWinT<FA_R> w;
+    ApplyShared ap;
+};
+
+__device__ __forceinline__ void flux_apply_tile(const FilmDev& F, uint32_t _t, FluxApplyShared& S) {
+    const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;
+    CZ_TILE_CELL(F)
+    const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+    const int wx0 = ttx * 16 - FA_R, wy0 = tty * 16 - FA_R;
+    const int W_ = (int)F.W, H_ = (int)F.H;
+    flux_stage<FA_R, true>(F, S.w, ttx, tty, wx0, wy0);
+    if (threadIdx.x < 9u) S.ap.s_tf[threadIdx.x] = S.w.tf[threadIdx.x];
+    const WinReadT<FA_R, true> g{F, S.w, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
+    // the fluxes of the active cells of the tile + 2 ring into the apply's window
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += 256u) {
+        const int gx = ttx * 16 - AP_R + (int)(q % AP_WIN), gy = tty * 16 - AP_R + (int)(q / AP_WIN);
+        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+            S.ap.s_hh[q] = S.w.h[g.q(gx, gy)];
+            float4 qv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (g.active(gx, gy)) qv = cell_fluxes(F, g, gx, gy);
+            S.ap.s_f[q] = qv;
+        }
+    }
+    __syncthreads();
+    if (in_grid) F.flux[c] = S.ap.s_f[(int)(_ly + AP_R) * AP_WIN + (int)(_lx + AP_R)];
+    apply_core(F, _t, S.ap);
+}
+
+template <class WS>
+__global__ __launch_bounds__(256) void k_flux_apply(WS ws) {
+    __shared__ FluxApplyShared S;
+    CZ_FOR_TILES(ws) flux_apply_tile(F, _t, S);
 }
 
 template <class WS>
@@ -2096,6 +2189,7 @@ CZ_INST1(k_cure, ManyFilms)
 CZ_INST2(k_fluxes)
 CZ_INST2(k_fluxes_faces)
 CZ_INST2(k_limit_apply)
+CZ_INST2(k_flux_apply)
 CZ_INST2(k_strata_split)
 CZ_INST2(k_strata_limit)
 CZ_INST2(k_strata_update)

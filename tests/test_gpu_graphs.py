@@ -53,13 +53,15 @@ def test_graph_replay_equals_direct(gpu):
 
 @pytest.mark.parametrize("batched", [False, True])
 def test_flux_shapes_agree(gpu, batched):
-    """The two launch shapes of the flux pass (and the online choice between
-    them) give bit-identical frames."""
+    """The launch shapes of the flux and apply passes (window, face-parallel,
+    flux + apply in one launch) and the online choice between them give
+    bit-identical frames, for a single film and in a batch."""
     w = h = 112
     s = scenes.blobs_settings(w, tilt_deg=35.0)
     st = scenes.blobs_state(w, h, seed=8, nblobs=10, sigma_cells=(2.0, 10.0))
+    shapes = (0, 1, 2, -1)
     films = []
-    for shape in (0, 1, -1):
+    for shape in shapes:
         f = czf.Film(w, h, s)
         assert f.write_state_arrays(st)
         films.append(f)
@@ -67,12 +69,12 @@ def test_flux_shapes_agree(gpu, batched):
             f.flux_shape(shape)
     batches = []
     if batched:
-        for f, shape in zip(films, (0, 1, -1)):
+        for f, shape in zip(films, shapes):
             b = czf.Batch(2)
             b.flux_shape(shape)
             b.add(f)
             batches.append(b)
-    for _ in range(12):
+    for _ in range(16):
         if batched:
             for b in batches:
                 b.step()
@@ -83,5 +85,8 @@ def test_flux_shapes_agree(gpu, batched):
         b.sync()
     for f in films:
         f.sync()
-    assert_same(films[0], films[1], "window vs faces")
-    assert_same(films[0], films[2], "window vs tuned")
+    for k, shape in enumerate(shapes[1:], 1):
+        assert_same(films[0], films[k], f"window vs shape {shape}")
+    if not batched:
+        used = films[2].stats().flux_shape_cell_substeps
+        assert used[2] > 0 and used[0] == used[1] == 0, list(used)

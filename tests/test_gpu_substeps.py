@@ -1,6 +1,7 @@
 """The substeps of a frame launched as one kernel with grid-wide barriers
-between the passes (k_substeps) give the bits of the three launches per
-substep: films in each launch mode step the same scenes (a settling blob
+between the passes (k_substeps), and the flux and apply passes fused into
+one launch, give the bits of the three launches per substep: films in each
+launch mode step the same scenes (a settling blob
 canvas with deposits, a thick canvas with implicit steps, and a canvas whose
 active set outgrows the one-launch grid so that blocks take several tiles)."""
 import numpy as np
@@ -42,11 +43,13 @@ def test_one_launch_equals_per_pass(gpu, case, graphs):
     s.height_m = h * s.width_m / w
     st = scenes.blobs_state(w, h, **kw)
     films = []
-    for mode in (0, 1, -1):
+    # (substep launch mode, flux shape): per-pass, one launch, automatic, and
+    # per-pass with flux + apply fused
+    for mode, shape in ((0, 0), (1, 0), (-1, 0), (0, 2)):
         f = czf.Film(w, h, s)
         f.substep_launch(mode)
         f.use_graphs(graphs)
-        f.flux_shape(0)
+        f.flux_shape(shape)
         assert f.write_state_arrays(st)
         films.append(f)
     pig = scenes.jet_paint()
@@ -67,10 +70,12 @@ def test_one_launch_equals_per_pass(gpu, case, graphs):
             peak_active = max(peak_active, films[0].stats().active_tiles)
             assert_same(films[0], films[1], f"{case} frame {fr + 1} forced")
             assert_same(films[0], films[2], f"{case} frame {fr + 1} auto")
+            assert_same(films[0], films[3], f"{case} frame {fr + 1} fused flux + apply")
     for f in films:
         f.sync()
     assert_same(films[0], films[1], f"{case} end forced")
     assert_same(films[0], films[2], f"{case} end auto")
+    assert_same(films[0], films[3], f"{case} end fused flux + apply")
     if case == "implicit":
         assert implicit_seen, "the thick canvas should take implicit steps"
     if case == "large":
