@@ -463,6 +463,24 @@ int cz_server_run(cz_server* s, const cz_request* reqs, uint32_t n, cz_result* r
 /* A finished canvas kept with keep_films (owned by the server), or NULL. */
 cz_film* cz_server_film(cz_server* s, uint64_t id);
 
+/* Painting files (.canvaz; src/session.rs:59-99,248-317).  save_painting:
+ * the 44-byte header ("PSIM", version 1, w, h, tilt_deg, tilt_dir_deg,
+ * cure_rate, h_max, s_max, total_deposited) and pack() of the eight
+ * read_state parts, each cut into 32 MiB chunks compressed as LZ4 blocks
+ * with the uncompressed size prepended (lz4_flex::compress_prepend_size).
+ * open_painting for a film of the file's size: write_state, then the
+ * header's settings and scalars.  The file is read whole; a size mismatch
+ * is CZ_ESIZE. */
+int cz_film_save_canvaz(cz_film* f, const char* path);
+int cz_film_load_canvaz(cz_film* f, const char* path);
+/* The LZ4 framing on its own: one block of src with its size prepended
+ * (lz4_flex::compress_prepend_size), its inverse
+ * (lz4_flex::decompress_size_prepended), and the worst-case output size of
+ * the former. */
+int cz_lz4_compress(const void* src, uint64_t n, void* dst, uint64_t cap, uint64_t* out_len);
+int cz_lz4_decompress(const void* src, uint64_t n, void* dst, uint64_t cap, uint64_t* out_len);
+uint64_t cz_lz4_bound(uint64_t n);
+
 /* Device self-check with no counterpart in the reference: the kernels pack
  * and unpack f16 slots (pigments, substrate bands) with the hardware
  * conversions; this counts, over every non-NaN f32 and f16 bit pattern, the

@@ -93,6 +93,11 @@ def lib():
         "cz_film_graph_counts": (i32, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "cz_f16_selftest": (i32, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "cz_film_substep_launch": (i32, [P, i32]),
+        "cz_film_save_canvaz": (i32, [P, C.c_char_p]),
+        "cz_film_load_canvaz": (i32, [P, C.c_char_p]),
+        "cz_lz4_compress": (i32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+        "cz_lz4_decompress": (i32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+        "cz_lz4_bound": (C.c_uint64, [C.c_uint64]),
         "cz_film_flux_shape": (i32, [P, i32]),
         "cz_film_profile": (i32, [P, i32]),
         "cz_film_profile_read": (i32, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)]),
@@ -195,6 +200,27 @@ def _v2(p):
 
 def _pig8(pig):
     return (C.c_uint32 * 8)(*[int(v) for v in pig])
+
+
+def lz4_compress(data):
+    """One LZ4 block of `data` with its u32 size prepended (lz4_flex::compress_prepend_size)."""
+    src = bytes(data)
+    cap = lib().cz_lz4_bound(len(src))
+    out = C.create_string_buffer(cap)
+    n = C.c_uint64()
+    check(lib().cz_lz4_compress(src, len(src), out, cap, C.byref(n)))
+    return out.raw[:n.value]
+
+
+def lz4_decompress(data, cap=None):
+    """Inverse of lz4_compress (lz4_flex::decompress_size_prepended)."""
+    src = bytes(data)
+    if cap is None:
+        cap = int.from_bytes(src[:4], "little") if len(src) >= 4 else 0
+    out = C.create_string_buffer(max(cap, 1))
+    n = C.c_uint64()
+    check(lib().cz_lz4_decompress(src, len(src), out, cap, C.byref(n)))
+    return out.raw[:n.value]
 
 
 def deposit_capsule(p0, p1, r, volume, pig, dx):
@@ -382,6 +408,21 @@ class Film:
     def flux_shape(self, shape):
         """-1 = tuned online, 0 = LDS window per tile, 1 = block per (tile, face)."""
         check(lib().cz_film_flux_shape(self._p, shape))
+
+    def pool(self):
+        """pool8 of the last frame: (w/4)*(h/4) surface maxima, then as many relief maxima."""
+        n = 2 * ((self.w + 3) // 4) * ((self.h + 3) // 4)
+        a = np.empty(n, np.float32)
+        check(lib().cz_film_pool(self._p, _fp(a), n))
+        return a
+
+    def save_canvaz(self, path):
+        """Write the film as a painting file (save_painting, src/session.rs:248-280)."""
+        check(lib().cz_film_save_canvaz(self._p, os.fsencode(path)))
+
+    def load_canvaz(self, path):
+        """Read a painting file of this film's size (open_painting, src/session.rs:282-317)."""
+        check(lib().cz_film_load_canvaz(self._p, os.fsencode(path)))
 
     def substep_launch(self, mode):
         """-1 = by active-set size, 0 = three launches per substep, 1 = one launch per frame."""

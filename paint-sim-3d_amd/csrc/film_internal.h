@@ -128,6 +128,7 @@ struct cz_film {
     bool ov_from_env = true;
     cz_params P;
     float h_max = 0.0f, mu_min = 1.0f, h_max_pending = 0.0f;
+    float s_max = 0.0f;  // the surface maximum a painting file carries (src/film/mod.rs:567)
     uint32_t implicit_steps = 1, substeps_last = 1, implicit_last = 0, frame = 0, active_tiles = 0;
     uint32_t active_hint = 0xffffffffu;  // grid sizing: last folded active count
     // flux pass shape: 0 = one block per tile over an LDS window, 1 = one
