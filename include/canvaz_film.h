@@ -209,7 +209,10 @@ enum {
     CZ_BUF_POOL = 29,       /* f32 [2][pw*ph] */
     CZ_BUF_TILE_LIST = 30,  /* u32 [tiles16] (ascending) */
     CZ_BUF_ACC_S = 31,      /* f32 [n][4][4] */
-    CZ_BUF_ALPHA_S = 32     /* f32 [n][4] */
+    CZ_BUF_ALPHA_S = 32,    /* f32 [n][4] */
+    CZ_BUF_DEP_START = 33,  /* u32 [tiles64]: first item of each 64-tile (upload_deposits) */
+    CZ_BUF_DEP_COUNT = 34,  /* u32 [tiles64]: its item count */
+    CZ_BUF_DEP_ITEMS = 35   /* u32 [CZ_MAX_TILE_ITEMS]: deposit indices, per tile ascending */
 };
 
 typedef struct cz_film cz_film;
@@ -335,6 +338,12 @@ int cz_film_use_graphs(cz_film* f, int on);
  * Results are identical; no counterpart in the reference, whose encoder
  * records the passes of every substep. */
 int cz_film_substep_launch(cz_film* f, int mode);
+/* Where upload_deposits' tile binning (src/film/mod.rs:1175-1224) runs: 0 on
+ * the host (as the reference), 1 on the device (three kernels in the frame:
+ * rectangles and counts, an ordered scan, an ordered fill), -1 = by deposit
+ * count (default: the device from 1024 deposits in a frame).  The bins are
+ * identical either way; the host keeps the volume and height accounting. */
+int cz_film_bin_on_device(cz_film* f, int mode);
 /* Frame graphs captured and frames replayed from them so far. */
 int cz_film_graph_counts(const cz_film* f, uint64_t* captures, uint64_t* replays);
 /* Shape of the flux and limiter + apply passes: 0 = flux with one block per

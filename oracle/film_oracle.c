@@ -2248,10 +2248,13 @@ uint32_t of_upload_deposits(ofilm* F, const cz_deposit* deps, uint32_t n) {
             F->total_deposited += (double)d->vol * (double)of_inside_fraction(d, F->s.width_m, F->s.height_m);
         float minx = fminf(d->p0[0], d->p1[0]) - rb, maxx = fmaxf(d->p0[0], d->p1[0]) + rb;
         float miny = fminf(d->p0[1], d->p1[1]) - rb, maxy = fmaxf(d->p0[1], d->p1[1]) + rb;
-        int64_t tx0 = (int64_t)fmaxf(floorf(minx / tile_m), 0.0f), tx1 = (int64_t)fmaxf(floorf(maxx / tile_m), 0.0f);
-        int64_t ty0 = (int64_t)fmaxf(floorf(miny / tile_m), 0.0f), ty1 = (int64_t)fmaxf(floorf(maxy / tile_m), 0.0f);
+        /* ((x / tile_m).floor().max(0.0) as i64).min(m) (src/film/mod.rs:1196-1199):
+         * Rust's float-to-int cast saturates, so clamp to m before converting */
         int64_t mx = (int64_t)F->t64x - 1, my = (int64_t)F->t64y - 1;
-        if (tx0 > mx) tx0 = mx; if (tx1 > mx) tx1 = mx; if (ty0 > my) ty0 = my; if (ty1 > my) ty1 = my;
+        int64_t tx0 = (int64_t)fminf(fmaxf(floorf(minx / tile_m), 0.0f), (float)mx);
+        int64_t tx1 = (int64_t)fminf(fmaxf(floorf(maxx / tile_m), 0.0f), (float)mx);
+        int64_t ty0 = (int64_t)fminf(fmaxf(floorf(miny / tile_m), 0.0f), (float)my);
+        int64_t ty1 = (int64_t)fminf(fmaxf(floorf(maxy / tile_m), 0.0f), (float)my);
         if (maxx < 0.0f || maxy < 0.0f) continue;
         rect[4 * di] = (int32_t)tx0; rect[4 * di + 1] = (int32_t)tx1; rect[4 * di + 2] = (int32_t)ty0; rect[4 * di + 3] = (int32_t)ty1;
         for (int64_t ty = ty0; ty <= ty1; ty++)

@@ -94,6 +94,7 @@ def lib():
         "cz_f16_selftest": (i32, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "cz_film_substep_launch": (i32, [P, i32]),
         "cz_film_save_canvaz": (i32, [P, C.c_char_p]),
+        "cz_film_bin_on_device": (i32, [P, i32]),
         "cz_film_load_canvaz": (i32, [P, C.c_char_p]),
         "cz_lz4_compress": (i32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
         "cz_lz4_decompress": (i32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
@@ -252,9 +253,10 @@ BUFFER_IDS = {
     "h": 0, "m": 1, "rheo": 2, "strata_h": 3, "strata_pig": 4, "strata_cure": 5, "sub_refl": 6, "lay": 7,
     "delta": 16, "fnorm": 17, "pres": 18, "h_frame": 19, "cg_m": 20, "cg_p": 21, "tile_moved": 22,
     "tile_flags": 23, "tmp": 24, "flux": 25, "acc": 26, "alpha": 27, "stats": 28, "pool": 29, "tile_list": 30,
-    "acc_s": 31, "alpha_s": 32,
+    "acc_s": 31, "alpha_s": 32, "dep_start": 33, "dep_count": 34, "dep_items": 35,
 }
-_U32 = {"strata_pig", "sub_refl", "lay", "delta", "tile_moved", "tile_flags", "tile_list"}
+_U32 = {"strata_pig", "sub_refl", "lay", "delta", "tile_moved", "tile_flags", "tile_list", "dep_start", "dep_count",
+        "dep_items"}
 _CELL_SHAPE = {"h": (), "m": (), "rheo": (2,), "strata_h": (4,), "strata_pig": (4, 8), "strata_cure": (4,),
                "sub_refl": (8,), "lay": (), "delta": (), "fnorm": (2,), "pres": (2,), "h_frame": (), "cg_m": (4,),
                "cg_p": (), "tmp": (), "flux": (4,), "acc": (4,), "alpha": (), "acc_s": (4, 4), "alpha_s": (4,)}
@@ -415,6 +417,10 @@ class Film:
         a = np.empty(n, np.float32)
         check(lib().cz_film_pool(self._p, _fp(a), n))
         return a
+
+    def bin_on_device(self, mode):
+        """Deposit binning: -1 by deposit count, 0 on the host, 1 on the device."""
+        check(lib().cz_film_bin_on_device(self._p, mode))
 
     def save_canvaz(self, path):
         """Write the film as a painting file (save_painting, src/session.rs:248-280)."""

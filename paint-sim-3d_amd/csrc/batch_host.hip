@@ -284,7 +284,7 @@ int record(cz_batch* b, const FrameShape& sh) {
     hipLaunchKernelGGL(k_strata_limit<ManyFilms>, ga, b256, 0, st, w);
     hipLaunchKernelGGL(k_strata_update<ManyFilms>, ga, b256, 0, st, w);
     hipLaunchKernelGGL(k_strata_fill<ManyFilms>, ga, b256, 0, st, w);
-    hipLaunchKernelGGL(k_frame_stats<ManyCanvases>, dim3(sh.members), b256, 0, st,
+    hipLaunchKernelGGL(k_frame_stats<ManyCanvases>, dim3(sh.members), dim3(1024), 0, st,
                        ManyCanvases{b->desc, nullptr, nullptr, 3u});
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(b->frame_out_h, b->frame_out, (size_t)sh.members * 16 * sizeof(float),

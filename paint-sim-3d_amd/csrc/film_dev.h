@@ -79,13 +79,16 @@ struct FilmDev {
     const float* pigtab;
     // deposits
     const cz_deposit* deposits;
-    const uint32_t* tile_start;
-    const uint32_t* tile_count;
-    const uint32_t* tile_items;
+    uint32_t* tile_start;   // per 64-tile: first item (written by the binning, read by k_deposit)
+    uint32_t* tile_count;
+    uint32_t* tile_items;
     float* frame_out;     // [16] per-frame scalars read by the host
     uint32_t* grid_sync;  // [4] barrier of the one-launch substeps: count, generation, timeout
     uint8_t* tile_bits;   // [nt16] compaction: 1 active, 2 cure
     uint32_t* compact_cnt; // [2 per 256 tiles] compaction: active and cure counts per block
+    int32_t* dep_rect;     // [CZ_MAX_DEPOSITS][4] device binning: 64-tile rectangle per deposit (-1: none)
+    uint32_t* dep_cnt;     // [tiles64] device binning: deposits per tile (zero between frames)
+    uint32_t* dep_n;       // [1] device binning: the frame's deposit count
 };
 
 __device__ __forceinline__ uint32_t idx(const FilmDev& F, uint32_t i, uint32_t j) { return j * F.W + i; }

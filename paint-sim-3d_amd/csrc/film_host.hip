@@ -178,9 +178,13 @@ int upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* se
     if (rc) return rc;
     uint32_t count = std::min<uint32_t>(n, CZ_MAX_DEPOSITS);
     f->deposit_count = count;
+    // the tile binning on the device for many deposits (k_dep_rects, k_dep_scan,
+    // k_dep_fill in the frame); the volume and height accounting stays here
+    const bool on_dev = count > 0 && (f->bin_mode == 1 || (f->bin_mode < 0 && count >= czi::kDevBinMin));
+    f->bin_on_device = on_dev;
     uint32_t nt = f->t64x * f->t64y;
-    std::vector<uint32_t> cnt(nt, 0u);
-    std::vector<int32_t> rect((size_t)count * 4, -1);
+    std::vector<uint32_t> cnt(on_dev ? 0 : nt, 0u);
+    std::vector<int32_t> rect(on_dev ? 0 : (size_t)count * 4, -1);
     float tile_m = 64.0f * f->dx;
     for (uint32_t di = 0; di < count; di++) {
         const cz_deposit& d = deps[di];
@@ -191,38 +195,47 @@ int upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* se
         if (!inside) continue;
         if ((d.kind & CZ_DEPOSIT_RECYCLED) == 0u)
             f->total_deposited += (double)d.vol * (double)czh::inside_fraction(d, f->s.width_m, f->s.height_m);
+        if (on_dev) continue;
         float minx = fminf(d.p0[0], d.p1[0]) - rb, maxx = fmaxf(d.p0[0], d.p1[0]) + rb;
         float miny = fminf(d.p0[1], d.p1[1]) - rb, maxy = fmaxf(d.p0[1], d.p1[1]) + rb;
         int64_t mx = (int64_t)f->t64x - 1, my = (int64_t)f->t64y - 1;
-        int64_t tx0 = std::min((int64_t)fmaxf(floorf(minx / tile_m), 0.0f), mx);
-        int64_t tx1 = std::min((int64_t)fmaxf(floorf(maxx / tile_m), 0.0f), mx);
-        int64_t ty0 = std::min((int64_t)fmaxf(floorf(miny / tile_m), 0.0f), my);
-        int64_t ty1 = std::min((int64_t)fmaxf(floorf(maxy / tile_m), 0.0f), my);
+        // ((x / tile_m).floor().max(0.0) as i64).min(m): Rust's float-to-int
+        // cast saturates, so the clamp to m comes first here
+        int64_t tx0 = (int64_t)fminf(fmaxf(floorf(minx / tile_m), 0.0f), (float)mx);
+        int64_t tx1 = (int64_t)fminf(fmaxf(floorf(maxx / tile_m), 0.0f), (float)mx);
+        int64_t ty0 = (int64_t)fminf(fmaxf(floorf(miny / tile_m), 0.0f), (float)my);
+        int64_t ty1 = (int64_t)fminf(fmaxf(floorf(maxy / tile_m), 0.0f), (float)my);
         if (maxx < 0.0f || maxy < 0.0f) continue;
         rect[4 * di] = (int32_t)tx0; rect[4 * di + 1] = (int32_t)tx1;
         rect[4 * di + 2] = (int32_t)ty0; rect[4 * di + 3] = (int32_t)ty1;
         for (int64_t ty = ty0; ty <= ty1; ty++)
             for (int64_t tx = tx0; tx <= tx1; tx++) cnt[(size_t)(ty * f->t64x + tx)]++;
     }
-    uint32_t cap = CZ_MAX_TILE_ITEMS, items = 0;
-    for (uint32_t t = 0; t < nt; t++) {
-        uint32_t take = std::min(cnt[t], cap - items);
-        f->tstart_h[t] = items;
-        f->tcount_h[t] = take;
-        items += take;
-    }
-    std::vector<uint32_t> fill(nt, 0u);
-    for (uint32_t di = 0; di < count; di++) {
-        if (rect[4 * di] < 0) continue;
-        for (int32_t ty = rect[4 * di + 2]; ty <= rect[4 * di + 3]; ty++)
-            for (int32_t tx = rect[4 * di]; tx <= rect[4 * di + 1]; tx++) {
-                uint32_t t = (uint32_t)ty * f->t64x + (uint32_t)tx;
-                if (fill[t] < f->tcount_h[t]) f->titems_h[f->tstart_h[t] + fill[t]++] = di;
-            }
-    }
     if (count > 0) {  // the deposit pass runs only when there are deposits
         memcpy(f->dep_h, deps, (size_t)count * sizeof(cz_deposit));
         HIP_TRY(hipMemcpyAsync(f->deposits, f->dep_h, (size_t)count * sizeof(cz_deposit), hipMemcpyHostToDevice, f->stream));
+        if (on_dev) {
+            *f->dep_n_h = count;
+            HIP_TRY(hipMemcpyAsync(f->dep_n, f->dep_n_h, sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+        }
+    }
+    if (count > 0 && !on_dev) {
+        uint32_t cap = CZ_MAX_TILE_ITEMS, items = 0;
+        for (uint32_t t = 0; t < nt; t++) {
+            uint32_t take = std::min(cnt[t], cap - items);
+            f->tstart_h[t] = items;
+            f->tcount_h[t] = take;
+            items += take;
+        }
+        std::vector<uint32_t> fill(nt, 0u);
+        for (uint32_t di = 0; di < count; di++) {
+            if (rect[4 * di] < 0) continue;
+            for (int32_t ty = rect[4 * di + 2]; ty <= rect[4 * di + 3]; ty++)
+                for (int32_t tx = rect[4 * di]; tx <= rect[4 * di + 1]; tx++) {
+                    uint32_t t = (uint32_t)ty * f->t64x + (uint32_t)tx;
+                    if (fill[t] < f->tcount_h[t]) f->titems_h[f->tstart_h[t] + fill[t]++] = di;
+                }
+        }
         HIP_TRY(hipMemcpyAsync(f->tile_start, f->tstart_h, nt * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
         HIP_TRY(hipMemcpyAsync(f->tile_count, f->tcount_h, nt * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
         if (items > 0) HIP_TRY(hipMemcpyAsync(f->tile_items, f->titems_h, items * sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
@@ -244,6 +257,12 @@ int record_frame(cz_film* f, uint32_t substeps) {
     const uint32_t n64 = f->t64x * f->t64y;
     FilmDev F = f->dev();
     if (f->deposit_count > 0) {
+        if (f->bin_on_device) {
+            const uint32_t nt64 = f->t64x * f->t64y;
+            LAUNCH("dep_bin", k_dep_rects, dim3(kDepRectBlocks), b256, F, f->s.width_m, f->s.height_m, 64.0f * f->dx);
+            LAUNCH("dep_bin", k_dep_scan, dim3(1), dim3(1024), F);
+            LAUNCH("dep_bin", k_dep_fill, dim3((nt64 + 3u) / 4u), b256, F);
+        }
         const dim3 gcell((f->W + 15u) / 16u, (f->H + 15u) / 16u);
         LAUNCH("deposit", k_deposit, gcell, b256, F);
     }
@@ -332,7 +351,7 @@ int record_frame(cz_film* f, uint32_t substeps) {
     LAUNCH("strata_limit", k_strata_limit<OneFilm>, gt, b256, w);
     LAUNCH("strata_update", k_strata_update<OneFilm>, gt, b256, w);
     LAUNCH("strata_fill", k_strata_fill<OneFilm>, gt, b256, w);
-    LAUNCH("frame_stats", k_frame_stats<OneCanvas>, dim3(1), b256, OneCanvas{w.F});
+    LAUNCH("frame_stats", k_frame_stats<OneCanvas>, dim3(1), dim3(1024), OneCanvas{w.F});
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(f->frame_out_h, f->frame_out, 16 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
     return CZ_OK;
@@ -365,7 +384,7 @@ int encode_frame(cz_film* f, uint32_t substeps) {
     key.substeps = substeps;
     key.implicit_steps = f->implicit_steps;
     key.implicit_on = f->s.implicit_hydro;
-    key.has_dep = f->deposit_count > 0;
+    key.has_dep = f->deposit_count == 0 ? 0u : (f->bin_on_device ? 2u : 1u);
     key.parity = f->delta_cur_a ? 1u : 0u;
     key.grid1 = f->tile_grid(1);
     key.grid4 = f->tile_grid(4);
@@ -459,6 +478,9 @@ BufDesc buffer_of(const cz_film* f, int which) {
     case CZ_BUF_TILE_LIST: return {f->tile_list, 4 * (uint64_t)f->nt16};
     case CZ_BUF_ACC_S: return {f->acc_s, 64 * n};
     case CZ_BUF_ALPHA_S: return {f->alpha_s, 16 * n};
+    case CZ_BUF_DEP_START: return {f->tile_start, 4 * (uint64_t)f->t64x * f->t64y};
+    case CZ_BUF_DEP_COUNT: return {f->tile_count, 4 * (uint64_t)f->t64x * f->t64y};
+    case CZ_BUF_DEP_ITEMS: return {f->tile_items, 4 * (uint64_t)CZ_MAX_TILE_ITEMS};
     default: return {nullptr, 0};
     }
 }
@@ -490,6 +512,7 @@ void destroy(cz_film* f) {
     if (f->tstart_h) hipHostFree(f->tstart_h);
     if (f->tcount_h) hipHostFree(f->tcount_h);
     if (f->titems_h) hipHostFree(f->titems_h);
+    if (f->dep_n_h) hipHostFree(f->dep_n_h);
     for (auto& s : f->prof_pending) {
         hipEventDestroy(s.a);
         hipEventDestroy(s.b);
@@ -591,7 +614,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     A(acc_s, 4 * n); A(alpha_s, n); A(strata_h2, n); A(strata_pig2, 8 * n); A(strata_cure2, n); A(h_frame, n);
     A(fnorm, n); A(cg_m, n); A(cg_d, n); A(cg_x, n); A(cg_r, n); A(cg_p, n); A(cg_q, n); A(cg_t, n); A(cg_t2, n);
     A(cg_part, nt16); A(cg_part2, nt16); A(cg_s, 8); A(tile_flags, nt16); A(tile_list, nt16); A(tile_moved, nt16);
-    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(tile_bits, nt16); A(compact_cnt, 2 * ((nt16 + 255) / 256) + 2); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
+    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(tile_bits, nt16); A(dep_rect, 4 * (size_t)CZ_MAX_DEPOSITS); A(dep_cnt, nt64); A(dep_n, 4); A(compact_cnt, 2 * ((nt16 + 255) / 256) + 2); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
     A(pigtab, CZ_NPIG * CZ_PIG_STRIDE); A(deposits, CZ_MAX_DEPOSITS); A(tile_start, nt64); A(tile_count, nt64);
     A(tile_items, CZ_MAX_TILE_ITEMS); A(frame_out, 16);
 #undef A
@@ -601,6 +624,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
         if (e == hipSuccess) e = hipHostMalloc((void**)&f->tstart_h, nt64 * 4, hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc((void**)&f->tcount_h, nt64 * 4, hipHostMallocDefault);
         if (e == hipSuccess) e = hipHostMalloc((void**)&f->titems_h, CZ_MAX_TILE_ITEMS * 4, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&f->dep_n_h, 4, hipHostMallocDefault);
         if (e != hipSuccess) rc = set_err(CZ_ENOMEM, "hipHostMalloc: %s", hipGetErrorString(e));
     }
     if (!rc) {
@@ -929,6 +953,12 @@ int cz_film_clear(cz_film* f) {
 }
 
 void* cz_film_stream(cz_film* f) { return f ? (void*)f->stream : nullptr; }
+
+int cz_film_bin_on_device(cz_film* f, int mode) {
+    if (!f || mode < -1 || mode > 1) return set_err(CZ_EINVAL, "bin_on_device: -1, 0 or 1");
+    f->bin_mode = mode;
+    return CZ_OK;
+}
 
 int cz_film_substep_launch(cz_film* f, int mode) {
     if (!f || mode < -1 || mode > 1) return set_err(CZ_EINVAL, "substep_launch: -1, 0 or 1");

@@ -40,6 +40,8 @@ constexpr uint32_t kMaxTileBlocks = 2048u;  // resident 256-thread blocks on 256
 // against 3.3 us for a dependent launch inside a hipGraph, so the three
 // launches per substep are faster at every active-set size measured.
 constexpr uint32_t kPersistTiles = 0u, kPersistBlocks = 256u;
+// deposit binning on the device from this many deposits in a frame
+constexpr uint32_t kDevBinMin = 1024u;
 
 struct ProfSlot {
     std::string label;
@@ -184,6 +186,12 @@ struct cz_film {
     uint32_t* grid_sync = nullptr;
     uint8_t* tile_bits = nullptr;
     uint32_t* compact_cnt = nullptr;
+    int32_t* dep_rect = nullptr;   // [CZ_MAX_DEPOSITS][4] device binning: 64-tile rectangle per deposit
+    uint32_t* dep_cnt = nullptr;   // [tiles64] device binning: counts (zero between frames)
+    uint32_t* dep_n = nullptr;     // [1] device binning: the frame's deposit count
+    uint32_t* dep_n_h = nullptr;   // pinned staging of it
+    int bin_mode = -1;             // deposit binning: -1 by count, 0 host, 1 device
+    bool bin_on_device = false;    // the uploaded deposits are binned in the frame
     float2* stats_out = nullptr;
     float* pool_out = nullptr;
     float* tsum_a = nullptr;
@@ -229,7 +237,7 @@ struct cz_film {
         F.cg_part = cg_part; F.cg_part2 = cg_part2; F.cg_s = cg_s;
         F.tile_flags = tile_flags; F.tile_list = tile_list; F.tile_moved = tile_moved; F.active_count = active_count;
         F.cure_list = cure_list; F.cure_count = cure_count;
-        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.tile_bits = tile_bits; F.compact_cnt = compact_cnt; F.pigtab = pigtab;
+        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.tile_bits = tile_bits; F.compact_cnt = compact_cnt; F.dep_rect = dep_rect; F.dep_cnt = dep_cnt; F.dep_n = dep_n; F.pigtab = pigtab;
         F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
         F.frame_out = frame_out;
         return F;

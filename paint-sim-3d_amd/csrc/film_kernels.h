@@ -117,6 +117,10 @@ __global__ void k_deposit(FilmDev F);
 __global__ void k_substeps(FilmDev Fa, FilmDev Fb, uint32_t substeps, int clear_first);
 __global__ void k_compact_bits(FilmDev F);
 __global__ void k_compact_list(FilmDev F);
+constexpr uint32_t kDepRectBlocks = 64u;  // grid of k_dep_rects (a grid-stride loop over the deposits)
+__global__ void k_dep_rects(FilmDev F, float width_m, float height_m, float tile_m);
+__global__ void k_dep_scan(FilmDev F);
+__global__ void k_dep_fill(FilmDev F);
 __global__ void k_batch_plan(BatchPlan B);
 
 #define CZ_TILE_KERNELS(X)                                          \

@@ -57,6 +57,91 @@ __device__ inline float ribbon_height(float px, float py, const cz_deposit& d) {
     return (xx < 1.0f) ? d.h0 * (rt / d.r0) * (1.0f - xx * xx) : 0.0f;
 }
 
+// upload_deposits' tile binning (src/film/mod.rs:1175-1224) on the device,
+// with the host's float arithmetic for the rectangles.  k_dep_rects: one
+// thread per deposit, its 64-tile rectangle (or none) and the tile counts;
+// k_dep_scan: the tiles' first items in tile order under the item cap, and
+// the counts cleared for the next frame; k_dep_fill: one wave per tile scans
+// the deposits in index order and appends the ones covering it (ballot
+// prefix), so every tile's items are ascending as the host writes them.
+__global__ __launch_bounds__(256) void k_dep_rects(FilmDev F, float width_m, float height_m, float tile_m) {
+    const uint32_t count = *F.dep_n;
+    for (uint32_t di = blockIdx.x * 256u + threadIdx.x; di < count; di += gridDim.x * 256u) {
+    const cz_deposit& d = F.deposits[di];
+    int32_t r[4] = {-1, -1, -1, -1};
+    const float rb = fmaxf(d.r0, d.r1);
+    const bool inside = fmaxf(d.p0[0], d.p1[0]) + rb > 0.0f && fmaxf(d.p0[1], d.p1[1]) + rb > 0.0f &&
+                        fminf(d.p0[0], d.p1[0]) - rb < width_m && fminf(d.p0[1], d.p1[1]) - rb < height_m;
+    if (inside) {
+        const float minx = fminf(d.p0[0], d.p1[0]) - rb, maxx = fmaxf(d.p0[0], d.p1[0]) + rb;
+        const float miny = fminf(d.p0[1], d.p1[1]) - rb, maxy = fmaxf(d.p0[1], d.p1[1]) + rb;
+        const int64_t mx = (int64_t)F.t64x - 1, my = (int64_t)F.t64y - 1;
+        const int64_t tx0 = (int64_t)fminf(fmaxf(floorf(minx / tile_m), 0.0f), (float)mx);
+        const int64_t tx1 = (int64_t)fminf(fmaxf(floorf(maxx / tile_m), 0.0f), (float)mx);
+        const int64_t ty0 = (int64_t)fminf(fmaxf(floorf(miny / tile_m), 0.0f), (float)my);
+        const int64_t ty1 = (int64_t)fminf(fmaxf(floorf(maxy / tile_m), 0.0f), (float)my);
+        if (!(maxx < 0.0f || maxy < 0.0f)) {
+            r[0] = (int32_t)tx0; r[1] = (int32_t)tx1; r[2] = (int32_t)ty0; r[3] = (int32_t)ty1;
+            for (int64_t ty = ty0; ty <= ty1; ty++)
+                for (int64_t tx = tx0; tx <= tx1; tx++) atomicAdd(&F.dep_cnt[(uint32_t)ty * F.t64x + (uint32_t)tx], 1u);
+        }
+    }
+    *(int4*)(F.dep_rect + 4u * di) = make_int4(r[0], r[1], r[2], r[3]);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_dep_scan(FilmDev F) {
+    __shared__ uint64_t s_sum[1024];
+    const uint32_t nt = F.t64x * F.t64y;
+    const uint32_t run = (nt + 1023u) / 1024u;
+    const uint32_t t0 = threadIdx.x * run, t1 = t0 + run < nt ? t0 + run : nt;
+    uint64_t own = 0u;
+    for (uint32_t t = t0; t < t1; t++) own += F.dep_cnt[t];
+    s_sum[threadIdx.x] = own;
+    __syncthreads();
+    for (uint32_t off = 1u; off < 1024u; off <<= 1) {
+        const uint64_t v = threadIdx.x >= off ? s_sum[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_sum[threadIdx.x] += v;
+        __syncthreads();
+    }
+    // items before this thread's run, then the host's take = min(count, cap - items)
+    uint64_t items = s_sum[threadIdx.x] - own;
+    const uint64_t cap = CZ_MAX_TILE_ITEMS;
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint64_t c = F.dep_cnt[t];
+        const uint64_t start = items < cap ? items : cap;
+        const uint64_t end = items + c < cap ? items + c : cap;
+        F.tile_start[t] = (uint32_t)start;
+        F.tile_count[t] = (uint32_t)(end - start);
+        F.dep_cnt[t] = 0u;
+        items += c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dep_fill(FilmDev F) {
+    const uint32_t count = *F.dep_n;
+    const uint32_t t = blockIdx.x * 4u + (threadIdx.x >> 6), lane = threadIdx.x & 63u;
+    if (t >= F.t64x * F.t64y) return;
+    const uint32_t want = F.tile_count[t], base = F.tile_start[t];
+    if (want == 0u) return;
+    const int32_t tx = (int32_t)(t % F.t64x), ty = (int32_t)(t / F.t64x);
+    const unsigned long long lmask = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+    uint32_t filled = 0u;
+    for (uint32_t d0 = 0; d0 < count && filled < want; d0 += 64u) {
+        const uint32_t di = d0 + lane;
+        bool hit = false;
+        if (di < count) {
+            const int4 r = *(const int4*)(F.dep_rect + 4u * di);
+            hit = r.x >= 0 && tx >= r.x && tx <= r.y && ty >= r.z && ty <= r.w;
+        }
+        const unsigned long long b = __ballot(hit);
+        const uint32_t pos = filled + (uint32_t)__popcll(b & lmask);
+        if (hit && pos < want) F.tile_items[base + pos] = di;
+        filled += (uint32_t)__popcll(b);
+    }
+}
+
 // deposit.wgsl:31-75 (full grid, 16x16 blocks)
 __global__ __launch_bounds__(256) void k_deposit(FilmDev F) {
     uint32_t i = blockIdx.x * 16u + (threadIdx.x & 15u);
@@ -1417,44 +1502,52 @@ __global__ __launch_bounds__(256) void k_reduce_pool(WS ws) {
 // frame scalars the host reads back: [0] h_max, [1] mu_min, [2] active count
 // (bits), [3] cells of the active tiles (bits), [4..7] cg_s[1].
 template <class CS>
-__global__ __launch_bounds__(256) void k_frame_stats(CS cs) {
+__global__ __launch_bounds__(1024) void k_frame_stats(CS cs) {
+    // one 1024-thread block per canvas: each thread takes a strided share of
+    // the 64-tile stats and of the active list, then wave and block
+    // reductions (max and min are exact in any order, the cell count is an
+    // integer sum)
     const FilmDev& F = cs.film(blockIdx.x);
-    __shared__ float s_h[256];
-    __shared__ float s_mu[256];
-    __shared__ uint32_t s_cells;
-    if (threadIdx.x == 0) s_cells = 0u;
-    uint32_t nt = F.t64x * F.t64y;
+    __shared__ float s_h[16], s_mu[16];
+    __shared__ uint32_t s_cells[16];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t nt = F.t64x * F.t64y;
     float hm = 0.0f, mm = 1e9f;
-    for (uint32_t t = threadIdx.x; t < nt; t += 256u) {
-        float2 v = F.stats_out[t];
+    for (uint32_t t = threadIdx.x; t < nt; t += 1024u) {
+        const float2 v = F.stats_out[t];
         hm = cz_max(hm, v.x);
         mm = cz_min(mm, v.y);
     }
-    uint32_t cells = 0u, nact = *F.active_count;
-    for (uint32_t a = threadIdx.x; a < nact; a += 256u) {
-        uint32_t t = F.tile_list[a];
-        uint32_t tx = t % F.tw, ty = t / F.tw;
-        uint32_t cw = F.W - tx * 16u < 16u ? F.W - tx * 16u : 16u;
-        uint32_t ch = F.H - ty * 16u < 16u ? F.H - ty * 16u : 16u;
+    uint32_t cells = 0u;
+    const uint32_t nact = *F.active_count;
+    for (uint32_t a = threadIdx.x; a < nact; a += 1024u) {
+        const uint32_t t = F.tile_list[a];
+        const uint32_t tx = t % F.tw, ty = t / F.tw;
+        const uint32_t cw = F.W - tx * 16u < 16u ? F.W - tx * 16u : 16u;
+        const uint32_t ch = F.H - ty * 16u < 16u ? F.H - ty * 16u : 16u;
         cells += cw * ch;
     }
-    __syncthreads();
-    atomicAdd(&s_cells, cells);
-    s_h[threadIdx.x] = hm;
-    s_mu[threadIdx.x] = mm;
-    __syncthreads();
-    for (uint32_t s = 128u; s > 0u; s >>= 1) {
-        if (threadIdx.x < s) {
-            s_h[threadIdx.x] = cz_max(s_h[threadIdx.x], s_h[threadIdx.x + s]);
-            s_mu[threadIdx.x] = cz_min(s_mu[threadIdx.x], s_mu[threadIdx.x + s]);
-        }
-        __syncthreads();
+    hm = wave_max(hm);
+    mm = wave_min(mm);
+    for (int off = 32; off > 0; off >>= 1) cells += __shfl_xor(cells, off, 64);
+    if (lane == 0u) {
+        s_h[wave] = hm;
+        s_mu[wave] = mm;
+        s_cells[wave] = cells;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        F.frame_out[0] = s_h[0];
-        F.frame_out[1] = s_mu[0];
-        F.frame_out[2] = cz_bits2f(*F.active_count);
-        F.frame_out[3] = cz_bits2f(s_cells);
+        float h = s_h[0], mu = s_mu[0];
+        uint32_t cs_ = s_cells[0];
+        for (int w = 1; w < 16; w++) {
+            h = cz_max(h, s_h[w]);
+            mu = cz_min(mu, s_mu[w]);
+            cs_ += s_cells[w];
+        }
+        F.frame_out[0] = h;
+        F.frame_out[1] = mu;
+        F.frame_out[2] = cz_bits2f(nact);
+        F.frame_out[3] = cz_bits2f(cs_);
         for (int k = 0; k < 4; k++) F.frame_out[4 + k] = F.cg_s[4 + k];
         F.frame_out[15] = cz_bits2f(F.grid_sync ? F.grid_sync[2] : 0u);
     }

@@ -44,8 +44,8 @@ uint64_t film_bytes_for(uint32_t w, uint32_t h) {
     const uint64_t np = (uint64_t)((w + 3u) / 4u) * ((h + 3u) / 4u);
     const uint64_t per_cell = 4 + 4 + 8 + 16 + 128 + 16 + 32 + 4 + 8 + 16 + 16 + 4 + 4 + 4 + 4 + 4 + 64 + 16 + 16 +
                               128 + 16 + 4 + 8 + 16 + 7 * 4 + 4;
-    return n * per_cell + nt16 * (8 + 8 + 4 + 4 + 4 + 4 + 1 + 1) + nt64 * (8 + 4 + 4) + np * 8 +
-           (uint64_t)CZ_MAX_DEPOSITS * sizeof(cz_deposit) + (uint64_t)CZ_MAX_TILE_ITEMS * 4 + 64 * 1024;
+    return n * per_cell + nt16 * (8 + 8 + 4 + 4 + 4 + 4 + 1 + 1) + nt64 * (8 + 4 + 4 + 4) + np * 8 +
+           (uint64_t)CZ_MAX_DEPOSITS * (sizeof(cz_deposit) + 16) + (uint64_t)CZ_MAX_TILE_ITEMS * 4 + 64 * 1024;
 }
 
 bool fits(const cz_film* f, uint32_t w, uint32_t h) {
