@@ -212,7 +212,9 @@ enum {
     CZ_BUF_ALPHA_S = 32,    /* f32 [n][4] */
     CZ_BUF_DEP_START = 33,  /* u32 [tiles64]: first item of each 64-tile (upload_deposits) */
     CZ_BUF_DEP_COUNT = 34,  /* u32 [tiles64]: its item count */
-    CZ_BUF_DEP_ITEMS = 35   /* u32 [CZ_MAX_TILE_ITEMS]: deposit indices, per tile ascending */
+    CZ_BUF_DEP_ITEMS = 35,  /* u32 [CZ_MAX_TILE_ITEMS]: deposit indices, per tile ascending */
+    CZ_BUF_COLOR = 36,      /* f16 [n][4]: shade color_out (linear rgb, roughness); with cz_film_shade_output */
+    CZ_BUF_DETAIL = 37      /* f16 [n][4]: shade detail_out (method + turn, cure, yield stress, thickness) */
 };
 
 typedef struct cz_film cz_film;
@@ -338,6 +340,13 @@ int cz_film_use_graphs(cz_film* f, int on);
  * Results are identical; no counterpart in the reference, whose encoder
  * records the passes of every substep. */
 int cz_film_substep_launch(cz_film* f, int mode);
+/* The display pass (shade_full, shaders/shade.wgsl:29-83, src/film/mod.rs:
+ * 1449-1456) at the end of every frame: 1 = on, 0 = off (default; the
+ * reference runs it every frame for its renderer).  Its two outputs, the
+ * color and detail textures as f16 x 4 per cell (round to nearest even),
+ * are read with CZ_BUF_COLOR and CZ_BUF_DETAIL; they are allocated when
+ * shading is first turned on. */
+int cz_film_shade_output(cz_film* f, int on);
 /* Where upload_deposits' tile binning (src/film/mod.rs:1175-1224) runs: 0 on
  * the host (as the reference), 1 on the device (three kernels in the frame:
  * rectangles and counts, an ordered scan, an ordered fill), -1 = by deposit

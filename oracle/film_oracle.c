@@ -770,6 +770,87 @@ static void layer_ks(const ofilm* F, const float* pig, uint32_t band, float* ko,
     *so = s * sc;
 }
 
+/* shade.wgsl:29-83 (shade_cell) with km.wgsl:35-62 (load_refl,
+ * spectrum_to_rgb): per cell, the substrate reflectance composited with
+ * every stratum's KM layer, the surface (Saunderson) correction, CIE 1931 x
+ * D65 to linear sRGB, roughness and the detail channels; both outputs as
+ * f16 (round to nearest even) x 4 per cell. */
+static inline float of_smoothstep(float e0, float e1, float x) {
+    float t = cz_clamp((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+static void shade_cell(const ofilm* F, uint32_t c, uint16_t* color, uint16_t* detail) {
+    float r[NBAND];
+    for (uint32_t b = 0; b < 8u; b++) {
+        r[2u * b] = cz_unpack_lo(F->sub_refl[c * 8u + b]);
+        r[2u * b + 1u] = cz_unpack_hi(F->sub_refl[c * 8u + b]);
+    }
+    const float hv = F->h[c];
+    float wet = 0.0f, cure_top = 0.0f;
+    if (hv > 0.0f) {
+        for (uint32_t k = 0; k < 4u; k++) {
+            const float t = F->strata_h[4u * c + k];
+            if (t <= 0.0f) continue;
+            float pig[16];
+            unpack_pig(&F->strata_pig[(c * 4u + k) * 8u], pig);
+            for (uint32_t b = 0; b < NBAND; b++) {
+                float kk, ss, R, T;
+                layer_ks(F, pig, b, &kk, &ss);
+                km_layer(kk, ss, t, &R, &T);
+                r[b] = composite(R, T, r[b]);
+            }
+            wet = 1.0f - F->strata_cure[4u * c + k];
+            cure_top = F->strata_cure[4u * c + k];
+        }
+    }
+    const float k1 = 0.04f;
+    const float k2 = cz_mix(0.4f, 0.6f, wet);
+    for (uint32_t b = 0; b < NBAND; b++) r[b] = (1.0f - k1) * (1.0f - k2) * r[b] / cz_max(1.0f - k2 * r[b], 1e-3f);
+    float xyz[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t b = 0; b < NBAND; b++)
+        for (int q = 0; q < 3; q++) xyz[q] = xyz[q] + r[b] * F->cmf[b * 3u + (uint32_t)q];
+    float rgb[3] = {3.2406f * xyz[0] - 1.5372f * xyz[1] - 0.4986f * xyz[2],
+                    -0.9689f * xyz[0] + 1.8758f * xyz[1] + 0.0415f * xyz[2],
+                    0.0557f * xyz[0] - 0.2040f * xyz[1] + 1.0570f * xyz[2]};
+    for (int q = 0; q < 3; q++) rgb[q] = cz_max(rgb[q], 0.0f);
+    const float mc = F->m[c];
+    const float painted = of_smoothstep(60e-6f, 150e-6f, mc);
+    cure_top = cz_max(cure_top, painted * (1.0f - of_smoothstep(0.0f, 50e-6f, hv)));
+    float rough = cz_mix(0.06f, 0.35f, sqrtf(cure_top));
+    if (hv <= 0.0f) rough = cz_mix(0.85f, 0.35f, painted);
+    const uint32_t lv = F->lay[c];
+    const float turn = (float)((lv >> 8u) & 0xffffu) / 65536.0f;
+    const float thickness = hv + cz_max(mc - 60e-6f, 0.0f);
+    const float det[4] = {(float)(lv & 0xffu) + turn, cure_top, F->rheo[2u * c + 1u], thickness};
+    if (F->P.flags[0] == 1u && hv > 0.0f) {
+        const float* th = &F->strata_h[4u * c];
+        const float ssum = th[0] + th[1] + th[2] + th[3];
+        int bad = 0;
+        uint32_t nst = 0;
+        for (uint32_t k = 0; k < 4u; k++) {
+            if (th[k] > 0.0f) nst = k + 1u;
+            if (th[k] <= 0.0f) continue;
+            float pig[16];
+            unpack_pig(&F->strata_pig[(c * 4u + k) * 8u], pig);
+            float cs = 0.0f;
+            for (int i = 0; i < 13; i++) cs = cs + pig[i];
+            if (!(cs > 0.0f)) bad = 1;
+        }
+        if (ssum <= 0.0f) { rgb[0] = 0.0f; rgb[1] = 0.0f; rgb[2] = 1.0f; }
+        else if (bad) { rgb[0] = 1.0f; rgb[1] = 0.0f; rgb[2] = 0.0f; }
+        else if (cz_abs(ssum - hv) > 0.01f * hv) { rgb[0] = 0.0f; rgb[1] = 1.0f; rgb[2] = 0.0f; }
+        else { for (int q = 0; q < 3; q++) rgb[q] = 0.3f + 0.15f * (float)nst; }
+    }
+    for (int q = 0; q < 3; q++) color[4u * c + (uint32_t)q] = (uint16_t)cz_f2h(rgb[q]);
+    color[4u * c + 3u] = (uint16_t)cz_f2h(rough);
+    for (int q = 0; q < 4; q++) detail[4u * c + (uint32_t)q] = (uint16_t)cz_f2h(det[q]);
+}
+
+void of_shade(const ofilm* F, uint16_t* color, uint16_t* detail) {
+    for (uint32_t c = 0; c < F->n; c++) shade_cell(F, c, color, detail);
+}
+
 /* ------------------------------------------------------------------------ */
 /* cure.wgsl                                                               */
 

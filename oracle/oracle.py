@@ -72,6 +72,7 @@ def lib():
         L.of_host_state.argtypes = [P, C.POINTER(f32), C.POINTER(C.c_double)]
         L.of_set_host_state.argtypes = [P, f32, f32, f32]
         L.of_mass_report.argtypes = [P, C.POINTER(MassReport)]
+        L.of_shade.argtypes = [P, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16)]
         for k in ("of_k_mark_compact", "of_k_laplacian", "of_k_plug", "of_k_fluxes", "of_k_cure"):
             getattr(L, k).argtypes = [P]
         for name in ("of_expf", "of_logf", "of_atanf"):
@@ -221,6 +222,15 @@ class OracleFilm:
                               self.implicit_steps() if implicit_steps is None else implicit_steps,
                               s.implicit_iters if cg_iters is None else cg_iters,
                               s.implicit_hydro if run_implicit is None else run_implicit)
+
+    def shade(self):
+        """shade_cell over the grid (shade.wgsl:29-83): (color, detail) as f16
+        bits, [h, w, 4] each."""
+        color = np.zeros(self.w * self.h * 4, np.uint16)
+        detail = np.zeros(self.w * self.h * 4, np.uint16)
+        u16p = C.POINTER(C.c_uint16)
+        lib().of_shade(self._p, color.ctypes.data_as(u16p), detail.ctypes.data_as(u16p))
+        return color.reshape(self.h, self.w, 4), detail.reshape(self.h, self.w, 4)
 
     def step(self, deps=(), frame_dt=1.0 / 60.0):
         arr, n = cz_types.deposit_array(deps)

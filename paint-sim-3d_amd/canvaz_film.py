@@ -95,6 +95,7 @@ def lib():
         "cz_film_substep_launch": (i32, [P, i32]),
         "cz_film_save_canvaz": (i32, [P, C.c_char_p]),
         "cz_film_bin_on_device": (i32, [P, i32]),
+        "cz_film_shade_output": (i32, [P, i32]),
         "cz_film_load_canvaz": (i32, [P, C.c_char_p]),
         "cz_lz4_compress": (i32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
         "cz_lz4_decompress": (i32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
@@ -254,11 +255,12 @@ BUFFER_IDS = {
     "delta": 16, "fnorm": 17, "pres": 18, "h_frame": 19, "cg_m": 20, "cg_p": 21, "tile_moved": 22,
     "tile_flags": 23, "tmp": 24, "flux": 25, "acc": 26, "alpha": 27, "stats": 28, "pool": 29, "tile_list": 30,
     "acc_s": 31, "alpha_s": 32, "dep_start": 33, "dep_count": 34, "dep_items": 35,
+    "color": 36, "detail": 37,
 }
 _U32 = {"strata_pig", "sub_refl", "lay", "delta", "tile_moved", "tile_flags", "tile_list", "dep_start", "dep_count",
         "dep_items"}
 _CELL_SHAPE = {"h": (), "m": (), "rheo": (2,), "strata_h": (4,), "strata_pig": (4, 8), "strata_cure": (4,),
-               "sub_refl": (8,), "lay": (), "delta": (), "fnorm": (2,), "pres": (2,), "h_frame": (), "cg_m": (4,),
+               "sub_refl": (8,), "lay": (), "delta": (), "color": (4,), "detail": (4,), "fnorm": (2,), "pres": (2,), "h_frame": (), "cg_m": (4,),
                "cg_p": (), "tmp": (), "flux": (4,), "acc": (4,), "alpha": (), "acc_s": (4, 4), "alpha_s": (4,)}
 
 
@@ -356,8 +358,8 @@ class Film:
         which = BUFFER_IDS[name]
         nb = C.c_uint64()
         check(lib().cz_film_buffer_size(self._p, which, C.byref(nb)))
-        dt = np.uint32 if name in _U32 else np.float32
-        a = np.empty(nb.value // 4, dt)
+        dt = np.uint16 if name in ("color", "detail") else (np.uint32 if name in _U32 else np.float32)
+        a = np.empty(nb.value // np.dtype(dt).itemsize, dt)
         check(lib().cz_film_read_buffer(self._p, which, a.ctypes.data_as(C.c_void_p), nb.value))
         if name in _CELL_SHAPE:
             return a.reshape((self.h, self.w) + _CELL_SHAPE[name])
@@ -417,6 +419,10 @@ class Film:
         a = np.empty(n, np.float32)
         check(lib().cz_film_pool(self._p, _fp(a), n))
         return a
+
+    def shade_output(self, on=True):
+        """Run the shade pass at the end of every frame (buffers "color", "detail": f16 bits)."""
+        check(lib().cz_film_shade_output(self._p, 1 if on else 0))
 
     def bin_on_device(self, mode):
         """Deposit binning: -1 by deposit count, 0 on the host, 1 on the device."""

@@ -89,6 +89,9 @@ struct FilmDev {
     int32_t* dep_rect;     // [CZ_MAX_DEPOSITS][4] device binning: 64-tile rectangle per deposit (-1: none)
     uint32_t* dep_cnt;     // [tiles64] device binning: deposits per tile (zero between frames)
     uint32_t* dep_n;       // [1] device binning: the frame's deposit count
+    const float* cmf;      // [16][3] CIE 1931 x D65 per band pair (shade)
+    uint2* color;          // [n] shade output: rgb + roughness, f16 x 4
+    uint2* detail;         // [n] shade output: method + turn, cure, yield stress, thickness, f16 x 4
 };
 
 __device__ __forceinline__ uint32_t idx(const FilmDev& F, uint32_t i, uint32_t j) { return j * F.W + i; }

@@ -351,6 +351,9 @@ int record_frame(cz_film* f, uint32_t substeps) {
     LAUNCH("strata_limit", k_strata_limit<OneFilm>, gt, b256, w);
     LAUNCH("strata_update", k_strata_update<OneFilm>, gt, b256, w);
     LAUNCH("strata_fill", k_strata_fill<OneFilm>, gt, b256, w);
+    // the display pass of every frame (shade_full, src/film/mod.rs:1449-1456)
+    if (f->shade_on)
+        LAUNCH("shade", k_shade<OneFilmAll>, dim3(std::min(f->nt16, czi::kMaxTileBlocks)), b256, OneFilmAll{w.F, f->nt16});
     LAUNCH("frame_stats", k_frame_stats<OneCanvas>, dim3(1), dim3(1024), OneCanvas{w.F});
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(f->frame_out_h, f->frame_out, 16 * sizeof(float), hipMemcpyDeviceToHost, f->stream));
@@ -361,7 +364,7 @@ int record_frame(cz_film* f, uint32_t substeps) {
 // shape is everything the launches depend on (substeps, implicit steps,
 // deposit pass, offset-buffer parity, params, grid sizes).
 struct GraphKey {
-    uint32_t substeps, implicit_steps, implicit_on, has_dep, parity, grid1, grid4, flux_shape, persist;
+    uint32_t substeps, implicit_steps, implicit_on, has_dep, parity, grid1, grid4, flux_shape, persist, shade;
     cz_params P;
     bool operator==(const GraphKey& o) const { return memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
@@ -392,6 +395,7 @@ int encode_frame(cz_film* f, uint32_t substeps) {
     bool timed = false;
     key.flux_shape = pg > 0u ? 0u : (uint32_t)f->flux_tuner.choose(flux_items, false, bk, timed);
     key.persist = pg;
+    key.shade = f->shade_on ? 1u : 0u;
     key.P = f->P;
     hipGraphExec_t exec = nullptr;
     for (size_t k = 0; k < f->graphs.size(); k++) {
@@ -481,6 +485,8 @@ BufDesc buffer_of(const cz_film* f, int which) {
     case CZ_BUF_DEP_START: return {f->tile_start, 4 * (uint64_t)f->t64x * f->t64y};
     case CZ_BUF_DEP_COUNT: return {f->tile_count, 4 * (uint64_t)f->t64x * f->t64y};
     case CZ_BUF_DEP_ITEMS: return {f->tile_items, 4 * (uint64_t)CZ_MAX_TILE_ITEMS};
+    case CZ_BUF_COLOR: return {f->color, f->color ? 8 * n : 0};
+    case CZ_BUF_DETAIL: return {f->detail, f->detail ? 8 * n : 0};
     default: return {nullptr, 0};
     }
 }
@@ -614,7 +620,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     A(acc_s, 4 * n); A(alpha_s, n); A(strata_h2, n); A(strata_pig2, 8 * n); A(strata_cure2, n); A(h_frame, n);
     A(fnorm, n); A(cg_m, n); A(cg_d, n); A(cg_x, n); A(cg_r, n); A(cg_p, n); A(cg_q, n); A(cg_t, n); A(cg_t2, n);
     A(cg_part, nt16); A(cg_part2, nt16); A(cg_s, 8); A(tile_flags, nt16); A(tile_list, nt16); A(tile_moved, nt16);
-    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(tile_bits, nt16); A(dep_rect, 4 * (size_t)CZ_MAX_DEPOSITS); A(dep_cnt, nt64); A(dep_n, 4); A(compact_cnt, 2 * ((nt16 + 255) / 256) + 2); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
+    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(tile_bits, nt16); A(dep_rect, 4 * (size_t)CZ_MAX_DEPOSITS); A(dep_cnt, nt64); A(dep_n, 4); A(cmf, CZ_NBAND * 3); A(compact_cnt, 2 * ((nt16 + 255) / 256) + 2); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
     A(pigtab, CZ_NPIG * CZ_PIG_STRIDE); A(deposits, CZ_MAX_DEPOSITS); A(tile_start, nt64); A(tile_count, nt64);
     A(tile_items, CZ_MAX_TILE_ITEMS); A(frame_out, 16);
 #undef A
@@ -629,7 +635,8 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     }
     if (!rc) {
         e = hipMemcpyAsync(f->pigtab, f->pigtab_h, sizeof(f->pigtab_h), hipMemcpyHostToDevice, f->stream);
-        if (e != hipSuccess) rc = set_err(CZ_EHIP, "pigtab upload: %s", hipGetErrorString(e));
+        if (e == hipSuccess) e = hipMemcpyAsync(f->cmf, f->cmf_h, sizeof(f->cmf_h), hipMemcpyHostToDevice, f->stream);
+        if (e != hipSuccess) rc = set_err(CZ_EHIP, "table upload: %s", hipGetErrorString(e));
     }
     if (!rc && init) rc = init_state(f);
     if (!rc && !init) {
@@ -953,6 +960,22 @@ int cz_film_clear(cz_film* f) {
 }
 
 void* cz_film_stream(cz_film* f) { return f ? (void*)f->stream : nullptr; }
+
+int cz_film_shade_output(cz_film* f, int on) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    if (on && !f->color) {
+        rc = dalloc(f, &f->color, f->cap_n);
+        if (!rc) rc = dalloc(f, &f->detail, f->cap_n);
+        if (rc) return rc;
+        for (auto& g : f->graphs) hipGraphExecDestroy(g.second);  // the descriptors change
+        f->graphs.clear();
+        f->graph_seen.clear();
+    }
+    f->shade_on = on != 0;
+    return CZ_OK;
+}
 
 int cz_film_bin_on_device(cz_film* f, int mode) {
     if (!f || mode < -1 || mode > 1) return set_err(CZ_EINVAL, "bin_on_device: -1, 0 or 1");

@@ -27,7 +27,7 @@ int cz_set_err(int code, const char* fmt, ...);
     } while (0)
 
 struct GraphKeyStore {
-    unsigned char bytes[9 * 4 + sizeof(cz_params)];
+    unsigned char bytes[10 * 4 + sizeof(cz_params)];
 };
 
 namespace czi {
@@ -190,6 +190,9 @@ struct cz_film {
     uint32_t* dep_cnt = nullptr;   // [tiles64] device binning: counts (zero between frames)
     uint32_t* dep_n = nullptr;     // [1] device binning: the frame's deposit count
     uint32_t* dep_n_h = nullptr;   // pinned staging of it
+    float* cmf = nullptr;          // [16][3] device copy of cmf_h
+    uint2 *color = nullptr, *detail = nullptr;  // shade outputs, allocated when shading is turned on
+    bool shade_on = false;         // run the shade pass at the end of every frame
     int bin_mode = -1;             // deposit binning: -1 by count, 0 host, 1 device
     bool bin_on_device = false;    // the uploaded deposits are binned in the frame
     float2* stats_out = nullptr;
@@ -237,7 +240,7 @@ struct cz_film {
         F.cg_part = cg_part; F.cg_part2 = cg_part2; F.cg_s = cg_s;
         F.tile_flags = tile_flags; F.tile_list = tile_list; F.tile_moved = tile_moved; F.active_count = active_count;
         F.cure_list = cure_list; F.cure_count = cure_count;
-        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.tile_bits = tile_bits; F.compact_cnt = compact_cnt; F.dep_rect = dep_rect; F.dep_cnt = dep_cnt; F.dep_n = dep_n; F.pigtab = pigtab;
+        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.tile_bits = tile_bits; F.compact_cnt = compact_cnt; F.dep_rect = dep_rect; F.dep_cnt = dep_cnt; F.dep_n = dep_n; F.cmf = cmf; F.color = color; F.detail = detail; F.pigtab = pigtab;
         F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
         F.frame_out = frame_out;
         return F;

@@ -125,6 +125,7 @@ __global__ void k_batch_plan(BatchPlan B);
 
 #define CZ_TILE_KERNELS(X)                                          \
     X(k_mark, (WS ws), (ws))                                        \
+    X(k_shade, (WS ws), (ws))                                       \
     X(k_cure, (WS ws), (ws))                                        \
     X(k_cure_clear, (WS ws, int do_clear), (ws, do_clear))          \
     X(k_lap_plug, (WS ws, int do_nose, int do_plug, int do_clear), (ws, do_nose, do_plug, do_clear)) \

@@ -386,6 +386,101 @@ __global__ __launch_bounds__(256) void k_compact_list(FilmDev F) {
 }
 
 // ---------------------------------------------------------------------------
+// shade_cell (shade.wgsl:29-83) with load_refl and spectrum_to_rgb
+// (km.wgsl:35-62): the display colour and detail channels of every cell.
+__device__ __forceinline__ float smoothstep_w(float e0, float e1, float x) {
+    const float t = cz_clamp((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+template <class WS>
+__global__ __launch_bounds__(256) void k_shade(WS ws) {
+    CZ_FOR_TILES(ws) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        float r[NBAND];
+#pragma unroll
+        for (uint32_t b = 0; b < 8u; b++) {
+            const uint32_t v = F.sub_refl[c * 8u + b];
+            r[2u * b] = cz_unpack_lo(v);
+            r[2u * b + 1u] = cz_unpack_hi(v);
+        }
+        const float hv = F.h[c];
+        float wet = 0.0f, cure_top = 0.0f;
+        const float4 th4 = F.strata_h[c];
+        if (hv > 0.0f) {
+            const float4 cu4 = F.strata_cure[c];
+#pragma unroll 1
+            for (uint32_t k = 0; k < 4u; k++) {
+                const float t = k == 0u ? th4.x : (k == 1u ? th4.y : (k == 2u ? th4.z : th4.w));
+                if (t <= 0.0f) continue;
+                float pig[16];
+                unpack8(F.strata_pig[(c * 4u + k) * 2u], F.strata_pig[(c * 4u + k) * 2u + 1u], pig);
+#pragma unroll
+                for (uint32_t b = 0; b < NBAND; b++) {
+                    float kk, ss, R, T;
+                    layer_ks(F.pigtab, pig, b, kk, ss);
+                    km_layer(kk, ss, t, R, T);
+                    r[b] = composite(R, T, r[b]);
+                }
+                const float cu = k == 0u ? cu4.x : (k == 1u ? cu4.y : (k == 2u ? cu4.z : cu4.w));
+                wet = 1.0f - cu;
+                cure_top = cu;
+            }
+        }
+        const float k1 = 0.04f;
+        const float k2 = cz_mix(0.4f, 0.6f, wet);
+        float xyz[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (uint32_t b = 0; b < NBAND; b++) {
+            const float rb = (1.0f - k1) * (1.0f - k2) * r[b] / cz_max(1.0f - k2 * r[b], 1e-3f);
+#pragma unroll
+            for (uint32_t q = 0; q < 3u; q++) xyz[q] = xyz[q] + rb * F.cmf[b * 3u + q];
+        }
+        float rgb[3] = {3.2406f * xyz[0] - 1.5372f * xyz[1] - 0.4986f * xyz[2],
+                        -0.9689f * xyz[0] + 1.8758f * xyz[1] + 0.0415f * xyz[2],
+                        0.0557f * xyz[0] - 0.2040f * xyz[1] + 1.0570f * xyz[2]};
+#pragma unroll
+        for (int q = 0; q < 3; q++) rgb[q] = cz_max(rgb[q], 0.0f);
+        const float mc = F.m[c];
+        const float painted = smoothstep_w(60e-6f, 150e-6f, mc);
+        cure_top = cz_max(cure_top, painted * (1.0f - smoothstep_w(0.0f, 50e-6f, hv)));
+        float rough = cz_mix(0.06f, 0.35f, sqrtf(cure_top));
+        if (hv <= 0.0f) rough = cz_mix(0.85f, 0.35f, painted);
+        const uint32_t lv = F.lay[c];
+        const float turn = (float)((lv >> 8u) & 0xffffu) / 65536.0f;
+        const float thickness = hv + cz_max(mc - 60e-6f, 0.0f);
+        if (F.P.flags[0] == 1u && hv > 0.0f) {
+            // the strata debug view
+            const float ssum = th4.x + th4.y + th4.z + th4.w;
+            bool bad = false;
+            uint32_t nst = 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; k++) {
+                const float t = k == 0u ? th4.x : (k == 1u ? th4.y : (k == 2u ? th4.z : th4.w));
+                if (t > 0.0f) nst = k + 1u;
+                if (t <= 0.0f) continue;
+                float pig[16];
+                unpack8(F.strata_pig[(c * 4u + k) * 2u], F.strata_pig[(c * 4u + k) * 2u + 1u], pig);
+                float cs = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 13; i++) cs = cs + pig[i];
+                if (!(cs > 0.0f)) bad = true;
+            }
+            if (ssum <= 0.0f) { rgb[0] = 0.0f; rgb[1] = 0.0f; rgb[2] = 1.0f; }
+            else if (bad) { rgb[0] = 1.0f; rgb[1] = 0.0f; rgb[2] = 0.0f; }
+            else if (cz_abs(ssum - hv) > 0.01f * hv) { rgb[0] = 0.0f; rgb[1] = 1.0f; rgb[2] = 0.0f; }
+            else {
+#pragma unroll
+                for (int q = 0; q < 3; q++) rgb[q] = 0.3f + 0.15f * (float)nst;
+            }
+        }
+        F.color[c] = make_uint2(cz_pack2(rgb[0], rgb[1]), cz_pack2(rgb[2], rough));
+        F.detail[c] = make_uint2(cz_pack2((float)(lv & 0xffu) + turn, cure_top), cz_pack2(F.rheo[c].y, thickness));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // cure.wgsl:3-48,62-123
 
 __device__ inline void composition_rheo(const FilmDev& F, const Stack& s, float& mu_o, float& ty_o) {
@@ -2276,6 +2371,7 @@ CZ_INST1(k_reduce_pool, ManyFilms)
 #define CZ_INST2(name, ...) CZ_INST1(name, OneFilm, ##__VA_ARGS__) CZ_INST1(name, ManyFilms, ##__VA_ARGS__)
 CZ_INST2(k_lap_plug, int, int, int)
 CZ_INST1(k_mark, OneFilmAll)
+CZ_INST1(k_shade, OneFilmAll)
 CZ_INST1(k_mark, ManyFilms)
 CZ_INST1(k_cure, OneFilmCure)
 CZ_INST1(k_cure, ManyFilms)
