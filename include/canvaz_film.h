@@ -83,6 +83,53 @@ typedef struct cz_deposit {
 enum { CZ_METHOD_THREAD = 1, CZ_METHOD_DROP = 2, CZ_METHOD_POUR = 3,
        CZ_METHOD_BRUSH = 4, CZ_METHOD_KNIFE = 5, CZ_METHOD_ROLLER = 6 };
 
+#define CZ_MAX_CONTACTS 65536     /* src/film/mod.rs:235 */
+#define CZ_MAX_SCRAPS (1u << 18)  /* src/film/mod.rs:236 */
+#define CZ_MAX_BRISTLES 512       /* src/film/mod.rs:237 */
+
+/* One tool contact segment, byte-identical to `Contact`
+ * (src/film/mod.rs:239-248, shaders/brush.wgsl:1-8): 32 bytes.  pad is the
+ * contact kind: 0 bristle, 1 roller, 2 sponge, 3 body. */
+typedef struct cz_contact {
+    float p0[2];
+    float p1[2];
+    float r;
+    float zb;
+    uint32_t bristle;
+    uint32_t pad;
+} cz_contact;
+enum { CZ_CONTACT_BRISTLE = 0, CZ_CONTACT_ROLLER = 1, CZ_CONTACT_SPONGE = 2, CZ_CONTACT_BODY = 3 };
+
+/* The palette knife of one frame, byte-identical to `KnifeParams`
+ * (src/film/mod.rs:250-270, shaders/knife.wgsl:6-23): 384 bytes. */
+typedef struct cz_knife {
+    float a[2];
+    float b[2];
+    float dir[2];
+    float sweep;
+    float height;
+    float half_width;
+    uint32_t active;
+    uint32_t frame;
+    uint32_t pad1;
+    uint32_t bands;
+    uint32_t pad2, pad3, pad4;
+    uint32_t pig[8][8];   /* per band: 16 f16 pigment fractions */
+    float fill[8];
+    float smear[8];
+} cz_knife;
+
+/* One knife scrap, byte-identical to `Scrap` (src/film/mod.rs:272-279,
+ * shaders/knife.wgsl:25-32): 48 bytes.  vol > 0: paint taken off the cell;
+ * vol < 0: paint laid from the knife's load. */
+typedef struct cz_scrap {
+    uint32_t cell;
+    float vol;
+    uint32_t frame;
+    uint32_t pad1;
+    uint32_t pig[8];
+} cz_scrap;
+
 /* `FilmSettings` (src/film/mod.rs:281-313); bools as uint32 0/1.
  * cz_settings_default() fills the reference defaults (mod.rs:315-350). */
 typedef struct cz_settings {
@@ -214,7 +261,10 @@ enum {
     CZ_BUF_DEP_COUNT = 34,  /* u32 [tiles64]: its item count */
     CZ_BUF_DEP_ITEMS = 35,  /* u32 [CZ_MAX_TILE_ITEMS]: deposit indices, per tile ascending */
     CZ_BUF_COLOR = 36,      /* f16 [n][4]: shade color_out (linear rgb, roughness); with cz_film_shade_output */
-    CZ_BUF_DETAIL = 37      /* f16 [n][4]: shade detail_out (method + turn, cure, yield stress, thickness) */
+    CZ_BUF_DETAIL = 37,     /* f16 [n][4]: shade detail_out (method + turn, cure, yield stress, thickness) */
+    CZ_BUF_JMETA = 38,      /* u32 [journal cap][4]: slot (tile16, generation, sequence, valid) */
+    CZ_BUF_JMARK = 39,      /* u32 [tiles16]: generation each tile was last saved in */
+    CZ_BUF_JDATA = 40       /* u32 [journal cap][53][256]: word w of cell li of slot s at (s*53 + w)*256 + li */
 };
 
 typedef struct cz_film cz_film;
@@ -353,6 +403,57 @@ int cz_film_shade_output(cz_film* f, int on);
  * count (default: the device from 1024 deposits in a frame).  The bins are
  * identical either way; the host keeps the volume and height accounting. */
 int cz_film_bin_on_device(cz_film* f, int mode);
+/* ---- Tool contacts and the palette knife (shaders/brush.wgsl,
+ * shaders/knife.wgsl, src/film/mod.rs:1030-1173) ----------------------------
+ * upload_contacts (1044-1084): the contacts for the next frame's brush pass
+ * (before the deposits), binned per 64-cell tile on the host; they touch the
+ * journal.  Consumed by that frame.  n > CZ_MAX_CONTACTS keeps the first. */
+int cz_film_upload_contacts(cz_film* f, const cz_contact* cs, uint32_t n);
+/* set_knife (1030-1042): the knife for the frames that follow (after the
+ * deposits), NULL or active == 0 = none; an active knife touches the journal
+ * over its reach. */
+int cz_film_set_knife(cz_film* f, const cz_knife* k);
+/* request_reservoir + poll_reservoir (1086-1133): the paint the brush pass
+ * scraped since the last call, per bristle [CZ_MAX_BRISTLES][17] u32
+ * (volume, then 16 pigment volumes, in units of 1e-15 m^3); the device sums
+ * are cleared.  Waits for the stream. */
+int cz_film_take_reservoir(cz_film* f, uint32_t* out);
+/* request_scraps + poll_scraps (1135-1173): the knife's scraps since the
+ * last call, in ascending cell order, taken paint before laid paint of a
+ * cell, frame by frame (the reference appends them with an atomic counter in
+ * whatever order cells run).  *count = scraps recorded (at most
+ * CZ_MAX_SCRAPS are kept); min(*count, cap) are copied to out.  The device
+ * list is cleared.  Waits for the stream. */
+int cz_film_take_scraps(cz_film* f, cz_scrap* out, uint32_t cap, uint32_t* count);
+
+/* ---- Undo journal (src/film/mod.rs:357-443, 1246-1275; shaders/journal.wgsl)
+ * While the generation is nonzero, the first pass of every frame saves each
+ * 16-cell tile that the frame's deposits or cz_film_touch_rect touched, or
+ * that lies next to a tile active in the last frame, once per generation,
+ * into a ring of `cap` tile records (53 words per cell: h, m, rheo,
+ * strata_h, strata_pig, strata_cure, sub_refl, lay).  The tiles of one frame
+ * take ring sequence numbers in ascending tile order (the reference's
+ * atomic order is unspecified).  A film stepped by a batch cannot journal. */
+/* Journal.gen (src/session.rs:139-143); 0 = off (the default).  The ring
+ * is allocated at the first nonzero generation. */
+int cz_film_journal_gen(cz_film* f, uint32_t gen);
+/* PAINT_UNDO_MB (src/film/mod.rs:374-376): the ring's budget in bytes, set
+ * before the first nonzero generation; cap = max(bytes / 54272, 64) tiles
+ * (default 384 MiB: 7419 tiles). */
+int cz_film_journal_budget(cz_film* f, uint64_t bytes);
+/* Journal::head (src/film/mod.rs:414-424), the sequence number of the next
+ * record (waits for the stream), and the ring's capacity in tiles (0 before
+ * the ring exists). */
+int cz_film_journal_head(cz_film* f, uint32_t* head, uint32_t* cap);
+/* Film::touch_rect (src/film/mod.rs:1246-1266): mark the 16-cell tiles of a
+ * rectangle in metres for the next frame's journal pass (ignored while the
+ * generation is 0 or for a non-finite rectangle). */
+int cz_film_touch_rect(cz_film* f, float minx, float miny, float maxx, float maxy);
+/* Film::restore_journal (src/film/mod.rs:1272-1275, journal.wgsl:100-123):
+ * every tile with a live record of generation >= target goes back to its
+ * oldest such record and is marked moved. */
+int cz_film_restore_journal(cz_film* f, uint32_t target);
+
 /* Frame graphs captured and frames replayed from them so far. */
 int cz_film_graph_counts(const cz_film* f, uint64_t* captures, uint64_t* replays);
 /* Shape of the flux and limiter + apply passes: 0 = flux with one block per

@@ -93,8 +93,25 @@ typedef struct ofilm {
     uint32_t implicit_steps, substeps_last, frame;
     double total_deposited, weave_volume;
     float cg_report[4];
+    /* undo journal (src/film/mod.rs:357-443): generation, ring capacity in
+     * tiles, sequence head, per-slot meta (tile, gen, seq, valid), per-slot
+     * cell records (256 x 53 words, the reference's record), per-tile touch
+     * (host) and mark */
+    uint32_t j_gen, j_cap, j_head;
+    uint32_t *j_meta, *j_data, *j_touch, *j_mark;
+    /* tool contacts and knife (src/film/mod.rs:1030-1173), allocated at first use */
+    uint32_t contact_count;
+    cz_contact* contacts;   /* [CZ_MAX_CONTACTS] */
+    uint32_t* ctile_range;  /* [tiles64][2] first item, count */
+    uint32_t* ctile_items;  /* [CZ_MAX_CONTACTS * 16] */
+    uint32_t* reservoir;    /* [CZ_MAX_BRISTLES][17] */
+    int knife_active;
+    cz_knife knife;
+    cz_scrap* scraps;       /* [CZ_MAX_SCRAPS] */
+    uint32_t scrap_count;
 } ofilm;
 
+void of_journal_touch_rect(ofilm* F, float minx, float miny, float maxx, float maxy);
 /* ------------------------------------------------------------------------ */
 /* small helpers                                                            */
 
@@ -769,6 +786,440 @@ static void layer_ks(const ofilm* F, const float* pig, uint32_t band, float* ko,
     *ko = k * sc;
     *so = s * sc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* tool contacts and the knife: shaders/brush.wgsl, shaders/knife.wgsl,      */
+/* src/film/mod.rs:1030-1173                                                */
+
+#define RES_UNIT 1e15f
+#define LEAVE_FILM 8e-5f
+#define BRISTLE_SCRAPE 0.25f
+#define SPONGE_FILM 1e-5f
+#define KNIFE_BANDS 8u
+#define KNIFE_BLADE_LEN 0.07f
+#define KNIFE_SLOPE 0.25f
+#define KNIFE_FLAT 0.01f
+#define KNIFE_JOURNAL_REACH 0.08f
+
+static void tools_alloc(ofilm* F) {
+    if (F->reservoir) return;
+    F->contacts = (cz_contact*)calloc(CZ_MAX_CONTACTS, sizeof(cz_contact));
+    F->ctile_range = (uint32_t*)calloc(2u * F->nt64, 4);
+    F->ctile_items = (uint32_t*)calloc((size_t)CZ_MAX_CONTACTS * 16u, 4);
+    F->reservoir = (uint32_t*)calloc(CZ_MAX_BRISTLES * 17u, 4);
+    F->scraps = (cz_scrap*)calloc(CZ_MAX_SCRAPS, sizeof(cz_scrap));
+}
+
+/* f32 -> u32 as a saturating conversion (out-of-range and NaN values are
+ * clamped; NaN gives 0) */
+static uint32_t f2u_sat(float v) {
+    if (!(v > 0.0f)) return 0u;
+    if (v >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)v;
+}
+
+/* upload_contacts (src/film/mod.rs:1044-1084) */
+void of_upload_contacts(ofilm* F, const cz_contact* cs, uint32_t n) {
+    tools_alloc(F);
+    uint32_t count = n < CZ_MAX_CONTACTS ? n : CZ_MAX_CONTACTS;
+    F->contact_count = count;
+    if (count == 0) return;
+    const uint32_t nt = F->t64x * F->t64y;
+    const float tile_m = 64.0f * F->dx;
+    uint32_t* cnt = (uint32_t*)calloc(nt, 4);
+    int32_t* rect = (int32_t*)malloc((size_t)count * 16);
+    for (uint32_t ci = 0; ci < count; ci++) {
+        const cz_contact* c = &cs[ci];
+        float minx = fminf(c->p0[0], c->p1[0]) - c->r, maxx = fmaxf(c->p0[0], c->p1[0]) + c->r;
+        float miny = fminf(c->p0[1], c->p1[1]) - c->r, maxy = fmaxf(c->p0[1], c->p1[1]) + c->r;
+        of_journal_touch_rect(F, minx, miny, maxx, maxy);
+        rect[4 * ci] = -1;
+        if (maxx < 0.0f || maxy < 0.0f || minx > F->s.width_m || miny > F->s.height_m || !(isfinite(minx) && isfinite(miny)))
+            continue;
+        const float mx = (float)(F->t64x - 1u), my = (float)(F->t64y - 1u);
+        /* ((v / tile_m).floor().max(0.0) as i64).min(n - 1), the cast saturating */
+        rect[4 * ci] = (int32_t)fminf(fmaxf(floorf(minx / tile_m), 0.0f), mx);
+        rect[4 * ci + 1] = (int32_t)fminf(fmaxf(floorf(maxx / tile_m), 0.0f), mx);
+        rect[4 * ci + 2] = (int32_t)fminf(fmaxf(floorf(miny / tile_m), 0.0f), my);
+        rect[4 * ci + 3] = (int32_t)fminf(fmaxf(floorf(maxy / tile_m), 0.0f), my);
+        for (int32_t ty = rect[4 * ci + 2]; ty <= rect[4 * ci + 3]; ty++)
+            for (int32_t tx = rect[4 * ci]; tx <= rect[4 * ci + 1]; tx++) cnt[(uint32_t)ty * F->t64x + (uint32_t)tx]++;
+    }
+    /* per-tile lists in contact order, the item total capped */
+    uint32_t items = 0, cap = CZ_MAX_CONTACTS * 16u;
+    for (uint32_t t = 0; t < nt; t++) {
+        uint32_t take = cnt[t] < cap - items ? cnt[t] : cap - items;
+        F->ctile_range[2 * t] = items;
+        F->ctile_range[2 * t + 1] = take;
+        items += take;
+    }
+    memset(cnt, 0, nt * 4);
+    for (uint32_t ci = 0; ci < count; ci++) {
+        if (rect[4 * ci] < 0) continue;
+        for (int32_t ty = rect[4 * ci + 2]; ty <= rect[4 * ci + 3]; ty++)
+            for (int32_t tx = rect[4 * ci]; tx <= rect[4 * ci + 1]; tx++) {
+                uint32_t t = (uint32_t)ty * F->t64x + (uint32_t)tx;
+                if (cnt[t] < F->ctile_range[2 * t + 1]) F->ctile_items[F->ctile_range[2 * t] + cnt[t]++] = ci;
+            }
+    }
+    memcpy(F->contacts, cs, (size_t)count * sizeof(cz_contact));
+    free(cnt);
+    free(rect);
+}
+
+/* brush.wgsl:23-31 */
+static float seg_dist(float px, float py, const float* a, const float* b) {
+    float abx = b[0] - a[0], aby = b[1] - a[1];
+    float l2 = abx * abx + aby * aby;
+    float t = 0.0f;
+    if (l2 > 0.0f) t = cz_clamp(((px - a[0]) * abx + (py - a[1]) * aby) / l2, 0.0f, 1.0f);
+    return cz_len2(px - (a[0] + abx * t), py - (a[1] + aby * t));
+}
+
+/* take paint off the top of the stack, strata 3 down to 0 (brush.wgsl:93-107,
+ * knife.wgsl:89-103): returns the volume taken, its pigment mix in pig */
+static float strip_top(ofilm* F, Stack* s, float excess, float* pig) {
+    (void)F;
+    float removed = 0.0f;
+    for (int i = 0; i < 16; i++) pig[i] = 0.0f;
+    for (int k = 3; k >= 0; k--) {
+        if (excess <= 0.0f) break;
+        float th = s->th[k];
+        if (th > 0.0f) {
+            float take = cz_min(th, excess);
+            pig_mix(pig, removed, s->pg[k], take, pig);
+            removed += take;
+            excess -= take;
+            s->th[k] = th - take;
+        }
+    }
+    return removed;
+}
+
+/* brush_scrape (brush.wgsl:33-122) */
+static void k_brush(ofilm* F) {
+    for (uint32_t j = 0; j < F->H; j++) {
+        for (uint32_t i = 0; i < F->W; i++) {
+            uint32_t tile = (i / TILE) + (j / TILE) * F->P.tiles[0];
+            uint32_t n = F->ctile_range[2 * tile + 1];
+            if (n == 0u) continue;
+            uint32_t c = idx(F, i, j);
+            float hv = F->h[c];
+            if (hv <= SPONGE_FILM) continue;
+            uint32_t start = F->ctile_range[2 * tile];
+            float px = ((float)i + 0.5f) * F->P.dx, py = ((float)j + 0.5f) * F->P.dx;
+            float cap = 1e9f;
+            uint32_t who = 0xffffffffu, method = CZ_METHOD_BRUSH;
+            float sx = 1.0f, sy = 0.0f, tx = 0.0f, ty = 0.0f;
+            int swept = 0;
+            for (uint32_t q = 0; q < n; q++) {
+                const cz_contact* k = &F->contacts[F->ctile_items[start + q]];
+                if (!(seg_dist(px, py, k->p0, k->p1) < k->r)) continue;
+                if (k->pad == 0u) {
+                    swept = 1;
+                    tx = tx + (k->p1[0] - k->p0[0]);
+                    ty = ty + (k->p1[1] - k->p0[1]);
+                }
+                float cc = cz_max(k->zb - F->m[c], LEAVE_FILM);
+                if (k->pad == CZ_CONTACT_ROLLER) cc = cz_max(k->zb, LEAVE_FILM);
+                if (k->pad == CZ_CONTACT_SPONGE) cc = cz_max(k->zb, SPONGE_FILM);
+                if (cc < cap) {
+                    cap = cc;
+                    who = k->bristle;
+                    if (k->pad == CZ_CONTACT_ROLLER) {
+                        method = CZ_METHOD_ROLLER;
+                        sx = k->p1[1] - k->p0[1];
+                        sy = k->p0[0] - k->p1[0];
+                    } else if (k->pad == CZ_CONTACT_SPONGE) {
+                        method = CZ_METHOD_ROLLER;
+                        sx = k->p1[0] - k->p0[0];
+                        sy = k->p1[1] - k->p0[1];
+                    } else {
+                        method = CZ_METHOD_BRUSH;
+                        sx = k->p1[0] - k->p0[0];
+                        sy = k->p1[1] - k->p0[1];
+                    }
+                }
+            }
+            if (who == 0xffffffffu) continue;
+            if (swept && method == CZ_METHOD_BRUSH) {
+                sx = tx;
+                sy = ty;
+            }
+            {
+                float vx = sx + 1e-9f, vy = sy + 0.0f;
+                float vl = cz_len2(vx, vy);
+                F->lay[c] = lay_pack(method, vx / vl, vy / vl);
+            }
+            if (hv <= cap) continue;
+            Stack s;
+            stack_load(F, c, &s);
+            float excess = hv - cap;
+            if (method == CZ_METHOD_BRUSH) excess *= BRISTLE_SCRAPE;
+            float pig[16];
+            float removed = strip_top(F, &s, excess, pig);
+            if (removed <= 0.0f) continue;
+            F->h[c] = hv - removed;
+            F->tile_moved[(i / 16u) + (j / 16u) * F->tw] = 1u;
+            stack_compact(F, &s);
+            stack_store_to(F->strata_h, F->strata_cure, F->strata_pig, c, &s);
+            /* an out-of-range bristle index adds to no reservoir */
+            if (who >= CZ_MAX_BRISTLES) continue;
+            float vol = removed * F->P.dx * F->P.dx;
+            uint32_t base = who * 17u;
+            F->reservoir[base] += f2u_sat(vol * RES_UNIT + 0.5f);
+            for (int b = 0; b < 16; b++) {
+                float sc = b >= 13 ? 0.01f : 1.0f;
+                F->reservoir[base + 1u + (uint32_t)b] += f2u_sat(cz_max(vol * pig[b] * sc, 0.0f) * RES_UNIT + 0.5f);
+            }
+        }
+    }
+}
+
+/* set_knife (src/film/mod.rs:1030-1042) */
+void of_set_knife(ofilm* F, const cz_knife* k) {
+    tools_alloc(F);
+    if (k) F->knife = *k;
+    else memset(&F->knife, 0, sizeof(F->knife));
+    F->knife_active = F->knife.active != 0u;
+    if (F->knife_active) {
+        const cz_knife* p = &F->knife;
+        float reach = KNIFE_JOURNAL_REACH + p->sweep;
+        of_journal_touch_rect(F, fminf(p->a[0], p->b[0]) - reach, fminf(p->a[1], p->b[1]) - reach,
+                              fmaxf(p->a[0], p->b[0]) + reach, fmaxf(p->a[1], p->b[1]) + reach);
+    }
+}
+
+/* knife.wgsl:34-40 */
+static float cell_hash(uint32_t x, uint32_t y) {
+    uint32_t h = (x * 0x8da6b343u) ^ (y * 0xd8163841u);
+    h = (h ^ (h >> 15u)) * 0x2c1b3c6du;
+    h = (h ^ (h >> 12u)) * 0x297a2d39u;
+    h = h ^ (h >> 15u);
+    return (float)(h & 0xffffu) / 65535.0f;
+}
+
+static void emit_scrap(ofilm* F, uint32_t c, float vol, const float* pig) {
+    uint32_t slot = F->scrap_count++;
+    if (slot >= CZ_MAX_SCRAPS) return;
+    cz_scrap* r = &F->scraps[slot];
+    r->cell = c;
+    r->vol = vol;
+    r->frame = F->knife.frame;
+    r->pad1 = 0u;
+    pack_pig(pig, r->pig);
+}
+
+/* knife (knife.wgsl:56-128), cells in ascending index order */
+static void k_knife(ofilm* F) {
+    const cz_knife* K = &F->knife;
+    const float dxm = F->P.dx;
+    for (uint32_t j = 0; j < F->H; j++) {
+        for (uint32_t i = 0; i < F->W; i++) {
+            uint32_t c = idx(F, i, j);
+            float px = ((float)i + 0.5f) * dxm, py = ((float)j + 0.5f) * dxm;
+            float abx = K->b[0] - K->a[0], aby = K->b[1] - K->a[1];
+            float mvx = K->dir[0] * K->sweep, mvy = K->dir[1] * K->sweep;
+            float det = abx * mvy - aby * mvx;
+            if (fabsf(det) < 1e-12f) continue;
+            float qx = px - K->a[0], qy = py - K->a[1];
+            float t = (qx * mvy - qy * mvx) / det;
+            float u = -(abx * qy - aby * qx) / det;
+            if (t < 0.0f || t > 1.0f) continue;
+            float behind = u * K->sweep;
+            if (behind < -dxm || behind > KNIFE_BLADE_LEN) continue;
+            int at_edge = u <= 1.0f;
+            uint32_t nb = K->bands < KNIFE_BANDS ? K->bands : KNIFE_BANDS;
+            if (nb < 1u) nb = 1u;
+            uint32_t bi = f2u_sat(t * (float)nb);
+            if (bi > nb - 1u) bi = nb - 1u;
+            float ratio = at_edge ? K->fill[bi] : 0.0f;
+            int covered = cell_hash(i / 2u, j / 2u) < cz_powf(ratio, 0.6f);
+            float level_b = covered ? K->height * cz_min(1.0f, 1.5f * ratio) : 0.0f;
+            float smear_b = at_edge ? K->smear[bi] : 0.0f;
+            float band_pig[16];
+            unpack_pig(K->pig[bi], band_pig);
+            float hv = F->h[c];
+            float cap = cz_max(K->height + cz_max(behind - KNIFE_FLAT, 0.0f) * KNIFE_SLOPE - F->m[c], 0.0f);
+            float level = cz_min(level_b, cap);
+            float keep = cz_min(hv, cap);
+            float swap = keep * smear_b;
+            if (hv <= cap && hv >= level && swap <= 0.0f) continue;
+            Stack s;
+            stack_load(F, c, &s);
+            float pig[16];
+            float removed = strip_top(F, &s, (hv - cap) + swap, pig);
+            float hn = hv - removed;
+            float add = cz_max(cz_max(level, keep) - hn, 0.0f);
+            if (add > 0.0f) {
+                stack_push_top(F, &s, add, band_pig, 0.0f);
+                hn += add;
+            }
+            F->h[c] = hn;
+            {
+                float vx = mvx + 1e-9f, vy = mvy + 0.0f;
+                float vl = cz_len2(vx, vy);
+                F->lay[c] = lay_pack(CZ_METHOD_KNIFE, vx / vl, vy / vl);
+            }
+            F->tile_moved[(i / 16u) + (j / 16u) * F->tw] = 1u;
+            stack_compact(F, &s);
+            stack_store_to(F->strata_h, F->strata_cure, F->strata_pig, c, &s);
+            if (removed > 0.0f) emit_scrap(F, c, removed * dxm * dxm, pig);
+            if (add > 0.0f) {
+                float none[16] = {0};
+                emit_scrap(F, c, -add * dxm * dxm, none);
+            }
+        }
+    }
+}
+
+/* the two passes on their own, outside a frame (tests) */
+void of_brush_pass(ofilm* F) {
+    if (F->contact_count > 0) k_brush(F);
+}
+void of_knife_pass(ofilm* F) {
+    if (F->knife_active) k_knife(F);
+}
+
+/* request_/poll_scraps: up to cap records out, the count recorded */
+uint32_t of_take_scraps(ofilm* F, cz_scrap* out, uint32_t cap) {
+    if (!F->scraps) return 0u;
+    uint32_t n = F->scrap_count < CZ_MAX_SCRAPS ? F->scrap_count : CZ_MAX_SCRAPS;
+    memcpy(out, F->scraps, (size_t)(n < cap ? n : cap) * sizeof(cz_scrap));
+    F->scrap_count = 0u;
+    return n;
+}
+
+/* request_/poll_reservoir: the sums out, then cleared */
+void of_take_reservoir(ofilm* F, uint32_t* out) {
+    if (!F->reservoir) {
+        memset(out, 0, CZ_MAX_BRISTLES * 17u * 4u);
+        return;
+    }
+    memcpy(out, F->reservoir, CZ_MAX_BRISTLES * 17u * 4u);
+    memset(F->reservoir, 0, CZ_MAX_BRISTLES * 17u * 4u);
+}
+
+/* ------------------------------------------------------------------------ */
+/* undo journal: shaders/journal.wgsl, src/film/mod.rs:357-443,1246-1274      */
+
+#define J_WORDS 53u
+
+/* Journal gen (src/session.rs:141) with a ring of cap tiles. */
+void of_journal_begin(ofilm* F, uint32_t gen, uint32_t cap) {
+    if (!F->j_meta) {
+        F->j_cap = cap < 64u ? 64u : cap;
+        F->j_meta = (uint32_t*)calloc((size_t)F->j_cap * 4u, 4);
+        F->j_data = (uint32_t*)calloc((size_t)F->j_cap * 256u * J_WORDS, 4);
+        F->j_touch = (uint32_t*)calloc(F->nt16, 4);
+        F->j_mark = (uint32_t*)calloc(F->nt16, 4);
+    }
+    F->j_gen = gen;
+}
+
+uint32_t of_journal_head(const ofilm* F) { return F->j_head; }
+
+/* touch_rect (src/film/mod.rs:1246-1266): the 16-tiles of a rectangle. */
+void of_journal_touch_rect(ofilm* F, float minx, float miny, float maxx, float maxy) {
+    if (F->j_gen == 0u || !isfinite(minx) || !isfinite(miny) || !isfinite(maxx) || !isfinite(maxy)) return;
+    const float tile_m = 16.0f * F->dx;
+    const float tw = (float)(F->tw - 1u), th = (float)(F->th - 1u);
+    /* (v.floor() as i64).clamp(0, n - 1): the cast saturates, so clamp in float */
+    const int64_t x0 = (int64_t)fminf(fmaxf(floorf(minx / tile_m), 0.0f), tw);
+    const int64_t x1 = (int64_t)fminf(fmaxf(floorf(maxx / tile_m), 0.0f), tw);
+    const int64_t y0 = (int64_t)fminf(fmaxf(floorf(miny / tile_m), 0.0f), th);
+    const int64_t y1 = (int64_t)fminf(fmaxf(floorf(maxy / tile_m), 0.0f), th);
+    if (maxx < 0.0f || maxy < 0.0f) return;
+    for (int64_t y = y0; y <= y1; y++)
+        for (int64_t x = x0; x <= x1; x++) F->j_touch[y * F->tw + x] = 1u;
+}
+
+static int tile_busy(const ofilm* F, uint32_t t) {
+    const int tx = (int)(t % F->tw), ty = (int)(t / F->tw);
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+            const int x = tx + dx, y = ty + dy;
+            if (x < 0 || y < 0 || x >= (int)F->tw || y >= (int)F->th) continue;
+            if (F->tile_flags[(uint32_t)x + (uint32_t)y * F->tw] & 32u) return 1;
+        }
+    return 0;
+}
+
+static void save_cell(const ofilm* F, uint32_t c, uint32_t* o) {
+    memcpy(&o[0], &F->h[c], 4);
+    memcpy(&o[1], &F->m[c], 4);
+    memcpy(&o[2], &F->rheo[2u * c], 8);
+    memcpy(&o[4], &F->strata_h[4u * c], 16);
+    memcpy(&o[8], &F->strata_pig[32u * c], 128);
+    memcpy(&o[40], &F->strata_cure[4u * c], 16);
+    memcpy(&o[44], &F->sub_refl[8u * c], 32);
+    o[52] = F->lay[c];
+}
+
+static void load_cell(ofilm* F, uint32_t c, const uint32_t* o) {
+    memcpy(&F->h[c], &o[0], 4);
+    memcpy(&F->m[c], &o[1], 4);
+    memcpy(&F->rheo[2u * c], &o[2], 8);
+    memcpy(&F->strata_h[4u * c], &o[4], 16);
+    memcpy(&F->strata_pig[32u * c], &o[8], 128);
+    memcpy(&F->strata_cure[4u * c], &o[40], 16);
+    memcpy(&F->sub_refl[8u * c], &o[44], 32);
+    F->lay[c] = o[52];
+}
+
+/* journal (journal.wgsl:79-98) at the start of a frame: every tile touched
+ * this frame or next to an active tile of the last frame, not yet saved in
+ * this generation, takes the next ring slot, in ascending tile order (the
+ * reference's atomic order is not reproducible), then the touches are
+ * cleared (flush, src/film/mod.rs:405-412). */
+static void k_journal(ofilm* F) {
+    for (uint32_t t = 0; t < F->nt16; t++) {
+        if (F->j_mark[t] == F->j_gen || !(F->j_touch[t] != 0u || tile_busy(F, t))) continue;
+        const uint32_t seq = F->j_head++;
+        const uint32_t slot = seq % F->j_cap;
+        uint32_t* mt = &F->j_meta[4u * slot];
+        mt[0] = t; mt[1] = F->j_gen; mt[2] = seq; mt[3] = 1u;
+        F->j_mark[t] = F->j_gen;
+        for (uint32_t li = 0; li < 256u; li++) {
+            const uint32_t i = (t % F->tw) * 16u + (li & 15u), j = (t / F->tw) * 16u + (li >> 4);
+            if (i >= F->W || j >= F->H) continue;
+            save_cell(F, j * F->W + i, &F->j_data[((size_t)slot * 256u + li) * J_WORDS]);
+        }
+    }
+    memset(F->j_touch, 0, F->nt16 * 4u);
+}
+
+static int slot_valid(const ofilm* F, uint32_t s) {
+    const uint32_t* mt = &F->j_meta[4u * s];
+    return mt[3] == 1u && mt[2] < F->j_head && F->j_head - mt[2] <= F->j_cap;
+}
+
+/* restore (journal.wgsl:100-123): every tile with a valid slot of generation
+ * >= target goes back to its oldest such slot. */
+void of_journal_restore(ofilm* F, uint32_t target) {
+    if (!F->j_meta) return;
+    uint32_t* best = (uint32_t*)malloc(F->nt16 * 4u);
+    memset(best, 0xff, F->nt16 * 4u);
+    for (uint32_t s = 0; s < F->j_cap; s++) {
+        if (!slot_valid(F, s) || F->j_meta[4u * s + 1u] < target) continue;
+        const uint32_t t = F->j_meta[4u * s];
+        if (F->j_meta[4u * s + 2u] < best[t]) best[t] = F->j_meta[4u * s + 2u];
+    }
+    for (uint32_t s = 0; s < F->j_cap; s++) {
+        if (!slot_valid(F, s) || F->j_meta[4u * s + 1u] < target) continue;
+        const uint32_t t = F->j_meta[4u * s];
+        if (best[t] != F->j_meta[4u * s + 2u]) continue;
+        F->tile_moved[t] = 1u;
+        for (uint32_t li = 0; li < 256u; li++) {
+            const uint32_t i = (t % F->tw) * 16u + (li & 15u), j = (t / F->tw) * 16u + (li >> 4);
+            if (i >= F->W || j >= F->H) continue;
+            load_cell(F, j * F->W + i, &F->j_data[((size_t)s * 256u + li) * J_WORDS]);
+        }
+    }
+    free(best);
+}
+
+const uint32_t* of_journal_meta(const ofilm* F) { return F->j_meta; }
 
 /* shade.wgsl:29-83 (shade_cell) with km.wgsl:35-62 (load_refl,
  * spectrum_to_rgb): per cell, the substrate reflectance composited with
@@ -1965,7 +2416,10 @@ static void cg_solve(ofilm* F, uint32_t iters) {
 
 /* flags bit 0: run the implicit steps (settings.implicit_hydro) */
 void of_encode_frame(ofilm* F, uint32_t substeps, uint32_t implicit_steps, uint32_t cg_iters, uint32_t run_implicit) {
+    if (F->j_gen != 0u) k_journal(F);
+    if (F->contact_count > 0) k_brush(F);
     if (F->deposit_count > 0) k_deposit(F);
+    if (F->knife_active) k_knife(F);
     k_mark_tiles(F);
     k_compact_tiles(F);
     k_cure_step(F);
@@ -1995,6 +2449,7 @@ void of_encode_frame(ofilm* F, uint32_t substeps, uint32_t implicit_steps, uint3
     for (int k = 0; k < 6; k++) k_strata_fill(F);
     F->substeps_last = substeps;
     F->frame++;
+    F->contact_count = 0u;  /* the contacts are consumed by the frame */
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2321,6 +2776,8 @@ uint32_t of_upload_deposits(ofilm* F, const cz_deposit* deps, uint32_t n) {
         float mh = d->h0 * fmaxf(d->r0, d->r1) / d->r0;
         F->h_max_pending = fmaxf(F->h_max_pending, mh + F->h_max);
         float rb = fmaxf(d->r0, d->r1);
+        of_journal_touch_rect(F, fminf(d->p0[0], d->p1[0]) - rb, fminf(d->p0[1], d->p1[1]) - rb,
+                              fmaxf(d->p0[0], d->p1[0]) + rb, fmaxf(d->p0[1], d->p1[1]) + rb);
         rect[4 * di] = -1;
         int inside = fmaxf(d->p0[0], d->p1[0]) + rb > 0.0f && fmaxf(d->p0[1], d->p1[1]) + rb > 0.0f &&
                      fminf(d->p0[0], d->p1[0]) - rb < F->s.width_m && fminf(d->p0[1], d->p1[1]) - rb < F->s.height_m;
@@ -2411,6 +2868,15 @@ void of_destroy(ofilm* F) {
                   F->cg_t, F->cg_t2, F->cg_part, F->cg_part2, F->tile_flags, F->tile_list, F->tile_moved,
                   F->stats_out, F->pool_out, F->snap, F->deposits, F->tile_start, F->tile_count, F->tile_items};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
+    free(F->j_meta);
+    free(F->j_data);
+    free(F->j_touch);
+    free(F->j_mark);
+    free(F->contacts);
+    free(F->ctile_range);
+    free(F->ctile_items);
+    free(F->reservoir);
+    free(F->scraps);
     free(F);
 }
 
@@ -2434,6 +2900,10 @@ void* of_buffer(ofilm* F, int which, uint64_t* bytes) {
     case CZ_BUF_CG_P: *bytes = 4 * n; return F->cg_p;
     case CZ_BUF_TILE_MOVED: *bytes = 4 * (uint64_t)F->nt16; return F->tile_moved;
     case CZ_BUF_TILE_FLAGS: *bytes = 4 * (uint64_t)F->nt16; return F->tile_flags;
+    /* the journal ring as the reference lays it out: record s, cell li, word w at (s*256 + li)*53 + w */
+    case CZ_BUF_JMETA: *bytes = F->j_meta ? 16 * (uint64_t)F->j_cap : 0; return F->j_meta;
+    case CZ_BUF_JMARK: *bytes = F->j_mark ? 4 * (uint64_t)F->nt16 : 0; return F->j_mark;
+    case CZ_BUF_JDATA: *bytes = F->j_data ? 53 * 256 * 4 * (uint64_t)F->j_cap : 0; return F->j_data;
     case CZ_BUF_TMP: *bytes = 4 * n; return F->tmp;
     case CZ_BUF_FLUX: *bytes = 16 * n; return F->flux;
     case CZ_BUF_ACC: *bytes = 16 * n; return F->acc;

@@ -73,6 +73,18 @@ def lib():
         L.of_set_host_state.argtypes = [P, f32, f32, f32]
         L.of_mass_report.argtypes = [P, C.POINTER(MassReport)]
         L.of_shade.argtypes = [P, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16)]
+        L.of_journal_begin.argtypes = [P, u32, u32]
+        L.of_journal_head.restype = u32
+        L.of_journal_head.argtypes = [P]
+        L.of_journal_touch_rect.argtypes = [P, f32, f32, f32, f32]
+        L.of_journal_restore.argtypes = [P, u32]
+        L.of_upload_contacts.argtypes = [P, C.POINTER(cz_types.Contact), u32]
+        L.of_set_knife.argtypes = [P, C.POINTER(cz_types.Knife)]
+        L.of_take_scraps.restype = u32
+        L.of_take_scraps.argtypes = [P, C.c_void_p, u32]
+        L.of_take_reservoir.argtypes = [P, C.POINTER(u32)]
+        L.of_brush_pass.argtypes = [P]
+        L.of_knife_pass.argtypes = [P]
         for k in ("of_k_mark_compact", "of_k_laplacian", "of_k_plug", "of_k_fluxes", "of_k_cure"):
             getattr(L, k).argtypes = [P]
         for name in ("of_expf", "of_logf", "of_atanf"):
@@ -132,6 +144,7 @@ BUFFERS = {
     "flux": (25, np.float32, (4,)), "acc": (26, np.float32, (4,)), "alpha": (27, np.float32, ()),
     "stats": (28, np.float32, None), "pool": (29, np.float32, None), "tile_list": (30, np.uint32, None),
     "acc_s": (31, np.float32, (4, 4)), "alpha_s": (32, np.float32, (4,)),
+    "jmeta": (38, np.uint32, None), "jmark": (39, np.uint32, None), "jdata": (40, np.uint32, None),
 }
 
 
@@ -231,6 +244,45 @@ class OracleFilm:
         u16p = C.POINTER(C.c_uint16)
         lib().of_shade(self._p, color.ctypes.data_as(u16p), detail.ctypes.data_as(u16p))
         return color.reshape(self.h, self.w, 4), detail.reshape(self.h, self.w, 4)
+
+    def journal_begin(self, gen, cap):
+        """Journal.gen = gen (src/session.rs:141); the ring of cap tiles is
+        made at the first call."""
+        lib().of_journal_begin(self._p, gen, cap)
+
+    def journal_head(self):
+        return lib().of_journal_head(self._p)
+
+    def touch_rect(self, minx, miny, maxx, maxy):
+        lib().of_journal_touch_rect(self._p, minx, miny, maxx, maxy)
+
+    def restore_journal(self, target):
+        lib().of_journal_restore(self._p, target)
+
+    def upload_contacts(self, contacts):
+        arr, n = cz_types.contact_array(contacts)
+        lib().of_upload_contacts(self._p, arr, n)
+
+    def set_knife(self, knife):
+        lib().of_set_knife(self._p, C.byref(knife) if knife is not None else None)
+
+    def brush_pass(self):
+        """brush_scrape alone, on the uploaded contacts."""
+        lib().of_brush_pass(self._p)
+
+    def knife_pass(self):
+        """knife alone, with the set knife."""
+        lib().of_knife_pass(self._p)
+
+    def take_reservoir(self):
+        a = np.zeros((512, 17), np.uint32)
+        lib().of_take_reservoir(self._p, a.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return a
+
+    def take_scraps(self, cap=1 << 18):
+        a = np.zeros(cap, cz_types.SCRAP_DTYPE)
+        n = lib().of_take_scraps(self._p, a.ctypes.data_as(C.c_void_p), cap)
+        return a[:min(n, cap)]
 
     def step(self, deps=(), frame_dt=1.0 / 60.0):
         arr, n = cz_types.deposit_array(deps)

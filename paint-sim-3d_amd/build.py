@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libcanvaz_film.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CZ_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["film_kernels.hip", "film_host.hip", "batch_host.hip", "scene_gen.hip", "serve_host.hip", "host_ref.cpp", "km_mix.hip", "selftest.hip", "canvaz_io.hip"]
+SOURCES = ["film_kernels.hip", "film_host.hip", "batch_host.hip", "scene_gen.hip", "serve_host.hip", "host_ref.cpp", "km_mix.hip", "selftest.hip", "canvaz_io.hip", "journal.hip", "tools.hip"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

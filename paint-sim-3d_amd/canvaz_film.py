@@ -14,9 +14,9 @@ import os
 
 import numpy as np
 
-from cz_types import (Deposit, HostState, MassReport, Overrides, Params, Request, Result, Scene, ServerConfig,
-                      ServerStats, Settings, Stats, PART_NAMES,
-                      default_overrides, default_settings, deposit_array)
+from cz_types import (Contact, Deposit, HostState, Knife, MassReport, Overrides, Params, Request, Result, Scene,
+                      ServerConfig, ServerStats, Settings, Stats, PART_NAMES, SCRAP_DTYPE,
+                      contact_array, default_overrides, default_settings, deposit_array)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcanvaz_film.so")
@@ -96,6 +96,15 @@ def lib():
         "cz_film_save_canvaz": (i32, [P, C.c_char_p]),
         "cz_film_bin_on_device": (i32, [P, i32]),
         "cz_film_shade_output": (i32, [P, i32]),
+        "cz_film_journal_gen": (i32, [P, C.c_uint32]),
+        "cz_film_upload_contacts": (i32, [P, C.POINTER(Contact), C.c_uint32]),
+        "cz_film_set_knife": (i32, [P, C.POINTER(Knife)]),
+        "cz_film_take_reservoir": (i32, [P, C.POINTER(C.c_uint32)]),
+        "cz_film_take_scraps": (i32, [P, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+        "cz_film_journal_budget": (i32, [P, C.c_uint64]),
+        "cz_film_journal_head": (i32, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "cz_film_touch_rect": (i32, [P, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "cz_film_restore_journal": (i32, [P, C.c_uint32]),
         "cz_film_load_canvaz": (i32, [P, C.c_char_p]),
         "cz_lz4_compress": (i32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
         "cz_lz4_decompress": (i32, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
@@ -255,10 +264,10 @@ BUFFER_IDS = {
     "delta": 16, "fnorm": 17, "pres": 18, "h_frame": 19, "cg_m": 20, "cg_p": 21, "tile_moved": 22,
     "tile_flags": 23, "tmp": 24, "flux": 25, "acc": 26, "alpha": 27, "stats": 28, "pool": 29, "tile_list": 30,
     "acc_s": 31, "alpha_s": 32, "dep_start": 33, "dep_count": 34, "dep_items": 35,
-    "color": 36, "detail": 37,
+    "color": 36, "detail": 37, "jmeta": 38, "jmark": 39, "jdata": 40,
 }
 _U32 = {"strata_pig", "sub_refl", "lay", "delta", "tile_moved", "tile_flags", "tile_list", "dep_start", "dep_count",
-        "dep_items"}
+        "dep_items", "jmeta", "jmark", "jdata"}
 _CELL_SHAPE = {"h": (), "m": (), "rheo": (2,), "strata_h": (4,), "strata_pig": (4, 8), "strata_cure": (4,),
                "sub_refl": (8,), "lay": (), "delta": (), "color": (4,), "detail": (4,), "fnorm": (2,), "pres": (2,), "h_frame": (), "cg_m": (4,),
                "cg_p": (), "tmp": (), "flux": (4,), "acc": (4,), "alpha": (), "acc_s": (4, 4), "alpha_s": (4,)}
@@ -424,6 +433,54 @@ class Film:
         """Run the shade pass at the end of every frame (buffers "color", "detail": f16 bits)."""
         check(lib().cz_film_shade_output(self._p, 1 if on else 0))
 
+    # ---- tool contacts and the knife (src/film/mod.rs:1030-1173) ----
+    def upload_contacts(self, contacts):
+        """upload_contacts: Contact segments for the next frame's brush pass."""
+        arr, n = contact_array(contacts)
+        check(lib().cz_film_upload_contacts(self._p, arr, n))
+
+    def set_knife(self, knife):
+        """set_knife: a Knife (KnifeParams) for the following frames, or None."""
+        check(lib().cz_film_set_knife(self._p, C.byref(knife) if knife is not None else None))
+
+    def take_reservoir(self):
+        """Paint the brush pass took since the last call, [512 bristles][17] u32
+        (volume, 16 pigment volumes; units of 1e-15 m^3); the sums restart."""
+        a = np.zeros((512, 17), np.uint32)
+        check(lib().cz_film_take_reservoir(self._p, a.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return a
+
+    def take_scraps(self, cap=1 << 18):
+        """The knife's scraps since the last call (structured array, ascending
+        cell order, taken before laid paint of a cell); the list restarts."""
+        a = np.zeros(cap, SCRAP_DTYPE)
+        n = C.c_uint32()
+        check(lib().cz_film_take_scraps(self._p, a.ctypes.data_as(C.c_void_p), cap, C.byref(n)))
+        return a[:min(n.value, cap)]
+
+    # ---- undo journal (src/film/mod.rs:357-443, 1246-1275) ----
+    def journal_gen(self, gen):
+        """Journal.gen (src/session.rs:139-143): 0 = off."""
+        check(lib().cz_film_journal_gen(self._p, gen))
+
+    def journal_budget(self, nbytes):
+        """PAINT_UNDO_MB in bytes, before the first nonzero generation."""
+        check(lib().cz_film_journal_budget(self._p, nbytes))
+
+    def journal_head(self):
+        """(head, cap): the next record's sequence number (Journal::head) and the ring's size in tiles."""
+        h, c = C.c_uint32(), C.c_uint32()
+        check(lib().cz_film_journal_head(self._p, C.byref(h), C.byref(c)))
+        return h.value, c.value
+
+    def touch_rect(self, minx, miny, maxx, maxy):
+        """Film::touch_rect: journal the tiles of this rectangle (metres) in the next frame."""
+        check(lib().cz_film_touch_rect(self._p, minx, miny, maxx, maxy))
+
+    def restore_journal(self, target):
+        """Film::restore_journal: tiles with records of generation >= target go back to the oldest."""
+        check(lib().cz_film_restore_journal(self._p, target))
+
     def bin_on_device(self, mode):
         """Deposit binning: -1 by deposit count, 0 on the host, 1 on the device."""
         check(lib().cz_film_bin_on_device(self._p, mode))
@@ -483,6 +540,65 @@ class Film:
         cnt = C.c_uint64()
         check(lib().cz_film_profile_read(self._p, label.encode(), C.byref(avg), C.byref(cnt)))
         return avg.value, cnt.value
+
+
+class UndoHistory:
+    """The undo levels of a painting session (src/session.rs:9-51, 139-211).
+    A stroke level keeps the journal head at its start and is undone with
+    restore_journal; a full level keeps a copy of the state parts and is
+    undone with write_state.  Every level and every undo starts a new
+    journal generation."""
+
+    MAX_LEVELS = 64
+    MAX_FULL = 3
+
+    def __init__(self, film):
+        self.film = film
+        self.gen = 0
+        self.levels = []  # (gen, "journal", start head) or (gen, "full", state parts)
+
+    def _next_gen(self):
+        self.gen += 1
+        self.film.journal_gen(self.gen)
+        return self.gen
+
+    def _trim(self):
+        while len(self.levels) > self.MAX_LEVELS:
+            self.levels.pop(0)
+        fulls = [i for i, lv in enumerate(self.levels) if lv[1] == "full"]
+        if len(fulls) > self.MAX_FULL:
+            del self.levels[:fulls[len(fulls) - self.MAX_FULL - 1] + 1]
+
+    def begin_stroke(self):
+        start, _ = self.film.journal_head()
+        gen = self._next_gen()
+        self.levels.append((gen, "journal", start))
+        self._trim()
+
+    def begin_full(self):
+        parts = self.film.read_state()
+        gen = self._next_gen()
+        self.levels.append((gen, "full", parts))
+        self._trim()
+
+    def can_undo(self):
+        return bool(self.levels)
+
+    def rollback(self):
+        """Undo the newest level (App::rollback); FilmError with the
+        reference's message when it cannot."""
+        if not self.levels:
+            raise FilmError("Nothing to undo")
+        gen, kind, val = self.levels.pop()
+        if kind == "journal":
+            head, cap = self.film.journal_head()
+            if (head - val) & 0xFFFFFFFF > cap:
+                self.levels = [lv for lv in self.levels if lv[1] == "full"]
+                raise FilmError("That stroke changed too much of the canvas to undo")
+            self.film.restore_journal(gen)
+        elif not self.film.write_state(val):
+            raise FilmError("Undo failed")
+        self._next_gen()
 
 
 class Batch:

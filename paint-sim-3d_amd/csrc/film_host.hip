@@ -154,6 +154,7 @@ void frame_done(cz_film* f, uint32_t substeps, bool parity_done) {
     for (int k = 0; k < 3; k++) f->shape_subs_last[k] = f->shape_subs_now[k];
     f->frame++;
     f->deposit_count = 0;
+    f->contact_count = 0;  // the contacts are consumed by the frame
 }
 
 }  // namespace czi
@@ -190,6 +191,8 @@ int upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* se
         const cz_deposit& d = deps[di];
         f->h_max_pending = fmaxf(f->h_max_pending, czh::max_height(d) + f->h_max);
         float rb = fmaxf(d.r0, d.r1);
+        journal_touch_rect(f, fminf(d.p0[0], d.p1[0]) - rb, fminf(d.p0[1], d.p1[1]) - rb, fmaxf(d.p0[0], d.p1[0]) + rb,
+                           fmaxf(d.p0[1], d.p1[1]) + rb);
         bool inside = fmaxf(d.p0[0], d.p1[0]) + rb > 0.0f && fmaxf(d.p0[1], d.p1[1]) + rb > 0.0f &&
                       fminf(d.p0[0], d.p1[0]) - rb < f->s.width_m && fminf(d.p0[1], d.p1[1]) - rb < f->s.height_m;
         if (!inside) continue;
@@ -256,6 +259,15 @@ int record_frame(cz_film* f, uint32_t substeps) {
     const dim3 gt(f->tile_grid());
     const uint32_t n64 = f->t64x * f->t64y;
     FilmDev F = f->dev();
+    // the journal pass comes first (src/film/mod.rs:1335-1338)
+    if (f->j_gen != 0u) {
+        Launch _l(f, "journal");
+        journal_record(f);
+    }
+    if (f->contact_count > 0) {
+        Launch _l(f, "brush");
+        tools_record_brush(f);
+    }
     if (f->deposit_count > 0) {
         if (f->bin_on_device) {
             const uint32_t nt64 = f->t64x * f->t64y;
@@ -265,6 +277,10 @@ int record_frame(cz_film* f, uint32_t substeps) {
         }
         const dim3 gcell((f->W + 15u) / 16u, (f->H + 15u) / 16u);
         LAUNCH("deposit", k_deposit, gcell, b256, F);
+    }
+    if (f->knife_active && f->knife_grid[0] > 0u) {
+        Launch _l(f, "knife");
+        tools_record_knife(f);
     }
     LAUNCH("mark", k_mark<OneFilmAll>, dim3((f->nt16 + 3u) / 4u), b256, OneFilmAll{F, f->nt16});
     LAUNCH("compact", k_compact_bits, dim3((f->nt16 + 255u) / 256u), b256, F);
@@ -364,7 +380,8 @@ int record_frame(cz_film* f, uint32_t substeps) {
 // shape is everything the launches depend on (substeps, implicit steps,
 // deposit pass, offset-buffer parity, params, grid sizes).
 struct GraphKey {
-    uint32_t substeps, implicit_steps, implicit_on, has_dep, parity, grid1, grid4, flux_shape, persist, shade;
+    uint32_t substeps, implicit_steps, implicit_on, has_dep, parity, grid1, grid4, flux_shape, persist, shade, journal;
+    uint32_t brush, knife_gx, knife_gy;
     cz_params P;
     bool operator==(const GraphKey& o) const { return memcmp(this, &o, sizeof(GraphKey)) == 0; }
 };
@@ -378,6 +395,8 @@ int encode_direct(cz_film* f, uint32_t substeps) {
 
 int encode_frame(cz_film* f, uint32_t substeps) {
     if (f->sync_fault) return set_err(CZ_EHIP, "the film's last one-launch substeps failed; it takes no more frames");
+    int jrc = journal_flush(f);  // flush_journal (src/main.rs:1894) before the frame
+    if (jrc) return jrc;
     const uint64_t flux_items = std::min(f->active_hint, f->nt16);
     const uint32_t pg = f->persist_grid();
     if (!f->use_graphs || f->prof_on || (pg == 0u && f->flux_tuner.learning(flux_items)))
@@ -396,6 +415,10 @@ int encode_frame(cz_film* f, uint32_t substeps) {
     key.flux_shape = pg > 0u ? 0u : (uint32_t)f->flux_tuner.choose(flux_items, false, bk, timed);
     key.persist = pg;
     key.shade = f->shade_on ? 1u : 0u;
+    key.journal = f->j_gen != 0u ? 1u : 0u;
+    key.brush = f->contact_count > 0 ? 1u : 0u;
+    key.knife_gx = f->knife_active ? f->knife_grid[0] : 0u;
+    key.knife_gy = f->knife_active ? f->knife_grid[1] : 0u;
     key.P = f->P;
     hipGraphExec_t exec = nullptr;
     for (size_t k = 0; k < f->graphs.size(); k++) {
@@ -487,6 +510,9 @@ BufDesc buffer_of(const cz_film* f, int which) {
     case CZ_BUF_DEP_ITEMS: return {f->tile_items, 4 * (uint64_t)CZ_MAX_TILE_ITEMS};
     case CZ_BUF_COLOR: return {f->color, f->color ? 8 * n : 0};
     case CZ_BUF_DETAIL: return {f->detail, f->detail ? 8 * n : 0};
+    case CZ_BUF_JMETA: return {f->j_meta, f->j_meta ? 16 * (uint64_t)f->j_cap : 0};
+    case CZ_BUF_JMARK: return {f->j_mark, f->j_mark ? 4 * (uint64_t)f->nt16 : 0};
+    case CZ_BUF_JDATA: return {f->j_data, f->j_data ? 53 * 256 * 4 * (uint64_t)f->j_cap : 0};
     default: return {nullptr, 0};
     }
 }
@@ -519,6 +545,8 @@ void destroy(cz_film* f) {
     if (f->tcount_h) hipHostFree(f->tcount_h);
     if (f->titems_h) hipHostFree(f->titems_h);
     if (f->dep_n_h) hipHostFree(f->dep_n_h);
+    journal_free_host(f);
+    tools_free_host(f);
     for (auto& s : f->prof_pending) {
         hipEventDestroy(s.a);
         hipEventDestroy(s.b);
@@ -677,6 +705,9 @@ int film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s) {
     HIP_TRY(hipMemsetAsync(f->cg_m, 0, 16 * n, f->stream));
     HIP_TRY(hipMemsetAsync(f->cg_p, 0, 4 * n, f->stream));
     HIP_TRY(hipMemsetAsync(f->h_frame, 0, 4 * n, f->stream));
+    rc = journal_reset(f);
+    if (!rc) rc = tools_reset(f);
+    if (rc) return rc;
     return wait_idle(f);
 }
 

@@ -27,7 +27,7 @@ int cz_set_err(int code, const char* fmt, ...);
     } while (0)
 
 struct GraphKeyStore {
-    unsigned char bytes[10 * 4 + sizeof(cz_params)];
+    unsigned char bytes[14 * 4 + sizeof(cz_params)];
 };
 
 namespace czi {
@@ -193,6 +193,38 @@ struct cz_film {
     float* cmf = nullptr;          // [16][3] device copy of cmf_h
     uint2 *color = nullptr, *detail = nullptr;  // shade outputs, allocated when shading is turned on
     bool shade_on = false;         // run the shade pass at the end of every frame
+    // undo journal (src/film/mod.rs:357-443): the ring and its per-tile arrays
+    // exist from the first nonzero generation
+    uint32_t j_gen = 0, j_cap = 0;
+    uint64_t j_budget = 384ull << 20;
+    uint32_t *j_meta = nullptr, *j_data = nullptr;   // [cap][4], [cap][53][256]
+    uint32_t *j_mark = nullptr, *j_touch = nullptr, *j_best = nullptr;  // [cap_nt16] each
+    uint8_t* j_bits = nullptr;                       // [cap_nt16] tiles saved this frame
+    uint32_t* j_cnt = nullptr;                       // [blocks of 256 tiles] their counts
+    uint2* j_list = nullptr;                         // [cap_nt16] (tile, slot) to save
+    uint32_t* j_ctl = nullptr;                       // [4] head, records to save, generation, tiles marked
+    uint32_t* j_ctl_h = nullptr;                     // pinned staging of j_ctl
+    uint32_t* j_touch_h = nullptr;                   // pinned staging of the touches
+    std::vector<uint32_t> j_touch_host;              // Journal.touch
+    bool j_touched = false;
+    // tool contacts and the knife (src/film/mod.rs:1030-1173), allocated at first use
+    bool tools_ready = false;
+    uint32_t contact_count = 0;                      // contacts for the next frame's brush pass
+    cz_contact* contacts = nullptr;                  // [CZ_MAX_CONTACTS]
+    uint2* ctile_range = nullptr;                    // [tiles64] first item, count
+    uint32_t* ctile_items = nullptr;                 // [CZ_MAX_CONTACTS * 16]
+    uint32_t* reservoir = nullptr;                   // [CZ_MAX_BRISTLES][17]
+    void* knife_dev = nullptr;                       // the knife and its cell box (tools.hip)
+    cz_scrap* scraps = nullptr;                      // [CZ_MAX_SCRAPS]
+    uint32_t* kcnt = nullptr;                        // [knife blocks] scraps per block, then offsets
+    uint32_t* kctl = nullptr;                        // [4] scrap count, frame base, frame total
+    uint32_t kcap = 0;                               // knife blocks kcnt holds
+    bool knife_active = false;
+    uint32_t knife_grid[2] = {0, 0};                 // blocks of the knife passes (0 = nothing to do)
+    cz_contact* contacts_h = nullptr;                // pinned staging
+    uint2* crange_h = nullptr;
+    uint32_t* citems_h = nullptr;
+    void* knife_h = nullptr;
     int bin_mode = -1;             // deposit binning: -1 by count, 0 host, 1 device
     bool bin_on_device = false;    // the uploaded deposits are binned in the frame
     float2* stats_out = nullptr;
@@ -296,4 +328,24 @@ int film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s);
 void fold_tuners(std::vector<TunerSlot>& pending, std::vector<hipEvent_t>& pool);
 // Take a film out of its batch (before it is destroyed).
 void batch_detach(cz_film* f);
+// The journal pass at the start of a frame (journal.wgsl:79-98): three
+// launches on the film's stream; only while the generation is nonzero.
+void journal_record(cz_film* f);
+// Journal::flush (src/film/mod.rs:405-412): the touches since the last frame
+// to the device, before a frame is enqueued.
+int journal_flush(cz_film* f);
+// touch_rect (src/film/mod.rs:1246-1266) on the host touch array.
+void journal_touch_rect(cz_film* f, float minx, float miny, float maxx, float maxy);
+// A reshaped film starts a fresh journal (generation 0, empty ring).
+int journal_reset(cz_film* f);
+// Pinned staging the journal allocated (freed with the film).
+void journal_free_host(cz_film* f);
+// The brush pass (brush.wgsl) before the deposits and the knife passes
+// (knife.wgsl) after them, on the film's stream.
+void tools_record_brush(cz_film* f);
+void tools_record_knife(cz_film* f);
+// Pinned staging of the tool passes (freed with the film).
+void tools_free_host(cz_film* f);
+// A reshaped film drops its contacts, knife and unread outputs.
+int tools_reset(cz_film* f);
 }  // namespace czi

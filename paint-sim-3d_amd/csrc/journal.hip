@@ -1,0 +1,388 @@
+// journal.hip - the undo journal of the film step (gfx950): the pass at the
+// start of a frame that saves the tiles the frame may change, once per
+// generation, into a ring of tile records, and the restore that brings them
+// back.  shaders/journal.wgsl and src/film/mod.rs:357-443, 1246-1275.
+//
+// Record layout: slot s holds one 16x16-cell tile as 53 words per cell,
+// word-major (word w of cell li at (s*53 + w)*256 + li), so that the 256
+// threads of a block write and read each word as one coalesced 1 KiB row.
+// The reference keeps the same 53 words cell-major; the layout is private
+// to the ring.
+//
+// Slot assignment is ordered: the tiles a frame journals take the sequence
+// numbers head, head+1, ... in ascending tile order (k_journal_mark counts
+// per block of 256 tiles, k_journal_list places them by a prefix over the
+// blocks), so the ring's contents are a function of the inputs.  The
+// reference takes them by an atomic add per workgroup, in whatever order the
+// workgroups run.
+#include "film_internal.h"
+
+namespace {
+
+constexpr uint32_t kCellWords = 53u;  // h, m, rheo 2, strata_h 4, strata_pig 32, strata_cure 4, sub_refl 8, lay
+constexpr uint64_t kTileBytes = (uint64_t)kCellWords * 4u * 256u;
+
+struct JournalDev {
+    uint32_t* meta;   // [cap] uint4: tile, generation, sequence, valid
+    uint32_t* data;   // [cap][53][256]
+    uint32_t* mark;   // [tiles16] generation the tile was last saved in
+    uint32_t* touch;  // [tiles16] touched since the last frame (consumed by k_journal_mark)
+    uint32_t* best;   // [tiles16] restore: oldest sequence per tile
+    uint8_t* bits;    // [tiles16] saved this frame
+    uint32_t* cnt;    // [blocks of 256 tiles]
+    uint2* list;      // [tiles16] (tile, slot) of the records to write
+    uint32_t* ctl;    // [4] head, records to write, generation, tiles saved this frame
+    uint32_t cap;
+};
+
+JournalDev jdev(const cz_film* f) {
+    return JournalDev{f->j_meta, f->j_data, f->j_mark, f->j_touch, f->j_best, f->j_bits, f->j_cnt, f->j_list, f->j_ctl,
+                      f->j_cap};
+}
+
+// tile_busy (journal.wgsl:20-33): the tile or one of its eight neighbours
+// was active in the last frame (tile flag 32).
+__device__ bool tile_busy(const FilmDev& F, uint32_t t) {
+    const int tx = (int)(t % F.tw), ty = (int)(t / F.tw);
+    bool busy = false;
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+            const int x = tx + dx, y = ty + dy;
+            if (x < 0 || y < 0 || x >= (int)F.tw || y >= (int)F.th) continue;
+            busy = busy || (F.tile_flags[(uint32_t)x + (uint32_t)y * F.tw] & 32u) != 0u;
+        }
+    return busy;
+}
+
+// journal.wgsl:79-90: a tile not yet saved in this generation is saved if
+// it was touched or is busy.  Its mark is set here; the touches are cleared
+// as they are read (the host sends only new ones).
+__global__ __launch_bounds__(256) void k_journal_mark(FilmDev F, JournalDev J) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    bool take = false;
+    if (t < F.nt16) {
+        const uint32_t gen = J.ctl[2];
+        const uint32_t touched = J.touch[t];
+        take = J.mark[t] != gen && (touched != 0u || tile_busy(F, t));
+        if (touched != 0u) J.touch[t] = 0u;
+        if (take) J.mark[t] = gen;
+        J.bits[t] = take ? 1u : 0u;
+    }
+    const uint32_t nb = (uint32_t)__syncthreads_count(take ? 1 : 0);
+    if (threadIdx.x == 0u) J.cnt[blockIdx.x] = nb;
+}
+
+// Sequence numbers in ascending tile order and the slot records
+// (journal.wgsl:86-89).  When a frame saves more tiles than the ring holds,
+// only the last cap of them keep their slots (as if the earlier ones had
+// been written and overwritten in sequence order).
+__global__ __launch_bounds__(256) void k_journal_list(FilmDev F, JournalDev J) {
+    __shared__ uint32_t s_base, s_total, s_w[4];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    if (wave == 0u) {
+        uint32_t before = 0u, total = 0u;
+        for (uint32_t q = lane; q < gridDim.x; q += 64u) {
+            const uint32_t c = J.cnt[q];
+            total += c;
+            if (q < blockIdx.x) before += c;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            before += __shfl_xor(before, off, 64);
+            total += __shfl_xor(total, off, 64);
+        }
+        if (lane == 0u) {
+            s_base = before;
+            s_total = total;
+        }
+    }
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const bool take = t < F.nt16 && J.bits[t] != 0u;
+    const unsigned long long bm = __ballot(take);
+    if (lane == 0u) s_w[wave] = (uint32_t)__popcll(bm);
+    __syncthreads();
+    const unsigned long long lmask = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
+    uint32_t k = s_base + (uint32_t)__popcll(bm & lmask);
+    for (uint32_t w = 0; w < wave; w++) k += s_w[w];
+    const uint32_t total = s_total;
+    const uint32_t skip = total > J.cap ? total - J.cap : 0u;
+    if (take && k >= skip) {
+        const uint32_t seq = J.ctl[0] + k;
+        const uint32_t slot = seq % J.cap;
+        reinterpret_cast<uint4*>(J.meta)[slot] = make_uint4(t, J.ctl[2], seq, 1u);
+        J.list[k - skip] = make_uint2(t, slot);
+    }
+    if (blockIdx.x + 1u == gridDim.x && threadIdx.x == 0u) {
+        J.ctl[1] = total - skip;
+        J.ctl[3] = total;
+    }
+}
+
+// save_cell (journal.wgsl:35-52)
+__device__ __forceinline__ void save_cell(const FilmDev& F, uint32_t c, uint32_t* o) {
+    o[0] = __float_as_uint(F.h[c]);
+    o[256u] = __float_as_uint(F.m[c]);
+    const float2 r = F.rheo[c];
+    o[2u * 256u] = __float_as_uint(r.x);
+    o[3u * 256u] = __float_as_uint(r.y);
+    const float4 sh = F.strata_h[c];
+    o[4u * 256u] = __float_as_uint(sh.x);
+    o[5u * 256u] = __float_as_uint(sh.y);
+    o[6u * 256u] = __float_as_uint(sh.z);
+    o[7u * 256u] = __float_as_uint(sh.w);
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) {
+        const uint4 v = F.strata_pig[8u * c + k];
+        o[(8u + 4u * k) * 256u] = v.x;
+        o[(9u + 4u * k) * 256u] = v.y;
+        o[(10u + 4u * k) * 256u] = v.z;
+        o[(11u + 4u * k) * 256u] = v.w;
+    }
+    const float4 cu = F.strata_cure[c];
+    o[40u * 256u] = __float_as_uint(cu.x);
+    o[41u * 256u] = __float_as_uint(cu.y);
+    o[42u * 256u] = __float_as_uint(cu.z);
+    o[43u * 256u] = __float_as_uint(cu.w);
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) o[(44u + k) * 256u] = F.sub_refl[8u * c + k];
+    o[52u * 256u] = F.lay[c];
+}
+
+// load_cell (journal.wgsl:54-71)
+__device__ __forceinline__ void load_cell(const FilmDev& F, uint32_t c, const uint32_t* o) {
+    F.h[c] = __uint_as_float(o[0]);
+    F.m[c] = __uint_as_float(o[256u]);
+    F.rheo[c] = make_float2(__uint_as_float(o[2u * 256u]), __uint_as_float(o[3u * 256u]));
+    F.strata_h[c] = make_float4(__uint_as_float(o[4u * 256u]), __uint_as_float(o[5u * 256u]),
+                                __uint_as_float(o[6u * 256u]), __uint_as_float(o[7u * 256u]));
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++)
+        F.strata_pig[8u * c + k] = make_uint4(o[(8u + 4u * k) * 256u], o[(9u + 4u * k) * 256u],
+                                              o[(10u + 4u * k) * 256u], o[(11u + 4u * k) * 256u]);
+    F.strata_cure[c] = make_float4(__uint_as_float(o[40u * 256u]), __uint_as_float(o[41u * 256u]),
+                                   __uint_as_float(o[42u * 256u]), __uint_as_float(o[43u * 256u]));
+#pragma unroll
+    for (uint32_t k = 0; k < 8u; k++) F.sub_refl[8u * c + k] = o[(44u + k) * 256u];
+    F.lay[c] = o[52u * 256u];
+}
+
+// journal.wgsl:92-97: one block per record, one thread per cell; block 0
+// advances the head once every block has read its list entries' count.
+__global__ __launch_bounds__(256) void k_journal_save(FilmDev F, JournalDev J) {
+    const uint32_t n = J.ctl[1];
+    const uint32_t li = threadIdx.x;
+    for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
+        const uint2 e = J.list[k];
+        const uint32_t i = (e.x % F.tw) * 16u + (li & 15u), j = (e.x / F.tw) * 16u + (li >> 4);
+        if (i < F.W && j < F.H) save_cell(F, j * F.W + i, J.data + (size_t)e.y * kCellWords * 256u + li);
+    }
+    // no block of this launch reads the head; the next launch that does
+    // (k_journal_list of the next frame, the restore) is stream-ordered
+    if (blockIdx.x == 0u && threadIdx.x == 0u) J.ctl[0] = J.ctl[0] + J.ctl[3];
+}
+
+// slot_valid (journal.wgsl:73-78)
+__device__ __forceinline__ bool slot_valid(const JournalDev& J, uint32_t s, uint4& mt) {
+    mt = reinterpret_cast<const uint4*>(J.meta)[s];
+    const uint32_t head = J.ctl[0];
+    return mt.w == 1u && mt.z < head && head - mt.z <= J.cap;
+}
+
+__global__ __launch_bounds__(256) void k_journal_best(JournalDev J, uint32_t nt) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t < nt) J.best[t] = 0xffffffffu;
+}
+
+// restore_scan (journal.wgsl:100-107)
+__global__ __launch_bounds__(256) void k_journal_scan(FilmDev F, JournalDev J, uint32_t target) {
+    const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+    if (s >= J.cap) return;
+    uint4 mt;
+    if (!slot_valid(J, s, mt) || mt.y < target || mt.x >= F.nt16) return;
+    atomicMin(&J.best[mt.x], mt.z);
+}
+
+// restore_apply (journal.wgsl:109-123)
+__global__ __launch_bounds__(256) void k_journal_apply(FilmDev F, JournalDev J, uint32_t target) {
+    const uint32_t s = blockIdx.x;
+    uint4 mt;
+    if (!slot_valid(J, s, mt) || mt.y < target || mt.x >= F.nt16 || J.best[mt.x] != mt.z) return;
+    const uint32_t li = threadIdx.x;
+    if (li == 0u) F.tile_moved[mt.x] = 1u;
+    const uint32_t i = (mt.x % F.tw) * 16u + (li & 15u), j = (mt.x / F.tw) * 16u + (li >> 4);
+    if (i < F.W && j < F.H) load_cell(F, j * F.W + i, J.data + (size_t)s * kCellWords * 256u + li);
+}
+
+uint32_t cap_for(uint64_t budget) {
+    const uint64_t c = budget / kTileBytes;
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(c, 64u), 1u << 26);
+}
+
+template <typename T>
+int jalloc(cz_film* f, T** p, size_t count) {
+    void* v = nullptr;
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    hipError_t e = hipMalloc(&v, bytes);
+    if (e != hipSuccess) return set_err(CZ_ENOMEM, "journal: hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    f->allocs.push_back(v);
+    f->bytes += bytes;
+    *p = (T*)v;
+    HIP_TRY(hipMemsetAsync(v, 0, bytes, f->stream));
+    return CZ_OK;
+}
+
+// Journal::new (src/film/mod.rs:372-403), at the first nonzero generation.
+int journal_alloc(cz_film* f) {
+    if (f->j_meta) return CZ_OK;
+    const uint32_t cap = cap_for(f->j_budget);
+    const size_t nt = f->cap_nt16;
+    int rc = CZ_OK;
+    if (!rc) rc = jalloc(f, &f->j_meta, (size_t)cap * 4u);
+    if (!rc) rc = jalloc(f, &f->j_data, (size_t)cap * kCellWords * 256u);
+    if (!rc) rc = jalloc(f, &f->j_mark, nt);
+    if (!rc) rc = jalloc(f, &f->j_touch, nt);
+    if (!rc) rc = jalloc(f, &f->j_best, nt);
+    if (!rc) rc = jalloc(f, &f->j_bits, nt);
+    if (!rc) rc = jalloc(f, &f->j_cnt, (nt + 255u) / 256u);
+    if (!rc) rc = jalloc(f, &f->j_list, nt);
+    if (!rc) rc = jalloc(f, &f->j_ctl, 4);
+    if (!rc && !f->j_ctl_h) {
+        hipError_t e = hipHostMalloc((void**)&f->j_ctl_h, 4 * sizeof(uint32_t), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&f->j_touch_h, nt * sizeof(uint32_t), hipHostMallocDefault);
+        if (e != hipSuccess) rc = set_err(CZ_ENOMEM, "journal: hipHostMalloc: %s", hipGetErrorString(e));
+    }
+    if (rc) return rc;  // what was allocated stays with the film and is freed with it
+    f->j_touch_host.assign(nt, 0u);
+    f->j_touched = false;
+    f->j_cap = cap;
+    return czi::wait_idle(f);
+}
+
+}  // namespace
+
+namespace czi {
+
+void journal_record(cz_film* f) {
+    const FilmDev F = f->dev();
+    const JournalDev J = jdev(f);
+    const uint32_t blocks = (f->nt16 + 255u) / 256u;
+    hipLaunchKernelGGL(k_journal_mark, dim3(blocks), dim3(256), 0, f->stream, F, J);
+    hipLaunchKernelGGL(k_journal_list, dim3(blocks), dim3(256), 0, f->stream, F, J);
+    hipLaunchKernelGGL(k_journal_save, dim3(std::min(f->nt16, czi::kMaxTileBlocks)), dim3(256), 0, f->stream, F, J);
+}
+
+int journal_flush(cz_film* f) {
+    if (f->j_gen == 0u || !f->j_touched) return CZ_OK;
+    int rc = wait_idle(f);  // the pinned staging may still feed the last copy
+    if (rc) return rc;
+    memcpy(f->j_touch_h, f->j_touch_host.data(), (size_t)f->nt16 * sizeof(uint32_t));
+    HIP_TRY(hipMemcpyAsync(f->j_touch, f->j_touch_h, (size_t)f->nt16 * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           f->stream));
+    std::fill(f->j_touch_host.begin(), f->j_touch_host.end(), 0u);
+    f->j_touched = false;
+    return CZ_OK;
+}
+
+void journal_touch_rect(cz_film* f, float minx, float miny, float maxx, float maxy) {
+    if (f->j_gen == 0u || !(std::isfinite(minx) && std::isfinite(miny) && std::isfinite(maxx) && std::isfinite(maxy)))
+        return;
+    const float tile_m = 16.0f * f->dx;
+    const float mx = (float)(f->tw - 1u), my = (float)(f->th - 1u);
+    // ((v / tile_m).floor() as i64).clamp(0, n - 1): the cast saturates, so
+    // the clamp can be taken in float
+    const int64_t x0 = (int64_t)fminf(fmaxf(floorf(minx / tile_m), 0.0f), mx);
+    const int64_t x1 = (int64_t)fminf(fmaxf(floorf(maxx / tile_m), 0.0f), mx);
+    const int64_t y0 = (int64_t)fminf(fmaxf(floorf(miny / tile_m), 0.0f), my);
+    const int64_t y1 = (int64_t)fminf(fmaxf(floorf(maxy / tile_m), 0.0f), my);
+    if (maxx < 0.0f || maxy < 0.0f) return;
+    for (int64_t y = y0; y <= y1; y++)
+        for (int64_t x = x0; x <= x1; x++) f->j_touch_host[(size_t)(y * f->tw + x)] = 1u;
+    f->j_touched = true;
+}
+
+int journal_reset(cz_film* f) {
+    f->j_gen = 0u;
+    if (!f->j_meta) return CZ_OK;
+    HIP_TRY(hipMemsetAsync(f->j_meta, 0, (size_t)f->j_cap * 16u, f->stream));
+    HIP_TRY(hipMemsetAsync(f->j_mark, 0, (size_t)f->cap_nt16 * 4u, f->stream));
+    HIP_TRY(hipMemsetAsync(f->j_touch, 0, (size_t)f->cap_nt16 * 4u, f->stream));
+    HIP_TRY(hipMemsetAsync(f->j_ctl, 0, 16u, f->stream));
+    std::fill(f->j_touch_host.begin(), f->j_touch_host.end(), 0u);
+    f->j_touched = false;
+    return CZ_OK;
+}
+
+void journal_free_host(cz_film* f) {
+    if (f->j_ctl_h) hipHostFree(f->j_ctl_h);
+    if (f->j_touch_h) hipHostFree(f->j_touch_h);
+    f->j_ctl_h = f->j_touch_h = nullptr;
+}
+
+}  // namespace czi
+
+using namespace czi;
+
+extern "C" {
+
+int cz_film_journal_gen(cz_film* f, uint32_t gen) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    if (f->batch) return set_err(CZ_EINVAL, "journal_gen: the film is stepped by a batch (cz_batch_remove it first)");
+    if (gen != 0u) {
+        int rc = journal_alloc(f);
+        if (rc) return rc;
+    }
+    if (f->j_meta) {
+        int rc = wait_idle(f);
+        if (rc) return rc;
+        f->j_ctl_h[2] = gen;
+        HIP_TRY(hipMemcpyAsync(f->j_ctl + 2, f->j_ctl_h + 2, sizeof(uint32_t), hipMemcpyHostToDevice, f->stream));
+        rc = wait_idle(f);
+        if (rc) return rc;
+    }
+    f->j_gen = gen;
+    return CZ_OK;
+}
+
+int cz_film_journal_budget(cz_film* f, uint64_t bytes) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    if (f->j_meta) return set_err(CZ_EINVAL, "journal_budget: the ring exists (set the budget before the first nonzero generation)");
+    f->j_budget = bytes;
+    return CZ_OK;
+}
+
+int cz_film_journal_head(cz_film* f, uint32_t* head, uint32_t* cap) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    uint32_t h = 0u;
+    if (f->j_meta) {
+        int rc = wait_idle(f);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(f->j_ctl_h, f->j_ctl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, f->stream));
+        rc = wait_idle(f);
+        if (rc) return rc;
+        h = f->j_ctl_h[0];
+    }
+    if (head) *head = h;
+    if (cap) *cap = f->j_meta ? f->j_cap : cap_for(f->j_budget);
+    return CZ_OK;
+}
+
+int cz_film_touch_rect(cz_film* f, float minx, float miny, float maxx, float maxy) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    if (f->batch) return set_err(CZ_EINVAL, "touch_rect: the film is stepped by a batch (cz_batch_remove it first)");
+    journal_touch_rect(f, minx, miny, maxx, maxy);
+    return CZ_OK;
+}
+
+int cz_film_restore_journal(cz_film* f, uint32_t target) {
+    if (!f) return set_err(CZ_EINVAL, "null film");
+    if (f->batch) return set_err(CZ_EINVAL, "restore_journal: the film is stepped by a batch (cz_batch_remove it first)");
+    if (!f->j_meta) return CZ_OK;  // nothing was ever recorded
+    const FilmDev F = f->dev();
+    const JournalDev J = jdev(f);
+    hipLaunchKernelGGL(k_journal_best, dim3((f->nt16 + 255u) / 256u), dim3(256), 0, f->stream, J, f->nt16);
+    hipLaunchKernelGGL(k_journal_scan, dim3((f->j_cap + 255u) / 256u), dim3(256), 0, f->stream, F, J, target);
+    hipLaunchKernelGGL(k_journal_apply, dim3(f->j_cap), dim3(256), 0, f->stream, F, J, target);
+    HIP_TRY(hipGetLastError());
+    return CZ_OK;
+}
+
+}  // extern "C"

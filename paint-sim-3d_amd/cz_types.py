@@ -21,6 +21,24 @@ class Deposit(C.Structure):
                 ("h0", f32), ("vol", f32), ("kind", u32), ("pad", u32 * 3), ("pig", u32 * 8)]
 
 
+class Contact(C.Structure):
+    """`Contact` (src/film/mod.rs:239-248); pad is the kind: 0 bristle, 1 roller, 2 sponge, 3 body."""
+    _fields_ = [("p0", f32 * 2), ("p1", f32 * 2), ("r", f32), ("zb", f32), ("bristle", u32), ("pad", u32)]
+
+
+class Knife(C.Structure):
+    """`KnifeParams` (src/film/mod.rs:250-270)."""
+    _fields_ = [("a", f32 * 2), ("b", f32 * 2), ("dir", f32 * 2), ("sweep", f32), ("height", f32),
+                ("half_width", f32), ("active", u32), ("frame", u32), ("pad1", u32), ("bands", u32),
+                ("pad2", u32), ("pad3", u32), ("pad4", u32), ("pig", (u32 * 8) * 8), ("fill", f32 * 8),
+                ("smear", f32 * 8)]
+
+
+class Scrap(C.Structure):
+    """`Scrap` (src/film/mod.rs:272-279): vol > 0 taken off the cell, vol < 0 laid on it."""
+    _fields_ = [("cell", u32), ("vol", f32), ("frame", u32), ("pad1", u32), ("pig", u32 * 8)]
+
+
 class Settings(C.Structure):
     _fields_ = [("width_m", f32), ("height_m", f32), ("rho", f32), ("sigma", f32), ("n_index", f32),
                 ("m_cap", f32), ("contact_angle_deg", f32), ("hysteresis_deg", f32), ("h_star", f32),
@@ -85,6 +103,9 @@ class MassReport(C.Structure):
 
 assert C.sizeof(Params) == 176
 assert C.sizeof(Deposit) == 96
+assert C.sizeof(Contact) == 32
+assert C.sizeof(Knife) == 384
+assert C.sizeof(Scrap) == 48
 
 PART_NAMES = ("h", "m", "rheo", "strata_h", "strata_pig", "strata_cure", "sub_refl", "lay")
 
@@ -106,6 +127,18 @@ def default_overrides():
     o = Overrides()
     o.capcond, o.fcap, o.kshape, o.kline = 80.0, 2.5, 1.0, 0.0
     return o
+
+
+def contact_array(cs):
+    """(ctypes array, count) from a sequence of Contact."""
+    cs = list(cs)
+    arr = (Contact * max(len(cs), 1))()
+    for i, c in enumerate(cs):
+        arr[i] = c
+    return arr, len(cs)
+
+
+SCRAP_DTYPE = [("cell", "<u4"), ("vol", "<f4"), ("frame", "<u4"), ("pad1", "<u4"), ("pig", "<u4", (8,))]
 
 
 def deposit_array(deps):
