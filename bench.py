@@ -128,6 +128,188 @@ def bench_km(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_next(args):
+    """The passes beside the film step, each on the 2048^2 blob canvas (or
+    the empty canvas where the pass wants bare paint), device time from HIP
+    events around its launches (average over the frames it ran in) and its
+    compulsory HBM bytes:
+    - deposit binning on the device (k_dep_rects/scan/fill) for 16384
+      capsules in one frame, beside the host binning's wall time;
+    - the undo journal saving every tile of the canvas in one frame (53
+      words read and written per cell) and the restore of all of them;
+    - the shade pass over every cell (212 B read: the state parts it uses,
+      16 B written);
+    - the brush scrape of 4096 bristle contacts and the knife across the
+      canvas (their cells read, the changed cells' 212 B state read and
+      written);
+    - the .canvaz save and open of the canvas (host LZ4; wall time).
+    One JSON line with a record per pass."""
+    import math
+    import tempfile
+    import numpy as np
+    import canvaz_film as czf
+    import cz_types
+    import scenes
+    n = 2048
+    state = scenes.blobs_state(n, n, seed=2048)
+    s = scenes.blobs_settings(n, tilt_deg=15.0)
+    dx = s.width_m / n
+    pig = scenes.jet_paint()
+    rng = np.random.default_rng(77)
+    out = {}
+
+    def film_with(st=True):
+        f = czf.Film(n, n, s)
+        if st:
+            assert f.write_state_arrays(state)
+        return f
+
+    def rec(name, label_us, launches, bytes_per_frame, note):
+        ach = bytes_per_frame / (label_us * 1e-6) / 1e9 if label_us > 0 else 0.0
+        out[name] = {"device_us_per_frame": label_us, "launch_groups": launches, "algo_bytes": bytes_per_frame,
+                     "achieved_GBps": ach, "frac": ach / HBM_PEAK_GBS, "note": note}
+        log(f"bench next: {name}: {label_us:.1f} us/frame, {bytes_per_frame / 1e6:.1f} MB, {ach:.0f} GB/s")
+
+    # deposit binning
+    deps = []
+    for _ in range(16384):
+        x, y = rng.uniform(0.05, 0.95, 2) * n * dx
+        a = rng.uniform(0, 2 * math.pi)
+        ln = rng.uniform(1, 6) * dx
+        deps.append(czf.deposit_capsule((x, y), (x + ln * math.cos(a), y + ln * math.sin(a)), 2.5 * dx, 2e-10,
+                                        pig, dx))
+    f = film_with()
+    for mode, name in ((0, "binning_host"), (1, "binning_device")):
+        f.bin_on_device(mode)
+        f.profile(True)
+        host_s = 0.0
+        frames = 6
+        for _ in range(frames):
+            t0 = time.perf_counter()
+            f.upload_deposits(deps)
+            host_s += time.perf_counter() - t0
+            f.encode_frame(4)
+            f.sync()
+        us, cnt = f.profile_read("dep_bin")
+        dep_us, _ = f.profile_read("deposit")
+        if mode == 1:
+            # three launches; 96 B per deposit read twice (rectangles, fill),
+            # 16 B rectangle written and read, 4 B per item written
+            nb = len(deps) * (96 * 2 + 16 * 2) + 4 * 16 * len(deps)
+            rec(name, us * cnt / frames, cnt // frames, nb,
+                f"{len(deps)} capsules in one frame; host upload_deposits {host_s / frames * 1e3:.2f} ms/frame, "
+                f"k_deposit {dep_us:.0f} us")
+        else:
+            out[name] = {"host_ms_per_frame": host_s / frames * 1e3, "k_deposit_us": dep_us,
+                         "note": f"{len(deps)} capsules binned on the host (upload_deposits wall time)"}
+            log(f"bench next: {name}: {host_s / frames * 1e3:.2f} ms/frame on the host")
+        f.profile(False)
+    f.close()
+
+    # journal: every tile saved in one frame, then restored
+    f = film_with()
+    f.journal_budget(20000 * 53 * 256 * 4)
+    f.step()
+    f.sync()
+    f.journal_gen(1)
+    f.profile(True)
+    f.touch_rect(-1.0, -1.0, 10.0, 10.0)
+    f.step()
+    f.sync()
+    us, cnt = f.profile_read("journal")
+    head, cap = f.journal_head()
+    rec("journal_save", us, cnt, head * 256 * 53 * 4 * 2,
+        f"{head} tiles saved in one frame (ring of {cap})")
+    f.profile(False)
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f.restore_journal(1)
+    f.sync()
+    dt = time.perf_counter() - t0
+    rec("journal_restore", dt * 1e6, 3, head * 256 * 53 * 4 * 2,
+        "wall time of cz_film_restore_journal + sync (three launches)")
+    f.close()
+
+    # shade over every cell
+    f = film_with()
+    f.shade_output(True)
+    f.profile(True)
+    for _ in range(5):
+        f.step()
+    f.sync()
+    us, cnt = f.profile_read("shade")
+    rec("shade", us, cnt, n * n * (4 + 4 + 8 + 16 + 128 + 16 + 32 + 4 + 16), "2048^2 cells, every cell shaded")
+    f.close()
+
+    # brush: 4096 bristle contacts over a 512-cell square of the painted canvas
+    f = film_with()
+    f.profile(True)
+    cs = []
+    for b in range(4096):
+        c = cz_types.Contact()
+        x, y = (rng.uniform(768, 1280, 2)) * dx
+        c.p0[:] = (x, y)
+        c.p1[:] = (x + 3 * dx, y)
+        c.r, c.zb, c.bristle, c.pad = 1.5 * dx, 2e-5, b % 512, 0
+        cs.append(c)
+    frames = 5
+    for _ in range(frames):
+        f.upload_contacts(cs)
+        f.step()
+    f.sync()
+    us, cnt = f.profile_read("brush")
+    visited = 9 * 64 * 64  # the 64-tiles of the 512-cell square (and its edge tiles)
+    rec("brush", us, cnt, visited * 8 + 512 * 512 * 212 * 2 // 4,
+        "4096 bristle contacts in a 512x512 square, bytes assume a quarter of its cells scraped")
+    f.close()
+
+    # knife: a 0.3 m blade swept 8 cells per frame across the canvas
+    f = film_with()
+    f.profile(True)
+    frames = 5
+    for k in range(frames):
+        kn = cz_types.Knife()
+        y0 = (600 + 8 * k) * dx
+        kn.a[:] = (300 * dx, y0)
+        kn.b[:] = (300 * dx + 0.3, y0)
+        kn.dir[:] = (0.0, 1.0)
+        kn.sweep, kn.height, kn.half_width, kn.active, kn.frame, kn.bands = 8 * dx, 5e-5, 0.15, 1, k, 8
+        f.set_knife(kn)
+        f.step()
+    f.set_knife(None)
+    f.sync()
+    us, cnt = f.profile_read("knife")
+    box_cells = int((0.3 / dx + 17) * ((0.07 + dx) / dx + 17))
+    cut = int(0.3 / dx) * 8
+    rec("knife", us, cnt, box_cells * 8 * 2 + cut * (212 * 2 + 96),
+        f"{box_cells} cells visited per frame, about {cut} cut")
+    sc = f.take_scraps()
+    out["knife"]["scraps"] = int(len(sc))
+    f.close()
+
+    # .canvaz save / open (host LZ4 over read_state / write_state)
+    f = film_with()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "bench.canvaz")
+        t0 = time.perf_counter()
+        f.save_canvaz(path)
+        t_save = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        f.load_canvaz(path)
+        t_load = time.perf_counter() - t0
+    raw = sum(f.state_sizes())
+    out["canvaz"] = {"state_bytes": raw, "file_bytes": size, "save_s": t_save, "open_s": t_load,
+                     "save_MBps": raw / t_save / 1e6, "open_MBps": raw / t_load / 1e6,
+                     "note": "2048^2 blob canvas; PCIe copies and host LZ4 included"}
+    log(f"bench next: canvaz: {raw / 1e6:.0f} MB state, {size / 1e6:.0f} MB file, save {t_save:.2f} s, "
+        f"open {t_load:.2f} s")
+    f.close()
+    print(json.dumps({"metric": "beside-the-step passes", "unit": "us per frame", "n_gpus": 1,
+                      "config": {"workload": "2048x2048 blob canvas"}, "passes": out}), flush=True)
+
+
 def batch_canvases(count=64, seed=4):
     """The batched workload: `count` square canvases, side log-uniform in
     [256, 2048], seeded blob states as the plain canvas, tilt U[0, 60] deg."""
@@ -278,7 +460,7 @@ def main():
                     help="substeps as one launch per frame on small active sets (auto), always, or never")
     ap.add_argument("--events-in-timed", action="store_true",
                     help="record the roofline's HIP events inside the timed frames (default: a separate replay)")
-    ap.add_argument("--workload", default="film", choices=("film", "km", "batch", "serve"))
+    ap.add_argument("--workload", default="film", choices=("film", "km", "batch", "serve", "next"))
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload in the film run")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-budget-gb", type=int, default=200)
@@ -314,6 +496,8 @@ def main():
     if args.workload == "serve":
         print(json.dumps(bench_serve(args)), flush=True)
         return
+    if args.workload == "next":
+        return bench_next(args)
 
     n = args.size
     log(f"bench: building the {n}x{n} blob canvas")

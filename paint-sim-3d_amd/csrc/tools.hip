@@ -339,10 +339,10 @@ int tools_alloc(cz_film* f) {
     return czi::wait_idle(f);
 }
 
-// The cells the knife can reach: a + t*(b - a) + behind*dir for t in [0, 1]
-// and behind in [-dx, KNIFE_BLADE_LEN] (knife.wgsl:66-72 with behind =
-// u*sweep and u*mv = behind*dir), as a box widened by a margin for the
-// rounding of t and u.  A nearly degenerate blade/sweep pair (where that
+// The cells the knife can reach: a + t*(b - a) - behind*dir for t in [0, 1]
+// and behind in [-dx, KNIFE_BLADE_LEN] (knife.wgsl:66-72: q = t*(b - a) -
+// u*mv with mv = dir*sweep and behind = u*sweep, so behind runs against the
+// sweep), as a box widened by a margin for the rounding of t and u.  A nearly degenerate blade/sweep pair (where that
 // rounding is unbounded) takes the whole canvas.  false: no cell can pass.
 bool knife_box(const cz_film* f, const cz_knife& K, uint32_t box[4]) {
     const float abx = K.b[0] - K.a[0], aby = K.b[1] - K.a[1];
@@ -355,8 +355,8 @@ bool knife_box(const cz_film* f, const cz_knife& K, uint32_t box[4]) {
     for (int ti = 0; ti < 2; ti++)
         for (int bi = 0; bi < 2; bi++) {
             const double b = bi ? (double)KNIFE_BLADE_LEN : -dx;
-            const double x = (double)K.a[0] + ti * (double)abx + b * (double)K.dir[0];
-            const double y = (double)K.a[1] + ti * (double)aby + b * (double)K.dir[1];
+            const double x = (double)K.a[0] + ti * (double)abx - b * (double)K.dir[0];
+            const double y = (double)K.a[1] + ti * (double)aby - b * (double)K.dir[1];
             lo_x = std::min(lo_x, x); hi_x = std::max(hi_x, x);
             lo_y = std::min(lo_y, y); hi_y = std::max(hi_y, y);
         }

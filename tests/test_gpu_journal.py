@@ -127,10 +127,19 @@ def test_ring_overflow_in_one_frame_equals_restatement(gpu):
         compare_state(g, o, f"restore({target})")
 
 
+def differing(a, b):
+    """Names of the state parts that differ (read_state lists)."""
+    return [PARTS[k] for k in range(8) if a[k] != b[k]]
+
+
 def test_undo_history_round_trip_at_512(gpu):
     w, h = 512, 384
     s = scenes.blobs_settings(w, tilt_deg=25.0)
     s.height_m = h * s.width_m / w
+    # Curing goes on in tiles no frame journals (paint at rest), and restore
+    # leaves them as they are, as the reference's does; without curing every
+    # change is journaled and undo is exact.
+    s.cure_rate = 0.0
     g = czf.Film(w, h, s)
     g.journal_budget(4096 * TILE_BYTES)  # room for every tile of the three levels
     for _ in range(3):
@@ -154,18 +163,18 @@ def test_undo_history_round_trip_at_512(gpu):
     u.begin_stroke()
     paint(0.3)
     s1 = g.read_state()
-    assert s1 != s0
+    assert differing(s1, s0)
     u.begin_stroke()
     paint(0.6)
-    assert g.read_state() != s1
+    assert differing(g.read_state(), s1)
     u.rollback()
-    assert g.read_state() == s1
+    assert differing(g.read_state(), s1) == []
     u.begin_full()
     paint(0.45)
     u.rollback()
-    assert g.read_state() == s1
+    assert differing(g.read_state(), s1) == []
     u.rollback()
-    assert g.read_state() == s0
+    assert differing(g.read_state(), s0) == []
     assert u.gen == 6 and g.journal_head()[1] == 4096
 
 

@@ -148,6 +148,41 @@ def test_restore_returns_each_generations_start():
     assert diff_cells(o, s0) == dict.fromkeys(PARTS, 0)
 
 
+def test_restore_leaves_unjournaled_curing_alone():
+    """Paint at rest keeps curing in tiles the generation has not journaled
+    yet (a tile is saved when a frame first touches it or finds it busy);
+    restore brings a tile back to the frame it was saved in, or leaves it
+    (journal.wgsl:109-123).  So the heights, strata and lay come back exactly
+    and only cure levels (and the rheology that follows them) differ."""
+    w, h = 192, 144
+    s = scenes.blobs_settings(w, tilt_deg=25.0)
+    s.height_m = h * s.width_m / w
+    _, o = make_pair(w, h, s, None, gpu=False)
+    for _ in range(3):
+        o.step([])
+    pig = scenes.jet_paint()
+    dx = s.width_m / w
+
+    def paint(yf):
+        for f in range(8):
+            deps = []
+            if f < 3:
+                x = (0.1 + 0.25 * f) * w * dx
+                deps = [oracle_capsule((x, yf * h * dx), (x + 0.15 * w * dx, (yf + 0.1) * h * dx), 6 * dx, 4.2e-9,
+                                       pig, dx)]
+            o.step(deps)
+
+    o.journal_begin(1, 4096)
+    paint(0.3)
+    s1 = snapshot(o)
+    o.journal_begin(2, 4096)
+    paint(0.6)
+    o.restore_journal(2)
+    d = diff_cells(o, s1)
+    assert d["h"] == 0 and d["strata_h"] == 0 and d["strata_pig"] == 0 and d["lay"] == 0
+    assert d["strata_cure"] > 0
+
+
 @pytest.mark.parametrize("target", [1, 2, 3])
 def test_restore_takes_the_oldest_record_at_or_after_target(target):
     """Three generations each touching tile 0 with a different state: the

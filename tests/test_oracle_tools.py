@@ -159,7 +159,7 @@ def knife(a, b, dirv, sweep, height, fill=0.0, smear=0.0, bands=8, frame=5, pig_
 
 
 def blade_cells(a, b, dirv, sweep):
-    """(t, behind) of every cell centre, in float64: p = a + t (b - a) + behind dir."""
+    """(t, behind, u) of every cell centre, in float64: p = a + t (b - a) - behind dir."""
     px, py = centres()
     ab = np.array(b, np.float64) - a
     mv = np.array(dirv, np.float64) * sweep
