@@ -264,7 +264,8 @@ enum {
     CZ_BUF_DETAIL = 37,     /* f16 [n][4]: shade detail_out (method + turn, cure, yield stress, thickness) */
     CZ_BUF_JMETA = 38,      /* u32 [journal cap][4]: slot (tile16, generation, sequence, valid) */
     CZ_BUF_JMARK = 39,      /* u32 [tiles16]: generation each tile was last saved in */
-    CZ_BUF_JDATA = 40       /* u32 [journal cap][53][256]: word w of cell li of slot s at (s*53 + w)*256 + li */
+    CZ_BUF_JDATA = 40       /* u32 [journal cap][13568]: per slot the parts h, m, rheo, strata_h, strata_pig,
+                               strata_cure, sub_refl, lay, each as [256 cells (tile rows)][its words] */
 };
 
 typedef struct cz_film cz_film;

@@ -3,11 +3,13 @@
 // generation, into a ring of tile records, and the restore that brings them
 // back.  shaders/journal.wgsl and src/film/mod.rs:357-443, 1246-1275.
 //
-// Record layout: slot s holds one 16x16-cell tile as 53 words per cell,
-// word-major (word w of cell li at (s*53 + w)*256 + li), so that the 256
-// threads of a block write and read each word as one coalesced 1 KiB row.
-// The reference keeps the same 53 words cell-major; the layout is private
-// to the ring.
+// Record layout: slot s holds one 16x16-cell tile, 53 words per cell, as
+// the eight state parts one after another, each part's 256 cells in tile
+// row order (the part's word k of cell li at s*13568 + 256*w0 + li*n + k,
+// w0 the part's first word in save_cell order and n its words per cell).
+// A tile row of a part is then contiguous on both sides of the copy.  The
+// reference keeps the same 53 words cell by cell; the layout is private to
+// the ring.
 //
 // Slot assignment is ordered: the tiles a frame journals take the sequence
 // numbers head, head+1, ... in ascending tile order (k_journal_mark counts
@@ -117,63 +119,67 @@ __global__ __launch_bounds__(256) void k_journal_list(FilmDev F, JournalDev J) {
     }
 }
 
-// save_cell (journal.wgsl:35-52)
-__device__ __forceinline__ void save_cell(const FilmDev& F, uint32_t c, uint32_t* o) {
-    o[0] = __float_as_uint(F.h[c]);
-    o[256u] = __float_as_uint(F.m[c]);
-    const float2 r = F.rheo[c];
-    o[2u * 256u] = __float_as_uint(r.x);
-    o[3u * 256u] = __float_as_uint(r.y);
-    const float4 sh = F.strata_h[c];
-    o[4u * 256u] = __float_as_uint(sh.x);
-    o[5u * 256u] = __float_as_uint(sh.y);
-    o[6u * 256u] = __float_as_uint(sh.z);
-    o[7u * 256u] = __float_as_uint(sh.w);
+// A record keeps the tile's cells part by part, in save_cell's order
+// (journal.wgsl:35-52): h, m, rheo, strata_h, strata_pig, strata_cure,
+// sub_refl, lay, each part as 256 cells x its bytes per cell, tile row by
+// tile row.  A tile row of a part is then one contiguous run both in the
+// state buffer and in the record, and save and restore are copies of those
+// runs in 16-byte (or 8- or 4-byte) lanes.
+template <uint32_t B>
+struct PartUnit;
+template <> struct PartUnit<4> { using T = uint32_t; };
+template <> struct PartUnit<8> { using T = uint2; };
+template <> struct PartUnit<16> { using T = uint4; };
+
+// Copy one part of one tile between the state (row pitch W cells) and a
+// record (row pitch 16 cells); TO_RECORD picks the direction.  rc, rr: the
+// tile's columns and rows inside the canvas.
+template <uint32_t B, bool TO_RECORD>
+__device__ __forceinline__ void copy_part(void* state, uint32_t* rec_words, uint32_t W, uint32_t x0, uint32_t y0,
+                                          uint32_t rc, uint32_t rr) {
+    constexpr uint32_t U = B < 16u ? B : 16u;   // bytes per lane
+    constexpr uint32_t UB = B / U;              // lanes per cell
+    constexpr uint32_t L = 16u * UB;            // lanes per tile row
+    using T = typename PartUnit<U>::T;
+    T* st = reinterpret_cast<T*>(state);
+    T* rec = reinterpret_cast<T*>(rec_words);
 #pragma unroll
-    for (uint32_t k = 0; k < 8u; k++) {
-        const uint4 v = F.strata_pig[8u * c + k];
-        o[(8u + 4u * k) * 256u] = v.x;
-        o[(9u + 4u * k) * 256u] = v.y;
-        o[(10u + 4u * k) * 256u] = v.z;
-        o[(11u + 4u * k) * 256u] = v.w;
+    for (uint32_t it = 0; it < UB; it++) {
+        const uint32_t e = threadIdx.x + it * 256u;   // 256 * UB lanes per tile
+        const uint32_t row = e / L, col = e % L;
+        if (row < rr && col / UB < rc) {
+            const size_t s_idx = ((size_t)(y0 + row) * W + x0) * UB + col;
+            if (TO_RECORD) rec[e] = st[s_idx];
+            else st[s_idx] = rec[e];
+        }
     }
-    const float4 cu = F.strata_cure[c];
-    o[40u * 256u] = __float_as_uint(cu.x);
-    o[41u * 256u] = __float_as_uint(cu.y);
-    o[42u * 256u] = __float_as_uint(cu.z);
-    o[43u * 256u] = __float_as_uint(cu.w);
-#pragma unroll
-    for (uint32_t k = 0; k < 8u; k++) o[(44u + k) * 256u] = F.sub_refl[8u * c + k];
-    o[52u * 256u] = F.lay[c];
 }
 
-// load_cell (journal.wgsl:54-71)
-__device__ __forceinline__ void load_cell(const FilmDev& F, uint32_t c, const uint32_t* o) {
-    F.h[c] = __uint_as_float(o[0]);
-    F.m[c] = __uint_as_float(o[256u]);
-    F.rheo[c] = make_float2(__uint_as_float(o[2u * 256u]), __uint_as_float(o[3u * 256u]));
-    F.strata_h[c] = make_float4(__uint_as_float(o[4u * 256u]), __uint_as_float(o[5u * 256u]),
-                                __uint_as_float(o[6u * 256u]), __uint_as_float(o[7u * 256u]));
-#pragma unroll
-    for (uint32_t k = 0; k < 8u; k++)
-        F.strata_pig[8u * c + k] = make_uint4(o[(8u + 4u * k) * 256u], o[(9u + 4u * k) * 256u],
-                                              o[(10u + 4u * k) * 256u], o[(11u + 4u * k) * 256u]);
-    F.strata_cure[c] = make_float4(__uint_as_float(o[40u * 256u]), __uint_as_float(o[41u * 256u]),
-                                   __uint_as_float(o[42u * 256u]), __uint_as_float(o[43u * 256u]));
-#pragma unroll
-    for (uint32_t k = 0; k < 8u; k++) F.sub_refl[8u * c + k] = o[(44u + k) * 256u];
-    F.lay[c] = o[52u * 256u];
+// The word offset of each part in a record (256 cells x the part's words).
+constexpr uint32_t kOffM = 256u * 1u, kOffRheo = 256u * 2u, kOffSh = 256u * 4u, kOffPig = 256u * 8u,
+                   kOffCure = 256u * 40u, kOffSub = 256u * 44u, kOffLay = 256u * 52u;
+
+template <bool TO_RECORD>
+__device__ __forceinline__ void copy_tile(const FilmDev& F, uint32_t t, uint32_t* rec) {
+    const uint32_t x0 = (t % F.tw) * 16u, y0 = (t / F.tw) * 16u;
+    const uint32_t rc = min(16u, F.W - x0), rr = min(16u, F.H - y0);
+    copy_part<4, TO_RECORD>(F.h, rec, F.W, x0, y0, rc, rr);
+    copy_part<4, TO_RECORD>(F.m, rec + kOffM, F.W, x0, y0, rc, rr);
+    copy_part<8, TO_RECORD>(F.rheo, rec + kOffRheo, F.W, x0, y0, rc, rr);
+    copy_part<16, TO_RECORD>(F.strata_h, rec + kOffSh, F.W, x0, y0, rc, rr);
+    copy_part<128, TO_RECORD>(F.strata_pig, rec + kOffPig, F.W, x0, y0, rc, rr);
+    copy_part<16, TO_RECORD>(F.strata_cure, rec + kOffCure, F.W, x0, y0, rc, rr);
+    copy_part<32, TO_RECORD>(F.sub_refl, rec + kOffSub, F.W, x0, y0, rc, rr);
+    copy_part<4, TO_RECORD>(F.lay, rec + kOffLay, F.W, x0, y0, rc, rr);
 }
 
 // journal.wgsl:92-97: one block per record, one thread per cell; block 0
 // advances the head once every block has read its list entries' count.
 __global__ __launch_bounds__(256) void k_journal_save(FilmDev F, JournalDev J) {
     const uint32_t n = J.ctl[1];
-    const uint32_t li = threadIdx.x;
     for (uint32_t k = blockIdx.x; k < n; k += gridDim.x) {
         const uint2 e = J.list[k];
-        const uint32_t i = (e.x % F.tw) * 16u + (li & 15u), j = (e.x / F.tw) * 16u + (li >> 4);
-        if (i < F.W && j < F.H) save_cell(F, j * F.W + i, J.data + (size_t)e.y * kCellWords * 256u + li);
+        copy_tile<true>(F, e.x, J.data + (size_t)e.y * kCellWords * 256u);
     }
     // no block of this launch reads the head; the next launch that does
     // (k_journal_list of the next frame, the restore) is stream-ordered
@@ -206,10 +212,8 @@ __global__ __launch_bounds__(256) void k_journal_apply(FilmDev F, JournalDev J, 
     const uint32_t s = blockIdx.x;
     uint4 mt;
     if (!slot_valid(J, s, mt) || mt.y < target || mt.x >= F.nt16 || J.best[mt.x] != mt.z) return;
-    const uint32_t li = threadIdx.x;
-    if (li == 0u) F.tile_moved[mt.x] = 1u;
-    const uint32_t i = (mt.x % F.tw) * 16u + (li & 15u), j = (mt.x / F.tw) * 16u + (li >> 4);
-    if (i < F.W && j < F.H) load_cell(F, j * F.W + i, J.data + (size_t)s * kCellWords * 256u + li);
+    if (threadIdx.x == 0u) F.tile_moved[mt.x] = 1u;
+    copy_tile<false>(F, mt.x, J.data + (size_t)s * kCellWords * 256u);
 }
 
 uint32_t cap_for(uint64_t budget) {

@@ -18,6 +18,21 @@ PARTS = ("h", "m", "rheo", "strata_h", "strata_pig", "strata_cure", "sub_refl", 
 TILE_BYTES = 53 * 256 * 4
 
 
+def _aos_index():
+    """Word index in a device record of word w of cell li: the record holds
+    the parts one after another (first word w0, n words per cell), each as
+    its 256 cells' words."""
+    parts = [(0, 1), (1, 1), (2, 2), (4, 4), (8, 32), (40, 4), (44, 8), (52, 1)]
+    idx = np.zeros((256, 53), np.int64)
+    for w0, n in parts:
+        for k in range(n):
+            idx[:, w0 + k] = 256 * w0 + np.arange(256) * n + k
+    return idx
+
+
+AOS_INDEX = _aos_index()
+
+
 def compare_journal(g, o, w, h, where):
     head, cap = g.journal_head()
     assert head == o.journal_head(), f"{where}: head {head} vs {o.journal_head()}"
@@ -27,7 +42,7 @@ def compare_journal(g, o, w, h, where):
     bad = np.argwhere(mg != mo)
     assert bad.size == 0, f"{where}: jmeta differs at slots {bad[:6, 0].tolist()}: {mg[bad[0][0]]} vs {mo[bad[0][0]]}"
     assert np.array_equal(g.buffer("jmark"), o.buf("jmark")), f"{where}: jmark"
-    dg = g.buffer("jdata").reshape(cap, 53, 256).transpose(0, 2, 1)   # to [slot][cell][word]
+    dg = g.buffer("jdata").reshape(cap, 256 * 53)[:, AOS_INDEX]       # to [slot][cell][word]
     do = o.buf("jdata").reshape(cap, 256, 53)
     tw = (w + 15) // 16
     li = np.arange(256)
