@@ -844,11 +844,10 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
                     cy = oy + dj;
                     const int tx = i + cx, ty = j + cy;
                     if (tx < 0 || ty < 0 || tx >= W_ || ty >= H_) continue;
-                    // both reads issued, then a select (no pointer select between spaces)
-                    const bool inside = in_win(tx, ty);
-                    const float hl = sw.h[inside ? wq(tx, ty) : 0];
-                    const float hg = F.h[idx(F, (uint32_t)tx, (uint32_t)ty)];
-                    const float ht = inside ? hl : hg;
+                    // the window when the target is inside it, else HBM
+                    float ht;
+                    if (in_win(tx, ty)) ht = sw.h[wq(tx, ty)];
+                    else ht = F.h[idx(F, (uint32_t)tx, (uint32_t)ty)];
                     if (ht > 0.0f) continue;
                 } else if (di == 0 && dj == 0) {
                     continue;
@@ -1074,27 +1073,19 @@ struct WinReadT {
     float hold1_, theta_;
     __device__ __forceinline__ float hold1() const { return hold1_; }
     __device__ __forceinline__ float theta() const { return theta_; }
-    // the far steps of cell_excess: the window when inside, else HBM (both
-    // reads are issued, so no pointer selects between the two spaces)
+    // the far steps of cell_excess: the window when inside, else HBM
     __device__ __forceinline__ bool in(int i, int j) const {
         return (unsigned)(i - wx0) < (unsigned)N && (unsigned)(j - wy0) < (unsigned)N;
     }
     __device__ __forceinline__ float h_far(int i, int j) const {
-        const bool inside = in(i, j);
+        if (in(i, j)) return w.h[q(i, j)];
         const uint32_t c = (uint32_t)j * F.W + (uint32_t)i;
-        const float vl = w.h[inside ? q(i, j) : 0];
-        float vg = F.h[c];
-        if (SNAP) {
-            const float vs = F.h_sub[c];
-            vg = cell_active(F, (uint32_t)i, (uint32_t)j) ? vs : vg;
-        }
-        return inside ? vl : vg;
+        if (SNAP && cell_active(F, (uint32_t)i, (uint32_t)j)) return F.h_sub[c];
+        return F.h[c];
     }
     __device__ __forceinline__ uint32_t delta_far(int i, int j) const {
-        const bool inside = in(i, j);
-        const uint32_t vl = w.d[inside ? q(i, j) : 0];
-        const uint32_t vg = F.delta[(uint32_t)j * F.W + (uint32_t)i];
-        return inside ? vl : vg;
+        if (in(i, j)) return w.d[q(i, j)];
+        return F.delta[(uint32_t)j * F.W + (uint32_t)i];
     }
     // cells within 16 of the tile: one of the 3x3 tiles around it
     __device__ __forceinline__ bool active(int i, int j) const {

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import canvaz_film as czf
+import cz_types
 import scenes
 from helpers import make_pair, oracle_capsule
 
@@ -217,6 +218,15 @@ def test_batch_refuses_a_journaling_film(gpu):
     with pytest.raises(czf.FilmError, match="undo journal"):
         b.add(g)
     g.journal_gen(0)
+    k = cz_types.Knife()
+    k.a[:] = (0.0, 0.0)
+    k.b[:] = (0.01, 0.0)
+    k.dir[:] = (0.0, 1.0)
+    k.sweep, k.active = 0.001, 1
+    g.set_knife(k)
+    with pytest.raises(czf.FilmError, match="knife"):
+        b.add(g)
+    g.set_knife(None)
     b.add(g)
     with pytest.raises(czf.FilmError, match="batch"):
         g.journal_gen(2)
