@@ -763,9 +763,9 @@ struct LapWin {
 
 struct LapShared {
     LapWin sw;
-    float s_q[WIN * WIN];
-    float s_h[WIN * WIN];
-    uint8_t s_y[WIN * WIN];
+    // plug per window cell: x = its height (0 dry), negated when it has a
+    // yield stress; y = its hb_flux.  One 8-byte read per disc cell.
+    float2 s_hq[WIN * WIN];
     uint32_t s_d[RING_SPAN * RING_SPAN];
     float2 s_fn[RING_SPAN * RING_SPAN];
     float s_us[RING_SPAN * RING_SPAN];
@@ -775,9 +775,7 @@ struct LapShared {
 __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, LapShared& S, int do_nose, int do_plug,
                                               int do_clear) {
     LapWin& sw = S.sw;
-    float* s_q = S.s_q;
-    float* s_h = S.s_h;
-    uint8_t* s_y = S.s_y;
+    float2* s_hq = S.s_hq;
     uint32_t* s_d = S.s_d;
     float2* s_fn = S.s_fn;
     float* s_us = S.s_us;
@@ -904,18 +902,16 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
     if (plug_on) {
         for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += 256u) {
             float hn = sw.h[q], qv = 0.0f;
-            uint8_t yl = 0;
+            bool yl = false;
             if (hn > 0.0f) {
                 const float2 r = sw.rheo[q];
-                yl = r.y > 0.0f ? 1 : 0;
+                yl = r.y > 0.0f;
                 float oy_, oup_;
                 qv = hb_flux(F.P.phys[2], gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
             } else {
                 hn = 0.0f;
             }
-            s_q[q] = qv;
-            s_h[q] = hn;
-            s_y[q] = yl;
+            s_hq[q] = make_float2(yl ? -hn : hn, qv);
         }
     }
     __syncthreads();
@@ -979,11 +975,11 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
         for (int dy = 0; dy < 2 * PLUG_RADIUS + 1; dy++) {
             for (int dxi = 0; dxi < 2 * PLUG_RADIUS + 1; dxi++) {
                 const int q = (int)(_ly + dy) * PLUG_SPAN + (int)(_lx + dxi);
-                const float hn = s_h[q];
-                if (hn <= 0.0f) continue;
-                if (s_y[q]) yielding = true;
-                sum = sum + s_q[q];
-                mass = mass + hn;
+                const float2 v = s_hq[q];
+                if (v.x == 0.0f) continue;
+                if (v.x < 0.0f) yielding = true;
+                sum = sum + v.y;
+                mass = mass + cz_abs(v.x);
             }
         }
         if (yielding && mass > 0.0f) tv = sum / mass;
