@@ -17,10 +17,29 @@ namespace cz {
 
 // Binds, per item of the launch: _a (item slot), F (the item's canvas) and
 // _t (its tile).
+// Which list items a block takes.  Workgroups are dealt to the 8 XCDs in
+// turn (block b on XCD b % 8), each XCD with its own L2; cutting the list
+// (ascending tiles) into 8 contiguous chunks, one per XCD, puts neighbouring
+// tiles on the same XCD, so the halos of their LDS windows are fetched into
+// one L2 instead of several.  Tiles are independent within a launch, so any
+// assignment gives the same results; a grid that is not a multiple of 8
+// keeps the plain stride.
+struct XcdRange {
+    uint32_t first, end, step;
+};
+__device__ __forceinline__ XcdRange xcd_range(uint32_t n) {
+    constexpr uint32_t NX = 8u;
+    if (gridDim.x < NX || (gridDim.x & (NX - 1u)) != 0u) return {blockIdx.x, n, gridDim.x};
+    const uint32_t chunk = (n + NX - 1u) / NX;
+    const uint32_t lo = (blockIdx.x % NX) * chunk;
+    return {lo + blockIdx.x / NX, min(n, lo + chunk), gridDim.x / NX};
+}
+
 #define CZ_FOR_TILES(WS)                                                             \
     const uint32_t _nitems = (WS).count();                                           \
     const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;                  \
-    for (uint32_t _a = blockIdx.x; _a < _nitems; _a += gridDim.x)                    \
+    const XcdRange _rg = xcd_range(_nitems);                                         \
+    for (uint32_t _a = _rg.first; _a < _rg.end; _a += _rg.step)                      \
         if (const uint32_t _item = (WS).item(_a); true)                              \
             if (const FilmDev& F = (WS).film(_item); true)                           \
                 if (const uint32_t _t = (WS).tile(F, _item); true)
@@ -1001,7 +1020,8 @@ template <class WS>
 __global__ __launch_bounds__(256) void k_fluxes_faces(WS ws) {
     const uint32_t nact = ws.count();
     const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
-    for (uint32_t wk = blockIdx.x; wk < 4u * nact; wk += gridDim.x) {
+    const XcdRange rg = xcd_range(4u * nact);
+    for (uint32_t wk = rg.first; wk < rg.end; wk += rg.step) {
         const uint32_t it = ws.item(wk >> 2);
         const FilmDev& F = ws.film(it);
         float* flux = (float*)F.flux;
