@@ -189,12 +189,16 @@ struct Reader {
     }
 };
 
-// unpack into parts of the expected sizes (nparts of them).
+// unpack into parts of the expected sizes (nparts of them): the chunk table
+// is read in file order, then the chunks are decoded on worker threads; the
+// first failing chunk in file order is the one reported.
 int unpack(const uint8_t* data, uint64_t n, uint8_t* const* parts, const uint64_t* sizes, uint32_t nparts) {
     Reader r{data, n};
     uint32_t count = 0;
     if (!r.take(&count, 4)) return set_err(CZ_EINVAL, "canvaz: truncated");
     if (count != nparts) return set_err(CZ_EINVAL, "canvaz: %u parts, expected %u", count, nparts);
+    struct Job { uint32_t part, chunk; const uint8_t* src; uint64_t n; uint8_t* dst; uint32_t raw; };
+    std::vector<Job> jobs;
     for (uint32_t k = 0; k < nparts; k++) {
         uint64_t len = 0;
         uint32_t nch = 0;
@@ -207,14 +211,26 @@ int unpack(const uint8_t* data, uint64_t n, uint8_t* const* parts, const uint64_
             if (!r.take(&clen, 8) || clen < 4u || clen > r.n - r.off) return set_err(CZ_EINVAL, "canvaz: bad chunk");
             uint32_t raw = 0;
             memcpy(&raw, r.p + r.off, 4);
-            uint64_t got = 0;
-            if (raw > len - op || !decompress_block(r.p + r.off + 4, clen - 4u, parts[k] + op, raw, &got) || got != raw)
-                return set_err(CZ_EINVAL, "canvaz: part %u chunk %u does not decode", k, c);
+            if (raw > len - op) return set_err(CZ_EINVAL, "canvaz: part %u chunk %u does not decode", k, c);
+            jobs.push_back({k, c, r.p + r.off + 4, clen - 4u, parts[k] + op, raw});
             r.off += clen;
             op += raw;
         }
         if (op != len) return set_err(CZ_EINVAL, "canvaz: part %u decodes to %llu bytes", k, (unsigned long long)op);
     }
+    std::vector<uint8_t> ok(jobs.size(), 0);
+    const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 16u));
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nt; w++)
+        pool.emplace_back([&, w]() {
+            for (size_t j = w; j < jobs.size(); j += nt) {
+                uint64_t got = 0;
+                ok[j] = decompress_block(jobs[j].src, jobs[j].n, jobs[j].dst, jobs[j].raw, &got) && got == jobs[j].raw;
+            }
+        });
+    for (auto& t : pool) t.join();
+    for (size_t j = 0; j < jobs.size(); j++)
+        if (!ok[j]) return set_err(CZ_EINVAL, "canvaz: part %u chunk %u does not decode", jobs[j].part, jobs[j].chunk);
     return CZ_OK;
 }
 
@@ -301,10 +317,16 @@ int cz_film_load_canvaz(cz_film* f, const char* path) {
     FILE* fp = fopen(path, "rb");
     if (!fp) return set_err(CZ_EINVAL, "load_canvaz: cannot open %s", path);
     std::vector<uint8_t> data;
-    uint8_t buf[1 << 16];
-    size_t got;
-    while ((got = fread(buf, 1, sizeof(buf), fp)) > 0) data.insert(data.end(), buf, buf + got);
+    long fsize = -1;
+    if (fseek(fp, 0, SEEK_END) == 0) fsize = ftell(fp);
+    if (fsize < 0 || fseek(fp, 0, SEEK_SET) != 0) {
+        fclose(fp);
+        return set_err(CZ_EINVAL, "load_canvaz: cannot read %s", path);
+    }
+    data.resize((size_t)fsize);
+    const size_t got = fsize > 0 ? fread(data.data(), 1, data.size(), fp) : 0;
     fclose(fp);
+    if (got != data.size()) return set_err(CZ_EINVAL, "load_canvaz: reading %s failed", path);
     uint32_t magic = 0, version = 0, w = 0, h = 0;
     if (data.size() < 44u) return set_err(CZ_EINVAL, "load_canvaz: not a painting file");
     memcpy(&magic, data.data(), 4);

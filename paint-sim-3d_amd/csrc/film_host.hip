@@ -510,9 +510,9 @@ BufDesc buffer_of(const cz_film* f, int which) {
     case CZ_BUF_DEP_ITEMS: return {f->tile_items, 4 * (uint64_t)CZ_MAX_TILE_ITEMS};
     case CZ_BUF_COLOR: return {f->color, f->color ? 8 * n : 0};
     case CZ_BUF_DETAIL: return {f->detail, f->detail ? 8 * n : 0};
-    case CZ_BUF_JMETA: return {f->j_meta, f->j_meta ? 16 * (uint64_t)f->j_cap : 0};
-    case CZ_BUF_JMARK: return {f->j_mark, f->j_mark ? 4 * (uint64_t)f->nt16 : 0};
-    case CZ_BUF_JDATA: return {f->j_data, f->j_data ? 53 * 256 * 4 * (uint64_t)f->j_cap : 0};
+    case CZ_BUF_JMETA: return {f->j_cap ? f->j_meta : nullptr, 16 * (uint64_t)f->j_cap};
+    case CZ_BUF_JMARK: return {f->j_cap ? f->j_mark : nullptr, f->j_cap ? 4 * (uint64_t)f->nt16 : 0};
+    case CZ_BUF_JDATA: return {f->j_cap ? f->j_data : nullptr, 53 * 256 * 4 * (uint64_t)f->j_cap};
     default: return {nullptr, 0};
     }
 }

@@ -235,30 +235,35 @@ int jalloc(cz_film* f, T** p, size_t count) {
 }
 
 // Journal::new (src/film/mod.rs:372-403), at the first nonzero generation.
+// The ring exists once j_cap is set; a failed allocation leaves j_cap 0 and
+// a later call allocates what is still missing.
 int journal_alloc(cz_film* f) {
-    if (f->j_meta) return CZ_OK;
+    if (f->j_cap) return CZ_OK;
     const uint32_t cap = cap_for(f->j_budget);
     const size_t nt = f->cap_nt16;
     int rc = CZ_OK;
-    if (!rc) rc = jalloc(f, &f->j_meta, (size_t)cap * 4u);
-    if (!rc) rc = jalloc(f, &f->j_data, (size_t)cap * kCellWords * 256u);
-    if (!rc) rc = jalloc(f, &f->j_mark, nt);
-    if (!rc) rc = jalloc(f, &f->j_touch, nt);
-    if (!rc) rc = jalloc(f, &f->j_best, nt);
-    if (!rc) rc = jalloc(f, &f->j_bits, nt);
-    if (!rc) rc = jalloc(f, &f->j_cnt, (nt + 255u) / 256u);
-    if (!rc) rc = jalloc(f, &f->j_list, nt);
-    if (!rc) rc = jalloc(f, &f->j_ctl, 4);
-    if (!rc && !f->j_ctl_h) {
-        hipError_t e = hipHostMalloc((void**)&f->j_ctl_h, 4 * sizeof(uint32_t), hipHostMallocDefault);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&f->j_touch_h, nt * sizeof(uint32_t), hipHostMallocDefault);
+    if (!rc && !f->j_meta) rc = jalloc(f, &f->j_meta, (size_t)cap * 4u);
+    if (!rc && !f->j_data) rc = jalloc(f, &f->j_data, (size_t)cap * kCellWords * 256u);
+    if (!rc && !f->j_mark) rc = jalloc(f, &f->j_mark, nt);
+    if (!rc && !f->j_touch) rc = jalloc(f, &f->j_touch, nt);
+    if (!rc && !f->j_best) rc = jalloc(f, &f->j_best, nt);
+    if (!rc && !f->j_bits) rc = jalloc(f, &f->j_bits, nt);
+    if (!rc && !f->j_cnt) rc = jalloc(f, &f->j_cnt, (nt + 255u) / 256u);
+    if (!rc && !f->j_list) rc = jalloc(f, &f->j_list, nt);
+    if (!rc && !f->j_ctl) rc = jalloc(f, &f->j_ctl, 4);
+    if (!rc) {
+        hipError_t e = hipSuccess;
+        if (!f->j_ctl_h) e = hipHostMalloc((void**)&f->j_ctl_h, 4 * sizeof(uint32_t), hipHostMallocDefault);
+        if (e == hipSuccess && !f->j_touch_h)
+            e = hipHostMalloc((void**)&f->j_touch_h, nt * sizeof(uint32_t), hipHostMallocDefault);
         if (e != hipSuccess) rc = set_err(CZ_ENOMEM, "journal: hipHostMalloc: %s", hipGetErrorString(e));
     }
     if (rc) return rc;  // what was allocated stays with the film and is freed with it
     f->j_touch_host.assign(nt, 0u);
     f->j_touched = false;
-    f->j_cap = cap;
-    return czi::wait_idle(f);
+    rc = czi::wait_idle(f);
+    if (!rc) f->j_cap = cap;
+    return rc;
 }
 
 }  // namespace
@@ -305,7 +310,7 @@ void journal_touch_rect(cz_film* f, float minx, float miny, float maxx, float ma
 
 int journal_reset(cz_film* f) {
     f->j_gen = 0u;
-    if (!f->j_meta) return CZ_OK;
+    if (!f->j_cap) return CZ_OK;
     HIP_TRY(hipMemsetAsync(f->j_meta, 0, (size_t)f->j_cap * 16u, f->stream));
     HIP_TRY(hipMemsetAsync(f->j_mark, 0, (size_t)f->cap_nt16 * 4u, f->stream));
     HIP_TRY(hipMemsetAsync(f->j_touch, 0, (size_t)f->cap_nt16 * 4u, f->stream));
@@ -334,7 +339,7 @@ int cz_film_journal_gen(cz_film* f, uint32_t gen) {
         int rc = journal_alloc(f);
         if (rc) return rc;
     }
-    if (f->j_meta) {
+    if (f->j_cap) {
         int rc = wait_idle(f);
         if (rc) return rc;
         f->j_ctl_h[2] = gen;
@@ -348,7 +353,7 @@ int cz_film_journal_gen(cz_film* f, uint32_t gen) {
 
 int cz_film_journal_budget(cz_film* f, uint64_t bytes) {
     if (!f) return set_err(CZ_EINVAL, "null film");
-    if (f->j_meta) return set_err(CZ_EINVAL, "journal_budget: the ring exists (set the budget before the first nonzero generation)");
+    if (f->j_cap) return set_err(CZ_EINVAL, "journal_budget: the ring exists (set the budget before the first nonzero generation)");
     f->j_budget = bytes;
     return CZ_OK;
 }
@@ -356,7 +361,7 @@ int cz_film_journal_budget(cz_film* f, uint64_t bytes) {
 int cz_film_journal_head(cz_film* f, uint32_t* head, uint32_t* cap) {
     if (!f) return set_err(CZ_EINVAL, "null film");
     uint32_t h = 0u;
-    if (f->j_meta) {
+    if (f->j_cap) {
         int rc = wait_idle(f);
         if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(f->j_ctl_h, f->j_ctl, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, f->stream));
@@ -365,7 +370,7 @@ int cz_film_journal_head(cz_film* f, uint32_t* head, uint32_t* cap) {
         h = f->j_ctl_h[0];
     }
     if (head) *head = h;
-    if (cap) *cap = f->j_meta ? f->j_cap : cap_for(f->j_budget);
+    if (cap) *cap = f->j_cap ? f->j_cap : cap_for(f->j_budget);
     return CZ_OK;
 }
 
@@ -379,7 +384,7 @@ int cz_film_touch_rect(cz_film* f, float minx, float miny, float maxx, float max
 int cz_film_restore_journal(cz_film* f, uint32_t target) {
     if (!f) return set_err(CZ_EINVAL, "null film");
     if (f->batch) return set_err(CZ_EINVAL, "restore_journal: the film is stepped by a batch (cz_batch_remove it first)");
-    if (!f->j_meta) return CZ_OK;  // nothing was ever recorded
+    if (!f->j_cap) return CZ_OK;  // nothing was ever recorded
     const FilmDev F = f->dev();
     const JournalDev J = jdev(f);
     hipLaunchKernelGGL(k_journal_best, dim3((f->nt16 + 255u) / 256u), dim3(256), 0, f->stream, J, f->nt16);

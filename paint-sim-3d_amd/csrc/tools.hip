@@ -317,20 +317,21 @@ int tools_alloc(cz_film* f) {
     const uint32_t kcap = f->cap_n / 256u + 16u * f->cap_nt16 + 1u;
     int rc = CZ_OK;
     KnifeDev* kd = nullptr;
-    if (!rc) rc = talloc(f, &f->contacts, CZ_MAX_CONTACTS);
-    if (!rc) rc = talloc(f, &f->ctile_range, f->cap_nt64);
-    if (!rc) rc = talloc(f, &f->ctile_items, (size_t)CZ_MAX_CONTACTS * 16u);
-    if (!rc) rc = talloc(f, &f->reservoir, (size_t)CZ_MAX_BRISTLES * 17u);
-    if (!rc) rc = talloc(f, &kd, 1);
-    if (!rc) rc = talloc(f, &f->scraps, CZ_MAX_SCRAPS);
-    if (!rc) rc = talloc(f, &f->kcnt, kcap);
-    if (!rc) rc = talloc(f, &f->kctl, 4);
+    if (!rc && !f->contacts) rc = talloc(f, &f->contacts, CZ_MAX_CONTACTS);
+    if (!rc && !f->ctile_range) rc = talloc(f, &f->ctile_range, f->cap_nt64);
+    if (!rc && !f->ctile_items) rc = talloc(f, &f->ctile_items, (size_t)CZ_MAX_CONTACTS * 16u);
+    if (!rc && !f->reservoir) rc = talloc(f, &f->reservoir, (size_t)CZ_MAX_BRISTLES * 17u);
+    if (!rc && !f->knife_dev) rc = talloc(f, &kd, 1);
+    if (!rc && !f->scraps) rc = talloc(f, &f->scraps, CZ_MAX_SCRAPS);
+    if (!rc && !f->kcnt) rc = talloc(f, &f->kcnt, kcap);
+    if (!rc && !f->kctl) rc = talloc(f, &f->kctl, 4);
     if (!rc) {
-        f->knife_dev = kd;
-        hipError_t e = hipHostMalloc((void**)&f->contacts_h, CZ_MAX_CONTACTS * sizeof(cz_contact), hipHostMallocDefault);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&f->crange_h, f->cap_nt64 * sizeof(uint2), hipHostMallocDefault);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&f->citems_h, (size_t)CZ_MAX_CONTACTS * 16u * 4u, hipHostMallocDefault);
-        if (e == hipSuccess) e = hipHostMalloc(&f->knife_h, sizeof(KnifeDev), hipHostMallocDefault);
+        if (kd) f->knife_dev = kd;
+        hipError_t e = hipSuccess;
+        if (!f->contacts_h) e = hipHostMalloc((void**)&f->contacts_h, CZ_MAX_CONTACTS * sizeof(cz_contact), hipHostMallocDefault);
+        if (e == hipSuccess && !f->crange_h) e = hipHostMalloc((void**)&f->crange_h, f->cap_nt64 * sizeof(uint2), hipHostMallocDefault);
+        if (e == hipSuccess && !f->citems_h) e = hipHostMalloc((void**)&f->citems_h, (size_t)CZ_MAX_CONTACTS * 16u * 4u, hipHostMallocDefault);
+        if (e == hipSuccess && !f->knife_h) e = hipHostMalloc(&f->knife_h, sizeof(KnifeDev), hipHostMallocDefault);
         if (e != hipSuccess) rc = set_err(CZ_ENOMEM, "tools: hipHostMalloc: %s", hipGetErrorString(e));
     }
     if (rc) return rc;  // what was allocated stays with the film and is freed with it
