@@ -12,6 +12,7 @@ namespace cz {
 
 // one canvas, its ordered list of active tiles
 struct OneFilm {
+    static constexpr bool kXcdChunks = true;   // blocks: a contiguous chunk of the list per XCD
     FilmDev F;
     __device__ __forceinline__ uint32_t count() const { return *F.active_count; }
     __device__ __forceinline__ uint32_t item(uint32_t a) const { return a; }
@@ -21,6 +22,7 @@ struct OneFilm {
 
 // one canvas, every tile 0..n-1
 struct OneFilmAll {
+    static constexpr bool kXcdChunks = false;  // blocks: the plain stride over every tile
     FilmDev F;
     uint32_t n;
     __device__ __forceinline__ uint32_t count() const { return n; }
@@ -40,6 +42,7 @@ __device__ __forceinline__ const FilmDev& desc_at(const FilmDev* table, uint32_t
 
 // one canvas, the tiles its compaction listed for the cure pass
 struct OneFilmCure {
+    static constexpr bool kXcdChunks = true;   // blocks: a contiguous chunk of the list per XCD
     FilmDev F;
     __device__ __forceinline__ uint32_t count() const { return *F.cure_count; }
     __device__ __forceinline__ uint32_t item(uint32_t a) const { return a; }
@@ -58,6 +61,7 @@ constexpr uint32_t CZ_VARIANTS = 4u;
 
 // many canvases: a device list of (member, tile) words and its device count
 struct ManyFilms {
+    static constexpr bool kXcdChunks = true;   // blocks: a contiguous chunk of the list per XCD
     const FilmDev* films;   // [member][CZ_VARIANTS]
     const uint32_t* items;
     const uint32_t* count_p;

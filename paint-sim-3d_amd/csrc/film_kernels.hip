@@ -11,6 +11,8 @@
 // dispatch: nose writes a second offset buffer, the strata fill passes carry
 // the stack sums in ping-pong buffers, and cg_finish leaves the new height in
 // a scratch buffer until the rescale pass.
+#include <type_traits>
+
 #include "film_kernels.h"
 
 namespace cz {
@@ -18,18 +20,21 @@ namespace cz {
 // Binds, per item of the launch: _a (item slot), F (the item's canvas) and
 // _t (its tile).
 // Which list items a block takes.  Workgroups are dealt to the 8 XCDs in
-// turn (block b on XCD b % 8), each XCD with its own L2; cutting the list
-// (ascending tiles) into 8 contiguous chunks, one per XCD, puts neighbouring
-// tiles on the same XCD, so the halos of their LDS windows are fetched into
-// one L2 instead of several.  Tiles are independent within a launch, so any
-// assignment gives the same results; a grid that is not a multiple of 8
-// keeps the plain stride.
+// turn (block b on XCD b % 8), each XCD with its own L2.  For the tile lists
+// (active, cure, batched), the list (ascending tiles) is cut into 8
+// contiguous chunks, one per XCD, so neighbouring tiles share an L2 and the
+// halos of their LDS windows are fetched once.  Passes over every tile of a
+// canvas, whose per-tile work follows where the paint is, keep the plain
+// stride, which spreads that work over all XCDs.  Tiles are independent
+// within a launch, so any assignment gives the same results; a grid that is
+// not a multiple of 8 keeps the plain stride.
 struct XcdRange {
     uint32_t first, end, step;
 };
+template <bool CHUNKS>
 __device__ __forceinline__ XcdRange xcd_range(uint32_t n) {
     constexpr uint32_t NX = 8u;
-    if (gridDim.x < NX || (gridDim.x & (NX - 1u)) != 0u) return {blockIdx.x, n, gridDim.x};
+    if (!CHUNKS || gridDim.x < NX || (gridDim.x & (NX - 1u)) != 0u) return {blockIdx.x, n, gridDim.x};
     const uint32_t chunk = (n + NX - 1u) / NX;
     const uint32_t lo = (blockIdx.x % NX) * chunk;
     return {lo + blockIdx.x / NX, min(n, lo + chunk), gridDim.x / NX};
@@ -38,7 +43,7 @@ __device__ __forceinline__ XcdRange xcd_range(uint32_t n) {
 #define CZ_FOR_TILES(WS)                                                             \
     const uint32_t _nitems = (WS).count();                                           \
     const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;                  \
-    const XcdRange _rg = xcd_range(_nitems);                                         \
+    const XcdRange _rg = xcd_range<std::decay_t<decltype(WS)>::kXcdChunks>(_nitems); \
     for (uint32_t _a = _rg.first; _a < _rg.end; _a += _rg.step)                      \
         if (const uint32_t _item = (WS).item(_a); true)                              \
             if (const FilmDev& F = (WS).film(_item); true)                           \
@@ -147,17 +152,24 @@ __global__ __launch_bounds__(256) void k_dep_fill(FilmDev F) {
     const int32_t tx = (int32_t)(t % F.t64x), ty = (int32_t)(t / F.t64x);
     const unsigned long long lmask = (lane == 0u) ? 0ull : (~0ull >> (64u - lane));
     uint32_t filled = 0u;
-    for (uint32_t d0 = 0; d0 < count && filled < want; d0 += 64u) {
-        const uint32_t di = d0 + lane;
-        bool hit = false;
-        if (di < count) {
-            const int4 r = *(const int4*)(F.dep_rect + 4u * di);
-            hit = r.x >= 0 && tx >= r.x && tx <= r.y && ty >= r.z && ty <= r.w;
+    // 8 x 64 rectangles per round, all loads issued before the first test
+    constexpr uint32_t U = 8u;
+    for (uint32_t d0 = 0; d0 < count && filled < want; d0 += 64u * U) {
+        int4 r[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t di = d0 + 64u * u + lane;
+            r[u] = di < count ? *(const int4*)(F.dep_rect + 4u * di) : make_int4(-1, -1, -1, -1);
         }
-        const unsigned long long b = __ballot(hit);
-        const uint32_t pos = filled + (uint32_t)__popcll(b & lmask);
-        if (hit && pos < want) F.tile_items[base + pos] = di;
-        filled += (uint32_t)__popcll(b);
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t di = d0 + 64u * u + lane;
+            const bool hit = r[u].x >= 0 && tx >= r[u].x && tx <= r[u].y && ty >= r[u].z && ty <= r[u].w;
+            const unsigned long long b = __ballot(hit);
+            const uint32_t pos = filled + (uint32_t)__popcll(b & lmask);
+            if (hit && pos < want) F.tile_items[base + pos] = di;
+            filled += (uint32_t)__popcll(b);
+        }
     }
 }
 
@@ -1020,7 +1032,7 @@ template <class WS>
 __global__ __launch_bounds__(256) void k_fluxes_faces(WS ws) {
     const uint32_t nact = ws.count();
     const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
-    const XcdRange rg = xcd_range(4u * nact);
+    const XcdRange rg = xcd_range<true>(4u * nact);
     for (uint32_t wk = rg.first; wk < rg.end; wk += rg.step) {
         const uint32_t it = ws.item(wk >> 2);
         const FilmDev& F = ws.film(it);
