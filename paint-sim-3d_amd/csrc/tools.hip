@@ -343,8 +343,9 @@ int tools_alloc(cz_film* f) {
 // The cells the knife can reach: a + t*(b - a) - behind*dir for t in [0, 1]
 // and behind in [-dx, KNIFE_BLADE_LEN] (knife.wgsl:66-72: q = t*(b - a) -
 // u*mv with mv = dir*sweep and behind = u*sweep, so behind runs against the
-// sweep), as a box widened by a margin for the rounding of t and u.  A nearly degenerate blade/sweep pair (where that
-// rounding is unbounded) takes the whole canvas.  false: no cell can pass.
+// sweep), as a box widened by a margin for the rounding of t and u.  A
+// nearly degenerate blade/sweep pair (where that rounding is unbounded)
+// takes the whole canvas.  false: no cell can pass.
 bool knife_box(const cz_film* f, const cz_knife& K, uint32_t box[4]) {
     const float abx = K.b[0] - K.a[0], aby = K.b[1] - K.a[1];
     const float mvx = K.dir[0] * K.sweep, mvy = K.dir[1] * K.sweep;
