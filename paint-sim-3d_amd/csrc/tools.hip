@@ -77,16 +77,38 @@ __device__ __forceinline__ float strip_top(Stack& s, float excess, float* pig) {
     return removed;
 }
 
+constexpr uint32_t kResWords = (uint32_t)CZ_MAX_BRISTLES * 17u;
+
 // brush_scrape (brush.wgsl:33-122): over the cells of 64-tiles with
 // contacts; the nearest-cap contact sets the cap, the laying direction and
-// the bristle whose reservoir receives the paint taken.
+// the bristle whose reservoir receives the paint taken.  The block's cells
+// add their amounts into an LDS copy of the reservoir first; the block then
+// adds the words it touched to the device reservoir (integer sums, so the
+// totals do not depend on the order).
+__device__ __forceinline__ void brush_cell(const FilmDev& F, const cz_contact* contacts, uint2 r,
+                                           const uint32_t* items, uint32_t* s_res, uint32_t i, uint32_t j);
+
 __global__ __launch_bounds__(256) void k_brush(FilmDev F, const cz_contact* contacts, const uint2* range,
                                                const uint32_t* items, uint32_t* reservoir) {
+    __shared__ uint32_t s_res[kResWords];
     const uint32_t i = blockIdx.x * 16u + (threadIdx.x & 15u);
     const uint32_t j = blockIdx.y * 16u + (threadIdx.x >> 4);
-    if (i >= F.W || j >= F.H) return;
-    const uint2 r = range[(i / TILE64) + (j / TILE64) * F.P.tiles[0]];
+    // a 16x16 block lies inside one 64-tile: the same contact range for all
+    // of its threads (i, j may pass the edge; the 64-tile of the block's
+    // first cell is the one)
+    const uint32_t bi = blockIdx.x * 16u, bj = blockIdx.y * 16u;
+    const uint2 r = range[(bi / TILE64) + (bj / TILE64) * F.P.tiles[0]];
     if (r.y == 0u) return;
+    for (uint32_t q = threadIdx.x; q < kResWords; q += 256u) s_res[q] = 0u;
+    __syncthreads();
+    if (i < F.W && j < F.H) brush_cell(F, contacts, r, items, s_res, i, j);
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < kResWords; q += 256u)
+        if (s_res[q] != 0u) atomicAdd(reservoir + q, s_res[q]);
+}
+
+__device__ __forceinline__ void brush_cell(const FilmDev& F, const cz_contact* contacts, uint2 r,
+                                           const uint32_t* items, uint32_t* s_res, uint32_t i, uint32_t j) {
     const uint32_t c = idx(F, i, j);
     const float hv = F.h[c];
     if (hv <= SPONGE_FILM) return;
@@ -140,7 +162,7 @@ __global__ __launch_bounds__(256) void k_brush(FilmDev F, const cz_contact* cont
     stack_store_to(F.strata_h, F.strata_cure, F.strata_pig, c, s);
     if (who >= (uint32_t)CZ_MAX_BRISTLES) return;  // no reservoir for it
     const float vol = removed * F.P.dx * F.P.dx;
-    uint32_t* res = reservoir + who * 17u;
+    uint32_t* res = s_res + who * 17u;
     atomicAdd(res, f2u_sat(vol * RES_UNIT + 0.5f));
 #pragma unroll
     for (int b = 0; b < 16; b++) {
