@@ -1397,6 +1397,7 @@ static void packed_rheo(const ofilm* F, uint32_t c, const float* t, const float*
 
 /* cure.wgsl:119-159 (G dispatch) */
 static void k_cure_step(ofilm* F) {
+    _Pragma("omp parallel for schedule(dynamic, 4096)")
     for (uint32_t c = 0; c < F->n; c++) {
         if (F->h[c] <= 0.0f) {
             F->rheo[2 * c] = 1.0f;
@@ -1475,7 +1476,10 @@ static float ribbon_height(float px, float py, const cz_deposit* d) {
 }
 
 static void k_deposit(ofilm* F) {
-    for (uint32_t j = 0; j < F->H; j++) {
+    /* one thread per band of 16 rows: a cell marks its own 16x16 tile only */
+    _Pragma("omp parallel for schedule(dynamic, 1)")
+    for (uint32_t jb = 0; jb < (F->H + 15u) / 16u; jb++) {
+    for (uint32_t j = jb * 16u; j < jb * 16u + 16u && j < F->H; j++) {
         for (uint32_t i = 0; i < F->W; i++) {
             uint32_t tile = (i / TILE) + (j / TILE) * F->P.tiles[0];
             uint32_t n = F->tile_count[tile];
@@ -1524,6 +1528,7 @@ static void k_deposit(ofilm* F) {
             }
         }
     }
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1531,6 +1536,7 @@ static void k_deposit(ofilm* F) {
 
 /* film.wgsl:1365-1391 */
 static void k_mark_tiles(ofilm* F) {
+    _Pragma("omp parallel for schedule(dynamic, 64)")
     for (uint32_t t = 0; t < F->nt16; t++) {
         uint32_t tx = t % F->tw, ty = t / F->tw;
         uint32_t any = 0;
@@ -1592,6 +1598,7 @@ static void k_compact_tiles(ofilm* F) {
 
 /* Iterate the cells of the active tiles (common.wgsl:132-138). */
 #define FOR_ACTIVE(F, ...)                                                       \
+    _Pragma("omp parallel for schedule(dynamic, 1)")                             \
     for (uint32_t _a = 0; _a < (F)->active_count; _a++) {                        \
         uint32_t _t = (F)->tile_list[_a];                                        \
         uint32_t _tx = _t % (F)->tw, _ty = _t / (F)->tw;                         \
@@ -1745,6 +1752,7 @@ static void inherit_from(ofilm* F, uint32_t c, uint32_t src, float hn, int copy_
 
 /* film.wgsl:803-863 */
 static void k_apply(ofilm* F) {
+    _Pragma("omp parallel for schedule(dynamic, 1)")
     for (uint32_t a = 0; a < F->active_count; a++) {
         uint32_t t = F->tile_list[a];
         uint32_t tx = t % F->tw, ty = t / F->tw;
@@ -2247,6 +2255,7 @@ static void cg_solve(ofilm* F, uint32_t iters) {
         F->cg_t2[c] = lap_w(F, x, y, c, 5u);
     })
     /* cg_init (film.wgsl:1110-1144) */
+    _Pragma("omp parallel for schedule(dynamic, 1)")
     for (uint32_t a = 0; a < F->active_count; a++) {
         uint32_t t = F->tile_list[a];
         uint32_t tx = t % F->tw, ty = t / F->tw;
@@ -2294,6 +2303,7 @@ static void cg_solve(ofilm* F, uint32_t iters) {
         /* cg_lap_p */
         FOR_ACTIVE(F, { F->cg_t[c] = lap_w(F, x, y, c, 0u); })
         /* cg_iter_a */
+        _Pragma("omp parallel for schedule(dynamic, 1)")
         for (uint32_t a = 0; a < F->active_count; a++) {
             uint32_t t = F->tile_list[a];
             uint32_t tx = t % F->tw, ty = t / F->tw;
@@ -2318,6 +2328,7 @@ static void cg_solve(ofilm* F, uint32_t iters) {
             F->cg_s[1] = pq; F->cg_s[2] = alpha; F->cg_s[3] = 0.0f;
         }
         /* cg_iter_b */
+        _Pragma("omp parallel for schedule(dynamic, 1)")
         for (uint32_t a = 0; a < F->active_count; a++) {
             uint32_t t = F->tile_list[a];
             uint32_t tx = t % F->tw, ty = t / F->tw;
@@ -2354,6 +2365,7 @@ static void cg_solve(ofilm* F, uint32_t iters) {
     /* cg_finish (film.wgsl:1261-1302), neighbours' h from before the pass */
     float* hold = F->snap;
     memcpy(hold, F->h, F->n * sizeof(float));
+    _Pragma("omp parallel for schedule(dynamic, 1)")
     for (uint32_t a = 0; a < F->active_count; a++) {
         uint32_t t = F->tile_list[a];
         uint32_t tx = t % F->tw, ty = t / F->tw;

@@ -1,6 +1,9 @@
 """Golden fixture of a long run: the CPU restatement advanced 1000 frames on a
 seeded 48x48 blob canvas (tilt 20 deg, 0.586 mm cells) fed by a circling
-nozzle for 800 frames; stores the state after 100 and 1000 frames.  The GPU test replays the same frames and compares."""
+nozzle for 800 frames; stores the state after 100 and 1000 frames.  The GPU test replays the same frames and compares.
+
+python make_long_run.py            -> long_run_48.npz      (n = 1, the reference's default)
+python make_long_run.py --n 0.7    -> long_run_48_n07.npz  (shear-thinning power law)"""
 import hashlib
 import os
 import sys
@@ -21,8 +24,12 @@ FRAMES = (100, 1000)
 KW = dict(seed=1000, nblobs=7, sigma_cells=(2.0, 6.0), peak=(2e-4, 1.5e-3), tau=(0.01, 0.5), mu=(2.0, 20.0))
 
 
-def scene():
+FIXTURES = {1.0: "long_run_48.npz", 0.7: "long_run_48_n07.npz"}
+
+
+def scene(n_index=1.0):
     s = scenes.blobs_settings(W, tilt_deg=20.0)
+    s.n_index = n_index
     st = scenes.blobs_state(W, W, **KW)
     return s, st
 
@@ -48,7 +55,8 @@ def digest(st):
 
 
 if __name__ == "__main__":
-    s, st = scene()
+    n_index = float(sys.argv[sys.argv.index("--n") + 1]) if "--n" in sys.argv else 1.0
+    s, st = scene(n_index)
     _, o = make_pair(W, W, s, st, gpu=False)
     out = {"init_sha256": np.array(digest(st))}
     t0 = time.time()
@@ -63,4 +71,4 @@ if __name__ == "__main__":
             out[f"f{f}_wet_ml"] = np.array(r.wet_ml)
             out[f"f{f}_substeps"] = np.array(round(o.params.frame_dt / o.params.dt))
             print(f"frame {f}: {time.time() - t0:.1f} s, active {o.active_count()}, wet {r.wet_ml:.6f} mL", flush=True)
-    np.savez_compressed(os.path.join(HERE, "long_run_48.npz"), **out)
+    np.savez_compressed(os.path.join(HERE, FIXTURES[n_index]), **out)

@@ -1,5 +1,6 @@
-"""1000 frames on the GPU against the restatement's committed trajectory
-(tests/golden/long_run_48.npz, written by tests/golden/make_long_run.py):
+"""1000 frames on the GPU against the restatement's committed trajectories
+(tests/golden/long_run_48.npz at n = 1 and long_run_48_n07.npz at the
+shear-thinning n = 0.7, written by tests/golden/make_long_run.py):
 heights and strata within 1e-3 relative after 1000 FP32 frames (1e-5 after
 100), tile flags and nose offsets bit-exact."""
 import os
@@ -15,13 +16,14 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def test_long_run_matches_golden(gpu):
+@pytest.mark.parametrize("n_index", [1.0, 0.7])
+def test_long_run_matches_golden(gpu, n_index):
     import importlib.util
     spec = importlib.util.spec_from_file_location("mlr", os.path.join(HERE, "golden", "make_long_run.py"))
     mlr = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mlr)
-    gold = np.load(os.path.join(HERE, "golden", "long_run_48.npz"))
-    s, st = mlr.scene()
+    gold = np.load(os.path.join(HERE, "golden", mlr.FIXTURES[n_index]))
+    s, st = mlr.scene(n_index)
     assert mlr.digest(st) == str(gold["init_sha256"]), "the seeded scene changed (numpy generator?)"
     g = czf.Film(mlr.W, mlr.W, s, czf.default_pigment_table(), czf.cmf_table())
     assert g.write_state_arrays(st)

@@ -4,7 +4,11 @@ the same seeded inputs.
 Bar: height and pigments within 1e-5 relative after one frame and 1e-3
 after long runs (both with a floor of DRAINED_FILM = 1e-9 m for heights);
 tile flags, active-tile lists, nose offsets (integer bits), lay tags and the
-cure masks bit-exact.
+cure masks bit-exact; the pool8 maxima within the height tolerance.
+
+The blob scenes run at the reference's Newtonian index n = 1 and at the
+shear-thinning n = 0.7 and 0.4, where hb_flux and band_flux take their
+power-law branches (film.wgsl:94-126).
 """
 import numpy as np
 import pytest
@@ -15,6 +19,7 @@ from helpers import make_pair, oracle_capsule, rel_err
 pytestmark = pytest.mark.gpu
 
 H_FLOOR = 1e-9
+N_INDEX = [1.0, 0.7, 0.4]
 
 
 def compare(g, o, tol, tag=""):
@@ -48,6 +53,10 @@ def compare_masks(g, o, tag=""):
     # cure mask: which strata hit cure >= 1 (the commit test of cure.wgsl:205)
     cg, co = g.buffer("strata_cure"), o.buf("strata_cure")
     assert np.array_equal(cg >= 1.0, co >= 1.0), f"{tag}: cure mask differs"
+    # pool8 (film.wgsl:1304-1324): 4x4 maxima of the surface and the relief
+    pg, po = g.pool(), o.buf("pool")[: g.pool().size]
+    e_pool = rel_err(pg, po, H_FLOOR)
+    assert e_pool <= 1e-5, f"{tag}: pool8 rel err {e_pool:.3e}"
 
 
 def run_pair(g, o, frames, deposits_of=None, check_every=0, tol=1e-5, tag=""):
@@ -62,27 +71,49 @@ def run_pair(g, o, frames, deposits_of=None, check_every=0, tol=1e-5, tag=""):
     g.sync()
 
 
-def test_blobs_one_frame(gpu):
+def blob_settings_n(w, tilt_deg, n_index):
+    s = scenes.blobs_settings(w, tilt_deg=tilt_deg)
+    s.n_index = n_index
+    return s
+
+
+@pytest.mark.parametrize("n_index", N_INDEX)
+def test_blobs_one_frame(gpu, n_index):
     w = h = 128
-    s = scenes.blobs_settings(w, tilt_deg=15.0)
+    s = blob_settings_n(w, 15.0, n_index)
     st = scenes.blobs_state(w, h, seed=11, nblobs=10, sigma_cells=(2.0, 10.0))
     g, o = make_pair(w, h, s, st)
     g.step()
     o.step()
     g.sync()
-    compare(g, o, 1e-5, "blobs 1 frame")
-    compare_masks(g, o, "blobs 1 frame")
+    compare(g, o, 1e-5, f"blobs 1 frame n {n_index}")
+    compare_masks(g, o, f"blobs 1 frame n {n_index}")
     assert abs(g.params.dt - o.params.dt) == 0.0
 
 
-def test_blobs_frames(gpu):
+@pytest.mark.parametrize("n_index", N_INDEX)
+def test_blobs_frames(gpu, n_index):
     w = h = 128
-    s = scenes.blobs_settings(w, tilt_deg=15.0)
+    s = blob_settings_n(w, 15.0, n_index)
     st = scenes.blobs_state(w, h, seed=12, nblobs=12, sigma_cells=(2.0, 10.0))
     g, o = make_pair(w, h, s, st)
-    run_pair(g, o, 20, check_every=5, tol=1e-5, tag="blobs")
+    run_pair(g, o, 20, check_every=5, tol=1e-5, tag=f"blobs n {n_index}")
     rg, ro = g.mass_report(), o.mass_report()
     assert abs(rg.wet_ml - ro.wet_ml) <= 1e-5 * abs(ro.wet_ml) + 1e-12
+
+
+@pytest.mark.parametrize("n_index", N_INDEX)
+def test_flowing_blobs(gpu, n_index):
+    """Low yield stress and a steep tilt: the paint keeps running for the
+    whole 30 frames, so the power-law flux and the strata split act on many
+    faces every substep."""
+    w = h = 128
+    s = blob_settings_n(w, 45.0, n_index)
+    st = scenes.blobs_state(w, h, seed=13, nblobs=14, sigma_cells=(3.0, 12.0), peak=(2e-4, 2e-3),
+                            tau=(0.01, 0.3), mu=(0.5, 5.0))
+    g, o = make_pair(w, h, s, st)
+    run_pair(g, o, 30, check_every=10, tol=1e-5, tag=f"flowing n {n_index}")
+    assert o.active_count() > 0, "the scene settled: the test does not exercise the flux"
 
 
 @pytest.mark.parametrize("tilt", [0.0, 30.0])
@@ -114,10 +145,11 @@ def test_jet_stream(gpu):
     run_pair(g, o, 40, deps, check_every=10, tol=1e-5, tag="jet")
 
 
-def test_implicit_cg(gpu):
+@pytest.mark.parametrize("n_index", N_INDEX)
+def test_implicit_cg(gpu, n_index):
     """Thick, fluid paint: the implicit hydrostatic CG runs (implicit_steps > 0)."""
     w = h = 96
-    s = scenes.blobs_settings(w, tilt_deg=5.0)
+    s = blob_settings_n(w, 5.0, n_index)
     st = scenes.blobs_state(w, h, seed=5, nblobs=6, sigma_cells=(6.0, 12.0), peak=(4e-3, 6e-3),
                             mu=(0.05, 0.1), tau=(1e-3, 2e-3))
     g, o = make_pair(w, h, s, st)
@@ -127,8 +159,8 @@ def test_implicit_cg(gpu):
     g.sync()
     assert o.implicit_steps() > 0, "scene does not exercise the implicit solve"
     assert g.stats().implicit_steps == o.implicit_steps()
-    compare(g, o, 1e-5, "implicit")
-    compare_masks(g, o, "implicit")
+    compare(g, o, 1e-5, f"implicit n {n_index}")
+    compare_masks(g, o, f"implicit n {n_index}")
     assert np.allclose(g.stats().cg_report[:], o.cg_s()[4:], rtol=1e-5, atol=0)
 
 
