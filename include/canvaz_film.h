@@ -223,6 +223,20 @@ typedef struct cz_mass_report {
     uint64_t nan_cells;
 } cz_mass_report;
 
+/* The FP64 sums behind the mass report (compute_stats, src/main.rs:1708-1720):
+ * over the cells whose height is finite, the sums of h and m, the wet count
+ * and the maxima of h and of h + m (film.s_max); the cells with a
+ * non-finite height are counted and skipped.  48 bytes. */
+typedef struct cz_mass_totals {
+    double sum_h;
+    double sum_m;
+    uint64_t wet_cells;
+    uint64_t nan_cells;
+    float h_max;
+    float s_max;
+    uint32_t pad[2];
+} cz_mass_totals;
+
 /* Indices of the eight state parts of read_state/write_state, in the order of
  * `state_buffers` (src/film/mod.rs:1277-1279). */
 enum {
@@ -360,8 +374,21 @@ int cz_film_sync(cz_film* f);
 int cz_film_stats(cz_film* f, cz_stats* st);
 /* pool8 output (mod.rs:956-963): pw*ph surface max then pw*ph relief max. */
 int cz_film_pool(cz_film* f, float* out, uint64_t n_floats);
-/* compute_stats mass report (src/main.rs:1679-1819). */
+/* compute_stats mass report (src/main.rs:1679-1819); the sums run on the
+ * device in FP64 (cz_mass_sums over the film's h and m). */
 int cz_film_mass_report(cz_film* f, cz_mass_report* r);
+
+/* The summing loop of compute_stats (src/main.rs:1708-1720), which the
+ * reference runs on the host after reading h and m back, as two device
+ * launches over device arrays h, m of n cells: per 65536-cell chunk a
+ * partial record into `workspace`, then a fixed-order fold into *out
+ * (device memory, 16-byte aligned), enqueued on `stream` (hipStream_t,
+ * NULL = default) of the current device.  The bits of the result depend
+ * only on the inputs.  workspace: at least cz_mass_sums_workspace(n)
+ * bytes, 16-byte aligned (NULL allowed when n = 0). */
+uint64_t cz_mass_sums_workspace(uint64_t n);
+int cz_mass_sums(const float* h, const float* m, uint64_t n, void* workspace, uint64_t workspace_bytes,
+                 cz_mass_totals* out, void* stream);
 /* state_buffers sizes, read_state, write_state (mod.rs:1277-1320). */
 int cz_film_state_sizes(const cz_film* f, uint64_t sizes[CZ_NPARTS]);
 int cz_film_read_state(cz_film* f, void* const parts[CZ_NPARTS]);

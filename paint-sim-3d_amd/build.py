@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libcanvaz_film.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CZ_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["film_kernels.hip", "film_host.hip", "batch_host.hip", "scene_gen.hip", "serve_host.hip", "host_ref.cpp", "km_mix.hip", "selftest.hip", "canvaz_io.hip", "journal.hip", "tools.hip"]
+SOURCES = ["film_kernels.hip", "film_host.hip", "batch_host.hip", "scene_gen.hip", "serve_host.hip", "host_ref.cpp", "km_mix.hip", "selftest.hip", "canvaz_io.hip", "journal.hip", "tools.hip", "mass_sums.hip"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -30,7 +30,11 @@ def build(verbose=False):
         cmd = [HIPCC] + FLAGS + (["-x", "hip"] if src.endswith(".hip") else []) + ["-c", path, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True, stderr=subprocess.PIPE if not verbose else None)
+        r = subprocess.run(cmd, stderr=subprocess.PIPE if not verbose else None, text=True)
+        if r.returncode != 0:
+            if r.stderr:
+                sys.stderr.write(r.stderr)
+            raise subprocess.CalledProcessError(r.returncode, cmd, stderr=r.stderr)
         objs.append(obj)
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB] + objs
     subprocess.run(cmd, check=True)

@@ -14,7 +14,7 @@ import os
 
 import numpy as np
 
-from cz_types import (Contact, Deposit, HostState, Knife, MassReport, Overrides, Params, Request, Result, Scene,
+from cz_types import (Contact, Deposit, HostState, Knife, MassReport, MassTotals, Overrides, Params, Request, Result, Scene,
                       ServerConfig, ServerStats, Settings, Stats, PART_NAMES, SCRAP_DTYPE,
                       contact_array, default_overrides, default_settings, deposit_array)
 
@@ -134,6 +134,8 @@ def lib():
         "cz_batch_graph_counts": (i32, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "cz_batch_flux_shape": (i32, [P, i32]),
         "cz_batch_stream": (C.c_void_p, [P]),
+        "cz_mass_sums_workspace": (u64, [u64]),
+        "cz_mass_sums": (i32, [C.c_void_p, C.c_void_p, u64, C.c_void_p, u64, C.c_void_p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -654,6 +656,23 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def mass_sums(h, m, workspace=None):
+    """cz_mass_sums over two device tensors (float32, same number of cells) on
+    torch's current stream: the FP64 totals of compute_stats's loop
+    (src/main.rs:1708-1720).  workspace: a uint8 device tensor of at least
+    cz_mass_sums_workspace(n) bytes (allocated here when None)."""
+    import torch
+    n = h.numel()
+    assert m.numel() == n and h.dtype == torch.float32 and m.dtype == torch.float32 and h.is_cuda and m.is_cuda
+    need = lib().cz_mass_sums_workspace(n)
+    if workspace is None:
+        workspace = torch.empty(max(need, 16), dtype=torch.uint8, device=h.device)
+    out = torch.empty(48, dtype=torch.uint8, device=h.device)
+    st = torch.cuda.current_stream().cuda_stream
+    check(lib().cz_mass_sums(h.data_ptr(), m.data_ptr(), n, workspace.data_ptr(), workspace.numel(), out.data_ptr(), st))
+    return MassTotals.from_buffer_copy(out.cpu().numpy().tobytes())
 
 
 def scene(seed=1, **kw):

@@ -651,6 +651,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(tile_bits, nt16); A(dep_rect, 4 * (size_t)CZ_MAX_DEPOSITS); A(dep_cnt, nt64); A(dep_n, 4); A(cmf, CZ_NBAND * 3); A(compact_cnt, 2 * ((nt16 + 255) / 256) + 2); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
     A(pigtab, CZ_NPIG * CZ_PIG_STRIDE); A(deposits, CZ_MAX_DEPOSITS); A(tile_start, nt64); A(tile_count, nt64);
     A(tile_items, CZ_MAX_TILE_ITEMS); A(frame_out, 16);
+    A(mass_ws, (cz_mass_sums_workspace(n) + 15) / 16); A(mass_out, 1);
 #undef A
     if (!rc) {
         e = hipHostMalloc((void**)&f->frame_out_h, 16 * sizeof(float), hipHostMallocDefault);
@@ -854,31 +855,24 @@ int cz_film_pool(cz_film* f, float* out, uint64_t n_floats) {
 // compute_stats (src/main.rs:1679-1819): host fold in FP64 of h and m.
 int cz_film_mass_report(cz_film* f, cz_mass_report* r) {
     if (!f || !r) return set_err(CZ_EINVAL, "null argument");
-    std::vector<float> hv(f->n), mv(f->n);
-    HIP_TRY(hipMemcpyAsync(hv.data(), f->h, (size_t)f->n * 4, hipMemcpyDeviceToHost, f->stream));
-    HIP_TRY(hipMemcpyAsync(mv.data(), f->m, (size_t)f->n * 4, hipMemcpyDeviceToHost, f->stream));
-    int rc = wait_idle(f);
+    int rc = cz_mass_sums(f->h, f->m, f->n, f->mass_ws, cz_mass_sums_workspace(f->cap_n), f->mass_out, f->stream);
     if (rc) return rc;
-    double dx = (double)f->dx, sum = 0.0, summ = 0.0;
-    float hmax = 0.0f;
-    uint64_t wet = 0, nan = 0;
-    for (uint32_t c = 0; c < f->n; c++) {
-        float a = hv[c], b = mv[c];
-        if (!std::isfinite(a)) { nan++; continue; }
-        sum += (double)a;
-        summ += (double)b;
-        if (a > 0.0f) wet++;
-        hmax = fmaxf(hmax, a);
-    }
-    double vol = sum * dx * dx, volm = summ * dx * dx - f->weave_volume;
+    cz_mass_totals ms;
+    HIP_TRY(hipMemcpyAsync(&ms, f->mass_out, sizeof(ms), hipMemcpyDeviceToHost, f->stream));
+    rc = wait_idle(f);
+    if (rc) return rc;
+    // src/main.rs:1721-1726,1812-1818
+    f->s_max = ms.s_max;
+    const double dx = (double)f->dx;
+    const double vol = ms.sum_h * dx * dx, volm = ms.sum_m * dx * dx - f->weave_volume;
     r->wet_ml = vol * 1e6;
     r->cured_ml = volm * 1e6;
     r->deposited_ml = f->total_deposited * 1e6;
     r->err_pct = f->total_deposited > 0.0 ? (vol + volm - f->total_deposited) / f->total_deposited * 100.0 : 0.0;
-    r->hmax = hmax;
+    r->hmax = ms.h_max;
     r->pad = 0;
-    r->wet_cells = wet;
-    r->nan_cells = nan;
+    r->wet_cells = ms.wet_cells;
+    r->nan_cells = ms.nan_cells;
     return CZ_OK;
 }
 

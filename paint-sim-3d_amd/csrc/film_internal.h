@@ -228,6 +228,8 @@ struct cz_film {
     int bin_mode = -1;             // deposit binning: -1 by count, 0 host, 1 device
     bool bin_on_device = false;    // the uploaded deposits are binned in the frame
     float2* stats_out = nullptr;
+    uint4* mass_ws = nullptr;        // [cz_mass_sums_workspace(cap_n) / 16] partials of the mass report
+    cz_mass_totals* mass_out = nullptr;  // [1] its result
     float* pool_out = nullptr;
     float* tsum_a = nullptr;
     float* pigtab = nullptr;

@@ -95,6 +95,11 @@ class HostState(C.Structure):
     _fields_ = [("h_max", f32), ("mu_min", f32), ("h_max_pending", f32), ("pad", u32)]
 
 
+class MassTotals(C.Structure):
+    _fields_ = [("sum_h", C.c_double), ("sum_m", C.c_double), ("wet_cells", C.c_uint64), ("nan_cells", C.c_uint64),
+                ("h_max", f32), ("s_max", f32), ("pad", u32 * 2)]
+
+
 class MassReport(C.Structure):
     _fields_ = [("wet_ml", C.c_double), ("cured_ml", C.c_double), ("deposited_ml", C.c_double),
                 ("err_pct", C.c_double), ("hmax", f32), ("pad", u32), ("wet_cells", C.c_uint64),
@@ -106,6 +111,7 @@ assert C.sizeof(Deposit) == 96
 assert C.sizeof(Contact) == 32
 assert C.sizeof(Knife) == 384
 assert C.sizeof(Scrap) == 48
+assert C.sizeof(MassTotals) == 48
 
 PART_NAMES = ("h", "m", "rheo", "strata_h", "strata_pig", "strata_cure", "sub_refl", "lay")
 
