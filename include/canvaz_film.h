@@ -385,7 +385,7 @@ int cz_film_mass_report(cz_film* f, cz_mass_report* r);
  * (device memory, 16-byte aligned), enqueued on `stream` (hipStream_t,
  * NULL = default) of the current device.  The bits of the result depend
  * only on the inputs.  workspace: at least cz_mass_sums_workspace(n)
- * bytes, 16-byte aligned (NULL allowed when n = 0). */
+ * bytes, 16-byte aligned (NULL allowed when n = 0, as are h and m). */
 uint64_t cz_mass_sums_workspace(uint64_t n);
 int cz_mass_sums(const float* h, const float* m, uint64_t n, void* workspace, uint64_t workspace_bytes,
                  cz_mass_totals* out, void* stream);

@@ -129,7 +129,7 @@ int launch_mass(const float* h, const float* m, uint64_t n, MassPart* parts, cz_
 }
 
 int check_args(const float* h, const float* m, uint64_t n, cz_mass_totals* out) {
-    if (!h || !m || !out) return set_err(CZ_EINVAL, "mass sums: null argument");
+    if (!out || (n > 0 && (!h || !m))) return set_err(CZ_EINVAL, "mass sums: null argument");
     if (((uintptr_t)out & 15u) != 0u) return set_err(CZ_EINVAL, "mass sums: out not 16-byte aligned");
     if (parts_for(n) > 0x7fffffffull) return set_err(CZ_EINVAL, "mass sums: %llu cells is too many", (unsigned long long)n);
     return CZ_OK;
