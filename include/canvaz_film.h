@@ -389,6 +389,14 @@ int cz_film_mass_report(cz_film* f, cz_mass_report* r);
 uint64_t cz_mass_sums_workspace(uint64_t n);
 int cz_mass_sums(const float* h, const float* m, uint64_t n, void* workspace, uint64_t workspace_bytes,
                  cz_mass_totals* out, void* stream);
+/* No counterpart in the reference.  cz_mass_sums for callers that cannot
+ * allocate (code under HIP graph capture, serving loops that forbid
+ * hipMalloc): the partials live in a static device array of the library
+ * (192 KiB per GPU, held until the process ends), so the call allocates
+ * nothing; the caller passes h, m and out as above.  At most 268,435,456
+ * cells (4096 chunks) per call, else CZ_EINVAL.  All calls on one GPU share
+ * the array: calls that overlap on a GPU race. */
+int cz_mass_sums_noalloc(const float* h, const float* m, uint64_t n, cz_mass_totals* out, void* stream);
 /* state_buffers sizes, read_state, write_state (mod.rs:1277-1320). */
 int cz_film_state_sizes(const cz_film* f, uint64_t sizes[CZ_NPARTS]);
 int cz_film_read_state(cz_film* f, void* const parts[CZ_NPARTS]);

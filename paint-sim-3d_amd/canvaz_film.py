@@ -136,6 +136,7 @@ def lib():
         "cz_batch_stream": (C.c_void_p, [P]),
         "cz_mass_sums_workspace": (u64, [u64]),
         "cz_mass_sums": (i32, [C.c_void_p, C.c_void_p, u64, C.c_void_p, u64, C.c_void_p, C.c_void_p]),
+        "cz_mass_sums_noalloc": (i32, [C.c_void_p, C.c_void_p, u64, C.c_void_p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

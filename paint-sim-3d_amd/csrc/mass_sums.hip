@@ -12,17 +12,28 @@
 // The chunking and both tree shapes are fixed, so the sums do not depend on
 // the launch or on the device: equal inputs give equal bits.
 //
-// cz_mass_sums takes the partials' workspace from the caller
-// (cz_mass_sums_workspace bytes); the film's mass report passes a workspace
-// its handle allocated at creation.
+// Two entry points share the kernels.  cz_mass_sums takes the partials'
+// workspace from the caller (cz_mass_sums_workspace bytes); the film's mass
+// report passes a workspace its handle allocated at creation.
+// cz_mass_sums_noalloc keeps the partials in a file-scope device array
+// (`scratch`, MASS_NOALLOC_PARTS records = 192 KiB per GPU, held until the
+// process ends) and calls no allocator, for callers that cannot allocate
+// (code under graph capture, serving loops that forbid it).  Every call on
+// one GPU uses that GPU's one array: calls that overlap on a GPU (other
+// streams, other threads) race.  Every record a call reads is written by
+// the same call first, so nothing needs clearing between calls.
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <mutex>
 
 #include "film_internal.h"
 
 namespace {
 
 constexpr uint64_t MASS_CHUNK = 65536;          // cells per partial record
+constexpr uint64_t MASS_NOALLOC_PARTS = 4096;   // records the static scratch holds
+constexpr int kMaxDevices = 64;
 
 struct __align__(16) MassPart {
     double sh, sm;
@@ -135,6 +146,34 @@ int check_args(const float* h, const float* m, uint64_t n, cz_mass_totals* out) 
     return CZ_OK;
 }
 
+// The allocation-free entry point's partials.  In host code the name is a
+// stand-in, not a device address: kernels get the address that
+// hipGetSymbolAddress returns for the current device (each GPU holds its
+// own copy of the array).
+__device__ __align__(256) MassPart scratch[MASS_NOALLOC_PARTS];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once
+// per device (with that device current) and kept.
+int scratch_of_current_device(MassPart** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_err(CZ_EINVAL, "mass sums: device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(CZ_EHIP, "mass sums: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                           hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = (MassPart*)g_scratch_ptr[dev];
+    return CZ_OK;
+}
+
 }  // namespace
 
 extern "C" uint64_t cz_mass_sums_workspace(uint64_t n) { return parts_for(n) * sizeof(MassPart); }
@@ -149,4 +188,16 @@ extern "C" int cz_mass_sums(const float* h, const float* m, uint64_t n, void* wo
                        (unsigned long long)need, (unsigned long long)workspace_bytes);
     if (((uintptr_t)workspace & 15u) != 0u) return set_err(CZ_EINVAL, "mass sums: workspace not 16-byte aligned");
     return launch_mass(h, m, n, (MassPart*)workspace, out, (hipStream_t)stream);
+}
+
+extern "C" int cz_mass_sums_noalloc(const float* h, const float* m, uint64_t n, cz_mass_totals* out, void* stream) {
+    int rc = check_args(h, m, n, out);
+    if (rc) return rc;
+    if (cz_mass_sums_workspace(n) > MASS_NOALLOC_PARTS * sizeof(MassPart))
+        return set_err(CZ_EINVAL, "mass sums (no-alloc): %llu cells exceed the static scratch's maximum of %llu cells",
+                       (unsigned long long)n, (unsigned long long)(MASS_NOALLOC_PARTS * MASS_CHUNK));
+    MassPart* parts = nullptr;
+    rc = scratch_of_current_device(&parts);
+    if (rc) return rc;
+    return launch_mass(h, m, n, parts, out, (hipStream_t)stream);
 }
