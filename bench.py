@@ -2,19 +2,28 @@
 
 Workload: the plain single-canvas film, 2048x2048 cells, FP32, initial height
 a sum of 64 seeded Gaussian blobs, per-cell yield stress and viscosity,
-Dirichlet(0.3) pigments, 15 deg tilt (paint-sim-3d_amd/scenes.py).  A step is
-one frame of the reference's encode_sim/encode_frame (src/main.rs:1875-1909,
-src/film/mod.rs:1322-1462) at frame_dt = 1/60 s with the substep count chosen
-from the previous frame's stats.  value = canvas cells x frames / wall time
-of the timed frames (state resident in HBM).
+Dirichlet(0.3) pigments, 15 deg tilt (paint-sim-3d_amd/scenes.py), at the
+reference's uniform power-law index n = 1; a second record at the
+shear-thinning n = 0.7.  A step is one frame of the reference's
+encode_sim/encode_frame (src/main.rs:1875-1909, src/film/mod.rs:1322-1462) at
+frame_dt = 1/60 s with the substep count chosen from the previous frame's
+stats; the display shade pass (shade_full, src/film/mod.rs:1449-1456) is off,
+as its output only feeds the renderer.  value = canvas cells x frames / wall
+time of the timed frames (state resident in HBM).
 
-The timed frames run as the library runs them by default (graph replay,
-flux shapes tuned online) with no events in them.  roofline: the same frames
-replayed on a fresh film with HIP events around every launch; the substep
-kernel family with the largest device time; achieved = algorithmic bytes
-(per active cell, below, times the active cells its launches covered) / its
-event time.  cpu_baseline: the CPU restatement (oracle/, one core) on a
-512x512 window of the same canvas, about 12 s of frames from frame 2.
+Phases, each on a fresh film from the same initial state: W warmup frames
+with HIP events around every launch (picks the dominant kernel family), the
+K timed frames as the library runs them by default (graph replay, flux
+shapes tuned online, no events), and the same K frames replayed with events
+around every launch for the roofline: the substep kernel family with the
+largest device time; achieved = algorithmic bytes (per active cell, below,
+times the active cells its launches covered) / its event time.  Between the
+phases a one-element fill kernel marks the boundary in a kernel trace.
+roofline.traffic: HBM bytes per launch of that kernel from the rocprof
+FETCH_SIZE / WRITE_SIZE runs of the same command recorded under profiles/
+(null when no record matches the command).  cpu_baseline: the CPU
+restatement (oracle/, its per-tile passes on every host thread) stepping the
+same canvas from the same initial state, frame by frame, for up to 25 s.
 """
 import argparse
 import json
@@ -31,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 
 # Compulsory HBM bytes per active cell of one launch of each kernel family.
 ALGO_BYTES = {
-    "lap_plug": 4 + 4 + 4 + 8 + 8 + 4 + 8 + 8 + 4,  # R h m delta fnorm rheo, W delta fnorm pres tmp
+    "lap_plug": 4 + 4 + 4 + 8 + 8 + 4 + 8 + 8 + 4 + 4,  # R h m delta fnorm rheo, W delta fnorm pres tmp h_sub
     "flux": 4 + 8 + 8 + 4 + 4 + 8 + 16,             # R h rheo pres tmp delta fnorm, W flux
     "apply": 16 + 4 + 16 + 4 + 16,                  # R flux h acc, W h acc
 }
@@ -50,81 +59,94 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(state_fn, settings_fn, budget_s=12.0, max_frames=60):
-    """The CPU restatement (one core) on a 512x512 window of the plain canvas:
-    frames 2.. from the window's initial state until about budget_s seconds
-    of stepping (at most max_frames frames); cells x frames / time."""
-    import numpy as np
+def cpu_baseline(state, settings, frames, budget_s=25.0):
+    """The CPU restatement on all host threads (its per-tile passes run in
+    parallel; the FP64 sums stay in a fixed order) stepping the same canvas
+    from the same initial state for the same frames, stopping after the first
+    frame that ends past budget_s seconds: canvas cells x frames / time."""
     import canvaz_film as czf
+    import oracle as orc
     from helpers import make_pair
-    win = 512
-    full = state_fn()
-    st = {k: np.ascontiguousarray(v[:win, :win]) for k, v in full.items()}
-    s = settings_fn(win)
-    _, o = make_pair(win, win, s, st, pigtab=czf.default_pigment_table(), cmf=czf.cmf_table(), gpu=False)
-    o.step()
-    frames, dt = 0, 0.0
-    while dt < budget_s and frames < max_frames:
+    n = state["h"].shape[1]
+    _, o = make_pair(n, state["h"].shape[0], settings, state, pigtab=czf.default_pigment_table(),
+                     cmf=czf.cmf_table(), gpu=False)
+    done, dt = 0, 0.0
+    while done < frames and dt < budget_s:
         t0 = time.perf_counter()
         o.step()
         dt += time.perf_counter() - t0
-        frames += 1
-    return {"value": win * win * frames / dt, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"CPU restatement (oracle, one core), 512x512 window of the 2048^2 blob canvas, frames 2-"
-                      f"{frames + 1} from the window's initial state ({o.active_count()} active tiles at the end, "
-                      f"{dt:.1f} s)"}
+        done += 1
+    cells = state["h"].size
+    threads = int(orc.lib().of_omp_threads())
+    return {"value": cells * done / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"CPU restatement (oracle/, not the reference), {threads} threads, the same "
+                      f"{state['h'].shape[1]}x{state['h'].shape[0]} canvas and settings, frames 1-{done} of the "
+                      f"timed {frames} from the same initial state ({dt:.1f} s)"}
 
 
-def bench_km(args):
+def km_record(steps, warmup, cells=4096 * 4096, bands=36):
     """Spectral Kubelka-Munk mix: 4096x4096 cells x 12 pigments x 36 bands,
-    BF16 concentrations ~ Dirichlet(0.3), FP32 K and S tables log-uniform in
-    [0.01, 10]; value = cells / s; the kernel reads 24 B and writes 144 B per
-    cell and does 2 x 12 x 72 x 2 MFMA flops per cell (hi and lo parts)."""
+    BF16 concentrations ~ Dirichlet(0.3) (drawn on the device), FP32 K and S
+    tables log-uniform in [0.01, 10]; the kernel reads 24 B and writes 4 B
+    per band per cell and does 2 x 12 x (2 x bands) x 2 MFMA flops per cell
+    (K and S, hi and lo table parts).  Average launch time from HIP events on
+    torch's current stream (the kernel's stream); error against the FP64
+    restatement on 4096 sampled cells."""
+    import ctypes as C
     import numpy as np
     import torch
     import km
-    cells, bands = 4096 * 4096, 36
+    import oracle as orc
     rng = np.random.default_rng(36)
-    log("bench km: generating inputs")
-    conc = torch.from_numpy(rng.dirichlet(np.full(12, 0.3), cells).astype(np.float32)).to(torch.bfloat16).cuda()
+    g = torch.Generator(device="cuda").manual_seed(36)
+    gam = torch._standard_gamma(torch.full((cells, 12), 0.3, device="cuda"), generator=g)
+    conc = (gam / gam.sum(dim=1, keepdim=True)).to(torch.bfloat16).contiguous()
+    del gam
     kt = np.exp(rng.uniform(np.log(0.01), np.log(10), (12, bands))).astype(np.float32)
     st = np.exp(rng.uniform(np.log(0.01), np.log(10), (12, bands))).astype(np.float32)
     ops = torch.from_numpy(km.prepare(kt, st).view(np.int16)).cuda()
     out = torch.empty((cells, bands), dtype=torch.float32, device="cuda")
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         km.mix(conc, ops, bands, out)
     torch.cuda.synchronize()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record()
-    for _ in range(args.steps):
+    for _ in range(steps):
         km.mix(conc, ops, bands, out)
     ev1.record()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    avg_us = ev0.elapsed_time(ev1) * 1e3 / args.steps
+    avg_us = ev0.elapsed_time(ev1) * 1e3 / steps
     bytes_per = (24 + 4 * bands) * cells
     achieved = bytes_per / (avg_us * 1e-6) / 1e9
     flops = 2 * 12 * 2 * (2 * bands) * cells
-    # check a sample against the FP64 restatement
-    import ctypes as C
-    import oracle as orc
-    idx = rng.integers(0, cells, 4096)
+    idx = torch.from_numpy(rng.integers(0, cells, 4096)).cuda()
     cs = conc[idx].cpu().view(torch.int16).numpy().view(np.uint16).copy()
     ref = np.zeros((len(idx), bands), np.float64)
     orc.lib().of_km_mix_f64(cs.ctypes.data_as(C.POINTER(C.c_uint16)), kt.ctypes.data_as(C.POINTER(C.c_float)),
                             st.ctypes.data_as(C.POINTER(C.c_float)), len(idx), bands,
                             ref.ctypes.data_as(C.POINTER(C.c_double)))
     err = float(np.abs(out[idx].cpu().numpy() - ref).max())
-    line = {"metric": "km mix cells/s", "value": cells * args.steps / elapsed, "unit": "cells/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
-            "data": "synthetic",
-            "config": {"workload": "KM mix 4096x4096 cells x 12 pigments x 36 bands", "max_abs_err_vs_fp64": err},
+    return {"value": cells * steps / elapsed, "unit": "cells/s", "ms_per_step": elapsed * 1e3 / steps,
+            "steps": steps, "dtype": "bf16", "max_abs_err_vs_fp64": err,
+            "workload": f"KM mix {int(round(cells ** 0.5))}x{int(round(cells ** 0.5))} cells x 12 pigments x "
+                        f"{bands} bands, BF16 concentrations, FP32 accumulate",
             "roofline": {"bound": "hbm", "kernel": "k_km_mix", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "avg_launch_us": avg_us, "mfma_tflops": flops / (avg_us * 1e-6) / 1e12},
-            "cpu_baseline": None}
+                         "avg_launch_us": avg_us, "mfma_tflops": flops / (avg_us * 1e-6) / 1e12,
+                         "mfma_peak_tflops": 2500.0}}
+
+
+def bench_km(args):
+    """The KM mix alone (--workload km): one JSON line."""
+    r = km_record(args.steps, args.warmup)
+    line = {"metric": "km mix cells/s", "value": r["value"], "unit": "cells/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+            "data": "synthetic",
+            "config": {"workload": r["workload"], "max_abs_err_vs_fp64": r["max_abs_err_vs_fp64"]},
+            "roofline": r["roofline"], "cpu_baseline": None}
     print(json.dumps(line), flush=True)
 
 
@@ -453,6 +475,32 @@ def frame_algo_bytes(n_grid, st):
             + (180 + 68 * 40) * st.act_cell_implicit)
 
 
+def phase_marker(tag):
+    """One tiny fill kernel between bench phases, so a kernel trace of the run
+    can be cut at the phase boundaries (warmup, timed, replay, ...)."""
+    import torch
+    torch.cuda.synchronize()
+    torch.full((1,), float(tag), device="cuda")
+    torch.cuda.synchronize()
+
+
+def recorded_traffic(kernel, n, steps, warmup):
+    """HBM bytes per launch of `kernel` in the timed frames of this command,
+    from the FETCH_SIZE / WRITE_SIZE profiler runs recorded in profiles/
+    (lines 'traffic: kernel=K size=N steps=S warmup=W bytes_per_launch=B'),
+    or None when no record matches."""
+    import re
+    path = os.path.join(ROOT, "profiles", "film_plain_2048_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    pat = re.compile(r"traffic: kernel=(\w+) size=(\d+) steps=(\d+) warmup=(\d+) bytes_per_launch=([0-9.]+)")
+    for line in open(path):
+        m = pat.search(line)
+        if m and m.group(1) == kernel and (int(m.group(2)), int(m.group(3)), int(m.group(4))) == (n, steps, warmup):
+            return float(m.group(5))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -478,6 +526,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload in the film run")
     ap.add_argument("--batch-canvases", type=int, default=64)
     ap.add_argument("--batch-frames", type=int, default=200)
+    ap.add_argument("--no-km", action="store_true", help="skip the KM mix record in the film run")
+    ap.add_argument("--no-n07", action="store_true", help="skip the n = 0.7 plain record in the film run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (canvases are independent "
@@ -523,6 +573,7 @@ def main():
         return f
 
     # warmup film: the first frames of the same run, every family timed, to pick the dominant one
+    phase_marker(1)
     film = fresh()
     film.profile(True)
     for _ in range(max(1, args.warmup)):
@@ -540,6 +591,7 @@ def main():
     del film
 
     # timed pass: a fresh film from the initial state, K frames
+    phase_marker(2)
     film = fresh()
     if args.events_in_timed:
         film.profile(True)
@@ -559,6 +611,7 @@ def main():
         # the same frames again with events around every launch
         film.close()
         del film
+        phase_marker(3)
         film = fresh()
         film.profile(True)
         film.reset_counters()
@@ -570,6 +623,7 @@ def main():
     film.profile(False)
     film.close()
     del film
+    phase_marker(4)
     dom = max(ROOFLINE_FAMILIES, key=lambda fam: prof[fam][0] * prof[fam][1])
     avg_us, launches = prof[dom]
     if dom == "substeps":
@@ -597,18 +651,47 @@ def main():
     log(f"bench: {ms_per_step:.3f} ms/frame, active cells/frame {st.act_cells / frames:.0f}, "
         f"substeps/frame {st.act_cell_substeps / max(1, st.act_cells):.1f}, {dom}: {avg_us:.2f} us x {launches}, "
         f"{cells_per_launch:.0f} cells/launch")
+    plain_n07 = None
+    if not args.no_n07:
+        # the same canvas and frames at the shear-thinning n = 0.7 (the
+        # power-law branches of hb_flux and band_flux, film.wgsl:94-126)
+        s07 = settings_fn(n)
+        s07.n_index = 0.7
+        f07 = czf.Film(n, n, s07)
+        czf.check(czf.lib().cz_film_use_graphs(f07._p, 0 if args.no_graphs else 1))
+        assert f07.write_state_arrays(state)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            f07.step()
+        f07.sync()
+        el07 = time.perf_counter() - t0
+        s7 = f07.stats()
+        plain_n07 = {"value": n * n * args.steps / el07, "unit": "cell-steps/s", "ms_per_step": el07 * 1e3 / args.steps,
+                     "n_index": 0.7, "active_cells_per_frame": s7.act_cells / max(1, s7.frames_counted),
+                     "substeps_per_frame": s7.act_cell_substeps / max(1, s7.act_cells)}
+        f07.close()
+        log(f"bench: n = 0.7: {el07 * 1e3 / args.steps:.3f} ms/frame")
+        phase_marker(5)
+    km_rec = None
+    if not args.no_km:
+        km_rec = km_record(max(5, min(args.steps, 50)), 3)
+        log(f"bench km: {km_rec['roofline']['avg_launch_us']:.1f} us, {km_rec['roofline']['achieved']:.0f} GB/s, "
+            f"max err {km_rec['max_abs_err_vs_fp64']:.2e}")
+        phase_marker(6)
     batched = None if args.no_batch else bench_batch(args, value)
     served = None if args.no_serve else bench_serve(args)
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU restatement baseline")
-        cpu = cpu_baseline(lambda: state, settings_fn)
+        cpu = cpu_baseline(state, settings_fn(n), args.steps)
     line = {
         "metric": "film cell-steps/s", "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"plain film {n}x{n} FP32, 64 seeded Gaussian blobs, tilt 15 deg, frame_dt 1/60, "
-                               f"{args.steps} frames from the initial state",
+        "config": {"workload": f"plain film {n}x{n} FP32, 64 seeded Gaussian blobs, uniform power-law index n = 1 "
+                               f"(the reference's default), tilt 15 deg, frame_dt 1/60, {args.steps} frames from "
+                               f"the initial state, display shade pass off",
                    "cells": n * n, "graphs": not args.no_graphs, "substep_launch": args.substep_launch,
                    "one_launch_frames": st.one_launch_frames / max(1, st.frames_counted),
                    "flux_shape": args.flux_shape,
@@ -618,10 +701,12 @@ def main():
                    "implicit_cell_steps": st.act_cell_implicit,
                    "frame_algo_GBps": frame_gbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algo_bytes_per_cell": algo_per_cell, "avg_launch_us": avg_us, "launches": launches,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": recorded_traffic(dom, n, args.steps, args.warmup),
+                     "algo_bytes_per_launch": bytes_per_launch, "algo_bytes_per_cell": algo_per_cell, "avg_launch_us": avg_us, "launches": launches,
                      "cells_per_launch": cells_per_launch},
         "cpu_baseline": cpu,
+        "plain_n07": plain_n07,
+        "km": km_rec,
         "batched": batched,
         "served": served,
     }

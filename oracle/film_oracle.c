@@ -31,6 +31,7 @@
  * Built with -ffp-contract=off: every float expression below is evaluated in
  * the association written, which follows the WGSL source left to right.
  */
+#include <omp.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -2933,6 +2934,8 @@ cz_params* of_params(ofilm* F) { return &F->P; }
 cz_settings* of_settings(ofilm* F) { return &F->s; }
 cz_overrides* of_overrides(ofilm* F) { return &F->ov; }
 uint32_t of_active_count(const ofilm* F) { return F->active_count; }
+/* threads the per-tile passes run on (OpenMP) */
+int of_omp_threads(void) { return omp_get_max_threads(); }
 uint32_t of_implicit_steps(const ofilm* F) { return F->implicit_steps; }
 const float* of_cg_s(const ofilm* F) { return F->cg_s; }
 

@@ -54,6 +54,8 @@ def lib():
         L.of_overrides.restype = C.POINTER(Overrides)
         L.of_overrides.argtypes = [P]
         L.of_active_count.restype = u32
+        L.of_omp_threads.restype = C.c_int
+        L.of_omp_threads.argtypes = []
         L.of_active_count.argtypes = [P]
         L.of_implicit_steps.restype = u32
         L.of_implicit_steps.argtypes = [P]

@@ -229,7 +229,7 @@ int record(cz_batch* b, const FrameShape& sh) {
         const ManyFilms w = many(b->items_imp, lim_imp + i, 0);
         const ManyCanvases c{b->desc, b->imp_members, b->imp_bases, 0u};
         const dim3 g(sh.g_imp[i]), gc(sh.m_imp[i]);
-        hipLaunchKernelGGL(k_lap_plug<ManyFilms>, g, b256, 0, st, w, 0, 0, i == 0 ? 1 : 0);
+        hipLaunchKernelGGL(k_lap_tail<ManyFilms>, g, b256, 0, st, w, 0, i == 0 ? 1 : 0);
         hipLaunchKernelGGL(k_cg_setup<ManyFilms>, g, b256, 0, st, w);
         hipLaunchKernelGGL(k_cg_lap<ManyFilms>, g, b256, 0, st, w, 1u);
         hipLaunchKernelGGL(k_cg_init<ManyFilms>, g, b256, 0, st, w);
@@ -277,7 +277,7 @@ int record(cz_batch* b, const FrameShape& sh) {
     }
     const uint32_t* all = lim_sub + sh.max_sub;
     const dim3 ga(sh.g_all);
-    hipLaunchKernelGGL(k_lap_plug<ManyFilms>, ga, b256, 0, st, many(b->items_sub, all, 2), 1, 0, 0);
+    hipLaunchKernelGGL(k_lap_tail<ManyFilms>, ga, b256, 0, st, many(b->items_sub, all, 2), 1, 0);
     const ManyFilms w = many(b->items_sub, all, 3);
     hipLaunchKernelGGL(k_strata_split<ManyFilms>, ga, b256, 0, st, w);
     hipLaunchKernelGGL(k_reduce_pool<ManyFilms>, dim3(sh.g64), b256, 0, st, many(b->all64, n_all64, 3));

@@ -3,6 +3,7 @@
 // buffers, update_params / choose_substeps, upload_deposits binning,
 // encode_frame order, stats readback, read/write_state, dry_now, clear.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -45,6 +46,10 @@ int dalloc(cz_film* f, T** p, size_t count) {
 }
 
 // RAII-free launch helper: optional event pair around a launch, by label.
+// Profiling of one launch: the two events go into the kernel's own dispatch
+// (hipExtLaunchKernelGGL), which stamps them when the kernel starts and
+// ends, so the time read back is the kernel's, without the event packets a
+// hipEventRecord on either side would add to it.
 struct Launch {
     cz_film* f;
     const char* label;
@@ -53,14 +58,15 @@ struct Launch {
         if (f->prof_on && (f->prof_only.empty() || f->prof_only == l)) {
             a = f->take_event();
             b = f->take_event();
-            if (a) hipEventRecord(a, f->stream);
+            if (!a || !b) {
+                if (a) f->ev_pool.push_back(a);
+                if (b) f->ev_pool.push_back(b);
+                a = b = nullptr;
+            }
         }
     }
     ~Launch() {
-        if (a && b) {
-            hipEventRecord(b, f->stream);
-            f->prof_pending.push_back({label, a, b});
-        }
+        if (a && b) f->prof_pending.push_back({label, a, b});
     }
 };
 
@@ -247,10 +253,13 @@ int upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* se
     return CZ_OK;
 }
 
-#define LAUNCH(label, kern, grid, block, ...)                              \
-    do {                                                                   \
-        Launch _l(f, label);                                               \
-        hipLaunchKernelGGL(kern, grid, block, 0, f->stream, __VA_ARGS__);  \
+#define LAUNCH(label, kern, grid, block, ...)                                                    \
+    do {                                                                                         \
+        Launch _l(f, label);                                                                     \
+        if (_l.a)                                                                                \
+            hipExtLaunchKernelGGL(kern, grid, block, 0, f->stream, _l.a, _l.b, 0u, __VA_ARGS__); \
+        else                                                                                     \
+            hipLaunchKernelGGL(kern, grid, block, 0, f->stream, __VA_ARGS__);                    \
     } while (0)
 
 // encode_frame (src/film/mod.rs:1322-1462): the launches of one frame.
@@ -291,7 +300,7 @@ int record_frame(cz_film* f, uint32_t substeps) {
         for (uint32_t it = 0; it < f->implicit_steps; it++) {
             const OneFilm w{f->dev()};
             const OneCanvas c1{w.F};
-            LAUNCH("lap", k_lap_plug<OneFilm>, gt, b256, w, 0, 0, it == 0 ? 1 : 0);
+            LAUNCH("lap", k_lap_tail<OneFilm>, gt, b256, w, 0, it == 0 ? 1 : 0);
             LAUNCH("cg_setup", k_cg_setup<OneFilm>, gt, b256, w);
             LAUNCH("cg_lap", k_cg_lap<OneFilm>, gt, b256, w, 1u);
             LAUNCH("cg_init", k_cg_init<OneFilm>, gt, b256, w);
@@ -358,7 +367,7 @@ int record_frame(cz_film* f, uint32_t substeps) {
     }
     // nose of the last substep + the laplacian strata_split reads
     if (pg == 0u) {
-        LAUNCH("lap", k_lap_plug<OneFilm>, gt, b256, OneFilm{f->dev()}, 1, 0, 0);
+        LAUNCH("lap", k_lap_tail<OneFilm>, gt, b256, OneFilm{f->dev()}, 1, 0);
         f->delta_cur_a = !f->delta_cur_a;
     }
     const OneFilm w{f->dev()};

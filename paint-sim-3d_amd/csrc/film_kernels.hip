@@ -1024,6 +1024,15 @@ __global__ __launch_bounds__(256) void k_lap_plug(WS ws, int do_nose, int do_plu
     CZ_FOR_TILES(ws) lap_plug_tile(F, _t, S, do_nose, do_plug, do_clear);
 }
 
+// The same pass without plug: the nose of a frame's last substep and the
+// laplacian of the implicit steps.  A kernel of its own, so that a kernel
+// trace tells the substep passes (k_lap_plug) from these.
+template <class WS>
+__global__ __launch_bounds__(256) void k_lap_tail(WS ws, int do_nose, int do_clear) {
+    __shared__ LapShared S;
+    CZ_FOR_TILES(ws) lap_plug_tile(F, _t, S, do_nose, 0, do_clear);
+}
+
 // fluxes (film.wgsl:375-455), face-parallel: each block takes one
 // (tile, face) pair and reads its fields from HBM through the caches; 4x the
 // waves of a cell-per-thread launch to hide the dependent loads of the
@@ -2389,6 +2398,7 @@ CZ_INST1(k_reduce_pool, OneFilmAll)
 CZ_INST1(k_reduce_pool, ManyFilms)
 #define CZ_INST2(name, ...) CZ_INST1(name, OneFilm, ##__VA_ARGS__) CZ_INST1(name, ManyFilms, ##__VA_ARGS__)
 CZ_INST2(k_lap_plug, int, int, int)
+CZ_INST2(k_lap_tail, int, int)
 CZ_INST1(k_mark, OneFilmAll)
 CZ_INST1(k_shade, OneFilmAll)
 CZ_INST1(k_mark, ManyFilms)
