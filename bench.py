@@ -133,7 +133,9 @@ def km_record(steps, warmup, cells=4096 * 4096, bands=36):
             "workload": f"KM mix {int(round(cells ** 0.5))}x{int(round(cells ** 0.5))} cells x 12 pigments x "
                         f"{bands} bands, BF16 concentrations, FP32 accumulate",
             "roofline": {"bound": "hbm", "kernel": "k_km_mix", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": recorded_traffic("km_mix_4096sq_36band_kernel_stats.md", kernel="k_km_mix",
+                                                     size=cells, bands=bands),
                          "avg_launch_us": avg_us, "mfma_tflops": flops / (avg_us * 1e-6) / 1e12,
                          "mfma_peak_tflops": 2500.0}}
 
@@ -475,6 +477,31 @@ def frame_algo_bytes(n_grid, st):
             + (180 + 68 * 40) * st.act_cell_implicit)
 
 
+def stream_peak(nbytes=1 << 30, reps=20):
+    """What HBM delivers to a kernel on this device: cz_stream_probe's copy
+    (read + write bytes counted) and fill of nbytes, GB/s from HIP events on
+    torch's current stream."""
+    import torch
+    import canvaz_film as czf
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    st = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for mode, name, moved in ((0, "copy_GBps", 2 * nbytes), (1, "fill_GBps", nbytes)):
+        for _ in range(3):
+            czf.check(czf.lib().cz_stream_probe(b.data_ptr(), a.data_ptr(), nbytes, mode, st))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            czf.check(czf.lib().cz_stream_probe(b.data_ptr(), a.data_ptr(), nbytes, mode, st))
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = moved / (e0.elapsed_time(e1) * 1e-3 / reps) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def phase_marker(tag):
     """One tiny fill kernel between bench phases, so a kernel trace of the run
     can be cut at the phase boundaries (warmup, timed, replay, ...)."""
@@ -484,20 +511,20 @@ def phase_marker(tag):
     torch.cuda.synchronize()
 
 
-def recorded_traffic(kernel, n, steps, warmup):
-    """HBM bytes per launch of `kernel` in the timed frames of this command,
-    from the FETCH_SIZE / WRITE_SIZE profiler runs recorded in profiles/
-    (lines 'traffic: kernel=K size=N steps=S warmup=W bytes_per_launch=B'),
-    or None when no record matches."""
-    import re
-    path = os.path.join(ROOT, "profiles", "film_plain_2048_hbm_traffic.md")
+def recorded_traffic(profile, **key):
+    """HBM bytes per launch recorded in profiles/<profile> from the FETCH_SIZE
+    / WRITE_SIZE profiler runs of a command (lines 'traffic: kernel=K
+    size=N ... bytes_per_launch=B'); None unless every key=value given here
+    matches the record."""
+    path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
-    pat = re.compile(r"traffic: kernel=(\w+) size=(\d+) steps=(\d+) warmup=(\d+) bytes_per_launch=([0-9.]+)")
     for line in open(path):
-        m = pat.search(line)
-        if m and m.group(1) == kernel and (int(m.group(2)), int(m.group(3)), int(m.group(4))) == (n, steps, warmup):
-            return float(m.group(5))
+        if not line.startswith("traffic: "):
+            continue
+        rec = dict(kv.split("=", 1) for kv in line.split()[1:])
+        if all(rec.get(k) == str(v) for k, v in key.items()):
+            return float(rec["bytes_per_launch"])
     return None
 
 
@@ -518,7 +545,7 @@ def main():
                     help="substeps as one launch per frame on small active sets (auto), always, or never")
     ap.add_argument("--events-in-timed", action="store_true",
                     help="record the roofline's HIP events inside the timed frames (default: a separate replay)")
-    ap.add_argument("--workload", default="film", choices=("film", "km", "batch", "serve", "next"))
+    ap.add_argument("--workload", default="film", choices=("film", "km", "batch", "serve", "next", "stream"))
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload in the film run")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-budget-gb", type=int, default=200)
@@ -548,6 +575,10 @@ def main():
         torch.cuda.synchronize()
         log(f"bench: copy probe, 3 device copies of {args.copy_probe_mib} MiB")
         del src, dst
+    if args.workload == "stream":
+        print(json.dumps({"metric": "HBM stream GB/s", "unit": "GB/s", "n_gpus": 1, "peak": HBM_PEAK_GBS,
+                          **stream_peak()}), flush=True)
+        return
     if args.workload == "km":
         return bench_km(args)
     if args.workload == "batch":
@@ -673,12 +704,17 @@ def main():
         f07.close()
         log(f"bench: n = 0.7: {el07 * 1e3 / args.steps:.3f} ms/frame")
         phase_marker(5)
+    measured = stream_peak()
+    log(f"bench: measured HBM stream: copy {measured['copy_GBps']:.0f} GB/s, fill {measured['fill_GBps']:.0f} GB/s")
+    phase_marker(6)
     km_rec = None
     if not args.no_km:
         km_rec = km_record(max(5, min(args.steps, 50)), 3)
+        km_rec["roofline"]["measured_copy_GBps"] = measured["copy_GBps"]
+        km_rec["roofline"]["frac_of_measured_copy"] = km_rec["roofline"]["achieved"] / measured["copy_GBps"]
         log(f"bench km: {km_rec['roofline']['avg_launch_us']:.1f} us, {km_rec['roofline']['achieved']:.0f} GB/s, "
             f"max err {km_rec['max_abs_err_vs_fp64']:.2e}")
-        phase_marker(6)
+        phase_marker(7)
     batched = None if args.no_batch else bench_batch(args, value)
     served = None if args.no_serve else bench_serve(args)
     cpu = None
@@ -701,7 +737,10 @@ def main():
                    "implicit_cell_steps": st.act_cell_implicit,
                    "frame_algo_GBps": frame_gbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": recorded_traffic(dom, n, args.steps, args.warmup),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": recorded_traffic("film_plain_2048_hbm_traffic.md", kernel=dom, size=n, steps=args.steps,
+                                                 warmup=args.warmup),
+                     "measured_copy_GBps": measured["copy_GBps"], "frac_of_measured_copy": achieved / measured["copy_GBps"],
                      "algo_bytes_per_launch": bytes_per_launch, "algo_bytes_per_cell": algo_per_cell, "avg_launch_us": avg_us, "launches": launches,
                      "cells_per_launch": cells_per_launch},
         "cpu_baseline": cpu,

@@ -642,6 +642,12 @@ uint64_t cz_lz4_bound(uint64_t n);
  * inputs where they differ from the software round-to-nearest-even forms
  * the host restatement uses (both counts are 0 on a conforming device). */
 int cz_f16_selftest(uint64_t* bad_f2h, uint64_t* bad_h2f);
+/* No counterpart in the reference: the bandwidth probe the measurements use
+ * as the device's delivered HBM ceiling.  mode 0 copies `bytes` from src to
+ * dst, mode 1 fills dst; grid-stride 16-byte non-temporal accesses, enqueued
+ * on `stream` (hipStream_t, NULL = default).  bytes and both pointers must be
+ * multiples of 16. */
+int cz_stream_probe(void* dst, const void* src, uint64_t bytes, int mode, void* stream);
 
 const char* cz_last_error(void);
 const char* cz_version(void);

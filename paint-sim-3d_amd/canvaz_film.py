@@ -35,6 +35,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise FilmError(f"{LIB_PATH} is missing: build it with paint-sim-3d_amd/build.py")
+    # PyTorch ships its own HIP runtime under the same SONAME as the one the
+    # library links (libamdhip64.so.7).  Loaded after the library, torch maps
+    # a second runtime, and whichever of the two opens the device second sees
+    # no device.  Loading torch first makes the library bind to torch's copy,
+    # so one runtime serves the process and device pointers pass between them.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P, f32, u32, u64, i32 = C.c_void_p, C.c_float, C.c_uint32, C.c_uint64, C.c_int
     fp, up = C.POINTER(f32), C.POINTER(u32)
@@ -134,6 +143,7 @@ def lib():
         "cz_batch_graph_counts": (i32, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
         "cz_batch_flux_shape": (i32, [P, i32]),
         "cz_batch_stream": (C.c_void_p, [P]),
+        "cz_stream_probe": (i32, [C.c_void_p, C.c_void_p, u64, i32, C.c_void_p]),
         "cz_mass_sums_workspace": (u64, [u64]),
         "cz_mass_sums": (i32, [C.c_void_p, C.c_void_p, u64, C.c_void_p, u64, C.c_void_p, C.c_void_p]),
         "cz_mass_sums_noalloc": (i32, [C.c_void_p, C.c_void_p, u64, C.c_void_p, C.c_void_p]),

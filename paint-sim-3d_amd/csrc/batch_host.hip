@@ -194,6 +194,7 @@ struct FrameShape {
 };
 
 int record(cz_batch* b, const FrameShape& sh) {
+    drop_stale_error();
     const dim3 b256(256);
     hipStream_t st = b->stream;
     const uint32_t* lim_sub = b->lims;

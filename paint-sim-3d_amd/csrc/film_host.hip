@@ -264,6 +264,7 @@ int upload_deposits(cz_film* f, const cz_deposit* deps, uint32_t n, uint32_t* se
 
 // encode_frame (src/film/mod.rs:1322-1462): the launches of one frame.
 int record_frame(cz_film* f, uint32_t substeps) {
+    drop_stale_error();
     const dim3 b256(256);
     const dim3 gt(f->tile_grid());
     const uint32_t n64 = f->t64x * f->t64y;
@@ -966,6 +967,7 @@ int cz_film_dry_now(cz_film* f) {
     if (rc) return rc;
     cz_params saved = f->P;
     f->P.cure[0] = 4.0f / fmaxf(f->P.frame_dt, 1e-4f);
+    drop_stale_error();
     hipLaunchKernelGGL(k_cure_clear<OneFilmAll>, dim3(f->nt16), dim3(256), 0, f->stream, OneFilmAll{f->dev(), f->nt16}, 0);
     HIP_TRY(hipGetLastError());
     f->P = saved;

@@ -20,6 +20,12 @@ using namespace cz;
 int cz_set_err(int code, const char* fmt, ...);
 #define set_err cz_set_err
 
+// hipGetLastError returns the last failed runtime call of the calling
+// thread, whoever made it (the caller's own HIP or torch calls included).
+// An entry point that checks its launches with it first drops what an
+// earlier call left, so that the check reports its own launches only.
+inline void drop_stale_error() { (void)hipGetLastError(); }
+
 #define HIP_TRY(expr)                                                                     \
     do {                                                                                  \
         hipError_t _e = (expr);                                                           \

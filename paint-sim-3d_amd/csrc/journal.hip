@@ -387,6 +387,7 @@ int cz_film_restore_journal(cz_film* f, uint32_t target) {
     if (!f->j_cap) return CZ_OK;  // nothing was ever recorded
     const FilmDev F = f->dev();
     const JournalDev J = jdev(f);
+    drop_stale_error();
     hipLaunchKernelGGL(k_journal_best, dim3((f->nt16 + 255u) / 256u), dim3(256), 0, f->stream, J, f->nt16);
     hipLaunchKernelGGL(k_journal_scan, dim3((f->j_cap + 255u) / 256u), dim3(256), 0, f->stream, F, J, target);
     hipLaunchKernelGGL(k_journal_apply, dim3(f->j_cap), dim3(256), 0, f->stream, F, J, target);

@@ -5,15 +5,16 @@
 // of (cells x 12) by (12 x bands), followed per (cell, band) by the
 // infinite-layer reflectance R_inf = 1 + a - sqrt(a^2 + 2a), a = K/S (the
 // t -> infinity limit of km_layer, km.wgsl:1-11), evaluated in the
-// cancellation-free form 1 / (1 + a + sqrt(a (a + 2))).
+// cancellation-free form 1 / (1 + a + sqrt(a (a + 2))) with the hardware
+// reciprocal and square root.
 //
 // MFMA mapping (v_mfma_f32_16x16x32_bf16, one wave64 per 16-cell group):
-//   A (16 cells x 32): k = 0..11 concentrations, 12..15 zero, 16..27 the same
-//     concentrations again, 28..31 zero (BF16 inputs, exact);
-//   B (32 x 16 columns): k = 0..11 the BF16 high part of the FP32 table,
+//   concentrations (32 x 16 cells): k = 0..11 concentrations, 12..15 zero,
+//     16..27 the same concentrations again, 28..31 zero (BF16 inputs, exact);
+//   table (16 bands x 32): k = 0..11 the BF16 high part of the FP32 table,
 //     16..27 its BF16 residual (table - high), so that one instruction sums
 //     conc x (high + low) with FP32 accumulation;
-//   columns: NT tiles of 16 bands for K, then NT tiles for S, so that K and S
+//   tiles: NT tiles of 16 bands for K, then NT tiles for S, so that K and S
 //     of one band land in the same lane of the accumulator.
 // The op is HBM-bound (24 B read + 4*bands B written per cell); results are
 // staged through LDS so that each wave writes its 16 x bands floats as one
@@ -35,10 +36,20 @@ namespace {
 constexpr int KM_NPIG = 12;
 constexpr int KM_MAX_TILES = 3;  // bands <= 48
 
+// R_inf = 1 / (1 + a + sqrt(a (a + 2))) with a = K / S, multiplied through
+// by S: S / (S + K + sqrt(K (K + 2 S))), exact algebra that needs one
+// reciprocal and one square root per (cell, band) instead of two
+// reciprocals and a square root.  Both are the hardware instructions
+// (v_rcp_f32, v_sqrt_f32, each within 1 ulp): the relative error stays within
+// a few 1e-7, far inside the 1e-4 bar (the BF16 table split contributes
+// ~1e-6).  The transcendental unit is the kernel's VALU bound, so the count of
+// these two instructions is what sets its speed.  A cell without pigment
+// (S = K = 0) gives 0, as km_layer does for k + s < 1e-6 (km.wgsl:3); the
+// guard is a select, not a branch.
 __device__ __forceinline__ float r_inf(float k, float s) {
-    if (!(s > 0.0f)) return 0.0f;
-    float a = k / s;
-    return 1.0f / (1.0f + a + sqrtf(a * (a + 2.0f)));
+    const float q = __builtin_amdgcn_sqrtf(k * fmaf(2.0f, s, k));
+    const float r = s * __builtin_amdgcn_rcpf(s + k + q);
+    return (s > 0.0f) ? r : 0.0f;
 }
 
 // A operand of one lane for the 16-cell group at `base`: 8 concentrations of
@@ -68,50 +79,70 @@ __device__ __forceinline__ u16x8 load_a(const uint16_t* __restrict__ conc, uint6
 // all of their loads before computing, to keep enough bytes in flight.
 constexpr int KM_BATCH = 4;
 
+// The product is formed transposed, D = table^T (16 bands x 32) . conc^T
+// (32 x 16 cells): the prepared table tile is the A operand and the lane's
+// concentrations the B operand (the two operand layouts of the instruction
+// are the same, lane (grp, i) holding k = 8 grp .. 8 grp + 7 of row/column
+// i).  The accumulator of lane (grp, i) then holds bands 4 grp .. 4 grp + 3
+// of the tile for cell i, K and S of a band in the same lane and slot: a lane
+// puts its four reflectances into the wave's LDS stage with one 16-byte
+// write.  The stage (the group's 16 cells x bands floats, row-major as in
+// HBM) is then written out as one contiguous run of 16-byte stores, whole
+// cache lines: storing each lane's 16 bytes straight to HBM leaves partial
+// lines and doubles the bytes written.
 template <int NT>
 __global__ __launch_bounds__(256) void k_km_mix(const uint16_t* __restrict__ conc, const u16x8* __restrict__ bops,
                                                 uint64_t cells, uint32_t bands, float* __restrict__ out) {
-    __shared__ float stage[4][16 * 16 * KM_MAX_TILES];
+    __shared__ __align__(16) float stage[4][16 * 16 * KM_MAX_TILES];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t row = lane & 15u, grp = lane >> 4;
-    // B operands: constant for the whole launch
-    bf16x8 b[2 * NT];
+    // table operands: constant for the whole launch
+    bf16x8 tab[2 * NT];
 #pragma unroll
     for (int t = 0; t < 2 * NT; t++) {
         u16x8 v = bops[t * 64 + lane];
-        b[t] = __builtin_bit_cast(bf16x8, v);
+        tab[t] = __builtin_bit_cast(bf16x8, v);
     }
     const uint64_t groups = (cells + 15u) / 16u;
     const uint64_t batches = (groups + KM_BATCH - 1) / KM_BATCH;
     const uint64_t wave_id = (uint64_t)blockIdx.x * 4u + wave;
     const uint64_t nwaves = (uint64_t)gridDim.x * 4u;
+    const bool quads = (bands & 3u) == 0u;
     float* st = stage[wave];
     for (uint64_t bi = wave_id; bi < batches; bi += nwaves) {
-        u16x8 av[KM_BATCH];
+        u16x8 cv[KM_BATCH];
 #pragma unroll
         for (int g = 0; g < KM_BATCH; g++) {
             const uint64_t base = (bi * KM_BATCH + g) * 16u;
-            av[g] = load_a(conc, base + row, cells, grp);
+            cv[g] = load_a(conc, base + row, cells, grp);
         }
 #pragma unroll
         for (int g = 0; g < KM_BATCH; g++) {
             const uint64_t base = (bi * KM_BATCH + g) * 16u;
             if (base >= cells) break;
-            const bf16x8 a = __builtin_bit_cast(bf16x8, av[g]);
+            const bf16x8 c = __builtin_bit_cast(bf16x8, cv[g]);
             f32x4 acc[2 * NT];
 #pragma unroll
             for (int t = 0; t < 2 * NT; t++) {
                 f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], z, 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(tab[t], c, z, 0, 0, 0);
             }
-            // accumulator: lane holds rows 4*grp + j (cells), column `row` (band in tile)
+            float* sr = st + row * bands;
 #pragma unroll
             for (int t = 0; t < NT; t++) {
-                const uint32_t band = 16u * t + row;
+                const uint32_t b0 = 16u * t + 4u * grp;
+                if (b0 >= bands) continue;
+                if (quads) {
+                    f32x4 v;
+                    v[0] = r_inf(acc[t][0], acc[NT + t][0]);
+                    v[1] = r_inf(acc[t][1], acc[NT + t][1]);
+                    v[2] = r_inf(acc[t][2], acc[NT + t][2]);
+                    v[3] = r_inf(acc[t][3], acc[NT + t][3]);
+                    *(f32x4*)(sr + b0) = v;
+                } else {
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const uint32_t cl = 4u * grp + j;
-                    if (band < bands) st[cl * bands + band] = r_inf(acc[t][j], acc[NT + t][j]);
+                    for (int j = 0; j < 4; j++)
+                        if (b0 + j < bands) sr[b0 + j] = r_inf(acc[t][j], acc[NT + t][j]);
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -120,7 +151,7 @@ __global__ __launch_bounds__(256) void k_km_mix(const uint16_t* __restrict__ con
             const uint64_t ncell = (cells - base) < 16u ? (cells - base) : 16u;
             const uint32_t nflt = (uint32_t)ncell * bands;
             float* dst = out + base * bands;
-            if ((bands & 3u) == 0u) {
+            if (quads) {
                 for (uint32_t q = lane; q < nflt / 4u; q += 64u) {
                     f32x4 v = *(const f32x4*)(st + 4u * q);
                     __builtin_nontemporal_store(v, (f32x4*)(dst + 4u * q));
@@ -208,6 +239,10 @@ int cz_km_mix(const uint16_t* conc_bf16, const uint16_t* ops, uint64_t cells, ui
     const uint32_t grid = (uint32_t)(want < 4096u ? want : 4096u);
     hipStream_t s = (hipStream_t)stream;
     const u16x8* b = (const u16x8*)ops;
+    // hipGetLastError reports the last failed runtime call of the thread,
+    // whoever made it: drop what an earlier call left before launching, so
+    // that the check below reports this launch only
+    (void)hipGetLastError();
     if (NT == 1) hipLaunchKernelGGL(k_km_mix<1>, dim3(grid), dim3(256), 0, s, conc_bf16, b, cells, bands, out_refl);
     else if (NT == 2) hipLaunchKernelGGL(k_km_mix<2>, dim3(grid), dim3(256), 0, s, conc_bf16, b, cells, bands, out_refl);
     else hipLaunchKernelGGL(k_km_mix<3>, dim3(grid), dim3(256), 0, s, conc_bf16, b, cells, bands, out_refl);

@@ -130,6 +130,7 @@ uint64_t parts_for(uint64_t n) { return (n + MASS_CHUNK - 1) / MASS_CHUNK; }
 
 int launch_mass(const float* h, const float* m, uint64_t n, MassPart* parts, cz_mass_totals* out, hipStream_t st) {
     const uint64_t np = parts_for(n);
+    drop_stale_error();
     if (np > 0) {
         hipLaunchKernelGGL(k_mass_parts, dim3((uint32_t)np), dim3(256), 0, st, h, m, n, parts);
         HIP_TRY(hipGetLastError());

@@ -114,6 +114,7 @@ int cz_film_seed_blobs(cz_film* f, const cz_scene* s) {
     const uint64_t n = f->n;
     // one partial per block in cg_part (nt16 >= n / 256 entries)
     const uint32_t grid = (uint32_t)std::min<uint64_t>(std::min<uint64_t>((n + 255u) / 256u, f->nt16), 8192u);
+    drop_stale_error();
     hipLaunchKernelGGL(k_seed_blobs, dim3(grid), dim3(256), 0, f->stream, f->dev(), P);
     HIP_TRY(hipGetLastError());
     std::vector<double> parts(grid);

@@ -28,6 +28,7 @@ def declared_functions():
 def test_every_declared_symbol_is_exported():
     names = declared_functions()
     assert len(names) > 40
+    czf.lib()  # loads torch's HIP runtime first, as every user of the mirror does
     lib = C.CDLL(czf.LIB_PATH)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
@@ -54,3 +55,18 @@ def test_create_without_device_is_an_error():
     assert rc < 0
     assert czf.lib().cz_last_error()
     assert not ptr.value
+
+
+def test_library_shares_torchs_hip_runtime():
+    """One HIP runtime per process: the mirror loads torch's before the
+    library, which then binds to it instead of mapping a second copy (a
+    second runtime sees no device once the first has opened it)."""
+    pytest.importorskip("torch")
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r); import canvaz_film as czf; czf.lib(); "
+            "m = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}; print(len(m))"
+            % os.path.join(ROOT, "paint-sim-3d_amd"))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip().splitlines()[-1] == "1"
