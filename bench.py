@@ -50,9 +50,12 @@ ALGO_BYTES["lap"] = 4 + 4 + 4 + 8 + 4 + 8 + 8       # last nose: R h m delta fno
 # flux + apply in one launch: the flux pass's reads and the apply's, the
 # tile's fluxes written once and never read back
 ALGO_BYTES["flux_apply"] = (4 + 8 + 8 + 4 + 4 + 8) + (4 + 16) + (16 + 4 + 16)
-# families the roofline is reported for: the per-pass substep kernels and the
-# one-launch substeps kernel (k_substeps) that replaces them on small active sets
-ROOFLINE_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "substeps")
+# the 1024-thread shapes of the same passes move the same bytes
+for _fam in ("lap_plug", "flux", "apply", "flux_apply"):
+    ALGO_BYTES[_fam + "_w"] = ALGO_BYTES[_fam]
+# families the roofline is reported for: the per-pass substep kernels in both
+# block sizes and the one-launch substeps kernel (k_substeps)
+ROOFLINE_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "lap_plug_w", "flux_w", "apply_w", "flux_apply_w", "substeps")
 
 
 def log(msg):
@@ -373,6 +376,7 @@ def bench_batch(args, plain_value):
         films.append(f)
     b = czf.Batch(max(len(films), 1))
     b.use_graphs(not args.no_graphs)
+    b.flux_shape(args.flux_shape)
     for f in films:
         b.add(f)
     log(f"bench batch: {len(films)} canvases, {cells / 1e6:.1f} M cells, timing {frames} frames")
@@ -464,7 +468,7 @@ def bench_serve(args):
     return out
 
 
-FRAME_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "substeps", "lap", "mark", "compact", "compact_list", "cure", "deposit", "split", "reduce_pool",
+FRAME_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "lap_plug_w", "flux_w", "apply_w", "flux_apply_w", "substeps", "lap", "mark", "compact", "compact_list", "cure", "deposit", "split", "reduce_pool",
                                      "strata_limit", "strata_update", "strata_fill",
                                      "frame_stats", "cg_setup", "cg_lap", "cg_init", "cg_iter_a", "cg_iter_b",
                                      "cg_iter_c", "cg_reduce", "cg_finish", "cg_rescale")
@@ -539,8 +543,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device to device 3 times (a known-size dispatch that "
                          "calibrates the HBM byte counters of a profiler run)")
-    ap.add_argument("--flux-shape", type=int, default=-1, choices=(-1, 0, 1, 2),
-                    help="flux pass shape: -1 tuned online, 0 window, 1 face-parallel, 2 flux + apply fused")
+    ap.add_argument("--flux-shape", type=int, default=-1, choices=(-1, 0, 1, 2, 3, 4),
+                    help="substep shape: -1 tuned online, 0 window, 1 face-parallel, 2 flux + apply fused, "
+                         "3 and 4 as 0 and 2 with 1024 threads per tile")
     ap.add_argument("--substep-launch", default="auto", choices=("auto", "per-pass", "one"),
                     help="substeps as one launch per frame on small active sets (auto), always, or never")
     ap.add_argument("--events-in-timed", action="store_true",
@@ -667,8 +672,9 @@ def main():
         # cells x substeps the family's launches covered: every per-pass
         # substep runs lap_plug; flux, apply and flux_apply by flux shape
         shp = list(st.flux_shape_cell_substeps)
-        covered = {"lap_plug": st.act_cell_substeps - st.one_launch_cell_substeps,
-                   "flux": shp[0] + shp[1], "apply": shp[0] + shp[1], "flux_apply": shp[2]}[dom]
+        covered = {"lap_plug": shp[0] + shp[1] + shp[2], "lap_plug_w": shp[3] + shp[4],
+                   "flux": shp[0] + shp[1], "apply": shp[0] + shp[1], "flux_apply": shp[2],
+                   "flux_w": shp[3], "apply_w": shp[3], "flux_apply_w": shp[4]}[dom]
         cells_per_launch = covered / max(1, launches)
         bytes_per_launch = ALGO_BYTES[dom] * cells_per_launch
         algo_per_cell = ALGO_BYTES[dom]

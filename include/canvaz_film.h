@@ -206,9 +206,10 @@ typedef struct cz_stats {
     uint64_t one_launch_frames;
     uint64_t one_launch_cells;
     uint64_t one_launch_cell_substeps;
-    /* active cells times substeps run with each flux-pass shape (0 window,
-     * 1 face-parallel, 2 flux + apply in one launch) */
-    uint64_t flux_shape_cell_substeps[3];
+    /* active cells times substeps run with each substep shape (0 window
+     * flux, 1 face-parallel flux, 2 flux + apply in one launch, 3 and 4 as 0
+     * and 2 with 1024 threads per tile; see cz_film_flux_shape) */
+    uint64_t flux_shape_cell_substeps[5];
 } cz_stats;
 
 /* The mass report of `compute_stats` (src/main.rs:1679-1819), sums in FP64. */
@@ -492,13 +493,15 @@ int cz_film_restore_journal(cz_film* f, uint32_t target);
 
 /* Frame graphs captured and frames replayed from them so far. */
 int cz_film_graph_counts(const cz_film* f, uint64_t* captures, uint64_t* replays);
-/* Shape of the flux and limiter + apply passes: 0 = flux with one block per
- * tile over an LDS window of the tile and a 4-cell ring, 1 = flux with one
- * block per (tile, face) reading through the caches, 2 = flux and apply in
- * one launch (each block computes the fluxes of its tile and a 2-cell ring
- * itself; for small active sets), -1 = chosen online (default): per
- * active-set size the shapes are timed with HIP events on live frames (they
- * give bit-identical results) and the fastest is used. */
+/* Launch shape of a substep (lap + plug, flux, limiter + apply): 0 = flux
+ * with one block per tile over an LDS window of the tile and a 4-cell ring,
+ * 1 = flux with one block per (tile, face) reading through the caches, 2 =
+ * flux and apply in one launch (each block computes the fluxes of its tile
+ * and a 2-cell ring itself; for small active sets), 3 and 4 = shapes 0 and 2
+ * with 1024 threads per tile (every window phase in one pass, the flux one
+ * face per thread; for a few hundred active tiles), -1 = chosen online
+ * (default): per active-set size the shapes are timed with HIP events on live
+ * substeps (they give bit-identical results) and the fastest is used. */
 int cz_film_flux_shape(cz_film* f, int shape);
 /* Average per-kernel device time over the last frames (HIP events on the
  * film's stream), in microseconds; label = kernel family name. */

@@ -432,7 +432,9 @@ class Film:
         self.dx = d.value
 
     def flux_shape(self, shape):
-        """-1 = tuned online, 0 = LDS window per tile, 1 = block per (tile, face)."""
+        """Substep launch shape: -1 = tuned online, 0 = LDS window per tile,
+        1 = block per (tile, face), 2 = flux + apply fused, 3 and 4 = 0 and 2
+        with 1024 threads per tile."""
         check(lib().cz_film_flux_shape(self._p, shape))
 
     def pool(self):

@@ -182,6 +182,7 @@ struct FrameShape {
     std::vector<uint32_t> g_sub, g_flux, g_imp;   // g_flux: the flux shape of each substep
     std::vector<int> f_bucket;
     std::vector<uint8_t> f_timed;
+    std::vector<double> f_items;  // active tiles of substep s (estimate), the tuner's normaliser
     uint32_t g_all = 0, g16 = 0, g64 = 0, g_cure = 0;
     std::vector<uint32_t> key() const {
         std::vector<uint32_t> k = {members, max_sub, max_imp, iters, g_all, g16, g64, g_cure};
@@ -248,32 +249,47 @@ int record(cz_batch* b, const FrameShape& sh) {
         hipLaunchKernelGGL(k_cg_reduce<ManyCanvases>, gc, b256, 0, st, c, 3);
         hipLaunchKernelGGL(k_cg_rescale<ManyFilms>, g, b256, 0, st, w);
     }
+    const dim3 b1024(1024);
     for (uint32_t s = 0; s < sh.max_sub; s++) {
         const uint32_t v_lap = s > 0 ? ((s - 1u) & 1u) : 0u, v = s & 1u;
         const dim3 g(sh.g_sub[s]);
-        hipLaunchKernelGGL(k_lap_plug<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v_lap), s > 0 ? 1 : 0, 1,
-                           s == 0 ? 2 : 0);
+        const uint32_t shape = sh.g_flux[s];
         hipEvent_t ea = nullptr, eb = nullptr;
         if (sh.f_timed[s]) {
             ea = take_event(b);
             eb = take_event(b);
             if (ea && eb) hipEventRecord(ea, st);
         }
-        // the tuner times the flux and apply passes together (shape 2 is both
-        // in one launch)
-        if (sh.g_flux[s] == 2u) {
-            hipLaunchKernelGGL(k_flux_apply<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v));
-        } else {
-            if (sh.g_flux[s] == 1u)
+        // the substep in the shape the tuner picked (it times the whole substep)
+        const ManyFilms wl = many(b->items_sub, lim_sub + s, v_lap), w = many(b->items_sub, lim_sub + s, v);
+        if (shape >= 3u)
+            hipLaunchKernelGGL(k_lap_plug_w<ManyFilms>, g, b1024, 0, st, wl, s > 0 ? 1 : 0, 1, s == 0 ? 2 : 0);
+        else
+            hipLaunchKernelGGL(k_lap_plug<ManyFilms>, g, b256, 0, st, wl, s > 0 ? 1 : 0, 1, s == 0 ? 2 : 0);
+        switch (shape) {
+            case 1u:
                 hipLaunchKernelGGL(k_fluxes_faces<ManyFilms>, dim3(std::min<uint32_t>(4u * sh.g_sub[s], 4u * kGridCap)),
-                                   b256, 0, st, many(b->items_sub, lim_sub + s, v));
-            else
-                hipLaunchKernelGGL(k_fluxes<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v));
-            hipLaunchKernelGGL(k_limit_apply<ManyFilms>, g, b256, 0, st, many(b->items_sub, lim_sub + s, v));
+                                   b256, 0, st, w);
+                hipLaunchKernelGGL(k_limit_apply<ManyFilms>, g, b256, 0, st, w);
+                break;
+            case 2u:
+                hipLaunchKernelGGL(k_flux_apply<ManyFilms>, g, b256, 0, st, w);
+                break;
+            case 3u:
+                hipLaunchKernelGGL(k_fluxes_w<ManyFilms>, g, b1024, 0, st, w);
+                hipLaunchKernelGGL(k_limit_apply_w<ManyFilms>, g, b1024, 0, st, w);
+                break;
+            case 4u:
+                hipLaunchKernelGGL(k_flux_apply_w<ManyFilms>, g, b1024, 0, st, w);
+                break;
+            default:
+                hipLaunchKernelGGL(k_fluxes<ManyFilms>, g, b256, 0, st, w);
+                hipLaunchKernelGGL(k_limit_apply<ManyFilms>, g, b256, 0, st, w);
+                break;
         }
         if (ea && eb) {
             hipEventRecord(eb, st);
-            b->tuner_pending.push_back({&b->flux_tuner, sh.f_bucket[s], (int)sh.g_flux[s], ea, eb});
+            b->tuner_pending.push_back({&b->flux_tuner, sh.f_bucket[s], (int)shape, ea, eb, sh.f_items[s]});
         }
     }
     const uint32_t* all = lim_sub + sh.max_sub;
@@ -330,7 +346,7 @@ int cz_batch_create(uint32_t max_members, cz_batch** out) {
     *out = nullptr;
     cz_batch* b = new cz_batch();
     b->max_members = max_members;
-    b->flux_tuner.ns = 3;  // window, face-parallel, flux + apply in one launch
+    b->flux_tuner.ns = czi::kSubstepShapes;
     int rc = CZ_OK;
     hipError_t e = hipGetDevice(&b->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -411,7 +427,7 @@ int cz_batch_use_graphs(cz_batch* b, int on) {
 }
 
 int cz_batch_flux_shape(cz_batch* b, int shape) {
-    if (!b || shape < -1 || shape > 2) return set_err(CZ_EINVAL, "flux_shape: -1, 0, 1 or 2");
+    if (!b || shape < -1 || shape >= czi::kSubstepShapes) return set_err(CZ_EINVAL, "flux_shape: -1 (tuned) or 0-4");
     b->flux_tuner.fixed = shape;
     return CZ_OK;
 }
@@ -480,6 +496,7 @@ int cz_batch_step(cz_batch* b, float frame_dt) {
         sh.g_flux.push_back((uint32_t)shape);
         sh.f_bucket.push_back(bk);
         sh.f_timed.push_back(timed ? 1 : 0);
+        sh.f_items.push_back((double)est_sub[s]);
     }
     for (uint32_t i = 0; i < sh.max_imp; i++) {
         uint64_t est = 0, tot = 0;

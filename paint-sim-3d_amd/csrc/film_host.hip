@@ -92,7 +92,7 @@ namespace czi {
 void fold_tuners(std::vector<TunerSlot>& pending, std::vector<hipEvent_t>& pool) {
     for (auto& s : pending) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) s.t->record(s.bucket, s.shape, (double)ms * 1000.0);
+        if (hipEventElapsedTime(&ms, s.a, s.b) == hipSuccess) s.t->record(s.bucket, s.shape, (double)ms * 1000.0, s.items);
         pool.push_back(s.a);
         pool.push_back(s.b);
     }
@@ -117,7 +117,7 @@ void fold_frame_out(cz_film* f, const float* o) {
     f->cnt_act_cells += cells;
     f->cnt_act_cell_substeps += cells * (uint64_t)f->substeps_last;
     f->cnt_act_cell_implicit += cells * (uint64_t)f->implicit_last;
-    for (int k = 0; k < 3; k++) f->cnt_shape_cell_substeps[k] += cells * (uint64_t)f->shape_subs_last[k];
+    for (int k = 0; k < czi::kSubstepShapes; k++) f->cnt_shape_cell_substeps[k] += cells * (uint64_t)f->shape_subs_last[k];
     if (f->one_launch_last) {
         f->cnt_one_frames += 1;
         f->cnt_one_cells += cells;
@@ -157,7 +157,7 @@ void frame_done(cz_film* f, uint32_t substeps, bool parity_done) {
     f->substeps_last = substeps;
     f->implicit_last = f->s.implicit_hydro ? f->implicit_steps : 0u;
     f->one_launch_last = f->one_launch_now;
-    for (int k = 0; k < 3; k++) f->shape_subs_last[k] = f->shape_subs_now[k];
+    for (int k = 0; k < czi::kSubstepShapes; k++) f->shape_subs_last[k] = f->shape_subs_now[k];
     f->frame++;
     f->deposit_count = 0;
     f->contact_count = 0;  // the contacts are consumed by the frame
@@ -334,12 +334,11 @@ int record_frame(cz_film* f, uint32_t substeps) {
         // one flip per nose (substeps 1.. and the last one)
         if (substeps & 1u) f->delta_cur_a = !f->delta_cur_a;
     }
-    // substeps: [nose of the previous substep +] lap + plug, flux, limiter + apply
+    // substeps: [nose of the previous substep +] lap + plug, flux, limiter + apply,
+    // in the launch shape the substep tuner picks for the active-set size (it
+    // times the whole substep)
+    const dim3 b1024(1024);
     for (uint32_t s = 0; s < (pg > 0u ? 0u : substeps); s++) {
-        LAUNCH("lap_plug", k_lap_plug<OneFilm>, gt, b256, OneFilm{f->dev()}, s > 0 ? 1 : 0, 1,
-               (s == 0 && !implicit) ? 1 : 0);
-        if (s > 0) f->delta_cur_a = !f->delta_cur_a;
-        const OneFilm w{f->dev()};
         int bk = 0;
         bool timed = false;
         const int shape = f->flux_tuner.choose(std::min(f->active_hint, f->nt16), !f->capturing, bk, timed);
@@ -349,20 +348,37 @@ int record_frame(cz_film* f, uint32_t substeps) {
             eb = f->take_event();
             if (ea && eb) hipEventRecord(ea, f->stream);
         }
-        // the tuner times the flux and apply passes together (shape 2 is both
-        // in one launch)
-        if (shape == 2) {
-            LAUNCH("flux_apply", k_flux_apply<OneFilm>, gt, b256, w);
-        } else {
-            if (shape == 1)
+        const OneFilm wl{f->dev()};
+        const int nose = s > 0 ? 1 : 0, clear = (s == 0 && !implicit) ? 1 : 0;
+        if (shape >= 3)
+            LAUNCH("lap_plug_w", k_lap_plug_w<OneFilm>, gt, b1024, wl, nose, 1, clear);
+        else
+            LAUNCH("lap_plug", k_lap_plug<OneFilm>, gt, b256, wl, nose, 1, clear);
+        if (s > 0) f->delta_cur_a = !f->delta_cur_a;
+        const OneFilm w{f->dev()};
+        switch (shape) {
+            case 1:
                 LAUNCH("flux", k_fluxes_faces<OneFilm>, dim3(f->tile_grid(4)), b256, w);
-            else
+                LAUNCH("apply", k_limit_apply<OneFilm>, gt, b256, w);
+                break;
+            case 2:
+                LAUNCH("flux_apply", k_flux_apply<OneFilm>, gt, b256, w);
+                break;
+            case 3:
+                LAUNCH("flux_w", k_fluxes_w<OneFilm>, gt, b1024, w);
+                LAUNCH("apply_w", k_limit_apply_w<OneFilm>, gt, b1024, w);
+                break;
+            case 4:
+                LAUNCH("flux_apply_w", k_flux_apply_w<OneFilm>, gt, b1024, w);
+                break;
+            default:
                 LAUNCH("flux", k_fluxes<OneFilm>, gt, b256, w);
-            LAUNCH("apply", k_limit_apply<OneFilm>, gt, b256, w);
+                LAUNCH("apply", k_limit_apply<OneFilm>, gt, b256, w);
+                break;
         }
         if (ea && eb) {
             hipEventRecord(eb, f->stream);
-            f->tuner_pending.push_back({&f->flux_tuner, bk, shape, ea, eb});
+            f->tuner_pending.push_back({&f->flux_tuner, bk, shape, ea, eb, (double)std::min(f->active_hint, f->nt16)});
         }
         f->shape_subs_now[shape]++;
     }
@@ -624,7 +640,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     if ((w + 15u) / 16u > 21845u) return set_err(CZ_EINVAL, "cz_film_create: width %u exceeds 349520 cells", w);
     *out = nullptr;
     cz_film* f = new cz_film();
-    f->flux_tuner.ns = 3;  // window, face-parallel, flux + apply in one launch
+    f->flux_tuner.ns = czi::kSubstepShapes;
     if (s) f->s = *s; else cz_settings_default(&f->s);
     if (f->s.slump) {
         destroy(f);
@@ -850,7 +866,7 @@ int cz_film_stats(cz_film* f, cz_stats* st) {
     st->one_launch_frames = f->cnt_one_frames;
     st->one_launch_cells = f->cnt_one_cells;
     st->one_launch_cell_substeps = f->cnt_one_cell_substeps;
-    for (int k = 0; k < 3; k++) st->flux_shape_cell_substeps[k] = f->cnt_shape_cell_substeps[k];
+    for (int k = 0; k < czi::kSubstepShapes; k++) st->flux_shape_cell_substeps[k] = f->cnt_shape_cell_substeps[k];
     return CZ_OK;
 }
 
@@ -1026,7 +1042,7 @@ int cz_film_substep_launch(cz_film* f, int mode) {
 }
 
 int cz_film_flux_shape(cz_film* f, int shape) {
-    if (!f || shape < -1 || shape > 2) return set_err(CZ_EINVAL, "flux_shape: -1, 0, 1 or 2");
+    if (!f || shape < -1 || shape >= czi::kSubstepShapes) return set_err(CZ_EINVAL, "flux_shape: -1 (tuned) or 0-4");
     f->flux_tuner.fixed = shape;
     return CZ_OK;
 }

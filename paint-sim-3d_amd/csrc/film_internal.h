@@ -48,6 +48,17 @@ constexpr uint32_t kMaxTileBlocks = 2048u;  // resident 256-thread blocks on 256
 constexpr uint32_t kPersistTiles = 0u, kPersistBlocks = 256u;
 // deposit binning on the device from this many deposits in a frame
 constexpr uint32_t kDevBinMin = 1024u;
+// Launch shapes of a substep, picked online per active-set size by the
+// substep tuner (all give the same bits):
+//   0 lap + plug, staged flux (window per tile), limiter + apply;
+//   1 lap + plug, face-parallel flux (a block per tile and face), limiter + apply;
+//   2 lap + plug, flux + limiter + apply in one launch;
+//   3 as 0 with 1024 threads per tile (flux one face per thread);
+//   4 as 2 with 1024 threads per tile.
+// 256-thread blocks hold 4 waves per tile; a few hundred active tiles leave
+// the GPU at 1-2 waves per SIMD, where the wide shapes take each window phase
+// in one pass with 16 waves per tile.
+constexpr int kSubstepShapes = 5;
 
 struct ProfSlot {
     std::string label;
@@ -57,13 +68,18 @@ struct ProfSlot {
 // Online choice between launch shapes of one kernel family whose results are
 // bit-identical (so exploring costs nothing but time): per size bucket
 // (log2 of the work items) each shape is timed with HIP events until it has
-// WARM samples, then the faster one is used, re-timed once every RECHECK
-// uses.  fixed >= 0 pins a shape.
+// WARM samples, then the one with the least time per work item is used and
+// one of the others re-timed once every RECHECK uses.  A sample's time is
+// known only once its frame has run, so exploration counts the samples
+// issued, not the ones folded: the first frame of a bucket times every shape
+// in turn (WARM substeps each), and the frame after it uses the fastest.
+// fixed >= 0 pins a shape.
 struct ShapeTuner {
-    static constexpr int NB = 24, NS = 3, WARM = 3, RECHECK = 64;
+    static constexpr int NB = 24, NS = 5, WARM = 3, RECHECK = 64;
     int ns = 2;  // shapes in use (at most NS)
     double sum[NB][NS] = {};
     uint32_t cnt[NB][NS] = {};
+    uint32_t issued[NB][NS] = {};
     uint32_t uses[NB] = {};
     int fixed = -1;
     static int bucket(uint64_t items) {
@@ -78,11 +94,15 @@ struct ShapeTuner {
         timed = false;
         if (fixed >= 0) return fixed;
         if (!can_time) return best_of(bk);
+        // explore: the shape with the fewest samples issued, round robin
+        int least = -1;
         for (int k = 0; k < ns; k++)
-            if (cnt[bk][k] < (uint32_t)WARM) {
-                timed = true;
-                return k;
-            }
+            if (issued[bk][k] < (uint32_t)WARM && (least < 0 || issued[bk][k] < issued[bk][least])) least = k;
+        if (least >= 0) {
+            issued[bk][least]++;
+            timed = true;
+            return least;
+        }
         const int best = best_of(bk);
         if (++uses[bk] % RECHECK == 0) {
             // re-time one of the others, in turn
@@ -91,7 +111,8 @@ struct ShapeTuner {
         }
         return best;
     }
-    // the fastest shape with samples (shape 0 if none has any)
+    // the shape with the least time per item among those with samples
+    // (shape 0 if none has any)
     int best_of(int bk) const {
         int best = -1;
         double bt = 0.0;
@@ -114,8 +135,9 @@ struct ShapeTuner {
             if (cnt[b][k] < (uint32_t)WARM) return true;
         return false;
     }
-    void record(int bk, int shape, double us) {
-        sum[bk][shape] += us;
+    // one timed launch: us of device time over `items` work items
+    void record(int bk, int shape, double us, double items) {
+        sum[bk][shape] += us / (items > 1.0 ? items : 1.0);
         cnt[bk][shape] += 1;
     }
 };
@@ -124,6 +146,7 @@ struct TunerSlot {
     ShapeTuner* t;
     int bucket, shape;
     hipEvent_t a, b;
+    double items;  // work items of the timed launch (active tiles)
 };
 
 }  // namespace czi
@@ -163,8 +186,8 @@ struct cz_film {
     uint64_t cnt_act_cells = 0, cnt_act_cell_substeps = 0, cnt_frames = 0, cnt_act_cell_implicit = 0;
     uint64_t cnt_one_frames = 0, cnt_one_cells = 0, cnt_one_cell_substeps = 0;
     bool one_launch_last = false;  // the frame being folded ran its substeps as one launch
-    uint64_t cnt_shape_cell_substeps[3] = {0, 0, 0};
-    uint32_t shape_subs_now[3] = {0, 0, 0}, shape_subs_last[3] = {0, 0, 0};  // substeps per flux shape
+    uint64_t cnt_shape_cell_substeps[czi::kSubstepShapes] = {};
+    uint32_t shape_subs_now[czi::kSubstepShapes] = {}, shape_subs_last[czi::kSubstepShapes] = {};  // substeps per shape
     int device = 0;
     hipStream_t stream = nullptr;
     void* batch = nullptr;  // the cz_batch this film is a member of

@@ -802,7 +802,10 @@ struct LapShared {
     float s_us[RING_SPAN * RING_SPAN];
 };
 
-// One tile of the lap/plug pass (every thread of the block calls it).
+// One tile of the lap/plug pass (every thread of the block calls it).  BT:
+// threads of the block (256, or 1024 for the wide shape, whose window
+// phases then take one pass; the per-cell tail runs on threads 0-255).
+template <int BT = 256>
 __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, LapShared& S, int do_nose, int do_plug,
                                               int do_clear) {
     LapWin& sw = S.sw;
@@ -810,8 +813,9 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
     uint32_t* s_d = S.s_d;
     float2* s_fn = S.s_fn;
     float* s_us = S.s_us;
-    const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;
+    const uint32_t _lx = threadIdx.x & 15u, _ly = (threadIdx.x >> 4) & 15u;
     CZ_TILE_CELL(F)
+    const bool cell_th = BT == 256 || threadIdx.x < 256u;
     const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
     const bool plug_on = do_plug && !(gb < 1e-6f);
     const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
@@ -819,7 +823,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
     const int wx0 = tx0 - WIN_R, wy0 = ty0 - WIN_R;
     const int W_ = (int)F.W, H_ = (int)F.H;
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += 256u) {
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += (uint32_t)BT) {
         const int gx = wx0 + (int)(q % WIN), gy = wy0 + (int)(q / WIN);
         if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
             const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
@@ -905,7 +909,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
             fn_out = (l < 1e-6f) ? make_float2(0.0f, 0.0f) : make_float2(dx / l, dy / l);
         }
     };
-    for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += (uint32_t)BT) {
         const int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
         uint32_t dv = 0u;
         float2 fn = make_float2(0.0f, 0.0f);
@@ -931,7 +935,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
         s_us[q] = us;
     }
     if (plug_on) {
-        for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += 256u) {
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += (uint32_t)BT) {
             float hn = sw.h[q], qv = 0.0f;
             bool yl = false;
             if (hn > 0.0f) {
@@ -946,7 +950,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
         }
     }
     __syncthreads();
-    if (!in_grid) return;
+    if (!in_grid || !cell_th) return;
     // clear_acc (film.wgsl:1427-1434) after the cure pass, with the first
     // active-tile kernel of the frame: do_clear 1 = clear, 2 = clear when
     // the canvas has no implicit step this frame (its implicit lap cleared)
@@ -1022,6 +1026,13 @@ template <class WS>
 __global__ __launch_bounds__(256) void k_lap_plug(WS ws, int do_nose, int do_plug, int do_clear) {
     __shared__ LapShared S;
     CZ_FOR_TILES(ws) lap_plug_tile(F, _t, S, do_nose, do_plug, do_clear);
+}
+
+// The wide shape: 1024 threads per tile.
+template <class WS>
+__global__ __launch_bounds__(1024) void k_lap_plug_w(WS ws, int do_nose, int do_plug, int do_clear) {
+    __shared__ LapShared S;
+    CZ_FOR_TILES(ws) lap_plug_tile<1024>(F, _t, S, do_nose, do_plug, do_clear);
 }
 
 // The same pass without plug: the nose of a frame's last substep and the
@@ -1133,7 +1144,7 @@ using WinRead = WinReadT<FLUX_R, false>;
 
 // Stage the window of tile (ttx, tty) with origin (wx0, wy0).  The 3x3 tile
 // flags are loaded first (they pick the heights' source when SNAP).
-template <int R, bool SNAP>
+template <int R, bool SNAP, int BT = 256>
 __device__ __forceinline__ void flux_stage(const FilmDev& F, FluxWinT<R>& sw, int ttx, int tty, int wx0, int wy0) {
     constexpr int N = FluxWinT<R>::N;
     const int W_ = (int)F.W, H_ = (int)F.H;
@@ -1144,7 +1155,7 @@ __device__ __forceinline__ void flux_stage(const FilmDev& F, FluxWinT<R>& sw, in
                                  ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
     }
     if (SNAP) __syncthreads();
-    for (uint32_t q = threadIdx.x; q < (uint32_t)(N * N); q += 256u) {
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(N * N); q += (uint32_t)BT) {
         const int gx = wx0 + (int)(q % N), gy = wy0 + (int)(q / N);
         if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
             const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
@@ -1205,6 +1216,34 @@ __global__ __launch_bounds__(256) void k_fluxes(WS ws) {
     CZ_FOR_TILES(ws) flux_tile(F, _t, sw);
 }
 
+// The wide shape of the staged flux pass: 1024 threads per tile, one face of
+// one cell per thread (cell = thread % 256, face = thread / 256), each face
+// evaluated exactly as cell_fluxes evaluates it.
+__device__ __forceinline__ void flux_tile_w(const FilmDev& F, uint32_t _t, FluxWin& sw) {
+    const uint32_t k = threadIdx.x >> 8;
+    const uint32_t _lx = threadIdx.x & 15u, _ly = (threadIdx.x >> 4) & 15u;
+    CZ_TILE_CELL(F)
+    const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+    const int wx0 = ttx * 16 - FLUX_R, wy0 = tty * 16 - FLUX_R;
+    flux_stage<FLUX_R, false, 1024>(F, sw, ttx, tty, wx0, wy0);
+    if (!in_grid) return;
+    const WinRead g{F, sw, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
+    const float2 p00 = pressure_at(F, g, (int)x, (int)y);
+    float q = 0.0f;
+    FaceGeom fg = face_geom(F, g, (int)x, (int)y, (int)k, true, p00);
+    if (fg.has) {
+        FaceFlux ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+        q = ff.q * ((k < 2u) ? W_AXIS : W_DIAG_FLUX);
+    }
+    ((float*)F.flux)[4u * c + k] = q;
+}
+
+template <class WS>
+__global__ __launch_bounds__(1024) void k_fluxes_w(WS ws) {
+    __shared__ FluxWin sw;
+    CZ_FOR_TILES(ws) flux_tile_w(F, _t, sw);
+}
+
 struct Inflow { float best; uint32_t src; float div; float accv; };
 
 // Copy of the top stratum (and rheology) of src into the newly wetted cell c
@@ -1248,13 +1287,15 @@ struct ApplyShared {
 
 // The limiter + apply of a tile from the staged fluxes and heights of the tile
 // and a 2-cell ring (S.s_f, S.s_hh) and the 3x3 tile flags (S.s_tf).
+template <int BT = 256>
 __device__ __forceinline__ void apply_core(const FilmDev& F, uint32_t _t, ApplyShared& S) {
     float4* s_f = S.s_f;
     float* s_hh = S.s_hh;
     float* s_a = S.s_a;
     uint32_t* s_tf = S.s_tf;
-    const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;
+    const uint32_t _lx = threadIdx.x & 15u, _ly = (threadIdx.x >> 4) & 15u;
     CZ_TILE_CELL(F)
+    const bool cell_th = BT == 256 || threadIdx.x < 256u;
     const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
     const int tx0 = ttx * 16, ty0 = tty * 16;
     const int wx0 = tx0 - AP_R, wy0 = ty0 - AP_R;
@@ -1277,13 +1318,13 @@ __device__ __forceinline__ void apply_core(const FilmDev& F, uint32_t _t, ApplyS
         if (out > 0.0f) a = cz_min(1.0f, 0.5f * s_hh[wq(gi, gj)] * F.P.dx / (F.P.dt * out));
         return a;
     };
-    for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += 256u) {
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += (uint32_t)BT) {
         const int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
         s_a[q] = nb_ok(gx, gy) ? limiter(gx, gy) : 0.0f;
     }
     __syncthreads();
     bool moved = false;
-    if (in_grid) {
+    if (in_grid && cell_th) {
         const int i = (int)x, j = (int)y;
         auto alpha_at = [&](int gi, int gj) -> float { return s_a[(gj - ty0 + 1) * RING_SPAN + (gi - tx0 + 1)]; };
         const float ac = alpha_at(i, j);
@@ -1336,12 +1377,13 @@ __device__ __forceinline__ void apply_core(const FilmDev& F, uint32_t _t, ApplyS
 }
 
 // One tile of the limiter + apply pass (every thread of the block calls it).
+template <int BT = 256>
 __device__ __forceinline__ void apply_tile(const FilmDev& F, uint32_t _t, ApplyShared& S) {
     const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
     const int wx0 = ttx * 16 - AP_R, wy0 = tty * 16 - AP_R;
     const int W_ = (int)F.W, H_ = (int)F.H;
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += 256u) {
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += (uint32_t)BT) {
         const int gx = wx0 + (int)(q % AP_WIN), gy = wy0 + (int)(q / AP_WIN);
         if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
             const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
@@ -1355,7 +1397,7 @@ __device__ __forceinline__ void apply_tile(const FilmDev& F, uint32_t _t, ApplyS
                                   ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
     }
     __syncthreads();
-    apply_core(F, _t, S);
+    apply_core<BT>(F, _t, S);
 }
 
 // fluxes + limiter + apply of a tile in one launch, for small active sets
@@ -1375,28 +1417,51 @@ struct FluxApplyShared {
     ApplyShared ap;
 };
 
+template <int BT = 256>
 __device__ __forceinline__ void flux_apply_tile(const FilmDev& F, uint32_t _t, FluxApplyShared& S) {
-    const uint32_t _lx = threadIdx.x & 15u, _ly = threadIdx.x >> 4;
+    const uint32_t _lx = threadIdx.x & 15u, _ly = (threadIdx.x >> 4) & 15u;
     CZ_TILE_CELL(F)
+    const bool cell_th = BT == 256 || threadIdx.x < 256u;
     const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
     const int wx0 = ttx * 16 - FA_R, wy0 = tty * 16 - FA_R;
     const int W_ = (int)F.W, H_ = (int)F.H;
-    flux_stage<FA_R, true>(F, S.w, ttx, tty, wx0, wy0);
+    flux_stage<FA_R, true, BT>(F, S.w, ttx, tty, wx0, wy0);
     if (threadIdx.x < 9u) S.ap.s_tf[threadIdx.x] = S.w.tf[threadIdx.x];
     const WinReadT<FA_R, true> g{F, S.w, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
     // the fluxes of the active cells of the tile + 2 ring into the apply's window
-    for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += 256u) {
-        const int gx = ttx * 16 - AP_R + (int)(q % AP_WIN), gy = tty * 16 - AP_R + (int)(q / AP_WIN);
-        if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
-            S.ap.s_hh[q] = S.w.h[g.q(gx, gy)];
-            float4 qv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (g.active(gx, gy)) qv = cell_fluxes(F, g, gx, gy);
-            S.ap.s_f[q] = qv;
+    if (BT == 256) {
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += 256u) {
+            const int gx = ttx * 16 - AP_R + (int)(q % AP_WIN), gy = tty * 16 - AP_R + (int)(q / AP_WIN);
+            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+                S.ap.s_hh[q] = S.w.h[g.q(gx, gy)];
+                float4 qv = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (g.active(gx, gy)) qv = cell_fluxes(F, g, gx, gy);
+                S.ap.s_f[q] = qv;
+            }
+        }
+    } else {
+        // one face of one window cell per item, as cell_fluxes evaluates it
+        for (uint32_t it = threadIdx.x; it < (uint32_t)(AP_WIN * AP_WIN * 4); it += (uint32_t)BT) {
+            const uint32_t q = it >> 2, k = it & 3u;
+            const int gx = ttx * 16 - AP_R + (int)(q % AP_WIN), gy = tty * 16 - AP_R + (int)(q / AP_WIN);
+            if (gx >= 0 && gy >= 0 && gx < W_ && gy < H_) {
+                if (k == 0u) S.ap.s_hh[q] = S.w.h[g.q(gx, gy)];
+                float qk = 0.0f;
+                if (g.active(gx, gy)) {
+                    const float2 p00 = pressure_at(F, g, gx, gy);
+                    FaceGeom fg = face_geom(F, g, gx, gy, (int)k, true, p00);
+                    if (fg.has) {
+                        FaceFlux ff = face(F, g, gx, gy, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+                        qk = ff.q * ((k < 2u) ? W_AXIS : W_DIAG_FLUX);
+                    }
+                }
+                ((float*)&S.ap.s_f[q])[k] = qk;
+            }
         }
     }
     __syncthreads();
-    if (in_grid) F.flux[c] = S.ap.s_f[(int)(_ly + AP_R) * AP_WIN + (int)(_lx + AP_R)];
-    apply_core(F, _t, S.ap);
+    if (in_grid && cell_th) F.flux[c] = S.ap.s_f[(int)(_ly + AP_R) * AP_WIN + (int)(_lx + AP_R)];
+    apply_core<BT>(F, _t, S.ap);
 }
 
 template <class WS>
@@ -1406,9 +1471,21 @@ __global__ __launch_bounds__(256) void k_flux_apply(WS ws) {
 }
 
 template <class WS>
+__global__ __launch_bounds__(1024) void k_flux_apply_w(WS ws) {
+    __shared__ FluxApplyShared S;
+    CZ_FOR_TILES(ws) flux_apply_tile<1024>(F, _t, S);
+}
+
+template <class WS>
 __global__ __launch_bounds__(256) void k_limit_apply(WS ws) {
     __shared__ ApplyShared S;
     CZ_FOR_TILES(ws) apply_tile(F, _t, S);
+}
+
+template <class WS>
+__global__ __launch_bounds__(1024) void k_limit_apply_w(WS ws) {
+    __shared__ ApplyShared S;
+    CZ_FOR_TILES(ws) apply_tile<1024>(F, _t, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -2399,6 +2476,10 @@ CZ_INST1(k_reduce_pool, ManyFilms)
 #define CZ_INST2(name, ...) CZ_INST1(name, OneFilm, ##__VA_ARGS__) CZ_INST1(name, ManyFilms, ##__VA_ARGS__)
 CZ_INST2(k_lap_plug, int, int, int)
 CZ_INST2(k_lap_tail, int, int)
+CZ_INST2(k_lap_plug_w, int, int, int)
+CZ_INST2(k_fluxes_w)
+CZ_INST2(k_limit_apply_w)
+CZ_INST2(k_flux_apply_w)
 CZ_INST1(k_mark, OneFilmAll)
 CZ_INST1(k_shade, OneFilmAll)
 CZ_INST1(k_mark, ManyFilms)

@@ -128,3 +128,23 @@ def test_member_calls_are_refused(gpu):
     f.step()
     f.sync()
     assert f.stats().frame == 2
+
+
+@pytest.mark.parametrize("shape", [3, 4])
+def test_batch_wide_shapes_equal_alone(gpu, shape):
+    """The 1024-thread substep shapes in a batch: members equal to the same
+    films stepped alone in the default shapes."""
+    alone = [make(k) for k in (1, 2, 4)]
+    inb = [make(k) for k in (1, 2, 4)]
+    b = czf.Batch(4)
+    b.flux_shape(shape)
+    for f in inb:
+        b.add(f)
+    for frame in range(8):
+        b.step()
+        for f in alone:
+            f.step()
+    b.sync()
+    for k in range(3):
+        alone[k].sync()
+        assert_same(alone[k], inb[k], f"shape {shape} member {k}")

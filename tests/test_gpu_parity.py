@@ -182,3 +182,19 @@ def test_cure_commit(gpu):
     assert rel_err(mg, mo, 1e-9) <= 1e-5
     assert np.array_equal(g.buffer("sub_refl"), o.buf("sub_refl"))
     assert (mo > st["m"]).any(), "no stratum was committed"
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
+def test_every_substep_shape(gpu, shape):
+    """Each launch shape of the substep pinned (cz_film_flux_shape): 256- and
+    1024-thread tiles, staged, face-parallel and fused flux give the
+    restatement's results."""
+    w = h = 112
+    s = blob_settings_n(w, 35.0, 0.7)
+    st = scenes.blobs_state(w, h, seed=14, nblobs=12, sigma_cells=(3.0, 12.0), peak=(2e-4, 2e-3),
+                            tau=(0.01, 0.3), mu=(0.5, 5.0))
+    g, o = make_pair(w, h, s, st)
+    g.flux_shape(shape)
+    run_pair(g, o, 12, check_every=4, tol=1e-5, tag=f"shape {shape}")
+    used = list(g.stats().flux_shape_cell_substeps)
+    assert used[shape] > 0 and sum(used) == used[shape], used
