@@ -2,9 +2,11 @@
 
 Workload: the plain single-canvas film, 2048x2048 cells, FP32, initial height
 a sum of 64 seeded Gaussian blobs, per-cell yield stress and viscosity,
-Dirichlet(0.3) pigments, 15 deg tilt (paint-sim-3d_amd/scenes.py), at the
-reference's uniform power-law index n = 1; a second record at the
-shear-thinning n = 0.7.  A step is one frame of the reference's
+Dirichlet(0.3) pigments, 15 deg tilt (paint-sim-3d_amd/scenes.py), and a
+power-law index per wet cell uniform in [0.4, 1] (settings.n_per_cell; the
+batched and served canvases are seeded the same way); records beside it at
+one film index, the reference's n = 1 and the shear-thinning n = 0.7
+(--index 1 or 0.7 puts every workload on a uniform index).  A step is one frame of the reference's
 encode_sim/encode_frame (src/main.rs:1875-1909, src/film/mod.rs:1322-1462) at
 frame_dt = 1/60 s with the substep count chosen from the previous frame's
 stats; the display shade pass (shade_full, src/film/mod.rs:1449-1456) is off,
@@ -53,6 +55,37 @@ ALGO_BYTES["flux_apply"] = (4 + 8 + 8 + 4 + 4 + 8) + (4 + 16) + (16 + 4 + 16)
 # the 1024-thread shapes of the same passes move the same bytes
 for _fam in ("lap_plug", "flux", "apply", "flux_apply"):
     ALGO_BYTES[_fam + "_w"] = ALGO_BYTES[_fam]
+# with an index per cell: the plug window and the face pass read it
+CELL_N_BYTES = {"lap_plug": 4, "flux": 4, "flux_apply": 4, "lap_plug_w": 4, "flux_w": 4, "flux_apply_w": 4}
+
+
+def algo_bytes(fam, cell_n):
+    return ALGO_BYTES[fam] + (CELL_N_BYTES.get(fam, 0) if cell_n else 0)
+
+
+# the canvases of every workload: an index per wet cell, uniform in [0.4, 1]
+# (slot 15 of its stratum), or the reference's single film index n = 1
+INDEX_RANGE = (0.4, 1.0)
+
+
+def canvas_settings(w, tilt_deg, cell_n):
+    import scenes
+    s = scenes.blobs_settings(w, tilt_deg=tilt_deg)
+    s.n_per_cell = 1 if cell_n else 0
+    return s
+
+
+def index_words(index):
+    if index == "cell":
+        return f"power-law index per wet cell U[{INDEX_RANGE[0]}, {INDEX_RANGE[1]}]"
+    return f"uniform power-law index n = {index}"
+
+
+def canvas_scene(seed, cell_n):
+    import canvaz_film as czf
+    if cell_n:
+        return czf.scene(seed=seed, n_lo=INDEX_RANGE[0], n_hi=INDEX_RANGE[1])
+    return czf.scene(seed=seed)
 # families the roofline is reported for: the per-pass substep kernels in both
 # block sizes and the one-launch substeps kernel (k_substeps)
 ROOFLINE_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "lap_plug_w", "flux_w", "apply_w", "flux_apply_w", "substeps")
@@ -369,10 +402,11 @@ def bench_batch(args, plain_value):
     specs = batch_canvases(args.batch_canvases)
     frames = args.batch_frames
     cells = sum(w * w for (w, _, _) in specs)
+    cell_n = args.index == "cell"
     films = []
     for (w, tilt, sd) in specs:
-        f = czf.Film(w, w, scenes.blobs_settings(w, tilt_deg=tilt))
-        f.seed_blobs(czf.scene(seed=sd))
+        f = czf.Film(w, w, canvas_settings(w, tilt, cell_n))
+        f.seed_blobs(canvas_scene(sd, cell_n))
         films.append(f)
     b = czf.Batch(max(len(films), 1))
     b.use_graphs(not args.no_graphs)
@@ -393,8 +427,8 @@ def bench_batch(args, plain_value):
         f"plain comparison")
     t_plain, worst = 0.0, 0.0
     for k, (w, tilt, sd) in enumerate(specs):
-        g = czf.Film(w, w, scenes.blobs_settings(w, tilt_deg=tilt))
-        g.seed_blobs(czf.scene(seed=sd))
+        g = czf.Film(w, w, canvas_settings(w, tilt, cell_n))
+        g.seed_blobs(canvas_scene(sd, cell_n))
         g.sync()
         t0 = time.perf_counter()
         for _ in range(frames):
@@ -415,14 +449,15 @@ def bench_batch(args, plain_value):
             "max_rel_err_vs_plain": worst, "active_cells_per_frame": act,
             "mean_substeps": float(np.mean(subs)), "graphs": not args.no_graphs,
             "workload": f"{len(specs)} square canvases, side log-uniform [256, 2048], seeded blob canvases "
-                        f"(64 blobs, device generator), tilt U[0, 60] deg, {frames} frames from the initial states"}
+                        f"(64 blobs, device generator, {index_words(args.index)}), tilt U[0, 60] deg, "
+                        f"{frames} frames from the initial states"}
 
 
-def serve_trace(seconds, rate=200.0, seed=5):
+def serve_trace(seconds, rate=200.0, seed=5, cell_n=True):
     """Poisson arrivals at `rate` per second for `seconds`; canvas side
     log-uniform in [512, 4096]; steps Pareto(1.5, min 10) truncated at 2000;
     each canvas a seeded blob scene with the plain workload's cell size and
-    a 15 deg tilt."""
+    a 15 deg tilt (with an index per cell when cell_n)."""
     import numpy as np
     import canvaz_film as czf
     import scenes
@@ -434,7 +469,7 @@ def serve_trace(seconds, rate=200.0, seed=5):
             break
         w = int(round(float(np.exp(rng.uniform(np.log(512.0), np.log(4096.0))))))
         steps = int(min(2000, np.floor(10.0 / rng.uniform(0.0, 1.0) ** (1.0 / 1.5))))
-        reqs.append(czf.request(k, w, w, steps, t, scenes.blobs_settings(w, tilt_deg=15.0), czf.scene(seed=7000 + k)))
+        reqs.append(czf.request(k, w, w, steps, t, canvas_settings(w, 15.0, cell_n), canvas_scene(7000 + k, cell_n)))
         k += 1
     return reqs
 
@@ -443,7 +478,7 @@ def bench_serve(args):
     """The serving loop on a Poisson trace: latency = completion - arrival."""
     import numpy as np
     import canvaz_film as czf
-    reqs = serve_trace(args.serve_seconds)
+    reqs = serve_trace(args.serve_seconds, cell_n=args.index == "cell")
     offered = sum(r.w * r.h * r.steps for r in reqs) / args.serve_seconds
     log(f"bench serve: {len(reqs)} requests over {args.serve_seconds} s, offered {offered:.3e} cell-steps/s")
     srv = czf.Server(hbm_budget=args.serve_budget_gb << 30, max_members=args.serve_members)
@@ -461,7 +496,8 @@ def bench_serve(args):
            "offered_cell_steps_per_s": offered, "frames": st.frames, "peak_members": st.peak_members,
            "peak_gb": st.peak_bytes / 2 ** 30, "films_created": st.films_created, "films_reused": st.films_reused,
            "workload": f"Poisson 200 req/s for {args.serve_seconds} s, side log-uniform [512, 4096], steps "
-                       f"Pareto(1.5, min 10, max 2000), seeded blob canvases, budget {args.serve_budget_gb} GB, "
+                       f"Pareto(1.5, min 10, max 2000), seeded blob canvases ({index_words(args.index)}), "
+                       f"budget {args.serve_budget_gb} GB, "
                        f"at most {args.serve_members} canvases per batch"}
     log(f"bench serve: p50 {out['p50_latency_s']:.3f} s, p99 {out['p99_latency_s']:.3f} s, "
         f"{out['throughput_cell_steps_per_s']:.3e} cell-steps/s over {st.wall_s:.1f} s")
@@ -474,11 +510,14 @@ FRAME_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "lap_plug_w", "flux_w", "appl
                                      "cg_iter_c", "cg_reduce", "cg_finish", "cg_rescale")
 
 
-def frame_algo_bytes(n_grid, st):
+def frame_algo_bytes(n_grid, st, cell_n=False):
     """Compulsory bytes of the frames folded in `st`:
-    N_grid*20 + N_act*(632 + 76*S + I*(180 + 68*K)) per frame, K = 40 CG iterations."""
+    N_grid*20 + N_act*(632 + 76*S + I*(180 + 68*K)) per frame, K = 40 CG iterations;
+    with an index per cell 4 more per grid cell (the cure writes it) and 8
+    more per active cell and substep (plug window and faces read it)."""
+    extra = (4 * n_grid * st.frames_counted + 8 * st.act_cell_substeps) if cell_n else 0
     return (n_grid * 20 * st.frames_counted + 632 * st.act_cells + 76 * st.act_cell_substeps
-            + (180 + 68 * 40) * st.act_cell_implicit)
+            + (180 + 68 * 40) * st.act_cell_implicit + extra)
 
 
 def stream_peak(nbytes=1 << 30, reps=20):
@@ -559,7 +598,11 @@ def main():
     ap.add_argument("--batch-canvases", type=int, default=64)
     ap.add_argument("--batch-frames", type=int, default=200)
     ap.add_argument("--no-km", action="store_true", help="skip the KM mix record in the film run")
-    ap.add_argument("--no-n07", action="store_true", help="skip the n = 0.7 plain record in the film run")
+    ap.add_argument("--index", default="cell", choices=("cell", "1", "0.7"),
+                    help="power-law index of every workload's canvases: one per wet cell in [0.4, 1] "
+                         "(default), or uniform")
+    ap.add_argument("--no-uniform", action="store_true",
+                    help="skip the uniform-index (n = 1 and 0.7) plain records in the film run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (canvases are independent "
@@ -596,9 +639,15 @@ def main():
         return bench_next(args)
 
     n = args.size
-    log(f"bench: building the {n}x{n} blob canvas")
-    state = scenes.blobs_state(n, n, seed=2048)
-    settings_fn = lambda w: scenes.blobs_settings(w, tilt_deg=15.0)  # noqa: E731
+    cell_n = args.index == "cell"
+    log(f"bench: building the {n}x{n} blob canvas ({index_words(args.index)})")
+    state = scenes.blobs_state(n, n, seed=2048, n_range=INDEX_RANGE if cell_n else None)
+
+    def settings_fn(w):
+        s = canvas_settings(w, 15.0, cell_n)
+        if not cell_n:
+            s.n_index = float(args.index)
+        return s
 
     def fresh():
         f = czf.Film(n, n, settings_fn(n))
@@ -665,9 +714,10 @@ def main():
     if dom == "substeps":
         # one launch per frame: every substep's three passes and the last nose
         cells_per_launch = st.one_launch_cell_substeps / max(1, launches)
-        bytes_per_launch = (SUBSTEP_BYTES * st.one_launch_cell_substeps
+        sub_bytes = SUBSTEP_BYTES + (8 if cell_n else 0)
+        bytes_per_launch = (sub_bytes * st.one_launch_cell_substeps
                             + ALGO_BYTES["lap"] * st.one_launch_cells) / max(1, launches)
-        algo_per_cell = SUBSTEP_BYTES
+        algo_per_cell = sub_bytes
     else:
         # cells x substeps the family's launches covered: every per-pass
         # substep runs lap_plug; flux, apply and flux_apply by flux shape
@@ -676,39 +726,43 @@ def main():
                    "flux": shp[0] + shp[1], "apply": shp[0] + shp[1], "flux_apply": shp[2],
                    "flux_w": shp[3], "apply_w": shp[3], "flux_apply_w": shp[4]}[dom]
         cells_per_launch = covered / max(1, launches)
-        bytes_per_launch = ALGO_BYTES[dom] * cells_per_launch
-        algo_per_cell = ALGO_BYTES[dom]
+        algo_per_cell = algo_bytes(dom, cell_n)
+        bytes_per_launch = algo_per_cell * cells_per_launch
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * n * args.steps / elapsed
     frames = max(1, st.frames_counted)
-    frame_gbs = frame_algo_bytes(n * n, st) / elapsed / 1e9
+    frame_gbs = frame_algo_bytes(n * n, st, cell_n) / elapsed / 1e9
     log(f"bench: replay, per-family device time (us avg x launches): " +
         ", ".join(f"{k} {v[0]:.2f}x{v[1]}" for k, v in prof.items() if v[1]))
     log(f"bench: {ms_per_step:.3f} ms/frame, active cells/frame {st.act_cells / frames:.0f}, "
         f"substeps/frame {st.act_cell_substeps / max(1, st.act_cells):.1f}, {dom}: {avg_us:.2f} us x {launches}, "
         f"{cells_per_launch:.0f} cells/launch")
-    plain_n07 = None
-    if not args.no_n07:
-        # the same canvas and frames at the shear-thinning n = 0.7 (the
-        # power-law branches of hb_flux and band_flux, film.wgsl:94-126)
-        s07 = settings_fn(n)
-        s07.n_index = 0.7
-        f07 = czf.Film(n, n, s07)
-        czf.check(czf.lib().cz_film_use_graphs(f07._p, 0 if args.no_graphs else 1))
-        assert f07.write_state_arrays(state)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            f07.step()
-        f07.sync()
-        el07 = time.perf_counter() - t0
-        s7 = f07.stats()
-        plain_n07 = {"value": n * n * args.steps / el07, "unit": "cell-steps/s", "ms_per_step": el07 * 1e3 / args.steps,
-                     "n_index": 0.7, "active_cells_per_frame": s7.act_cells / max(1, s7.frames_counted),
-                     "substeps_per_frame": s7.act_cell_substeps / max(1, s7.act_cells)}
-        f07.close()
-        log(f"bench: n = 0.7: {el07 * 1e3 / args.steps:.3f} ms/frame")
+    uniform = None
+    if not args.no_uniform:
+        # the same canvas and frames at one film index: the reference's
+        # Newtonian n = 1 and the shear-thinning n = 0.7 (the power-law
+        # branches of hb_flux and band_flux, film.wgsl:94-126)
+        uniform = []
+        for nu in (1.0, 0.7):
+            su = canvas_settings(n, 15.0, False)
+            su.n_index = nu
+            fu = czf.Film(n, n, su)
+            czf.check(czf.lib().cz_film_use_graphs(fu._p, 0 if args.no_graphs else 1))
+            assert fu.write_state_arrays(state)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fu.step()
+            fu.sync()
+            elu = time.perf_counter() - t0
+            su_st = fu.stats()
+            uniform.append({"value": n * n * args.steps / elu, "unit": "cell-steps/s",
+                            "ms_per_step": elu * 1e3 / args.steps, "n_index": nu,
+                            "active_cells_per_frame": su_st.act_cells / max(1, su_st.frames_counted),
+                            "substeps_per_frame": su_st.act_cell_substeps / max(1, su_st.act_cells)})
+            fu.close()
+            log(f"bench: uniform n = {nu}: {elu * 1e3 / args.steps:.3f} ms/frame")
         phase_marker(5)
     measured = stream_peak()
     log(f"bench: measured HBM stream: copy {measured['copy_GBps']:.0f} GB/s, fill {measured['fill_GBps']:.0f} GB/s")
@@ -731,9 +785,11 @@ def main():
         "metric": "film cell-steps/s", "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"plain film {n}x{n} FP32, 64 seeded Gaussian blobs, uniform power-law index n = 1 "
-                               f"(the reference's default), tilt 15 deg, frame_dt 1/60, {args.steps} frames from "
-                               f"the initial state, display shade pass off",
+        "config": {"workload": f"plain film {n}x{n} FP32, 64 seeded Gaussian blobs (radius 2-40 cells, peak "
+                               f"20 um-2 mm), {index_words(args.index)}, yield stress log-U[0.1, 50] Pa, viscosity "
+                               f"log-U[0.5, 50] Pa s, Dirichlet(0.3) pigments, tilt 15 deg, frame_dt 1/60, "
+                               f"{args.steps} frames from the initial state, display shade pass off",
+                   "index": args.index,
                    "cells": n * n, "graphs": not args.no_graphs, "substep_launch": args.substep_launch,
                    "one_launch_frames": st.one_launch_frames / max(1, st.frames_counted),
                    "flux_shape": args.flux_shape,
@@ -750,7 +806,7 @@ def main():
                      "algo_bytes_per_launch": bytes_per_launch, "algo_bytes_per_cell": algo_per_cell, "avg_launch_us": avg_us, "launches": launches,
                      "cells_per_launch": cells_per_launch},
         "cpu_baseline": cpu,
-        "plain_n07": plain_n07,
+        "plain_uniform_index": uniform,
         "km": km_rec,
         "batched": batched,
         "served": served,

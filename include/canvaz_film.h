@@ -162,7 +162,18 @@ typedef struct cz_settings {
     uint32_t slump;
     uint32_t capillary;
     uint32_t capillary_implicit;
+    /* Not in the reference (whose power-law index is the film's n_index,
+     * film.wgsl:95): 1 = per-cell power-law index.  Each stratum's pigment
+     * slot 15 (unused by the reference) holds its index in f16, 0 standing
+     * for n_index; a cell's index is the thickness-weighted mean over its
+     * strata (formed with the rheology by the cure pass), a face's the
+     * height-weighted mean of its two cells' (as for the yield stress).
+     * update_params sets Params.flags[2] bit 0 (CZ_FLAG_CELL_N) from it.
+     * 0 (default) = the reference's global index. */
+    uint32_t n_per_cell;
 } cz_settings;
+
+#define CZ_FLAG_CELL_N 1u   /* Params.flags[2]: per-cell power-law index (n_per_cell) */
 
 /* The environment toggles read by `update_params` every frame
  * (src/film/mod.rs:870-915).  cz_overrides_from_env() reads PAINT_NO_IMPLICIT,
@@ -279,6 +290,7 @@ enum {
     CZ_BUF_DETAIL = 37,     /* f16 [n][4]: shade detail_out (method + turn, cure, yield stress, thickness) */
     CZ_BUF_JMETA = 38,      /* u32 [journal cap][4]: slot (tile16, generation, sequence, valid) */
     CZ_BUF_JMARK = 39,      /* u32 [tiles16]: generation each tile was last saved in */
+    CZ_BUF_N_CELL = 41,     /* f32 [n]: per-cell power-law index (settings.n_per_cell; derived by the cure pass) */
     CZ_BUF_JDATA = 40       /* u32 [journal cap][13568]: per slot the parts h, m, rheo, strata_h, strata_pig,
                                strata_cure, sub_refl, lay, each as [256 cells (tile rows)][its words] */
 };
@@ -520,8 +532,10 @@ int cz_film_reset_counters(cz_film* f);
  * zero below 1e-9 m); every wet cell one stratum of Dirichlet(alpha)
  * pigments with log-uniform yield stress and viscosity in slots 13/14;
  * relief the canvas weave (mod.rs:212-233), substrate the raw canvas
- * (mod.rs:188-210).  Counter-based randomness: the result depends only on
- * the scene and the canvas size. */
+ * (mod.rs:188-210).  With n_hi > 0 every wet cell also carries its own
+ * power-law index, uniform in [n_lo, n_hi], in pigment slot 15 (read only
+ * when the film's settings set n_per_cell).  Counter-based randomness: the
+ * result depends only on the scene and the canvas size. */
 typedef struct cz_scene {
     uint64_t seed;
     uint32_t nblobs;          /* <= 64 */
@@ -531,6 +545,7 @@ typedef struct cz_scene {
     float mu_lo, mu_hi;       /* Pa s */
     float alpha;
     float weave_um;
+    float n_lo, n_hi;         /* per-cell power-law index; n_hi = 0: none */
 } cz_scene;
 void cz_scene_default(cz_scene* s);
 /* Replace the film's state with the scene (as write_state does). */

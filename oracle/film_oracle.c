@@ -75,6 +75,7 @@ typedef struct ofilm {
     float *strata_h2, *strata_cure2;
     uint32_t *strata_pig2;
     float *h_frame, *fnorm;
+    float *n_cell;   /* per-cell power-law index (settings.n_per_cell) */
     float *cg_m, *cg_d, *cg_x, *cg_r, *cg_p, *cg_q, *cg_t, *cg_t2;
     double *cg_part, *cg_part2;
     float cg_s[8];
@@ -238,9 +239,30 @@ static inline float yield_depth(const ofilm* F, uint32_t ca, uint32_t cb, float 
     return (g.x >= 0.0f) ? F->h[ca] : F->h[cb];
 }
 
-/* film.wgsl:94-107: returns q, writes y and up */
-static inline float hb_flux(const ofilm* F, float gmag, float hf, float mu, float ty, float* oy, float* oup) {
-    float n = F->P.phys[2];
+/* Per-cell power-law index (settings.n_per_cell, not in the reference, whose
+ * index is the film's P.phys.z, film.wgsl:95): with CZ_FLAG_CELL_N in
+ * Params.flags[2], a stratum's pigment slot 15 holds its index (0 = the
+ * film's), a cell's index is the thickness-weighted mean over its strata
+ * (the cure pass forms it with the rheology; a dry cell takes the film's,
+ * as its rheology takes the defaults) and a face's the height-weighted mean
+ * of its two cells' indices. */
+static inline int cell_n_on(const ofilm* F) { return (F->P.flags[2] & CZ_FLAG_CELL_N) != 0u; }
+static float strata_n(const ofilm* F, const float* t, const float* n15) {
+    float tot = 0.0f, acc = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        if (t[k] <= 0.0f) continue;
+        float nk = (n15[k] > 0.0f) ? n15[k] : F->P.phys[2];
+        tot = tot + t[k];
+        acc = acc + t[k] * nk;
+    }
+    return (tot > 0.0f) ? acc / tot : F->P.phys[2];
+}
+static inline float face_n(const ofilm* F, uint32_t ca, uint32_t cb, float wa, float wb) {
+    return cell_n_on(F) ? wa * F->n_cell[ca] + wb * F->n_cell[cb] : F->P.phys[2];
+}
+
+/* film.wgsl:94-107: returns q, writes y and up; n = the power-law index */
+static inline float hb_flux_n(float n, float gmag, float hf, float mu, float ty, float* oy, float* oup) {
     *oy = 0.0f;
     *oup = 0.0f;
     if (gmag < 1e-9f || hf <= 0.0f) return 0.0f;
@@ -257,10 +279,12 @@ static inline float hb_flux(const ofilm* F, float gmag, float hf, float mu, floa
     *oup = up;
     return q;
 }
+static inline float hb_flux(const ofilm* F, float gmag, float hf, float mu, float ty, float* oy, float* oup) {
+    return hb_flux_n(F->P.phys[2], gmag, hf, mu, ty, oy, oup);
+}
 
 /* film.wgsl:109-126 */
-static inline float band_flux(const ofilm* F, float up, float y, float za, float zb) {
-    float n = F->P.phys[2];
+static inline float band_flux_n(float n, float up, float y, float za, float zb) {
     float ta = 0.0f, tb = 0.0f;
     if (y > 0.0f) {
         float ra = cz_max(y - za, 0.0f) / y;
@@ -275,6 +299,9 @@ static inline float band_flux(const ofilm* F, float up, float y, float za, float
         }
     }
     return up * ((zb - za) - n / (2.0f * n + 1.0f) * y * (ta - tb));
+}
+static inline float band_flux(const ofilm* F, float up, float y, float za, float zb) {
+    return band_flux_n(F->P.phys[2], up, y, za, zb);
 }
 
 /* film.wgsl:134-145 */
@@ -435,11 +462,11 @@ static float rim_translation(const ofilm* F, uint32_t ca, uint32_t cb, float hf,
     return u * hf * room * cz_mix(1.0f, cz_abs(s), F->P.front2[0]);
 }
 
-typedef struct { float q, y, up; } FaceFlux;
+typedef struct { float q, y, up, n; } FaceFlux;
 
 /* film.wgsl:297-372 */
 static FaceFlux face(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd, v2 gtilt, int diag, v2 face_dir) {
-    FaceFlux out = {0.0f, 0.0f, 0.0f};
+    FaceFlux out = {0.0f, 0.0f, 0.0f, F->P.phys[2]};
     float ha = F->h[ca];
     float hb = F->h[cb];
     float hf = 0.5f * (ha + hb);
@@ -466,6 +493,7 @@ static FaceFlux face(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd,
     float wb = 1.0f - wa;
     float mu = cz_expf(wa * cz_logf(cz_max(rax, 1e-4f)) + wb * cz_logf(cz_max(rbx, 1e-4f)));
     float ty = wa * ray + wb * rby;
+    float nf = face_n(F, ca, cb, wa, wb);
     float thin = cz_min(ha, hb) / hf;
     float fcap = thin * thin * thin;
     v2 gc = {gcap.x * fcap, gcap.y * fcap};
@@ -473,7 +501,7 @@ static FaceFlux face(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd,
     v2 g = {yc.x + ghyd.x + gtilt.x, yc.y + ghyd.y + gtilt.y};
     float gmag = cz_len2(g.x, g.y);
     float qy_y, qy_up;
-    float qy_x = hb_flux(F, gmag, yield_depth(F, ca, cb, hf, ty, g), mu, ty, &qy_y, &qy_up);
+    float qy_x = hb_flux_n(nf, gmag, yield_depth(F, ca, cb, hf, ty, g), mu, ty, &qy_y, &qy_up);
     float meff = (qy_x > 0.0f) ? qy_x / cz_max(gmag, 1e-9f) : 0.0f;
     float mtilt = meff;
     if (ty > 0.0f) mtilt = plug_mobility(F, ca, cb, gtilt);
@@ -508,6 +536,7 @@ static FaceFlux face(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd,
     out.q = qn;
     out.y = qy_y;
     out.up = qy_up;
+    out.n = nf;
     return out;
 }
 
@@ -1403,6 +1432,7 @@ static void k_cure_step(ofilm* F) {
         if (F->h[c] <= 0.0f) {
             F->rheo[2 * c] = 1.0f;
             F->rheo[2 * c + 1] = 0.0f;
+            if (cell_n_on(F)) F->n_cell[c] = F->P.phys[2];
             continue;
         }
         float rate = F->P.cure[0] * F->P.frame_dt;
@@ -1415,6 +1445,11 @@ static void k_cure_step(ofilm* F) {
         if (!(t[0] > 0.0f && u[0] >= 1.0f)) {
             for (int k = 0; k < 4; k++) F->strata_cure[4 * c + k] = u[k];
             packed_rheo(F, c, t, u, &F->rheo[2 * c], &F->rheo[2 * c + 1]);
+            if (cell_n_on(F)) {
+                float n15[4];
+                for (int k = 0; k < 4; k++) n15[k] = (t[k] > 0.0f) ? cz_unpack_hi(F->strata_pig[(c * 4u + k) * 8u + 7u]) : 0.0f;
+                F->n_cell[c] = strata_n(F, t, n15);
+            }
             continue;
         }
         Stack s;
@@ -1444,6 +1479,10 @@ static void k_cure_step(ofilm* F) {
         }
         if (committed > 0.0f) F->h[c] = cz_max(F->h[c] - committed, 0.0f);
         composition_rheo(F, &s, &F->rheo[2 * c], &F->rheo[2 * c + 1]);
+        if (cell_n_on(F)) {
+            float n15[4] = {s.pg[0][15], s.pg[1][15], s.pg[2][15], s.pg[3][15]};
+            F->n_cell[c] = strata_n(F, s.th, n15);
+        }
         stack_store_to(F->strata_h, F->strata_cure, F->strata_pig, c, &s);
     }
 }
@@ -1665,7 +1704,8 @@ static void k_plug(ofilm* F) {
                 float rx = F->rheo[2 * nn], ry = F->rheo[2 * nn + 1];
                 if (ry > 0.0f) yielding = 1;
                 float yy_, uu_;
-                float q = hb_flux(F, gb, hn, cz_max(rx, 1e-4f), ry, &yy_, &uu_);
+                float nq = cell_n_on(F) ? F->n_cell[nn] : F->P.phys[2];
+                float q = hb_flux_n(nq, gb, hn, cz_max(rx, 1e-4f), ry, &yy_, &uu_);
                 sum = sum + q;
                 mass = mass + hn;
             }
@@ -1732,6 +1772,7 @@ static void inherit_from(ofilm* F, uint32_t c, uint32_t src, float hn, int copy_
     if (copy_rheo) {
         F->rheo[2 * c] = F->rheo[2 * src];
         F->rheo[2 * c + 1] = F->rheo[2 * src + 1];
+        if (cell_n_on(F)) F->n_cell[c] = F->n_cell[src];
     }
     uint32_t ks = 0;
     for (uint32_t k = 0; k < 4u; k++)
@@ -1852,7 +1893,7 @@ static void k_nose(ofilm* F) {
 }
 
 /* film.wgsl:457-473 */
-static void band_split(const ofilm* F, uint32_t cu, float hf, float y, float up, float total, float* out) {
+static void band_split(const ofilm* F, uint32_t cu, float hf, float y, float up, float total, float n, float* out) {
     const float* sh = &F->strata_h[4 * cu];
     float hu = sh[0] + sh[1] + sh[2] + sh[3];
     if (hu <= 0.0f) { out[0] = total; out[1] = 0.0f; out[2] = 0.0f; out[3] = 0.0f; return; }
@@ -1863,7 +1904,7 @@ static void band_split(const ofilm* F, uint32_t cu, float hf, float y, float up,
         float z2 = z1 + sh[1] * k;
         float z3 = z2 + sh[2] * k;
         float z4 = z3 + sh[3] * k;
-        float b[4] = {band_flux(F, up, y, 0.0f, z1), band_flux(F, up, y, z1, z2), band_flux(F, up, y, z2, z3), band_flux(F, up, y, z3, z4)};
+        float b[4] = {band_flux_n(n, up, y, 0.0f, z1), band_flux_n(n, up, y, z1, z2), band_flux_n(n, up, y, z2, z3), band_flux_n(n, up, y, z3, z4)};
         float bs = b[0] + b[1] + b[2] + b[3];
         if (bs > 0.0f) for (int k2 = 0; k2 < 4; k2++) frac[k2] = b[k2] / bs;
     }
@@ -1881,22 +1922,22 @@ static void k_strata_split(ofilm* F) {
         if (a[0] != 0.0f && x + 1u < F->W) {
             uint32_t cb = idx(F, x + 1u, y);
             uint32_t cu = (a[0] < 0.0f) ? cb : c;
-            band_split(F, cu, 0.5f * (F->h[c] + F->h[cb]), ff[0].y, ff[0].up, a[0], o);
+            band_split(F, cu, 0.5f * (F->h[c] + F->h[cb]), ff[0].y, ff[0].up, a[0], ff[0].n, o);
         }
         if (a[1] != 0.0f && y + 1u < F->H) {
             uint32_t cb = idx(F, x, y + 1u);
             uint32_t cu = (a[1] < 0.0f) ? cb : c;
-            band_split(F, cu, 0.5f * (F->h[c] + F->h[cb]), ff[1].y, ff[1].up, a[1], o + 4);
+            band_split(F, cu, 0.5f * (F->h[c] + F->h[cb]), ff[1].y, ff[1].up, a[1], ff[1].n, o + 4);
         }
         if (a[2] != 0.0f && x + 1u < F->W && y + 1u < F->H) {
             uint32_t cb = idx(F, x + 1u, y + 1u);
             uint32_t cu = (a[2] < 0.0f) ? cb : c;
-            band_split(F, cu, 0.5f * (F->h[c] + F->h[cb]), ff[2].y, ff[2].up, a[2], o + 8);
+            band_split(F, cu, 0.5f * (F->h[c] + F->h[cb]), ff[2].y, ff[2].up, a[2], ff[2].n, o + 8);
         }
         if (a[3] != 0.0f && x + 1u < F->W && y > 0u) {
             uint32_t cb = idx(F, x + 1u, y - 1u);
             uint32_t cu = (a[3] < 0.0f) ? cb : c;
-            band_split(F, cu, 0.5f * (F->h[c] + F->h[cb]), ff[3].y, ff[3].up, a[3], o + 12);
+            band_split(F, cu, 0.5f * (F->h[c] + F->h[cb]), ff[3].y, ff[3].up, a[3], ff[3].n, o + 12);
         }
     })
 }
@@ -2116,6 +2157,7 @@ static float face_mob(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd
     float wb = 1.0f - wa;
     float mu = cz_expf(wa * cz_logf(cz_max(rax, 1e-4f)) + wb * cz_logf(cz_max(rbx, 1e-4f)));
     float ty = wa * ray + wb * rby;
+    float nf = face_n(F, ca, cb, wa, wb);
     float thin = cz_min(ha, hb) / hf;
     float fcap = thin * thin * thin;
     v2 gc = {gcap.x * fcap, gcap.y * fcap};
@@ -2123,7 +2165,7 @@ static float face_mob(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd
     v2 g = {yc.x + ghyd.x + gtilt.x, yc.y + ghyd.y + gtilt.y};
     float gmag = cz_len2(g.x, g.y);
     float qy, qu;
-    float q = hb_flux(F, gmag, yield_depth(F, ca, cb, hf, ty, g), mu, ty, &qy, &qu);
+    float q = hb_flux_n(nf, gmag, yield_depth(F, ca, cb, hf, ty, g), mu, ty, &qy, &qu);
     if (q <= 0.0f) return 0.0f;
     float mob = cz_min(cz_max(q / gmag - F->P.misc[3], 0.0f), F->P.front[2]);
     if (cz_min(ha, hb) < 0.5f * cz_max(ha, hb)) {
@@ -2526,7 +2568,7 @@ void of_update_params(ofilm* F, float frame_dt, uint32_t substeps) {
     P->ein[0] = fmaxf(htanf(theta), 0.05f); P->ein[1] = lc * hsinf(0.5f * theta); P->ein[2] = pi_a; P->ein[3] = s->h_min;
     P->cure[0] = s->cure_rate; P->cure[1] = s->tau_cure; P->cure[2] = s->lnmu_cure; P->cure[3] = s->h_star;
     P->misc[0] = 2.0e-6f; P->misc[1] = 0.12f; P->misc[2] = s->medium_mu; P->misc[3] = fminf(m_exp, s->m_cap_hydro);
-    P->flags[0] = s->debug_view; P->flags[1] = s->slump; P->flags[2] = 0; P->flags[3] = 0;
+    P->flags[0] = s->debug_view; P->flags[1] = s->slump; P->flags[2] = s->n_per_cell ? CZ_FLAG_CELL_N : 0u; P->flags[3] = 0;
     float lc_z = sqrtf(s->sigma / (s->rho * fmaxf(g[2], 0.1f)));
     float cap_explicit = fminf(m_exp, s->m_cap_hydro);
     float h_big = fmaxf(fmaxf(F->h_max, F->h_max_pending), 1e-4f);
@@ -2846,7 +2888,7 @@ ofilm* of_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigt
     F->dx = s->width_m / (float)w;
     size_t n = F->n;
 #define AL(p, cnt) p = calloc((cnt), sizeof(*(p)))
-    AL(F->h, n); AL(F->m, n); AL(F->rheo, 2 * n); AL(F->strata_h, 4 * n); AL(F->strata_cure, 4 * n);
+    AL(F->h, n); AL(F->m, n); AL(F->rheo, 2 * n); AL(F->n_cell, n); AL(F->strata_h, 4 * n); AL(F->strata_cure, 4 * n);
     AL(F->strata_pig, 32 * n); AL(F->sub_refl, 8 * n); AL(F->lay, n);
     AL(F->pres, 2 * n); AL(F->flux, 4 * n); AL(F->acc, 4 * n); AL(F->alpha, n); AL(F->delta, n); AL(F->tmp, n);
     AL(F->acc_s, 16 * n); AL(F->alpha_s, 4 * n); AL(F->strata_h2, 4 * n); AL(F->strata_cure2, 4 * n);
@@ -2879,7 +2921,8 @@ void of_destroy(ofilm* F) {
                   F->flux, F->acc, F->alpha, F->delta, F->tmp, F->acc_s, F->alpha_s, F->strata_h2, F->strata_cure2,
                   F->strata_pig2, F->h_frame, F->fnorm, F->cg_m, F->cg_d, F->cg_x, F->cg_r, F->cg_p, F->cg_q,
                   F->cg_t, F->cg_t2, F->cg_part, F->cg_part2, F->tile_flags, F->tile_list, F->tile_moved,
-                  F->stats_out, F->pool_out, F->snap, F->deposits, F->tile_start, F->tile_count, F->tile_items};
+                  F->stats_out, F->pool_out, F->snap, F->deposits, F->tile_start, F->tile_count, F->tile_items,
+                  F->n_cell};
     for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
     free(F->j_meta);
     free(F->j_data);
@@ -2909,6 +2952,7 @@ void* of_buffer(ofilm* F, int which, uint64_t* bytes) {
     case CZ_BUF_FNORM: *bytes = 8 * n; return F->fnorm;
     case CZ_BUF_PRES: *bytes = 8 * n; return F->pres;
     case CZ_BUF_H_FRAME: *bytes = 4 * n; return F->h_frame;
+    case CZ_BUF_N_CELL: *bytes = 4 * n; return F->n_cell;
     case CZ_BUF_CG_M: *bytes = 16 * n; return F->cg_m;
     case CZ_BUF_CG_P: *bytes = 4 * n; return F->cg_p;
     case CZ_BUF_TILE_MOVED: *bytes = 4 * (uint64_t)F->nt16; return F->tile_moved;

@@ -147,6 +147,7 @@ BUFFERS = {
     "stats": (28, np.float32, None), "pool": (29, np.float32, None), "tile_list": (30, np.uint32, None),
     "acc_s": (31, np.float32, (4, 4)), "alpha_s": (32, np.float32, (4,)),
     "jmeta": (38, np.uint32, None), "jmark": (39, np.uint32, None), "jdata": (40, np.uint32, None),
+    "n_cell": (41, np.float32, ()),
 }
 
 

@@ -277,12 +277,12 @@ BUFFER_IDS = {
     "delta": 16, "fnorm": 17, "pres": 18, "h_frame": 19, "cg_m": 20, "cg_p": 21, "tile_moved": 22,
     "tile_flags": 23, "tmp": 24, "flux": 25, "acc": 26, "alpha": 27, "stats": 28, "pool": 29, "tile_list": 30,
     "acc_s": 31, "alpha_s": 32, "dep_start": 33, "dep_count": 34, "dep_items": 35,
-    "color": 36, "detail": 37, "jmeta": 38, "jmark": 39, "jdata": 40,
+    "color": 36, "detail": 37, "jmeta": 38, "jmark": 39, "jdata": 40, "n_cell": 41,
 }
 _U32 = {"strata_pig", "sub_refl", "lay", "delta", "tile_moved", "tile_flags", "tile_list", "dep_start", "dep_count",
         "dep_items", "jmeta", "jmark", "jdata"}
 _CELL_SHAPE = {"h": (), "m": (), "rheo": (2,), "strata_h": (4,), "strata_pig": (4, 8), "strata_cure": (4,),
-               "sub_refl": (8,), "lay": (), "delta": (), "color": (4,), "detail": (4,), "fnorm": (2,), "pres": (2,), "h_frame": (), "cg_m": (4,),
+               "sub_refl": (8,), "lay": (), "delta": (), "color": (4,), "detail": (4,), "fnorm": (2,), "pres": (2,), "h_frame": (), "n_cell": (), "cg_m": (4,),
                "cg_p": (), "tmp": (), "flux": (4,), "acc": (4,), "alpha": (), "acc_s": (4, 4), "alpha_s": (4,)}
 
 

@@ -92,9 +92,31 @@ struct FilmDev {
     const float* cmf;      // [16][3] CIE 1931 x D65 per band pair (shade)
     uint2* color;          // [n] shade output: rgb + roughness, f16 x 4
     uint2* detail;         // [n] shade output: method + turn, cure, yield stress, thickness, f16 x 4
+    float* n_cell;         // [n] per-cell power-law index (with CZ_FLAG_CELL_N)
 };
 
 __device__ __forceinline__ uint32_t idx(const FilmDev& F, uint32_t i, uint32_t j) { return j * F.W + i; }
+
+// Per-cell power-law index, an extension of the reference (whose index is the
+// film's P.phys.z, film.wgsl:95), on when Params.flags[2] has CZ_FLAG_CELL_N:
+// each stratum's pigment slot 15, unused by the reference, holds its index
+// (0 = the film's); a cell's index is the thickness-weighted mean over its
+// strata, formed by the cure pass with the rheology (F.n_cell), and a face
+// takes the height-weighted mean of its two cells' indices, as it takes the
+// yield stress (film.wgsl:326-329).  Off, every index is P.phys.z.
+__device__ __forceinline__ bool cell_n_on(const FilmDev& F) { return (F.P.flags[2] & CZ_FLAG_CELL_N) != 0u; }
+// t: stratum thicknesses, n15: their slot-15 values
+__device__ __forceinline__ float strata_n(const FilmDev& F, const float* t, const float* n15) {
+    float tot = 0.0f, acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (t[k] <= 0.0f) continue;
+        const float nk = (n15[k] > 0.0f) ? n15[k] : F.P.phys[2];
+        tot = tot + t[k];
+        acc = acc + t[k] * nk;
+    }
+    return (tot > 0.0f) ? acc / tot : F.P.phys[2];
+}
 
 // shaders/common.wgsl:152-166
 __device__ __forceinline__ uint32_t refl_i(const FilmDev& F, int i) {
@@ -233,6 +255,7 @@ struct GlobalRead {
     __device__ __forceinline__ float2 fnorm(int i, int j) const { return F.fnorm[at(i, j)]; }
     __device__ __forceinline__ float2 pres(int i, int j) const { return F.pres[at(i, j)]; }
     __device__ __forceinline__ float tmp(int i, int j) const { return F.tmp[at(i, j)]; }
+    __device__ __forceinline__ float ncell(int i, int j) const { return F.n_cell[at(i, j)]; }
     __device__ __forceinline__ bool active(int i, int j) const { return cell_active(F, (uint32_t)i, (uint32_t)j); }
     __device__ __forceinline__ float h_far(int i, int j) const { return h(i, j); }
     __device__ __forceinline__ uint32_t delta_far(int i, int j) const { return delta(i, j); }
@@ -402,13 +425,13 @@ __device__ __forceinline__ float rim_translation(const FilmDev& F, const A& a, i
     return u * hf * room * cz_mix(1.0f, cz_abs(s), F.P.front2[0]);
 }
 
-struct FaceFlux { float q, y, up; };
+struct FaceFlux { float q, y, up, n; };
 
 // film.wgsl:297-372: face between cell a = (ia, ja) and cell b = (ib, jb)
 template <class A>
 __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia, int ja, int ib, int jb, float2 gcap,
                                 float2 ghyd, float2 gtilt, bool diag, float2 face_dir) {
-    FaceFlux out = {0.0f, 0.0f, 0.0f};
+    FaceFlux out = {0.0f, 0.0f, 0.0f, F.P.phys[2]};
     float ha = acc.h(ia, ja);
     float hb = acc.h(ib, jb);
     float hf = 0.5f * (ha + hb);
@@ -435,6 +458,7 @@ __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia,
     float wb = 1.0f - wa;
     float mu = cz_expf(wa * acc.lmu(ia, ja) + wb * acc.lmu(ib, jb));
     float ty = wa * ra.y + wb * rb.y;
+    const float nf = cell_n_on(F) ? wa * acc.ncell(ia, ja) + wb * acc.ncell(ib, jb) : F.P.phys[2];
     float thin = cz_min(ha, hb) / hf;
     float fcap = thin * thin * thin;
     float2 yc = yield_cap(F, make_float2(gcap.x * fcap, gcap.y * fcap), ty);
@@ -442,7 +466,7 @@ __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia,
     float gmag = cz_len2(g.x, g.y);
     float ydep = (ty <= 0.0f) ? hf : ((g.x >= 0.0f) ? ha : hb);
     float qy_y, qy_up;
-    float qy_x = hb_flux(F.P.phys[2], gmag, ydep, mu, ty, qy_y, qy_up);
+    float qy_x = hb_flux(nf, gmag, ydep, mu, ty, qy_y, qy_up);
     float meff = (qy_x > 0.0f) ? qy_x / cz_max(gmag, 1e-9f) : 0.0f;
     float mtilt = meff;
     if (ty > 0.0f) {
@@ -485,6 +509,7 @@ __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia,
     out.q = qn;
     out.y = qy_y;
     out.up = qy_up;
+    out.n = nf;
     return out;
 }
 

@@ -539,6 +539,7 @@ BufDesc buffer_of(const cz_film* f, int which) {
     case CZ_BUF_JMETA: return {f->j_cap ? f->j_meta : nullptr, 16 * (uint64_t)f->j_cap};
     case CZ_BUF_JMARK: return {f->j_cap ? f->j_mark : nullptr, f->j_cap ? 4 * (uint64_t)f->nt16 : 0};
     case CZ_BUF_JDATA: return {f->j_cap ? f->j_data : nullptr, 53 * 256 * 4 * (uint64_t)f->j_cap};
+    case CZ_BUF_N_CELL: return {f->n_cell, 4 * n};
     default: return {nullptr, 0};
     }
 }
@@ -677,7 +678,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(tile_bits, nt16); A(dep_rect, 4 * (size_t)CZ_MAX_DEPOSITS); A(dep_cnt, nt64); A(dep_n, 4); A(cmf, CZ_NBAND * 3); A(compact_cnt, 2 * ((nt16 + 255) / 256) + 2); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
     A(pigtab, CZ_NPIG * CZ_PIG_STRIDE); A(deposits, CZ_MAX_DEPOSITS); A(tile_start, nt64); A(tile_count, nt64);
     A(tile_items, CZ_MAX_TILE_ITEMS); A(frame_out, 16);
-    A(mass_ws, (cz_mass_sums_workspace(n) + 15) / 16); A(mass_out, 1);
+    A(mass_ws, (cz_mass_sums_workspace(n) + 15) / 16); A(mass_out, 1); A(n_cell, n);
 #undef A
     if (!rc) {
         e = hipHostMalloc((void**)&f->frame_out_h, 16 * sizeof(float), hipHostMallocDefault);

@@ -257,6 +257,7 @@ struct cz_film {
     int bin_mode = -1;             // deposit binning: -1 by count, 0 host, 1 device
     bool bin_on_device = false;    // the uploaded deposits are binned in the frame
     float2* stats_out = nullptr;
+    float* n_cell = nullptr;         // [n] per-cell power-law index (settings.n_per_cell)
     uint4* mass_ws = nullptr;        // [cz_mass_sums_workspace(cap_n) / 16] partials of the mass report
     cz_mass_totals* mass_out = nullptr;  // [1] its result
     float* pool_out = nullptr;
@@ -303,6 +304,7 @@ struct cz_film {
         F.cg_part = cg_part; F.cg_part2 = cg_part2; F.cg_s = cg_s;
         F.tile_flags = tile_flags; F.tile_list = tile_list; F.tile_moved = tile_moved; F.active_count = active_count;
         F.cure_list = cure_list; F.cure_count = cure_count;
+        F.n_cell = n_cell;
         F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.tile_bits = tile_bits; F.compact_cnt = compact_cnt; F.dep_rect = dep_rect; F.dep_cnt = dep_cnt; F.dep_n = dep_n; F.cmf = cmf; F.color = color; F.detail = detail; F.pigtab = pigtab;
         F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
         F.frame_out = frame_out;

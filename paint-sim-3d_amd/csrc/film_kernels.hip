@@ -617,6 +617,12 @@ __device__ inline float cure_wet_cell(const FilmDev& F, uint32_t c, float hc) {
     if (!(t[0] > 0.0f && u[0] >= 1.0f)) {
         F.strata_cure[c] = make_float4(u[0], u[1], u[2], u[3]);
         F.rheo[c] = packed_rheo(F, c, t, u);
+        if (cell_n_on(F)) {
+            float n15[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) n15[k] = (t[k] > 0.0f) ? cz_unpack_hi(F.strata_pig[(c * 4u + k) * 2u + 1u].w) : 0.0f;
+            F.n_cell[c] = strata_n(F, t, n15);
+        }
     } else {
         Stack s;
         stack_load(F, c, s);
@@ -656,6 +662,10 @@ __device__ inline float cure_wet_cell(const FilmDev& F, uint32_t c, float hc) {
         float mu, ty;
         composition_rheo(F, s, mu, ty);
         F.rheo[c] = make_float2(mu, ty);
+        if (cell_n_on(F)) {
+            const float n15[4] = {s.pg[0][15], s.pg[1][15], s.pg[2][15], s.pg[3][15]};
+            F.n_cell[c] = strata_n(F, s.th, n15);
+        }
         stack_store_to(F.strata_h, F.strata_cure, F.strata_pig, c, s);
     }
     return hc;
@@ -671,8 +681,12 @@ __global__ __launch_bounds__(256) void k_cure_clear(WS ws, int do_clear) {
         if (i >= F.W || j >= F.H) continue;
         uint32_t c = idx(F, i, j);
         float hc = F.h[c];
-        if (hc <= 0.0f) F.rheo[c] = make_float2(1.0f, 0.0f);
-        else hc = cure_wet_cell(F, c, hc);
+        if (hc <= 0.0f) {
+            F.rheo[c] = make_float2(1.0f, 0.0f);
+            if (cell_n_on(F)) F.n_cell[c] = F.P.phys[2];
+        } else {
+            hc = cure_wet_cell(F, c, hc);
+        }
         if (do_clear && cell_active(F, i, j)) {
             F.acc[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             F.h_frame[c] = hc;
@@ -683,7 +697,8 @@ __global__ __launch_bounds__(256) void k_cure_clear(WS ws, int do_clear) {
 // mark_tiles (film.wgsl:1365-1391), one wave per 16x16 tile: lane (r, q)
 // owns cells 4q..4q+3 of tile row r, read as one 16-byte load when rows are
 // 4-aligned.  The same pass writes the rheology of dry cells (the dry branch
-// of cure_step, cure.wgsl:122-125, which reads nothing but the height) and
+// of cure_step, cure.wgsl:122-125, which reads nothing but the height; with
+// per-cell indices on, a dry cell's index is reset to the film's) and
 // marks with NEEDS_CURE the tiles holding cells for the wet branch; those
 // run in k_cure over the list the compaction builds.  Each cell's mark bits
 // use its height before its cure, as with the passes run in the reference's
@@ -733,10 +748,17 @@ __global__ __launch_bounds__(256) void k_mark(WS ws) {
                 float4* rp = (float4*)(F.rheo + c0);
                 rp[0] = make_float4(1.0f, 0.0f, 1.0f, 0.0f);
                 rp[1] = make_float4(1.0f, 0.0f, 1.0f, 0.0f);
+                if (cell_n_on(F)) {
+                    const float nf = F.P.phys[2];
+                    *(float4*)(F.n_cell + c0) = make_float4(nf, nf, nf, nf);
+                }
             } else {
 #pragma unroll
                 for (uint32_t k = 0; k < 4u; k++)
-                    if (x0 + k < F.W && hv[k] <= 0.0f) F.rheo[c0 + k] = make_float2(1.0f, 0.0f);
+                    if (x0 + k < F.W && hv[k] <= 0.0f) {
+                        F.rheo[c0 + k] = make_float2(1.0f, 0.0f);
+                        if (cell_n_on(F)) F.n_cell[c0 + k] = F.P.phys[2];
+                    }
             }
         }
         uint32_t flags = 0u;
@@ -942,7 +964,10 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
                 const float2 r = sw.rheo[q];
                 yl = r.y > 0.0f;
                 float oy_, oup_;
-                qv = hb_flux(F.P.phys[2], gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
+                float nq = F.P.phys[2];
+                if (cell_n_on(F))
+                    nq = F.n_cell[idx(F, (uint32_t)(wx0 + (int)(q % WIN)), (uint32_t)(wy0 + (int)(q / WIN)))];
+                qv = hb_flux(nq, gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
             } else {
                 hn = 0.0f;
             }
@@ -1118,6 +1143,8 @@ struct WinReadT {
     __device__ __forceinline__ float tmp(int i, int j) const { return w.tmp[q(i, j)]; }
     __device__ __forceinline__ float lmu(int i, int j) const { return w.lmu[q(i, j)]; }
     __device__ __forceinline__ float fdist(int i, int j) const { return w.fd[q(i, j)]; }
+    // read when the per-cell index is on only
+    __device__ __forceinline__ float ncell(int i, int j) const { return F.n_cell[(uint32_t)j * F.W + (uint32_t)i]; }
     float hold1_, theta_;
     __device__ __forceinline__ float hold1() const { return hold1_; }
     __device__ __forceinline__ float theta() const { return theta_; }
@@ -1249,7 +1276,10 @@ struct Inflow { float best; uint32_t src; float div; float accv; };
 // Copy of the top stratum (and rheology) of src into the newly wetted cell c
 // (film.wgsl:847-860, strata.wgsl:312-322).
 __device__ inline void inherit_from(const FilmDev& F, uint32_t c, uint32_t src, float hn, bool copy_rheo) {
-    if (copy_rheo) F.rheo[c] = F.rheo[src];
+    if (copy_rheo) {
+        F.rheo[c] = F.rheo[src];
+        if (cell_n_on(F)) F.n_cell[c] = F.n_cell[src];
+    }
     float4 ts = F.strata_h[src];
     uint32_t ks = 0;
     if (ts.x > 0.0f) ks = 0;
@@ -1572,13 +1602,12 @@ __global__ __launch_bounds__(256) void k_substeps(FilmDev Fa, FilmDev Fb, uint32
 // ---------------------------------------------------------------------------
 // strata_split (film.wgsl:457-514)
 
-__device__ inline float4 band_split(const FilmDev& F, uint32_t cu, float hf, float y, float up, float total) {
+__device__ inline float4 band_split(const FilmDev& F, uint32_t cu, float hf, float y, float up, float total, float n) {
     float4 sh = F.strata_h[cu];
     float hu = sh.x + sh.y + sh.z + sh.w;
     if (hu <= 0.0f) return make_float4(total, 0.0f, 0.0f, 0.0f);
     float fr[4] = {sh.x / hu, sh.y / hu, sh.z / hu, sh.w / hu};
     if (y > 0.0f && up > 0.0f) {
-        float n = F.P.phys[2];
         float k = hf / hu;
         float z1 = sh.x * k;
         float z2 = z1 + sh.y * k;
@@ -1620,7 +1649,7 @@ __global__ __launch_bounds__(256) void k_strata_split(WS ws) {
                 if (fg.has) ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
                 uint32_t cb = (k == 0) ? c + 1u : (k == 1) ? c + F.W : (k == 2) ? c + F.W + 1u : c - F.W + 1u;
                 uint32_t cu = (av[k] < 0.0f) ? cb : c;
-                o[k] = band_split(F, cu, 0.5f * (F.h[c] + F.h[cb]), ff.y, ff.up, av[k]);
+                o[k] = band_split(F, cu, 0.5f * (F.h[c] + F.h[cb]), ff.y, ff.up, av[k], ff.n);
             }
         }
         for (int k = 0; k < 4; k++) F.acc_s[4u * c + k] = o[k];
@@ -2053,6 +2082,7 @@ __device__ inline float face_mob(const FilmDev& F, uint32_t ca, uint32_t cb, flo
     float wb = 1.0f - wa;
     float mu = cz_expf(wa * cz_logf(cz_max(ra.x, 1e-4f)) + wb * cz_logf(cz_max(rb.x, 1e-4f)));
     float ty = wa * ra.y + wb * rb.y;
+    const float nf = cell_n_on(F) ? wa * F.n_cell[ca] + wb * F.n_cell[cb] : F.P.phys[2];
     float thin = cz_min(ha, hb) / hf;
     float fcap = thin * thin * thin;
     float2 yc = yield_cap(F, make_float2(gcap.x * fcap, gcap.y * fcap), ty);
@@ -2060,7 +2090,7 @@ __device__ inline float face_mob(const FilmDev& F, uint32_t ca, uint32_t cb, flo
     float gmag = cz_len2(g.x, g.y);
     float ydep = (ty <= 0.0f) ? hf : ((g.x >= 0.0f) ? ha : hb);
     float qy, qu;
-    float q = hb_flux(F.P.phys[2], gmag, ydep, mu, ty, qy, qu);
+    float q = hb_flux(nf, gmag, ydep, mu, ty, qy, qu);
     if (q <= 0.0f) return 0.0f;
     float mob = cz_min(cz_max(q / gmag - F.P.misc[3], 0.0f), F.P.front[2]);
     if (cz_min(ha, hb) < 0.5f * cz_max(ha, hb)) {

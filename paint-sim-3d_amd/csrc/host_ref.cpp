@@ -396,7 +396,7 @@ void cz_update_params_host(const cz_settings* sp, const cz_overrides* ov, uint32
     P->ein[0] = fmaxf(htanf(theta), 0.05f); P->ein[1] = lc * hsinf(0.5f * theta); P->ein[2] = pi_a; P->ein[3] = s.h_min;
     P->cure[0] = s.cure_rate; P->cure[1] = s.tau_cure; P->cure[2] = s.lnmu_cure; P->cure[3] = s.h_star;
     P->misc[0] = 2.0e-6f; P->misc[1] = 0.12f; P->misc[2] = s.medium_mu; P->misc[3] = fminf(m_exp, s.m_cap_hydro);
-    P->flags[0] = s.debug_view; P->flags[1] = s.slump; P->flags[2] = 0; P->flags[3] = 0;
+    P->flags[0] = s.debug_view; P->flags[1] = s.slump; P->flags[2] = s.n_per_cell ? CZ_FLAG_CELL_N : 0u; P->flags[3] = 0;
     float lc_z = sqrtf(s.sigma / (s.rho * fmaxf(g[2], 0.1f)));
     float cap_explicit = fminf(m_exp, s.m_cap_hydro);
     float h_big = fmaxf(fmaxf(hs->h_max, hs->h_max_pending), 1e-4f);

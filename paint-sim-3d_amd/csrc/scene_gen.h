@@ -40,6 +40,7 @@ struct ScenePlan {
     float ltau0, ltau1;  // ln of the yield-stress range, Pa
     float lmu0, lmu1;    // ln of the viscosity range, Pa s
     float alpha;         // Dirichlet parameter
+    float n0, n1;        // per-cell power-law index range; n1 = 0: none
     float weave_amp, dx, wk;
     uint32_t sub[8];     // packed raw canvas reflectance
     SceneBlob blobs[SCENE_MAX_BLOBS];
@@ -139,7 +140,7 @@ CZ_HD inline void scene_cell(const ScenePlan& P, uint32_t i, uint32_t j, SceneCe
     frac[12] = 0.0f;
     frac[13] = tau;
     frac[14] = lmu + 10.0f;
-    frac[15] = 0.0f;
+    frac[15] = P.n1 > 0.0f ? P.n0 + (P.n1 - P.n0) * rng.uniform() : 0.0f;
     for (int q = 0; q < 8; q++) o.pig0[q] = cz_pack2(frac[2 * q], frac[2 * q + 1]);
     o.rheo[0] = cz_expf(cz_unpack_lo(o.pig0[7]) - 10.0f);
     o.rheo[1] = cz_unpack_hi(o.pig0[6]);

@@ -56,6 +56,8 @@ int make_plan(uint32_t w, uint32_t h, float dx, const cz_scene* s, ScenePlan& P)
     if (!(s->sigma_lo > 0.0f) || s->sigma_hi < s->sigma_lo || !(s->peak_lo > 0.0f) || s->peak_hi < s->peak_lo ||
         !(s->tau_lo > 0.0f) || s->tau_hi < s->tau_lo || !(s->mu_lo > 0.0f) || s->mu_hi < s->mu_lo || !(s->alpha > 0.0f))
         return set_err(CZ_EINVAL, "scene: bad ranges");
+    if (s->n_hi != 0.0f && !(s->n_lo > 0.0f && s->n_hi >= s->n_lo && s->n_hi <= 4.0f))
+        return set_err(CZ_EINVAL, "scene: power-law index range must lie in (0, 4]");
     memset(&P, 0, sizeof(P));
     P.seed = s->seed;
     P.nblobs = s->nblobs;
@@ -66,6 +68,8 @@ int make_plan(uint32_t w, uint32_t h, float dx, const cz_scene* s, ScenePlan& P)
     P.lmu0 = cz_logf(s->mu_lo);
     P.lmu1 = cz_logf(s->mu_hi);
     P.alpha = s->alpha;
+    P.n0 = s->n_hi != 0.0f ? s->n_lo : 0.0f;
+    P.n1 = s->n_hi;
     P.weave_amp = s->weave_um * 1e-6f;
     P.dx = dx;
     P.wk = (2.0f * 3.14159265358979323846f) / 1.4e-3f;

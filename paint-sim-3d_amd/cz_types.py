@@ -46,7 +46,7 @@ class Settings(C.Structure):
                 ("lnmu_cure", f32), ("v_cap", f32), ("tilt_deg", f32), ("tilt_dir_deg", f32), ("gravity", f32),
                 ("medium_mu", f32), ("debug_view", u32), ("h_min", f32), ("m_cap_hydro", f32),
                 ("max_substeps", u32), ("min_substeps", u32), ("implicit_iters", u32), ("implicit_hydro", u32),
-                ("slump", u32), ("capillary", u32), ("capillary_implicit", u32)]
+                ("slump", u32), ("capillary", u32), ("capillary_implicit", u32), ("n_per_cell", u32)]
 
 
 class Overrides(C.Structure):
@@ -67,7 +67,7 @@ class Stats(C.Structure):
 class Scene(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("nblobs", u32), ("sigma_lo", f32), ("sigma_hi", f32), ("peak_lo", f32),
                 ("peak_hi", f32), ("tau_lo", f32), ("tau_hi", f32), ("mu_lo", f32), ("mu_hi", f32),
-                ("alpha", f32), ("weave_um", f32)]
+                ("alpha", f32), ("weave_um", f32), ("n_lo", f32), ("n_hi", f32)]
 
 
 class Request(C.Structure):
@@ -126,6 +126,7 @@ def default_settings():
     s.debug_view, s.h_min, s.m_cap_hydro = 0, 1.5e-5, 1.0e-7
     s.max_substeps, s.min_substeps, s.implicit_iters, s.implicit_hydro = 64, 4, 40, 1
     s.slump, s.capillary, s.capillary_implicit = 0, 1, 0
+    s.n_per_cell = 0
     return s
 
 

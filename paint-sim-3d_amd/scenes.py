@@ -47,8 +47,11 @@ def blobs_settings(w, tilt_deg=15.0, dx=1.2 / 2048):
 
 
 def blobs_state(w, h, seed=1, nblobs=64, sigma_cells=(2.0, 40.0), peak=(20e-6, 2e-3), tau=(0.1, 50.0),
-                mu=(0.5, 50.0), dirichlet=0.3, weave=None, cell_m=None):
-    """State parts (dict of numpy arrays) of the plain blob canvas."""
+                mu=(0.5, 50.0), dirichlet=0.3, weave=None, cell_m=None, n_range=None):
+    """State parts (dict of numpy arrays) of the plain blob canvas.
+
+    n_range=(lo, hi) gives every wet cell its own power-law index, uniform in
+    [lo, hi], in pigment slot 15; a film reads it when n_per_cell is set."""
     rng = np.random.Generator(np.random.PCG64(seed))
     hgt = np.zeros((h, w), np.float64)
     cx = rng.uniform(0, w, nblobs)
@@ -80,6 +83,8 @@ def blobs_state(w, h, seed=1, nblobs=64, sigma_cells=(2.0, 40.0), peak=(20e-6, 2
     fw[:, :12] = conc
     fw[:, 13] = ty
     fw[:, 14] = np.log(mu_v) + 10.0
+    if n_range is not None:
+        fw[:, 15] = rng.uniform(n_range[0], n_range[1], nw).astype(np.float32)
     frac[wet] = fw
     pig = np.zeros((h, w, 4, 8), np.uint32)
     pig[:, :, 0, :] = pack_pig(frac)

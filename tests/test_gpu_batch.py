@@ -148,3 +148,36 @@ def test_batch_wide_shapes_equal_alone(gpu, shape):
     for k in range(3):
         alone[k].sync()
         assert_same(alone[k], inb[k], f"shape {shape} member {k}")
+
+
+def test_batch_cell_index_equals_alone(gpu):
+    """Members with an index per cell (settings.n_per_cell) beside one
+    without: each equal to its twin stepped alone, cell indices included."""
+    def make_n(k, per_cell):
+        w, h, s, st = scene(k)
+        if per_cell:
+            s.n_per_cell = 1
+            st = scenes.blobs_state(w, h, **dict(seed=k + 1, nblobs=8, sigma_cells=(2.0, 10.0),
+                                                  n_range=(0.4, 1.0)))
+        f = czf.Film(w, h, s)
+        assert f.write_state_arrays(st)
+        return f
+    plan = [(1, True), (3, False), (4, True)]
+    alone = [make_n(k, p) for k, p in plan]
+    inb = [make_n(k, p) for k, p in plan]
+    batch = czf.Batch(4)
+    for f in inb:
+        batch.add(f)
+    for frame in range(8):
+        for f in alone:
+            f.step()
+        batch.step()
+    batch.sync()
+    for i, (k, p) in enumerate(plan):
+        alone[i].sync()
+        assert_same(alone[i], inb[i], f"member {k}")
+        if p:
+            a, b = alone[i].buffer("n_cell"), inb[i].buffer("n_cell")
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), f"member {k}: cell indices differ"
+            wet = alone[i].buffer("h") > 0
+            assert a[wet].min() < 0.6 and a[wet].max() > 0.8

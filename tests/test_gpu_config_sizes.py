@@ -3,7 +3,8 @@ restatement (the HIP path through the C ABI on one side, the oracle on the
 other, same seeded inputs):
 
 - the plain 2048x2048 blob canvas (64 seeded blobs, 15 deg tilt) for two
-  frames, at the Newtonian n = 1 and at the shear-thinning n = 0.7;
+  frames, at the Newtonian n = 1, at the shear-thinning n = 0.7 and with an
+  index per wet cell drawn in [0.4, 1];
 - the headless jet scene at the reference's grid, 512x341 cells
   (`--auto 400 --script jet --res 512`, src/main.rs:222-223,1410-1432), for
   400 frames, with the mass report of src/main.rs:1679-1819 at the end;
@@ -53,12 +54,16 @@ def assert_same(a, b, tag):
             raise AssertionError(f"{tag}: {name} differs at {bad.size} words, first {bad[:6]}")
 
 
-@pytest.mark.parametrize("n_index", [1.0, 0.7])
+@pytest.mark.parametrize("n_index", [1.0, 0.7, "cell"])
 def test_plain_2048_two_frames(gpu, n_index):
     n = 2048
     s = scenes.blobs_settings(n, tilt_deg=15.0)
-    s.n_index = n_index
-    st = scenes.blobs_state(n, n, seed=2048)
+    if n_index == "cell":
+        s.n_per_cell = 1
+        st = scenes.blobs_state(n, n, seed=2048, n_range=(0.4, 1.0))
+    else:
+        s.n_index = n_index
+        st = scenes.blobs_state(n, n, seed=2048)
     g, o = make_pair(n, n, s, st)
     for f in range(2):
         g.step()

@@ -8,7 +8,8 @@ cure masks bit-exact; the pool8 maxima within the height tolerance.
 
 The blob scenes run at the reference's Newtonian index n = 1 and at the
 shear-thinning n = 0.7 and 0.4, where hb_flux and band_flux take their
-power-law branches (film.wgsl:94-126).
+power-law branches (film.wgsl:94-126), and with an index per cell drawn in
+[0.4, 1] (settings.n_per_cell), where the cell indices are compared too.
 """
 import numpy as np
 import pytest
@@ -37,6 +38,9 @@ def compare(g, o, tol, tag=""):
     rg, ro = g.buffer("rheo"), o.buf("rheo")
     e_r = rel_err(rg, ro, 1e-6)
     assert e_r <= max(tol, 1e-5) * 10, f"{tag}: rheo rel err {e_r:.3e}"
+    if o.settings.n_per_cell:
+        e_n = rel_err(g.buffer("n_cell"), o.buf("n_cell"), 1e-6)
+        assert e_n <= max(tol, 1e-5), f"{tag}: cell index rel err {e_n:.3e}"
     return e_h
 
 
@@ -198,3 +202,53 @@ def test_every_substep_shape(gpu, shape):
     run_pair(g, o, 12, check_every=4, tol=1e-5, tag=f"shape {shape}")
     used = list(g.stats().flux_shape_cell_substeps)
     assert used[shape] > 0 and sum(used) == used[shape], used
+
+
+def cell_n_settings(w, tilt_deg):
+    s = blob_settings_n(w, tilt_deg, 0.7)
+    s.n_per_cell = 1
+    return s
+
+
+def test_cell_index_flowing_blobs(gpu):
+    """Every wet cell with its own index in [0.4, 1]: faces between cells of
+    different indices, and the indices carried by the strata as the paint
+    runs, for 30 frames."""
+    w = h = 128
+    s = cell_n_settings(w, 45.0)
+    st = scenes.blobs_state(w, h, seed=13, nblobs=14, sigma_cells=(3.0, 12.0), peak=(2e-4, 2e-3),
+                            tau=(0.01, 0.3), mu=(0.5, 5.0), n_range=(0.4, 1.0))
+    g, o = make_pair(w, h, s, st)
+    run_pair(g, o, 30, check_every=10, tol=1e-5, tag="flowing cell n")
+    assert o.active_count() > 0
+    n = o.buf("n_cell")[o.buf("h") > 0]
+    assert n.min() < 0.6 and n.max() > 0.8, "the scene does not mix indices"
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
+def test_cell_index_every_substep_shape(gpu, shape):
+    w = h = 112
+    s = cell_n_settings(w, 35.0)
+    st = scenes.blobs_state(w, h, seed=14, nblobs=12, sigma_cells=(3.0, 12.0), peak=(2e-4, 2e-3),
+                            tau=(0.01, 0.3), mu=(0.5, 5.0), n_range=(0.4, 1.0))
+    g, o = make_pair(w, h, s, st)
+    g.flux_shape(shape)
+    run_pair(g, o, 12, check_every=4, tol=1e-5, tag=f"cell n shape {shape}")
+    used = list(g.stats().flux_shape_cell_substeps)
+    assert used[shape] > 0 and sum(used) == used[shape], used
+
+
+def test_cell_index_implicit_cg(gpu):
+    """The implicit hydrostatic solve's face mobilities with per-cell indices."""
+    w = h = 96
+    s = cell_n_settings(w, 5.0)
+    st = scenes.blobs_state(w, h, seed=5, nblobs=6, sigma_cells=(6.0, 12.0), peak=(4e-3, 6e-3),
+                            mu=(0.05, 0.1), tau=(1e-3, 2e-3), n_range=(0.4, 1.0))
+    g, o = make_pair(w, h, s, st)
+    for _ in range(3):
+        g.step()
+        o.step()
+    g.sync()
+    assert o.implicit_steps() > 0, "scene does not exercise the implicit solve"
+    compare(g, o, 1e-5, "implicit cell n")
+    compare_masks(g, o, "implicit cell n")

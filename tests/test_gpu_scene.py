@@ -11,11 +11,12 @@ import scenes
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("w,h,seed", [(200, 150, 21), (257, 64, 22)])
-def test_device_scene_equals_host(gpu, w, h, seed):
+@pytest.mark.parametrize("w,h,seed,n_hi", [(200, 150, 21, 0.0), (257, 64, 22, 0.0), (192, 160, 23, 1.0)])
+def test_device_scene_equals_host(gpu, w, h, seed, n_hi):
     s = scenes.blobs_settings(w, tilt_deg=10.0)
+    s.n_per_cell = 1 if n_hi else 0
     f = czf.Film(w, h, s)
-    sc = czf.scene(seed=seed, nblobs=24, sigma_lo=2.0, sigma_hi=12.0)
+    sc = czf.scene(seed=seed, nblobs=24, sigma_lo=2.0, sigma_hi=12.0, n_lo=0.4 if n_hi else 0.0, n_hi=n_hi)
     f.seed_blobs(sc)
     hv, mv, rv, pv = czf.scene_cells_host(w, h, f.dx, sc)
     assert np.array_equal(f.buffer("h"), hv)

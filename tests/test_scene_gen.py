@@ -87,3 +87,28 @@ def test_deterministic_and_row_split():
         assert np.array_equal(x, np.concatenate([y, z], axis=0))
     d = czf.scene_cells_host(96, 80, DX, czf.scene(seed=6))
     assert not np.array_equal(a[0], d[0])
+
+
+def test_cell_index_range():
+    """n_hi > 0: every wet cell draws its power-law index uniform in
+    [n_lo, n_hi] into slot 15 (one more draw after the viscosity, so the
+    other fields are those of the same scene without it)."""
+    sc0 = czf.scene(seed=11)
+    sc1 = czf.scene(seed=11, n_lo=0.4, n_hi=1.0)
+    h0, m0, r0, p0 = czf.scene_cells_host(160, 144, DX, sc0)
+    h1, m1, r1, p1 = czf.scene_cells_host(160, 144, DX, sc1)
+    assert np.array_equal(h0, h1) and np.array_equal(r0, r1)
+    f0, f1 = decode(p0), decode(p1)
+    assert np.array_equal(f0[..., :15], f1[..., :15])
+    wet = h1 > 0
+    n = f1[wet][:, 15]
+    assert (f0[..., 15] == 0).all() and (f1[~wet][:, 15] == 0).all()
+    assert n.min() >= 0.4 - 2e-4 and n.max() <= 1.0
+    u = (n - 0.4) / 0.6
+    assert np.allclose(np.quantile(u, [0.25, 0.5, 0.75]), [0.25, 0.5, 0.75], atol=0.02)
+
+
+@pytest.mark.parametrize("lo,hi", [(0.0, 1.0), (0.5, 0.4), (0.5, 5.0)])
+def test_cell_index_range_rejected(lo, hi):
+    with pytest.raises(czf.FilmError):
+        czf.scene_cells_host(16, 16, DX, czf.scene(seed=1, n_lo=lo, n_hi=hi))
