@@ -2291,7 +2291,8 @@ static double total_parts(const ofilm* F, const double* part) {
     return v;
 }
 
-static void cg_solve(ofilm* F, uint32_t iters) {
+/* cg_lap_h, cg_init and `iters` CG iterations (film.wgsl:1068-1259) */
+static void cg_iterate(ofilm* F, uint32_t iters) {
     /* cg_lap_h (film.wgsl:1068-1075) */
     FOR_ACTIVE(F, {
         F->cg_t[c] = lap_w(F, x, y, c, 1u);
@@ -2403,6 +2404,10 @@ static void cg_solve(ofilm* F, uint32_t iters) {
             F->cg_p[c] = F->cg_r[c] / F->cg_d[c] + beta * F->cg_p[c];
         })
     }
+}
+
+/* cg_lap_u, cg_finish, cg_reduce_mass, cg_rescale (film.wgsl:1077-1085,1205-1302) */
+static void cg_finish(ofilm* F) {
     /* cg_lap_u */
     FOR_ACTIVE(F, { F->cg_t[c] = lap_w(F, x, y, c, 4u); })
     /* cg_finish (film.wgsl:1261-1302), neighbours' h from before the pass */
@@ -2464,6 +2469,11 @@ static void cg_solve(ofilm* F, uint32_t iters) {
         float fct = 1.0f - s1y / s1z;
         F->h[c] = cz_max(F->h[c] * fct, 0.0f);
     })
+}
+
+static void cg_solve(ofilm* F, uint32_t iters) {
+    cg_iterate(F, iters);
+    cg_finish(F);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2970,6 +2980,10 @@ void* of_buffer(ofilm* F, int which, uint64_t* bytes) {
     case CZ_BUF_TILE_LIST: *bytes = 4 * (uint64_t)F->nt16; return F->tile_list;
     case CZ_BUF_ACC_S: *bytes = 64 * n; return F->acc_s;
     case CZ_BUF_ALPHA_S: *bytes = 16 * n; return F->alpha_s;
+    /* the CG vectors (restatement only) */
+    case 100: *bytes = 4 * n; return F->cg_x;
+    case 101: *bytes = 4 * n; return F->cg_d;
+    case 102: *bytes = 4 * n; return F->cg_r;
     default: *bytes = 0; return NULL;
     }
 }
@@ -3050,6 +3064,54 @@ void of_k_laplacian(ofilm* F) { k_laplacian(F); }
 void of_k_plug(ofilm* F) { k_plug(F); }
 void of_k_fluxes(ofilm* F) { k_fluxes(F); }
 void of_k_cure(ofilm* F) { k_cure_step(F); }
+
+/* Single functions and passes of the frame, for the closed-form and
+ * brute-force checks of the restatement (tests only). */
+uint32_t of_fn_feeders(const ofilm* F, uint32_t c) { return feeders(F, c); }
+void of_fn_front_normal(const ofilm* F, uint32_t c, float* o) {
+    v2 n = front_normal(F, c);
+    o[0] = n.x;
+    o[1] = n.y;
+}
+float of_fn_wet_gain(float nx, float ny) {
+    v2 n = {nx, ny};
+    return wet_gain(n);
+}
+float of_fn_front_shape(const ofilm* F, uint32_t c, float nx, float ny) {
+    v2 n = {nx, ny};
+    return front_shape(F, c, n);
+}
+float of_fn_wet_threshold(const ofilm* F, uint32_t c) { return wet_threshold(F, c); }
+float of_fn_hold_level(const ofilm* F, float d) { return hold_level(F, d); }
+float of_fn_cox_speed(const ofilm* F, float excess, float mu) { return cox_speed(F, excess, mu); }
+float of_fn_cell_excess(const ofilm* F, uint32_t c) { return cell_excess(F, c); }
+float of_fn_rim_translation(const ofilm* F, uint32_t ca, uint32_t cb, float hf, float fx, float fy, float* tsign) {
+    v2 d = {fx, fy};
+    return rim_translation(F, ca, cb, hf, d, tsign);
+}
+void of_k_nose(ofilm* F) { k_nose(F); }
+void of_k_limiter(ofilm* F) { k_limiter(F); }
+void of_k_apply(ofilm* F) { k_apply(F); }
+void of_k_clear_acc(ofilm* F) { k_clear_acc(F); }
+void of_k_strata(ofilm* F, int which) {
+    switch (which) {
+    case 0: k_strata_split(F); break;
+    case 1: k_strata_limit(F); break;
+    case 2: k_strata_update(F); break;
+    case 3: k_strata_copy_back(F); break;
+    default: k_strata_fill(F); break;
+    }
+}
+void of_k_cg_setup(ofilm* F) { k_cg_setup(F); }
+void of_cg_iterate(ofilm* F, uint32_t iters) { cg_iterate(F, iters); }
+/* q = A p of the implicit system (cg_lap_p + the operator of cg_iter_a) on
+ * the cells of the active tiles; p and q cover the whole canvas */
+void of_cg_apply(ofilm* F, const float* p, float* q) {
+    memcpy(F->cg_p, p, F->n * sizeof(float));
+    FOR_ACTIVE(F, { F->cg_t[c] = lap_w(F, x, y, c, 0u); })
+    memset(q, 0, F->n * sizeof(float));
+    FOR_ACTIVE(F, { q[c] = apply_a(F, x, y, c); })
+}
 
 /* Elementary functions and conversions, exported for the math checks. */
 float of_expf(float x) { return cz_expf(x); }

@@ -87,8 +87,28 @@ def lib():
         L.of_take_reservoir.argtypes = [P, C.POINTER(u32)]
         L.of_brush_pass.argtypes = [P]
         L.of_knife_pass.argtypes = [P]
-        for k in ("of_k_mark_compact", "of_k_laplacian", "of_k_plug", "of_k_fluxes", "of_k_cure"):
+        for k in ("of_k_mark_compact", "of_k_laplacian", "of_k_plug", "of_k_fluxes", "of_k_cure", "of_k_nose",
+                  "of_k_limiter", "of_k_apply", "of_k_clear_acc", "of_k_cg_setup"):
             getattr(L, k).argtypes = [P]
+        L.of_k_strata.argtypes = [P, C.c_int]
+        L.of_cg_iterate.argtypes = [P, u32]
+        L.of_cg_apply.argtypes = [P, C.POINTER(f32), C.POINTER(f32)]
+        L.of_fn_feeders.restype = u32
+        L.of_fn_feeders.argtypes = [P, u32]
+        L.of_fn_front_normal.argtypes = [P, u32, C.POINTER(f32)]
+        L.of_fn_wet_gain.restype = f32
+        L.of_fn_wet_gain.argtypes = [f32, f32]
+        L.of_fn_front_shape.restype = f32
+        L.of_fn_front_shape.argtypes = [P, u32, f32, f32]
+        for k in ("of_fn_wet_threshold", "of_fn_cell_excess"):
+            getattr(L, k).restype = f32
+            getattr(L, k).argtypes = [P, u32]
+        L.of_fn_hold_level.restype = f32
+        L.of_fn_hold_level.argtypes = [P, f32]
+        L.of_fn_cox_speed.restype = f32
+        L.of_fn_cox_speed.argtypes = [P, f32, f32]
+        L.of_fn_rim_translation.restype = f32
+        L.of_fn_rim_translation.argtypes = [P, u32, u32, f32, f32, f32, C.POINTER(f32)]
         for name in ("of_expf", "of_logf", "of_atanf"):
             getattr(L, name).restype = f32
             getattr(L, name).argtypes = [f32]
@@ -148,6 +168,7 @@ BUFFERS = {
     "acc_s": (31, np.float32, (4, 4)), "alpha_s": (32, np.float32, (4,)),
     "jmeta": (38, np.uint32, None), "jmark": (39, np.uint32, None), "jdata": (40, np.uint32, None),
     "n_cell": (41, np.float32, ()),
+    "cg_x": (100, np.float32, ()), "cg_d": (101, np.float32, ()), "cg_r": (102, np.float32, ()),
 }
 
 

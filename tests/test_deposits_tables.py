@@ -7,7 +7,8 @@
 - The deposit kernel's 4x4 sub-samples coincide with the host lattice, so the
   volume a frame adds equals `vol` for an interior deposit (README.md:75).
 - default_pigments / pigment_table against the reference's assets/pigments.ron
-  (tests/golden/pigments_ron.json); cmf_table normalisation.
+  (tests/golden/pigments_ron.json); cmf_table against the reference's CMF
+  and D65 data (tests/golden/cmf_d65.json).
 - The product's host functions (libcanvaz_film.so) against the restatement:
   update_params, choose_substeps, deposits, weave, substrate, bit for bit.
 """
@@ -180,11 +181,29 @@ def test_default_pigments_match_reference_asset():
 
 
 def test_cmf_table():
+    """cmf_table (src/color/mod.rs:145-168) from the reference's CMF and D65
+    data (fixture tests/golden/cmf_d65.json): band b sums samples 2b and
+    2b + 1 weighted by D65 in f32, normalised by the f32 sum of y over the
+    bands.  A swapped column, a shifted sample or another illuminant fails."""
+    g = json.load(open(os.path.join(GOLDEN, "cmf_d65.json")))
+    cols = [np.array(g[k], np.float32) for k in ("xbar", "ybar", "zbar")]
+    d65 = np.array(g["d65"], np.float32)
+    want = np.zeros((16, 3), np.float32)
+    ysum = f32(0.0)
+    for b in range(16):
+        for q in range(3):
+            acc = f32(0.0)
+            for s in range(2):
+                acc = f32(acc + f32(cols[q][2 * b + s] * d65[2 * b + s]))
+            want[b, q] = acc
+        ysum = f32(ysum + want[b, 1])
+    want = (want / ysum).astype(np.float32)
     t = czf.cmf_table().reshape(16, 3)
+    assert np.array_equal(t, want)
     assert t[:, 1].astype(np.float64).sum() == pytest.approx(1.0, rel=1e-6)
     o = np.zeros(48, np.float32)
     OL.of_cmf_table(o.ctypes.data_as(C.POINTER(C.c_float)))
-    assert np.array_equal(o.reshape(16, 3), t)
+    assert np.array_equal(o.reshape(16, 3), want)
 
 
 def test_host_params_match_restatement():
