@@ -474,14 +474,17 @@ def serve_trace(seconds, rate=200.0, seed=5, cell_n=True):
     return reqs
 
 
-def bench_serve(args):
-    """The serving loop on a Poisson trace: latency = completion - arrival."""
+def serve_once(args, rate, max_wait):
+    """One run of the serving loop on a Poisson trace at `rate` requests per
+    second: latency = completion - arrival."""
     import numpy as np
     import canvaz_film as czf
-    reqs = serve_trace(args.serve_seconds, cell_n=args.index == "cell")
+    reqs = serve_trace(args.serve_seconds, rate=rate, cell_n=args.index == "cell")
     offered = sum(r.w * r.h * r.steps for r in reqs) / args.serve_seconds
-    log(f"bench serve: {len(reqs)} requests over {args.serve_seconds} s, offered {offered:.3e} cell-steps/s")
-    srv = czf.Server(hbm_budget=args.serve_budget_gb << 30, max_members=args.serve_members)
+    policy = "first come first served" if max_wait <= 0 else f"least work first, aged after {max_wait} s (evicting)"
+    log(f"bench serve: {len(reqs)} requests at {rate} req/s over {args.serve_seconds} s, offered {offered:.3e} "
+        f"cell-steps/s, {policy}")
+    srv = czf.Server(hbm_budget=args.serve_budget_gb << 30, max_members=args.serve_members, max_wait_s=max_wait)
     res, st = srv.run(reqs)
     srv.close()
     ok = [r for r in res if r.status == 0]
@@ -495,12 +498,25 @@ def bench_serve(args):
            "throughput_cell_steps_per_s": cs / st.wall_s, "wall_s": st.wall_s,
            "offered_cell_steps_per_s": offered, "frames": st.frames, "peak_members": st.peak_members,
            "peak_gb": st.peak_bytes / 2 ** 30, "films_created": st.films_created, "films_reused": st.films_reused,
-           "workload": f"Poisson 200 req/s for {args.serve_seconds} s, side log-uniform [512, 4096], steps "
+           "evictions": st.evictions, "resumes": st.resumes, "peak_host_gb": st.peak_host_bytes / 2 ** 30,
+           "policy": policy,
+           "workload": f"Poisson {rate} req/s for {args.serve_seconds} s, side log-uniform [512, 4096], steps "
                        f"Pareto(1.5, min 10, max 2000), seeded blob canvases ({index_words(args.index)}), "
                        f"budget {args.serve_budget_gb} GB, "
                        f"at most {args.serve_members} canvases per batch"}
     log(f"bench serve: p50 {out['p50_latency_s']:.3f} s, p99 {out['p99_latency_s']:.3f} s, "
-        f"{out['throughput_cell_steps_per_s']:.3e} cell-steps/s over {st.wall_s:.1f} s")
+        f"{out['throughput_cell_steps_per_s']:.3e} cell-steps/s over {st.wall_s:.1f} s, "
+        f"{st.evictions} evictions")
+    return out
+
+
+def bench_serve(args):
+    """The serving workload at its stated 200 req/s (several times what one
+    GPU serves, so latency is mostly queueing) under the scheduling policy,
+    and the same trace shape at a rate one GPU sustains."""
+    out = serve_once(args, args.serve_rate, args.serve_max_wait)
+    if args.serve_sustainable_rate > 0:
+        out["sustainable"] = serve_once(args, args.serve_sustainable_rate, args.serve_max_wait)
     return out
 
 
@@ -594,6 +610,12 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--serve-budget-gb", type=int, default=200)
     ap.add_argument("--serve-members", type=int, default=256)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="arrivals per second of the serving workload")
+    ap.add_argument("--serve-max-wait", type=float, default=0.0,
+                    help="serving policy: 0 first come first served; > 0 least work first, requests waiting "
+                         "longer than this (s) first, evicting long-running canvases to host memory")
+    ap.add_argument("--serve-sustainable-rate", type=float, default=30.0,
+                    help="a second serving run at this arrival rate (0: none)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload in the film run")
     ap.add_argument("--batch-canvases", type=int, default=64)
     ap.add_argument("--batch-frames", type=int, default=200)

@@ -592,12 +592,20 @@ void* cz_batch_stream(cz_batch* b);
 
 /* ---- Serving ---------------------------------------------------------------
  * A loop that admits step requests (a seeded canvas and a frame count) as
- * they arrive, first come first served, under a device-memory budget and a
- * batch-size limit, steps all admitted canvases together (cz_batch; each
- * canvas exactly as alone) and retires each when its frames are done;
- * canvases stay resident from admission to completion.  No counterpart in
- * the reference (one canvas per App); its per-canvas step is encode_sim
- * (src/main.rs:1875-1909). */
+ * they arrive, under a device-memory budget and a batch-size limit, steps all
+ * admitted canvases together (cz_batch; each canvas exactly as alone) and
+ * retires each when its frames are done.  With max_wait_s = 0 admission is
+ * first come, first served, and canvases stay resident from admission to
+ * completion.  With max_wait_s > 0 the waiting canvas with the least work
+ * left (cells x frames) goes first, and any that fits is admitted past one
+ * that does not; a request or evicted canvas waiting longer than max_wait_s
+ * goes before all others and, when it does not fit, running canvases that
+ * have run for at least max_wait_s since their admission and have more work
+ * left than it are evicted to host memory (most work left first, within
+ * host_budget) until it does.  An evicted canvas keeps all
+ * it carries between frames (film_snapshot) and resumes exactly where it
+ * stopped.  No counterpart in the reference (one canvas per App); its
+ * per-canvas step is encode_sim (src/main.rs:1875-1909). */
 typedef struct cz_request {
     uint64_t id;
     uint32_t w, h;
@@ -618,13 +626,16 @@ typedef struct cz_server_config {
     uint64_t hbm_budget;     /* bytes of films held at once (active and pooled) */
     uint32_t max_members;    /* canvases stepped together, <= 1024 */
     float frame_dt;
-    int32_t keep_films;      /* 1: keep finished canvases for cz_server_film */
-    uint32_t pad;
+    int32_t keep_films;      /* 1: keep finished canvases for cz_server_film (outside the budget) */
+    float max_wait_s;        /* 0: first come, first served; > 0: least work first, see above */
+    uint64_t host_budget;    /* bytes of evicted canvas state held in host memory at once */
 } cz_server_config;
 typedef struct cz_server_stats {
     double wall_s;
     uint64_t frames, admitted, cell_steps, peak_members, peak_bytes;
     uint64_t films_created, films_reused;
+    uint64_t evictions, resumes;   /* canvases moved to host memory and back */
+    uint64_t peak_host_bytes;      /* evicted canvas state held at once */
 } cz_server_stats;
 typedef struct cz_server cz_server;
 void cz_server_config_default(cz_server_config* c);

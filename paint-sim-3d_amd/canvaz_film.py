@@ -725,13 +725,20 @@ class _Borrowed(Film):
 
 
 class Server:
-    """The serving loop: requests of seeded canvases, admitted by arrival
-    time under a device-memory budget, stepped together, retired when done."""
+    """The serving loop: requests of seeded canvases, admitted under a
+    device-memory budget (first come first served with max_wait_s = 0, else
+    least work first with aged requests first, evicting long-running canvases
+    to host memory when an aged one does not fit), stepped together, retired
+    when done."""
 
-    def __init__(self, hbm_budget=200 << 30, max_members=256, frame_dt=1.0 / 60.0, keep_films=False):
+    def __init__(self, hbm_budget=200 << 30, max_members=256, frame_dt=1.0 / 60.0, keep_films=False, max_wait_s=0.0,
+                 host_budget=None):
         cfg = ServerConfig()
         lib().cz_server_config_default(C.byref(cfg))
         cfg.hbm_budget, cfg.max_members, cfg.frame_dt, cfg.keep_films = hbm_budget, max_members, frame_dt, int(keep_films)
+        cfg.max_wait_s = max_wait_s
+        if host_budget is not None:
+            cfg.host_budget = host_budget
         ptr = C.c_void_p()
         check(lib().cz_server_create(C.byref(cfg), C.byref(ptr)))
         self._p = ptr

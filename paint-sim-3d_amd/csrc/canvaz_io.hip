@@ -282,14 +282,9 @@ int cz_film_save_canvaz(cz_film* f, const char* path) {
     }
     rc = cz_film_read_state(f, ptrs);
     if (rc) return rc;
-    // s_max: the largest finite surface value of the last pooled frame
-    // (poll_stats, src/film/mod.rs:956-962)
-    std::vector<float> pool((size_t)2 * f->pw * f->ph);
-    rc = cz_film_pool(f, pool.data(), pool.size());
-    if (rc) return rc;
-    float s_max = 0.0f;
-    for (size_t k = 0; k < pool.size() / 2; k++)
-        if (std::isfinite(pool[k])) s_max = std::max(s_max, pool[k]);
+    // s_max as the last poll left it: the largest finite surface value of the
+    // polled frame's pool (poll_stats, src/film/mod.rs:956-962)
+    const float s_max = f->s_max;
     std::vector<uint8_t> out;
     put_u32(out, kMagic);
     put_u32(out, kVersion);

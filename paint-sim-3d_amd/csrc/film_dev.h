@@ -74,6 +74,7 @@ struct FilmDev {
     uint32_t* cure_list;      // tiles with cells for the wet cure branch
     uint32_t* cure_count;
     float2* stats_out;
+    float* smax_out;       // [tiles64] largest finite surface of the 64-tile's pool cells
     float* pool_out;
     float* tsum_a;        // strata sums, fill pass input
     const float* pigtab;

@@ -108,6 +108,7 @@ void update_params(cz_film* f, float frame_dt, uint32_t substeps) {
 void fold_frame_out(cz_film* f, const float* o) {
     f->h_max = o[0];
     f->mu_min = (o[1] > 1e8f) ? 1.0f : o[1];
+    f->s_max = o[8];
     f->active_tiles = cz_f2bits(o[2]);
     f->active_hint = f->active_tiles;
     for (int k = 0; k < 4; k++) f->cg_report[k] = o[4 + k];
@@ -612,7 +613,7 @@ bool set_dims(cz_film* f, uint32_t w, uint32_t h) {
 // Host state of a film that starts afresh (Film::new): lag-1 stats,
 // counters, parities, params of update_params(1/60, 1).
 void reset_host_state(cz_film* f) {
-    f->h_max = 0.0f; f->mu_min = 1.0f; f->h_max_pending = 0.0f;
+    f->h_max = 0.0f; f->s_max = 0.0f; f->mu_min = 1.0f; f->h_max_pending = 0.0f;
     f->implicit_steps = 1; f->substeps_last = 1; f->implicit_last = 0; f->frame = 0; f->active_tiles = 0;
     f->active_hint = f->nt16;
     f->total_deposited = 0.0;
@@ -675,7 +676,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     A(acc_s, 4 * n); A(alpha_s, n); A(strata_h2, n); A(strata_pig2, 8 * n); A(strata_cure2, n); A(h_frame, n);
     A(fnorm, n); A(cg_m, n); A(cg_d, n); A(cg_x, n); A(cg_r, n); A(cg_p, n); A(cg_q, n); A(cg_t, n); A(cg_t2, n);
     A(cg_part, nt16); A(cg_part2, nt16); A(cg_s, 8); A(tile_flags, nt16); A(tile_list, nt16); A(tile_moved, nt16);
-    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(tile_bits, nt16); A(dep_rect, 4 * (size_t)CZ_MAX_DEPOSITS); A(dep_cnt, nt64); A(dep_n, 4); A(cmf, CZ_NBAND * 3); A(compact_cnt, 2 * ((nt16 + 255) / 256) + 2); A(stats_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
+    A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(tile_bits, nt16); A(dep_rect, 4 * (size_t)CZ_MAX_DEPOSITS); A(dep_cnt, nt64); A(dep_n, 4); A(cmf, CZ_NBAND * 3); A(compact_cnt, 2 * ((nt16 + 255) / 256) + 2); A(stats_out, nt64); A(smax_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
     A(pigtab, CZ_NPIG * CZ_PIG_STRIDE); A(deposits, CZ_MAX_DEPOSITS); A(tile_start, nt64); A(tile_count, nt64);
     A(tile_items, CZ_MAX_TILE_ITEMS); A(frame_out, 16);
     A(mass_ws, (cz_mass_sums_workspace(n) + 15) / 16); A(mass_out, 1); A(n_cell, n);
@@ -736,6 +737,106 @@ int film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s) {
     rc = journal_reset(f);
     if (!rc) rc = tools_reset(f);
     if (rc) return rc;
+    return wait_idle(f);
+}
+
+namespace {
+struct SnapPart {
+    void* p;
+    uint64_t b;
+};
+std::vector<SnapPart> snapshot_parts(cz_film* f) {
+    const uint64_t n = f->n;
+    std::vector<SnapPart> v;
+    for (int k = 0; k < CZ_NPARTS; k++) {
+        BufDesc d = buffer_of(f, k);
+        v.push_back({d.ptr, d.bytes});
+    }
+    v.push_back({f->delta_a, 4 * n});
+    v.push_back({f->delta_b, 4 * n});
+    v.push_back({f->fnorm, 8 * n});
+    v.push_back({f->pres, 8 * n});
+    v.push_back({f->tmp, 4 * n});
+    v.push_back({f->h_frame, 4 * n});
+    v.push_back({f->cg_m, 16 * n});
+    v.push_back({f->cg_p, 4 * n});
+    v.push_back({f->acc, 16 * n});
+    v.push_back({f->n_cell, 4 * n});
+    v.push_back({f->tile_moved, 4 * (uint64_t)f->nt16});
+    v.push_back({f->tile_flags, 4 * (uint64_t)f->nt16});
+    return v;
+}
+}  // namespace
+
+uint64_t film_snapshot_bytes(cz_film* f) {
+    uint64_t total = 0;
+    for (const SnapPart& q : snapshot_parts(f)) total += q.b;
+    return total;
+}
+
+int film_snapshot(cz_film* f, FilmSnapshot& out) {
+    if (f->batch) return set_err(CZ_EINVAL, "snapshot: the film is a batch member");
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    if (f->stats_pending) fold_frame_out(f, f->frame_out_h);
+    const std::vector<SnapPart> parts = snapshot_parts(f);
+    uint64_t total = 0;
+    for (const SnapPart& q : parts) total += q.b;
+    out.data.resize(total);
+    uint64_t off = 0;
+    for (const SnapPart& q : parts) {
+        HIP_TRY(hipMemcpyAsync(out.data.data() + off, q.p, q.b, hipMemcpyDeviceToHost, f->stream));
+        off += q.b;
+    }
+    out.W = f->W;
+    out.H = f->H;
+    out.s = f->s;
+    out.h_max = f->h_max;
+    out.mu_min = f->mu_min;
+    out.h_max_pending = f->h_max_pending;
+    out.s_max = f->s_max;
+    out.implicit_steps = f->implicit_steps;
+    out.substeps_last = f->substeps_last;
+    out.implicit_last = f->implicit_last;
+    out.frame = f->frame;
+    out.active_tiles = f->active_tiles;
+    out.active_hint = f->active_hint;
+    out.delta_cur_a = f->delta_cur_a;
+    out.total_deposited = f->total_deposited;
+    out.weave_volume = f->weave_volume;
+    return wait_idle(f);
+}
+
+int film_restore(cz_film* f, const FilmSnapshot& snap) {
+    if (f->batch) return set_err(CZ_EINVAL, "restore: the film is a batch member");
+    if (f->W != snap.W || f->H != snap.H) return set_err(CZ_ESIZE, "restore: the film is %ux%u, the snapshot %ux%u",
+                                                         f->W, f->H, snap.W, snap.H);
+    int rc = wait_idle(f);
+    if (rc) return rc;
+    const std::vector<SnapPart> parts = snapshot_parts(f);
+    uint64_t total = 0;
+    for (const SnapPart& q : parts) total += q.b;
+    if (total != snap.data.size()) return set_err(CZ_ESIZE, "restore: snapshot of %llu bytes, the film needs %llu",
+                                                  (unsigned long long)snap.data.size(), (unsigned long long)total);
+    uint64_t off = 0;
+    for (const SnapPart& q : parts) {
+        HIP_TRY(hipMemcpyAsync(q.p, snap.data.data() + off, q.b, hipMemcpyHostToDevice, f->stream));
+        off += q.b;
+    }
+    f->h_max = snap.h_max;
+    f->mu_min = snap.mu_min;
+    f->h_max_pending = snap.h_max_pending;
+    f->s_max = snap.s_max;
+    f->implicit_steps = snap.implicit_steps;
+    f->substeps_last = snap.substeps_last;
+    f->implicit_last = snap.implicit_last;
+    f->frame = snap.frame;
+    f->active_tiles = snap.active_tiles;
+    f->active_hint = snap.active_hint;
+    f->delta_cur_a = snap.delta_cur_a;
+    f->total_deposited = snap.total_deposited;
+    f->weave_volume = snap.weave_volume;
+    f->stats_pending = false;
     return wait_idle(f);
 }
 
@@ -889,7 +990,6 @@ int cz_film_mass_report(cz_film* f, cz_mass_report* r) {
     rc = wait_idle(f);
     if (rc) return rc;
     // src/main.rs:1721-1726,1812-1818
-    f->s_max = ms.s_max;
     const double dx = (double)f->dx;
     const double vol = ms.sum_h * dx * dx, volm = ms.sum_m * dx * dx - f->weave_volume;
     r->wet_ml = vol * 1e6;
@@ -1008,6 +1108,7 @@ int cz_film_clear(cz_film* f) {
     if (rc) return rc;
     f->total_deposited = 0.0;
     f->h_max = 0.0f;
+    f->s_max = 0.0f;
     f->h_max_pending = 0.0f;
     return CZ_OK;
 }

@@ -257,6 +257,7 @@ struct cz_film {
     int bin_mode = -1;             // deposit binning: -1 by count, 0 host, 1 device
     bool bin_on_device = false;    // the uploaded deposits are binned in the frame
     float2* stats_out = nullptr;
+    float* smax_out = nullptr;
     float* n_cell = nullptr;         // [n] per-cell power-law index (settings.n_per_cell)
     uint4* mass_ws = nullptr;        // [cz_mass_sums_workspace(cap_n) / 16] partials of the mass report
     cz_mass_totals* mass_out = nullptr;  // [1] its result
@@ -305,7 +306,7 @@ struct cz_film {
         F.tile_flags = tile_flags; F.tile_list = tile_list; F.tile_moved = tile_moved; F.active_count = active_count;
         F.cure_list = cure_list; F.cure_count = cure_count;
         F.n_cell = n_cell;
-        F.stats_out = stats_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.tile_bits = tile_bits; F.compact_cnt = compact_cnt; F.dep_rect = dep_rect; F.dep_cnt = dep_cnt; F.dep_n = dep_n; F.cmf = cmf; F.color = color; F.detail = detail; F.pigtab = pigtab;
+        F.stats_out = stats_out; F.smax_out = smax_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.tile_bits = tile_bits; F.compact_cnt = compact_cnt; F.dep_rect = dep_rect; F.dep_cnt = dep_cnt; F.dep_n = dep_n; F.cmf = cmf; F.color = color; F.detail = detail; F.pigtab = pigtab;
         F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
         F.frame_out = frame_out;
         return F;
@@ -357,6 +358,25 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
 // Re-dimension a film within the capacity it was created with, as a new
 // Film with settings s (host state fresh; the caller writes the state).
 int film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s);
+// Everything a canvas carries from one frame to the next, on the host: the
+// state parts, the solver buffers later frames read where they did not write
+// them (both nose offset buffers, front normals, pressures, plug means, the
+// frame heights, CG mobilities and directions, flux sums, tile marks, cell
+// indices) and the host scalars that choose the next frame's substeps.  A
+// film restored from it (after film_reshape to the same size and settings)
+// steps on exactly as the film it was taken from.
+struct FilmSnapshot {
+    uint32_t W = 0, H = 0;
+    cz_settings s;
+    std::vector<uint8_t> data;
+    float h_max = 0.0f, mu_min = 1.0f, h_max_pending = 0.0f, s_max = 0.0f;
+    uint32_t implicit_steps = 1, substeps_last = 1, implicit_last = 0, frame = 0, active_tiles = 0, active_hint = 0;
+    bool delta_cur_a = true;
+    double total_deposited = 0.0, weave_volume = 0.0;
+};
+int film_snapshot(cz_film* f, FilmSnapshot& out);
+uint64_t film_snapshot_bytes(cz_film* f);
+int film_restore(cz_film* f, const FilmSnapshot& snap);
 // Fold the tuner timings of finished frames (events already complete).
 void fold_tuners(std::vector<TunerSlot>& pending, std::vector<hipEvent_t>& pool);
 // Take a film out of its batch (before it is destroyed).

@@ -1674,6 +1674,7 @@ template <class WS>
 __global__ __launch_bounds__(256) void k_reduce_pool(WS ws) {
     __shared__ float s_h[4];
     __shared__ float s_mu[4];
+    __shared__ float s_s[4];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     CZ_FOR_TILES(ws) {
         const uint32_t lx = _lx, ly = _ly;
@@ -1709,27 +1710,34 @@ __global__ __launch_bounds__(256) void k_reduce_pool(WS ws) {
             }
         }
         const uint32_t pi = bx * 16u + lx, pj = by * 16u + ly;
+        // s_max of poll_stats (src/film/mod.rs:961): the largest finite pool surface
+        float sm = 0.0f;
         if (pi < F.pw && pj < F.ph) {
             F.pool_out[pj * F.pw + pi] = mx;
             F.pool_out[F.pw * F.ph + pj * F.pw + pi] = ms;
+            if (isfinite(mx)) sm = mx;
         }
         hm = wave_max(hm);
         mm = wave_min(mm);
+        sm = wave_max(sm);
         __syncthreads();
         if (lane == 0u) {
             s_h[wave] = hm;
             s_mu[wave] = mm;
+            s_s[wave] = sm;
         }
         __syncthreads();
-        if (threadIdx.x == 0)
+        if (threadIdx.x == 0) {
             F.stats_out[_t] = make_float2(cz_max(cz_max(s_h[0], s_h[1]), cz_max(s_h[2], s_h[3])),
                                           cz_min(cz_min(s_mu[0], s_mu[1]), cz_min(s_mu[2], s_mu[3])));
+            F.smax_out[_t] = cz_max(cz_max(s_s[0], s_s[1]), cz_max(s_s[2], s_s[3]));
+        }
     }
 }
 
 // Fold of the per-64-tile stats (poll_stats, src/film/mod.rs:944-951) and the
 // frame scalars the host reads back: [0] h_max, [1] mu_min, [2] active count
-// (bits), [3] cells of the active tiles (bits), [4..7] cg_s[1].
+// (bits), [3] cells of the active tiles (bits), [4..7] cg_s[1], [8] s_max.
 template <class CS>
 __global__ __launch_bounds__(1024) void k_frame_stats(CS cs) {
     // one 1024-thread block per canvas: each thread takes a strided share of
@@ -1737,15 +1745,16 @@ __global__ __launch_bounds__(1024) void k_frame_stats(CS cs) {
     // reductions (max and min are exact in any order, the cell count is an
     // integer sum)
     const FilmDev& F = cs.film(blockIdx.x);
-    __shared__ float s_h[16], s_mu[16];
+    __shared__ float s_h[16], s_mu[16], s_s[16];
     __shared__ uint32_t s_cells[16];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t nt = F.t64x * F.t64y;
-    float hm = 0.0f, mm = 1e9f;
+    float hm = 0.0f, mm = 1e9f, sm = 0.0f;
     for (uint32_t t = threadIdx.x; t < nt; t += 1024u) {
         const float2 v = F.stats_out[t];
         hm = cz_max(hm, v.x);
         mm = cz_min(mm, v.y);
+        sm = cz_max(sm, F.smax_out[t]);
     }
     uint32_t cells = 0u;
     const uint32_t nact = *F.active_count;
@@ -1758,21 +1767,25 @@ __global__ __launch_bounds__(1024) void k_frame_stats(CS cs) {
     }
     hm = wave_max(hm);
     mm = wave_min(mm);
+    sm = wave_max(sm);
     for (int off = 32; off > 0; off >>= 1) cells += __shfl_xor(cells, off, 64);
     if (lane == 0u) {
         s_h[wave] = hm;
         s_mu[wave] = mm;
+        s_s[wave] = sm;
         s_cells[wave] = cells;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        float h = s_h[0], mu = s_mu[0];
+        float h = s_h[0], mu = s_mu[0], smax = s_s[0];
         uint32_t cs_ = s_cells[0];
         for (int w = 1; w < 16; w++) {
             h = cz_max(h, s_h[w]);
             mu = cz_min(mu, s_mu[w]);
+            smax = cz_max(smax, s_s[w]);
             cs_ += s_cells[w];
         }
+        F.frame_out[8] = smax;
         F.frame_out[0] = h;
         F.frame_out[1] = mu;
         F.frame_out[2] = cz_bits2f(nact);

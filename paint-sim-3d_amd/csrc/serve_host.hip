@@ -1,13 +1,14 @@
 // serve_host.hip - the serving loop over batched films (cz_server_* of
 // include/canvaz_film.h).
 //
-// A request asks for `steps` frames of a seeded canvas.  Requests are
-// admitted first come, first served, when their arrival time has passed,
-// a batch slot is free and the canvas fits the device-memory budget; an
-// admitted canvas joins the running batch between two frames, takes one
-// frame per batched frame (with its own substeps, exactly as it would alone)
-// and leaves the batch once its steps are done.  Canvas state stays resident
-// from admission to completion.  Films are recycled: a finished canvas's
+// A request asks for `steps` frames of a seeded canvas.  A request is
+// admitted when its arrival time has passed, a batch slot is free and the
+// canvas fits the device-memory budget, first come first served or, with
+// max_wait_s > 0, least work first with the aged first (an aged canvas that
+// does not fit evicts long-running ones to host memory); an admitted canvas
+// joins the running batch between two frames, takes one frame per batched
+// frame (with its own substeps, exactly as it would alone) and leaves the
+// batch once its steps are done.  Films are recycled: a finished canvas's
 // film goes to a pool, and a later canvas that fits its capacity reuses it
 // (re-dimensioned and reseeded on the device) instead of allocating;
 // pooled films count against the budget and are released when a new
@@ -17,7 +18,7 @@
 
 #include <algorithm>
 #include <chrono>
-#include <deque>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -33,6 +34,15 @@ struct Slot {
     cz_film* film;
     uint32_t req;        // index into the request array
     uint32_t left;       // frames still to run
+    double since;        // when it was admitted or resumed
+};
+
+// A canvas evicted to host memory, waiting to resume.
+struct Parked {
+    uint32_t req;
+    uint32_t left;
+    double since;        // when it was evicted
+    czi::FilmSnapshot snap;
 };
 
 uint64_t film_bytes_for(uint32_t w, uint32_t h) {
@@ -63,7 +73,7 @@ struct cz_server {
     cz_batch* batch = nullptr;
     std::vector<cz_film*> pool;      // idle films
     std::vector<std::pair<uint64_t, cz_film*>> kept;  // finished canvases kept for reading
-    uint64_t held = 0;               // device bytes of every film the server owns
+    uint64_t held = 0;               // device bytes of the active and pooled films
 };
 
 namespace {
@@ -128,6 +138,8 @@ void cz_server_config_default(cz_server_config* c) {
     c->max_members = 256;
     c->frame_dt = 1.0f / 60.0f;
     c->keep_films = 0;
+    c->max_wait_s = 0.0f;
+    c->host_budget = 64ull << 30;
 }
 
 int cz_server_create(const cz_server_config* cfg, cz_server** out) {
@@ -164,59 +176,230 @@ int cz_server_run(cz_server* s, const cz_request* reqs, uint32_t n, cz_result* r
         res[k].arrival_s = reqs[k].arrival_s;
         res[k].status = 1;  // pending
     }
-    std::deque<uint32_t> waiting;
+    const double max_wait = s->cfg.max_wait_s > 0.0f ? (double)s->cfg.max_wait_s : 0.0;
+    std::vector<uint32_t> waiting;  // arrived, not yet admitted (arrival order)
+    std::vector<Parked> parked;     // evicted, waiting to resume
     std::vector<Slot> active;
+    uint64_t host_bytes = 0;
     uint32_t next = 0, done = 0;
     const Clock::time_point t0 = Clock::now();
+    // on an error: every running canvas leaves the batch and its film is
+    // released, so that neither the batch nor the server keeps it
+    auto fail = [&](int rc) -> int {
+        const std::string msg = cz_last_error();
+        for (Slot& a : active) {
+            cz_batch_remove(s->batch, a.film);
+            release(s, a.film);
+        }
+        active.clear();
+        parked.clear();
+        set_err(rc, "%s", msg.c_str());
+        return rc;
+    };
+    // admit (or resume) a canvas onto film f
+    auto start = [&](cz_film* f, uint32_t k, uint32_t left, const FilmSnapshot* snap, double now) -> int {
+        int rc = snap ? film_restore(f, *snap) : cz_film_seed_blobs(f, &reqs[k].scene);
+        if (!rc) rc = cz_batch_add(s->batch, f);
+        if (rc) {
+            release(s, f);
+            return rc;
+        }
+        active.push_back({f, k, left, now});
+        return CZ_OK;
+    };
+    // take the running canvas active[q] out to host memory
+    auto evict = [&](size_t q, double now) -> int {
+        Slot a = active[q];
+        int rc = cz_batch_remove(s->batch, a.film);
+        if (rc) return rc;
+        active[q] = active.back();
+        active.pop_back();
+        Parked p;
+        p.req = a.req;
+        p.left = a.left;
+        p.since = now;
+        rc = film_snapshot(a.film, p.snap);
+        if (rc) {
+            release(s, a.film);
+            return rc;
+        }
+        s->pool.push_back(a.film);
+        host_bytes += p.snap.data.size();
+        stats.peak_host_bytes = std::max<uint64_t>(stats.peak_host_bytes, host_bytes);
+        parked.push_back(std::move(p));
+        stats.evictions++;
+        return CZ_OK;
+    };
     int rc = CZ_OK;
     while (done < n) {
         double now = since(t0);
         while (next < n && reqs[next].arrival_s <= now) waiting.push_back(next++);
-        // admission, first come first served
-        while (!waiting.empty() && active.size() < s->cfg.max_members) {
-            const uint32_t k = waiting.front();
-            const cz_request& r = reqs[k];
-            if (r.steps == 0) {
-                res[k].admit_s = res[k].done_s = since(t0);
-                res[k].status = 0;
-                waiting.pop_front();
-                done++;
-                continue;
+        if (max_wait <= 0.0) {
+            // first come, first served: the head waits until it fits
+            size_t w = 0;
+            while (w < waiting.size() && active.size() < s->cfg.max_members) {
+                const uint32_t k = waiting[w];
+                const cz_request& r = reqs[k];
+                if (r.steps == 0) {
+                    res[k].admit_s = res[k].done_s = since(t0);
+                    res[k].status = 0;
+                    w++;
+                    done++;
+                    continue;
+                }
+                cz_film* f = nullptr;
+                rc = acquire(s, r, &f, stats);
+                if (rc) {
+                    res[k].status = rc;
+                    res[k].admit_s = res[k].done_s = since(t0);
+                    w++;
+                    done++;
+                    rc = CZ_OK;
+                    continue;
+                }
+                if (!f) break;  // not enough memory until a canvas finishes
+                rc = start(f, k, r.steps, nullptr, now);
+                if (rc) return fail(rc);
+                res[k].admit_s = since(t0);
+                stats.admitted++;
+                w++;
             }
-            cz_film* f = nullptr;
-            rc = acquire(s, r, &f, stats);
-            if (rc) {
-                res[k].status = rc;
-                res[k].admit_s = res[k].done_s = since(t0);
-                waiting.pop_front();
-                done++;
-                rc = CZ_OK;
-                continue;
+            waiting.erase(waiting.begin(), waiting.begin() + (long)w);
+        } else {
+            // least work first; the aged (waiting longer than max_wait) before
+            // all others, oldest first, and allowed to evict
+            struct Cand {
+                bool parked, aged;
+                double key;
+                uint64_t work;
+                uint32_t idx;
+            };
+            std::vector<Cand> cand;
+            for (size_t w = 0; w < waiting.size(); w++) {
+                const cz_request& r = reqs[waiting[w]];
+                cand.push_back({false, now - r.arrival_s >= max_wait, r.arrival_s,
+                                (uint64_t)r.w * r.h * r.steps, (uint32_t)w});
             }
-            if (!f) break;  // not enough memory until a canvas finishes
-            rc = cz_film_seed_blobs(f, &r.scene);
-            if (!rc) rc = cz_batch_add(s->batch, f);
-            if (rc) {
-                release(s, f);
-                return rc;
+            for (size_t q = 0; q < parked.size(); q++) {
+                const cz_request& r = reqs[parked[q].req];
+                cand.push_back({true, now - parked[q].since >= max_wait, parked[q].since,
+                                (uint64_t)r.w * r.h * parked[q].left, (uint32_t)q});
             }
-            res[k].admit_s = since(t0);
-            active.push_back({f, k, r.steps});
-            waiting.pop_front();
-            stats.admitted++;
+            std::sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) {
+                if (a.aged != b.aged) return a.aged;
+                if (a.aged) return a.key < b.key;
+                if (a.work != b.work) return a.work < b.work;
+                return a.key < b.key;
+            });
+            std::vector<char> took_w(waiting.size(), 0), took_p(parked.size(), 0);
+            for (const Cand& c : cand) {
+                if (active.size() >= s->cfg.max_members) break;
+                const uint32_t k = c.parked ? parked[c.idx].req : waiting[c.idx];
+                const cz_request& r = reqs[k];
+                if (!c.parked && r.steps == 0) {
+                    res[k].admit_s = res[k].done_s = since(t0);
+                    res[k].status = 0;
+                    took_w[c.idx] = 1;
+                    done++;
+                    continue;
+                }
+                cz_film* f = nullptr;
+                rc = acquire(s, r, &f, stats);
+                if (!rc && !f && c.aged) {
+                    // plan the evictions: running canvases that have run for
+                    // max_wait since they started and have more work left
+                    // than the candidate (so two canvases never evict each
+                    // other in turn), most work left first, each within the
+                    // host budget, until the candidate's film fits
+                    std::vector<std::pair<uint64_t, cz_film*>> order;
+                    for (const Slot& a : active) {
+                        const uint64_t work = (uint64_t)a.film->W * a.film->H * a.left;
+                        if (now - a.since >= max_wait && work > c.work) order.push_back({work, a.film});
+                    }
+                    std::sort(order.begin(), order.end(),
+                              [](const std::pair<uint64_t, cz_film*>& x, const std::pair<uint64_t, cz_film*>& y) {
+                                  return x.first > y.first;
+                              });
+                    const uint64_t need = film_bytes_for(r.w, r.h);
+                    uint64_t freeable = 0, to_host = host_bytes;
+                    for (cz_film* pf : s->pool) freeable += pf->bytes;
+                    std::vector<cz_film*> plan;
+                    for (const auto& o : order) {
+                        if (s->held - freeable + need <= s->cfg.hbm_budget) break;
+                        const uint64_t sb = film_snapshot_bytes(o.second);
+                        if (to_host + sb > s->cfg.host_budget) continue;
+                        plan.push_back(o.second);
+                        to_host += sb;
+                        freeable += o.second->bytes;
+                    }
+                    if (s->held - freeable + need > s->cfg.hbm_budget) break;  // nothing passes it
+                    for (cz_film* victim : plan) {
+                        size_t q = 0;
+                        while (active[q].film != victim) q++;
+                        rc = evict(q, now);
+                        if (rc) return fail(rc);
+                        took_p.push_back(0);
+                    }
+                    rc = acquire(s, r, &f, stats);
+                }
+                if (rc) {
+                    if (c.parked) return fail(rc);  // it fitted before: a device failure
+                    res[k].status = rc;
+                    res[k].admit_s = res[k].done_s = since(t0);
+                    took_w[c.idx] = 1;
+                    done++;
+                    rc = CZ_OK;
+                    continue;
+                }
+                if (!f) {
+                    if (c.aged) break;  // nothing passes an aged canvas that does not fit
+                    continue;           // a smaller one may fit
+                }
+                if (c.parked) {
+                    Parked& p = parked[c.idx];
+                    rc = start(f, k, p.left, &p.snap, now);
+                    if (rc) return fail(rc);
+                    host_bytes -= p.snap.data.size();
+                    std::vector<uint8_t>().swap(p.snap.data);
+                    took_p[c.idx] = 1;
+                    stats.resumes++;
+                } else {
+                    rc = start(f, k, r.steps, nullptr, now);
+                    if (rc) return fail(rc);
+                    res[k].admit_s = since(t0);
+                    took_w[c.idx] = 1;
+                    stats.admitted++;
+                }
+            }
+            size_t o = 0;
+            for (size_t w = 0; w < waiting.size(); w++)
+                if (!took_w[w]) waiting[o++] = waiting[w];
+            waiting.resize(o);
+            std::vector<Parked> keep;
+            for (size_t q = 0; q < parked.size(); q++)
+                if (q >= took_p.size() || !took_p[q]) keep.push_back(std::move(parked[q]));
+            parked.swap(keep);
         }
         stats.peak_members = std::max<uint64_t>(stats.peak_members, active.size());
         stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, s->held);
         if (active.empty()) {
-            if (next >= n && waiting.empty()) break;
-            if (!waiting.empty()) return set_err(CZ_ENOMEM, "cz_server_run: request does not fit an empty server");
+            if (next >= n && waiting.empty() && parked.empty()) break;
+            if (!waiting.empty() || !parked.empty()) {
+                const cz_request& r = reqs[!waiting.empty() ? waiting[0] : parked[0].req];
+                return fail(set_err(CZ_ENOMEM,
+                                    "cz_server_run: request %llu (%ux%u, %llu bytes) does not fit an empty server "
+                                    "(%zu waiting, %zu evicted, %llu bytes held in %zu pooled films, budget %llu)",
+                                    (unsigned long long)r.id, r.w, r.h, (unsigned long long)film_bytes_for(r.w, r.h),
+                                    waiting.size(), parked.size(), (unsigned long long)s->held, s->pool.size(),
+                                    (unsigned long long)s->cfg.hbm_budget));
+            }
             // idle until the next arrival
             double wait = reqs[next].arrival_s - since(t0);
             if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 1e-3)));
             continue;
         }
         rc = cz_batch_step(s->batch, s->cfg.frame_dt);
-        if (rc) return rc;
+        if (rc) return fail(rc);
         stats.frames++;
         bool finishing = false;
         for (Slot& a : active) {
@@ -226,7 +409,7 @@ int cz_server_run(cz_server* s, const cz_request* reqs, uint32_t n, cz_result* r
         }
         if (!finishing) continue;
         rc = cz_batch_sync(s->batch);
-        if (rc) return rc;
+        if (rc) return fail(rc);
         const double t_done = since(t0);
         for (size_t q = 0; q < active.size();) {
             Slot& a = active[q];
@@ -235,12 +418,18 @@ int cz_server_run(cz_server* s, const cz_request* reqs, uint32_t n, cz_result* r
                 continue;
             }
             rc = cz_batch_remove(s->batch, a.film);
-            if (rc) return rc;
+            if (rc) return fail(rc);
             res[a.req].done_s = t_done;
             res[a.req].status = 0;
             res[a.req].cell_steps = (uint64_t)a.film->W * a.film->H * reqs[a.req].steps;
-            if (s->cfg.keep_films) s->kept.push_back({reqs[a.req].id, a.film});
-            else s->pool.push_back(a.film);
+            if (s->cfg.keep_films) {
+                // kept for reading, outside the budget (which covers the
+                // films the loop steps and pools)
+                s->held -= a.film->bytes;
+                s->kept.push_back({reqs[a.req].id, a.film});
+            } else {
+                s->pool.push_back(a.film);
+            }
             active[q] = active.back();
             active.pop_back();
             done++;

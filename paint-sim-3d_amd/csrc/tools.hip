@@ -343,12 +343,14 @@ int tools_alloc(cz_film* f) {
     if (!rc && !f->ctile_range) rc = talloc(f, &f->ctile_range, f->cap_nt64);
     if (!rc && !f->ctile_items) rc = talloc(f, &f->ctile_items, (size_t)CZ_MAX_CONTACTS * 16u);
     if (!rc && !f->reservoir) rc = talloc(f, &f->reservoir, (size_t)CZ_MAX_BRISTLES * 17u);
-    if (!rc && !f->knife_dev) rc = talloc(f, &kd, 1);
+    if (!rc && !f->knife_dev) {
+        rc = talloc(f, &kd, 1);
+        if (!rc) f->knife_dev = kd;  // kept at once, so a retry after a later failure reuses it
+    }
     if (!rc && !f->scraps) rc = talloc(f, &f->scraps, CZ_MAX_SCRAPS);
     if (!rc && !f->kcnt) rc = talloc(f, &f->kcnt, kcap);
     if (!rc && !f->kctl) rc = talloc(f, &f->kctl, 4);
     if (!rc) {
-        if (kd) f->knife_dev = kd;
         hipError_t e = hipSuccess;
         if (!f->contacts_h) e = hipHostMalloc((void**)&f->contacts_h, CZ_MAX_CONTACTS * sizeof(cz_contact), hipHostMallocDefault);
         if (e == hipSuccess && !f->crange_h) e = hipHostMalloc((void**)&f->crange_h, f->cap_nt64 * sizeof(uint2), hipHostMallocDefault);

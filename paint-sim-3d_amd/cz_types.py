@@ -82,13 +82,14 @@ class Result(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_members", u32), ("frame_dt", f32), ("keep_films", C.c_int32),
-                ("pad", u32)]
+                ("max_wait_s", f32), ("host_budget", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("frames", C.c_uint64), ("admitted", C.c_uint64),
                 ("cell_steps", C.c_uint64), ("peak_members", C.c_uint64), ("peak_bytes", C.c_uint64),
-                ("films_created", C.c_uint64), ("films_reused", C.c_uint64)]
+                ("films_created", C.c_uint64), ("films_reused", C.c_uint64), ("evictions", C.c_uint64),
+                ("resumes", C.c_uint64), ("peak_host_bytes", C.c_uint64)]
 
 
 class HostState(C.Structure):

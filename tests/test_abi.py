@@ -70,3 +70,34 @@ def test_library_shares_torchs_hip_runtime():
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip().splitlines()[-1] == "1"
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every ctypes mirror of a header struct has the C compiler's size and
+    field offsets (a field added on one side only shifts everything after it)."""
+    import cz_types as T
+    pairs = {"cz_params": T.Params, "cz_deposit": T.Deposit, "cz_contact": T.Contact, "cz_knife": T.Knife,
+             "cz_scrap": T.Scrap, "cz_settings": T.Settings, "cz_overrides": T.Overrides, "cz_stats": T.Stats,
+             "cz_mass_report": T.MassReport, "cz_mass_totals": T.MassTotals, "cz_host_state": T.HostState,
+             "cz_scene": T.Scene, "cz_request": T.Request, "cz_result": T.Result,
+             "cz_server_config": T.ServerConfig, "cz_server_stats": T.ServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "canvaz_film.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'  printf("{cname} __sizeof__ %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            a, b, v = ln.split()
+            got[(a, b)] = int(v)
+    for cname, py in pairs.items():
+        assert got[(cname, "__sizeof__")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

@@ -11,7 +11,8 @@ import scenes
 
 pytestmark = pytest.mark.gpu
 
-PARTS = ("h", "strata_h", "strata_pig", "strata_cure", "rheo", "m", "sub_refl", "lay", "delta", "tile_flags")
+PARTS = ("h", "strata_h", "strata_pig", "strata_cure", "rheo", "m", "sub_refl", "lay", "delta", "tile_flags", "fnorm",
+         "pres", "tmp")
 
 
 def trace(n=10, seed=3):
@@ -94,3 +95,41 @@ def test_reshaped_film_equals_fresh(gpu):
     assert_same(big, alone(w, h, 5, s, sc), "reshaped")
     with pytest.raises(czf.FilmError):
         big.reshape(200, 200, scenes.blobs_settings(200))
+
+
+def test_least_work_first_with_eviction_equals_alone(gpu):
+    """Budget for one large canvas and two small ones: a long large canvas,
+    shorter large ones and small ones behind it.  Least work first admits the
+    small ones past a large one that does not fit; a large one waiting past
+    max_wait evicts the long one (run at least max_wait, more work left) to
+    host memory, and the evicted canvas resumes later.  Every result equals
+    the canvas stepped alone, bit for bit."""
+    specs, reqs = [], []
+    plan = [(256, 256, 90, 0.0), (256, 256, 20, 0.001), (60, 50, 6, 0.002), (48, 64, 5, 0.004),
+            (56, 56, 8, 0.006), (256, 256, 30, 0.008), (40, 40, 4, 0.010)]
+    for k, (w, h, steps, t) in enumerate(plan):
+        s = scenes.blobs_settings(w, tilt_deg=20.0 + 5 * k)
+        s.height_m = h * s.width_m / w
+        sc = czf.scene(seed=300 + k, nblobs=12, sigma_lo=2.0, sigma_hi=10.0)
+        reqs.append(czf.request(k, w, h, steps, t, s, sc))
+        specs.append((w, h, steps, s, sc))
+    # one large canvas and two small ones fit, two large ones do not
+    large, small = 256 * 256 * 600 + 12 * 2 ** 20, 64 * 64 * 600 + 12 * 2 ** 20
+    budget = large + 2 * small + 4 * 2 ** 20
+    assert 2 * large > budget
+    srv = czf.Server(hbm_budget=budget, max_members=4, keep_films=True, max_wait_s=0.02)
+    res, st = srv.run(reqs)
+    print("evictions", st.evictions, "resumes", st.resumes, "peak members", st.peak_members, flush=True)
+    assert st.evictions > 0 and st.resumes == st.evictions
+    assert st.peak_bytes <= budget
+    assert st.peak_host_bytes > 0
+    for r, (w, h, steps, s, sc) in zip(res, specs):
+        assert r.status == 0, (r.id, r.status)
+        assert r.arrival_s <= r.admit_s <= r.done_s
+        g = srv.film(r.id, w, h)
+        assert g is not None and g.stats().frame == steps
+        assert_same(g, alone(w, h, steps, s, sc), f"request {r.id}")
+    # the small requests did not wait for the large ones to finish
+    small_done = max(res[k].done_s for k in (2, 3, 4, 6))
+    assert small_done < res[0].done_s and small_done < res[1].done_s
+    srv.close()
