@@ -133,3 +133,25 @@ def test_least_work_first_with_eviction_equals_alone(gpu):
     small_done = max(res[k].done_s for k in (2, 3, 4, 6))
     assert small_done < res[0].done_s and small_done < res[1].done_s
     srv.close()
+
+
+@pytest.mark.parametrize("max_wait", [0.0, 0.02])
+def test_failed_admission_leaves_a_clean_server(gpu, max_wait):
+    """A request whose canvas cannot be seeded (a bad scene) fails the run
+    after other canvases were admitted; the run releases them, and the same
+    server then serves a new trace whose results equal the canvases alone."""
+    reqs, specs = trace(6, seed=8)
+    bad = reqs[3]
+    bad.scene.n_lo, bad.scene.n_hi = 0.5, 9.0
+    srv = czf.Server(hbm_budget=8 << 30, max_members=4, keep_films=True, max_wait_s=max_wait)
+    with pytest.raises(czf.FilmError, match="power-law index"):
+        srv.run(reqs)
+    reqs2, specs2 = trace(5, seed=9)
+    for r in reqs2:
+        r.id += 100        # the first run's finished canvases are still kept under their ids
+    res, st = srv.run(reqs2)
+    assert st.admitted == len(reqs2)
+    for r, (w, h, steps, s, sc) in zip(res, specs2):
+        assert r.status == 0
+        assert_same(srv.film(r.id, w, h), alone(w, h, steps, s, sc), f"request {r.id}")
+    srv.close()
