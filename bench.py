@@ -55,8 +55,11 @@ ALGO_BYTES["flux_apply"] = (4 + 8 + 8 + 4 + 4 + 8) + (4 + 16) + (16 + 4 + 16)
 # the 1024-thread shapes of the same passes move the same bytes
 for _fam in ("lap_plug", "flux", "apply", "flux_apply"):
     ALGO_BYTES[_fam + "_w"] = ALGO_BYTES[_fam]
+# the whole substep in one launch: R h m delta rheo fnorm, RW acc, W h delta fnorm tmp
+ALGO_BYTES["substep_fused"] = (4 + 4 + 4 + 8 + 8) + (16 + 16) + (4 + 4 + 8 + 4)
 # with an index per cell: the plug window and the face pass read it
-CELL_N_BYTES = {"lap_plug": 4, "flux": 4, "flux_apply": 4, "lap_plug_w": 4, "flux_w": 4, "flux_apply_w": 4}
+CELL_N_BYTES = {"lap_plug": 4, "flux": 4, "flux_apply": 4, "lap_plug_w": 4, "flux_w": 4, "flux_apply_w": 4,
+                "substep_fused": 4}
 
 
 def algo_bytes(fam, cell_n):
@@ -88,7 +91,8 @@ def canvas_scene(seed, cell_n):
     return czf.scene(seed=seed)
 # families the roofline is reported for: the per-pass substep kernels in both
 # block sizes and the one-launch substeps kernel (k_substeps)
-ROOFLINE_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "lap_plug_w", "flux_w", "apply_w", "flux_apply_w", "substeps")
+ROOFLINE_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "lap_plug_w", "flux_w", "apply_w", "flux_apply_w", "substep_fused",
+                                         "substeps")
 
 
 def log(msg):
@@ -410,7 +414,7 @@ def bench_batch(args, plain_value):
         films.append(f)
     b = czf.Batch(max(len(films), 1))
     b.use_graphs(not args.no_graphs)
-    b.flux_shape(args.flux_shape)
+    b.flux_shape(args.flux_shape if args.flux_shape < 5 else -1)
     for f in films:
         b.add(f)
     log(f"bench batch: {len(films)} canvases, {cells / 1e6:.1f} M cells, timing {frames} frames")
@@ -520,7 +524,8 @@ def bench_serve(args):
     return out
 
 
-FRAME_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "lap_plug_w", "flux_w", "apply_w", "flux_apply_w", "substeps", "lap", "mark", "compact", "compact_list", "cure", "deposit", "split", "reduce_pool",
+FRAME_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "lap_plug_w", "flux_w", "apply_w", "flux_apply_w", "substep_fused",
+                                     "substeps", "lap", "mark", "compact", "compact_list", "cure", "deposit", "split", "reduce_pool",
                                      "strata_limit", "strata_update", "strata_fill",
                                      "frame_stats", "cg_setup", "cg_lap", "cg_init", "cg_iter_a", "cg_iter_b",
                                      "cg_iter_c", "cg_reduce", "cg_finish", "cg_rescale")
@@ -598,9 +603,10 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device to device 3 times (a known-size dispatch that "
                          "calibrates the HBM byte counters of a profiler run)")
-    ap.add_argument("--flux-shape", type=int, default=-1, choices=(-1, 0, 1, 2, 3, 4),
+    ap.add_argument("--flux-shape", type=int, default=-1, choices=(-1, 0, 1, 2, 3, 4, 5),
                     help="substep shape: -1 tuned online, 0 window, 1 face-parallel, 2 flux + apply fused, "
-                         "3 and 4 as 0 and 2 with 1024 threads per tile")
+                         "3 and 4 as 0 and 2 with 1024 threads per tile, 5 the whole substep in one launch "
+                         "(single canvases; the batch keeps its tuned shapes)")
     ap.add_argument("--substep-launch", default="auto", choices=("auto", "per-pass", "one"),
                     help="substeps as one launch per frame on small active sets (auto), always, or never")
     ap.add_argument("--events-in-timed", action="store_true",
@@ -746,7 +752,7 @@ def main():
         shp = list(st.flux_shape_cell_substeps)
         covered = {"lap_plug": shp[0] + shp[1] + shp[2], "lap_plug_w": shp[3] + shp[4],
                    "flux": shp[0] + shp[1], "apply": shp[0] + shp[1], "flux_apply": shp[2],
-                   "flux_w": shp[3], "apply_w": shp[3], "flux_apply_w": shp[4]}[dom]
+                   "flux_w": shp[3], "apply_w": shp[3], "flux_apply_w": shp[4], "substep_fused": shp[5]}[dom]
         cells_per_launch = covered / max(1, launches)
         algo_per_cell = algo_bytes(dom, cell_n)
         bytes_per_launch = algo_per_cell * cells_per_launch

@@ -219,8 +219,9 @@ typedef struct cz_stats {
     uint64_t one_launch_cell_substeps;
     /* active cells times substeps run with each substep shape (0 window
      * flux, 1 face-parallel flux, 2 flux + apply in one launch, 3 and 4 as 0
-     * and 2 with 1024 threads per tile; see cz_film_flux_shape) */
-    uint64_t flux_shape_cell_substeps[5];
+     * and 2 with 1024 threads per tile, 5 the whole substep in one launch;
+     * see cz_film_flux_shape) */
+    uint64_t flux_shape_cell_substeps[6];
 } cz_stats;
 
 /* The mass report of `compute_stats` (src/main.rs:1679-1819), sums in FP64. */
@@ -511,7 +512,11 @@ int cz_film_graph_counts(const cz_film* f, uint64_t* captures, uint64_t* replays
  * flux and apply in one launch (each block computes the fluxes of its tile
  * and a 2-cell ring itself; for small active sets), 3 and 4 = shapes 0 and 2
  * with 1024 threads per tile (every window phase in one pass, the flux one
- * face per thread; for a few hundred active tiles), -1 = chosen online
+ * face per thread; for a few hundred active tiles), 5 = the whole substep in
+ * one launch (each block recomputes the nose, pressures and plug means of a
+ * 7-cell ring; heights alternate between two buffers over the frame's
+ * substeps, and a frame whose parity would make its first substep read and
+ * write the same buffer runs that substep as shape 4), -1 = chosen online
  * (default): per active-set size the shapes are timed with HIP events on live
  * substeps (they give bit-identical results) and the fastest is used. */
 int cz_film_flux_shape(cz_film* f, int shape);

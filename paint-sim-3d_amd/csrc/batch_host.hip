@@ -429,7 +429,7 @@ int cz_batch_use_graphs(cz_batch* b, int on) {
 }
 
 int cz_batch_flux_shape(cz_batch* b, int shape) {
-    if (!b || shape < -1 || shape >= czi::kSubstepShapes) return set_err(CZ_EINVAL, "flux_shape: -1 (tuned) or 0-4");
+    if (!b || shape < -1 || shape >= czi::kBatchShapes) return set_err(CZ_EINVAL, "flux_shape: -1 (tuned) or 0-4");
     b->flux_tuner.fixed = shape;
     return CZ_OK;
 }

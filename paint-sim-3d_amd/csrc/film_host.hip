@@ -339,10 +339,39 @@ int record_frame(cz_film* f, uint32_t substeps) {
     // in the launch shape the substep tuner picks for the active-set size (it
     // times the whole substep)
     const dim3 b1024(1024);
-    for (uint32_t s = 0; s < (pg > 0u ? 0u : substeps); s++) {
-        int bk = 0;
+    const uint32_t nsub = pg > 0u ? 0u : substeps;
+    std::vector<int> shp(nsub), bks(nsub);
+    std::vector<char> tim(nsub), to_main(nsub, 1);
+    for (uint32_t s = 0; s < nsub; s++) {
         bool timed = false;
-        const int shape = f->flux_tuner.choose(std::min(f->active_hint, f->nt16), !f->capturing, bk, timed);
+        shp[s] = f->flux_tuner.choose(std::min(f->active_hint, f->nt16), !f->capturing, bks[s], timed);
+        tim[s] = timed ? 1 : 0;
+    }
+    // The fused shape (5) reads the heights before its apply from one of F.h
+    // and h_sub and writes the new ones to the other.  A run of fused
+    // substeps starts from F.h (the frame's start or a separate-pass
+    // substep) and must end in F.h (the next separate-pass substep or the
+    // frame's end), so it alternates over an even number of substeps: a run
+    // of odd length runs its first substep as separate passes (shape 4).
+    for (uint32_t a = 0; a < nsub;) {
+        if (shp[a] != 5) {
+            a++;
+            continue;
+        }
+        uint32_t b = a;
+        while (b + 1 < nsub && shp[b + 1] == 5) b++;
+        if (((b - a + 1) & 1u) != 0u) {
+            if (tim[a]) f->flux_tuner.reissue(bks[a], 5, 4);
+            shp[a++] = 4;
+        }
+        for (uint32_t s = a; s <= b && a <= b; s++) to_main[s] = ((b - s) & 1u) == 0u ? 1 : 0;
+        a = b + 1;
+    }
+    bool cur_main = true;  // where the current heights are
+    for (uint32_t s = 0; s < nsub; s++) {
+        const int bk = bks[s];
+        const bool timed = tim[s] != 0;
+        const int shape = shp[s];
         hipEvent_t ea = nullptr, eb = nullptr;
         if (timed) {
             ea = f->take_event();
@@ -351,6 +380,14 @@ int record_frame(cz_film* f, uint32_t substeps) {
         }
         const OneFilm wl{f->dev()};
         const int nose = s > 0 ? 1 : 0, clear = (s == 0 && !implicit) ? 1 : 0;
+        if (shape == 5) {
+            if (cur_main == (to_main[s] != 0)) return set_err(CZ_EINVAL, "fused substep: height buffers collide");
+            float* hr = cur_main ? f->h : f->h_sub;
+            float* hw = to_main[s] ? f->h : f->h_sub;
+            LAUNCH("substep_fused", k_substep_fused<OneFilm>, gt, b1024, wl, (const float*)hr, hw, nose, clear);
+            if (s > 0) f->delta_cur_a = !f->delta_cur_a;
+            cur_main = to_main[s] != 0;
+        } else {
         if (shape >= 3)
             LAUNCH("lap_plug_w", k_lap_plug_w<OneFilm>, gt, b1024, wl, nose, 1, clear);
         else
@@ -376,6 +413,8 @@ int record_frame(cz_film* f, uint32_t substeps) {
                 LAUNCH("flux", k_fluxes<OneFilm>, gt, b256, w);
                 LAUNCH("apply", k_limit_apply<OneFilm>, gt, b256, w);
                 break;
+        }
+        cur_main = true;
         }
         if (ea && eb) {
             hipEventRecord(eb, f->stream);
@@ -1144,7 +1183,7 @@ int cz_film_substep_launch(cz_film* f, int mode) {
 }
 
 int cz_film_flux_shape(cz_film* f, int shape) {
-    if (!f || shape < -1 || shape >= czi::kSubstepShapes) return set_err(CZ_EINVAL, "flux_shape: -1 (tuned) or 0-4");
+    if (!f || shape < -1 || shape >= czi::kSubstepShapes) return set_err(CZ_EINVAL, "flux_shape: -1 (tuned) or 0-5");
     f->flux_tuner.fixed = shape;
     return CZ_OK;
 }

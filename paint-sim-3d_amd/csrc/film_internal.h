@@ -54,11 +54,13 @@ constexpr uint32_t kDevBinMin = 1024u;
 //   1 lap + plug, face-parallel flux (a block per tile and face), limiter + apply;
 //   2 lap + plug, flux + limiter + apply in one launch;
 //   3 as 0 with 1024 threads per tile (flux one face per thread);
-//   4 as 2 with 1024 threads per tile.
+//   4 as 2 with 1024 threads per tile;
+//   5 the whole substep in one launch (k_substep_fused; single films).
 // 256-thread blocks hold 4 waves per tile; a few hundred active tiles leave
 // the GPU at 1-2 waves per SIMD, where the wide shapes take each window phase
 // in one pass with 16 waves per tile.
-constexpr int kSubstepShapes = 5;
+constexpr int kSubstepShapes = 6;
+constexpr int kBatchShapes = 5;  // a batch runs shapes 0-4
 
 struct ProfSlot {
     std::string label;
@@ -68,16 +70,18 @@ struct ProfSlot {
 // Online choice between launch shapes of one kernel family whose results are
 // bit-identical (so exploring costs nothing but time): per size bucket
 // (log2 of the work items) each shape is timed with HIP events until it has
-// WARM samples, then the one with the least time per work item is used and
-// one of the others re-timed once every RECHECK uses.  A sample's time is
+// WARM samples, then the one with the least time per work item (its best
+// sample: the first launches of a kernel carry one-off costs, and a shape's
+// best sample is what it can do) is used and one of the others re-timed once
+// every RECHECK uses.  A sample's time is
 // known only once its frame has run, so exploration counts the samples
 // issued, not the ones folded: the first frame of a bucket times every shape
 // in turn (WARM substeps each), and the frame after it uses the fastest.
 // fixed >= 0 pins a shape.
 struct ShapeTuner {
-    static constexpr int NB = 24, NS = 5, WARM = 3, RECHECK = 64;
+    static constexpr int NB = 24, NS = 6, WARM = 3, RECHECK = 64;
     int ns = 2;  // shapes in use (at most NS)
-    double sum[NB][NS] = {};
+    double best_t[NB][NS] = {};   // least time per item over the samples
     uint32_t cnt[NB][NS] = {};
     uint32_t issued[NB][NS] = {};
     uint32_t uses[NB] = {};
@@ -94,14 +98,15 @@ struct ShapeTuner {
         timed = false;
         if (fixed >= 0) return fixed;
         if (!can_time) return best_of(bk);
-        // explore: the shape with the fewest samples issued, round robin
-        int least = -1;
-        for (int k = 0; k < ns; k++)
-            if (issued[bk][k] < (uint32_t)WARM && (least < 0 || issued[bk][k] < issued[bk][least])) least = k;
-        if (least >= 0) {
-            issued[bk][least]++;
-            timed = true;
-            return least;
+        // explore: the first shape still short of WARM samples issued, so
+        // that a shape's samples come in consecutive substeps (the fused
+        // substep runs in pairs)
+        for (int k = 0; k < ns; k++) {
+            if (issued[bk][k] < (uint32_t)WARM) {
+                issued[bk][k]++;
+                timed = true;
+                return k;
+            }
         }
         const int best = best_of(bk);
         if (++uses[bk] % RECHECK == 0) {
@@ -111,6 +116,11 @@ struct ShapeTuner {
         }
         return best;
     }
+    // a substep issued as shape `from` ran as shape `to`
+    void reissue(int bk, int from, int to) {
+        if (issued[bk][from] > 0) issued[bk][from]--;
+        issued[bk][to]++;
+    }
     // the shape with the least time per item among those with samples
     // (shape 0 if none has any)
     int best_of(int bk) const {
@@ -118,7 +128,7 @@ struct ShapeTuner {
         double bt = 0.0;
         for (int k = 0; k < ns; k++) {
             if (cnt[bk][k] == 0) continue;
-            const double t = sum[bk][k] / cnt[bk][k];
+            const double t = best_t[bk][k];
             if (best < 0 || t < bt) {
                 best = k;
                 bt = t;
@@ -137,7 +147,8 @@ struct ShapeTuner {
     }
     // one timed launch: us of device time over `items` work items
     void record(int bk, int shape, double us, double items) {
-        sum[bk][shape] += us / (items > 1.0 ? items : 1.0);
+        const double t = us / (items > 1.0 ? items : 1.0);
+        if (cnt[bk][shape] == 0 || t < best_t[bk][shape]) best_t[bk][shape] = t;
         cnt[bk][shape] += 1;
     }
 };

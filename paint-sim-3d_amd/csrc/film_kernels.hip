@@ -1519,6 +1519,391 @@ __global__ __launch_bounds__(1024) void k_limit_apply_w(WS ws) {
 }
 
 // ---------------------------------------------------------------------------
+// A whole substep of a tile in one launch: [nose of the previous substep] +
+// laplacian + plug + fluxes + limiter + apply (film.wgsl:16-87, 375-455,
+// 521-537, 680-723, 803-863), for small active sets where the launch between
+// the lap/plug pass and the flux + apply pass costs more than recomputing a
+// wider ring.  The apply of the tile needs fluxes of the tile + 2 ring, whose
+// window (tile + 6) needs pressures and plug means there, which need the
+// nose of the tile + 7 ring and per-cell plug fluxes of the tile + 9 ring:
+// the block stages that 34x34 window and computes every ring itself.
+//
+// Blocks of one launch apply while others read, so the heights before this
+// substep are read from hr (cells of active tiles; inactive tiles keep F.h,
+// which no apply writes) and the new heights written to hw; the caller
+// alternates the two buffers over the frame's substeps and ends in F.h.  The
+// offsets are read from F.delta (before the nose) and written to
+// F.delta_out; the flux terms read, like the separate flux pass after the
+// offset flip, the new offsets of active cells (from the block's nose, or
+// recomputed from hr and F.delta for the far steps of cell_excess beyond the
+// window) and F.delta_out at cells of inactive tiles.
+constexpr int SB_R = 9, SB_N = 16 + 2 * SB_R;   // staged window
+constexpr int SN_R = 7, SN_N = 16 + 2 * SN_R;   // nose, usurf
+constexpr int SL_R = AP_R + FLUX_R;             // lap, plug means, flux window
+
+struct SubstepWin {
+    float h[SB_N * SB_N];
+    float m[SB_N * SB_N];
+    uint32_t d[SB_N * SB_N];
+    float2 rheo[SB_N * SB_N];
+    float2 fn[SB_N * SB_N];
+    float2 hq[SB_N * SB_N];       // plug: height (negated with a yield stress), hb_flux
+    uint32_t nd[SN_N * SN_N];     // offsets the laplacian reads (new on active cells)
+    float2 nfn[SN_N * SN_N];      // front normals likewise
+    float us[SN_N * SN_N];        // usurf
+    uint32_t tf[9];
+};
+
+struct SubstepFusedShared {
+    SubstepWin s;
+    FluxWinT<SL_R> w;
+    ApplyShared ap;
+};
+
+// heights before this substep's apply at any canvas cell
+__device__ __forceinline__ float h_before(const FilmDev& F, const float* hr, int i, int j) {
+    const uint32_t c = idx(F, (uint32_t)i, (uint32_t)j);
+    return cell_active(F, (uint32_t)i, (uint32_t)j) ? hr[c] : F.h[c];
+}
+
+// nose (film.wgsl:680-723) of one active cell from HBM: the offset only
+__device__ uint32_t nose_offset_global(const FilmDev& F, const float* hr, int i, int j) {
+    const int W_ = (int)F.W, H_ = (int)F.H;
+    if (h_before(F, hr, i, j) <= 0.0f) return 0u;
+    int bx = 64, by = 64;
+    float bd = 1e9f;
+    for (int dj = -1; dj <= 1; dj++) {
+        for (int di = -1; di <= 1; di++) {
+            const int xx = i + di, yy = j + dj;
+            if (xx < 0 || yy < 0 || xx >= W_ || yy >= H_) continue;
+            int cx = di, cy = dj;
+            if (h_before(F, hr, xx, yy) > 0.0f) {
+                const uint32_t dv = F.delta[idx(F, (uint32_t)xx, (uint32_t)yy)];
+                if ((dv & 0x10000u) == 0u) continue;
+                int ox, oy;
+                unpack_off(dv, ox, oy);
+                cx = ox + di;
+                cy = oy + dj;
+                const int tx = i + cx, ty = j + cy;
+                if (tx < 0 || ty < 0 || tx >= W_ || ty >= H_) continue;
+                if (h_before(F, hr, tx, ty) > 0.0f) continue;
+            } else if (di == 0 && dj == 0) {
+                continue;
+            }
+            const float d = cz_len2((float)cx, (float)cy);
+            if (d < bd) { bd = d; bx = cx; by = cy; }
+        }
+    }
+    if (bd > 1e8f) return 0u;
+    return pack_off(bx, by);
+}
+
+// The flux window reader of the fused substep: as WinReadT, with the far
+// steps of cell_excess served as described above.
+struct FusedRead {
+    static constexpr int N = FluxWinT<SL_R>::N;
+    const FilmDev& F;
+    const FluxWinT<SL_R>& w;
+    const float* hr;
+    int wx0, wy0, ttx, tty;
+    float hold1_, theta_;
+    int do_nose;
+    __device__ __forceinline__ int q(int i, int j) const { return (j - wy0) * N + (i - wx0); }
+    __device__ __forceinline__ float h(int i, int j) const { return w.h[q(i, j)]; }
+    __device__ __forceinline__ uint32_t delta(int i, int j) const { return w.d[q(i, j)]; }
+    __device__ __forceinline__ float2 rheo(int i, int j) const { return w.rheo[q(i, j)]; }
+    __device__ __forceinline__ float2 fnorm(int i, int j) const { return w.fn[q(i, j)]; }
+    __device__ __forceinline__ float2 pres(int i, int j) const { return w.pres[q(i, j)]; }
+    __device__ __forceinline__ float tmp(int i, int j) const { return w.tmp[q(i, j)]; }
+    __device__ __forceinline__ float lmu(int i, int j) const { return w.lmu[q(i, j)]; }
+    __device__ __forceinline__ float fdist(int i, int j) const { return w.fd[q(i, j)]; }
+    __device__ __forceinline__ float ncell(int i, int j) const { return F.n_cell[(uint32_t)j * F.W + (uint32_t)i]; }
+    __device__ __forceinline__ float hold1() const { return hold1_; }
+    __device__ __forceinline__ float theta() const { return theta_; }
+    __device__ __forceinline__ bool in(int i, int j) const {
+        return (unsigned)(i - wx0) < (unsigned)N && (unsigned)(j - wy0) < (unsigned)N;
+    }
+    __device__ __forceinline__ float h_far(int i, int j) const {
+        if (in(i, j)) return w.h[q(i, j)];
+        return h_before(F, hr, i, j);
+    }
+    __device__ __forceinline__ uint32_t delta_far(int i, int j) const {
+        if (in(i, j)) return w.d[q(i, j)];
+        const uint32_t c = (uint32_t)j * F.W + (uint32_t)i;
+        if (!do_nose) return F.delta[c];
+        return cell_active(F, (uint32_t)i, (uint32_t)j) ? nose_offset_global(F, hr, i, j) : F.delta_out[c];
+    }
+    __device__ __forceinline__ bool active(int i, int j) const {
+        return (w.tf[((j >> 4) - tty + 1) * 3 + ((i >> 4) - ttx + 1)] & 32u) != 0u;
+    }
+};
+
+__device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float* hr, float* hw, uint32_t _t,
+                                                   SubstepFusedShared& S, int do_nose, int do_clear) {
+    constexpr int BT = 1024;
+    SubstepWin& sw = S.s;
+    FluxWinT<SL_R>& fw = S.w;
+    const uint32_t _lx = threadIdx.x & 15u, _ly = (threadIdx.x >> 4) & 15u;
+    CZ_TILE_CELL(F)
+    const bool cell_th = threadIdx.x < 256u;
+    const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
+    const bool plug_on = !(gb < 1e-6f);
+    const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
+    const int tx0 = ttx * 16, ty0 = tty * 16;
+    const int bx0 = tx0 - SB_R, by0 = ty0 - SB_R;     // staged window origin
+    const int nx0 = tx0 - SN_R, ny0 = ty0 - SN_R;     // nose region origin
+    const int lx0 = tx0 - SL_R, ly0 = ty0 - SL_R;     // lap / flux window origin
+    const int W_ = (int)F.W, H_ = (int)F.H;
+    __syncthreads();
+    if (threadIdx.x < 9u) {
+        const int ax = ttx - 1 + (int)(threadIdx.x % 3u), ay = tty - 1 + (int)(threadIdx.x / 3u);
+        const uint32_t fl = (ax >= 0 && ay >= 0 && ax < (int)F.tw && ay < (int)F.th)
+                                ? F.tile_flags[(uint32_t)ax + (uint32_t)ay * F.tw] : 0u;
+        sw.tf[threadIdx.x] = fl;
+        fw.tf[threadIdx.x] = fl;
+        S.ap.s_tf[threadIdx.x] = fl;
+    }
+    __syncthreads();
+    auto act_w = [&](int gx, int gy) -> bool {
+        return (sw.tf[((gy >> 4) - tty + 1) * 3 + ((gx >> 4) - ttx + 1)] & 32u) != 0u;
+    };
+    auto in_canvas = [&](int gx, int gy) -> bool { return gx >= 0 && gy >= 0 && gx < W_ && gy < H_; };
+    // stage the window: heights before the apply, relief, offsets before the
+    // nose, rheology, stored front normals
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(SB_N * SB_N); q += (uint32_t)BT) {
+        const int gx = bx0 + (int)(q % SB_N), gy = by0 + (int)(q / SB_N);
+        if (in_canvas(gx, gy)) {
+            const uint32_t nn = idx(F, (uint32_t)gx, (uint32_t)gy);
+            sw.h[q] = act_w(gx, gy) ? hr[nn] : F.h[nn];
+            sw.m[q] = F.m[nn];
+            sw.d[q] = F.delta[nn];
+            sw.rheo[q] = F.rheo[nn];
+            sw.fn[q] = F.fnorm[nn];
+        } else {
+            sw.h[q] = 0.0f;
+            sw.m[q] = 0.0f;
+            sw.d[q] = 0u;
+            sw.rheo[q] = make_float2(0.0f, 0.0f);
+            sw.fn[q] = make_float2(0.0f, 0.0f);
+        }
+    }
+    __syncthreads();
+    auto bq = [&](int gx, int gy) -> int { return (gy - by0) * SB_N + (gx - bx0); };
+    auto in_b = [&](int gx, int gy) -> bool {
+        return gx >= bx0 && gy >= by0 && gx < bx0 + SB_N && gy < by0 + SB_N;
+    };
+    // nose of cell (i, j) from the window (as lap_plug_tile's)
+    auto nose_w = [&](int i, int j, uint32_t& d_out, float2& fn_out) {
+        d_out = 0u;
+        fn_out = make_float2(0.0f, 0.0f);
+        if (sw.h[bq(i, j)] <= 0.0f) return;
+        int bxx = 64, byy = 64;
+        float bd = 1e9f;
+        for (int dj = -1; dj <= 1; dj++) {
+            for (int di = -1; di <= 1; di++) {
+                const int xx = i + di, yy = j + dj;
+                if (xx < 0 || yy < 0 || xx >= W_ || yy >= H_) continue;
+                int cx = di, cy = dj;
+                if (sw.h[bq(xx, yy)] > 0.0f) {
+                    const uint32_t dv = sw.d[bq(xx, yy)];
+                    if ((dv & 0x10000u) == 0u) continue;
+                    int ox, oy;
+                    unpack_off(dv, ox, oy);
+                    cx = ox + di;
+                    cy = oy + dj;
+                    const int tx = i + cx, ty = j + cy;
+                    if (tx < 0 || ty < 0 || tx >= W_ || ty >= H_) continue;
+                    const float ht = in_b(tx, ty) ? sw.h[bq(tx, ty)] : h_before(F, hr, tx, ty);
+                    if (ht > 0.0f) continue;
+                } else if (di == 0 && dj == 0) {
+                    continue;
+                }
+                const float d = cz_len2((float)cx, (float)cy);
+                if (d < bd) { bd = d; bxx = cx; byy = cy; }
+            }
+        }
+        if (bd > 1e8f) return;
+        d_out = pack_off(bxx, byy);
+        if (bd <= 3.0f) {
+            float dx = 0.0f, dy = 0.0f;
+            for (int dj = -2; dj <= 2; dj++) {
+                for (int di = -2; di <= 2; di++) {
+                    if (di == 0 && dj == 0) continue;
+                    const int xx = i + di, yy = j + dj;
+                    if (xx < 0 || yy < 0 || xx >= W_ || yy >= H_) continue;
+                    if (di * di + dj * dj > 5) continue;
+                    if (sw.h[bq(xx, yy)] <= F.P.ein[3]) {
+                        dx = dx + FN_QUOT.v[di + 2][dj + 2];
+                        dy = dy + FN_QUOT.v[dj + 2][di + 2];
+                    }
+                }
+            }
+            const float l = cz_len2(dx, dy);
+            fn_out = (l < 1e-6f) ? make_float2(0.0f, 0.0f) : make_float2(dx / l, dy / l);
+        }
+    };
+    // nose region: the offsets and normals the laplacian reads, usurf
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(SN_N * SN_N); q += (uint32_t)BT) {
+        const int gx = nx0 + (int)(q % SN_N), gy = ny0 + (int)(q / SN_N);
+        uint32_t dv = 0u;
+        float2 fn = make_float2(0.0f, 0.0f);
+        float us = 0.0f;
+        if (in_canvas(gx, gy)) {
+            if (do_nose && act_w(gx, gy)) {
+                nose_w(gx, gy, dv, fn);
+            } else {
+                dv = sw.d[bq(gx, gy)];
+                fn = sw.fn[bq(gx, gy)];
+            }
+            const float hc = sw.h[bq(gx, gy)];
+            float hold = 0.0f;
+            if (hc > 0.0f) hold = hold_level(F, nose_dist_b(dv));
+            us = hc + sw.m[bq(gx, gy)] - hold;
+        }
+        sw.nd[q] = dv;
+        sw.nfn[q] = fn;
+        sw.us[q] = us;
+    }
+    // plug: hb_flux once per window cell
+    if (plug_on) {
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(SB_N * SB_N); q += (uint32_t)BT) {
+            float hn = sw.h[q], qv = 0.0f;
+            bool yl = false;
+            if (hn > 0.0f) {
+                const float2 r = sw.rheo[q];
+                yl = r.y > 0.0f;
+                float oy_, oup_;
+                float nq = F.P.phys[2];
+                if (cell_n_on(F))
+                    nq = F.n_cell[idx(F, (uint32_t)(bx0 + (int)(q % SB_N)), (uint32_t)(by0 + (int)(q / SB_N)))];
+                qv = hb_flux(nq, gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
+            } else {
+                hn = 0.0f;
+            }
+            sw.hq[q] = make_float2(yl ? -hn : hn, qv);
+        }
+    }
+    __syncthreads();
+    auto nqi = [&](int gx, int gy) -> int { return (gy - ny0) * SN_N + (gx - nx0); };
+    // laplacian, pressures and plug means over the flux window (active
+    // cells; cells of inactive tiles keep their stored values), and the flux
+    // window's other fields
+    constexpr int LN = FluxWinT<SL_R>::N;
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(LN * LN); q += (uint32_t)BT) {
+        const int i = lx0 + (int)(q % LN), j = ly0 + (int)(q / LN);
+        if (!in_canvas(i, j)) continue;
+        const uint32_t c = idx(F, (uint32_t)i, (uint32_t)j);
+        const bool act = act_w(i, j);
+        const float hv = sw.h[bq(i, j)];
+        const uint32_t dl = sw.nd[nqi(i, j)];
+        fw.h[q] = hv;
+        fw.rheo[q] = sw.rheo[bq(i, j)];
+        fw.lmu[q] = cz_logf(cz_max(fw.rheo[q].x, 1e-4f));
+        const uint32_t dflux = act ? dl : (do_nose ? F.delta_out[c] : sw.d[bq(i, j)]);
+        fw.d[q] = dflux;
+        fw.fn[q] = sw.nfn[nqi(i, j)];
+        fw.fd[q] = hv <= 0.0f ? 0.0f : nose_dist_b(dflux);
+        if (!act) {
+            fw.pres[q] = F.pres[c];
+            fw.tmp[q] = F.tmp[c];
+            continue;
+        }
+        auto usurf_l = [&](int ii, int jj) -> float {
+            return sw.us[nqi((int)refl_i(F, ii), (int)refl_j(F, jj))];
+        };
+        const float s0 = hv + sw.m[bq(i, j)];
+        float hold = 0.0f, pl = 0.0f;
+        const float dself = nose_dist_b(dl);
+        if (hv > 0.0f) hold = hold_level(F, dself);
+        const float u0 = s0 - hold;
+        const float ax = usurf_l(i - 1, j) + usurf_l(i + 1, j) + usurf_l(i, j - 1) + usurf_l(i, j + 1);
+        const float dg = usurf_l(i - 1, j - 1) + usurf_l(i + 1, j - 1) + usurf_l(i - 1, j + 1) + usurf_l(i + 1, j + 1);
+        const float l = (W_AXIS * ax + W_DIAG_LAP * dg - (4.0f * W_AXIS + 4.0f * W_DIAG_LAP) * u0) * F.P.inv_dx * F.P.inv_dx;
+        if (hv > 0.0f) {
+            const float d = dself;
+            if (d <= 3.0f) {
+                const int own = nqi(i, j);
+                const float2 nn = sw.nfn[own];
+                if (nn.x * nn.x + nn.y * nn.y > 0.5f) {
+                    float lp = 0.0f;
+                    if (!(i < 1 || j < 1 || i + 1 >= W_ || j + 1 >= H_)) {
+                        const float2 nl = sw.nfn[own - 1], nr = sw.nfn[own + 1];
+                        const float2 nd = sw.nfn[own - SN_N], nu = sw.nfn[own + SN_N];
+                        if (!(nl.x * nl.x + nl.y * nl.y < 0.5f || nr.x * nr.x + nr.y * nr.y < 0.5f ||
+                              nd.x * nd.x + nd.y * nd.y < 0.5f || nu.x * nu.x + nu.y * nu.y < 0.5f)) {
+                            const float curv = (nr.x - nl.x + nu.y - nd.y) * 0.5f * F.P.inv_dx;
+                            lp = F.P.phys[1] * curv * F.P.front2[2];
+                        }
+                    }
+                    pl = lp * cz_expf(-(d - 1.0f) * 0.7f);
+                }
+            }
+        }
+        fw.pres[q] = make_float2(-F.P.phys[1] * l + disjoining(F, hv), F.P.phys[0] * F.P.g[2] * (s0 - hold) + pl);
+        float tv = 0.0f;
+        if (hv > 0.0f && plug_on) {
+            float sum = 0.0f, mass = 0.0f;
+            bool yielding = false;
+            const int wq0 = (j - PLUG_RADIUS - by0) * SB_N + (i - PLUG_RADIUS - bx0);
+            for (int dy = 0; dy < 2 * PLUG_RADIUS + 1; dy++) {
+                for (int dxi = 0; dxi < 2 * PLUG_RADIUS + 1; dxi++) {
+                    const float2 v = sw.hq[wq0 + dy * SB_N + dxi];
+                    if (v.x == 0.0f) continue;
+                    if (v.x < 0.0f) yielding = true;
+                    sum = sum + v.y;
+                    mass = mass + cz_abs(v.x);
+                }
+            }
+            if (yielding && mass > 0.0f) tv = sum / mass;
+        }
+        fw.tmp[q] = tv;
+    }
+    __syncthreads();
+    // the tile's own outputs of the lap/plug part
+    if (in_grid && cell_th) {
+        const float hv = sw.h[bq((int)x, (int)y)];
+        if (do_clear == 1 || (do_clear == 2 && F.imp_steps == 0u)) {
+            F.acc[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            F.h_frame[c] = hv;
+        }
+        if (do_nose) {
+            F.delta_out[c] = sw.nd[nqi((int)x, (int)y)];
+            F.fnorm[c] = sw.nfn[nqi((int)x, (int)y)];
+        }
+        F.tmp[c] = fw.tmp[((int)y - ly0) * LN + ((int)x - lx0)];
+    }
+    // fluxes of the tile + 2 ring, one face per thread
+    const FusedRead g{F, fw, hr, lx0, ly0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0]), do_nose};
+    for (uint32_t it = threadIdx.x; it < (uint32_t)(AP_WIN * AP_WIN * 4); it += (uint32_t)BT) {
+        const uint32_t q = it >> 2, k = it & 3u;
+        const int gx = tx0 - AP_R + (int)(q % AP_WIN), gy = ty0 - AP_R + (int)(q / AP_WIN);
+        if (in_canvas(gx, gy)) {
+            if (k == 0u) S.ap.s_hh[q] = fw.h[g.q(gx, gy)];
+            float qk = 0.0f;
+            if (g.active(gx, gy)) {
+                const float2 p00 = pressure_at(F, g, gx, gy);
+                FaceGeom fg = face_geom(F, g, gx, gy, (int)k, true, p00);
+                if (fg.has) {
+                    FaceFlux ff = face(F, g, gx, gy, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+                    qk = ff.q * ((k < 2u) ? W_AXIS : W_DIAG_FLUX);
+                }
+            }
+            ((float*)&S.ap.s_f[q])[k] = qk;
+        }
+    }
+    __syncthreads();
+    FilmDev Fw = F;
+    Fw.h = hw;
+    apply_core<BT>(Fw, _t, S.ap);
+}
+
+template <class WS>
+__global__ __launch_bounds__(1024) void k_substep_fused(WS ws, const float* hr, float* hw, int do_nose, int do_clear) {
+    __shared__ SubstepFusedShared S;
+    CZ_FOR_TILES(ws) substep_fused_tile(F, hr, hw, _t, S, do_nose, do_clear);
+}
+
+// ---------------------------------------------------------------------------
 // All substeps of a frame in one launch, for small active sets, where the
 // three launches per substep would cost more in dispatch gaps than in work:
 // the lap/plug, flux and limiter+apply passes of every substep run over the
@@ -2523,6 +2908,7 @@ CZ_INST2(k_lap_plug_w, int, int, int)
 CZ_INST2(k_fluxes_w)
 CZ_INST2(k_limit_apply_w)
 CZ_INST2(k_flux_apply_w)
+CZ_INST1(k_substep_fused, OneFilm, const float*, float*, int, int)
 CZ_INST1(k_mark, OneFilmAll)
 CZ_INST1(k_shade, OneFilmAll)
 CZ_INST1(k_mark, ManyFilms)

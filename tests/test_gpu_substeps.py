@@ -1,6 +1,7 @@
 """The substeps of a frame launched as one kernel with grid-wide barriers
-between the passes (k_substeps), and the flux and apply passes fused into
-one launch, give the bits of the three launches per substep: films in each
+between the passes (k_substeps), the flux and apply passes fused into one
+launch, and each whole substep in one launch (k_substep_fused) give the bits
+of the three launches per substep: films in each
 launch mode step the same scenes (a settling blob
 canvas with deposits, a thick canvas with implicit steps, and a canvas whose
 active set outgrows the one-launch grid so that blocks take several tiles)."""
@@ -43,9 +44,9 @@ def test_one_launch_equals_per_pass(gpu, case, graphs):
     s.height_m = h * s.width_m / w
     st = scenes.blobs_state(w, h, **kw)
     films = []
-    # (substep launch mode, flux shape): per-pass, one launch, automatic, and
-    # per-pass with flux + apply fused
-    for mode, shape in ((0, 0), (1, 0), (-1, 0), (0, 2)):
+    # (substep launch mode, flux shape): per-pass, one launch, automatic,
+    # per-pass with flux + apply fused, and the whole substep in one launch
+    for mode, shape in ((0, 0), (1, 0), (-1, 0), (0, 2), (0, 5)):
         f = czf.Film(w, h, s)
         f.substep_launch(mode)
         f.use_graphs(graphs)
@@ -71,11 +72,15 @@ def test_one_launch_equals_per_pass(gpu, case, graphs):
             assert_same(films[0], films[1], f"{case} frame {fr + 1} forced")
             assert_same(films[0], films[2], f"{case} frame {fr + 1} auto")
             assert_same(films[0], films[3], f"{case} frame {fr + 1} fused flux + apply")
+            assert_same(films[0], films[4], f"{case} frame {fr + 1} whole substep fused")
     for f in films:
         f.sync()
     assert_same(films[0], films[1], f"{case} end forced")
     assert_same(films[0], films[2], f"{case} end auto")
     assert_same(films[0], films[3], f"{case} end fused flux + apply")
+    assert_same(films[0], films[4], f"{case} end whole substep fused")
+    used = list(films[4].stats().flux_shape_cell_substeps)
+    assert used[5] > 0 and sum(used) == used[4] + used[5], used
     if case == "implicit":
         assert implicit_seen, "the thick canvas should take implicit steps"
     if case == "large":
