@@ -261,8 +261,14 @@ static inline float face_n(const ofilm* F, uint32_t ca, uint32_t cb, float wa, f
     return cell_n_on(F) ? wa * F->n_cell[ca] + wb * F->n_cell[cb] : F->P.phys[2];
 }
 
-/* film.wgsl:94-107: returns q, writes y and up; n = the power-law index */
-static inline float hb_flux_n(float n, float gmag, float hf, float mu, float ty, float* oy, float* oup) {
+/* film.wgsl:94-107: returns q, writes y and up; n = the power-law index.
+ * The shader's power-law velocity scale n/(n+1) pow(g/mu, 1/n)
+ * pow(y, (n+1)/n), with pow(x, e) = exp(e ln x) (the WGSL definition), is
+ * the same real number as n/(n+1) exp((ln g - ln mu + (n+1) ln y) / n), which
+ * is evaluated here.  mu_is_log: `mu` holds ln mu (the faces pass the
+ * height-weighted mean of the cells' ln mu, whose exponential is the
+ * shader's face viscosity). */
+static inline float hb_flux_v(float n, float gmag, float hf, float mu, int mu_is_log, float ty, float* oy, float* oup) {
     *oy = 0.0f;
     *oup = 0.0f;
     if (gmag < 1e-9f || hf <= 0.0f) return 0.0f;
@@ -270,14 +276,19 @@ static inline float hb_flux_n(float n, float gmag, float hf, float mu, float ty,
     if (y <= 0.0f) return 0.0f;
     float up;
     if (cz_abs(n - 1.0f) < 1e-4f) {
-        up = 0.5f * gmag / mu * y * y;
+        float mv = mu_is_log ? cz_expf(mu) : mu;
+        up = 0.5f * gmag / mv * y * y;
     } else {
-        up = n / (n + 1.0f) * cz_powf(gmag / mu, 1.0f / n) * cz_powf(y, (n + 1.0f) / n);
+        float lm = mu_is_log ? mu : cz_logf(mu);
+        up = n / (n + 1.0f) * cz_expf((cz_logf(gmag) - lm + (n + 1.0f) * cz_logf(y)) / n);
     }
     float q = up * (hf - n * y / (2.0f * n + 1.0f));
     *oy = y;
     *oup = up;
     return q;
+}
+static inline float hb_flux_n(float n, float gmag, float hf, float mu, float ty, float* oy, float* oup) {
+    return hb_flux_v(n, gmag, hf, mu, 0, ty, oy, oup);
 }
 static inline float hb_flux(const ofilm* F, float gmag, float hf, float mu, float ty, float* oy, float* oup) {
     return hb_flux_n(F->P.phys[2], gmag, hf, mu, ty, oy, oup);
@@ -491,7 +502,8 @@ static FaceFlux face(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd,
     float rbx = F->rheo[2 * cb], rby = F->rheo[2 * cb + 1];
     float wa = ha / (ha + hb);
     float wb = 1.0f - wa;
-    float mu = cz_expf(wa * cz_logf(cz_max(rax, 1e-4f)) + wb * cz_logf(cz_max(rbx, 1e-4f)));
+    /* ln of the face viscosity exp(wa ln mu_a + wb ln mu_b) */
+    float lmu = wa * cz_logf(cz_max(rax, 1e-4f)) + wb * cz_logf(cz_max(rbx, 1e-4f));
     float ty = wa * ray + wb * rby;
     float nf = face_n(F, ca, cb, wa, wb);
     float thin = cz_min(ha, hb) / hf;
@@ -501,12 +513,12 @@ static FaceFlux face(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd,
     v2 g = {yc.x + ghyd.x + gtilt.x, yc.y + ghyd.y + gtilt.y};
     float gmag = cz_len2(g.x, g.y);
     float qy_y, qy_up;
-    float qy_x = hb_flux_n(nf, gmag, yield_depth(F, ca, cb, hf, ty, g), mu, ty, &qy_y, &qy_up);
+    float qy_x = hb_flux_v(nf, gmag, yield_depth(F, ca, cb, hf, ty, g), lmu, 1, ty, &qy_y, &qy_up);
     float meff = (qy_x > 0.0f) ? qy_x / cz_max(gmag, 1e-9f) : 0.0f;
     float mtilt = meff;
     if (ty > 0.0f) mtilt = plug_mobility(F, ca, cb, gtilt);
     if (meff <= 0.0f && mtilt <= 0.0f) return out;
-    float fcap2 = cz_powf(thin, F->P.front3[3]);
+    float fcap2 = cz_pow_fcap(thin, F->P.front3[3]);
     float qn = mtilt * gtilt.x + cz_min(meff, F->P.misc[3]) * ghyd.x + cz_min(meff, F->P.phys[3]) * (gcap.x * fcap2);
     float trs;
     float trx = rim_translation(F, ca, cb, hf, face_dir, &trs);
@@ -528,8 +540,7 @@ static FaceFlux face(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd,
             } else {
                 uint32_t tgt = cb;
                 if (hb > ha) tgt = ca;
-                uint32_t nf = feeders(F, tgt);
-                qn = qn / cz_powf((float)(nf > 1u ? nf : 1u), 0.7f);
+                qn = qn / cz_feed_pow(feeders(F, tgt)); /* pow(max(nf, 1), 0.7) */
             }
         }
     }
@@ -2155,7 +2166,7 @@ static float face_mob(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd
     float rbx = F->rheo[2 * cb], rby = F->rheo[2 * cb + 1];
     float wa = ha / (ha + hb);
     float wb = 1.0f - wa;
-    float mu = cz_expf(wa * cz_logf(cz_max(rax, 1e-4f)) + wb * cz_logf(cz_max(rbx, 1e-4f)));
+    float lmu = wa * cz_logf(cz_max(rax, 1e-4f)) + wb * cz_logf(cz_max(rbx, 1e-4f));
     float ty = wa * ray + wb * rby;
     float nf = face_n(F, ca, cb, wa, wb);
     float thin = cz_min(ha, hb) / hf;
@@ -2165,15 +2176,14 @@ static float face_mob(const ofilm* F, uint32_t ca, uint32_t cb, v2 gcap, v2 ghyd
     v2 g = {yc.x + ghyd.x + gtilt.x, yc.y + ghyd.y + gtilt.y};
     float gmag = cz_len2(g.x, g.y);
     float qy, qu;
-    float q = hb_flux_n(nf, gmag, yield_depth(F, ca, cb, hf, ty, g), mu, ty, &qy, &qu);
+    float q = hb_flux_v(nf, gmag, yield_depth(F, ca, cb, hf, ty, g), lmu, 1, ty, &qy, &qu);
     if (q <= 0.0f) return 0.0f;
     float mob = cz_min(cz_max(q / gmag - F->P.misc[3], 0.0f), F->P.front[2]);
     if (cz_min(ha, hb) < 0.5f * cz_max(ha, hb)) {
         if (diag) return 0.0f;
         uint32_t tgt = cb;
         if (hb > ha) tgt = ca;
-        uint32_t nf = feeders(F, tgt);
-        mob = mob / cz_powf((float)(nf > 1u ? nf : 1u), 0.7f);
+        mob = mob / cz_feed_pow(feeders(F, tgt));
     }
     return mob;
 }
@@ -3117,6 +3127,8 @@ void of_cg_apply(ofilm* F, const float* p, float* q) {
 float of_expf(float x) { return cz_expf(x); }
 float of_logf(float x) { return cz_logf(x); }
 float of_powf(float x, float y) { return cz_powf(x, y); }
+float of_pow_fcap(float x, float y) { return cz_pow_fcap(x, y); }
+float of_feed_pow(uint32_t k) { return cz_feed_pow(k); }
 float of_atanf(float x) { return cz_atanf(x); }
 float of_atan2f(float y, float x) { return cz_atan2f(y, x); }
 uint32_t of_f2h(float x) { return cz_f2h(x); }

@@ -114,6 +114,10 @@ def lib():
             getattr(L, name).argtypes = [f32]
         L.of_powf.restype = f32
         L.of_powf.argtypes = [f32, f32]
+        L.of_pow_fcap.restype = f32
+        L.of_pow_fcap.argtypes = [f32, f32]
+        L.of_feed_pow.restype = f32
+        L.of_feed_pow.argtypes = [u32]
         L.of_atan2f.restype = f32
         L.of_atan2f.argtypes = [f32, f32]
         L.of_f2h.restype = u32

@@ -12,7 +12,7 @@
  *         that subnormal results round once.                (<= 2 ULP)
  *   log   x = 2^e m, m in [sqrt(1/2), sqrt(2)], log(m) = 2 atanh(s),
  *         s = (m-1)/(m+1), odd series to s^11.               (<= 2 ULP)
- *   pow   exp(y log x), the WGSL definition.
+ *   pow   exp(y log x), the WGSL definition; x^2.5 as x^2 sqrt(x).
  *   atan  reduction to |t| <= tan(pi/8) by 1/x and (x-1)/(x+1), odd Taylor
  *         series to t^19.                                    (<= 2 ULP)
  *   f16   IEEE round-to-nearest-even with subnormals (pack2x16float), and
@@ -120,6 +120,21 @@ CZ_INLINE float cz_logf(float x) {
 }
 
 CZ_INLINE float cz_powf(float x, float y) { return cz_expf(y * cz_logf(x)); }
+
+/* x^y for x >= 0 and the capillary cap's exponent y: x^2.5 (its default) as
+ * x^2 sqrt(x), equal in real arithmetic and within 2 ULP of the exact
+ * power (three roundings); any other exponent through cz_powf. */
+CZ_INLINE float cz_pow_fcap(float x, float y) { return (y == 2.5f) ? (x * x) * sqrtf(x) : cz_powf(x, y); }
+
+/* max(k, 1)^0.7 for the feeder count k = 0..4 of a face: the values cz_powf
+ * gives (1 and the bit patterns of cz_powf(2, 0.7), cz_powf(3, 0.7),
+ * cz_powf(4, 0.7)), so a lookup returns the same bits as the call. */
+CZ_INLINE float cz_feed_pow(uint32_t k) {
+    return (k <= 1u) ? 1.0f
+         : (k == 2u) ? cz_bits2f(0x3fcfefc6u)
+         : (k == 3u) ? cz_bits2f(0x400a1741u)
+                     : cz_bits2f(0x4028e5a3u);
+}
 
 CZ_INLINE float cz_atanf(float x) {
     if (x != x) return x;

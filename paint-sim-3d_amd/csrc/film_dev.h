@@ -94,7 +94,50 @@ struct FilmDev {
     uint2* color;          // [n] shade output: rgb + roughness, f16 x 4
     uint2* detail;         // [n] shade output: method + turn, cure, yield stress, thickness, f16 x 4
     float* n_cell;         // [n] per-cell power-law index (with CZ_FLAG_CELL_N)
+    // Constants of the face terms that depend on P alone, computed when the
+    // descriptor is built (derive_consts below) with the IEEE operations the
+    // face code would otherwise repeat per face: the same bits.
+    float k_hold1;         // hold_level(1) (film.wgsl:154-156)
+    float k_theta;         // atan(tan theta), the static contact angle (film.wgsl:238)
+    float k_rim_reach;     // 3 / front.y: reach of the rim translation (film.wgsl:273)
+    float k_gc_max;        // sigma / meniscus^2, yield_cap's bound (film.wgsl:46)
+    float k_gb_plug;       // rho |g_xy|, the plug's tilt gradient (film.wgsl:62)
+    float k_lg_plug;       // ln(rho |g_xy|)
+    float k_gt_norm[4];    // |g_tilt| of the owned faces +x, +y, +x+y, +x-y (plug_mobility, film.wgsl:52)
 };
+
+// film.wgsl:154-156, on the parameter block (host and device)
+CZ_HD inline float hold_level_p(const cz_params& P, float d) {
+    return P.front[0] * (1.0f - cz_expf(-(cz_max(d, 1.0f) - 0.5f) * P.front[1]));
+}
+
+// The tilt gradient of owned face k (film.wgsl:387-439): +x, +y, +x+y, +x-y.
+CZ_HD inline void face_tilt(const cz_params& P, int k, float& tx, float& ty) {
+    const float gx = P.phys[0] * P.g[0], gy = P.phys[0] * P.g[1];
+    const float s2 = 0.70710678f;
+    if (k == 0) { tx = gx; ty = gy; }
+    else if (k == 1) { tx = gy; ty = gx; }
+    else if (k == 2) { tx = (gx + gy) * s2; ty = (gx - gy) * s2; }
+    else { tx = (gx - gy) * s2; ty = (gx + gy) * s2; }
+}
+
+constexpr float YIELD_MENISCUS_SQ = YIELD_MENISCUS * YIELD_MENISCUS;
+
+// Fill the k_* constants of a descriptor from its P.
+inline void derive_consts(FilmDev& F) {
+    const cz_params& P = F.P;
+    F.k_hold1 = hold_level_p(P, 1.0f);
+    F.k_theta = cz_atanf(P.ein[0]);
+    F.k_rim_reach = 3.0f / P.front[1];
+    F.k_gc_max = P.phys[1] / YIELD_MENISCUS_SQ;
+    F.k_gb_plug = P.phys[0] * cz_len2(P.g[0], P.g[1]);
+    F.k_lg_plug = cz_logf(F.k_gb_plug);
+    for (int k = 0; k < 4; k++) {
+        float tx, ty;
+        face_tilt(P, k, tx, ty);
+        F.k_gt_norm[k] = cz_len2(tx, ty);
+    }
+}
 
 __device__ __forceinline__ uint32_t idx(const FilmDev& F, uint32_t i, uint32_t j) { return j * F.W + i; }
 
@@ -119,21 +162,12 @@ __device__ __forceinline__ float strata_n(const FilmDev& F, const float* t, cons
     return (tot > 0.0f) ? acc / tot : F.P.phys[2];
 }
 
-// shaders/common.wgsl:152-166
-__device__ __forceinline__ uint32_t refl_i(const FilmDev& F, int i) {
-    int w = (int)F.W;
-    int r = i;
-    if (r < 0) r = -r - 1;
-    if (r >= w) r = 2 * w - r - 1;
-    return (uint32_t)cz_clamp_i(r, 0, w - 1);
-}
-__device__ __forceinline__ uint32_t refl_j(const FilmDev& F, int j) {
-    int hh = (int)F.H;
-    int r = j;
-    if (r < 0) r = -r - 1;
-    if (r >= hh) r = 2 * hh - r - 1;
-    return (uint32_t)cz_clamp_i(r, 0, hh - 1);
-}
+// Mirrored coordinates (shaders/common.wgsl:152-166: -1 - i below 0,
+// 2W - 1 - i from W, then clamped), for coordinates at most one cell outside
+// the canvas, which is every caller's case (-1 <= i <= W): the mirror of -1
+// is 0 and of W is W - 1, i.e. the clamp.
+__device__ __forceinline__ uint32_t refl1_i(const FilmDev& F, int i) { return (uint32_t)cz_clamp_i(i, 0, (int)F.W - 1); }
+__device__ __forceinline__ uint32_t refl1_j(const FilmDev& F, int j) { return (uint32_t)cz_clamp_i(j, 0, (int)F.H - 1); }
 
 // common.wgsl:127-130
 __device__ __forceinline__ bool cell_active(const FilmDev& F, uint32_t i, uint32_t j) {
@@ -193,20 +227,31 @@ struct FnQuot {
 __device__ constexpr FnQuot FN_QUOT{};
 
 // film.wgsl:154-156
-__device__ __forceinline__ float hold_level(const FilmDev& F, float d) {
-    return F.P.front[0] * (1.0f - cz_expf(-(cz_max(d, 1.0f) - 0.5f) * F.P.front[1]));
-}
+__device__ __forceinline__ float hold_level(const FilmDev& F, float d) { return hold_level_p(F.P, d); }
 
 // film.wgsl:44-49
 __device__ __forceinline__ float2 yield_cap(const FilmDev& F, float2 gc, float ty) {
     float gcl = cz_len2(gc.x, gc.y);
-    float gc_max = F.P.phys[1] / (YIELD_MENISCUS * YIELD_MENISCUS);
+    float gc_max = F.k_gc_max;
     if (ty <= 0.0f || gcl <= gc_max) return gc;
     return make_float2(gc.x * (gc_max / gcl), gc.y * (gc_max / gcl));
 }
 
-// film.wgsl:94-107
-__device__ __forceinline__ float hb_flux(float n, float gmag, float hf, float mu, float ty, float& oy, float& oup) {
+// The viscosity argument of hb_flux: the value, or its logarithm.
+struct Visc {
+    float v;
+    bool is_log;
+};
+
+// film.wgsl:94-107.  The power-law velocity scale n/(n+1) (g/mu)^(1/n)
+// y^((n+1)/n) is evaluated as n/(n+1) exp((ln g - ln mu + (n+1) ln y) / n),
+// one exponential and three logarithms at most where the shader's two pow
+// calls (exp(e ln x) each, by the WGSL definition) take two of each and two
+// more divisions.  lg: ln g when the caller has it (lg_known); mu: the
+// viscosity or its logarithm, whichever the caller has; the Newtonian
+// branch reads the value, the power-law branch the logarithm.
+__device__ __forceinline__ float hb_flux(float n, float gmag, bool lg_known, float lg, float hf, Visc mu, float ty,
+                                         float& oy, float& oup) {
     oy = 0.0f;
     oup = 0.0f;
     if (gmag < 1e-9f || hf <= 0.0f) return 0.0f;
@@ -214,9 +259,12 @@ __device__ __forceinline__ float hb_flux(float n, float gmag, float hf, float mu
     if (y <= 0.0f) return 0.0f;
     float up;
     if (cz_abs(n - 1.0f) < 1e-4f) {
-        up = 0.5f * gmag / mu * y * y;
+        const float mv = mu.is_log ? cz_expf(mu.v) : mu.v;
+        up = 0.5f * gmag / mv * y * y;
     } else {
-        up = n / (n + 1.0f) * cz_powf(gmag / mu, 1.0f / n) * cz_powf(y, (n + 1.0f) / n);
+        const float lm = mu.is_log ? mu.v : cz_logf(mu.v);
+        const float lgv = lg_known ? lg : cz_logf(gmag);
+        up = n / (n + 1.0f) * cz_expf((lgv - lm + (n + 1.0f) * cz_logf(y)) / n);
     }
     float q = up * (hf - n * y / (2.0f * n + 1.0f));
     oy = y;
@@ -267,8 +315,8 @@ struct GlobalRead {
     __device__ __forceinline__ float fdist(int i, int j) const {
         return h(i, j) <= 0.0f ? 0.0f : nose_dist_b(delta(i, j));
     }
-    __device__ __forceinline__ float hold1() const { return hold_level(F, 1.0f); }
-    __device__ __forceinline__ float theta() const { return cz_atanf(F.P.ein[0]); }
+    __device__ __forceinline__ float hold1() const { return F.k_hold1; }
+    __device__ __forceinline__ float theta() const { return F.k_theta; }
 };
 
 // film.wgsl:134-145
@@ -394,7 +442,7 @@ __device__ __forceinline__ float rim_translation(const FilmDev& F, const A& a, i
     if ((((uint32_t)F.P.front2[3]) & 1u) != 0u) return 0.0f;
     float da = front_dist(a, ia, ja);
     float db = front_dist(a, ib, jb);
-    if (cz_max(da, db) > 3.0f / F.P.front[1]) return 0.0f;
+    if (cz_max(da, db) > F.k_rim_reach) return 0.0f;
     float ha = a.h(ia, ja), hb = a.h(ib, jb);
     float nx = 0.0f, ny = 0.0f;
     if (da > db && ha > 0.0f) {
@@ -429,9 +477,10 @@ __device__ __forceinline__ float rim_translation(const FilmDev& F, const A& a, i
 struct FaceFlux { float q, y, up, n; };
 
 // film.wgsl:297-372: face between cell a = (ia, ja) and cell b = (ib, jb)
+// gtn: |gtilt|, the face's entry of F.k_gt_norm
 template <class A>
 __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia, int ja, int ib, int jb, float2 gcap,
-                                float2 ghyd, float2 gtilt, bool diag, float2 face_dir) {
+                                float2 ghyd, float2 gtilt, float gtn, bool diag, float2 face_dir) {
     FaceFlux out = {0.0f, 0.0f, 0.0f, F.P.phys[2]};
     float ha = acc.h(ia, ja);
     float hb = acc.h(ib, jb);
@@ -457,7 +506,8 @@ __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia,
     float2 rb = acc.rheo(ib, jb);
     float wa = ha / (ha + hb);
     float wb = 1.0f - wa;
-    float mu = cz_expf(wa * acc.lmu(ia, ja) + wb * acc.lmu(ib, jb));
+    // the face viscosity exp(wa ln mu_a + wb ln mu_b), kept as its logarithm
+    const Visc mu{wa * acc.lmu(ia, ja) + wb * acc.lmu(ib, jb), true};
     float ty = wa * ra.y + wb * rb.y;
     const float nf = cell_n_on(F) ? wa * acc.ncell(ia, ja) + wb * acc.ncell(ib, jb) : F.P.phys[2];
     float thin = cz_min(ha, hb) / hf;
@@ -467,12 +517,12 @@ __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia,
     float gmag = cz_len2(g.x, g.y);
     float ydep = (ty <= 0.0f) ? hf : ((g.x >= 0.0f) ? ha : hb);
     float qy_y, qy_up;
-    float qy_x = hb_flux(nf, gmag, ydep, mu, ty, qy_y, qy_up);
+    float qy_x = hb_flux(nf, gmag, false, 0.0f, ydep, mu, ty, qy_y, qy_up);
     float meff = (qy_x > 0.0f) ? qy_x / cz_max(gmag, 1e-9f) : 0.0f;
     float mtilt = meff;
     if (ty > 0.0f) {
         // plug_mobility (film.wgsl:51-56)
-        float gb = cz_len2(gtilt.x, gtilt.y);
+        const float gb = gtn;
         if (gb < 1e-6f) {
             mtilt = 0.0f;
         } else {
@@ -481,7 +531,7 @@ __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia,
         }
     }
     if (meff <= 0.0f && mtilt <= 0.0f) return out;
-    float fcap2 = cz_powf(thin, F.P.front3[3]);
+    float fcap2 = cz_pow_fcap(thin, F.P.front3[3]);
     float qn = mtilt * gtilt.x + cz_min(meff, F.P.misc[3]) * ghyd.x + cz_min(meff, F.P.phys[3]) * (gcap.x * fcap2);
     float trs;
     float trx = rim_translation(F, acc, ia, ja, ib, jb, hf, face_dir, trs);
@@ -502,8 +552,7 @@ __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia,
                 qn = 0.0f;
             } else {
                 const int ti = (hb > ha) ? ia : ib, tj = (hb > ha) ? ja : jb;
-                uint32_t nf = feeders(F, acc, ti, tj);
-                qn = qn / cz_powf((float)(nf > 1u ? nf : 1u), 0.7f);
+                qn = qn / cz_feed_pow(feeders(F, acc, ti, tj));
             }
         }
     }
@@ -514,15 +563,17 @@ __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia,
     return out;
 }
 
+// pressure_at / pressure_wet: (i, j) at most one cell outside the canvas, as
+// every face's neighbours are
 template <class A>
 __device__ __forceinline__ float2 pressure_at(const FilmDev& F, const A& a, int i, int j) {
-    return a.pres((int)refl_i(F, i), (int)refl_j(F, j));
+    return a.pres((int)refl1_i(F, i), (int)refl1_j(F, j));
 }
 
 // film.wgsl:9-14
 template <class A>
 __device__ __forceinline__ float2 pressure_wet(const FilmDev& F, const A& a, int i, int j, float2 fallback) {
-    const int ri = (int)refl_i(F, i), rj = (int)refl_j(F, j);
+    const int ri = (int)refl1_i(F, i), rj = (int)refl1_j(F, j);
     float2 p = a.pres(ri, rj);
     if (a.h(ri, rj) > 0.0f) return p;
     return make_float2(fallback.x, p.y);
@@ -533,6 +584,7 @@ struct FaceGeom {
     bool diag;
     int ib, jb;   // the other cell of the face
     float2 gcap, ghyd, gtilt, dir;
+    float gtn;    // |gtilt|
 };
 
 // Face geometry of the k-th owned face (+x, +y, +x+y, +x-y) of cell (x, y):
@@ -544,6 +596,7 @@ __device__ __forceinline__ FaceGeom face_geom(const FilmDev& F, const A& a, int 
     fg.diag = false;
     fg.ib = i;
     fg.jb = j;
+    fg.gtn = 0.0f;
     const int W_ = (int)F.W, H_ = (int)F.H;
     float2 gxy = make_float2(F.P.phys[0] * F.P.g[0], F.P.phys[0] * F.P.g[1]);
     const float s2 = 0.70710678f;
@@ -560,6 +613,7 @@ __device__ __forceinline__ FaceGeom face_geom(const FilmDev& F, const A& a, int 
         fg.ghyd = make_float2(gmy, -pty * 0.25f * idx_);
         fg.gtilt = gxy;
         fg.dir = make_float2(1.0f, 0.0f);
+        fg.gtn = F.k_gt_norm[0];
         fg.has = true;
     } else if (k == 1) {
         if (!(j + 1 < H_ && a.active(i, j + 1))) return fg;
@@ -573,6 +627,7 @@ __device__ __forceinline__ FaceGeom face_geom(const FilmDev& F, const A& a, int 
         fg.ghyd = make_float2(gmy, -pty * 0.25f * idx_);
         fg.gtilt = make_float2(gxy.y, gxy.x);
         fg.dir = make_float2(0.0f, 1.0f);
+        fg.gtn = F.k_gt_norm[1];
         fg.has = true;
     } else if (k == 2) {
         if (!(i + 1 < W_ && j + 1 < H_ && a.active(i + 1, j + 1))) return fg;
@@ -590,6 +645,7 @@ __device__ __forceinline__ FaceGeom face_geom(const FilmDev& F, const A& a, int 
         fg.ghyd = make_float2(-(p11.y - p00.y) * idx_ * s2, -(pa.y - pb.y) * idx_ * s2);
         fg.gtilt = make_float2((gxy.x + gxy.y) * s2, (gxy.x - gxy.y) * s2);
         fg.dir = make_float2(s2, s2);
+        fg.gtn = F.k_gt_norm[2];
         fg.diag = true;
         fg.has = true;
     } else {
@@ -608,6 +664,7 @@ __device__ __forceinline__ FaceGeom face_geom(const FilmDev& F, const A& a, int 
         fg.ghyd = make_float2(-(p1m.y - p00.y) * idx_ * s2, -(pa.y - pb.y) * idx_ * s2);
         fg.gtilt = make_float2((gxy.x - gxy.y) * s2, (gxy.x + gxy.y) * s2);
         fg.dir = make_float2(s2, -s2);
+        fg.gtn = F.k_gt_norm[3];
         fg.diag = true;
         fg.has = true;
     }

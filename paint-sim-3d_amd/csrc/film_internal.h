@@ -320,6 +320,7 @@ struct cz_film {
         F.stats_out = stats_out; F.smax_out = smax_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.tile_bits = tile_bits; F.compact_cnt = compact_cnt; F.dep_rect = dep_rect; F.dep_cnt = dep_cnt; F.dep_n = dep_n; F.cmf = cmf; F.color = color; F.detail = detail; F.pigtab = pigtab;
         F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
         F.frame_out = frame_out;
+        derive_consts(F);
         return F;
     }
 

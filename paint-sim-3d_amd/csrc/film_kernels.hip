@@ -802,6 +802,12 @@ __global__ __launch_bounds__(256) void k_cure(WS ws) {
 //  - the laplacian and pressures of the tile's cells.
 // The tile's results go to F.delta_out / F.fnorm / F.pres / F.tmp.
 constexpr int PLUG_SPAN = 16 + 2 * PLUG_RADIUS;   // = the staged window
+// nose compares candidate offsets by their squared length, an integer: the
+// offsets are small integers (|cx|, |cy| <= 129), so cx^2 + cy^2 is exact in
+// FP32 and its correctly rounded square root is strictly increasing over the
+// integers up to 2^24, so the shader's comparisons of length(c) pick the same
+// candidate, ties included; length <= 3 is cx^2 + cy^2 <= 9.
+constexpr int kNoseNone = 1 << 30;
 constexpr int RING_SPAN = 18;
 constexpr int WIN = PLUG_SPAN, WIN_R = PLUG_RADIUS;
 
@@ -811,6 +817,7 @@ struct LapWin {
     uint32_t d[WIN * WIN];
     float2 rheo[WIN * WIN];
     float2 fn[WIN * WIN];
+    float nc[WIN * WIN];    // the cells' power-law index (staged with CZ_FLAG_CELL_N only)
     uint32_t tf[9];
 };
 
@@ -838,7 +845,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
     const uint32_t _lx = threadIdx.x & 15u, _ly = (threadIdx.x >> 4) & 15u;
     CZ_TILE_CELL(F)
     const bool cell_th = BT == 256 || threadIdx.x < 256u;
-    const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
+    const float gb = F.k_gb_plug;
     const bool plug_on = do_plug && !(gb < 1e-6f);
     const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
     const int tx0 = ttx * 16, ty0 = tty * 16;
@@ -854,6 +861,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
             sw.d[q] = F.delta[nn];
             sw.rheo[q] = F.rheo[nn];
             sw.fn[q] = F.fnorm[nn];
+            if (cell_n_on(F)) sw.nc[q] = F.n_cell[nn];
         } else {
             sw.h[q] = 0.0f;
             sw.m[q] = 0.0f;
@@ -884,7 +892,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
         fn_out = make_float2(0.0f, 0.0f);
         if (sw.h[wq(i, j)] <= 0.0f) return;
         int bx = 64, by = 64;
-        float bd = 1e9f;
+        int bd = kNoseNone;
         for (int dj = -1; dj <= 1; dj++) {
             for (int di = -1; di <= 1; di++) {
                 const int xx = i + di, yy = j + dj;
@@ -907,13 +915,13 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
                 } else if (di == 0 && dj == 0) {
                     continue;
                 }
-                const float d = cz_len2((float)cx, (float)cy);
+                const int d = cx * cx + cy * cy;
                 if (d < bd) { bd = d; bx = cx; by = cy; }
             }
         }
-        if (bd > 1e8f) return;
+        if (bd == kNoseNone) return;
         d_out = pack_off(bx, by);
-        if (bd <= 3.0f) {
+        if (bd <= 9) {
             float dx = 0.0f, dy = 0.0f;
             for (int dj = -2; dj <= 2; dj++) {
                 for (int di = -2; di <= 2; di++) {
@@ -964,10 +972,8 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
                 const float2 r = sw.rheo[q];
                 yl = r.y > 0.0f;
                 float oy_, oup_;
-                float nq = F.P.phys[2];
-                if (cell_n_on(F))
-                    nq = F.n_cell[idx(F, (uint32_t)(wx0 + (int)(q % WIN)), (uint32_t)(wy0 + (int)(q / WIN)))];
-                qv = hb_flux(nq, gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
+                const float nq = cell_n_on(F) ? sw.nc[q] : F.P.phys[2];
+                qv = hb_flux(nq, gb, true, F.k_lg_plug, hn, Visc{cz_max(r.x, 1e-4f), false}, r.y, oy_, oup_);
             } else {
                 hn = 0.0f;
             }
@@ -995,7 +1001,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
     }
     // usurf at mirrored coordinates, from the ring values
     auto usurf_l = [&](int ii, int jj) -> float {
-        const int ri = (int)refl_i(F, ii), rj = (int)refl_j(F, jj);
+        const int ri = (int)refl1_i(F, ii), rj = (int)refl1_j(F, jj);
         return s_us[(rj - ty0 + 1) * RING_SPAN + (ri - tx0 + 1)];
     };
     const float s0 = hv + sw.m[wq(i, j)];
@@ -1035,9 +1041,10 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
         for (int dy = 0; dy < 2 * PLUG_RADIUS + 1; dy++) {
             for (int dxi = 0; dxi < 2 * PLUG_RADIUS + 1; dxi++) {
                 const int q = (int)(_ly + dy) * PLUG_SPAN + (int)(_lx + dxi);
+                // a dry cell holds (+0, +0): adding it leaves the sums' bits
+                // as the shader's skip does (both sums are >= +0)
                 const float2 v = s_hq[q];
-                if (v.x == 0.0f) continue;
-                if (v.x < 0.0f) yielding = true;
+                yielding = yielding || v.x < 0.0f;
                 sum = sum + v.y;
                 mass = mass + cz_abs(v.x);
             }
@@ -1092,7 +1099,7 @@ __global__ __launch_bounds__(256) void k_fluxes_faces(WS ws) {
         float q = 0.0f;
         FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, true, p00);
         if (fg.has) {
-            FaceFlux ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+            FaceFlux ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.gtn, fg.diag, fg.dir);
             q = ff.q * ((k < 2) ? W_AXIS : W_DIAG_FLUX);
         }
         flux[4u * c + (uint32_t)k] = q;
@@ -1119,6 +1126,7 @@ struct FluxWinT {
     float tmp[N * N];
     float lmu[N * N];   // ln max(mu, 1e-4)
     float fd[N * N];    // front distance (0 when dry)
+    float nc[N * N];    // the cells' power-law index (staged with CZ_FLAG_CELL_N only)
     uint32_t tf[9];
 };
 using FluxWin = FluxWinT<FLUX_R>;
@@ -1144,7 +1152,7 @@ struct WinReadT {
     __device__ __forceinline__ float lmu(int i, int j) const { return w.lmu[q(i, j)]; }
     __device__ __forceinline__ float fdist(int i, int j) const { return w.fd[q(i, j)]; }
     // read when the per-cell index is on only
-    __device__ __forceinline__ float ncell(int i, int j) const { return F.n_cell[(uint32_t)j * F.W + (uint32_t)i]; }
+    __device__ __forceinline__ float ncell(int i, int j) const { return w.nc[q(i, j)]; }
     float hold1_, theta_;
     __device__ __forceinline__ float hold1() const { return hold1_; }
     __device__ __forceinline__ float theta() const { return theta_; }
@@ -1200,6 +1208,7 @@ __device__ __forceinline__ void flux_stage(const FilmDev& F, FluxWinT<R>& sw, in
             sw.tmp[q] = F.tmp[nn];
             sw.lmu[q] = cz_logf(cz_max(sw.rheo[q].x, 1e-4f));
             sw.fd[q] = hv <= 0.0f ? 0.0f : nose_dist_b(sw.d[q]);
+            if (cell_n_on(F)) sw.nc[q] = F.n_cell[nn];
         }
     }
     __syncthreads();
@@ -1214,7 +1223,7 @@ __device__ __forceinline__ float4 cell_fluxes(const FilmDev& F, const A& g, int 
     for (int k = 0; k < 4; k++) {
         FaceGeom fg = face_geom(F, g, x, y, k, true, p00);
         if (!fg.has) continue;
-        FaceFlux ff = face(F, g, x, y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+        FaceFlux ff = face(F, g, x, y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.gtn, fg.diag, fg.dir);
         const float q = ff.q * ((k < 2) ? W_AXIS : W_DIAG_FLUX);
         if (k == 0) qv.x = q;
         else if (k == 1) qv.y = q;
@@ -1233,7 +1242,7 @@ __device__ __forceinline__ void flux_tile(const FilmDev& F, uint32_t _t, FluxWin
     const int wx0 = ttx * 16 - FLUX_R, wy0 = tty * 16 - FLUX_R;
     flux_stage<FLUX_R, false>(F, sw, ttx, tty, wx0, wy0);
     if (!in_grid) return;
-    const WinRead g{F, sw, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
+    const WinRead g{F, sw, wx0, wy0, ttx, tty, F.k_hold1, F.k_theta};
     F.flux[c] = cell_fluxes(F, g, (int)x, (int)y);
 }
 
@@ -1254,12 +1263,12 @@ __device__ __forceinline__ void flux_tile_w(const FilmDev& F, uint32_t _t, FluxW
     const int wx0 = ttx * 16 - FLUX_R, wy0 = tty * 16 - FLUX_R;
     flux_stage<FLUX_R, false, 1024>(F, sw, ttx, tty, wx0, wy0);
     if (!in_grid) return;
-    const WinRead g{F, sw, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
+    const WinRead g{F, sw, wx0, wy0, ttx, tty, F.k_hold1, F.k_theta};
     const float2 p00 = pressure_at(F, g, (int)x, (int)y);
     float q = 0.0f;
     FaceGeom fg = face_geom(F, g, (int)x, (int)y, (int)k, true, p00);
     if (fg.has) {
-        FaceFlux ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+        FaceFlux ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.gtn, fg.diag, fg.dir);
         q = ff.q * ((k < 2u) ? W_AXIS : W_DIAG_FLUX);
     }
     ((float*)F.flux)[4u * c + k] = q;
@@ -1457,7 +1466,7 @@ __device__ __forceinline__ void flux_apply_tile(const FilmDev& F, uint32_t _t, F
     const int W_ = (int)F.W, H_ = (int)F.H;
     flux_stage<FA_R, true, BT>(F, S.w, ttx, tty, wx0, wy0);
     if (threadIdx.x < 9u) S.ap.s_tf[threadIdx.x] = S.w.tf[threadIdx.x];
-    const WinReadT<FA_R, true> g{F, S.w, wx0, wy0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0])};
+    const WinReadT<FA_R, true> g{F, S.w, wx0, wy0, ttx, tty, F.k_hold1, F.k_theta};
     // the fluxes of the active cells of the tile + 2 ring into the apply's window
     if (BT == 256) {
         for (uint32_t q = threadIdx.x; q < (uint32_t)(AP_WIN * AP_WIN); q += 256u) {
@@ -1481,7 +1490,7 @@ __device__ __forceinline__ void flux_apply_tile(const FilmDev& F, uint32_t _t, F
                     const float2 p00 = pressure_at(F, g, gx, gy);
                     FaceGeom fg = face_geom(F, g, gx, gy, (int)k, true, p00);
                     if (fg.has) {
-                        FaceFlux ff = face(F, g, gx, gy, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+                        FaceFlux ff = face(F, g, gx, gy, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.gtn, fg.diag, fg.dir);
                         qk = ff.q * ((k < 2u) ? W_AXIS : W_DIAG_FLUX);
                     }
                 }
@@ -1571,7 +1580,7 @@ __device__ uint32_t nose_offset_global(const FilmDev& F, const float* hr, int i,
     const int W_ = (int)F.W, H_ = (int)F.H;
     if (h_before(F, hr, i, j) <= 0.0f) return 0u;
     int bx = 64, by = 64;
-    float bd = 1e9f;
+    int bd = kNoseNone;
     for (int dj = -1; dj <= 1; dj++) {
         for (int di = -1; di <= 1; di++) {
             const int xx = i + di, yy = j + dj;
@@ -1590,11 +1599,11 @@ __device__ uint32_t nose_offset_global(const FilmDev& F, const float* hr, int i,
             } else if (di == 0 && dj == 0) {
                 continue;
             }
-            const float d = cz_len2((float)cx, (float)cy);
+            const int d = cx * cx + cy * cy;
             if (d < bd) { bd = d; bx = cx; by = cy; }
         }
     }
-    if (bd > 1e8f) return 0u;
+    if (bd == kNoseNone) return 0u;
     return pack_off(bx, by);
 }
 
@@ -1646,7 +1655,7 @@ __device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float
     const uint32_t _lx = threadIdx.x & 15u, _ly = (threadIdx.x >> 4) & 15u;
     CZ_TILE_CELL(F)
     const bool cell_th = threadIdx.x < 256u;
-    const float gb = F.P.phys[0] * cz_len2(F.P.g[0], F.P.g[1]);
+    const float gb = F.k_gb_plug;
     const bool plug_on = !(gb < 1e-6f);
     const int ttx = (int)(_t % F.tw), tty = (int)(_t / F.tw);
     const int tx0 = ttx * 16, ty0 = tty * 16;
@@ -1698,7 +1707,7 @@ __device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float
         fn_out = make_float2(0.0f, 0.0f);
         if (sw.h[bq(i, j)] <= 0.0f) return;
         int bxx = 64, byy = 64;
-        float bd = 1e9f;
+        int bd = kNoseNone;
         for (int dj = -1; dj <= 1; dj++) {
             for (int di = -1; di <= 1; di++) {
                 const int xx = i + di, yy = j + dj;
@@ -1718,13 +1727,13 @@ __device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float
                 } else if (di == 0 && dj == 0) {
                     continue;
                 }
-                const float d = cz_len2((float)cx, (float)cy);
+                const int d = cx * cx + cy * cy;
                 if (d < bd) { bd = d; bxx = cx; byy = cy; }
             }
         }
-        if (bd > 1e8f) return;
+        if (bd == kNoseNone) return;
         d_out = pack_off(bxx, byy);
-        if (bd <= 3.0f) {
+        if (bd <= 9) {
             float dx = 0.0f, dy = 0.0f;
             for (int dj = -2; dj <= 2; dj++) {
                 for (int di = -2; di <= 2; di++) {
@@ -1776,7 +1785,7 @@ __device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float
                 float nq = F.P.phys[2];
                 if (cell_n_on(F))
                     nq = F.n_cell[idx(F, (uint32_t)(bx0 + (int)(q % SB_N)), (uint32_t)(by0 + (int)(q / SB_N)))];
-                qv = hb_flux(nq, gb, hn, cz_max(r.x, 1e-4f), r.y, oy_, oup_);
+                qv = hb_flux(nq, gb, true, F.k_lg_plug, hn, Visc{cz_max(r.x, 1e-4f), false}, r.y, oy_, oup_);
             } else {
                 hn = 0.0f;
             }
@@ -1809,7 +1818,7 @@ __device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float
             continue;
         }
         auto usurf_l = [&](int ii, int jj) -> float {
-            return sw.us[nqi((int)refl_i(F, ii), (int)refl_j(F, jj))];
+            return sw.us[nqi((int)refl1_i(F, ii), (int)refl1_j(F, jj))];
         };
         const float s0 = hv + sw.m[bq(i, j)];
         float hold = 0.0f, pl = 0.0f;
@@ -1847,9 +1856,8 @@ __device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float
             const int wq0 = (j - PLUG_RADIUS - by0) * SB_N + (i - PLUG_RADIUS - bx0);
             for (int dy = 0; dy < 2 * PLUG_RADIUS + 1; dy++) {
                 for (int dxi = 0; dxi < 2 * PLUG_RADIUS + 1; dxi++) {
-                    const float2 v = sw.hq[wq0 + dy * SB_N + dxi];
-                    if (v.x == 0.0f) continue;
-                    if (v.x < 0.0f) yielding = true;
+                    const float2 v = sw.hq[wq0 + dy * SB_N + dxi];   // dry cells: (+0, +0)
+                    yielding = yielding || v.x < 0.0f;
                     sum = sum + v.y;
                     mass = mass + cz_abs(v.x);
                 }
@@ -1873,7 +1881,7 @@ __device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float
         F.tmp[c] = fw.tmp[((int)y - ly0) * LN + ((int)x - lx0)];
     }
     // fluxes of the tile + 2 ring, one face per thread
-    const FusedRead g{F, fw, hr, lx0, ly0, ttx, tty, hold_level(F, 1.0f), cz_atanf(F.P.ein[0]), do_nose};
+    const FusedRead g{F, fw, hr, lx0, ly0, ttx, tty, F.k_hold1, F.k_theta, do_nose};
     for (uint32_t it = threadIdx.x; it < (uint32_t)(AP_WIN * AP_WIN * 4); it += (uint32_t)BT) {
         const uint32_t q = it >> 2, k = it & 3u;
         const int gx = tx0 - AP_R + (int)(q % AP_WIN), gy = ty0 - AP_R + (int)(q / AP_WIN);
@@ -1884,7 +1892,7 @@ __device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float
                 const float2 p00 = pressure_at(F, g, gx, gy);
                 FaceGeom fg = face_geom(F, g, gx, gy, (int)k, true, p00);
                 if (fg.has) {
-                    FaceFlux ff = face(F, g, gx, gy, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+                    FaceFlux ff = face(F, g, gx, gy, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.gtn, fg.diag, fg.dir);
                     qk = ff.q * ((k < 2u) ? W_AXIS : W_DIAG_FLUX);
                 }
             }
@@ -2031,7 +2039,7 @@ __global__ __launch_bounds__(256) void k_strata_split(WS ws) {
             if (av[k] != 0.0f && exists) {
                 FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, true, p00);
                 FaceFlux ff = {0.0f, 0.0f, 0.0f};
-                if (fg.has) ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.diag, fg.dir);
+                if (fg.has) ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.gtn, fg.diag, fg.dir);
                 uint32_t cb = (k == 0) ? c + 1u : (k == 1) ? c + F.W : (k == 2) ? c + F.W + 1u : c - F.W + 1u;
                 uint32_t cu = (av[k] < 0.0f) ? cb : c;
                 o[k] = band_split(F, cu, 0.5f * (F.h[c] + F.h[cb]), ff.y, ff.up, av[k], ff.n);
@@ -2478,7 +2486,7 @@ __device__ inline float face_mob(const FilmDev& F, uint32_t ca, uint32_t cb, flo
     float2 ra = F.rheo[ca], rb = F.rheo[cb];
     float wa = ha / (ha + hb);
     float wb = 1.0f - wa;
-    float mu = cz_expf(wa * cz_logf(cz_max(ra.x, 1e-4f)) + wb * cz_logf(cz_max(rb.x, 1e-4f)));
+    const Visc mu{wa * cz_logf(cz_max(ra.x, 1e-4f)) + wb * cz_logf(cz_max(rb.x, 1e-4f)), true};
     float ty = wa * ra.y + wb * rb.y;
     const float nf = cell_n_on(F) ? wa * F.n_cell[ca] + wb * F.n_cell[cb] : F.P.phys[2];
     float thin = cz_min(ha, hb) / hf;
@@ -2488,15 +2496,14 @@ __device__ inline float face_mob(const FilmDev& F, uint32_t ca, uint32_t cb, flo
     float gmag = cz_len2(g.x, g.y);
     float ydep = (ty <= 0.0f) ? hf : ((g.x >= 0.0f) ? ha : hb);
     float qy, qu;
-    float q = hb_flux(nf, gmag, ydep, mu, ty, qy, qu);
+    float q = hb_flux(nf, gmag, false, 0.0f, ydep, mu, ty, qy, qu);
     if (q <= 0.0f) return 0.0f;
     float mob = cz_min(cz_max(q / gmag - F.P.misc[3], 0.0f), F.P.front[2]);
     if (cz_min(ha, hb) < 0.5f * cz_max(ha, hb)) {
         if (diag) return 0.0f;
         uint32_t tgt = cb;
         if (hb > ha) tgt = ca;
-        uint32_t nf = feeders(F, GlobalRead{F}, (int)(tgt % F.W), (int)(tgt / F.W));
-        mob = mob / cz_powf((float)(nf > 1u ? nf : 1u), 0.7f);
+        mob = mob / cz_feed_pow(feeders(F, GlobalRead{F}, (int)(tgt % F.W), (int)(tgt / F.W)));
     }
     return mob;
 }

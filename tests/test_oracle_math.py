@@ -67,6 +67,30 @@ def test_pow_matches_definition():
     assert L.of_powf(4.0, 0.5) == pytest.approx(2.0, rel=2e-7)
 
 
+def test_pow_fcap():
+    """x^2.5 as x^2 sqrt(x): within 2 ULP (three roundings) of the float64 power on [0, 1] (the film's
+    thinness ratio) and beyond; any other exponent is cz_powf, bit for bit."""
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(0, 1, 20000), np.exp(rng.uniform(-30, 5, 5000))]).astype(np.float32)
+    got = np.array([L.of_pow_fcap(float(a), 2.5) for a in x], np.float32)
+    ref = np.power(x.astype(np.float64), 2.5)
+    normal = ref >= np.finfo(np.float32).tiny
+    assert ulp_err(got[normal], ref[normal]).max() <= 2.0
+    assert L.of_pow_fcap(0.0, 2.5) == 0.0 and L.of_pow_fcap(1.0, 2.5) == 1.0
+    for e in (2.0, 3.0, 1.7):
+        for a in x[:200]:
+            assert L.of_pow_fcap(float(a), e) == L.of_powf(float(a), e)
+
+
+def test_feeder_power_table():
+    """max(k, 1)^0.7 for the feeder counts 0..4: the table holds exactly what cz_powf
+    returns, and those values are the correctly rounded powers or one ULP off."""
+    for k in range(5):
+        want = L.of_powf(float(max(k, 1)), 0.7)
+        assert L.of_feed_pow(k) == want
+        assert ulp_err([want], [max(k, 1) ** float(np.float32(0.7))])[0] <= 1.0
+
+
 def test_atan_atan2():
     rng = np.random.default_rng(4)
     xs = np.concatenate([rng.uniform(-5, 5, 5000), np.exp(rng.uniform(-20, 20, 2000))]).astype(np.float32)
