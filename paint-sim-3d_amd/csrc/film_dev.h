@@ -94,6 +94,11 @@ struct FilmDev {
     uint2* color;          // [n] shade output: rgb + roughness, f16 x 4
     uint2* detail;         // [n] shade output: method + turn, cure, yield stress, thickness, f16 x 4
     float* n_cell;         // [n] per-cell power-law index (with CZ_FLAG_CELL_N)
+    // [n] ln max(rheo.x, 1e-4), the log viscosity the faces and the plug
+    // read (film.wgsl:324, 68): written wherever the rheology is, by the
+    // cure pass for every cell at the start of each frame before any
+    // reader, and copied with it by apply's inheritance
+    float* lmu_cell;
     // Constants of the face terms that depend on P alone, computed when the
     // descriptor is built (derive_consts below) with the IEEE operations the
     // face code would otherwise repeat per face: the same bits.
@@ -237,19 +242,26 @@ __device__ __forceinline__ float2 yield_cap(const FilmDev& F, float2 gc, float t
     return make_float2(gc.x * (gc_max / gcl), gc.y * (gc_max / gcl));
 }
 
-// The viscosity argument of hb_flux: the value, or its logarithm.
+// The viscosity argument of hb_flux: its logarithm, and the value when the
+// caller has it (has_mu; else the Newtonian branch forms exp(lmu)).
 struct Visc {
-    float v;
-    bool is_log;
+    float mu;
+    float lmu;
+    bool has_mu;
 };
+
+// the rheology of cell c and its log viscosity
+__device__ __forceinline__ void set_rheo(const FilmDev& F, uint32_t c, float2 r) {
+    F.rheo[c] = r;
+    F.lmu_cell[c] = cz_logf(cz_max(r.x, 1e-4f));
+}
 
 // film.wgsl:94-107.  The power-law velocity scale n/(n+1) (g/mu)^(1/n)
 // y^((n+1)/n) is evaluated as n/(n+1) exp((ln g - ln mu + (n+1) ln y) / n),
 // one exponential and three logarithms at most where the shader's two pow
 // calls (exp(e ln x) each, by the WGSL definition) take two of each and two
 // more divisions.  lg: ln g when the caller has it (lg_known); mu: the
-// viscosity or its logarithm, whichever the caller has; the Newtonian
-// branch reads the value, the power-law branch the logarithm.
+// viscosity's logarithm (the power-law branch) and value (the Newtonian).
 __device__ __forceinline__ float hb_flux(float n, float gmag, bool lg_known, float lg, float hf, Visc mu, float ty,
                                          float& oy, float& oup) {
     oy = 0.0f;
@@ -259,12 +271,11 @@ __device__ __forceinline__ float hb_flux(float n, float gmag, bool lg_known, flo
     if (y <= 0.0f) return 0.0f;
     float up;
     if (cz_abs(n - 1.0f) < 1e-4f) {
-        const float mv = mu.is_log ? cz_expf(mu.v) : mu.v;
+        const float mv = mu.has_mu ? mu.mu : cz_expf(mu.lmu);
         up = 0.5f * gmag / mv * y * y;
     } else {
-        const float lm = mu.is_log ? mu.v : cz_logf(mu.v);
         const float lgv = lg_known ? lg : cz_logf(gmag);
-        up = n / (n + 1.0f) * cz_expf((lgv - lm + (n + 1.0f) * cz_logf(y)) / n);
+        up = n / (n + 1.0f) * cz_expf((lgv - mu.lmu + (n + 1.0f) * cz_logf(y)) / n);
     }
     float q = up * (hf - n * y / (2.0f * n + 1.0f));
     oy = y;
@@ -311,7 +322,7 @@ struct GlobalRead {
     // ln of the clamped viscosity, front distance, hold level at d = 1 and
     // the static contact angle: evaluated on demand here, staged by the
     // flux kernel's window reader
-    __device__ __forceinline__ float lmu(int i, int j) const { return cz_logf(cz_max(rheo(i, j).x, 1e-4f)); }
+    __device__ __forceinline__ float lmu(int i, int j) const { return F.lmu_cell[at(i, j)]; }
     __device__ __forceinline__ float fdist(int i, int j) const {
         return h(i, j) <= 0.0f ? 0.0f : nose_dist_b(delta(i, j));
     }
@@ -507,7 +518,7 @@ __device__ __forceinline__ FaceFlux face(const FilmDev& F, const A& acc, int ia,
     float wa = ha / (ha + hb);
     float wb = 1.0f - wa;
     // the face viscosity exp(wa ln mu_a + wb ln mu_b), kept as its logarithm
-    const Visc mu{wa * acc.lmu(ia, ja) + wb * acc.lmu(ib, jb), true};
+    const Visc mu{0.0f, wa * acc.lmu(ia, ja) + wb * acc.lmu(ib, jb), false};
     float ty = wa * ra.y + wb * rb.y;
     const float nf = cell_n_on(F) ? wa * acc.ncell(ia, ja) + wb * acc.ncell(ib, jb) : F.P.phys[2];
     float thin = cz_min(ha, hb) / hf;

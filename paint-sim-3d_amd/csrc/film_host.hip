@@ -718,7 +718,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     A(active_count, 4); A(cure_list, nt16); A(cure_count, 4); A(grid_sync, 4); A(tile_bits, nt16); A(dep_rect, 4 * (size_t)CZ_MAX_DEPOSITS); A(dep_cnt, nt64); A(dep_n, 4); A(cmf, CZ_NBAND * 3); A(compact_cnt, 2 * ((nt16 + 255) / 256) + 2); A(stats_out, nt64); A(smax_out, nt64); A(pool_out, 2 * np); A(tsum_a, n);
     A(pigtab, CZ_NPIG * CZ_PIG_STRIDE); A(deposits, CZ_MAX_DEPOSITS); A(tile_start, nt64); A(tile_count, nt64);
     A(tile_items, CZ_MAX_TILE_ITEMS); A(frame_out, 16);
-    A(mass_ws, (cz_mass_sums_workspace(n) + 15) / 16); A(mass_out, 1); A(n_cell, n);
+    A(mass_ws, (cz_mass_sums_workspace(n) + 15) / 16); A(mass_out, 1); A(n_cell, n); A(lmu_cell, n);
 #undef A
     if (!rc) {
         e = hipHostMalloc((void**)&f->frame_out_h, 16 * sizeof(float), hipHostMallocDefault);
@@ -801,6 +801,7 @@ std::vector<SnapPart> snapshot_parts(cz_film* f) {
     v.push_back({f->cg_p, 4 * n});
     v.push_back({f->acc, 16 * n});
     v.push_back({f->n_cell, 4 * n});
+    v.push_back({f->lmu_cell, 4 * n});
     v.push_back({f->tile_moved, 4 * (uint64_t)f->nt16});
     v.push_back({f->tile_flags, 4 * (uint64_t)f->nt16});
     return v;

@@ -270,6 +270,7 @@ struct cz_film {
     float2* stats_out = nullptr;
     float* smax_out = nullptr;
     float* n_cell = nullptr;         // [n] per-cell power-law index (settings.n_per_cell)
+    float* lmu_cell = nullptr;       // [n] ln max(rheo.x, 1e-4), written with the rheology
     uint4* mass_ws = nullptr;        // [cz_mass_sums_workspace(cap_n) / 16] partials of the mass report
     cz_mass_totals* mass_out = nullptr;  // [1] its result
     float* pool_out = nullptr;
@@ -317,6 +318,7 @@ struct cz_film {
         F.tile_flags = tile_flags; F.tile_list = tile_list; F.tile_moved = tile_moved; F.active_count = active_count;
         F.cure_list = cure_list; F.cure_count = cure_count;
         F.n_cell = n_cell;
+        F.lmu_cell = lmu_cell;
         F.stats_out = stats_out; F.smax_out = smax_out; F.pool_out = pool_out; F.tsum_a = tsum_a; F.grid_sync = grid_sync; F.tile_bits = tile_bits; F.compact_cnt = compact_cnt; F.dep_rect = dep_rect; F.dep_cnt = dep_cnt; F.dep_n = dep_n; F.cmf = cmf; F.color = color; F.detail = detail; F.pigtab = pigtab;
         F.deposits = deposits; F.tile_start = tile_start; F.tile_count = tile_count; F.tile_items = tile_items;
         F.frame_out = frame_out;

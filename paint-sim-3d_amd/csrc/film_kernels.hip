@@ -616,7 +616,7 @@ __device__ inline float cure_wet_cell(const FilmDev& F, uint32_t c, float hc) {
     for (int k = 0; k < 4; k++) u[k] = (t[k] > 0.0f) ? cz_min(cu[k] + rate, 1.0f) : cu[k];
     if (!(t[0] > 0.0f && u[0] >= 1.0f)) {
         F.strata_cure[c] = make_float4(u[0], u[1], u[2], u[3]);
-        F.rheo[c] = packed_rheo(F, c, t, u);
+        set_rheo(F, c, packed_rheo(F, c, t, u));
         if (cell_n_on(F)) {
             float n15[4];
 #pragma unroll
@@ -661,7 +661,7 @@ __device__ inline float cure_wet_cell(const FilmDev& F, uint32_t c, float hc) {
         }
         float mu, ty;
         composition_rheo(F, s, mu, ty);
-        F.rheo[c] = make_float2(mu, ty);
+        set_rheo(F, c, make_float2(mu, ty));
         if (cell_n_on(F)) {
             const float n15[4] = {s.pg[0][15], s.pg[1][15], s.pg[2][15], s.pg[3][15]};
             F.n_cell[c] = strata_n(F, s.th, n15);
@@ -682,7 +682,7 @@ __global__ __launch_bounds__(256) void k_cure_clear(WS ws, int do_clear) {
         uint32_t c = idx(F, i, j);
         float hc = F.h[c];
         if (hc <= 0.0f) {
-            F.rheo[c] = make_float2(1.0f, 0.0f);
+            set_rheo(F, c, make_float2(1.0f, 0.0f));
             if (cell_n_on(F)) F.n_cell[c] = F.P.phys[2];
         } else {
             hc = cure_wet_cell(F, c, hc);
@@ -748,6 +748,7 @@ __global__ __launch_bounds__(256) void k_mark(WS ws) {
                 float4* rp = (float4*)(F.rheo + c0);
                 rp[0] = make_float4(1.0f, 0.0f, 1.0f, 0.0f);
                 rp[1] = make_float4(1.0f, 0.0f, 1.0f, 0.0f);
+                *(float4*)(F.lmu_cell + c0) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);   // ln 1
                 if (cell_n_on(F)) {
                     const float nf = F.P.phys[2];
                     *(float4*)(F.n_cell + c0) = make_float4(nf, nf, nf, nf);
@@ -756,7 +757,7 @@ __global__ __launch_bounds__(256) void k_mark(WS ws) {
 #pragma unroll
                 for (uint32_t k = 0; k < 4u; k++)
                     if (x0 + k < F.W && hv[k] <= 0.0f) {
-                        F.rheo[c0 + k] = make_float2(1.0f, 0.0f);
+                        set_rheo(F, c0 + k, make_float2(1.0f, 0.0f));
                         if (cell_n_on(F)) F.n_cell[c0 + k] = F.P.phys[2];
                     }
             }
@@ -818,6 +819,7 @@ struct LapWin {
     float2 rheo[WIN * WIN];
     float2 fn[WIN * WIN];
     float nc[WIN * WIN];    // the cells' power-law index (staged with CZ_FLAG_CELL_N only)
+    float lmu[WIN * WIN];   // ln max(mu, 1e-4)
     uint32_t tf[9];
 };
 
@@ -862,6 +864,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
             sw.rheo[q] = F.rheo[nn];
             sw.fn[q] = F.fnorm[nn];
             if (cell_n_on(F)) sw.nc[q] = F.n_cell[nn];
+            sw.lmu[q] = F.lmu_cell[nn];
         } else {
             sw.h[q] = 0.0f;
             sw.m[q] = 0.0f;
@@ -973,7 +976,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
                 yl = r.y > 0.0f;
                 float oy_, oup_;
                 const float nq = cell_n_on(F) ? sw.nc[q] : F.P.phys[2];
-                qv = hb_flux(nq, gb, true, F.k_lg_plug, hn, Visc{cz_max(r.x, 1e-4f), false}, r.y, oy_, oup_);
+                qv = hb_flux(nq, gb, true, F.k_lg_plug, hn, Visc{cz_max(r.x, 1e-4f), sw.lmu[q], true}, r.y, oy_, oup_);
             } else {
                 hn = 0.0f;
             }
@@ -1206,7 +1209,7 @@ __device__ __forceinline__ void flux_stage(const FilmDev& F, FluxWinT<R>& sw, in
             sw.fn[q] = F.fnorm[nn];
             sw.pres[q] = F.pres[nn];
             sw.tmp[q] = F.tmp[nn];
-            sw.lmu[q] = cz_logf(cz_max(sw.rheo[q].x, 1e-4f));
+            sw.lmu[q] = F.lmu_cell[nn];
             sw.fd[q] = hv <= 0.0f ? 0.0f : nose_dist_b(sw.d[q]);
             if (cell_n_on(F)) sw.nc[q] = F.n_cell[nn];
         }
@@ -1287,6 +1290,7 @@ struct Inflow { float best; uint32_t src; float div; float accv; };
 __device__ inline void inherit_from(const FilmDev& F, uint32_t c, uint32_t src, float hn, bool copy_rheo) {
     if (copy_rheo) {
         F.rheo[c] = F.rheo[src];
+        F.lmu_cell[c] = F.lmu_cell[src];
         if (cell_n_on(F)) F.n_cell[c] = F.n_cell[src];
     }
     float4 ts = F.strata_h[src];
@@ -1785,7 +1789,8 @@ __device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float
                 float nq = F.P.phys[2];
                 if (cell_n_on(F))
                     nq = F.n_cell[idx(F, (uint32_t)(bx0 + (int)(q % SB_N)), (uint32_t)(by0 + (int)(q / SB_N)))];
-                qv = hb_flux(nq, gb, true, F.k_lg_plug, hn, Visc{cz_max(r.x, 1e-4f), false}, r.y, oy_, oup_);
+                const float lmq = F.lmu_cell[idx(F, (uint32_t)(bx0 + (int)(q % SB_N)), (uint32_t)(by0 + (int)(q / SB_N)))];
+                qv = hb_flux(nq, gb, true, F.k_lg_plug, hn, Visc{cz_max(r.x, 1e-4f), lmq, true}, r.y, oy_, oup_);
             } else {
                 hn = 0.0f;
             }
@@ -1807,7 +1812,7 @@ __device__ __forceinline__ void substep_fused_tile(const FilmDev& F, const float
         const uint32_t dl = sw.nd[nqi(i, j)];
         fw.h[q] = hv;
         fw.rheo[q] = sw.rheo[bq(i, j)];
-        fw.lmu[q] = cz_logf(cz_max(fw.rheo[q].x, 1e-4f));
+        fw.lmu[q] = F.lmu_cell[c];
         const uint32_t dflux = act ? dl : (do_nose ? F.delta_out[c] : sw.d[bq(i, j)]);
         fw.d[q] = dflux;
         fw.fn[q] = sw.nfn[nqi(i, j)];
@@ -2486,7 +2491,7 @@ __device__ inline float face_mob(const FilmDev& F, uint32_t ca, uint32_t cb, flo
     float2 ra = F.rheo[ca], rb = F.rheo[cb];
     float wa = ha / (ha + hb);
     float wb = 1.0f - wa;
-    const Visc mu{wa * cz_logf(cz_max(ra.x, 1e-4f)) + wb * cz_logf(cz_max(rb.x, 1e-4f)), true};
+    const Visc mu{0.0f, wa * F.lmu_cell[ca] + wb * F.lmu_cell[cb], false};
     float ty = wa * ra.y + wb * rb.y;
     const float nf = cell_n_on(F) ? wa * F.n_cell[ca] + wb * F.n_cell[cb] : F.P.phys[2];
     float thin = cz_min(ha, hb) / hf;

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void k_seed_blobs(FilmDev F, ScenePlan P) {
         scene_cell(P, i, j, o);
         F.h[c] = o.h;
         F.m[c] = o.m;
-        F.rheo[c] = make_float2(o.rheo[0], o.rheo[1]);
+        set_rheo(F, (uint32_t)c, make_float2(o.rheo[0], o.rheo[1]));
         F.strata_h[c] = make_float4(o.h, 0.0f, 0.0f, 0.0f);
         F.strata_cure[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         F.strata_pig[c * 8u] = make_uint4(o.pig0[0], o.pig0[1], o.pig0[2], o.pig0[3]);
