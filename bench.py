@@ -21,8 +21,9 @@ around every launch for the roofline: the substep kernel family with the
 largest device time; achieved = algorithmic bytes (per active cell, below,
 times the active cells its launches covered) / its event time.  Between the
 phases a one-element fill kernel marks the boundary in a kernel trace.
-roofline.traffic: HBM bytes per launch of that kernel from the rocprof
-FETCH_SIZE / WRITE_SIZE runs of the same command recorded under profiles/
+roofline.traffic: HBM bytes per launch of that kernel: the HBM bytes per
+covered cell of its family from the rocprof FETCH_SIZE / WRITE_SIZE runs of
+the same command recorded under profiles/, times this run's cells per launch
 (null when no record matches the command).  cpu_baseline: the CPU
 restatement (oracle/, its per-tile passes on every host thread) stepping the
 same canvas from the same initial state, frame by frame, for up to 25 s.
@@ -42,8 +43,8 @@ HBM_PEAK_GBS = 8000.0
 
 # Compulsory HBM bytes per active cell of one launch of each kernel family.
 ALGO_BYTES = {
-    "lap_plug": 4 + 4 + 4 + 8 + 8 + 4 + 8 + 8 + 4 + 4,  # R h m delta fnorm rheo, W delta fnorm pres tmp h_sub
-    "flux": 4 + 8 + 8 + 4 + 4 + 8 + 16,             # R h rheo pres tmp delta fnorm, W flux
+    "lap_plug": 4 + 4 + 4 + 8 + 8 + 4 + 4 + 8 + 8 + 4 + 4,  # R h m delta fnorm rheo lmu, W delta fnorm pres tmp h_sub
+    "flux": 4 + 8 + 4 + 8 + 4 + 4 + 8 + 16,         # R h rheo lmu pres tmp delta fnorm, W flux
     "apply": 16 + 4 + 16 + 4 + 16,                  # R flux h acc, W h acc
 }
 SUBSTEP_FAMILIES = tuple(ALGO_BYTES)
@@ -51,12 +52,12 @@ SUBSTEP_BYTES = sum(ALGO_BYTES.values())            # per active cell and subste
 ALGO_BYTES["lap"] = 4 + 4 + 4 + 8 + 4 + 8 + 8       # last nose: R h m delta fnorm, W delta fnorm pres
 # flux + apply in one launch: the flux pass's reads and the apply's, the
 # tile's fluxes written once and never read back
-ALGO_BYTES["flux_apply"] = (4 + 8 + 8 + 4 + 4 + 8) + (4 + 16) + (16 + 4 + 16)
+ALGO_BYTES["flux_apply"] = (4 + 8 + 4 + 8 + 4 + 4 + 8) + (4 + 16) + (16 + 4 + 16)
 # the 1024-thread shapes of the same passes move the same bytes
 for _fam in ("lap_plug", "flux", "apply", "flux_apply"):
     ALGO_BYTES[_fam + "_w"] = ALGO_BYTES[_fam]
-# the whole substep in one launch: R h m delta rheo fnorm, RW acc, W h delta fnorm tmp
-ALGO_BYTES["substep_fused"] = (4 + 4 + 4 + 8 + 8) + (16 + 16) + (4 + 4 + 8 + 4)
+# the whole substep in one launch: R h m delta rheo lmu fnorm, RW acc, W h delta fnorm tmp
+ALGO_BYTES["substep_fused"] = (4 + 4 + 4 + 8 + 4 + 8) + (16 + 16) + (4 + 4 + 8 + 4)
 # with an index per cell: the plug window and the face pass read it
 CELL_N_BYTES = {"lap_plug": 4, "flux": 4, "flux_apply": 4, "lap_plug_w": 4, "flux_w": 4, "flux_apply_w": 4,
                 "substep_fused": 4}
@@ -89,10 +90,13 @@ def canvas_scene(seed, cell_n):
     if cell_n:
         return czf.scene(seed=seed, n_lo=INDEX_RANGE[0], n_hi=INDEX_RANGE[1])
     return czf.scene(seed=seed)
-# families the roofline is reported for: the per-pass substep kernels in both
-# block sizes and the one-launch substeps kernel (k_substeps)
-ROOFLINE_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "lap_plug_w", "flux_w", "apply_w", "flux_apply_w", "substep_fused",
-                                         "substeps")
+# families the roofline is reported for: the substep passes, each over both
+# block sizes (the 1024-thread shapes move the same bytes), the whole
+# substep in one launch (k_substep_fused) and all substeps in one launch
+# (k_substeps); profiled launch families of each
+ROOFLINE_FAMILIES = {"lap_plug": ("lap_plug", "lap_plug_w"), "flux": ("flux", "flux_w"), "apply": ("apply", "apply_w"),
+                     "flux_apply": ("flux_apply", "flux_apply_w"), "substep_fused": ("substep_fused",),
+                     "substeps": ("substeps",)}
 
 
 def log(msg):
@@ -535,8 +539,11 @@ def frame_algo_bytes(n_grid, st, cell_n=False):
     """Compulsory bytes of the frames folded in `st`:
     N_grid*20 + N_act*(632 + 76*S + I*(180 + 68*K)) per frame, K = 40 CG iterations;
     with an index per cell 4 more per grid cell (the cure writes it) and 8
-    more per active cell and substep (plug window and faces read it)."""
+    more per active cell and substep (plug window and faces read it); and
+    the log viscosity per cell: 4 per grid cell (written with the rheology)
+    and 8 per active cell and substep (read by the plug window and faces)."""
     extra = (4 * n_grid * st.frames_counted + 8 * st.act_cell_substeps) if cell_n else 0
+    extra += 4 * n_grid * st.frames_counted + 8 * st.act_cell_substeps
     return (n_grid * 20 * st.frames_counted + 632 * st.act_cells + 76 * st.act_cell_substeps
             + (180 + 68 * 40) * st.act_cell_implicit + extra)
 
@@ -575,11 +582,14 @@ def phase_marker(tag):
     torch.cuda.synchronize()
 
 
-def recorded_traffic(profile, **key):
-    """HBM bytes per launch recorded in profiles/<profile> from the FETCH_SIZE
-    / WRITE_SIZE profiler runs of a command (lines 'traffic: kernel=K
-    size=N ... bytes_per_launch=B'); None unless every key=value given here
-    matches the record."""
+def recorded_traffic(profile, field="bytes_per_launch", **key):
+    """HBM bytes recorded in profiles/<profile> from the FETCH_SIZE /
+    WRITE_SIZE profiler runs of a command (lines 'traffic: kernel=K size=N
+    ... bytes_per_launch=B' or 'bytes_per_cell=B'); None unless every
+    key=value given here matches the record.  The film's substep launches
+    cover as many cells as the shape tuner's choices give them, so their
+    record is per cell (the family's bytes over its launches / the cells
+    they covered), scaled by this run's cells per launch."""
     path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
@@ -588,7 +598,7 @@ def recorded_traffic(profile, **key):
             continue
         rec = dict(kv.split("=", 1) for kv in line.split()[1:])
         if all(rec.get(k) == str(v) for k, v in key.items()):
-            return float(rec["bytes_per_launch"])
+            return float(rec[field]) if field in rec else None
     return None
 
 
@@ -732,7 +742,12 @@ def main():
             film.step()
         film.sync()
         st = film.stats()
-    prof = {fam: film.profile_read(fam) for fam in ROOFLINE_FAMILIES}
+    prof_l = {f: film.profile_read(f) for fams in ROOFLINE_FAMILIES.values() for f in fams}
+    prof = {}
+    for fam, parts in ROOFLINE_FAMILIES.items():
+        tot = sum(prof_l[f][0] * prof_l[f][1] for f in parts)
+        cnt = sum(prof_l[f][1] for f in parts)
+        prof[fam] = (tot / cnt if cnt else 0.0, cnt)
     film.profile(False)
     film.close()
     del film
@@ -750,9 +765,8 @@ def main():
         # cells x substeps the family's launches covered: every per-pass
         # substep runs lap_plug; flux, apply and flux_apply by flux shape
         shp = list(st.flux_shape_cell_substeps)
-        covered = {"lap_plug": shp[0] + shp[1] + shp[2], "lap_plug_w": shp[3] + shp[4],
-                   "flux": shp[0] + shp[1], "apply": shp[0] + shp[1], "flux_apply": shp[2],
-                   "flux_w": shp[3], "apply_w": shp[3], "flux_apply_w": shp[4], "substep_fused": shp[5]}[dom]
+        covered = {"lap_plug": shp[0] + shp[1] + shp[2] + shp[3] + shp[4], "flux": shp[0] + shp[1] + shp[3],
+                   "apply": shp[0] + shp[1] + shp[3], "flux_apply": shp[2] + shp[4], "substep_fused": shp[5]}[dom]
         cells_per_launch = covered / max(1, launches)
         algo_per_cell = algo_bytes(dom, cell_n)
         bytes_per_launch = algo_per_cell * cells_per_launch
@@ -762,7 +776,8 @@ def main():
     frames = max(1, st.frames_counted)
     frame_gbs = frame_algo_bytes(n * n, st, cell_n) / elapsed / 1e9
     log(f"bench: replay, per-family device time (us avg x launches): " +
-        ", ".join(f"{k} {v[0]:.2f}x{v[1]}" for k, v in prof.items() if v[1]))
+        ", ".join(f"{k} {v[0]:.2f}x{v[1]}" for k, v in prof_l.items() if v[1]))
+    log(f"bench: replay cell-substeps per substep shape {list(st.flux_shape_cell_substeps)}")
     log(f"bench: {ms_per_step:.3f} ms/frame, active cells/frame {st.act_cells / frames:.0f}, "
         f"substeps/frame {st.act_cell_substeps / max(1, st.act_cells):.1f}, {dom}: {avg_us:.2f} us x {launches}, "
         f"{cells_per_launch:.0f} cells/launch")
@@ -828,8 +843,9 @@ def main():
                    "frame_algo_GBps": frame_gbs},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": recorded_traffic("film_plain_2048_hbm_traffic.md", kernel=dom, size=n, steps=args.steps,
-                                                 warmup=args.warmup),
+                     "traffic": (lambda b: None if b is None else b * cells_per_launch)(
+                         recorded_traffic("film_plain_2048_hbm_traffic.md", "bytes_per_cell", kernel=dom, size=n,
+                                          steps=args.steps, warmup=args.warmup)),
                      "measured_copy_GBps": measured["copy_GBps"], "frac_of_measured_copy": achieved / measured["copy_GBps"],
                      "algo_bytes_per_launch": bytes_per_launch, "algo_bytes_per_cell": algo_per_cell, "avg_launch_us": avg_us, "launches": launches,
                      "cells_per_launch": cells_per_launch},
