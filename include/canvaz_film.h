@@ -159,6 +159,10 @@ typedef struct cz_settings {
     uint32_t min_substeps;
     uint32_t implicit_iters;
     uint32_t implicit_hydro;
+    /* 1 = the slump pass after each substep's nose (shed, slump, commit_tmp;
+     * film.wgsl:563-801, 866-876), off by default as in the reference
+     * (src/film/mod.rs:345).  A film with it on runs alone: cz_batch_add
+     * refuses it with CZ_EUNSUP. */
     uint32_t slump;
     uint32_t capillary;
     uint32_t capillary_implicit;
@@ -578,7 +582,8 @@ typedef struct cz_batch cz_batch;
 int cz_batch_create(uint32_t max_members, cz_batch** out);   /* max_members <= 1024 */
 void cz_batch_destroy(cz_batch* b);
 /* Join / leave between frames; a joining film's own pending work is
- * finished first.  Members must share settings.implicit_iters. */
+ * finished first.  Members must share settings.implicit_iters; a film with
+ * settings.slump on is refused (CZ_EUNSUP). */
 int cz_batch_add(cz_batch* b, cz_film* f);
 int cz_batch_remove(cz_batch* b, cz_film* f);
 uint32_t cz_batch_size(const cz_batch* b);

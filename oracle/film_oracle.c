@@ -2155,6 +2155,191 @@ static void k_pool8(ofilm* F) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* slump: cone-cap shedding after each substep's nose, behind               */
+/* settings.slump (film.wgsl:563-801, 866-876; src/film/mod.rs:1422-1429)   */
+
+/* film.wgsl:563-568: the steepest slope the paint holds at rest */
+static inline float slope_cap(const ofilm* F, uint32_t c) {
+    float ty = F->rheo[2 * c + 1];
+    float hb = cz_max(F->h[c], 1e-4f);
+    float need = cz_max(ty / (F->P.phys[0] * hb) - cz_len2(F->P.g[0], F->P.g[1]), 0.0f);
+    return cz_max(F->P.ein[0], need / cz_max(F->P.g[2], 1e-3f));
+}
+
+/* film.wgsl:587-596: the height above the cone through the nearest dry cell */
+static inline float shed_excess(const ofilm* F, uint32_t c) {
+    float hc = F->h[c];
+    float hmin = 2.0f * F->P.ein[1];
+    if (hc <= hmin) return 0.0f;
+    float cap = slope_cap(F, c) * F->P.dx * cz_max(nose_dist(F, c), 1.0f);
+    return cz_max(hc - cap, 0.0f);
+}
+
+/* film.wgsl:598-608: heights and weights of active cells, 0 elsewhere */
+static inline float hat(const ofilm* F, int i, int j) {
+    if (i < 0 || j < 0 || i >= (int)F->W || j >= (int)F->H) return 0.0f;
+    if (!cell_active(F, (uint32_t)i, (uint32_t)j)) return 0.0f;
+    return F->h[idx(F, (uint32_t)i, (uint32_t)j)];
+}
+static inline float okf(const ofilm* F, int i, int j) {
+    if (i < 0 || j < 0 || i >= (int)F->W || j >= (int)F->H) return 0.0f;
+    if (!cell_active(F, (uint32_t)i, (uint32_t)j)) return 0.0f;
+    return 1.0f;
+}
+
+typedef struct { float amount; int i0, j0; float w[4]; } oshed;
+
+/* film.wgsl:621-668: what cell (i, j) sheds and where: the excess, at most
+ * half the drop to the bilinear point one cell along the outward direction
+ * (the sum of the offsets to the dry cells within radius sqrt 5, else down
+ * the height gradient), spread over that point's four cells */
+static oshed shed_of(const ofilm* F, int i, int j) {
+    oshed o = {0.0f, i, j, {0.0f, 0.0f, 0.0f, 0.0f}};
+    if (i < 0 || j < 0 || i >= (int)F->W || j >= (int)F->H) return o;
+    if (!cell_active(F, (uint32_t)i, (uint32_t)j)) return o;
+    uint32_t c = idx(F, (uint32_t)i, (uint32_t)j);
+    float e = shed_excess(F, c);
+    if (e <= 0.0f) return o;
+    float hc = F->h[c];
+    float dx = 0.0f, dy = 0.0f;
+    for (int dj = -2; dj <= 2; dj++)
+        for (int di = -2; di <= 2; di++)
+            if (di * di + dj * dj <= 5 && hat(F, i + di, j + dj) <= 0.0f) {
+                dx = dx + (float)di;
+                dy = dy + (float)dj;
+            }
+    if (cz_len2(dx, dy) < 1e-6f) {
+        dx = -(0.5f * (hat(F, i + 1, j) - hat(F, i - 1, j)));
+        dy = -(0.5f * (hat(F, i, j + 1) - hat(F, i, j - 1)));
+    }
+    float gm = cz_len2(dx, dy);
+    if (gm < 1e-9f) return o;
+    dx = dx / gm;
+    dy = dy / gm;
+    float px = (float)i + dx, py = (float)j + dy;
+    float fx0 = floorf(px), fy0 = floorf(py);
+    float tx = px - fx0, ty = py - fy0;
+    int i0 = (int)fx0, j0 = (int)fy0;
+    float w[4] = {(1.0f - tx) * (1.0f - ty), tx * (1.0f - ty), (1.0f - tx) * ty, tx * ty};
+    w[0] = w[0] * okf(F, i0, j0);
+    w[1] = w[1] * okf(F, i0 + 1, j0);
+    w[2] = w[2] * okf(F, i0, j0 + 1);
+    w[3] = w[3] * okf(F, i0 + 1, j0 + 1);
+    float ws = w[0] + w[1] + w[2] + w[3];
+    if (ws <= 1e-6f) return o;
+    for (int k = 0; k < 4; k++) w[k] = w[k] / ws;
+    float ht = w[0] * hat(F, i0, j0) + w[1] * hat(F, i0 + 1, j0) + w[2] * hat(F, i0, j0 + 1) + w[3] * hat(F, i0 + 1, j0 + 1);
+    float amount = cz_min(e, 0.5f * cz_max(hc - ht, 0.0f));
+    if (amount <= 0.0f) return o;
+    o.amount = amount;
+    o.i0 = i0;
+    o.j0 = j0;
+    for (int k = 0; k < 4; k++) o.w[k] = w[k];
+    return o;
+}
+
+/* film.wgsl:670-678: the part of a shed landing on cell (i, j) */
+static inline float share_to(const oshed* sd, int i, int j) {
+    if (sd->amount <= 0.0f) return 0.0f;
+    int di = i - sd->i0, dj = j - sd->j0;
+    if (di == 0 && dj == 0) return sd->amount * sd->w[0];
+    if (di == 1 && dj == 0) return sd->amount * sd->w[1];
+    if (di == 0 && dj == 1) return sd->amount * sd->w[2];
+    if (di == 1 && dj == 1) return sd->amount * sd->w[3];
+    return 0.0f;
+}
+
+/* film.wgsl:725-741: each cell's shed, kept in acc_s as (amount, i0, j0, 0)
+ * and the four weights, as the shader keeps it */
+static void store_shed(ofilm* F, uint32_t c, const oshed* sd) {
+    float* a = &F->acc_s[16 * c];
+    a[0] = sd->amount;
+    a[1] = cz_bits2f((uint32_t)sd->i0);
+    a[2] = cz_bits2f((uint32_t)sd->j0);
+    a[3] = 0.0f;
+    for (int k = 0; k < 4; k++) a[4 + k] = sd->w[k];
+}
+static oshed load_shed(const ofilm* F, uint32_t c) {
+    const float* a = &F->acc_s[16 * c];
+    oshed sd;
+    sd.amount = a[0];
+    sd.i0 = (int)cz_f2bits(a[1]);
+    sd.j0 = (int)cz_f2bits(a[2]);
+    for (int k = 0; k < 4; k++) sd.w[k] = a[4 + k];
+    return sd;
+}
+
+/* film.wgsl:743-748 */
+static void k_shed(ofilm* F) {
+    FOR_ACTIVE(F, {
+        oshed sd = shed_of(F, (int)x, (int)y);
+        store_shed(F, c, &sd);
+    })
+}
+
+/* film.wgsl:750-801: each cell's new height from what it keeps and what
+ * its active neighbours shed onto it (into tmp), the face volumes into acc,
+ * and a newly wetted cell inheriting from its largest donor */
+static void k_slump(ofilm* F) {
+    FOR_ACTIVE(F, {
+        int i = (int)x, j = (int)y;
+        float hc = F->h[c];
+        oshed mine = load_shed(F, c);
+        float self_share = share_to(&mine, i, j);
+        float inn = 0.0f, best_in = 0.0f;
+        uint32_t src = c;
+        float fx = 0.0f, fy = 0.0f, fd = 0.0f, fe = 0.0f;
+        for (int dj = -1; dj <= 1; dj++) {
+            for (int di = -1; di <= 1; di++) {
+                if (di == 0 && dj == 0) continue;
+                int xx = i + di, yy = j + dj;
+                if (xx < 0 || yy < 0 || xx >= (int)F->W || yy >= (int)F->H) continue;
+                if (!cell_active(F, (uint32_t)xx, (uint32_t)yy)) continue;
+                uint32_t nn = idx(F, (uint32_t)xx, (uint32_t)yy);
+                oshed sn = load_shed(F, nn);
+                float give = share_to(&sn, i, j);
+                float take = share_to(&mine, xx, yy);
+                inn = inn + give;
+                if (give > best_in) { best_in = give; src = nn; }
+                float net = take - give;
+                if (di == 1 && dj == 0) fx = net;
+                if (di == 0 && dj == 1) fy = net;
+                if (di == 1 && dj == 1) fd = net;
+                if (di == 1 && dj == -1) fe = net;
+            }
+        }
+        float outs = mine.amount - self_share;
+        float hn2 = cz_max(hc - outs + inn, 0.0f);
+        F->tmp[c] = hn2;
+        F->acc[4 * c] = F->acc[4 * c] + fx * F->P.dx;
+        F->acc[4 * c + 1] = F->acc[4 * c + 1] + fy * F->P.dx;
+        F->acc[4 * c + 2] = F->acc[4 * c + 2] + fd * F->P.dx;
+        F->acc[4 * c + 3] = F->acc[4 * c + 3] + fe * F->P.dx;
+        if (hc <= 0.0f && hn2 > 0.0f && src != c) inherit_from(F, c, src, hn2, 1);
+    })
+}
+
+/* film.wgsl:866-876 */
+static void k_commit_tmp(ofilm* F) {
+    _Pragma("omp parallel for schedule(dynamic, 1)")
+    for (uint32_t a = 0; a < F->active_count; a++) {
+        uint32_t t = F->tile_list[a];
+        uint32_t tx = t % F->tw, ty = t / F->tw;
+        int moved_any = 0;
+        for (uint32_t ly = 0; ly < 16u; ly++) {
+            for (uint32_t lx = 0; lx < 16u; lx++) {
+                uint32_t x = tx * 16u + lx, y = ty * 16u + ly;
+                if (x >= F->W || y >= F->H) continue;
+                uint32_t c = idx(F, x, y);
+                if (cz_abs(F->tmp[c] - F->h[c]) > MOVE_EPS) moved_any = 1;
+                F->h[c] = F->tmp[c];
+            }
+        }
+        if (moved_any) F->tile_moved[t] = 1u;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* implicit hydrostatic CG (film.wgsl:895-1302)                            */
 
 /* film.wgsl:895-920 */
@@ -2513,6 +2698,11 @@ void of_encode_frame(ofilm* F, uint32_t substeps, uint32_t implicit_steps, uint3
         k_limiter(F);
         k_apply(F);
         k_nose(F);
+        if (F->P.flags[1]) {  /* settings.slump */
+            k_shed(F);
+            k_slump(F);
+            k_commit_tmp(F);
+        }
     }
     k_laplacian(F);
     k_strata_split(F);
@@ -3129,6 +3319,11 @@ float of_logf(float x) { return cz_logf(x); }
 float of_powf(float x, float y) { return cz_powf(x, y); }
 float of_pow_fcap(float x, float y) { return cz_pow_fcap(x, y); }
 float of_feed_pow(uint32_t k) { return cz_feed_pow(k); }
+/* the slump pass, one stage at a time (pins) */
+void of_k_shed(ofilm* F) { k_shed(F); }
+void of_k_slump(ofilm* F) { k_slump(F); }
+void of_k_commit_tmp(ofilm* F) { k_commit_tmp(F); }
+float of_fn_shed_excess(const ofilm* F, uint32_t c) { return shed_excess(F, c); }
 float of_atanf(float x) { return cz_atanf(x); }
 float of_atan2f(float y, float x) { return cz_atan2f(y, x); }
 uint32_t of_f2h(float x) { return cz_f2h(x); }

@@ -88,8 +88,11 @@ def lib():
         L.of_brush_pass.argtypes = [P]
         L.of_knife_pass.argtypes = [P]
         for k in ("of_k_mark_compact", "of_k_laplacian", "of_k_plug", "of_k_fluxes", "of_k_cure", "of_k_nose",
-                  "of_k_limiter", "of_k_apply", "of_k_clear_acc", "of_k_cg_setup"):
+                  "of_k_limiter", "of_k_apply", "of_k_clear_acc", "of_k_cg_setup", "of_k_shed", "of_k_slump",
+                  "of_k_commit_tmp"):
             getattr(L, k).argtypes = [P]
+        L.of_fn_shed_excess.restype = f32
+        L.of_fn_shed_excess.argtypes = [P, u32]
         L.of_k_strata.argtypes = [P, C.c_int]
         L.of_cg_iterate.argtypes = [P, u32]
         L.of_cg_apply.argtypes = [P, C.POINTER(f32), C.POINTER(f32)]

@@ -383,6 +383,7 @@ int cz_batch_add(cz_batch* b, cz_film* f) {
     if (f->batch) return set_err(CZ_EINVAL, "cz_batch_add: the film is already a batch member");
     if (f->j_gen != 0u) return set_err(CZ_EINVAL, "cz_batch_add: the film keeps an undo journal (generation 0 first)");
     if (f->knife_active) return set_err(CZ_EINVAL, "cz_batch_add: the film has a knife set (cz_film_set_knife(NULL) first)");
+    if (f->s.slump) return set_err(CZ_EUNSUP, "cz_batch_add: the film runs the slump pass, which batches do not");
     if (b->members.size() >= b->max_members) return set_err(CZ_ENOMEM, "cz_batch_add: batch is full");
     if (f->device != b->device) return set_err(CZ_EINVAL, "cz_batch_add: film and batch are on different devices");
     if (f->nt16 > CZ_ITEM_TILE_MASK + 1u) return set_err(CZ_EINVAL, "cz_batch_add: canvas has too many tiles");

@@ -342,10 +342,17 @@ int record_frame(cz_film* f, uint32_t substeps) {
     const uint32_t nsub = pg > 0u ? 0u : substeps;
     std::vector<int> shp(nsub), bks(nsub);
     std::vector<char> tim(nsub), to_main(nsub, 1);
+    const bool slump = f->s.slump != 0u;
     for (uint32_t s = 0; s < nsub; s++) {
         bool timed = false;
         shp[s] = f->flux_tuner.choose(std::min(f->active_hint, f->nt16), !f->capturing, bks[s], timed);
         tim[s] = timed ? 1 : 0;
+        // the fused shape carries the nose into its substep; with the slump
+        // passes between the substeps the passes run separately
+        if (slump && shp[s] == 5) {
+            if (timed) f->flux_tuner.reissue(bks[s], 5, 4);
+            shp[s] = 4;
+        }
     }
     // The fused shape (5) reads the heights before its apply from one of F.h
     // and h_sub and writes the new ones to the other.  A run of fused
@@ -379,20 +386,21 @@ int record_frame(cz_film* f, uint32_t substeps) {
             if (ea && eb) hipEventRecord(ea, f->stream);
         }
         const OneFilm wl{f->dev()};
-        const int nose = s > 0 ? 1 : 0, clear = (s == 0 && !implicit) ? 1 : 0;
+        // with slump the previous substep's nose ran before its slump passes
+        const int nose = (s > 0 && !slump) ? 1 : 0, clear = (s == 0 && !implicit) ? 1 : 0;
         if (shape == 5) {
             if (cur_main == (to_main[s] != 0)) return set_err(CZ_EINVAL, "fused substep: height buffers collide");
             float* hr = cur_main ? f->h : f->h_sub;
             float* hw = to_main[s] ? f->h : f->h_sub;
             LAUNCH("substep_fused", k_substep_fused<OneFilm>, gt, b1024, wl, (const float*)hr, hw, nose, clear);
-            if (s > 0) f->delta_cur_a = !f->delta_cur_a;
+            if (nose) f->delta_cur_a = !f->delta_cur_a;
             cur_main = to_main[s] != 0;
         } else {
         if (shape >= 3)
             LAUNCH("lap_plug_w", k_lap_plug_w<OneFilm>, gt, b1024, wl, nose, 1, clear);
         else
             LAUNCH("lap_plug", k_lap_plug<OneFilm>, gt, b256, wl, nose, 1, clear);
-        if (s > 0) f->delta_cur_a = !f->delta_cur_a;
+        if (nose) f->delta_cur_a = !f->delta_cur_a;
         const OneFilm w{f->dev()};
         switch (shape) {
             case 1:
@@ -415,6 +423,15 @@ int record_frame(cz_film* f, uint32_t substeps) {
                 break;
         }
         cur_main = true;
+        if (slump) {
+            // this substep's nose, then shed and slump (+ commit_tmp), as
+            // encode_frame orders them (src/film/mod.rs:1420-1429)
+            LAUNCH("lap", k_lap_tail<OneFilm>, gt, b256, OneFilm{f->dev()}, 1, 0);
+            f->delta_cur_a = !f->delta_cur_a;
+            const OneFilm ws{f->dev()};
+            LAUNCH("shed", k_shed<OneFilm>, gt, b256, ws);
+            LAUNCH("slump", k_slump<OneFilm>, gt, b256, ws);
+        }
         }
         if (ea && eb) {
             hipEventRecord(eb, f->stream);
@@ -422,10 +439,10 @@ int record_frame(cz_film* f, uint32_t substeps) {
         }
         f->shape_subs_now[shape]++;
     }
-    // nose of the last substep + the laplacian strata_split reads
+    // nose of the last substep (with slump: done) + the laplacian strata_split reads
     if (pg == 0u) {
-        LAUNCH("lap", k_lap_tail<OneFilm>, gt, b256, OneFilm{f->dev()}, 1, 0);
-        f->delta_cur_a = !f->delta_cur_a;
+        LAUNCH("lap", k_lap_tail<OneFilm>, gt, b256, OneFilm{f->dev()}, slump ? 0 : 1, 0);
+        if (!slump) f->delta_cur_a = !f->delta_cur_a;
     }
     const OneFilm w{f->dev()};
     LAUNCH("split", k_strata_split<OneFilm>, gt, b256, w);
@@ -683,10 +700,6 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     cz_film* f = new cz_film();
     f->flux_tuner.ns = czi::kSubstepShapes;
     if (s) f->s = *s; else cz_settings_default(&f->s);
-    if (f->s.slump) {
-        destroy(f);
-        return set_err(CZ_EUNSUP, "cz_film_create: the slump pass (settings.slump) is not built");
-    }
     cz_overrides_from_env(&f->ov);
     f->cap_n = w * h;
     f->cap_nt16 = ((w + 15u) / 16u) * ((h + 15u) / 16u);
@@ -750,7 +763,6 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
 
 int film_reshape(cz_film* f, uint32_t w, uint32_t h, const cz_settings* s) {
     if (f->batch) return set_err(CZ_EINVAL, "reshape: the film is a batch member");
-    if (s->slump) return set_err(CZ_EUNSUP, "the slump pass (settings.slump) is not built");
     int rc = wait_idle(f);
     if (rc) return rc;
     const cz_settings keep = f->s;
@@ -905,8 +917,8 @@ int cz_film_dims(const cz_film* f, uint32_t* w, uint32_t* h, float* dx) {
 
 int cz_film_set_settings(cz_film* f, const cz_settings* s) {
     if (!f || !s) return set_err(CZ_EINVAL, "null argument");
-    if (s->slump) return set_err(CZ_EUNSUP, "the slump pass (settings.slump) is not built");
     if (s->width_m != f->s.width_m) return set_err(CZ_EINVAL, "width_m is fixed at creation");
+    if (f->batch && s->slump) return set_err(CZ_EUNSUP, "set_settings: a batch member takes no slump pass");
     f->s = *s;
     return CZ_OK;
 }

@@ -185,7 +185,8 @@ struct cz_film {
     bool sync_fault = false;  // a grid barrier of k_substeps timed out
     uint32_t persist_grid() const {
         const uint32_t act = std::min(active_hint, nt16);
-        if (substep_mode == 0 || (substep_mode < 0 && (czi::kPersistTiles == 0u || act > czi::kPersistTiles)))
+        // the slump passes run between the substeps, outside the one launch
+        if (s.slump || substep_mode == 0 || (substep_mode < 0 && (czi::kPersistTiles == 0u || act > czi::kPersistTiles)))
             return 0u;
         uint32_t p2 = 16u;  // power of two: few distinct grids, few distinct graphs
         while (p2 < act) p2 <<= 1;

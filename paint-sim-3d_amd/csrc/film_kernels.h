@@ -143,6 +143,8 @@ __global__ void k_batch_plan(BatchPlan B);
     X(k_limit_apply, (WS ws), (ws))                                 \
     X(k_flux_apply, (WS ws), (ws))                                  \
     X(k_substep_fused, (WS ws, const float* hr, float* hw, int do_nose, int do_clear), (ws, hr, hw, do_nose, do_clear)) \
+    X(k_shed, (WS ws), (ws))                                        \
+    X(k_slump, (WS ws), (ws))                                       \
     X(k_strata_split, (WS ws), (ws))                                \
     X(k_reduce_pool, (WS ws), (ws))                                 \
     X(k_strata_limit, (WS ws), (ws))                                \

@@ -1443,6 +1443,176 @@ __device__ __forceinline__ void apply_tile(const FilmDev& F, uint32_t _t, ApplyS
     apply_core<BT>(F, _t, S);
 }
 
+// ---------------------------------------------------------------------------
+// slump: cone-cap shedding after each substep's nose, on when
+// settings.slump (Params.flags[1]; film.wgsl:563-801, 866-876, encoded after
+// nose by src/film/mod.rs:1422-1429).  Two launches: k_shed stores each
+// active cell's shed in acc_s (as the shader does; strata_split rewrites
+// acc_s at the frame's end), then k_slump gathers what the 8 neighbours shed
+// and commits the new height at once: a cell's slump reads no other cell's
+// height, so commit_tmp's separate pass is not needed (tmp still receives the
+// height, as the shader leaves it).
+
+// film.wgsl:563-568: the steepest slope the paint holds at rest
+__device__ __forceinline__ float slope_cap(const FilmDev& F, uint32_t c) {
+    const float ty = F.rheo[c].y;
+    const float hb = cz_max(F.h[c], 1e-4f);
+    const float need = cz_max(ty / (F.P.phys[0] * hb) - cz_len2(F.P.g[0], F.P.g[1]), 0.0f);
+    return cz_max(F.P.ein[0], need / cz_max(F.P.g[2], 1e-3f));
+}
+
+// film.wgsl:587-596: the height above the cone through the nearest dry cell
+__device__ __forceinline__ float shed_excess(const FilmDev& F, uint32_t c) {
+    const float hc = F.h[c];
+    const float hmin = 2.0f * F.P.ein[1];
+    if (hc <= hmin) return 0.0f;
+    const float cap = slope_cap(F, c) * F.P.dx * cz_max(nose_dist(F, c), 1.0f);
+    return cz_max(hc - cap, 0.0f);
+}
+
+// film.wgsl:598-608
+__device__ __forceinline__ float hat(const FilmDev& F, int i, int j) {
+    if (i < 0 || j < 0 || i >= (int)F.W || j >= (int)F.H) return 0.0f;
+    if (!cell_active(F, (uint32_t)i, (uint32_t)j)) return 0.0f;
+    return F.h[idx(F, (uint32_t)i, (uint32_t)j)];
+}
+__device__ __forceinline__ float okf(const FilmDev& F, int i, int j) {
+    if (i < 0 || j < 0 || i >= (int)F.W || j >= (int)F.H) return 0.0f;
+    return cell_active(F, (uint32_t)i, (uint32_t)j) ? 1.0f : 0.0f;
+}
+
+struct Shed {
+    float amount;
+    int i0, j0;
+    float4 w;
+};
+
+// film.wgsl:621-668: the excess, at most half the drop to the bilinear point
+// one cell along the outward direction (the offsets to the dry cells within
+// radius sqrt 5, else down the height gradient), over its four cells
+__device__ Shed shed_of(const FilmDev& F, int i, int j) {
+    Shed o = {0.0f, i, j, make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+    if (!cell_active(F, (uint32_t)i, (uint32_t)j)) return o;
+    const uint32_t c = idx(F, (uint32_t)i, (uint32_t)j);
+    const float e = shed_excess(F, c);
+    if (e <= 0.0f) return o;
+    const float hc = F.h[c];
+    float dx = 0.0f, dy = 0.0f;
+    for (int dj = -2; dj <= 2; dj++)
+        for (int di = -2; di <= 2; di++)
+            if (di * di + dj * dj <= 5 && hat(F, i + di, j + dj) <= 0.0f) {
+                dx = dx + (float)di;
+                dy = dy + (float)dj;
+            }
+    if (cz_len2(dx, dy) < 1e-6f) {
+        dx = -(0.5f * (hat(F, i + 1, j) - hat(F, i - 1, j)));
+        dy = -(0.5f * (hat(F, i, j + 1) - hat(F, i, j - 1)));
+    }
+    const float gm = cz_len2(dx, dy);
+    if (gm < 1e-9f) return o;
+    dx = dx / gm;
+    dy = dy / gm;
+    const float px = (float)i + dx, py = (float)j + dy;
+    const float fx0 = floorf(px), fy0 = floorf(py);
+    const float tx = px - fx0, ty = py - fy0;
+    const int i0 = (int)fx0, j0 = (int)fy0;
+    float4 w = make_float4((1.0f - tx) * (1.0f - ty), tx * (1.0f - ty), (1.0f - tx) * ty, tx * ty);
+    w = make_float4(w.x * okf(F, i0, j0), w.y * okf(F, i0 + 1, j0), w.z * okf(F, i0, j0 + 1), w.w * okf(F, i0 + 1, j0 + 1));
+    const float ws = w.x + w.y + w.z + w.w;
+    if (ws <= 1e-6f) return o;
+    w = make_float4(w.x / ws, w.y / ws, w.z / ws, w.w / ws);
+    const float ht = w.x * hat(F, i0, j0) + w.y * hat(F, i0 + 1, j0) + w.z * hat(F, i0, j0 + 1) + w.w * hat(F, i0 + 1, j0 + 1);
+    const float amount = cz_min(e, 0.5f * cz_max(hc - ht, 0.0f));
+    if (amount <= 0.0f) return o;
+    o.amount = amount;
+    o.i0 = i0;
+    o.j0 = j0;
+    o.w = w;
+    return o;
+}
+
+// film.wgsl:670-678
+__device__ __forceinline__ float share_to(const Shed& sd, int i, int j) {
+    if (sd.amount <= 0.0f) return 0.0f;
+    const int di = i - sd.i0, dj = j - sd.j0;
+    if (di == 0 && dj == 0) return sd.amount * sd.w.x;
+    if (di == 1 && dj == 0) return sd.amount * sd.w.y;
+    if (di == 0 && dj == 1) return sd.amount * sd.w.z;
+    if (di == 1 && dj == 1) return sd.amount * sd.w.w;
+    return 0.0f;
+}
+
+// film.wgsl:725-741: acc_s[4c] = (amount, i0, j0, 0), acc_s[4c + 1] = w
+__device__ __forceinline__ Shed load_shed(const FilmDev& F, uint32_t c) {
+    const float4 a = F.acc_s[4u * c];
+    Shed sd;
+    sd.amount = a.x;
+    sd.i0 = (int)cz_f2bits(a.y);
+    sd.j0 = (int)cz_f2bits(a.z);
+    sd.w = F.acc_s[4u * c + 1u];
+    return sd;
+}
+
+// film.wgsl:743-748
+template <class WS>
+__global__ __launch_bounds__(256) void k_shed(WS ws) {
+    CZ_FOR_TILES(ws) {
+        CZ_TILE_CELL(F)
+        if (!in_grid) continue;
+        const Shed sd = shed_of(F, (int)x, (int)y);
+        F.acc_s[4u * c] = make_float4(sd.amount, cz_bits2f((uint32_t)sd.i0), cz_bits2f((uint32_t)sd.j0), 0.0f);
+        F.acc_s[4u * c + 1u] = sd.w;
+    }
+}
+
+// film.wgsl:750-801 and commit_tmp (866-876)
+template <class WS>
+__global__ __launch_bounds__(256) void k_slump(WS ws) {
+    CZ_FOR_TILES(ws) {
+        CZ_TILE_CELL(F)
+        bool moved = false;
+        if (in_grid) {
+            const int i = (int)x, j = (int)y;
+            const float hc = F.h[c];
+            const Shed mine = load_shed(F, c);
+            const float self_share = share_to(mine, i, j);
+            float inn = 0.0f, best_in = 0.0f;
+            uint32_t src = c;
+            float fx = 0.0f, fy = 0.0f, fd = 0.0f, fe = 0.0f;
+            for (int dj = -1; dj <= 1; dj++) {
+                for (int di = -1; di <= 1; di++) {
+                    if (di == 0 && dj == 0) continue;
+                    const int xx = i + di, yy = j + dj;
+                    if (xx < 0 || yy < 0 || xx >= (int)F.W || yy >= (int)F.H) continue;
+                    if (!cell_active(F, (uint32_t)xx, (uint32_t)yy)) continue;
+                    const uint32_t nn = idx(F, (uint32_t)xx, (uint32_t)yy);
+                    const Shed sn = load_shed(F, nn);
+                    const float give = share_to(sn, i, j);
+                    const float take = share_to(mine, xx, yy);
+                    inn = inn + give;
+                    if (give > best_in) { best_in = give; src = nn; }
+                    const float net = take - give;
+                    if (di == 1 && dj == 0) fx = net;
+                    if (di == 0 && dj == 1) fy = net;
+                    if (di == 1 && dj == 1) fd = net;
+                    if (di == 1 && dj == -1) fe = net;
+                }
+            }
+            const float outs = mine.amount - self_share;
+            const float hn2 = cz_max(hc - outs + inn, 0.0f);
+            F.tmp[c] = hn2;
+            const float4 a = F.acc[c];
+            F.acc[c] = make_float4(a.x + fx * F.P.dx, a.y + fy * F.P.dx, a.z + fd * F.P.dx, a.w + fe * F.P.dx);
+            // the donor is wet (it sheds), so no other cell rewrites its
+            // rheology or strata during this pass
+            if (hc <= 0.0f && hn2 > 0.0f && src != c) inherit_from(F, c, src, hn2, true);
+            moved = cz_abs(hn2 - hc) > MOVE_EPS;
+            F.h[c] = hn2;
+        }
+        if (__syncthreads_or(moved ? 1 : 0) && threadIdx.x == 0) F.tile_moved[_t] = 1u;
+    }
+}
+
 // fluxes + limiter + apply of a tile in one launch, for small active sets
 // where the launch between the two passes costs more than the work: the
 // block computes the fluxes of the tile and a 2-cell ring itself (the cells
@@ -2921,6 +3091,8 @@ CZ_INST2(k_fluxes_w)
 CZ_INST2(k_limit_apply_w)
 CZ_INST2(k_flux_apply_w)
 CZ_INST1(k_substep_fused, OneFilm, const float*, float*, int, int)
+CZ_INST1(k_shed, OneFilm)
+CZ_INST1(k_slump, OneFilm)
 CZ_INST1(k_mark, OneFilmAll)
 CZ_INST1(k_shade, OneFilmAll)
 CZ_INST1(k_mark, ManyFilms)
