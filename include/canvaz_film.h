@@ -175,6 +175,13 @@ typedef struct cz_settings {
      * update_params sets Params.flags[2] bit 0 (CZ_FLAG_CELL_N) from it.
      * 0 (default) = the reference's global index. */
     uint32_t n_per_cell;
+    /* `canvas_rgb: Option<[f32; 3]>` (src/film/mod.rs:293): with
+     * has_canvas_rgb = 1 the substrate reflects canvas_rgb, the colour the
+     * canvas shows (sRGB, 0..1), through color::canvas_albedo and
+     * reflectance_of (src/color/mod.rs:29-100); 0 (default) = the raw or the
+     * white canvas by raw_canvas. */
+    uint32_t has_canvas_rgb;
+    float canvas_rgb[3];
 } cz_settings;
 
 #define CZ_FLAG_CELL_N 1u   /* Params.flags[2]: per-cell power-law index (n_per_cell) */
@@ -322,6 +329,15 @@ void cz_cmf_table(float* out);
 void cz_weave(uint32_t w, uint32_t h, float dx, float amp, float* out);
 /* substrate_refl (src/film/mod.rs:188-210) without a canvas colour -> out[n*8]. */
 void cz_substrate_refl(uint64_t n, uint32_t raw, uint32_t* out);
+/* color::reflectance_of(color::canvas_albedo(rgb), cmf) (src/color/mod.rs:
+ * 29-100): the 16-band reflectance of a canvas shown in sRGB colour rgb,
+ * cmf = the cz_cmf_table layout.  Powers and exponentials are evaluated in
+ * double and rounded once. */
+void cz_canvas_reflectance(const float rgb[3], const float* cmf, float out[16]);
+/* The substrate of one cell for settings s (the raw or white canvas, or
+ * s->canvas_rgb's reflectance when s->has_canvas_rgb): 8 words of two f16
+ * bands each, as substrate_refl packs them. */
+void cz_substrate_words(const cz_settings* s, const float* cmf, uint32_t out[8]);
 /* Deposit::ribbon / capsule / stroke / laid_by (src/film/mod.rs:62-98). */
 void cz_deposit_ribbon(const float p0[2], const float p1[2], const float n0[2], const float n1[2],
                        float r0, float r1, uint32_t caps, float volume, const uint32_t pig[8],

@@ -3013,6 +3013,76 @@ void of_weave(uint32_t w, uint32_t h, float dx, float amp, float* v) {
     }
 }
 
+/* The canvas colour (src/color/mod.rs:27-100): f32 powf and exp evaluated
+ * in double and rounded once. */
+static float o_pow(float x, float y) { return (float)pow((double)x, (double)y); }
+static float o_exp(float x) { return (float)exp((double)x); }
+static float o_clamp(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+/* src/color/mod.rs:30-31 (the sRGB transfer function and its inverse) */
+static float o_srgb_dec(float c) { return c <= 0.04045f ? c / 12.92f : o_pow((c + 0.055f) / 1.055f, 2.4f); }
+static float o_srgb_enc(float c) { return c <= 0.0031308f ? c * 12.92f : 1.055f * o_pow(c, 1.0f / 2.4f) - 0.055f; }
+/* src/color/mod.rs:32-38: the root of y (x (2.43 x + 0.59) + 0.14) = x (2.51 x + 0.03) */
+static float o_untone(float y) {
+    float yy = o_clamp(y, 0.0f, 0.995f);
+    float qa = 2.51f - yy * 2.43f;
+    float qb = 0.03f - yy * 0.59f;
+    return (-qb + sqrtf(qb * qb + 4.0f * qa * yy * 0.14f)) / (2.0f * qa);
+}
+static float o_det3(float a[3][3]) {
+    return a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+}
+/* color::reflectance_of(color::canvas_albedo(rgb)) (src/color/mod.rs:29-100) */
+void of_canvas_reflectance(const float* rgb, const float* cmf, float* out) {
+    static const float gain[3] = {1.94f, 1.95f, 2.17f};
+    float lin[3];
+    for (int i = 0; i < 3; i++) {
+        float alb = o_srgb_enc(o_clamp(o_untone(o_srgb_dec(rgb[i])) / gain[i], 0.005f, 0.9f));
+        lin[i] = o_srgb_dec(alb);
+    }
+    float target[3];
+    target[0] = 0.4124f * lin[0] + 0.3576f * lin[1] + 0.1805f * lin[2];
+    target[1] = 0.2126f * lin[0] + 0.7152f * lin[1] + 0.0722f * lin[2];
+    target[2] = 0.0193f * lin[0] + 0.1192f * lin[1] + 0.9505f * lin[2];
+    float basis[16][3];
+    for (int b = 0; b < 16; b++) {
+        float l = 410.0f + 20.0f * (float)b;
+        float blue = 1.0f / (1.0f + o_exp((l - 490.0f) / 18.0f));
+        float red = 1.0f / (1.0f + o_exp(-(l - 585.0f) / 18.0f));
+        basis[b][0] = red;
+        basis[b][1] = cz_max(1.0f - blue - red, 0.0f);
+        basis[b][2] = blue;
+    }
+    float m[3][3] = {{0}};
+    for (int b = 0; b < 16; b++)
+        for (int i = 0; i < 3; i++)
+            for (int c = 0; c < 3; c++) m[i][c] = m[i][c] + basis[b][c] * cmf[b * 3 + i];
+    float d = o_det3(m);
+    float want[3] = {target[0], target[1], target[2]};
+    for (int it = 0; it < 6; it++) {
+        float k[3];
+        for (int c = 0; c < 3; c++) {
+            float a[3][3];
+            memcpy(a, m, sizeof(a));
+            for (int i = 0; i < 3; i++) a[i][c] = want[i];
+            k[c] = (cz_abs(d) > 1e-12f) ? o_det3(a) / d : 0.0f;
+        }
+        for (int b = 0; b < 16; b++)
+            out[b] = o_clamp(basis[b][0] * k[0] + basis[b][1] * k[1] + basis[b][2] * k[2], 0.005f, 0.97f);
+        float got[3] = {0.0f, 0.0f, 0.0f};
+        for (int b = 0; b < 16; b++)
+            for (int i = 0; i < 3; i++) got[i] = got[i] + out[b] * cmf[b * 3 + i];
+        for (int i = 0; i < 3; i++) want[i] = want[i] + (target[i] - got[i]);
+    }
+}
+
+/* substrate_refl with a canvas colour (mod.rs:188-210, 686) */
+static void of_substrate_refl_colour(uint64_t n, const float* refl, uint32_t* v) {
+    uint32_t packed[8];
+    for (int k = 0; k < 8; k++) packed[k] = cz_f2h(refl[2 * k]) | (cz_f2h(refl[2 * k + 1]) << 16);
+    for (uint64_t c = 0; c < n; c++) memcpy(&v[c * 8], packed, sizeof(packed));
+}
+
 /* substrate_refl without a canvas colour (mod.rs:188-210) */
 void of_substrate_refl(uint64_t n, uint32_t raw, uint32_t* v) {
     float bands[16];
@@ -3112,7 +3182,13 @@ ofilm* of_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigt
 #undef AL
     if (pigtab) memcpy(F->pigtab, pigtab, sizeof(F->pigtab));
     if (cmf) memcpy(F->cmf, cmf, sizeof(F->cmf));
-    of_substrate_refl(n, s->raw_canvas, F->sub_refl);
+    if (s->has_canvas_rgb) {
+        float refl[16];
+        of_canvas_reflectance(s->canvas_rgb, F->cmf, refl);
+        of_substrate_refl_colour(n, refl, F->sub_refl);
+    } else {
+        of_substrate_refl(n, s->raw_canvas, F->sub_refl);
+    }
     of_weave(w, h, F->dx, s->weave_um * 1e-6f, F->m);
     double wsum = 0.0;
     for (size_t c = 0; c < n; c++) wsum += (double)F->m[c];

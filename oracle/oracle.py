@@ -91,6 +91,7 @@ def lib():
                   "of_k_limiter", "of_k_apply", "of_k_clear_acc", "of_k_cg_setup", "of_k_shed", "of_k_slump",
                   "of_k_commit_tmp"):
             getattr(L, k).argtypes = [P]
+        L.of_canvas_reflectance.argtypes = [C.POINTER(f32), C.POINTER(f32), C.POINTER(f32)]
         L.of_fn_shed_excess.restype = f32
         L.of_fn_shed_excess.argtypes = [P, u32]
         L.of_k_strata.argtypes = [P, C.c_int]

@@ -58,6 +58,8 @@ def lib():
         "cz_cmf_table": (None, [fp]),
         "cz_weave": (None, [u32, u32, f32, f32, fp]),
         "cz_substrate_refl": (None, [u64, u32, up]),
+        "cz_canvas_reflectance": (None, [fp, fp, fp]),
+        "cz_substrate_words": (None, [C.POINTER(Settings), fp, up]),
         "cz_deposit_ribbon": (None, [fp, fp, fp, fp, f32, f32, u32, f32, up, f32, C.POINTER(Deposit)]),
         "cz_deposit_capsule": (None, [fp, fp, f32, f32, up, f32, C.POINTER(Deposit)]),
         "cz_deposit_stroke": (None, [fp, fp, f32, f32, up, f32, C.POINTER(Deposit)]),

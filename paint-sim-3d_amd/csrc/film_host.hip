@@ -610,7 +610,9 @@ int init_state(cz_film* f) {
     f->weave_volume = wsum * (double)f->dx * (double)f->dx;
     HIP_TRY(hipMemcpyAsync(f->m, wv.data(), wv.size() * 4, hipMemcpyHostToDevice, f->stream));
     std::vector<uint32_t> white((size_t)f->n * 8);
-    cz_substrate_refl(f->n, f->s.raw_canvas, white.data());
+    uint32_t words[8];
+    cz_substrate_words(&f->s, f->cmf_h, words);
+    for (uint64_t c = 0; c < f->n; c++) memcpy(&white[c * 8], words, sizeof(words));
     HIP_TRY(hipMemcpyAsync(f->sub_refl, white.data(), white.size() * 4, hipMemcpyHostToDevice, f->stream));
     std::vector<float2> ones((size_t)f->n, make_float2(1.0f, 0.0f));
     HIP_TRY(hipMemcpyAsync(f->rheo, ones.data(), ones.size() * 8, hipMemcpyHostToDevice, f->stream));

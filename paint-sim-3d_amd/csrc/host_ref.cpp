@@ -287,6 +287,91 @@ void cz_weave(uint32_t w, uint32_t h, float dx, float amp, float* v) {
     }
 }
 
+// The canvas colour (src/color/mod.rs:27-100).  f32::powf and f32::exp
+// are evaluated in double and rounded once.
+namespace {
+constexpr float VIEW_GAIN[3] = {1.94f, 1.95f, 2.17f};
+float powf_d(float x, float y) { return (float)pow((double)x, (double)y); }
+float expf_d(float x) { return (float)exp((double)x); }
+// sRGB transfer function and its inverse
+float srgb_dec(float c) { return c <= 0.04045f ? c / 12.92f : powf_d((c + 0.055f) / 1.055f, 2.4f); }
+float srgb_enc(float c) { return c <= 0.0031308f ? c * 12.92f : 1.055f * powf_d(c, 1.0f / 2.4f) - 0.055f; }
+float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+// the display tone curve's inverse: x with x (2.51 x + 0.03) / (x (2.43 x + 0.59) + 0.14) = y
+float untone(float y) {
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+    y = clampf(y, 0.0f, 0.995f);
+    const float qa = a - y * c;
+    const float qb = b - y * d;
+    return (-qb + sqrtf(qb * qb + 4.0f * qa * y * e)) / (2.0f * qa);
+}
+// color::canvas_albedo (src/color/mod.rs:29-44): the albedo that shows as `shown`
+void canvas_albedo(const float shown[3], float out[3]) {
+    for (int i = 0; i < 3; i++) out[i] = srgb_enc(clampf(untone(srgb_dec(shown[i])) / VIEW_GAIN[i], 0.005f, 0.9f));
+}
+float det3(const float a[3][3]) {
+    return a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+           a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+}
+}  // namespace
+
+// color::reflectance_of (src/color/mod.rs:46-100): three smooth band bases
+// (red, green, blue) mixed to hit the colour's XYZ through the CMF table, the
+// clamped result corrected six times
+void cz_canvas_reflectance(const float rgb[3], const float* cmf, float out[16]) {
+    float albedo[3];
+    canvas_albedo(rgb, albedo);
+    float lin[3];
+    for (int i = 0; i < 3; i++) lin[i] = srgb_dec(albedo[i]);
+    const float target[3] = {0.4124f * lin[0] + 0.3576f * lin[1] + 0.1805f * lin[2],
+                             0.2126f * lin[0] + 0.7152f * lin[1] + 0.0722f * lin[2],
+                             0.0193f * lin[0] + 0.1192f * lin[1] + 0.9505f * lin[2]};
+    float basis[16][3];
+    for (int b = 0; b < 16; b++) {
+        const float l = 410.0f + 20.0f * (float)b;
+        const float blue = 1.0f / (1.0f + expf_d((l - 490.0f) / 18.0f));
+        const float red = 1.0f / (1.0f + expf_d(-(l - 585.0f) / 18.0f));
+        const float green = 1.0f - blue - red;
+        basis[b][0] = red;
+        basis[b][1] = green > 0.0f ? green : 0.0f;
+        basis[b][2] = blue;
+    }
+    float m[3][3] = {};
+    for (int b = 0; b < 16; b++)
+        for (int i = 0; i < 3; i++)
+            for (int c = 0; c < 3; c++) m[i][c] = m[i][c] + basis[b][c] * cmf[b * 3 + i];
+    const float d = det3(m);
+    float want[3] = {target[0], target[1], target[2]};
+    for (int it = 0; it < 6; it++) {
+        float k[3];
+        for (int c = 0; c < 3; c++) {
+            float a[3][3];
+            memcpy(a, m, sizeof(a));
+            for (int i = 0; i < 3; i++) a[i][c] = want[i];
+            k[c] = fabsf(d) > 1e-12f ? det3(a) / d : 0.0f;
+        }
+        for (int b = 0; b < 16; b++)
+            out[b] = clampf(basis[b][0] * k[0] + basis[b][1] * k[1] + basis[b][2] * k[2], 0.005f, 0.97f);
+        float got[3] = {0.0f, 0.0f, 0.0f};
+        for (int b = 0; b < 16; b++)
+            for (int i = 0; i < 3; i++) got[i] = got[i] + out[b] * cmf[b * 3 + i];
+        for (int i = 0; i < 3; i++) want[i] = want[i] + (target[i] - got[i]);
+    }
+}
+
+void cz_substrate_words(const cz_settings* s, const float* cmf, uint32_t out[8]) {
+    float bands[16];
+    if (s->has_canvas_rgb) {
+        cz_canvas_reflectance(s->canvas_rgb, cmf, bands);
+    } else {
+        for (int b = 0; b < 16; b++) {
+            float t = (float)b / 15.0f;
+            bands[b] = s->raw_canvas ? 0.50f + 0.16f * t : 0.84f + 0.04f * t;
+        }
+    }
+    for (int k = 0; k < 8; k++) out[k] = cz_f2h(bands[2 * k]) | (cz_f2h(bands[2 * k + 1]) << 16);
+}
+
 // substrate_refl without a canvas colour (src/film/mod.rs:188-210)
 void cz_substrate_refl(uint64_t n, uint32_t raw, uint32_t* v) {
     float bands[16];

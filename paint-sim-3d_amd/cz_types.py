@@ -46,7 +46,8 @@ class Settings(C.Structure):
                 ("lnmu_cure", f32), ("v_cap", f32), ("tilt_deg", f32), ("tilt_dir_deg", f32), ("gravity", f32),
                 ("medium_mu", f32), ("debug_view", u32), ("h_min", f32), ("m_cap_hydro", f32),
                 ("max_substeps", u32), ("min_substeps", u32), ("implicit_iters", u32), ("implicit_hydro", u32),
-                ("slump", u32), ("capillary", u32), ("capillary_implicit", u32), ("n_per_cell", u32)]
+                ("slump", u32), ("capillary", u32), ("capillary_implicit", u32), ("n_per_cell", u32),
+                ("has_canvas_rgb", u32), ("canvas_rgb", f32 * 3)]
 
 
 class Overrides(C.Structure):
@@ -128,6 +129,7 @@ def default_settings():
     s.max_substeps, s.min_substeps, s.implicit_iters, s.implicit_hydro = 64, 4, 40, 1
     s.slump, s.capillary, s.capillary_implicit = 0, 1, 0
     s.n_per_cell = 0
+    s.has_canvas_rgb = 0   # canvas_rgb: None
     return s
 
 
