@@ -686,10 +686,14 @@ static float pig_dist(const float* a, const float* b) {
 }
 
 /* strata.wgsl:45-59 */
+/* strata.wgsl:45-59.  The weighted mean takes one reciprocal of the weight
+ * and sixteen products (within 1.5 ULP of the quotient; WGSL allows 2.5 ULP
+ * for the shader's sixteen divisions). */
 static void pig_mix(const float* a, float wa, const float* b, float wb, float* o) {
     float t[16];
     float w = cz_max(wa + wb, 1e-20f);
-    for (int i = 0; i < 16; i++) t[i] = (a[i] * wa + b[i] * wb) / w;
+    float rw = 1.0f / w;
+    for (int i = 0; i < 16; i++) t[i] = (a[i] * wa + b[i] * wb) * rw;
     float va = (a[14] > 0.0f) ? wa : 0.0f;
     float vb = (b[14] > 0.0f) ? wb : 0.0f;
     if (va + vb > 0.0f) t[14] = (a[14] * va + b[14] * vb) / (va + vb);

@@ -718,12 +718,15 @@ __device__ __forceinline__ float pig_dist(const float* a, const float* b) {
     return d;
 }
 
-// strata.wgsl:45-59 (o may alias a)
+// strata.wgsl:45-59 (o may alias a); one reciprocal of the weight and
+// sixteen products (within 1.5 ULP of the quotient, inside WGSL's 2.5 ULP
+// for the shader's divisions)
 __device__ __forceinline__ void pig_mix(const float* a, float wa, const float* b, float wb, float* o) {
     float a14 = a[14], b14 = b[14];
     float w = cz_max(wa + wb, 1e-20f);
+    float rw = 1.0f / w;
 #pragma unroll
-    for (int i = 0; i < 16; i++) o[i] = (a[i] * wa + b[i] * wb) / w;
+    for (int i = 0; i < 16; i++) o[i] = (a[i] * wa + b[i] * wb) * rw;
     float va = (a14 > 0.0f) ? wa : 0.0f;
     float vb = (b14 > 0.0f) ? wb : 0.0f;
     if (va + vb > 0.0f) o[14] = (a14 * va + b14 * vb) / (va + vb);
