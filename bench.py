@@ -14,17 +14,18 @@ as its output only feeds the renderer.  value = canvas cells x frames / wall
 time of the timed frames (state resident in HBM).
 
 Phases, each on a fresh film from the same initial state: W warmup frames
-with HIP events around every launch (picks the dominant kernel family), the
-K timed frames as the library runs them by default (graph replay, flux
-shapes tuned online, no events), and the same K frames replayed with events
-around every launch for the roofline: the substep kernel family with the
-largest device time; achieved = algorithmic bytes (per active cell, below,
-times the active cells its launches covered) / its event time.  Between the
+with HIP events around every launch, the K timed frames as the library runs
+them by default (graph replay, substep shapes tuned online, no events), and
+the same K frames replayed with events around every launch for the
+roofline, whose unit is the substep (the dominant work: about 85 % of the
+frame's device time), whatever launches the tuner split it into:
+achieved = 76 B (+ 4 with an index per cell) per active cell and substep /
+the substep's device time (its launches' event times summed).  Between the
 phases a one-element fill kernel marks the boundary in a kernel trace.
-roofline.traffic: HBM bytes per launch of that kernel: the HBM bytes per
-covered cell of its family from the rocprof FETCH_SIZE / WRITE_SIZE runs of
-the same command recorded under profiles/, times this run's cells per launch
-(null when no record matches the command).  cpu_baseline: the CPU
+roofline.traffic: HBM bytes per substep: the HBM bytes per active cell and
+substep of the substep launches from the rocprof FETCH_SIZE / WRITE_SIZE
+runs of the same command recorded under profiles/, times this run's cells
+per substep (null when no record matches the command).  cpu_baseline: the CPU
 restatement (oracle/, its per-tile passes on every host thread) stepping the
 same canvas from the same initial state, frame by frame, for up to 25 s.
 """
@@ -97,6 +98,11 @@ def canvas_scene(seed, cell_n):
 ROOFLINE_FAMILIES = {"lap_plug": ("lap_plug", "lap_plug_w"), "flux": ("flux", "flux_w"), "apply": ("apply", "apply_w"),
                      "flux_apply": ("flux_apply", "flux_apply_w"), "substep_fused": ("substep_fused",),
                      "substeps": ("substeps",)}
+SUBSTEP_LAUNCH_FAMILIES = tuple(f for fams in ROOFLINE_FAMILIES.values() for f in fams)
+# Compulsory bytes of one substep per active cell (a fused explicit substep:
+# R h, m, rheo, delta, fnorm, acc; W h, delta, fnorm, acc), + 4 with an index
+# per cell (R n)
+SUBSTEP_ALGO_BYTES = 76
 
 
 def log(msg):
@@ -742,34 +748,25 @@ def main():
             film.step()
         film.sync()
         st = film.stats()
-    prof_l = {f: film.profile_read(f) for fams in ROOFLINE_FAMILIES.values() for f in fams}
-    prof = {}
-    for fam, parts in ROOFLINE_FAMILIES.items():
-        tot = sum(prof_l[f][0] * prof_l[f][1] for f in parts)
-        cnt = sum(prof_l[f][1] for f in parts)
-        prof[fam] = (tot / cnt if cnt else 0.0, cnt)
+    prof_l = {f: film.profile_read(f) for f in SUBSTEP_LAUNCH_FAMILIES}
     film.profile(False)
     film.close()
     del film
     phase_marker(4)
-    dom = max(ROOFLINE_FAMILIES, key=lambda fam: prof[fam][0] * prof[fam][1])
-    avg_us, launches = prof[dom]
-    if dom == "substeps":
-        # one launch per frame: every substep's three passes and the last nose
-        cells_per_launch = st.one_launch_cell_substeps / max(1, launches)
-        sub_bytes = SUBSTEP_BYTES + (8 if cell_n else 0)
-        bytes_per_launch = (sub_bytes * st.one_launch_cell_substeps
-                            + ALGO_BYTES["lap"] * st.one_launch_cells) / max(1, launches)
-        algo_per_cell = sub_bytes
-    else:
-        # cells x substeps the family's launches covered: every per-pass
-        # substep runs lap_plug; flux, apply and flux_apply by flux shape
-        shp = list(st.flux_shape_cell_substeps)
-        covered = {"lap_plug": shp[0] + shp[1] + shp[2] + shp[3] + shp[4], "flux": shp[0] + shp[1] + shp[3],
-                   "apply": shp[0] + shp[1] + shp[3], "flux_apply": shp[2] + shp[4], "substep_fused": shp[5]}[dom]
-        cells_per_launch = covered / max(1, launches)
-        algo_per_cell = algo_bytes(dom, cell_n)
-        bytes_per_launch = algo_per_cell * cells_per_launch
+    # The roofline's unit is the substep.  The tuner runs a substep as three
+    # launches (lap_plug, flux, apply), two (lap_plug, flux_apply) or one
+    # (the whole substep), in 256- or 1024-thread blocks, and which it picks
+    # varies from run to run; a substep's device time (the sum of its
+    # launches) and its compulsory bytes compare whatever the mix.
+    sub_us = sum(prof_l[f][0] * prof_l[f][1] for f in SUBSTEP_LAUNCH_FAMILIES)
+    one_sub = st.one_launch_cell_substeps / max(1, st.one_launch_cells)   # substeps of a one-launch frame
+    n_sub = prof_l["lap_plug"][1] + prof_l["lap_plug_w"][1] + prof_l["substep_fused"][1] + \
+        round(prof_l["substeps"][1] * one_sub)
+    algo_per_cell = SUBSTEP_ALGO_BYTES + (4 if cell_n else 0)
+    cells_per_launch = st.act_cell_substeps / max(1, n_sub)
+    bytes_per_launch = algo_per_cell * cells_per_launch
+    avg_us = sub_us / max(1, n_sub)
+    launches = n_sub
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
     ms_per_step = elapsed * 1e3 / args.steps
     value = n * n * args.steps / elapsed
@@ -779,8 +776,8 @@ def main():
         ", ".join(f"{k} {v[0]:.2f}x{v[1]}" for k, v in prof_l.items() if v[1]))
     log(f"bench: replay cell-substeps per substep shape {list(st.flux_shape_cell_substeps)}")
     log(f"bench: {ms_per_step:.3f} ms/frame, active cells/frame {st.act_cells / frames:.0f}, "
-        f"substeps/frame {st.act_cell_substeps / max(1, st.act_cells):.1f}, {dom}: {avg_us:.2f} us x {launches}, "
-        f"{cells_per_launch:.0f} cells/launch")
+        f"substeps/frame {st.act_cell_substeps / max(1, st.act_cells):.1f}, substep: {avg_us:.2f} us x {launches}, "
+        f"{cells_per_launch:.0f} cells/substep")
     uniform = None
     if not args.no_uniform:
         # the same canvas and frames at one film index: the reference's
@@ -841,14 +838,19 @@ def main():
                    "substeps_per_frame": st.act_cell_substeps / max(1, st.act_cells),
                    "implicit_cell_steps": st.act_cell_implicit,
                    "frame_algo_GBps": frame_gbs},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": "substep (k_lap_plug[_w], k_fluxes[_w|_faces], k_limit_apply[_w], "
+                                                 "k_flux_apply[_w], k_substep_fused)",
+                     "unit_of_work": "one substep over the active tiles (the sum of its launches)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (lambda b: None if b is None else b * cells_per_launch)(
-                         recorded_traffic("film_plain_2048_hbm_traffic.md", "bytes_per_cell", kernel=dom, size=n,
+                         recorded_traffic("film_plain_2048_hbm_traffic.md", "bytes_per_cell", kernel="substep", size=n,
                                           steps=args.steps, warmup=args.warmup)),
                      "measured_copy_GBps": measured["copy_GBps"], "frac_of_measured_copy": achieved / measured["copy_GBps"],
-                     "algo_bytes_per_launch": bytes_per_launch, "algo_bytes_per_cell": algo_per_cell, "avg_launch_us": avg_us, "launches": launches,
-                     "cells_per_launch": cells_per_launch},
+                     "algo_bytes_per_substep": bytes_per_launch, "algo_bytes_per_cell": algo_per_cell,
+                     "avg_substep_us": avg_us, "substeps": launches, "cells_per_substep": cells_per_launch,
+                     "launches": {f: [round(prof_l[f][0], 3), prof_l[f][1]] for f in SUBSTEP_LAUNCH_FAMILIES
+                                  if prof_l[f][1]}},
         "cpu_baseline": cpu,
         "plain_uniform_index": uniform,
         "km": km_rec,
