@@ -14,11 +14,12 @@ as its output only feeds the renderer.  value = canvas cells x frames / wall
 time of the timed frames (state resident in HBM).
 
 Phases, each on a fresh film from the same initial state: W warmup frames
-with HIP events around every launch, the K timed frames as the library runs
-them by default (graph replay, substep shapes tuned online, no events), and
-the same K frames replayed with events around every launch for the
-roofline, whose unit is the substep (the dominant work: about 85 % of the
-frame's device time), whatever launches the tuner split it into:
+and the K timed frames as the library runs them by default (substep shapes
+tuned online by the device's tuner, which the warmup frames train, graph
+replay of small frames, no events), and the same K frames replayed with
+events around every launch (and the tuner's shapes) for the roofline,
+whose unit is the substep (the dominant work: about 85 % of the frame's
+device time), whatever launches the tuner split it into:
 achieved = 76 B (+ 4 with an index per cell) per active cell and substep /
 the substep's device time (its launches' event times summed).  Between the
 phases a one-element fill kernel marks the boundary in a kernel trace.
@@ -534,13 +535,6 @@ def bench_serve(args):
     return out
 
 
-FRAME_FAMILIES = SUBSTEP_FAMILIES + ("flux_apply", "lap_plug_w", "flux_w", "apply_w", "flux_apply_w", "substep_fused",
-                                     "substeps", "lap", "mark", "compact", "compact_list", "cure", "deposit", "split", "reduce_pool",
-                                     "strata_limit", "strata_update", "strata_fill",
-                                     "frame_stats", "cg_setup", "cg_lap", "cg_init", "cg_iter_a", "cg_iter_b",
-                                     "cg_iter_c", "cg_reduce", "cg_finish", "cg_rescale")
-
-
 def frame_algo_bytes(n_grid, st, cell_n=False):
     """Compulsory bytes of the frames folded in `st`:
     N_grid*20 + N_act*(632 + 76*S + I*(180 + 68*K)) per frame, K = 40 CG iterations;
@@ -619,9 +613,10 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first copy this many MiB device to device 3 times (a known-size dispatch that "
                          "calibrates the HBM byte counters of a profiler run)")
-    ap.add_argument("--flux-shape", type=int, default=-1, choices=(-1, 0, 1, 2, 3, 4, 5),
+    ap.add_argument("--flux-shape", type=int, default=-1, choices=(-1, 0, 1, 2, 3, 4, 5, 6),
                     help="substep shape: -1 tuned online, 0 window, 1 face-parallel, 2 flux + apply fused, "
-                         "3 and 4 as 0 and 2 with 1024 threads per tile, 5 the whole substep in one launch "
+                         "3 and 4 as 0 and 2 with 1024 threads per tile, 5 the whole substep in one launch, "
+                         "6 the flux of 1 between the lap/plug and apply of 3 "
                          "(single canvases; the batch keeps its tuned shapes)")
     ap.add_argument("--substep-launch", default="auto", choices=("auto", "per-pass", "one"),
                     help="substeps as one launch per frame on small active sets (auto), always, or never")
@@ -701,21 +696,15 @@ def main():
         assert f.write_state_arrays(state)
         return f
 
-    # warmup film: the first frames of the same run, every family timed, to pick the dominant one
+    # warmup film: the first frames of the same run, as the library runs them
+    # (the substep tuner of the device times its launch shapes here; later
+    # films on the device start from its measurements)
     phase_marker(1)
     film = fresh()
-    film.profile(True)
     for _ in range(max(1, args.warmup)):
         film.step()
     film.sync()
-    fam_time, breakdown = {}, {}
-    for fam in FRAME_FAMILIES:
-        avg, cnt = film.profile_read(fam)
-        if cnt:
-            breakdown[fam] = (avg, cnt)
-            fam_time[fam] = avg * cnt
-    log(f"bench: warmup {args.warmup} frames, per-family device time (us avg x launches): " +
-        ", ".join(f"{k} {v[0]:.1f}x{v[1]}" for k, v in sorted(breakdown.items(), key=lambda kv: -kv[1][0] * kv[1][1])))
+    log(f"bench: warmup {args.warmup} frames")
     film.close()
     del film
 

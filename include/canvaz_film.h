@@ -230,9 +230,10 @@ typedef struct cz_stats {
     uint64_t one_launch_cell_substeps;
     /* active cells times substeps run with each substep shape (0 window
      * flux, 1 face-parallel flux, 2 flux + apply in one launch, 3 and 4 as 0
-     * and 2 with 1024 threads per tile, 5 the whole substep in one launch;
-     * see cz_film_flux_shape) */
-    uint64_t flux_shape_cell_substeps[6];
+     * and 2 with 1024 threads per tile, 5 the whole substep in one launch,
+     * 6 the flux of 1 between the lap/plug and apply of 3; see
+     * cz_film_flux_shape) */
+    uint64_t flux_shape_cell_substeps[7];
 } cz_stats;
 
 /* The mass report of `compute_stats` (src/main.rs:1679-1819), sums in FP64. */
@@ -451,7 +452,9 @@ void* cz_film_stream(cz_film* f);
  * per frame shape (substeps, implicit steps, deposit pass, offset-buffer
  * parity, params, grid sizes, flux shape). 1 = on (default), 0 = direct
  * launches.  Frames whose flux size class is still being timed by the shape
- * tuner, and profiled frames, run direct. */
+ * tuner, frames of more than 64 active tiles (whose launches run long enough
+ * that a replay saves nothing) and profiled frames run direct; a shape is
+ * captured the second time it comes. */
 int cz_film_use_graphs(cz_film* f, int on);
 /* How the substeps of a frame are launched: 0 = three launches per substep
  * (lap/plug, flux, limiter + apply), 1 = one launch for all of them with
@@ -536,7 +539,9 @@ int cz_film_graph_counts(const cz_film* f, uint64_t* captures, uint64_t* replays
  * one launch (each block recomputes the nose, pressures and plug means of a
  * 7-cell ring; heights alternate between two buffers over the frame's
  * substeps, and a frame whose parity would make its first substep read and
- * write the same buffer runs that substep as shape 4), -1 = chosen online
+ * write the same buffer runs that substep as shape 4), 6 = the face-parallel
+ * flux of shape 1 between the 1024-thread lap/plug and limiter/apply of
+ * shape 3 (for a few hundred active tiles), -1 = chosen online
  * (default): per active-set size the shapes are timed with HIP events on live
  * substeps (they give bit-identical results) and the fastest is used. */
 int cz_film_flux_shape(cz_film* f, int shape);

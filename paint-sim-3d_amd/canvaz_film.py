@@ -436,7 +436,8 @@ class Film:
     def flux_shape(self, shape):
         """Substep launch shape: -1 = tuned online, 0 = LDS window per tile,
         1 = block per (tile, face), 2 = flux + apply fused, 3 and 4 = 0 and 2
-        with 1024 threads per tile."""
+        with 1024 threads per tile, 5 = the whole substep in one launch, 6 =
+        the flux of 1 between the 1024-thread lap/plug and apply of 3."""
         check(lib().cz_film_flux_shape(self._p, shape))
 
     def pool(self):

@@ -89,6 +89,18 @@ void fold_profile(cz_film* f) {
 
 namespace czi {
 
+ShapeTuner* device_substep_tuner(int device) {
+    static std::mutex m;
+    static std::map<int, ShapeTuner*> books;  // held until the process ends
+    std::lock_guard<std::mutex> g(m);
+    ShapeTuner*& t = books[device];
+    if (!t) {
+        t = new ShapeTuner();
+        t->ns = kSubstepShapes;
+    }
+    return t;
+}
+
 void fold_tuners(std::vector<TunerSlot>& pending, std::vector<hipEvent_t>& pool) {
     for (auto& s : pending) {
         float ms = 0.0f;
@@ -345,7 +357,9 @@ int record_frame(cz_film* f, uint32_t substeps) {
     const bool slump = f->s.slump != 0u;
     for (uint32_t s = 0; s < nsub; s++) {
         bool timed = false;
-        shp[s] = f->flux_tuner.choose(std::min(f->active_hint, f->nt16), !f->capturing, bks[s], timed);
+        // profiled frames time every launch: their substeps carry the
+        // events' gaps, so they take the tuner's choice and add no samples
+        shp[s] = f->flux_tuner.choose(std::min(f->active_hint, f->nt16), !f->capturing && !f->prof_on, bks[s], timed);
         tim[s] = timed ? 1 : 0;
         // the fused shape carries the nose into its substep; with the slump
         // passes between the substeps the passes run separately
@@ -417,6 +431,10 @@ int record_frame(cz_film* f, uint32_t substeps) {
             case 4:
                 LAUNCH("flux_apply_w", k_flux_apply_w<OneFilm>, gt, b1024, w);
                 break;
+            case 6:
+                LAUNCH("flux", k_fluxes_faces<OneFilm>, dim3(f->tile_grid(4)), b256, w);
+                LAUNCH("apply_w", k_limit_apply_w<OneFilm>, gt, b1024, w);
+                break;
             default:
                 LAUNCH("flux", k_fluxes<OneFilm>, gt, b256, w);
                 LAUNCH("apply", k_limit_apply<OneFilm>, gt, b256, w);
@@ -435,7 +453,7 @@ int record_frame(cz_film* f, uint32_t substeps) {
         }
         if (ea && eb) {
             hipEventRecord(eb, f->stream);
-            f->tuner_pending.push_back({&f->flux_tuner, bk, shape, ea, eb, (double)std::min(f->active_hint, f->nt16)});
+            f->tuner_pending.push_back({f->flux_tuner.t, bk, shape, ea, eb, (double)std::min(f->active_hint, f->nt16)});
         }
         f->shape_subs_now[shape]++;
     }
@@ -482,7 +500,8 @@ int encode_frame(cz_film* f, uint32_t substeps) {
     if (jrc) return jrc;
     const uint64_t flux_items = std::min(f->active_hint, f->nt16);
     const uint32_t pg = f->persist_grid();
-    if (!f->use_graphs || f->prof_on || (pg == 0u && f->flux_tuner.learning(flux_items)))
+    if (!f->use_graphs || f->prof_on || flux_items > czi::kGraphMaxTiles ||
+        (pg == 0u && f->flux_tuner.learning(flux_items)))
         return encode_direct(f, substeps);
     GraphKey key;
     memset(&key, 0, sizeof(key));
@@ -623,7 +642,8 @@ int init_state(cz_film* f) {
 void destroy(cz_film* f) {
     if (!f) return;
     batch_detach(f);
-    if (f->stream) hipStreamSynchronize(f->stream);
+    // the last frames' tuner samples go to the device's tuner
+    if (f->stream && hipStreamSynchronize(f->stream) == hipSuccess) fold_tuners(f->tuner_pending, f->ev_pool);
     for (void* p : f->allocs) hipFree(p);
     if (f->frame_out_h) hipHostFree(f->frame_out_h);
     if (f->dep_h) hipHostFree(f->dep_h);
@@ -700,7 +720,6 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
     if ((w + 15u) / 16u > 21845u) return set_err(CZ_EINVAL, "cz_film_create: width %u exceeds 349520 cells", w);
     *out = nullptr;
     cz_film* f = new cz_film();
-    f->flux_tuner.ns = czi::kSubstepShapes;
     if (s) f->s = *s; else cz_settings_default(&f->s);
     cz_overrides_from_env(&f->ov);
     f->cap_n = w * h;
@@ -723,6 +742,7 @@ int film_create(uint32_t w, uint32_t h, const cz_settings* s, const float* pigta
         destroy(f);
         return set_err(CZ_EHIP, "cz_film_create: %s", hipGetErrorString(e));
     }
+    f->flux_tuner.t = czi::device_substep_tuner(f->device);
     size_t n = f->n, nt16 = f->nt16, nt64 = (size_t)f->t64x * f->t64y, np = (size_t)f->pw * f->ph;
 #define A(p, cnt) if (!rc) rc = dalloc(f, &f->p, (cnt))
     A(h, n); A(m, n); A(rheo, n); A(strata_h, n); A(strata_pig, 8 * n); A(strata_cure, n); A(sub_refl, 8 * n);
@@ -1198,7 +1218,7 @@ int cz_film_substep_launch(cz_film* f, int mode) {
 }
 
 int cz_film_flux_shape(cz_film* f, int shape) {
-    if (!f || shape < -1 || shape >= czi::kSubstepShapes) return set_err(CZ_EINVAL, "flux_shape: -1 (tuned) or 0-5");
+    if (!f || shape < -1 || shape >= czi::kSubstepShapes) return set_err(CZ_EINVAL, "flux_shape: -1 (tuned) or 0-6");
     f->flux_tuner.fixed = shape;
     return CZ_OK;
 }

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,11 +56,23 @@ constexpr uint32_t kDevBinMin = 1024u;
 //   2 lap + plug, flux + limiter + apply in one launch;
 //   3 as 0 with 1024 threads per tile (flux one face per thread);
 //   4 as 2 with 1024 threads per tile;
-//   5 the whole substep in one launch (k_substep_fused; single films).
+//   5 the whole substep in one launch (k_substep_fused; single films);
+//   6 lap + plug and limiter + apply with 1024 threads per tile around the
+//     face-parallel flux of 1 (single films).
 // 256-thread blocks hold 4 waves per tile; a few hundred active tiles leave
 // the GPU at 1-2 waves per SIMD, where the wide shapes take each window phase
-// in one pass with 16 waves per tile.
-constexpr int kSubstepShapes = 6;
+// in one pass with 16 waves per tile.  The face-parallel flux already spreads
+// a tile over four blocks, so it beats the wide flux at every size; with the
+// wide lap and apply beside it, shape 6 is the fastest below a few hundred
+// active tiles and shape 1 above (2048^2 blob canvas: 19.9 against 23.2 us per
+// substep for shape 4 at 100-130 tiles, 62.8 against 110 for shape 3 at 1500).
+constexpr int kSubstepShapes = 7;
+// Frames of more active tiles than this run direct, not as graph replays:
+// their launches take long enough on the device that direct submissions keep
+// ahead of it, so a replay saves nothing, while a capture costs 0.5-0.8 ms of
+// host time (2048^2 blob canvas, 80-1500 active tiles: replayed and direct
+// frames within 2 %; at 0-20 tiles a replay is 20 % faster)
+constexpr uint64_t kGraphMaxTiles = 64;
 constexpr int kBatchShapes = 5;  // a batch runs shapes 0-4
 
 struct ProfSlot {
@@ -77,15 +90,19 @@ struct ProfSlot {
 // known only once its frame has run, so exploration counts the samples
 // issued, not the ones folded: the first frame of a bucket times every shape
 // in turn (WARM substeps each), and the frame after it uses the fastest.
-// fixed >= 0 pins a shape.
+// Until a bucket has samples, its substeps take the fastest shape of the
+// nearest bucket that has some (the ranking moves slowly with size), not a
+// fixed default.  fixed >= 0 pins a shape.
 struct ShapeTuner {
-    static constexpr int NB = 24, NS = 6, WARM = 3, RECHECK = 64;
+    static constexpr int NB = 24, NS = 7, WARM = 3, RECHECK = 64;
     int ns = 2;  // shapes in use (at most NS)
     double best_t[NB][NS] = {};   // least time per item over the samples
     uint32_t cnt[NB][NS] = {};
     uint32_t issued[NB][NS] = {};
     uint32_t uses[NB] = {};
     int fixed = -1;
+    // a tuner may be shared by films stepped from several host threads
+    mutable std::mutex mu;
     static int bucket(uint64_t items) {
         int b = 0;
         while (b + 1 < NB && (1ull << (b + 1)) <= items) b++;
@@ -97,6 +114,7 @@ struct ShapeTuner {
         bk = bucket(items);
         timed = false;
         if (fixed >= 0) return fixed;
+        std::lock_guard<std::mutex> g(mu);
         if (!can_time) return best_of(bk);
         // explore: the first shape still short of WARM samples issued, so
         // that a shape's samples come in consecutive substeps (the fused
@@ -116,14 +134,38 @@ struct ShapeTuner {
         }
         return best;
     }
-    // a substep issued as shape `from` ran as shape `to`
+    // a substep issued as shape `from` ran as shape `to` (the fused substep
+    // runs in pairs; a lone one runs as separate passes).  The issue still
+    // counts for `from`: were it taken back, a shape that can only be issued
+    // alone would be issued again on every frame and its bucket never leave
+    // the learning state
     void reissue(int bk, int from, int to) {
-        if (issued[bk][from] > 0) issued[bk][from]--;
+        (void)from;
+        std::lock_guard<std::mutex> g(mu);
         issued[bk][to]++;
     }
-    // the shape with the least time per item among those with samples
-    // (shape 0 if none has any)
-    int best_of(int bk) const {
+    // true while a shape of this size class still has warm-up samples to
+    // issue (a graph replay cannot time them: such frames run uncaptured)
+    bool learning(uint64_t items) const {
+        if (fixed >= 0) return false;
+        const int b = bucket(items);
+        std::lock_guard<std::mutex> g(mu);
+        for (int k = 0; k < ns; k++)
+            if (issued[b][k] < (uint32_t)WARM) return true;
+        return false;
+    }
+    // one timed launch: us of device time over `items` work items
+    void record(int bk, int shape, double us, double items) {
+        const double t = us / (items > 1.0 ? items : 1.0);
+        std::lock_guard<std::mutex> g(mu);
+        if (cnt[bk][shape] == 0 || t < best_t[bk][shape]) best_t[bk][shape] = t;
+        cnt[bk][shape] += 1;
+    }
+
+  private:
+    // the shape with the least time per item among the bucket's sampled
+    // shapes, -1 if it has none
+    int best_in(int bk) const {
         int best = -1;
         double bt = 0.0;
         for (int k = 0; k < ns; k++) {
@@ -134,23 +176,45 @@ struct ShapeTuner {
                 bt = t;
             }
         }
-        return best < 0 ? 0 : best;
+        return best;
     }
-    // true while a shape of this size class still lacks its warm-up samples
-    // (a graph replay cannot take them: such frames run uncaptured)
-    bool learning(uint64_t items) const {
-        if (fixed >= 0) return false;
-        const int b = bucket(items);
-        for (int k = 0; k < ns; k++)
-            if (cnt[b][k] < (uint32_t)WARM) return true;
-        return false;
+    // the bucket's fastest shape, else the nearest sampled bucket's (the
+    // smaller size first at equal distance), else shape 0
+    int best_of(int bk) const {
+        for (int d = 0; d < NB; d++) {
+            if (bk - d >= 0) {
+                const int k = best_in(bk - d);
+                if (k >= 0) return k;
+            }
+            if (d > 0 && bk + d < NB) {
+                const int k = best_in(bk + d);
+                if (k >= 0) return k;
+            }
+        }
+        return 0;
     }
-    // one timed launch: us of device time over `items` work items
-    void record(int bk, int shape, double us, double items) {
-        const double t = us / (items > 1.0 ? items : 1.0);
-        if (cnt[bk][shape] == 0 || t < best_t[bk][shape]) best_t[bk][shape] = t;
-        cnt[bk][shape] += 1;
+};
+
+// The substep tuner of one device, shared by the single films on it in the
+// process: a film created after others starts from their measurements
+// instead of timing every shape again, and films stepped in turn share the
+// exploration.  (A batch keeps a tuner of its own: its menu differs.)
+ShapeTuner* device_substep_tuner(int device);
+
+// A film's handle on its device's tuner; the pinned shape is the film's own.
+struct FilmTuner {
+    ShapeTuner* t = nullptr;
+    int fixed = -1;
+    int choose(uint64_t items, bool can_time, int& bk, bool& timed) {
+        if (fixed >= 0) {
+            bk = ShapeTuner::bucket(items);
+            timed = false;
+            return fixed;
+        }
+        return t->choose(items, can_time, bk, timed);
     }
+    bool learning(uint64_t items) const { return fixed < 0 && t->learning(items); }
+    void reissue(int bk, int from, int to) { t->reissue(bk, from, to); }
 };
 
 struct TunerSlot {
@@ -175,7 +239,7 @@ struct cz_film {
     uint32_t active_hint = 0xffffffffu;  // grid sizing: last folded active count
     // flux pass shape: 0 = one block per tile over an LDS window, 1 = one
     // block per (tile, face) reading through the caches
-    czi::ShapeTuner flux_tuner;
+    czi::FilmTuner flux_tuner;
     std::vector<czi::TunerSlot> tuner_pending;
     bool capturing = false;
     // substep launches: -1 chosen by the active-set size, 0 three launches per

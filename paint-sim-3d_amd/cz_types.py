@@ -62,7 +62,7 @@ class Stats(C.Structure):
                 ("weave_volume", C.c_double), ("act_cells", C.c_uint64), ("act_cell_substeps", C.c_uint64),
                 ("frames_counted", C.c_uint64), ("act_cell_implicit", C.c_uint64),
                 ("one_launch_frames", C.c_uint64), ("one_launch_cells", C.c_uint64),
-                ("one_launch_cell_substeps", C.c_uint64), ("flux_shape_cell_substeps", C.c_uint64 * 6)]
+                ("one_launch_cell_substeps", C.c_uint64), ("flux_shape_cell_substeps", C.c_uint64 * 7)]
 
 
 class Scene(C.Structure):

@@ -188,7 +188,7 @@ def test_cure_commit(gpu):
     assert (mo > st["m"]).any(), "no stratum was committed"
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6])
 def test_every_substep_shape(gpu, shape):
     """Each launch shape of the substep pinned (cz_film_flux_shape): 256- and
     1024-thread tiles, staged, face-parallel and fused flux, and the whole
@@ -228,7 +228,7 @@ def test_cell_index_flowing_blobs(gpu):
     assert n.min() < 0.6 and n.max() > 0.8, "the scene does not mix indices"
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5, 6])
 def test_cell_index_every_substep_shape(gpu, shape):
     w = h = 112
     s = cell_n_settings(w, 35.0)

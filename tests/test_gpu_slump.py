@@ -39,7 +39,7 @@ def test_scene_sheds():
     assert not np.array_equal(on.buf("h"), off.buf("h"))
 
 
-@pytest.mark.parametrize("shape", [-1, 0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("shape", [-1, 0, 1, 2, 3, 4, 5, 6])
 def test_slump_frames(gpu, shape):
     w = 96
     s, st = spiky_scene(w)
