@@ -894,45 +894,80 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
         d_out = 0u;
         fn_out = make_float2(0.0f, 0.0f);
         if (sw.h[wq(i, j)] <= 0.0f) return;
-        int bx = 64, by = 64;
-        int bd = kNoseNone;
-        for (int dj = -1; dj <= 1; dj++) {
-            for (int di = -1; di <= 1; di++) {
-                const int xx = i + di, yy = j + dj;
-                if (xx < 0 || yy < 0 || xx >= W_ || yy >= H_) continue;
-                int cx = di, cy = dj;
+        // The candidates of the 3x3 neighbourhood (k = 3 (dj + 1) + di + 1,
+        // the shader's order): a dry neighbour itself, a wet one's target
+        // when that target is dry.  A target in the window is read from
+        // LDS as its candidate is formed; the targets beyond it are read
+        // from HBM for all nine before any is compared, so that those reads
+        // are in flight together.  cand[k]: the candidate's offset (10-bit
+        // two's complement x and y) and kind (bits 20-21: 0 none, 1 dry
+        // neighbour, 2 target in the window, 3 target in HBM).
+        uint32_t cand[9];
+        float ht[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const int di = k % 3 - 1, dj = k / 3 - 1;
+            const int xx = i + di, yy = j + dj;
+            uint32_t kind = 0u;
+            int cx = di, cy = dj;
+            ht[k] = 0.0f;
+            if (xx >= 0 && yy >= 0 && xx < W_ && yy < H_) {
                 if (sw.h[wq(xx, yy)] > 0.0f) {
                     const uint32_t dv = sw.d[wq(xx, yy)];
-                    if ((dv & 0x10000u) == 0u) continue;
-                    int ox, oy;
-                    unpack_off(dv, ox, oy);
-                    cx = ox + di;
-                    cy = oy + dj;
-                    const int tx = i + cx, ty = j + cy;
-                    if (tx < 0 || ty < 0 || tx >= W_ || ty >= H_) continue;
-                    // the window when the target is inside it, else HBM
-                    float ht;
-                    if (in_win(tx, ty)) ht = sw.h[wq(tx, ty)];
-                    else ht = F.h[idx(F, (uint32_t)tx, (uint32_t)ty)];
-                    if (ht > 0.0f) continue;
-                } else if (di == 0 && dj == 0) {
-                    continue;
+                    if ((dv & 0x10000u) != 0u) {
+                        int ox, oy;
+                        unpack_off(dv, ox, oy);
+                        const int tx = i + ox + di, ty = j + oy + dj;
+                        if (tx >= 0 && ty >= 0 && tx < W_ && ty < H_) {
+                            cx = ox + di;
+                            cy = oy + dj;
+                            kind = in_win(tx, ty) ? 2u : 3u;
+                            if (kind == 2u) ht[k] = sw.h[wq(tx, ty)];
+                        }
+                    }
+                } else if (!(di == 0 && dj == 0)) {
+                    kind = 1u;
                 }
-                const int d = cx * cx + cy * cy;
-                if (d < bd) { bd = d; bx = cx; by = cy; }
             }
+            cand[k] = ((uint32_t)cx & 0x3ffu) | (((uint32_t)cy & 0x3ffu) << 10) | (kind << 20);
+        }
+        auto cand_x = [](uint32_t v) { return (int)(v << 22) >> 22; };
+        auto cand_y = [](uint32_t v) { return (int)(v << 12) >> 22; };
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            if ((cand[k] >> 20) == 3u) ht[k] = F.h[idx(F, (uint32_t)(i + cand_x(cand[k])), (uint32_t)(j + cand_y(cand[k])))];
+        int bx = 64, by = 64;
+        int bd = kNoseNone;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const uint32_t kind = cand[k] >> 20;
+            if (kind == 0u || (kind >= 2u && ht[k] > 0.0f)) continue;
+            const int cx = cand_x(cand[k]), cy = cand_y(cand[k]);
+            const int d = cx * cx + cy * cy;
+            if (d < bd) { bd = d; bx = cx; by = cy; }
         }
         if (bd == kNoseNone) return;
         d_out = pack_off(bx, by);
         if (bd <= 9) {
+            // the disc's heights (inside the window: the ring cell is at
+            // most one cell off the tile) read together, then summed in
+            // the shader's order
+            float hd[25];
+#pragma unroll
+            for (int q = 0; q < 25; q++) {
+                const int di = q % 5 - 2, dj = q / 5 - 2;
+                hd[q] = (di * di + dj * dj <= 5) ? sw.h[wq(i + di, j + dj)] : 0.0f;
+            }
             float dx = 0.0f, dy = 0.0f;
+#pragma unroll
             for (int dj = -2; dj <= 2; dj++) {
+#pragma unroll
                 for (int di = -2; di <= 2; di++) {
                     if (di == 0 && dj == 0) continue;
                     const int xx = i + di, yy = j + dj;
                     if (xx < 0 || yy < 0 || xx >= W_ || yy >= H_) continue;
                     if (di * di + dj * dj > 5) continue;
-                    if (sw.h[wq(xx, yy)] <= F.P.ein[3]) {
+                    if (hd[(dj + 2) * 5 + di + 2] <= F.P.ein[3]) {
                         dx = dx + FN_QUOT.v[di + 2][dj + 2];
                         dy = dy + FN_QUOT.v[dj + 2][di + 2];
                     }
@@ -942,7 +977,16 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
             fn_out = (l < 1e-6f) ? make_float2(0.0f, 0.0f) : make_float2(dx / l, dy / l);
         }
     };
-    for (uint32_t q = threadIdx.x; q < (uint32_t)(RING_SPAN * RING_SPAN); q += (uint32_t)BT) {
+    // A block with a thread for every ring cell and every window cell (the
+    // wide shape) gives the ring's nose and usurf and the window's plug
+    // fluxes to different threads, so that the two chains run side by side;
+    // a 256-thread block walks both in turn.
+    constexpr uint32_t RING_N = RING_SPAN * RING_SPAN, PLUG_N = WIN * WIN;
+    constexpr bool SPLIT = BT >= (int)(RING_N + PLUG_N);
+    const uint32_t ring_end = (!SPLIT || threadIdx.x < RING_N) ? RING_N : 0u;
+    const uint32_t plug_q0 = SPLIT ? threadIdx.x - RING_N : threadIdx.x;
+    const uint32_t plug_end = (!SPLIT || threadIdx.x >= RING_N) ? PLUG_N : 0u;
+    for (uint32_t q = threadIdx.x; q < ring_end; q += (uint32_t)BT) {
         const int gx = tx0 - 1 + (int)(q % RING_SPAN), gy = ty0 - 1 + (int)(q / RING_SPAN);
         uint32_t dv = 0u;
         float2 fn = make_float2(0.0f, 0.0f);
@@ -968,7 +1012,7 @@ __device__ __forceinline__ void lap_plug_tile(const FilmDev& F, uint32_t _t, Lap
         s_us[q] = us;
     }
     if (plug_on) {
-        for (uint32_t q = threadIdx.x; q < (uint32_t)(WIN * WIN); q += (uint32_t)BT) {
+        for (uint32_t q = plug_q0; q < plug_end; q += (uint32_t)BT) {
             float hn = sw.h[q], qv = 0.0f;
             bool yl = false;
             if (hn > 0.0f) {
