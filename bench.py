@@ -24,9 +24,10 @@ achieved = 76 B (+ 4 with an index per cell) per active cell and substep /
 the substep's device time (its launches' event times summed).  Between the
 phases a one-element fill kernel marks the boundary in a kernel trace.
 roofline.traffic: HBM bytes per substep: the HBM bytes per active cell and
-substep of the substep launches from the rocprof FETCH_SIZE / WRITE_SIZE
-runs of the same command recorded under profiles/, times this run's cells
-per substep (null when no record matches the command).  cpu_baseline: the CPU
+substep of each substep shape, from rocprof FETCH_SIZE / WRITE_SIZE runs of
+the same command with that shape pinned (recorded under profiles/),
+weighted by the cell-substeps this run's replay gave each shape, times its
+cells per substep (null when the records cover less than 99 % of them).  cpu_baseline: the CPU
 restatement (oracle/, its per-tile passes on every host thread) stepping the
 same canvas from the same initial state, frame by frame, for up to 25 s.
 """
@@ -602,6 +603,29 @@ def recorded_traffic(profile, field="bytes_per_launch", **key):
     return None
 
 
+TRAFFIC_PROFILE = "film_plain_2048_hbm_traffic.md"
+
+
+def substep_traffic(n, steps, warmup, shape_cells):
+    """HBM bytes per active cell and substep of the events replay: the
+    per-shape records (each measured with the substep shape pinned) weighted
+    by the cells x substeps the replay ran in each shape.  Returns (bytes,
+    covered fraction); bytes is None when the recorded shapes cover less
+    than 99 % of the replay's cell-substeps."""
+    total = float(sum(shape_cells))
+    covered = weighted = 0.0
+    for k, c in enumerate(shape_cells):
+        if not c:
+            continue
+        b = recorded_traffic(TRAFFIC_PROFILE, "bytes_per_cell", kernel="substep", shape=k, size=n, steps=steps,
+                             warmup=warmup)
+        if b is not None:
+            covered += c
+            weighted += b * c
+    frac = covered / total if total else 0.0
+    return (weighted / covered if covered and frac >= 0.99 else None), frac
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -810,6 +834,7 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: CPU restatement baseline")
         cpu = cpu_baseline(state, settings_fn(n), args.steps)
+    traffic_pc, traffic_cov = substep_traffic(n, args.steps, args.warmup, list(st.flux_shape_cell_substeps))
     line = {
         "metric": "film cell-steps/s", "value": value, "unit": "cell-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
@@ -832,9 +857,8 @@ def main():
                      "unit_of_work": "one substep over the active tiles (the sum of its launches)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (lambda b: None if b is None else b * cells_per_launch)(
-                         recorded_traffic("film_plain_2048_hbm_traffic.md", "bytes_per_cell", kernel="substep", size=n,
-                                          steps=args.steps, warmup=args.warmup)),
+                     "traffic": None if traffic_pc is None else traffic_pc * cells_per_launch,
+                     "traffic_bytes_per_cell": traffic_pc, "traffic_coverage": traffic_cov,
                      "measured_copy_GBps": measured["copy_GBps"], "frac_of_measured_copy": achieved / measured["copy_GBps"],
                      "algo_bytes_per_substep": bytes_per_launch, "algo_bytes_per_cell": algo_per_cell,
                      "avg_substep_us": avg_us, "substeps": launches, "cells_per_substep": cells_per_launch,
