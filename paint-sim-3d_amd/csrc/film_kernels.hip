@@ -2450,14 +2450,27 @@ __device__ __forceinline__ float sel4(float4 v, uint32_t k) {
     return k == 0u ? v.x : (k == 1u ? v.y : (k == 2u ? v.z : v.w));
 }
 
-// inflow_from (strata.wgsl:167-188): the strata of neighbour cn moving into
-// the cell with face fluxes f (one per stratum).  The stratum loop stays a
-// loop (one copy of stack_add_band) and reads its per-stratum values by
-// select, so nothing is indexed at run time.
-__device__ inline void inflow_from(const FilmDev& F, Stack& s, uint32_t cn, float4 f) {
-    const float4 an4 = F.alpha_s[cn];
-    const float4 thn4 = F.strata_h[cn];
-    const float4 cu4 = F.strata_cure[cn];
+// The tile's cells and a 1-cell ring staged in LDS for strata_update: the
+// limiter factors, strata heights, cure and pigment words of every cell a
+// tile cell takes an inflow from.  The eight inflows of a cell then read
+// their donor's strata from LDS; read from HBM, each inflow's and each
+// stratum's loads waited behind the previous merge.
+constexpr int SU_N = 18, SU_CELLS = SU_N * SU_N;
+struct StrataUpdShared {
+    float4 an[SU_CELLS];
+    float4 th[SU_CELLS];
+    float4 cu[SU_CELLS];
+    uint4 pg[SU_CELLS * 8];
+};
+
+// inflow_from (strata.wgsl:167-188): the strata of the neighbour in slot qn
+// of the staged ring moving into the cell with face fluxes f (one per
+// stratum).  The stratum loop stays a loop (one copy of stack_add_band) and
+// reads its per-stratum values by select, so nothing is indexed at run time.
+__device__ inline void inflow_from_lds(const FilmDev& F, Stack& s, const StrataUpdShared& S, int qn, float4 f) {
+    const float4 an4 = S.an[qn];
+    const float4 thn4 = S.th[qn];
+    const float4 cu4 = S.cu[qn];
     float z0 = 0.0f;
 #pragma unroll 1
     for (uint32_t k = 0; k < 4u; k++) {
@@ -2466,7 +2479,7 @@ __device__ inline void inflow_from(const FilmDev& F, Stack& s, uint32_t cn, floa
         float v = sel4(f, k) * sel4(an4, k) * F.P.inv_dx;
         if (v > 0.0f && thk > 0.0f) {
             float pig[16];
-            unpack8(F.strata_pig[(cn * 4u + k) * 2u], F.strata_pig[(cn * 4u + k) * 2u + 1u], pig);
+            unpack8(S.pg[qn * 8 + (int)k * 2], S.pg[qn * 8 + (int)k * 2 + 1], pig);
             stack_add_band(F, s, z0, z1, v, pig, sel4(cu4, k));
         }
         z0 = z1;
@@ -2480,9 +2493,54 @@ __device__ __forceinline__ void to4(const float4 v, float* o) { o[0] = v.x; o[1]
 // active cell for the fill passes.
 template <class WS>
 __global__ __launch_bounds__(256) void k_strata_update(WS ws) {
+    __shared__ StrataUpdShared S;
     CZ_FOR_TILES(ws) {
         CZ_TILE_CELL(F)
+        const int gx0 = (int)(_t % F.tw) * 16 - 1, gy0 = (int)(_t / F.tw) * 16 - 1;
+        const int W_ = (int)F.W, H_ = (int)F.H;
+        __syncthreads();  // the previous tile's cells are done with S
+        {
+            // every load first, then every store: a ring cell beyond the
+            // canvas (or a slot past the ring) reads a clamped cell, which
+            // nothing uses, so the loads carry no branch between them
+            constexpr int R1 = (SU_CELLS + 255) / 256, R2 = (SU_CELLS * 8 + 255) / 256;
+            float4 va[R1], vt[R1], vc[R1];
+            uint4 vp[R2];
+#pragma unroll
+            for (int r = 0; r < R1; r++) {
+                const int q = min((int)threadIdx.x + r * 256, SU_CELLS - 1);
+                const int gx = min(max(gx0 + q % SU_N, 0), W_ - 1), gy = min(max(gy0 + q / SU_N, 0), H_ - 1);
+                const uint32_t cn = idx(F, (uint32_t)gx, (uint32_t)gy);
+                va[r] = F.alpha_s[cn];
+                vt[r] = F.strata_h[cn];
+                vc[r] = F.strata_cure[cn];
+            }
+#pragma unroll
+            for (int r = 0; r < R2; r++) {
+                const int q = min((int)threadIdx.x + r * 256, SU_CELLS * 8 - 1);
+                const int cell = q >> 3;
+                const int gx = min(max(gx0 + cell % SU_N, 0), W_ - 1), gy = min(max(gy0 + cell / SU_N, 0), H_ - 1);
+                vp[r] = F.strata_pig[idx(F, (uint32_t)gx, (uint32_t)gy) * 8u + (uint32_t)(q & 7)];
+            }
+#pragma unroll
+            for (int r = 0; r < R1; r++) {
+                const int q = (int)threadIdx.x + r * 256;
+                if (q < SU_CELLS) {
+                    S.an[q] = va[r];
+                    S.th[q] = vt[r];
+                    S.cu[q] = vc[r];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R2; r++) {
+                const int q = (int)threadIdx.x + r * 256;
+                if (q < SU_CELLS * 8) S.pg[q] = vp[r];
+            }
+        }
+        __syncthreads();
         if (!in_grid) continue;
+        // slots of the cell and of its neighbours in S
+        const int own = ((int)_ly + 1) * SU_N + (int)_lx + 1;
         float4 fr = F.acc_s[4u * c], fu = F.acc_s[4u * c + 1u], fdg = F.acc_s[4u * c + 2u], feg = F.acc_s[4u * c + 3u];
         const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         float4 fl = z4, fd = z4, fll = z4, flu = z4;
@@ -2493,10 +2551,10 @@ __global__ __launch_bounds__(256) void k_strata_update(WS ws) {
         if (x > 0u && y + 1u < F.H && cell_active(F, x - 1u, y + 1u)) { clu = c + F.W - 1u; flu = F.acc_s[4u * clu + 3u]; }
         if (zero4(fr) && zero4(fu) && zero4(fdg) && zero4(feg) && zero4(fl) && zero4(fd) && zero4(fll) && zero4(flu)) {
             float hc = F.h[c];
-            float4 t4 = F.strata_h[c];
+            float4 t4 = S.th[own];
             float th[4] = {cz_max(t4.x, 0.0f), cz_max(t4.y, 0.0f), cz_max(t4.z, 0.0f), cz_max(t4.w, 0.0f)};
             float nt[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-            float4 cu = F.strata_cure[c];
+            float4 cu = S.cu[own];
             if (hc <= 0.0f) {
                 cu = z4;
             } else {
@@ -2515,14 +2573,20 @@ __global__ __launch_bounds__(256) void k_strata_update(WS ws) {
             if (plain) {
                 F.strata_h2[c] = make_float4(nt[0], nt[1], nt[2], nt[3]);
                 F.strata_cure2[c] = cu;
-                for (uint32_t k = 0; k < 8u; k++) F.strata_pig2[c * 8u + k] = F.strata_pig[c * 8u + k];
+                for (uint32_t k = 0; k < 8u; k++) F.strata_pig2[c * 8u + k] = S.pg[own * 8 + (int)k];
                 F.tsum_a[c] = nt[0] + nt[1] + nt[2] + nt[3];
                 continue;
             }
         }
         Stack s;
-        stack_load(F, c, s);
-        float4 a4 = F.alpha_s[c];
+        {
+            const float4 t = S.th[own], u = S.cu[own];
+            s.th[0] = t.x; s.th[1] = t.y; s.th[2] = t.z; s.th[3] = t.w;
+            s.cu[0] = u.x; s.cu[1] = u.y; s.cu[2] = u.z; s.cu[3] = u.w;
+#pragma unroll
+            for (int k = 0; k < 4; k++) unpack8(S.pg[own * 8 + k * 2], S.pg[own * 8 + k * 2 + 1], s.pg[k]);
+        }
+        float4 a4 = S.an[own];
         float a[4] = {a4.x, a4.y, a4.z, a4.w};
         float vr[4], vu[4], vdg[4], veg[4], vl[4], vd[4], vll[4], vlu[4];
         to4(fr, vr); to4(fu, vu); to4(fdg, vdg); to4(feg, veg); to4(fl, vl); to4(fd, vd); to4(fll, vll); to4(flu, vlu);
@@ -2536,21 +2600,21 @@ __global__ __launch_bounds__(256) void k_strata_update(WS ws) {
 #pragma unroll 1
         for (uint32_t q = 0; q < 8u; q++) {
             bool ok = false;
-            uint32_t cn = c;
+            int qn = own;
             float4 nf = z4;
             switch (q) {
-            case 0: ok = x + 1u < F.W; cn = c + 1u; nf = make_float4(-fr.x, -fr.y, -fr.z, -fr.w); break;
-            case 1: ok = y + 1u < F.H; cn = c + F.W; nf = make_float4(-fu.x, -fu.y, -fu.z, -fu.w); break;
-            case 2: ok = x + 1u < F.W && y + 1u < F.H; cn = c + F.W + 1u;
+            case 0: ok = x + 1u < F.W; qn = own + 1; nf = make_float4(-fr.x, -fr.y, -fr.z, -fr.w); break;
+            case 1: ok = y + 1u < F.H; qn = own + SU_N; nf = make_float4(-fu.x, -fu.y, -fu.z, -fu.w); break;
+            case 2: ok = x + 1u < F.W && y + 1u < F.H; qn = own + SU_N + 1;
                     nf = make_float4(-fdg.x, -fdg.y, -fdg.z, -fdg.w); break;
-            case 3: ok = x + 1u < F.W && y > 0u; cn = c - F.W + 1u;
+            case 3: ok = x + 1u < F.W && y > 0u; qn = own - SU_N + 1;
                     nf = make_float4(-feg.x, -feg.y, -feg.z, -feg.w); break;
-            case 4: ok = cl != c; cn = cl; nf = fl; break;
-            case 5: ok = cd != c; cn = cd; nf = fd; break;
-            case 6: ok = cll != c; cn = cll; nf = fll; break;
-            default: ok = clu != c; cn = clu; nf = flu; break;
+            case 4: ok = cl != c; qn = own - 1; nf = fl; break;
+            case 5: ok = cd != c; qn = own - SU_N; nf = fd; break;
+            case 6: ok = cll != c; qn = own - SU_N - 1; nf = fll; break;
+            default: ok = clu != c; qn = own + SU_N - 1; nf = flu; break;
             }
-            if (ok) inflow_from(F, s, cn, nf);
+            if (ok) inflow_from_lds(F, s, S, qn, nf);
         }
         float hc = F.h[c];
         float sum = s.th[0] + s.th[1] + s.th[2] + s.th[3];
