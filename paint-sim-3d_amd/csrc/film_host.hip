@@ -443,8 +443,12 @@ int record_frame(cz_film* f, uint32_t substeps) {
         cur_main = true;
         if (slump) {
             // this substep's nose, then shed and slump (+ commit_tmp), as
-            // encode_frame orders them (src/film/mod.rs:1420-1429)
-            LAUNCH("lap", k_lap_tail<OneFilm>, gt, b256, OneFilm{f->dev()}, 1, 0);
+            // encode_frame orders them (src/film/mod.rs:1420-1429); in the
+            // block size the substep's shape took
+            if (shape >= 3)
+                LAUNCH("lap", k_lap_tail_w<OneFilm>, gt, b1024, OneFilm{f->dev()}, 1, 0);
+            else
+                LAUNCH("lap", k_lap_tail<OneFilm>, gt, b256, OneFilm{f->dev()}, 1, 0);
             f->delta_cur_a = !f->delta_cur_a;
             const OneFilm ws{f->dev()};
             LAUNCH("shed", k_shed<OneFilm>, gt, b256, ws);
@@ -457,9 +461,14 @@ int record_frame(cz_film* f, uint32_t substeps) {
         }
         f->shape_subs_now[shape]++;
     }
-    // nose of the last substep (with slump: done) + the laplacian strata_split reads
+    // nose of the last substep (with slump: done) + the laplacian strata_split
+    // reads, in 1024-thread blocks when the last substep took a wide shape
+    // (a few hundred active tiles or fewer: 8-10 us against 12-14)
     if (pg == 0u) {
-        LAUNCH("lap", k_lap_tail<OneFilm>, gt, b256, OneFilm{f->dev()}, slump ? 0 : 1, 0);
+        if (nsub > 0 && shp[nsub - 1] >= 3)
+            LAUNCH("lap", k_lap_tail_w<OneFilm>, gt, b1024, OneFilm{f->dev()}, slump ? 0 : 1, 0);
+        else
+            LAUNCH("lap", k_lap_tail<OneFilm>, gt, b256, OneFilm{f->dev()}, slump ? 0 : 1, 0);
         if (!slump) f->delta_cur_a = !f->delta_cur_a;
     }
     const OneFilm w{f->dev()};

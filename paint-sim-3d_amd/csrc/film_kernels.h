@@ -134,6 +134,7 @@ __global__ void k_batch_plan(BatchPlan B);
     X(k_cure_clear, (WS ws, int do_clear), (ws, do_clear))          \
     X(k_lap_plug, (WS ws, int do_nose, int do_plug, int do_clear), (ws, do_nose, do_plug, do_clear)) \
     X(k_lap_tail, (WS ws, int do_nose, int do_clear), (ws, do_nose, do_clear)) \
+    X(k_lap_tail_w, (WS ws, int do_nose, int do_clear), (ws, do_nose, do_clear)) \
     X(k_lap_plug_w, (WS ws, int do_nose, int do_plug, int do_clear), (ws, do_nose, do_plug, do_clear)) \
     X(k_fluxes_w, (WS ws), (ws))                                    \
     X(k_limit_apply_w, (WS ws), (ws))                               \

@@ -1123,6 +1123,13 @@ __global__ __launch_bounds__(256) void k_lap_tail(WS ws, int do_nose, int do_cle
     CZ_FOR_TILES(ws) lap_plug_tile(F, _t, S, do_nose, 0, do_clear);
 }
 
+// ... in 1024-thread blocks, after substeps that ran in the wide shapes
+template <class WS>
+__global__ __launch_bounds__(1024) void k_lap_tail_w(WS ws, int do_nose, int do_clear) {
+    __shared__ LapShared S;
+    CZ_FOR_TILES(ws) lap_plug_tile<1024>(F, _t, S, do_nose, 0, do_clear);
+}
+
 // fluxes (film.wgsl:375-455), face-parallel: each block takes one
 // (tile, face) pair and reads its fields from HBM through the caches; 4x the
 // waves of a cell-per-thread launch to hide the dependent loads of the
@@ -3199,6 +3206,7 @@ CZ_INST2(k_fluxes_w)
 CZ_INST2(k_limit_apply_w)
 CZ_INST2(k_flux_apply_w)
 CZ_INST1(k_substep_fused, OneFilm, const float*, float*, int, int)
+CZ_INST1(k_lap_tail_w, OneFilm, int, int)
 CZ_INST1(k_shed, OneFilm)
 CZ_INST1(k_slump, OneFilm)
 CZ_INST1(k_mark, OneFilmAll)
