@@ -296,7 +296,7 @@ int record(cz_batch* b, const FrameShape& sh) {
     const dim3 ga(sh.g_all);
     hipLaunchKernelGGL(k_lap_tail<ManyFilms>, ga, b256, 0, st, many(b->items_sub, all, 2), 1, 0);
     const ManyFilms w = many(b->items_sub, all, 3);
-    hipLaunchKernelGGL(k_strata_split<ManyFilms>, ga, b256, 0, st, w);
+    hipLaunchKernelGGL(k_strata_split<ManyFilms>, dim3(std::min<uint32_t>(4u * sh.g_all, 4u * kGridCap)), b256, 0, st, w);
     hipLaunchKernelGGL(k_reduce_pool<ManyFilms>, dim3(sh.g64), b256, 0, st, many(b->all64, n_all64, 3));
     hipLaunchKernelGGL(k_strata_limit<ManyFilms>, ga, b256, 0, st, w);
     hipLaunchKernelGGL(k_strata_update<ManyFilms>, ga, b256, 0, st, w);

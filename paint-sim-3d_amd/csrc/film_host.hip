@@ -472,7 +472,7 @@ int record_frame(cz_film* f, uint32_t substeps) {
         if (!slump) f->delta_cur_a = !f->delta_cur_a;
     }
     const OneFilm w{f->dev()};
-    LAUNCH("split", k_strata_split<OneFilm>, gt, b256, w);
+    LAUNCH("split", k_strata_split<OneFilm>, dim3(f->tile_grid(4)), b256, w);
     LAUNCH("reduce_pool", k_reduce_pool<OneFilmAll>, dim3(n64), b256, OneFilmAll{w.F, n64});
     LAUNCH("strata_limit", k_strata_limit<OneFilm>, gt, b256, w);
     LAUNCH("strata_update", k_strata_update<OneFilm>, gt, b256, w);

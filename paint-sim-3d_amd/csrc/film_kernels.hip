@@ -2241,37 +2241,47 @@ __device__ inline float4 band_split(const FilmDev& F, uint32_t cu, float hf, flo
     return make_float4(fr[0] * total, fr[1] * total, fr[2] * total, fr[3] * total);
 }
 
-// The same launch copies the offsets of active cells from the buffer the last
-// nose wrote (F.delta, which split reads) to the other one (F.delta_out,
-// which nothing in the launch reads), so that both hold the same values
-// everywhere between frames.
+// strata_split (film.wgsl:457-514), face-parallel: each block takes one
+// (tile, face) pair, as k_fluxes_faces does; a cell's four faces are split
+// independently.  Against a cell's four faces per thread: 4x the waves and a
+// quarter of each thread's chain of face and band evaluations (2048^2 blob
+// canvas: 26-28 -> 8 us per frame at 100-250 active tiles, 343 -> 235 us with
+// every tile active).  The launch also copies the offsets of active cells
+// from the buffer the last nose wrote (F.delta, which split reads) to the
+// other one (F.delta_out, which nothing in the launch reads), so that both
+// hold the same values everywhere between frames.
 template <class WS>
 __global__ __launch_bounds__(256) void k_strata_split(WS ws) {
-    CZ_FOR_TILES(ws) {
-        CZ_TILE_CELL(F)
-        if (!in_grid) continue;
-        F.delta_out[c] = F.delta[c];
-        const GlobalRead g{F};
-        float2 p00 = pressure_at(F, g, (int)x, (int)y);
-        float4 a = F.acc[c];
-        float av[4] = {a.x, a.y, a.z, a.w};
-        float4 o[4];
-        for (int k = 0; k < 4; k++) {
-            o[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            bool exists = (k == 0) ? (x + 1u < F.W)
-                        : (k == 1) ? (y + 1u < F.H)
-                        : (k == 2) ? (x + 1u < F.W && y + 1u < F.H)
-                                   : (x + 1u < F.W && y > 0u);
-            if (av[k] != 0.0f && exists) {
-                FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, true, p00);
-                FaceFlux ff = {0.0f, 0.0f, 0.0f};
-                if (fg.has) ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.gtn, fg.diag, fg.dir);
-                uint32_t cb = (k == 0) ? c + 1u : (k == 1) ? c + F.W : (k == 2) ? c + F.W + 1u : c - F.W + 1u;
-                uint32_t cu = (av[k] < 0.0f) ? cb : c;
-                o[k] = band_split(F, cu, 0.5f * (F.h[c] + F.h[cb]), ff.y, ff.up, av[k], ff.n);
-            }
+    const uint32_t nact = ws.count();
+    const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+    const XcdRange rg = xcd_range<true>(4u * nact);
+    for (uint32_t wk = rg.first; wk < rg.end; wk += rg.step) {
+        const uint32_t it = ws.item(wk >> 2);
+        const FilmDev& F = ws.film(it);
+        const uint32_t t = ws.tile(F, it);
+        const int k = (int)(wk & 3u);
+        const uint32_t x = (t % F.tw) * 16u + lx, y = (t / F.tw) * 16u + ly;
+        if (x >= F.W || y >= F.H) continue;
+        const uint32_t c = idx(F, x, y);
+        if (k == 0) F.delta_out[c] = F.delta[c];
+        const float4 a4 = F.acc[c];
+        const float av = (k == 0) ? a4.x : (k == 1) ? a4.y : (k == 2) ? a4.z : a4.w;
+        float4 o = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const bool exists = (k == 0) ? (x + 1u < F.W)
+                          : (k == 1) ? (y + 1u < F.H)
+                          : (k == 2) ? (x + 1u < F.W && y + 1u < F.H)
+                                     : (x + 1u < F.W && y > 0u);
+        if (av != 0.0f && exists) {
+            const GlobalRead g{F};
+            const float2 p00 = pressure_at(F, g, (int)x, (int)y);
+            FaceGeom fg = face_geom(F, g, (int)x, (int)y, k, true, p00);
+            FaceFlux ff = {0.0f, 0.0f, 0.0f};
+            if (fg.has) ff = face(F, g, (int)x, (int)y, fg.ib, fg.jb, fg.gcap, fg.ghyd, fg.gtilt, fg.gtn, fg.diag, fg.dir);
+            const uint32_t cb = (k == 0) ? c + 1u : (k == 1) ? c + F.W : (k == 2) ? c + F.W + 1u : c - F.W + 1u;
+            const uint32_t cu = (av < 0.0f) ? cb : c;
+            o = band_split(F, cu, 0.5f * (F.h[c] + F.h[cb]), ff.y, ff.up, av, ff.n);
         }
-        for (int k = 0; k < 4; k++) F.acc_s[4u * c + k] = o[k];
+        F.acc_s[4u * c + (uint32_t)k] = o;
     }
 }
 
