@@ -307,6 +307,8 @@ __device__ __forceinline__ float band_flux(float n, float up, float y, float za,
 // halo from LDS and only the far steps of cell_excess (h_far, delta_far) from
 // HBM.  The arithmetic is the same whichever accessor is used.
 struct GlobalRead {
+    // far reads are plain loads: cell_excess may issue its three at once
+    static constexpr bool kFarCheap = true;
     const FilmDev& F;
     __device__ __forceinline__ uint32_t at(int i, int j) const { return (uint32_t)j * F.W + (uint32_t)i; }
     __device__ __forceinline__ float h(int i, int j) const { return F.h[at(i, j)]; }
@@ -431,15 +433,40 @@ __device__ __forceinline__ float cell_excess(const FilmDev& F, const A& a, int c
         float inv = -1.0f / ol;
         float dirx = (float)ox * inv, diry = (float)oy * inv;
         float bx = 0.5f * cz_sign((float)(-ox)), by = 0.5f * cz_sign((float)(-oy));
-        for (uint32_t k = 0; k < 3u; k++) {
-            float reach = (float)(4u << k);
-            int sx = (int)(dirx * reach + bx);
-            int sy = (int)(diry * reach + by);
-            int ri = ci + sx, rj = cj + sy;
-            if (ri < 0 || rj < 0 || ri >= (int)F.W || rj >= (int)F.H) break;
-            float hr = a.h_far(ri, rj);
-            if (hr <= 0.0f) break;
-            e = cz_max(e, hr - hold_level(F, nose_dist_b(a.delta_far(ri, rj))));
+        if constexpr (A::kFarCheap) {
+            // the three steps' heights and offsets read together (a step
+            // beyond the canvas reads the cell itself and is not used); the
+            // walk then stops where the step-by-step walk stops
+            float hs[3];
+            uint32_t ds[3];
+            bool ok[3];
+#pragma unroll
+            for (uint32_t k = 0; k < 3u; k++) {
+                float reach = (float)(4u << k);
+                int sx = (int)(dirx * reach + bx);
+                int sy = (int)(diry * reach + by);
+                int ri = ci + sx, rj = cj + sy;
+                ok[k] = !(ri < 0 || rj < 0 || ri >= (int)F.W || rj >= (int)F.H);
+                if (!ok[k]) { ri = ci; rj = cj; }
+                hs[k] = a.h_far(ri, rj);
+                ds[k] = a.delta_far(ri, rj);
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 3u; k++) {
+                if (!ok[k] || hs[k] <= 0.0f) break;
+                e = cz_max(e, hs[k] - hold_level(F, nose_dist_b(ds[k])));
+            }
+        } else {
+            for (uint32_t k = 0; k < 3u; k++) {
+                float reach = (float)(4u << k);
+                int sx = (int)(dirx * reach + bx);
+                int sy = (int)(diry * reach + by);
+                int ri = ci + sx, rj = cj + sy;
+                if (ri < 0 || rj < 0 || ri >= (int)F.W || rj >= (int)F.H) break;
+                float hr = a.h_far(ri, rj);
+                if (hr <= 0.0f) break;
+                e = cz_max(e, hr - hold_level(F, nose_dist_b(a.delta_far(ri, rj))));
+            }
         }
     }
     return e;

@@ -1192,6 +1192,7 @@ using FluxWin = FluxWinT<FLUX_R>;
 template <int R, bool SNAP>
 struct WinReadT {
     static constexpr int N = FluxWinT<R>::N;
+    static constexpr bool kFarCheap = true;   // far reads: LDS or a plain load
     const FilmDev& F;
     const FluxWinT<R>& w;
     int wx0, wy0, ttx, tty;
@@ -1836,6 +1837,9 @@ __device__ uint32_t nose_offset_global(const FilmDev& F, const float* hr, int i,
 // steps of cell_excess served as described above.
 struct FusedRead {
     static constexpr int N = FluxWinT<SL_R>::N;
+    // a far offset may be a nose evaluated in HBM: cell_excess reads the far
+    // steps one at a time and stops at the first dry one
+    static constexpr bool kFarCheap = false;
     const FilmDev& F;
     const FluxWinT<SL_R>& w;
     const float* hr;
