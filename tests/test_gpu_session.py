@@ -20,8 +20,9 @@ pytestmark = pytest.mark.gpu
 PARTS = ("h", "m", "rheo", "strata_h", "strata_pig", "strata_cure", "sub_refl", "lay")
 
 
-def test_session_of_strokes_tools_and_undo(gpu):
-    w, h = 144, 112
+@pytest.mark.parametrize("w,h", [(144, 112), (75, 53)])
+def test_session_of_strokes_tools_and_undo(gpu, w, h):
+    # 75 x 53: partial tiles on both axes and rows that are not 16-byte aligned
     s = scenes.blobs_settings(w, tilt_deg=25.0)
     s.height_m = h * s.width_m / w
     st = scenes.blobs_state(w, h, seed=77, nblobs=7, sigma_cells=(3.0, 9.0))
